@@ -1,0 +1,14 @@
+"""discamb_amd: Hansen-Coppens structure factors and derivatives on AMD MI355X (gfx950).
+
+A from-scratch HIP implementation of DiSCaMB's
+``HansenCoppensStructureFactorCalculator::calculateStructureFactorsAndDerivatives`` behind the
+same calculator interface; see include/discamb_amd.h for the C ABI.
+"""
+from .model import (AtomSet, Crystal, DerivativesSelector, HC_AtomTypeParameters, HC_ModelParameters,
+                    HC_WfnType, SlaterOrbital, TargetFunctionAtomicParamDerivatives, parse_symop,
+                    wfn_type_from_bank)
+from .calculator import HansenCoppensStructureFactorCalculator
+
+__all__ = ["AtomSet", "Crystal", "DerivativesSelector", "HC_AtomTypeParameters", "HC_ModelParameters",
+           "HC_WfnType", "SlaterOrbital", "TargetFunctionAtomicParamDerivatives", "parse_symop",
+           "wfn_type_from_bank", "HansenCoppensStructureFactorCalculator"]

@@ -1,0 +1,227 @@
+"""Python mirror of DiSCaMB's HansenCoppensStructureFactorCalculator on the HIP engine.
+
+Reference interface: include/discamb/Scattering/HansenCoppensStructureFactorCalculator.h:36-375
+(constructor ``(Crystal, HC_ModelParameters)``, ``setCalculationMode``,
+``calculateStructureFactorsAndDerivatives(atoms, lcs, hkl, f, dTarget_dparam, dTarget_df,
+countAtomContribution[, selector])``, ``calculateStructureFactors``).  Errors raise ``DhError``
+(the reference throws discamb::Exception).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .model import (AtomSet, Crystal, DerivativesSelector, HC_ModelParameters,
+                    TargetFunctionAtomicParamDerivatives, parse_symop)
+
+XYZ = {"fractional": 0, "cartesian": 1}
+ADP = {"U_cif": 0, "U_cart": 1}
+
+
+class _Keep:
+    """Holds numpy arrays alive while ctypes structs point into them."""
+
+    def __init__(self):
+        self.items = []
+
+    def arr(self, a, dtype):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        self.items.append(a)
+        return a
+
+
+def _model_desc(crystal: Crystal, params: HC_ModelParameters, d_xyz="fractional", d_adp="U_cif"):
+    keep = _Keep()
+    d = _lib.DhModelDesc()
+    d.cell = (ctypes.c_double * 6)(*[float(v) for v in crystal.cell])
+    rots, trans = [], []
+    for s in crystal.symops:
+        r, t = parse_symop(s)
+        rots.append(r)
+        trans.append(t)
+    rot = keep.arr(np.array(rots).reshape(-1), np.int32)
+    tr = keep.arr(np.array(trans).reshape(-1), np.float64)
+    d.n_symops = len(crystal.symops)
+    d.symop_rot = _lib.ptr(rot, ctypes.c_int)
+    d.symop_trans = _lib.ptr(tr, ctypes.c_double)
+    d.xyz_convention = XYZ[crystal.xyz_convention]
+    d.adp_convention = ADP[crystal.adp_convention]
+    d.d_xyz_convention = XYZ[d_xyz]
+    d.d_adp_convention = ADP[d_adp]
+    wo, to, nl, cc, ce, cp, occ, kind, dve, dvo, dvp, an = [0], [0], [], [], [], [], [], [], [], [0], [], []
+    for w in params.wfn_parameters:
+        for i, o in enumerate(w.orbitals):
+            nl += [o.n, o.l]
+            cc += list(o.coefficients)
+            ce += list(o.exponents)
+            cp += list(o.powers)
+            to.append(len(cc))
+            occ.append(w.orbital_occupancy[i])
+            kind.append(0 if i in w.core_orbitals_indices else (1 if i in w.valence_orbitals_indices else -1))
+        wo.append(len(occ))
+        dve.append(w.deformation_valence_exponent)
+        dvp += list(w.deformation_valence_power)
+        dvo.append(len(dvp))
+        an += [complex(w.anomalous_scattering).real, complex(w.anomalous_scattering).imag]
+    d.n_wfn = len(params.wfn_parameters)
+    for name, vals, dt, ct in [("wfn_orbital_offset", wo, np.int32, ctypes.c_int),
+                               ("orbital_term_offset", to, np.int32, ctypes.c_int),
+                               ("orbital_nl", nl + [0], np.int32, ctypes.c_int),
+                               ("term_coeff", cc + [0.0], np.float64, ctypes.c_double),
+                               ("term_exp", ce + [0.0], np.float64, ctypes.c_double),
+                               ("term_pow", cp + [0], np.int32, ctypes.c_int),
+                               ("orbital_occupancy", occ + [0], np.int32, ctypes.c_int),
+                               ("orbital_kind", kind + [0], np.int32, ctypes.c_int),
+                               ("def_val_exponent", dve + [0.0], np.float64, ctypes.c_double),
+                               ("def_val_pow_offset", dvo, np.int32, ctypes.c_int),
+                               ("def_val_pow", dvp + [0], np.int32, ctypes.c_int),
+                               ("anomalous", an + [0.0, 0.0], np.float64, ctypes.c_double)]:
+        setattr(d, name, _lib.ptr(keep.arr(vals, dt), ct))
+    tp = params.type_parameters
+    d.n_types = len(tp)
+    d.plm_n_l = _lib.ptr(keep.arr([len(t.p_lm) for t in tp] + [0], np.int32), ctypes.c_int)
+    d.plm = _lib.ptr(keep.arr(np.concatenate([t.plm25() for t in tp] + [np.zeros(25)]), np.float64), ctypes.c_double)
+    d.p_val = _lib.ptr(keep.arr([t.p_val for t in tp] + [0.0], np.float64), ctypes.c_double)
+    d.kappa = _lib.ptr(keep.arr([t.kappa_spherical_valence for t in tp] + [1.0], np.float64), ctypes.c_double)
+    d.kappa_prime = _lib.ptr(keep.arr([t.kappa_deformation_valence for t in tp] + [1.0], np.float64),
+                             ctypes.c_double)
+    d.n_atoms = len(params.atom_to_type_map)
+    d.atom_to_wfn = _lib.ptr(keep.arr(list(params.atom_to_wfn_map) + [0], np.int32), ctypes.c_int)
+    d.atom_to_type = _lib.ptr(keep.arr(list(params.atom_to_type_map) + [0], np.int32), ctypes.c_int)
+    d.atom_multiplicity = _lib.ptr(keep.arr(np.append(crystal.atoms.multiplicity, 0.0), np.float64),
+                                   ctypes.c_double)
+    return d, keep
+
+
+class HansenCoppensStructureFactorCalculator:
+    """F(h) and dT/dp for the Hansen-Coppens multipole model on one GPU."""
+
+    def __init__(self, crystal: Crystal, parameters: HC_ModelParameters, device: int = 0,
+                 d_xyz_convention: str = None, d_adp_convention: str = None):
+        """Derivative conventions default to the crystal's input conventions, as the reference
+        constructor sets them (HansenCoppensStructureFactorCalculator.cpp:57-73)."""
+        self._lib = _lib.load()
+        d_xyz_convention = d_xyz_convention or crystal.xyz_convention
+        d_adp_convention = d_adp_convention or crystal.adp_convention
+        desc, keep = _model_desc(crystal, parameters, d_xyz_convention, d_adp_convention)
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.dh_hc_create(ctypes.byref(desc), int(device), ctypes.byref(h)))
+        self._h = h
+        self.n_atoms = len(parameters.atom_to_type_map)
+        self._keep = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.dh_hc_destroy(h)
+            self._h = None
+
+    # -- reference API -----------------------------------------------------------------------
+    def setCalculationMode(self, mode: str):
+        _lib.check(self._lib.dh_hc_set_calculation_mode(self._h, mode.encode()))
+
+    def setTuning(self, hkl_chunk: int):
+        _lib.check(self._lib.dh_hc_set_tuning(self._h, int(hkl_chunk)))
+
+    def engineInfo(self):
+        n, o, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ce = ctypes.create_string_buffer(2)
+        _lib.check(self._lib.dh_hc_engine_info(self._h, ctypes.byref(n), ctypes.byref(o), ctypes.byref(c), ce))
+        return {"n_explicit_ops": n.value, "order": o.value, "centrosymmetric": bool(c.value),
+                "centering": ce.value.decode()}
+
+    def _atoms(self, atoms: AtomSet, lcs, include, anomalous):
+        keep = _Keep()
+        a = _lib.DhAtoms()
+        n = atoms.n
+        if n != self.n_atoms:
+            raise _lib.DhError(1, "number of atoms differs from the model's atom-to-type map")
+        a.n_atoms = n
+        a.xyz = _lib.ptr(keep.arr(atoms.xyz, np.float64), ctypes.c_double)
+        a.adp_size = _lib.ptr(keep.arr(atoms.adp_size, np.int32), ctypes.c_int)
+        a.adp = _lib.ptr(keep.arr(atoms.adp, np.float64), ctypes.c_double)
+        a.occupancy = _lib.ptr(keep.arr(atoms.occupancy, np.float64), ctypes.c_double)
+        if lcs is None:
+            lcs = np.broadcast_to(np.eye(3), (n, 3, 3))
+        a.lcs = _lib.ptr(keep.arr(np.asarray(lcs).reshape(n, 9), np.float64), ctypes.c_double)
+        if anomalous is not None:
+            an = np.asarray(anomalous, dtype=np.complex128).reshape(n)
+            a.anomalous = _lib.ptr(keep.arr(np.stack([an.real, an.imag], 1), np.float64), ctypes.c_double)
+        if include is not None:
+            a.include = _lib.ptr(keep.arr(np.asarray(include, dtype=bool).astype(np.uint8), np.uint8),
+                                 ctypes.c_ubyte)
+        return a, keep
+
+    @staticmethod
+    def _hkl(hkl, dtdf):
+        hkl = np.ascontiguousarray(np.asarray(hkl).reshape(-1, 3), dtype=np.int32)
+        d = np.asarray(dtdf, dtype=np.complex128).reshape(-1)
+        if d.shape[0] != hkl.shape[0]:
+            raise _lib.DhError(1, "Inconsistent size of hkl vectors set and (d target/ d F) table when "
+                                  "calculating multipolar structure factors.")
+        dd = np.ascontiguousarray(np.stack([d.real, d.imag], 1))
+        return hkl, dd
+
+    def calculateStructureFactorsAndDerivatives(self, atoms: AtomSet, localCoordinateSystems, hkl,
+                                                dTarget_dF, countAtomContribution=None,
+                                                derivativesSelector: DerivativesSelector = None,
+                                                anomalous=None):
+        """Returns (f (n_hkl,) complex, TargetFunctionAtomicParamDerivatives)."""
+        sel = (derivativesSelector or DerivativesSelector()).bits()
+        a, keep = self._atoms(atoms, localCoordinateSystems, countAtomContribution, anomalous)
+        hkl_i, dd = self._hkl(hkl, dTarget_dF)
+        nh, n = hkl_i.shape[0], atoms.n
+        f = np.zeros((max(nh, 1), 2))
+        dx = np.zeros((max(n, 1), 3))
+        du = np.zeros((max(n, 1), 6))
+        do = np.zeros(max(n, 1))
+        _lib.check(self._lib.dh_hc_sf_and_derivs(
+            self._h, ctypes.byref(a), nh, _lib.ptr(hkl_i, ctypes.c_int), _lib.ptr(dd, ctypes.c_double), sel,
+            _lib.ptr(f, ctypes.c_double), _lib.ptr(dx, ctypes.c_double), _lib.ptr(du, ctypes.c_double),
+            _lib.ptr(do, ctypes.c_double)))
+        fc = f[:nh, 0] + 1j * f[:nh, 1]
+        return fc, TargetFunctionAtomicParamDerivatives(dx[:n], du[:n], do[:n])
+
+    def calculateStructureFactors(self, atoms: AtomSet, localCoordinateSystems, hkl, countAtomContribution=None,
+                                  anomalous=None):
+        nh = np.asarray(hkl).reshape(-1, 3).shape[0]
+        f, _ = self.calculateStructureFactorsAndDerivatives(
+            atoms, localCoordinateSystems, hkl, np.ones(nh, dtype=np.complex128), countAtomContribution,
+            DerivativesSelector(False, False, False, False), anomalous)
+        return f
+
+    # -- device-resident split (benchmarks, refinement loops) -----------------------------------
+    def upload(self, atoms: AtomSet, localCoordinateSystems, hkl, dTarget_dF, countAtomContribution=None,
+               anomalous=None):
+        a, keep = self._atoms(atoms, localCoordinateSystems, countAtomContribution, anomalous)
+        hkl_i, dd = self._hkl(hkl, dTarget_dF)
+        self._keep = (keep, hkl_i, dd)
+        self._nh, self._n = hkl_i.shape[0], atoms.n
+        _lib.check(self._lib.dh_hc_upload(self._h, ctypes.byref(a), self._nh, _lib.ptr(hkl_i, ctypes.c_int),
+                                          _lib.ptr(dd, ctypes.c_double)))
+
+    def run(self, derivativesSelector: DerivativesSelector = None):
+        sel = (derivativesSelector or DerivativesSelector()).bits()
+        _lib.check(self._lib.dh_hc_run(self._h, sel))
+
+    def fetch(self):
+        nh, n = self._nh, self._n
+        f = np.zeros((max(nh, 1), 2))
+        dx = np.zeros((max(n, 1), 3))
+        du = np.zeros((max(n, 1), 6))
+        do = np.zeros(max(n, 1))
+        _lib.check(self._lib.dh_hc_fetch(self._h, _lib.ptr(f, ctypes.c_double), _lib.ptr(dx, ctypes.c_double),
+                                         _lib.ptr(du, ctypes.c_double), _lib.ptr(do, ctypes.c_double)))
+        return f[:nh, 0] + 1j * f[:nh, 1], TargetFunctionAtomicParamDerivatives(dx[:n], du[:n], do[:n])
+
+    def setTiming(self, on=True):
+        _lib.check(self._lib.dh_hc_set_timing(self._h, 1 if on else 0))
+
+    def lastTiming(self):
+        ms = ctypes.c_double()
+        gx, gy, ch = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(self._lib.dh_hc_last_timing(self._h, ctypes.byref(ms), ctypes.byref(gx), ctypes.byref(gy),
+                                               ctypes.byref(ch)))
+        return {"main_ms": ms.value, "grid": (gx.value, gy.value), "chunk": ch.value}

@@ -1,0 +1,409 @@
+// C ABI and the calculator object behind it: the pre/post-processing of
+// HansenCoppensStructureFactorCalculator::calculateStructureFactorsAndDerivatives
+// (src/Scattering/HansenCoppensStructureFactorCalculator.cpp:380-508) around the HIP engine.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/discamb_amd.h"
+#include "hc_engine.hpp"
+#include "hc_internal.hpp"
+
+namespace discamb_amd {
+
+static thread_local std::string g_last_error;
+
+struct Calculator {
+    Cell cell;
+    SpaceGroupInfo sg;
+    EngineModel model;
+    std::vector<double> mult_w;      // multiplicity / |G|
+    std::vector<int> atom_type, atom_wfn;
+    int xyz_conv = DH_XYZ_FRACTIONAL, adp_conv = DH_ADP_U_CIF, d_xyz_conv = DH_XYZ_FRACTIONAL,
+        d_adp_conv = DH_ADP_U_CIF;
+    bool centric_origin = false;
+    Precision prec = kFp64;
+    std::unique_ptr<Engine> engine;
+    // state of the last upload
+    int n_atoms = 0, n_hkl = 0;
+    std::vector<int> adp_size;
+    std::vector<double> occ;
+    std::vector<double> d10;
+    unsigned last_selector = 0;
+};
+
+static std::vector<EngineOp> make_ops(const Cell& c, const SpaceGroupInfo& g, bool centric_shift)
+{
+    std::vector<EngineOp> ops;
+    auto cart = [&](const int* rf, double* rc) {
+        double t[9];
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double s = 0;
+                for (int k = 0; k < 3; k++) s += c.f2c[3 * i + k] * rf[3 * k + j];
+                t[3 * i + j] = s;
+            }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double s = 0;
+                for (int k = 0; k < 3; k++) s += t[3 * i + k] * c.c2f[3 * k + j];
+                rc[3 * i + j] = s;
+            }
+    };
+    for (const SymOp& o : g.subset) {
+        EngineOp e;
+        std::memcpy(e.rf, o.r, sizeof(e.rf));
+        std::memcpy(e.tf, o.t, sizeof(e.tf));
+        cart(e.rf, e.rc);
+        e.anom_sign = 1.0;
+        ops.push_back(e);
+    }
+    if (centric_shift) {
+        // inversion partners (-R, -t + t_inv); the reference's term2 conjugates the whole
+        // atomic form factor (HansenCoppens_SF_Engine3.cpp:2811-2813), hence anom_sign -1
+        size_t n = ops.size();
+        for (size_t s = 0; s < n; s++) {
+            EngineOp e = ops[s];
+            for (int i = 0; i < 9; i++) {
+                e.rf[i] = -e.rf[i];
+                e.rc[i] = -e.rc[i];
+            }
+            for (int i = 0; i < 3; i++) e.tf[i] = -ops[s].tf[i] + g.inv_t[i];
+            e.anom_sign = -1.0;
+            ops.push_back(e);
+        }
+    }
+    return ops;
+}
+
+static Calculator* create(const dh_model_desc* d, int device)
+{
+    if (!d) throw Error(DH_E_ARGUMENT, "null model description");
+    std::unique_ptr<Calculator> c(new Calculator);
+    c->cell.set(d->cell);
+    if (!(d->cell[0] > 0 && d->cell[1] > 0 && d->cell[2] > 0)) throw Error(DH_E_MODEL, "invalid unit cell");
+    std::vector<SymOp> ops(d->n_symops);
+    for (int s = 0; s < d->n_symops; s++) {
+        std::memcpy(ops[s].r, d->symop_rot + 9 * s, 9 * sizeof(int));
+        std::memcpy(ops[s].t, d->symop_trans + 3 * s, 3 * sizeof(double));
+    }
+    c->sg = analyse_space_group(ops);
+    c->xyz_conv = d->xyz_convention;
+    c->adp_conv = d->adp_convention;
+    c->d_xyz_conv = d->d_xyz_convention;
+    c->d_adp_conv = d->d_adp_convention;
+    // wavefunction types
+    std::vector<WfnType> wfn(d->n_wfn);
+    for (int w = 0; w < d->n_wfn; w++) {
+        WfnType& t = wfn[w];
+        int o0 = d->wfn_orbital_offset[w], o1 = d->wfn_orbital_offset[w + 1];
+        for (int o = o0; o < o1; o++) {
+            SlaterOrbital so;
+            so.n = d->orbital_nl[2 * o];
+            so.l = d->orbital_nl[2 * o + 1];
+            for (int k = d->orbital_term_offset[o]; k < d->orbital_term_offset[o + 1]; k++) {
+                so.coeff.push_back(d->term_coeff[k]);
+                so.expo.push_back(d->term_exp[k]);
+                so.pow.push_back(d->term_pow[k]);
+            }
+            int local = o - o0;
+            t.orbitals.push_back(so);
+            t.occupancy.push_back(d->orbital_occupancy[o]);
+            if (d->orbital_kind[o] == 0) t.core.push_back(local);
+            if (d->orbital_kind[o] == 1) t.valence.push_back(local);
+        }
+        t.def_val_exponent = d->def_val_exponent[w];
+        for (int k = d->def_val_pow_offset[w]; k < d->def_val_pow_offset[w + 1]; k++)
+            t.def_val_power.push_back(d->def_val_pow[k]);
+        t.anomalous = {d->anomalous ? d->anomalous[2 * w] : 0.0, d->anomalous ? d->anomalous[2 * w + 1] : 0.0};
+    }
+    std::vector<AtomType> types(d->n_types);
+    for (int t = 0; t < d->n_types; t++) {
+        types[t].n_l = d->plm_n_l[t];
+        for (int i = 0; i < 25; i++) types[t].plm[i] = d->plm[25 * t + i];
+        types[t].p_val = d->p_val[t];
+        types[t].kappa = d->kappa[t];
+        types[t].kappa_prime = d->kappa_prime[t];
+        if (!(types[t].kappa > 0) || !(types[t].kappa_prime > 0)) throw Error(DH_E_MODEL, "kappa must be positive");
+    }
+    c->atom_type.assign(d->atom_to_type, d->atom_to_type + d->n_atoms);
+    c->atom_wfn.assign(d->atom_to_wfn, d->atom_to_wfn + d->n_atoms);
+    c->model = build_engine_model(wfn, types, c->atom_wfn, c->atom_type);
+    c->mult_w.resize(d->n_atoms);
+    for (int a = 0; a < d->n_atoms; a++) c->mult_w[a] = d->atom_multiplicity[a] / double(c->sg.order);
+    bool zero_t = c->sg.inv_t[0] == 0.0 && c->sg.inv_t[1] == 0.0 && c->sg.inv_t[2] == 0.0;
+    c->centric_origin = c->sg.centro && zero_t;
+    bool centric_shift = c->sg.centro && !zero_t;
+    std::vector<EngineOp> eops = make_ops(c->cell, c->sg, centric_shift);
+    c->engine.reset(new Engine(device, c->model, d->cell, eops, c->centric_origin));
+    return c.release();
+}
+
+static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl, const double* dtdf)
+{
+    if (!at) throw Error(DH_E_ARGUMENT, "null atoms");
+    const int n = at->n_atoms;
+    if (n != (int)c->atom_type.size())
+        throw Error(DH_E_ARGUMENT, "number of atoms differs from the model's atom-to-type map");
+    if (n_hkl < 0) throw Error(DH_E_ARGUMENT, "negative number of reflections");
+    if (n_hkl > 0 && (!hkl || !dtdf))
+        throw Error(DH_E_ARGUMENT,
+                    "Inconsistent size of hkl vectors set and (d target/ d F) table when calculating multipolar "
+                    "structure factors.");
+    const double two_pi2 = 2.0 * M_PI * M_PI;
+    EngineAtoms ea;
+    ea.n = n;
+    ea.xyz_frac.resize(3 * n);
+    ea.u6.assign(6 * n, 0.0);
+    ea.iso.assign(n, 0);
+    ea.weight.resize(n);
+    ea.anom.resize(2 * n);
+    ea.lcs.resize(9 * n);
+    ea.type = c->atom_type;
+    ea.include.resize(n);
+    c->adp_size.resize(n);
+    c->occ.resize(n);
+    for (int i = 0; i < n; i++) {
+        const double* x = at->xyz + 3 * i;
+        if (c->xyz_conv == DH_XYZ_CARTESIAN) {
+            for (int j = 0; j < 3; j++)
+                ea.xyz_frac[3 * i + j] = c->cell.c2f[3 * j] * x[0] + c->cell.c2f[3 * j + 1] * x[1] + c->cell.c2f[3 * j + 2] * x[2];
+        } else {
+            for (int j = 0; j < 3; j++) ea.xyz_frac[3 * i + j] = x[j];
+        }
+        int na = at->adp_size[i];
+        if (na != 0 && na != 1 && na != 6) throw Error(DH_E_ARGUMENT, "ADP must have 0, 1 or 6 components");
+        c->adp_size[i] = na;
+        const double* u = at->adp + 6 * i;
+        if (na == 6) {
+            double uc[6];
+            if (c->adp_conv == DH_ADP_U_CART) std::memcpy(uc, u, sizeof(uc));
+            else c->cell.u_cif_to_cart(u, uc);
+            for (int j = 0; j < 6; j++) ea.u6[6 * i + j] = uc[j] * two_pi2;
+        } else if (na == 1) {
+            ea.iso[i] = 1;
+            for (int j = 0; j < 3; j++) ea.u6[6 * i + j] = u[0] * two_pi2;
+        }
+        c->occ[i] = at->occupancy[i];
+        ea.weight[i] = at->occupancy[i] * c->mult_w[i];
+        int w = c->atom_wfn[i];
+        if (at->anomalous) {
+            ea.anom[2 * i] = at->anomalous[2 * i];
+            ea.anom[2 * i + 1] = at->anomalous[2 * i + 1];
+        } else {
+            ea.anom[2 * i] = c->model.anom[2 * w];
+            ea.anom[2 * i + 1] = c->model.anom[2 * w + 1];
+        }
+        for (int j = 0; j < 9; j++) ea.lcs[9 * i + j] = at->lcs ? at->lcs[9 * i + j] : (j % 4 == 0 ? 1.0 : 0.0);
+        ea.include[i] = at->include ? at->include[i] : 1;
+    }
+    // centering multipliers, dT/dF <- conj(dT/dF) * mult  (.cpp:393-404)
+    std::vector<double> dd(3 * (size_t)n_hkl);
+    for (int k = 0; k < n_hkl; k++) {
+        double m = centering_multiplier(c->sg.centering, c->sg.obverse, hkl + 3 * k);
+        double re = dtdf[2 * k] * m, im = -dtdf[2 * k + 1] * m;
+        if (c->centric_origin) {
+            // F = 2 Re(sum) and derivatives 2 Re(c) Re(.)  (Engine3.cpp:1924-1958)
+            re *= 2.0;
+            im = 0.0;
+        }
+        dd[2 * k] = re;
+        dd[2 * k + 1] = im;
+        dd[2 * (size_t)n_hkl + k] = m;
+    }
+    c->engine->upload(ea, n_hkl, hkl, dd.data());
+    c->n_atoms = n;
+    c->n_hkl = n_hkl;
+}
+
+static void fetch(Calculator* c, double* f, double* d_xyz, double* d_adp, double* d_occ)
+{
+    const int n = c->n_atoms;
+    c->d10.assign(10 * (size_t)std::max(n, 1), 0.0);
+    bool want_d = c->last_selector != 0;
+    c->engine->download(f, want_d ? c->d10.data() : nullptr);
+    if (!want_d) {
+        if (d_xyz) std::memset(d_xyz, 0, sizeof(double) * 3 * n);
+        if (d_adp) std::memset(d_adp, 0, sizeof(double) * 6 * n);
+        if (d_occ) std::memset(d_occ, 0, sizeof(double) * n);
+        return;
+    }
+    const double two_pi = 2.0 * M_PI, two_pi2 = 2.0 * M_PI * M_PI;
+    for (int i = 0; i < n; i++) {
+        const double* r = c->d10.data() + 10 * (size_t)i;
+        double w = c->occ[i] * c->mult_w[i];
+        double dxf[3] = {two_pi * w * r[0], two_pi * w * r[1], two_pi * w * r[2]};
+        if (d_xyz) {
+            if (c->d_xyz_conv == DH_XYZ_CARTESIAN) {
+                // d/dx_cart = c2f^T d/dx_frac
+                for (int j = 0; j < 3; j++)
+                    d_xyz[3 * i + j] = c->cell.c2f[j] * dxf[0] + c->cell.c2f[3 + j] * dxf[1] + c->cell.c2f[6 + j] * dxf[2];
+            } else {
+                for (int j = 0; j < 3; j++) d_xyz[3 * i + j] = dxf[j];
+            }
+            if (!(c->last_selector & DH_D_XYZ)) d_xyz[3 * i] = d_xyz[3 * i + 1] = d_xyz[3 * i + 2] = 0.0;
+        }
+        if (d_adp) {
+            double* o = d_adp + 6 * (size_t)i;
+            for (int j = 0; j < 6; j++) o[j] = 0.0;
+            if (c->last_selector & DH_D_ADP) {
+                double du[6];
+                for (int j = 0; j < 6; j++) du[j] = two_pi2 * w * r[3 + j];
+                if (c->adp_size[i] == 1) o[0] = du[0] + du[1] + du[2];
+                else if (c->adp_size[i] == 6) {
+                    if (c->d_adp_conv == DH_ADP_U_CART) std::memcpy(o, du, sizeof(du));
+                    else c->cell.d_ucart_to_d_ucif(du, o);
+                }
+            }
+        }
+        if (d_occ) d_occ[i] = (c->last_selector & DH_D_OCC) ? c->mult_w[i] * r[9] : 0.0;
+    }
+}
+
+}  // namespace discamb_amd
+
+using namespace discamb_amd;
+
+struct dh_calculator {
+    Calculator* c;
+};
+
+template <typename F>
+static int guarded(F&& f)
+{
+    try {
+        f();
+        return DH_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return DH_E_ARGUMENT;
+    }
+}
+
+extern "C" {
+
+const char* dh_last_error(void) { return g_last_error.c_str(); }
+
+const char* dh_version(void) { return "discamb_amd 0.1 (gfx950)"; }
+
+int dh_device_count(int* n)
+{
+    return guarded([&] {
+        int k = 0;
+        if (hipGetDeviceCount(&k) != hipSuccess) k = 0;
+        *n = k;
+    });
+}
+
+int dh_hc_create(const dh_model_desc* model, int device, dh_calculator** out)
+{
+    return guarded([&] {
+        if (!out) throw Error(DH_E_ARGUMENT, "null output pointer");
+        *out = nullptr;
+        Calculator* c = create(model, device);
+        *out = new dh_calculator{c};
+    });
+}
+
+void dh_hc_destroy(dh_calculator* calc)
+{
+    if (!calc) return;
+    delete calc->c;
+    delete calc;
+}
+
+int dh_hc_set_calculation_mode(dh_calculator* calc, const char* mode)
+{
+    return guarded([&] {
+        if (!calc || !mode) throw Error(DH_E_ARGUMENT, "null argument");
+        std::string m(mode);
+        if (m == "HIP" || m == "GPU" || m == "CPU") calc->c->prec = kFp64;
+        else if (m == "HIP_MIXED") calc->c->prec = kMixed;
+        else throw Error(DH_E_ARGUMENT, "unknown calculation mode for multipolar structure factor engine");
+    });
+}
+
+int dh_hc_set_tuning(dh_calculator* calc, int hkl_chunk)
+{
+    return guarded([&] {
+        if (!calc || hkl_chunk < 0) throw Error(DH_E_ARGUMENT, "invalid tuning");
+        calc->c->engine->tune.hkl_chunk = hkl_chunk;
+    });
+}
+
+int dh_hc_upload(dh_calculator* calc, const dh_atoms* atoms, int n_hkl, const int* hkl, const double* dtarget_df)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        upload(calc->c, atoms, n_hkl, hkl, dtarget_df);
+    });
+}
+
+int dh_hc_run(dh_calculator* calc, unsigned selector)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        calc->c->last_selector = selector & DH_D_ALL;
+        calc->c->engine->run(calc->c->prec, calc->c->last_selector != 0);
+    });
+}
+
+int dh_hc_fetch(dh_calculator* calc, double* f, double* d_xyz, double* d_adp, double* d_occ)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        fetch(calc->c, f, d_xyz, d_adp, d_occ);
+    });
+}
+
+int dh_hc_sf_and_derivs(dh_calculator* calc, const dh_atoms* atoms, int n_hkl, const int* hkl,
+                        const double* dtarget_df, unsigned selector, double* f, double* d_xyz, double* d_adp,
+                        double* d_occ)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        upload(calc->c, atoms, n_hkl, hkl, dtarget_df);
+        calc->c->last_selector = selector & DH_D_ALL;
+        calc->c->engine->run(calc->c->prec, calc->c->last_selector != 0);
+        fetch(calc->c, f, d_xyz, d_adp, d_occ);
+    });
+}
+
+int dh_hc_set_timing(dh_calculator* calc, int on)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        calc->c->engine->timing = on != 0;
+    });
+}
+
+int dh_hc_last_timing(dh_calculator* calc, double* ms, int* gx, int* gy, int* chunk)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        Engine& e = *calc->c->engine;
+        if (ms) *ms = e.last_main_ms;
+        if (gx) *gx = e.last_grid_x;
+        if (gy) *gy = e.last_grid_y;
+        if (chunk) *chunk = e.last_chunk;
+    });
+}
+
+int dh_hc_engine_info(dh_calculator* calc, int* n_ops, int* order, int* centro, char* centering)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        if (n_ops) *n_ops = calc->c->engine->n_ops();
+        if (order) *order = calc->c->sg.order;
+        if (centro) *centro = calc->c->sg.centro ? 1 : 0;
+        if (centering) *centering = calc->c->sg.centering;
+    });
+}
+
+}  // extern "C"

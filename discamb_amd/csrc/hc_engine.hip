@@ -1,0 +1,807 @@
+// HIP engine for the Hansen-Coppens structure factor F(h) and the adjoint dT/dp (xyz, U, occ).
+//
+// Work decomposition (one GPU):
+//   * prep kernels build, per call, (a) per (reflection, symmetry op) records - density-normalised
+//     real harmonics of the rotated unit vector, Debye-Waller multipliers and the integer
+//     fractional phase vector - (b) per (reflection, atom type) radial records - core +
+//     Pval * valence Slater transforms and the five deformation-valence Fourier-Bessel terms - and
+//     (c) per-atom records with P_lm rotated into the global frame;
+//   * the main kernel is atom-stationary: each lane owns one atom (kept in registers), a wave
+//     streams a chunk of reflections whose records are wave-uniform (scalar loads), evaluates
+//     X = f T exp(i phi) per (atom, op, reflection), reduces w * sum_s X across the wave for
+//     F(h) and accumulates the adjoint in registers - no atomics anywhere;
+//   * two atomic-free stage-2 kernels sum the per-wave F partials and per-chunk derivative
+//     partials in FP64 in a fixed order (bitwise reproducible).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "hc_device_math.hpp"
+#include "hc_engine.hpp"
+
+namespace discamb_amd {
+
+#define HIP_CHECK(x)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess)                                                                     \
+            throw Error(3, std::string("HIP error ") + hipGetErrorString(e_) + " at " #x);       \
+    } while (0)
+
+static constexpr int kHklopRec = 36;    // doubles / floats per (reflection, op) record
+static constexpr int kRadRec = 8;       // per (reflection, type) radial record
+static constexpr int kWave = 64;
+static constexpr int kWavesPerBlock = 4;
+
+struct TypeRec {
+    int wfn, nl, max_l, pad;
+    int n[5], pad2[3];
+    double pval, kappa, kappa_p, zeta;
+    double plm[25];
+};
+
+struct WfnRec {
+    int core_off, core_n, val_off, val_n;
+};
+
+struct TermRec {
+    double c, e;
+    int p, pad;
+};
+
+struct AtomIn {
+    double xf[3];
+    double u[6];
+    double w;
+    double anom[2];
+    double lcs[9];
+    int type;
+    int orig;
+};
+
+struct AtomRec {
+    double xf[3];
+    double u[6];
+    double w;
+    double anom[2];
+    double plm[25];
+    int type;
+    int orig;
+};
+
+void DeviceBuffer::ensure(size_t n)
+{
+    if (n <= bytes && p) return;
+    release();
+    size_t want = std::max<size_t>(n, 256);
+    if (hipMalloc(&p, want) != hipSuccess) {
+        p = nullptr;
+        bytes = 0;
+        throw Error(4, "device allocation failed");
+    }
+    bytes = want;
+}
+
+void DeviceBuffer::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+// ------------------------------------------------------------------ prep kernels
+
+struct OpDev {
+    int rf[9];
+    double tf[3];
+    double rc[9];
+    double anom_sign;
+};
+
+__global__ void k_hkl_prep(const int* __restrict__ hkl, const OpDev* __restrict__ ops, int n_ops, int n_hkl,
+                           const double* __restrict__ c2f, double* __restrict__ rec64, float* __restrict__ rec32,
+                           double* __restrict__ phase32, double* __restrict__ hlen)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_hkl) return;
+    const int h0 = hkl[3 * k], h1 = hkl[3 * k + 1], h2 = hkl[3 * k + 2];
+    double hc[3];
+    for (int j = 0; j < 3; j++) hc[j] = h0 * c2f[j] + h1 * c2f[3 + j] + h2 * c2f[6 + j];
+    double hl = sqrt(hc[0] * hc[0] + hc[1] * hc[1] + hc[2] * hc[2]);
+    hlen[k] = hl;
+    bool h000 = hl < 1e-10;
+    for (int s = 0; s < n_ops; s++) {
+        const OpDev& o = ops[s];
+        double hs[3], hf[3];
+        for (int j = 0; j < 3; j++) {
+            hs[j] = hc[0] * o.rc[j] + hc[1] * o.rc[3 + j] + hc[2] * o.rc[6 + j];
+            hf[j] = double(h0 * o.rf[j] + h1 * o.rf[3 + j] + h2 * o.rf[6 + j]);
+        }
+        double ht = h0 * o.tf[0] + h1 * o.tf[1] + h2 * o.tf[2];
+        double y[25];
+        if (h000) {
+            for (int i = 0; i < 25; i++) y[i] = 0.0;
+        } else {
+            double inv = 1.0 / hl;
+            ylm_density<double>(hs[0] * inv, hs[1] * inv, hs[2] * inv, y);
+        }
+        double m[6] = {hs[0] * hs[0], hs[1] * hs[1], hs[2] * hs[2], 2 * hs[0] * hs[1], 2 * hs[0] * hs[2],
+                       2 * hs[1] * hs[2]};
+        size_t base = ((size_t)k * n_ops + s) * kHklopRec;
+        if (rec64) {
+            double* r = rec64 + base;
+            for (int i = 0; i < 25; i++) r[i] = y[i];
+            for (int i = 0; i < 6; i++) r[25 + i] = m[i];
+            for (int i = 0; i < 3; i++) r[31 + i] = hf[i];
+            r[34] = ht;
+            r[35] = o.anom_sign;
+        }
+        if (rec32) {
+            float* r = rec32 + base;
+            for (int i = 0; i < 25; i++) r[i] = (float)y[i];
+            // Debye-Waller multipliers pre-scaled by log2(e) for exp2
+            for (int i = 0; i < 6; i++) r[25 + i] = (float)(m[i] * 1.4426950408889634074);
+            for (int i = 0; i < 3; i++) r[31 + i] = (float)hf[i];
+            r[34] = 0.f;
+            r[35] = (float)o.anom_sign;
+            double* p = phase32 + ((size_t)k * n_ops + s) * 4;
+            p[0] = hf[0]; p[1] = hf[1]; p[2] = hf[2]; p[3] = ht;
+        }
+    }
+}
+
+// S = f_core + Pval f_val(h/kappa);  G_l = 4 pi i^l-sign N_l g_l(n_l + 2, h/kappa', zeta)
+__global__ void k_radial_prep(const double* __restrict__ hlen, const TypeRec* __restrict__ types,
+                              const WfnRec* __restrict__ wfns, const TermRec* __restrict__ core,
+                              const TermRec* __restrict__ val, int n_types, int n_hkl,
+                              double* __restrict__ rad64, float* __restrict__ rad32)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_hkl * n_types) return;
+    int k = idx / n_types, t = idx - k * n_types;
+    const double four_pi = 12.566370614359172953850;
+    double h = hlen[k];
+    const TypeRec& ty = types[t];
+    const WfnRec& wf = wfns[ty.wfn];
+    double fc = 0.0;
+    for (int i = 0; i < wf.core_n; i++) {
+        const TermRec& tr = core[wf.core_off + i];
+        fc += tr.c * g_function(0, tr.p + 2, h, tr.e);
+    }
+    double fv = 0.0;
+    double hk = h / ty.kappa;
+    for (int i = 0; i < wf.val_n; i++) {
+        const TermRec& tr = val[wf.val_off + i];
+        fv += tr.c * g_function(0, tr.p + 2, hk, tr.e);
+    }
+    double r[kRadRec] = {0, 0, 0, 0, 0, 0, 0, 0};
+    r[0] = four_pi * fc + ty.pval * four_pi * fv;
+    double hp = h / ty.kappa_p;
+    for (int l = 0; l <= ty.max_l && l < ty.nl; l++) {
+        int n = ty.n[l];
+        double norm = 1.0;
+        for (int q = 0; q < n + 3; q++) norm *= ty.zeta;
+        for (int q = 2; q <= n + 2; q++) norm /= q;
+        double g = four_pi * norm * g_function(l, n + 2, hp, ty.zeta);
+        // i^l: l = 0 (+, real), 1 (+, imag), 2 (-, real), 3 (-, imag), 4 (+, real)
+        r[1 + l] = (l == 2 || l == 3) ? -g : g;
+    }
+    size_t base = (size_t)idx * kRadRec;
+    if (rad64)
+        for (int i = 0; i < kRadRec; i++) rad64[base + i] = r[i];
+    if (rad32)
+        for (int i = 0; i < kRadRec; i++) rad32[base + i] = (float)r[i];
+}
+
+__global__ void k_atom_prep(const AtomIn* __restrict__ in, const TypeRec* __restrict__ types,
+                            AtomRec* __restrict__ out, int n_slots)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_slots) return;
+    const AtomIn& a = in[i];
+    AtomRec r;
+    for (int j = 0; j < 3; j++) r.xf[j] = a.xf[j];
+    for (int j = 0; j < 6; j++) r.u[j] = a.u[j];
+    r.w = a.w;
+    r.anom[0] = a.anom[0];
+    r.anom[1] = a.anom[1];
+    r.type = a.type;
+    r.orig = a.orig;
+    for (int j = 0; j < 25; j++) r.plm[j] = 0.0;
+    const TypeRec& ty = types[a.type];
+    if (a.orig >= 0 && ty.max_l >= 0) {
+        const int nlm = (ty.max_l + 1) * (ty.max_l + 1);
+        for (int p = 0; p < 50; p++) {
+            double u0 = c_proj_dir[p][0], u1 = c_proj_dir[p][1], u2 = c_proj_dir[p][2];
+            // local coordinates: columns of the LCS are the local axes
+            double x = a.lcs[0] * u0 + a.lcs[3] * u1 + a.lcs[6] * u2;
+            double y = a.lcs[1] * u0 + a.lcs[4] * u1 + a.lcs[7] * u2;
+            double z = a.lcs[2] * u0 + a.lcs[5] * u1 + a.lcs[8] * u2;
+            double d[25];
+            ylm_density<double>(x, y, z, d);
+            double f = 0.0;
+            for (int j = 0; j < nlm; j++) f += ty.plm[j] * d[j];
+            for (int j = 0; j < nlm; j++) r.plm[j] += c_proj_w[p][j] * f;
+        }
+    }
+    out[i] = r;
+}
+
+// ------------------------------------------------------------------ main kernels
+
+struct MainArgs {
+    const void* hklop;        // double[36] or float[36] per (k, s)
+    const double* phase;      // mixed path: double[4] per (k, s)
+    const void* radial;       // [k][t][8]
+    const double* dtdf;       // [k][2]
+    const AtomRec* atoms;     // sorted slots
+    const int* wave_l;        // multipole class (2 or 4) per wave
+    double* fpart;            // [wave][k][2]
+    double* dpart;            // [chunk][slot][10]
+    int n_hkl, n_ops, n_types, n_slots, n_waves, chunk;
+};
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+template <int L, bool DERIV>
+__device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane, int k0, int k1)
+{
+    constexpr int NLM = (L + 1) * (L + 1);
+    const int slot = wave * kWave + lane;
+    const AtomRec& at = a.atoms[slot];
+    const double x0 = at.xf[0], x1 = at.xf[1], x2 = at.xf[2];
+    double u[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) u[j] = at.u[j];
+    double P[NLM];
+#pragma unroll
+    for (int j = 0; j < NLM; j++) P[j] = at.plm[j];
+    const double w = at.w, anr = at.anom[0], ani = at.anom[1];
+    const int t = at.type;
+    double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
+    const double* __restrict__ hop = static_cast<const double*>(a.hklop);
+    const double* __restrict__ rad = static_cast<const double*>(a.radial);
+    for (int k = k0; k < k1; ++k) {
+        const double* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
+        const double S = rr[0] + anr, G0 = rr[1], G1 = rr[2], G2 = rr[3], G3 = rr[4], G4 = rr[5];
+        const double cr = a.dtdf[2 * k], ci = a.dtdf[2 * k + 1];
+        double sr = 0, si = 0;
+        for (int s = 0; s < a.n_ops; ++s) {
+            const double* r = hop + ((size_t)k * a.n_ops + s) * kHklopRec;
+            double A0 = P[0] * r[0];
+            double A1 = P[1] * r[1] + P[2] * r[2] + P[3] * r[3];
+            double A2 = 0;
+#pragma unroll
+            for (int j = 4; j < 9; j++) A2 += P[j] * r[j];
+            double fr = S + G0 * A0 + G2 * A2;
+            double fi = ani * r[35] + G1 * A1;
+            if (L == 4) {
+                double A3 = 0, A4 = 0;
+#pragma unroll
+                for (int j = 9; j < 16; j++) A3 += P[j < NLM ? j : 0] * r[j];
+#pragma unroll
+                for (int j = 16; j < 25; j++) A4 += P[j < NLM ? j : 0] * r[j];
+                fr += G4 * A4;
+                fi += G3 * A3;
+            }
+            double ph = r[31] * x0 + r[32] * x1 + r[33] * x2 + r[34];
+            ph -= rint(ph);
+            double sn, cs;
+            sincospi(2.0 * ph, &sn, &cs);
+            double e = r[25] * u[0] + r[26] * u[1] + r[27] * u[2] + r[28] * u[3] + r[29] * u[4] + r[30] * u[5];
+            double T = exp(-e);
+            double Xr = T * (fr * cs - fi * sn), Xi = T * (fr * sn + fi * cs);
+            sr += Xr;
+            si += Xi;
+            if (DERIV) {
+                double zr = cr * Xr - ci * Xi, zi = cr * Xi + ci * Xr;
+#pragma unroll
+                for (int j = 0; j < 6; j++) du[j] -= r[25 + j] * zr;
+#pragma unroll
+                for (int j = 0; j < 3; j++) dx[j] -= r[31 + j] * zi;
+            }
+        }
+        if (DERIV) docc += cr * sr - ci * si;
+        double fr_w = wave_sum(w * sr), fi_w = wave_sum(w * si);
+        if (lane == 0) {
+            double* fp = a.fpart + ((size_t)wave * a.n_hkl + k) * 2;
+            fp[0] = fr_w;
+            fp[1] = fi_w;
+        }
+    }
+    if (DERIV) {
+        double* dp = a.dpart + ((size_t)blockIdx.y * a.n_slots + slot) * 10;
+        dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
+#pragma unroll
+        for (int j = 0; j < 6; j++) dp[3 + j] = du[j];
+        dp[9] = docc;
+    }
+}
+
+// Mixed precision: FP64 phase range reduction and FP64 accumulation across reflections, FP32
+// harmonics / radial / Debye-Waller / sincos / exp per term.
+template <int L, bool DERIV>
+__device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane, int k0, int k1)
+{
+    constexpr int NLM = (L + 1) * (L + 1);
+    constexpr int kFlush = 16;    // reflections accumulated in FP32 before an FP64 flush
+    const int slot = wave * kWave + lane;
+    const AtomRec& at = a.atoms[slot];
+    const double x0 = at.xf[0], x1 = at.xf[1], x2 = at.xf[2];
+    float u[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) u[j] = (float)at.u[j];
+    float P[NLM];
+#pragma unroll
+    for (int j = 0; j < NLM; j++) P[j] = (float)at.plm[j];
+    const float w = (float)at.w, anr = (float)at.anom[0], ani = (float)at.anom[1];
+    const int t = at.type;
+    double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
+    const float* __restrict__ hop = static_cast<const float*>(a.hklop);
+    const float* __restrict__ rad = static_cast<const float*>(a.radial);
+    for (int kb = k0; kb < k1; kb += kFlush) {
+        const int kend = min(kb + kFlush, k1);
+        float fx[3] = {0, 0, 0}, fu[6] = {0, 0, 0, 0, 0, 0}, focc = 0;
+        for (int k = kb; k < kend; ++k) {
+            const float* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
+            const float S = rr[0] + anr, G0 = rr[1], G1 = rr[2], G2 = rr[3], G3 = rr[4], G4 = rr[5];
+            const float cr = (float)a.dtdf[2 * k], ci = (float)a.dtdf[2 * k + 1];
+            float sr = 0, si = 0;
+            for (int s = 0; s < a.n_ops; ++s) {
+                const float* r = hop + ((size_t)k * a.n_ops + s) * kHklopRec;
+                const double* pr = a.phase + ((size_t)k * a.n_ops + s) * 4;
+                float A0 = P[0] * r[0];
+                float A1 = P[1] * r[1] + P[2] * r[2] + P[3] * r[3];
+                float A2 = 0;
+#pragma unroll
+                for (int j = 4; j < 9; j++) A2 += P[j] * r[j];
+                float fr = S + G0 * A0 + G2 * A2;
+                float fi = ani * r[35] + G1 * A1;
+                if (L == 4) {
+                    float A3 = 0, A4 = 0;
+#pragma unroll
+                    for (int j = 9; j < 16; j++) A3 += P[j < NLM ? j : 0] * r[j];
+#pragma unroll
+                    for (int j = 16; j < 25; j++) A4 += P[j < NLM ? j : 0] * r[j];
+                    fr += G4 * A4;
+                    fi += G3 * A3;
+                }
+                // FP64 range reduction of h.(R x + t) to [-1/2, 1/2) revolutions
+                double ph = pr[0] * x0 + pr[1] * x1 + pr[2] * x2 + pr[3];
+                float phf = (float)(ph - rint(ph));
+                float sn = __builtin_amdgcn_sinf(phf), cs = __builtin_amdgcn_cosf(phf);
+                float e = r[25] * u[0] + r[26] * u[1] + r[27] * u[2] + r[28] * u[3] + r[29] * u[4] + r[30] * u[5];
+                float T = __builtin_amdgcn_exp2f(-e);
+                float Xr = T * (fr * cs - fi * sn), Xi = T * (fr * sn + fi * cs);
+                sr += Xr;
+                si += Xi;
+                if (DERIV) {
+                    float zr = cr * Xr - ci * Xi, zi = cr * Xi + ci * Xr;
+#pragma unroll
+                    for (int j = 0; j < 6; j++) fu[j] -= r[25 + j] * zr;
+#pragma unroll
+                    for (int j = 0; j < 3; j++) fx[j] -= r[31 + j] * zi;
+                }
+            }
+            if (DERIV) focc += cr * sr - ci * si;
+            float fr_w = wave_sum(w * sr), fi_w = wave_sum(w * si);
+            if (lane == 0) {
+                double* fp = a.fpart + ((size_t)wave * a.n_hkl + k) * 2;
+                fp[0] = fr_w;
+                fp[1] = fi_w;
+            }
+        }
+        if (DERIV) {
+#pragma unroll
+            for (int j = 0; j < 3; j++) dx[j] += fx[j];
+#pragma unroll
+            for (int j = 0; j < 6; j++) du[j] += fu[j];
+            docc += focc;
+        }
+    }
+    if (DERIV) {
+        // U multipliers carried log2(e); undo for the derivative
+        const double inv_log2e = 0.69314718055994530942;
+        double* dp = a.dpart + ((size_t)blockIdx.y * a.n_slots + slot) * 10;
+        dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
+#pragma unroll
+        for (int j = 0; j < 6; j++) dp[3 + j] = du[j] * inv_log2e;
+        dp[9] = docc;
+    }
+}
+
+template <bool MIXED, bool DERIV>
+__global__ __launch_bounds__(256) void k_main(MainArgs a)
+{
+    const int wave = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x / kWave);
+    if (wave >= a.n_waves) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int k0 = blockIdx.y * a.chunk;
+    const int k1 = min(k0 + a.chunk, a.n_hkl);
+    const int cls = __builtin_amdgcn_readfirstlane(a.wave_l[wave]);
+    if (MIXED) {
+        if (cls <= 2) body_mixed<2, DERIV>(a, wave, lane, k0, k1);
+        else body_mixed<4, DERIV>(a, wave, lane, k0, k1);
+    } else {
+        if (cls <= 2) body_fp64<2, DERIV>(a, wave, lane, k0, k1);
+        else body_fp64<4, DERIV>(a, wave, lane, k0, k1);
+    }
+}
+
+// F(h) = mult(h) * sum_waves partial  (centric at origin: 2 Re, 0)
+__global__ void k_reduce_f(const double* __restrict__ fpart, const double* __restrict__ mult, int n_waves,
+                           int n_hkl, int centric_origin, double* __restrict__ f)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_hkl) return;
+    double re = 0, im = 0;
+    for (int w = 0; w < n_waves; w++) {
+        re += fpart[((size_t)w * n_hkl + k) * 2];
+        im += fpart[((size_t)w * n_hkl + k) * 2 + 1];
+    }
+    if (centric_origin) {
+        re *= 2;
+        im = 0;
+    }
+    f[2 * k] = re * mult[k];
+    f[2 * k + 1] = im * mult[k];
+}
+
+__global__ void k_reduce_d(const double* __restrict__ dpart, const AtomRec* __restrict__ atoms, int n_chunks,
+                           int n_slots, double* __restrict__ d)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_slots) return;
+    int orig = atoms[i].orig;
+    if (orig < 0) return;
+    double s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < n_chunks; c++)
+        for (int j = 0; j < 10; j++) s[j] += dpart[((size_t)c * n_slots + i) * 10 + j];
+    for (int j = 0; j < 10; j++) d[(size_t)orig * 10 + j] = s[j];
+}
+
+// ------------------------------------------------------------------ host side
+
+static bool g_constants_ready[64] = {false};
+
+static void legendre_nodes5(double* x, double* w)
+{
+    const int n = 5;
+    for (int i = 0; i < n; i++) {
+        double z = std::cos(3.14159265358979323846 * (i + 0.75) / (n + 0.5));
+        for (int it = 0; it < 100; it++) {
+            double p1 = 1.0, p2 = 0.0;
+            for (int j = 0; j < n; j++) {
+                double p3 = p2;
+                p2 = p1;
+                p1 = ((2.0 * j + 1.0) * z * p2 - j * p3) / (j + 1);
+            }
+            double pp = n * (z * p1 - p2) / (z * z - 1.0);
+            double z1 = z;
+            z = z1 - p1 / pp;
+            if (std::fabs(z - z1) < 1e-16) {
+                w[i] = 2.0 / ((1.0 - z * z) * pp * pp);
+                break;
+            }
+            w[i] = 2.0 / ((1.0 - z * z) * pp * pp);
+        }
+        x[i] = z;
+    }
+}
+
+static void host_ylm(const double* u, double* d)
+{
+    double x = u[0], y = u[1], z = u[2], xx = x * x, yy = y * y, zz = z * z;
+    double v[25] = {1.0, y, z, x, x * y, y * z, 2 * zz - xx - yy, x * z, xx - yy, (3 * xx - yy) * y, x * y * z,
+                    y * (4 * zz - xx - yy), z * (2 * zz - 3 * xx - 3 * yy), x * (4 * zz - xx - yy), (xx - yy) * z,
+                    (xx - 3 * yy) * x, x * y * (xx - yy), (3 * xx - yy) * y * z, x * y * (7 * zz - 1),
+                    y * z * (7 * zz - 3), 35 * zz * zz - 30 * zz + 3, x * z * (7 * zz - 3), (xx - yy) * (7 * zz - 1),
+                    (xx - 3 * yy) * x * z, xx * (xx - 3 * yy) - yy * (3 * xx - yy)};
+    for (int i = 0; i < 25; i++) d[i] = v[i] * kHcDensityNorm[i];
+}
+
+static void upload_constants(int device)
+{
+    if (device >= 0 && device < 64 && g_constants_ready[device]) return;
+    HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_density_norm), kHcDensityNorm, sizeof(kHcDensityNorm)));
+    static_assert(sizeof(kHcGCoef) <= 400 * sizeof(double), "g-function table too large");
+    HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_gcoef), kHcGCoef, sizeof(kHcGCoef)));
+    HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_goff), kHcGOffset, sizeof(kHcGOffset)));
+    double gx[5], gw[5];
+    legendre_nodes5(gx, gw);
+    double dir[50][3], wq[50];
+    int p = 0;
+    for (int a = 0; a < 5; a++)
+        for (int b = 0; b < 10; b++) {
+            double phi = 2 * 3.14159265358979323846 * b / 10.0;
+            double z = gx[a], rho = std::sqrt(1 - z * z);
+            dir[p][0] = rho * std::cos(phi);
+            dir[p][1] = rho * std::sin(phi);
+            dir[p][2] = z;
+            wq[p] = gw[a] * 2 * 3.14159265358979323846 / 10.0;
+            p++;
+        }
+    double norm[25] = {0}, dv[50][25];
+    for (int q = 0; q < 50; q++) {
+        host_ylm(dir[q], dv[q]);
+        for (int i = 0; i < 25; i++) norm[i] += wq[q] * dv[q][i] * dv[q][i];
+    }
+    double pw[50][25];
+    for (int q = 0; q < 50; q++)
+        for (int i = 0; i < 25; i++) pw[q][i] = wq[q] * dv[q][i] / norm[i];
+    HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_proj_dir), dir, sizeof(dir)));
+    HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_proj_w), pw, sizeof(pw)));
+    if (device >= 0 && device < 64) g_constants_ready[device] = true;
+}
+
+Engine::Engine(int device, const EngineModel& model, const double* cell6, const std::vector<EngineOp>& ops,
+               bool centric_origin)
+    : device_(device), model_(model), ops_(ops), centric_origin_(centric_origin)
+{
+    int n_dev = 0;
+    if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev == 0) throw Error(5, "no HIP device available");
+    if (device < 0 || device >= n_dev) throw Error(5, "HIP device index out of range");
+    HIP_CHECK(hipSetDevice(device));
+    upload_constants(device);
+    HIP_CHECK(hipStreamCreate(&stream_));
+    HIP_CHECK(hipEventCreate(&ev0_));
+    HIP_CHECK(hipEventCreate(&ev1_));
+    Cell c;
+    c.set(cell6);
+    std::memcpy(f2c_, c.f2c, sizeof(f2c_));
+    std::memcpy(c2f_, c.c2f, sizeof(c2f_));
+    // model tables
+    std::vector<TermRec> core, val;
+    std::vector<WfnRec> wr(model.n_wfn);
+    for (int w = 0; w < model.n_wfn; w++) {
+        wr[w].core_off = (int)core.size();
+        wr[w].core_n = (int)model.core[w].c.size();
+        for (size_t i = 0; i < model.core[w].c.size(); i++)
+            core.push_back({model.core[w].c[i], model.core[w].e[i], model.core[w].p[i], 0});
+        wr[w].val_off = (int)val.size();
+        wr[w].val_n = (int)model.valence[w].c.size();
+        for (size_t i = 0; i < model.valence[w].c.size(); i++)
+            val.push_back({model.valence[w].c[i], model.valence[w].e[i], model.valence[w].p[i], 0});
+    }
+    std::vector<TypeRec> tr(model.n_types);
+    for (int t = 0; t < model.n_types; t++) {
+        TypeRec& r = tr[t];
+        std::memset(&r, 0, sizeof(r));
+        r.wfn = model.type_wfn[t];
+        r.nl = model.dv_nl[r.wfn];
+        r.max_l = model.type_max_l[t];
+        for (int l = 0; l < 5; l++) r.n[l] = model.dv_n[5 * r.wfn + l];
+        r.pval = model.types[t].p_val;
+        r.kappa = model.types[t].kappa;
+        r.kappa_p = model.types[t].kappa_prime;
+        r.zeta = model.dv_zeta[r.wfn];
+        for (int l = 0; l <= r.max_l; l++)
+            for (int i = l * l; i < (l + 1) * (l + 1); i++) r.plm[i] = model.types[t].plm[i];
+    }
+    size_t bytes_ct = std::max<size_t>(1, core.size()) * sizeof(TermRec);
+    size_t bytes_vt = std::max<size_t>(1, val.size()) * sizeof(TermRec);
+    d_core_.ensure(bytes_ct);
+    d_val_.ensure(bytes_vt);
+    d_type_.ensure(std::max<size_t>(1, tr.size()) * sizeof(TypeRec) + std::max<size_t>(1, wr.size()) * sizeof(WfnRec));
+    if (!core.empty()) HIP_CHECK(hipMemcpy(d_core_.p, core.data(), core.size() * sizeof(TermRec), hipMemcpyHostToDevice));
+    if (!val.empty()) HIP_CHECK(hipMemcpy(d_val_.p, val.data(), val.size() * sizeof(TermRec), hipMemcpyHostToDevice));
+    if (!tr.empty()) HIP_CHECK(hipMemcpy(d_type_.p, tr.data(), tr.size() * sizeof(TypeRec), hipMemcpyHostToDevice));
+    if (!wr.empty())
+        HIP_CHECK(hipMemcpy(static_cast<char*>(d_type_.p) + tr.size() * sizeof(TypeRec), wr.data(),
+                            wr.size() * sizeof(WfnRec), hipMemcpyHostToDevice));
+    std::vector<OpDev> od(ops.size());
+    for (size_t s = 0; s < ops.size(); s++) {
+        std::memcpy(od[s].rf, ops[s].rf, sizeof(od[s].rf));
+        std::memcpy(od[s].tf, ops[s].tf, sizeof(od[s].tf));
+        std::memcpy(od[s].rc, ops[s].rc, sizeof(od[s].rc));
+        od[s].anom_sign = ops[s].anom_sign;
+    }
+    d_ops_.ensure(od.size() * sizeof(OpDev) + 9 * sizeof(double));
+    HIP_CHECK(hipMemcpy(d_ops_.p, od.data(), od.size() * sizeof(OpDev), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(static_cast<char*>(d_ops_.p) + od.size() * sizeof(OpDev), c2f_, 9 * sizeof(double),
+                        hipMemcpyHostToDevice));
+}
+
+Engine::~Engine()
+{
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::synchronize() { HIP_CHECK(hipStreamSynchronize(stream_)); }
+
+void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const double* dtdf)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    if (n_hkl < 0 || at.n < 0) throw Error(1, "negative size");
+    n_atoms_ = at.n;
+    n_hkl_ = n_hkl;
+    for (int i = 0; i < at.n; i++)
+        if (at.type[i] < 0 || at.type[i] >= model_.n_types) throw Error(1, "atom type index out of range");
+    // sort included atoms by multipole class, then type; pad each class to whole waves
+    std::vector<int> idx;
+    for (int i = 0; i < at.n; i++)
+        if (at.include[i]) idx.push_back(i);
+    auto cls_of = [&](int i) { return model_.type_max_l[at.type[i]] <= 2 ? 2 : 4; };
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+        int cx = cls_of(x), cy = cls_of(y);
+        if (cx != cy) return cx < cy;
+        return at.type[x] < at.type[y];
+    });
+    order_.clear();
+    std::vector<int> wave_cls;
+    for (int pass = 2; pass <= 4; pass += 2) {
+        size_t start = order_.size();
+        for (int i : idx)
+            if (cls_of(i) == pass) order_.push_back(i);
+        size_t cnt = order_.size() - start;
+        size_t padded = (cnt + kWave - 1) / kWave * kWave;
+        while (order_.size() < start + padded) order_.push_back(-1);
+        for (size_t wv = 0; wv < padded / kWave; wv++) wave_cls.push_back(pass);
+    }
+    n_sorted_ = (int)order_.size();
+    std::vector<AtomIn> ain(std::max(n_sorted_, 1));
+    for (int s = 0; s < n_sorted_; s++) {
+        AtomIn& r = ain[s];
+        std::memset(&r, 0, sizeof(r));
+        int i = order_[s];
+        r.orig = i;
+        if (i < 0) {
+            r.type = 0;
+            r.lcs[0] = r.lcs[4] = r.lcs[8] = 1.0;
+            continue;
+        }
+        for (int j = 0; j < 3; j++) r.xf[j] = at.xyz_frac[3 * i + j];
+        for (int j = 0; j < 6; j++) r.u[j] = at.u6[6 * i + j];
+        r.w = at.weight[i];
+        r.anom[0] = at.anom[2 * i];
+        r.anom[1] = at.anom[2 * i + 1];
+        for (int j = 0; j < 9; j++) r.lcs[j] = at.lcs[9 * i + j];
+        r.type = at.type[i];
+    }
+    d_atoms_in_.ensure(ain.size() * sizeof(AtomIn) + wave_cls.size() * sizeof(int) + 64);
+    HIP_CHECK(hipMemcpyAsync(d_atoms_in_.p, ain.data(), ain.size() * sizeof(AtomIn), hipMemcpyHostToDevice, stream_));
+    if (!wave_cls.empty())
+        HIP_CHECK(hipMemcpyAsync(static_cast<char*>(d_atoms_in_.p) + ain.size() * sizeof(AtomIn), wave_cls.data(),
+                                 wave_cls.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+    d_atoms_.ensure(ain.size() * sizeof(AtomRec));
+    // reflections
+    d_hkl_.ensure(std::max(n_hkl, 1) * 3 * sizeof(int));
+    d_dtdf_.ensure(std::max(n_hkl, 1) * 3 * sizeof(double));   // dtdf (2) + mult (1)
+    if (n_hkl > 0) {
+        HIP_CHECK(hipMemcpyAsync(d_hkl_.p, hkl, (size_t)n_hkl * 3 * sizeof(int), hipMemcpyHostToDevice, stream_));
+        HIP_CHECK(hipMemcpyAsync(d_dtdf_.p, dtdf, (size_t)n_hkl * 3 * sizeof(double), hipMemcpyHostToDevice, stream_));
+    }
+    // keep host staging alive until the copies complete
+    HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void Engine::run(Precision prec, bool derivs)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    const int n_ops = (int)ops_.size();
+    const int n_types = std::max(model_.n_types, 1);
+    const int n_hkl = n_hkl_;
+    const int n_slots = n_sorted_;
+    const int n_waves = n_slots / kWave;
+    const bool mixed = prec == kMixed;
+    d_f_.ensure(std::max(n_hkl, 1) * 2 * sizeof(double));
+    d_d_.ensure(std::max(n_atoms_, 1) * 10 * sizeof(double));
+    HIP_CHECK(hipMemsetAsync(d_d_.p, 0, std::max(n_atoms_, 1) * 10 * sizeof(double), stream_));
+    if (n_hkl == 0) return;
+    if (n_waves == 0) {
+        HIP_CHECK(hipMemsetAsync(d_f_.p, 0, (size_t)n_hkl * 2 * sizeof(double), stream_));
+        return;
+    }
+    const size_t n_rec = (size_t)n_hkl * n_ops;
+    d_radial_.ensure((size_t)n_hkl * n_types * kRadRec * (mixed ? sizeof(float) : sizeof(double)));
+    d_hklop_.ensure(n_rec * kHklopRec * (mixed ? sizeof(float) : sizeof(double)));
+    if (mixed) d_phase_.ensure(n_rec * 4 * sizeof(double));
+    DeviceBuffer& hlen = d_radial32_;   // scratch for |h|
+    hlen.ensure((size_t)n_hkl * sizeof(double));
+    const OpDev* ops = d_ops_.as<OpDev>();
+    const double* c2f = reinterpret_cast<const double*>(static_cast<const char*>(d_ops_.p) + n_ops * sizeof(OpDev));
+    const TypeRec* types = d_type_.as<TypeRec>();
+    const WfnRec* wfns = reinterpret_cast<const WfnRec*>(static_cast<const char*>(d_type_.p) +
+                                                         model_.n_types * sizeof(TypeRec));
+    const AtomIn* ain = d_atoms_in_.as<AtomIn>();
+    const int* wave_cls = reinterpret_cast<const int*>(static_cast<const char*>(d_atoms_in_.p) +
+                                                       std::max(n_sorted_, 1) * sizeof(AtomIn));
+    const int T = 256;
+    hipLaunchKernelGGL(k_hkl_prep, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, d_hkl_.as<int>(), ops, n_ops,
+                       n_hkl, c2f, mixed ? nullptr : d_hklop_.as<double>(), mixed ? d_hklop_.as<float>() : nullptr,
+                       mixed ? d_phase_.as<double>() : nullptr, hlen.as<double>());
+    HIP_CHECK(hipGetLastError());
+    if (model_.n_types > 0) {
+        int nt = n_hkl * model_.n_types;
+        hipLaunchKernelGGL(k_radial_prep, dim3((nt + T - 1) / T), dim3(T), 0, stream_, hlen.as<double>(), types,
+                           wfns, d_core_.as<TermRec>(), d_val_.as<TermRec>(), model_.n_types, n_hkl,
+                           mixed ? nullptr : d_radial_.as<double>(), mixed ? d_radial_.as<float>() : nullptr);
+        HIP_CHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_atom_prep, dim3((n_slots + T - 1) / T), dim3(T), 0, stream_, ain, types,
+                       d_atoms_.as<AtomRec>(), n_slots);
+    HIP_CHECK(hipGetLastError());
+    // launch geometry: enough waves to fill 256 CUs several times over
+    int chunk = tune.hkl_chunk;
+    if (chunk <= 0) {
+        const int target_waves = 8192;
+        int n_chunks = std::max(1, (target_waves + n_waves - 1) / n_waves);
+        chunk = std::max(16, (n_hkl + n_chunks - 1) / n_chunks);
+    }
+    const int n_chunks = (n_hkl + chunk - 1) / chunk;
+    d_fpart_.ensure((size_t)n_waves * n_hkl * 2 * sizeof(double));
+    d_dpart_.ensure((size_t)n_chunks * n_slots * 10 * sizeof(double));
+    MainArgs a;
+    a.hklop = d_hklop_.p;
+    a.phase = mixed ? d_phase_.as<double>() : nullptr;
+    a.radial = d_radial_.p;
+    a.dtdf = nullptr;
+    a.atoms = d_atoms_.as<AtomRec>();
+    a.wave_l = wave_cls;
+    a.fpart = d_fpart_.as<double>();
+    a.dpart = d_dpart_.as<double>();
+    a.n_hkl = n_hkl;
+    a.n_ops = n_ops;
+    a.n_types = n_types;
+    a.n_slots = n_slots;
+    a.n_waves = n_waves;
+    a.chunk = chunk;
+    // dtdf is stored as [re, im] pairs followed by the multipliers
+    a.dtdf = d_dtdf_.as<double>();
+    dim3 grid((n_waves + kWavesPerBlock - 1) / kWavesPerBlock, n_chunks);
+    last_grid_x = grid.x;
+    last_grid_y = grid.y;
+    last_chunk = chunk;
+    if (timing) HIP_CHECK(hipEventRecord(ev0_, stream_));
+    if (mixed) {
+        if (derivs) hipLaunchKernelGGL((k_main<true, true>), grid, dim3(256), 0, stream_, a);
+        else hipLaunchKernelGGL((k_main<true, false>), grid, dim3(256), 0, stream_, a);
+    } else {
+        if (derivs) hipLaunchKernelGGL((k_main<false, true>), grid, dim3(256), 0, stream_, a);
+        else hipLaunchKernelGGL((k_main<false, false>), grid, dim3(256), 0, stream_, a);
+    }
+    HIP_CHECK(hipGetLastError());
+    if (timing) HIP_CHECK(hipEventRecord(ev1_, stream_));
+    hipLaunchKernelGGL(k_reduce_f, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, a.fpart,
+                       d_dtdf_.as<double>() + 2 * (size_t)n_hkl, n_waves, n_hkl, centric_origin_ ? 1 : 0,
+                       d_f_.as<double>());
+    HIP_CHECK(hipGetLastError());
+    if (derivs) {
+        hipLaunchKernelGGL(k_reduce_d, dim3((n_slots + T - 1) / T), dim3(T), 0, stream_, a.dpart,
+                           d_atoms_.as<AtomRec>(), n_chunks, n_slots, d_d_.as<double>());
+        HIP_CHECK(hipGetLastError());
+    }
+    if (timing) {
+        HIP_CHECK(hipEventSynchronize(ev1_));
+        HIP_CHECK(hipEventElapsedTime(&last_main_ms, ev0_, ev1_));
+    }
+}
+
+void Engine::download(double* f, double* d10)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    if (f && n_hkl_ > 0)
+        HIP_CHECK(hipMemcpyAsync(f, d_f_.p, (size_t)n_hkl_ * 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    if (d10 && n_atoms_ > 0)
+        HIP_CHECK(hipMemcpyAsync(d10, d_d_.p, (size_t)n_atoms_ * 10 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+}  // namespace discamb_amd

@@ -1,0 +1,86 @@
+// Device engine of the Hansen-Coppens structure-factor path (one GPU, one HIP stream).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "hc_internal.hpp"
+
+namespace discamb_amd {
+
+enum Precision { kFp64 = 0, kMixed = 1 };
+
+// One explicit operation as the engine uses it.
+struct EngineOp {
+    int rf[9];          // fractional rotation
+    double tf[3];       // fractional translation
+    double rc[9];       // Cartesian rotation
+    double anom_sign;   // +1, or -1 for inversion partners (conjugated anomalous term)
+};
+
+struct DeviceBuffer {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t n);
+    void release();
+    ~DeviceBuffer() { release(); }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct TuneParams {
+    int hkl_chunk = 0;       // reflections per workgroup column; 0 = automatic
+};
+
+// Atom data for one call, in the order the caller supplies them (already converted to the
+// engine conventions: fractional xyz, U_cart * 2 pi^2 as a 6-vector, iso atoms as U I).
+struct EngineAtoms {
+    int n = 0;
+    std::vector<double> xyz_frac;   // 3n
+    std::vector<double> u6;         // 6n
+    std::vector<int> iso;           // n
+    std::vector<double> weight;     // occ * mult / |G|
+    std::vector<double> anom;       // 2n (per-atom f', f'')
+    std::vector<double> lcs;        // 9n, row-major, columns = local axes
+    std::vector<int> type;          // n
+    std::vector<unsigned char> include;
+};
+
+class Engine {
+public:
+    Engine(int device, const EngineModel& model, const double* cell6, const std::vector<EngineOp>& ops,
+           bool centric_origin);
+    ~Engine();
+    // Upload one call's inputs; hkl as int triples, dtdf already conj * centering multiplier.
+    void upload(const EngineAtoms& atoms, int n_hkl, const int* hkl, const double* dtdf);
+    // Run all kernels on the uploaded inputs (asynchronous on the engine stream).
+    void run(Precision prec, bool derivs);
+    // Copy F (2 n_hkl) and per-atom raw derivatives (10 per atom: dxyz_frac[3] (x 2 pi w),
+    // dU_cart[6] (x 2 pi^2 w), docc (x mult/|G|)) back, in caller atom order; synchronises.
+    void download(double* f, double* d10);
+    void synchronize();
+    hipStream_t stream() const { return stream_; }
+    TuneParams tune;
+    // last run: launch geometry of the main kernel
+    int last_grid_x = 0, last_grid_y = 0, last_chunk = 0;
+    float last_main_ms = 0.f;     // main-kernel time measured with events when timing_ is on
+    bool timing = false;
+    int n_ops() const { return (int)ops_.size(); }
+
+private:
+    int device_;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    EngineModel model_;
+    std::vector<EngineOp> ops_;
+    bool centric_origin_;
+    double f2c_[9], c2f_[9];
+    int n_atoms_ = 0, n_sorted_ = 0, n_hkl_ = 0;
+    std::vector<int> order_;        // sorted slot -> caller atom index (-1 padding)
+    // model tables
+    DeviceBuffer d_core_, d_val_, d_type_, d_ops_;
+    // per call
+    DeviceBuffer d_hkl_, d_dtdf_, d_atoms_in_, d_atoms_, d_hklop_, d_hklop32_, d_phase_, d_radial_,
+        d_radial32_, d_fpart_, d_dpart_, d_f_, d_d_, d_order_;
+};
+
+}  // namespace discamb_amd

@@ -1,0 +1,104 @@
+// Internal host-side types of the discamb_amd library (not part of the C ABI).
+#pragma once
+
+#include <array>
+#include <complex>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace discamb_amd {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// ---------------------------------------------------------------- crystal conventions
+// Orthogonalisation with a || x and c* || z (International Tables vol. B 3.3), the convention of
+// DiSCaMB's UnitCell::calculateTransformationMatrices (src/CrystalStructure/UnitCell.cpp:183-212).
+struct Cell {
+    double par[6] = {1, 1, 1, 90, 90, 90};
+    double f2c[9] = {0}, c2f[9] = {0};  // row-major
+    double rec_len[3] = {0};            // |a*|, |b*|, |c*|
+    void set(const double* p6);
+    // U_cart = A U_cif A^T with A = f2c diag(|a*|,|b*|,|c*|)
+    void u_cif_to_cart(const double* ucif6, double* ucart6) const;
+    // chain rule dT/dU_cif from dT/dU_cart (6-component parameter convention: off-diagonal
+    // parameters enter the symmetric tensor twice)
+    void d_ucart_to_d_ucif(const double* d_cart6, double* d_cif6) const;
+    void u_cart_to_cif(const double* ucart6, double* ucif6) const;
+};
+
+struct SymOp {
+    int r[9];        // fractional rotation, row-major
+    double t[3];     // fractional translation
+};
+
+// Space group split into an explicit subset of coset representatives, lattice centering and an
+// optional inversion centre - the representation DiSCaMB's SpaceGroup exposes through
+// nSymmetryOperationsInSubset / centering / inversionCenterTranslation (SpaceGroup.cpp:388-420).
+struct SpaceGroupInfo {
+    int order = 1;                 // |G|
+    std::vector<SymOp> subset;     // explicit operations
+    bool centro = false;
+    double inv_t[3] = {0, 0, 0};   // translation part of the inversion operation (fractional)
+    char centering = 'P';
+    bool obverse = true;
+    std::vector<std::array<double, 3>> centering_vectors;
+};
+SpaceGroupInfo analyse_space_group(const std::vector<SymOp>& ops);
+
+// Per-reflection centering multipliers (scattering_utilities.cpp:103-170), including that
+// routine's treatment of reverse R centering.
+double centering_multiplier(char centering, bool obverse, const int* hkl);
+
+// ---------------------------------------------------------------- Hansen-Coppens model
+struct SlaterOrbital {
+    int n = 1, l = 0;
+    std::vector<double> coeff, expo;
+    std::vector<int> pow;
+};
+
+struct WfnType {
+    std::string label;
+    std::vector<SlaterOrbital> orbitals;
+    std::vector<int> occupancy;
+    std::vector<int> core, valence;       // orbital indices
+    double def_val_exponent = 1.0;
+    std::vector<int> def_val_power;       // n_l, size 0..5
+    std::complex<double> anomalous{0, 0};
+};
+
+struct AtomType {
+    int n_l = 0;                 // size of p_lm
+    double plm[25] = {0};        // [l*l + l + m], density normalised, local frame
+    double p_val = 0, kappa = 1, kappa_prime = 1;
+};
+
+// Spherical Slater density sum_k c_k r^p_k exp(-e_k r) of one wavefunction type.
+struct SlaterDensity {
+    std::vector<double> c, e;
+    std::vector<int> p;
+};
+
+// Flattened, engine-ready Hansen-Coppens model (HansenCoppensStructureFactorCalculator::setModel,
+// .cpp:157-290, and Engine3::pre_hkl_loop_sf_calc, Engine3.cpp:195-256).
+struct EngineModel {
+    int n_wfn = 0, n_types = 0;
+    std::vector<SlaterDensity> core, valence;     // per wfn type; valence / valence occupancy
+    std::vector<double> dv_zeta;                  // per wfn
+    std::vector<int> dv_n;                        // per wfn, 5
+    std::vector<int> dv_nl;                       // per wfn
+    std::vector<double> anom;                     // per wfn, 2
+    std::vector<AtomType> types;
+    std::vector<int> type_wfn;                    // type -> wfn (last atom wins, as Engine3)
+    std::vector<int> type_max_l;                  // min(4, n_dv - 1, highest non-zero l)
+};
+
+void build_sphere_density(const WfnType& w, bool valence, SlaterDensity& out);
+EngineModel build_engine_model(const std::vector<WfnType>& wfn, const std::vector<AtomType>& types,
+                               const std::vector<int>& atom_wfn, const std::vector<int>& atom_type);
+
+}  // namespace discamb_amd

@@ -1,0 +1,141 @@
+/*
+ * discamb_amd - C ABI of the MI355X (gfx950) Hansen-Coppens structure-factor engine.
+ *
+ * Drop-in boundary for DiSCaMB's
+ *   HansenCoppensStructureFactorCalculator(const Crystal&, const HC_ModelParameters&)
+ *     include/discamb/Scattering/HansenCoppensStructureFactorCalculator.h:42
+ *   ::setCalculationMode(const std::string&)                    src/.../HansenCoppensStructureFactorCalculator.cpp:350-372
+ *   ::calculateStructureFactorsAndDerivatives(atoms, lcs, hkl, f, dTarget_dparam, dTarget_df,
+ *       countAtomContribution[, DerivativesSelector])            .h:148-165, .cpp:380-535
+ * and of the engine FFI it dispatches to for engine "GPU",
+ *   hc_sf_gpu::calculate_multipolar_structure_factors_gpu        src/GPU/cuda/HC_SF_GPU.h:10-69
+ * All arrays are plain host pointers in row-major order; nothing C++ crosses the boundary.
+ * Every function returns 0 on success or a positive DH_E_* code; dh_last_error() then returns a
+ * thread-local message.  One calculator object must not be used by two threads at once (as the
+ * reference calculator, whose calculate methods mutate internal state).
+ */
+#ifndef DISCAMB_AMD_H
+#define DISCAMB_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DH_OK 0
+#define DH_E_ARGUMENT 1   /* inconsistent sizes / indices (reference: on_error::throwException) */
+#define DH_E_MODEL 2      /* invalid model or space group */
+#define DH_E_DEVICE 3     /* HIP runtime failure */
+#define DH_E_MEMORY 4     /* device allocation failure */
+#define DH_E_NO_DEVICE 5  /* no usable GPU */
+
+/* engine precision (setCalculationMode strings "HIP" and "HIP_MIXED") */
+#define DH_PRECISION_FP64 0
+#define DH_PRECISION_MIXED 1
+
+/* DerivativesSelector bits (include/discamb/Scattering/SF_CalcDataTypes.h:44-56) */
+#define DH_D_XYZ 1u
+#define DH_D_ADP 2u
+#define DH_D_OCC 4u
+#define DH_D_ALL 7u
+
+/* xyz / ADP conventions (structural_parameters_convention) */
+#define DH_XYZ_FRACTIONAL 0
+#define DH_XYZ_CARTESIAN 1
+#define DH_ADP_U_CIF 0
+#define DH_ADP_U_CART 1
+
+typedef struct dh_calculator dh_calculator;
+
+/* Crystal + HC_ModelParameters, flattened.  Mirrors Crystal (unit cell, space group, atom
+ * multiplicities, conventions) and HC_ModelParameters (wfn_parameters, type_parameters,
+ * atom_to_wfn_map, atom_to_type_map; include/discamb/HC_Model/HC_ModelParameters.h:22-28). */
+typedef struct {
+    double cell[6];                 /* a, b, c (Angstrom), alpha, beta, gamma (degrees) */
+    int n_symops;                   /* the full space group, as listed in a CIF */
+    const int* symop_rot;           /* 9 * n_symops, fractional rotation, row-major */
+    const double* symop_trans;      /* 3 * n_symops, fractional translation */
+    int xyz_convention;             /* input coordinates: DH_XYZ_* */
+    int adp_convention;             /* input ADPs: DH_ADP_* */
+    int d_xyz_convention;           /* output derivatives */
+    int d_adp_convention;
+    /* wavefunction types (HC_WfnType, HC_WfnData.h:126-163) */
+    int n_wfn;
+    const int* wfn_orbital_offset;  /* n_wfn + 1, into the orbital arrays */
+    const int* orbital_term_offset; /* n_orbitals + 1, into the Slater-term arrays */
+    const int* orbital_nl;          /* 2 * n_orbitals: principal, azimuthal number */
+    const double* term_coeff;       /* normalised Slater coefficients */
+    const double* term_exp;         /* exponents, 1/Angstrom */
+    const int* term_pow;            /* powers of r */
+    const int* orbital_occupancy;   /* per orbital */
+    const int* orbital_kind;        /* per orbital: 0 core, 1 valence, -1 unused */
+    const double* def_val_exponent; /* n_wfn, zeta (1/Angstrom) */
+    const int* def_val_pow_offset;  /* n_wfn + 1, into def_val_pow */
+    const int* def_val_pow;         /* n_l */
+    const double* anomalous;        /* 2 * n_wfn: f', f'' */
+    /* atom types (HC_AtomTypeParameters, HC_AtomTypeParameters.h:19-50) */
+    int n_types;
+    const int* plm_n_l;             /* n_types: size of p_lm (number of l values, <= 5) */
+    const double* plm;              /* 25 * n_types, index l*l + l + m, density normalised */
+    const double* p_val;
+    const double* kappa;            /* kappa_spherical_valence */
+    const double* kappa_prime;      /* kappa_deformation_valence */
+    /* atoms */
+    int n_atoms;
+    const int* atom_to_wfn;
+    const int* atom_to_type;
+    const double* atom_multiplicity;  /* AtomInCrystal::multiplicity (|G| for a general position) */
+} dh_model_desc;
+
+/* Per-call atom parameters (vector<AtomInCrystal> + vector<Matrix3d> local frames). */
+typedef struct {
+    int n_atoms;
+    const double* xyz;              /* 3n, in the model's xyz_convention */
+    const int* adp_size;            /* n: 0, 1 (U_iso) or 6 (U11 U22 U33 U12 U13 U23) */
+    const double* adp;              /* 6n, first adp_size[i] entries used */
+    const double* occupancy;        /* n */
+    const double* lcs;              /* 9n, row-major 3x3 with local x, y, z axes as columns */
+    const double* anomalous;        /* 2n per-atom f', f'' or NULL (use the wavefunction type's) */
+    const unsigned char* include;   /* n, countAtomContribution, or NULL (all) */
+} dh_atoms;
+
+/* Outputs: f is 2*n_hkl (re, im); derivatives follow TargetFunctionAtomicParamDerivatives
+ * (SF_CalcDataTypes.h:36-41): d_xyz 3n, d_adp 6n (first adp_size[i] used), d_occ n.  Any
+ * derivative pointer may be NULL. */
+
+const char* dh_last_error(void);
+const char* dh_version(void);
+int dh_device_count(int* n);
+
+/* HansenCoppensStructureFactorCalculator(crystal, parameters) on GPU `device`. */
+int dh_hc_create(const dh_model_desc* model, int device, dh_calculator** out);
+void dh_hc_destroy(dh_calculator* calc);
+/* setCalculationMode: "HIP" (FP64 drop-in; "GPU" and "CPU" are accepted as aliases so reference
+ * configuration strings keep working - the computation always runs on the GPU) or "HIP_MIXED". */
+int dh_hc_set_calculation_mode(dh_calculator* calc, const char* mode);
+/* Launch tuning: reflections per workgroup column (0 = automatic). */
+int dh_hc_set_tuning(dh_calculator* calc, int hkl_chunk);
+
+/* calculateStructureFactorsAndDerivatives: synchronous, host pointers. hkl is 3*n_hkl int,
+ * dtarget_df 2*n_hkl (dT/dA + i dT/dB, not conjugated). selector: DH_D_* bits; 0 = F only. */
+int dh_hc_sf_and_derivs(dh_calculator* calc, const dh_atoms* atoms, int n_hkl, const int* hkl,
+                        const double* dtarget_df, unsigned selector, double* f, double* d_xyz,
+                        double* d_adp, double* d_occ);
+
+/* The same call split in three for callers that keep inputs resident on the GPU between steps
+ * (refinement loops, benchmarks): upload inputs, run the kernels, fetch results. */
+int dh_hc_upload(dh_calculator* calc, const dh_atoms* atoms, int n_hkl, const int* hkl,
+                 const double* dtarget_df);
+int dh_hc_run(dh_calculator* calc, unsigned selector);
+int dh_hc_fetch(dh_calculator* calc, double* f, double* d_xyz, double* d_adp, double* d_occ);
+/* Main-kernel time of the last dh_hc_run in ms (HIP events on the engine stream) and launch
+ * geometry; enable with dh_hc_set_timing(calc, 1). */
+int dh_hc_set_timing(dh_calculator* calc, int on);
+int dh_hc_last_timing(dh_calculator* calc, double* main_kernel_ms, int* grid_x, int* grid_y, int* chunk);
+/* explicit operations used by the engine (after inversion / centering folding) */
+int dh_hc_engine_info(dh_calculator* calc, int* n_explicit_ops, int* group_order, int* centrosymmetric,
+                      char* centering);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

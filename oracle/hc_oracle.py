@@ -1,0 +1,276 @@
+"""TEST INFRASTRUCTURE ONLY - calculator-level FP64 oracle.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use this module; the product
+(discamb_amd) never imports it.  It restates, in numpy, the pre/post-processing of
+HansenCoppensStructureFactorCalculator (src/Scattering/HansenCoppensStructureFactorCalculator.cpp):
+
+  setModel                         .cpp:157-290   Slater density products, valence / occupancy,
+                                                  multiplicity weights m/|G|
+  setEngineSpecificSpaceGroupRep.  .cpp:107-155   Cartesian explicit-subset operations
+  calculateStructureFactors...     .cpp:380-508   centering multipliers, conj(dT/dF) * mult, F * mult
+  prepareDataForEngine_part        .cpp:897-1011  U_cif -> U_cart * 2 pi^2, frac -> Cartesian,
+                                                  h_cart = h a* + k b* + l c*
+  convertDerivatives               .cpp:634-664   Cartesian -> fractional, U_cart -> U_cif
+
+and calls the C engine restatement in oracle/liboracle.so (hc_oracle.c).  ``mode="full_group"``
+instead sums every operation of the space group explicitly (no inversion / centering folding,
+no multipliers) - an independent check of the folding logic.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class _Model(ctypes.Structure):
+    _fields_ = [("n_wfn", ctypes.c_int),
+                ("core_off", ctypes.c_void_p), ("val_off", ctypes.c_void_p),
+                ("core_c", ctypes.c_void_p), ("core_e", ctypes.c_void_p), ("core_p", ctypes.c_void_p),
+                ("val_c", ctypes.c_void_p), ("val_e", ctypes.c_void_p), ("val_p", ctypes.c_void_p),
+                ("dv_zeta", ctypes.c_void_p), ("dv_n", ctypes.c_void_p), ("dv_nl", ctypes.c_void_p),
+                ("anom", ctypes.c_void_p),
+                ("n_types", ctypes.c_int),
+                ("pval", ctypes.c_void_p), ("kappa", ctypes.c_void_p), ("kappa_p", ctypes.c_void_p),
+                ("plm", ctypes.c_void_p), ("plm_nl", ctypes.c_void_p)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        _LIB = ctypes.CDLL(path)
+        _LIB.hc_oracle_engine.restype = ctypes.c_int
+    return _LIB
+
+
+def parse_op(text):
+    from fractions import Fraction
+    rot = np.zeros((3, 3))
+    tr = np.zeros(3)
+    for i, comp in enumerate(text.replace(" ", "").lower().split(",")):
+        comp = comp.replace("-", "+-")
+        for term in [t for t in comp.split("+") if t]:
+            s = -1.0 if term.startswith("-") else 1.0
+            term = term.lstrip("-")
+            if term[-1:] in ("x", "y", "z"):
+                rot[i, "xyz".index(term[-1])] += s * (float(term[:-1]) if term[:-1] else 1.0)
+            else:
+                tr[i] += s * float(Fraction(term))
+    return rot, tr
+
+
+def cell_f2c(cell):
+    a, b, c, al, be, ga = cell
+    f = np.pi / 180.0
+    ca, cb, cg, sg = np.cos(al * f), np.cos(be * f), np.cos(ga * f), np.sin(ga * f)
+    v = np.sqrt(1 - ca * ca - cb * cb - cg * cg + 2 * ca * cb * cg)
+    return np.array([[a, b * cg, c * cb], [0.0, b * sg, c * (ca - cb * cg) / sg], [0.0, 0.0, c * v / sg]])
+
+
+def decompose_group(symops):
+    """(order, subset [(R, t)], centro, t_inv, centering letter, obverse).  Coset representatives
+    are chosen by rotation matrix: one operation per rotation R, and of each (R, -R) pair only the
+    first listed when the group is centrosymmetric."""
+    ops = [parse_op(s) for s in symops]
+    order = len(ops)
+    cen = [t % 1.0 for R, t in ops if np.allclose(R, np.eye(3))]
+    inv = [t % 1.0 for R, t in ops if np.allclose(R, -np.eye(3))]
+    centro = len(inv) > 0
+    t_inv = inv[0] if centro else np.zeros(3)
+    subset, seen = [], []
+    for R, t in ops:
+        if any(np.allclose(R, S) for S in seen) or (centro and any(np.allclose(R, -S) for S in seen)):
+            continue
+        seen.append(R)
+        subset.append((R, t % 1.0))
+
+    def has(v):
+        return any(np.allclose(((c - np.array(v)) + 0.5) % 1.0 - 0.5, 0, atol=1e-8) for c in cen)
+
+    letter, obverse = "P", True
+    if len(cen) == 4:
+        letter = "F"
+    elif len(cen) == 2:
+        for L, v in (("A", (0, .5, .5)), ("B", (.5, 0, .5)), ("C", (.5, .5, 0)), ("I", (.5, .5, .5))):
+            if has(v):
+                letter = L
+    elif len(cen) == 3:
+        letter = "R"
+        obverse = has((2 / 3, 1 / 3, 1 / 3))
+    return order, subset, centro, t_inv, letter, obverse
+
+
+def centering_mult(letter, obverse, hkl):
+    h, k, l = hkl[:, 0], hkl[:, 1], hkl[:, 2]
+    if letter == "P":
+        return np.ones(len(hkl))
+    if letter == "C":
+        return np.where((h + k) % 2 == 0, 2.0, 0.0)
+    if letter == "A":
+        return np.where((k + l) % 2 == 0, 2.0, 0.0)
+    if letter == "B":
+        return np.where((h + l) % 2 == 0, 2.0, 0.0)
+    if letter == "I":
+        return np.where((h + k + l) % 2 == 0, 2.0, 0.0)
+    if letter == "F":
+        return np.where(((h % 2) == (k % 2)) & ((k % 2) == (l % 2)), 4.0, 0.0)
+    if letter == "R":
+        if obverse:
+            return np.where(np.abs(-h + k + l) % 3 == 0, 3.0, 0.0)
+        return np.where(np.abs(h - k + l) % 2 == 0, 3.0, 0.0)   # scattering_utilities.cpp:130
+    raise ValueError(letter)
+
+
+def sphere_density(w, valence):
+    """atomicStoWfnToSphericalDensity (HC_WfnData.cpp:11-73)."""
+    c, e, p = [], [], []
+    idx = w.valence_orbitals_indices if valence else w.core_orbitals_indices
+    for oi in idx:
+        occ = w.orbital_occupancy[oi]
+        if occ == 0:
+            continue
+        o = w.orbitals[oi]
+        for i in range(len(o.coefficients)):
+            for j in range(i + 1):
+                v = o.coefficients[i] * o.coefficients[j] * occ / (4 * np.pi)
+                if i != j:
+                    v *= 2
+                c.append(v)
+                e.append(o.exponents[i] + o.exponents[j])
+                p.append(o.powers[i] + o.powers[j])
+    if valence:
+        tot = sum(w.orbital_occupancy[i] for i in w.valence_orbitals_indices)
+        c = [v / tot for v in c]
+    return c, e, p
+
+
+def u_cif_to_cart_matrix(cell):
+    """6x6 linear map U_cart(6) = J U_cif(6) in the (11 22 33 12 13 23) parameter order."""
+    f2c = cell_f2c(cell)
+    c2f = np.linalg.inv(f2c)
+    A = f2c * np.linalg.norm(c2f, axis=1)[None, :]
+    J = np.zeros((6, 6))
+    idx = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    for j, (a, b) in enumerate(idx):
+        E = np.zeros((3, 3))
+        E[a, b] = E[b, a] = 1.0
+        M = A @ E @ A.T
+        J[:, j] = [M[x, y] for x, y in idx]
+    return J
+
+
+def sf_and_derivs(crystal, params, atoms, lcs, hkl, dtdf, include=None, n_threads=1, mode="reference",
+                  anomalous=None):
+    """Returns (F complex (m,), d_xyz (n,3), d_adp (n,6) (or U_iso in column 0), d_occ (n,)); derivative
+    conventions follow the crystal's input conventions."""
+    hkl = np.ascontiguousarray(np.asarray(hkl, dtype=np.int64).reshape(-1, 3))
+    m = hkl.shape[0]
+    n = atoms.n
+    cell = crystal.cell
+    f2c = cell_f2c(cell)
+    c2f = np.linalg.inv(f2c)
+    order, subset, centro, t_inv, letter, obverse = decompose_group(crystal.symops)
+    if mode == "full_group":
+        ops = [(R, t) for R, t in (parse_op(s) for s in crystal.symops)]
+        centro_flag, tinv, mult = 0, np.zeros(3), np.ones(m)
+    else:
+        ops = subset
+        centro_flag, tinv = (1 if centro else 0), t_inv
+        mult = centering_mult(letter, obverse, hkl)
+    rot = np.ascontiguousarray(np.array([f2c @ R @ c2f for R, t in ops]).reshape(-1))
+    trans = np.ascontiguousarray(np.array([f2c @ t for R, t in ops]).reshape(-1))
+    inv_cart = np.ascontiguousarray(f2c @ tinv)
+    # model
+    wfn = params.wfn_parameters
+    core, val = [sphere_density(w, False) for w in wfn], [sphere_density(w, True) for w in wfn]
+
+    def flat(parts):
+        off = np.cumsum([0] + [len(x[0]) for x in parts]).astype(np.int32)
+        c = np.array(sum([x[0] for x in parts], []) + [0.0])
+        e = np.array(sum([x[1] for x in parts], []) + [1.0])
+        p = np.array(sum([x[2] for x in parts], []) + [0], dtype=np.int32)
+        return off, c, e, p
+
+    co, cc, ce, cp = flat(core)
+    vo, vc, ve, vp = flat(val)
+    dv_zeta = np.array([w.deformation_valence_exponent for w in wfn])
+    dv_n = np.zeros((len(wfn), 5), dtype=np.int32)
+    dv_nl = np.zeros(len(wfn), dtype=np.int32)
+    for i, w in enumerate(wfn):
+        k = min(5, len(w.deformation_valence_power))
+        dv_nl[i] = k
+        dv_n[i, :k] = w.deformation_valence_power[:k]
+    anom = np.array([[complex(w.anomalous_scattering).real, complex(w.anomalous_scattering).imag] for w in wfn])
+    tp = params.type_parameters
+    pval = np.array([t.p_val for t in tp])
+    kap = np.array([t.kappa_spherical_valence for t in tp])
+    kapp = np.array([t.kappa_deformation_valence for t in tp])
+    plm = np.ascontiguousarray(np.array([t.plm25() for t in tp]))
+    plm_nl = np.array([len(t.p_lm) for t in tp], dtype=np.int32)
+    keep = [co, cc, ce, cp, vo, vc, ve, vp, dv_zeta, dv_n, dv_nl, anom, pval, kap, kapp, plm, plm_nl]
+    md = _Model(len(wfn), co.ctypes.data, vo.ctypes.data, cc.ctypes.data, ce.ctypes.data, cp.ctypes.data,
+                vc.ctypes.data, ve.ctypes.data, vp.ctypes.data, dv_zeta.ctypes.data, dv_n.ctypes.data,
+                dv_nl.ctypes.data, anom.ctypes.data, len(tp), pval.ctypes.data, kap.ctypes.data,
+                kapp.ctypes.data, plm.ctypes.data, plm_nl.ctypes.data)
+    # atoms (prepareDataForEngine_part)
+    xyz = atoms.xyz if crystal.xyz_convention == "fractional" else atoms.xyz @ c2f.T
+    xyz_cart = np.ascontiguousarray(xyz @ f2c.T)
+    J = u_cif_to_cart_matrix(cell)
+    adp_n = np.ascontiguousarray(atoms.adp_size.astype(np.int32))
+    adp_flat = []
+    for i in range(n):
+        if adp_n[i] == 6:
+            u = atoms.adp[i] if crystal.adp_convention == "U_cart" else J @ atoms.adp[i]
+            adp_flat += list(2 * np.pi ** 2 * u)
+        elif adp_n[i] == 1:
+            adp_flat.append(2 * np.pi ** 2 * atoms.adp[i, 0])
+    adp_flat = np.array(adp_flat + [0.0])
+    occ = np.ascontiguousarray(atoms.occupancy, dtype=np.float64)
+    mult_w = np.ascontiguousarray(crystal.atoms.multiplicity / float(order))
+    lcs_f = np.ascontiguousarray(np.asarray(lcs, dtype=np.float64).reshape(n, 9))
+    an_atom = None
+    if anomalous is not None:
+        a = np.asarray(anomalous, dtype=np.complex128)
+        an_atom = np.ascontiguousarray(np.stack([a.real, a.imag], 1))
+    inc = np.ones(n, dtype=np.uint8) if include is None else np.asarray(include, dtype=np.uint8)
+    a_wfn = np.ascontiguousarray(np.array(params.atom_to_wfn_map, dtype=np.int32))
+    a_type = np.ascontiguousarray(np.array(params.atom_to_type_map, dtype=np.int32))
+    # reflections
+    h_cart = np.ascontiguousarray(hkl.astype(np.float64) @ c2f)
+    d = np.asarray(dtdf, dtype=np.complex128) * 1.0
+    d = np.conj(d) * mult
+    dd = np.ascontiguousarray(np.stack([d.real, d.imag], 1))
+    fo = np.zeros((m, 2))
+    dx = np.zeros((n, 3))
+    du = np.zeros(max(1, int(adp_n.sum())))
+    do = np.zeros(n)
+    P = ctypes.c_void_p
+    rc = lib().hc_oracle_engine(
+        ctypes.byref(md), ctypes.c_int(n), P(a_wfn.ctypes.data), P(a_type.ctypes.data), P(xyz_cart.ctypes.data),
+        P(adp_n.ctypes.data), P(adp_flat.ctypes.data), P(occ.ctypes.data), P(mult_w.ctypes.data),
+        P(lcs_f.ctypes.data), P(an_atom.ctypes.data) if an_atom is not None else P(0), P(inc.ctypes.data),
+        ctypes.c_int(len(ops)), P(rot.ctypes.data), P(trans.ctypes.data), ctypes.c_int(centro_flag),
+        P(inv_cart.ctypes.data), ctypes.c_int(m), P(h_cart.ctypes.data), P(dd.ctypes.data), P(fo.ctypes.data),
+        P(dx.ctypes.data), P(du.ctypes.data), P(do.ctypes.data), ctypes.c_int(n_threads))
+    del keep
+    if rc != 0:
+        raise RuntimeError("oracle engine failed")
+    F = (fo[:, 0] + 1j * fo[:, 1]) * mult
+    # convertDerivatives: Cartesian -> fractional, U_cart -> U_cif
+    d_xyz = dx @ f2c if crystal.xyz_convention == "fractional" else dx
+    d_adp = np.zeros((n, 6))
+    off = 0
+    for i in range(n):
+        if adp_n[i] == 6:
+            g = du[off:off + 6]
+            d_adp[i] = g @ J if crystal.adp_convention == "U_cif" else g
+            off += 6
+        elif adp_n[i] == 1:
+            d_adp[i, 0] = du[off]
+            off += 1
+    return F, d_xyz, d_adp, do

@@ -1,0 +1,154 @@
+"""GPU parity: the HIP engine through the C ABI against the FP64 oracle on the same seeded inputs.
+
+Tolerances (normwise max|d - d_ref| / max|d_ref| per output class):
+  FP64 drop-in ("HIP")        1e-10 for F and every derivative class
+  mixed precision ("HIP_MIXED") 1e-6 for F, 1e-5 for derivatives
+"""
+import numpy as np
+import pytest
+
+import hc_oracle
+from conftest import rel_err
+from discamb_amd import (AtomSet, DerivativesSelector, HansenCoppensStructureFactorCalculator, synth)
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"HIP": (1e-10, 1e-10), "HIP_MIXED": (1e-6, 1e-5)}
+
+
+def _compare(w, mode, atoms=None, include=None, anomalous=None, hkl=None, dtdf=None, tol=None):
+    atoms = atoms or w.crystal.atoms
+    hkl = w.hkl if hkl is None else hkl
+    dtdf = w.dtdf if dtdf is None else dtdf
+    print("case %s mode=%s atoms=%d hkl=%d" % (w.name, mode, atoms.n, len(hkl)), flush=True)
+    calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+    calc.setCalculationMode(mode)
+    f, d = calc.calculateStructureFactorsAndDerivatives(atoms, w.lcs, hkl, dtdf, include, anomalous=anomalous)
+    rf, rx, ru, ro = hc_oracle.sf_and_derivs(w.crystal, w.params, atoms, w.lcs, hkl, dtdf, include=include,
+                                             n_threads=16, anomalous=anomalous)
+    tf, td = tol or TOL[mode]
+    errs = {"F": rel_err(f, rf), "dxyz": rel_err(d.atomic_position_derivatives, rx),
+            "dadp": rel_err(d.adp_derivatives, ru), "docc": rel_err(d.occupancy_derivatives, ro)}
+    print("  errors", errs, flush=True)
+    assert errs["F"] <= tf
+    for k in ("dxyz", "dadp", "docc"):
+        assert errs[k] <= td, k
+    return f, d
+
+
+@pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
+@pytest.mark.parametrize("sg", list(synth.SPACE_GROUPS))
+def test_space_groups(sg, mode):
+    _compare(synth.small_structure(sg, n_atoms=14, d_min=0.9), mode)
+
+
+def test_shifted_inversion_centre():
+    w = synth.small_structure("P-1", n_atoms=10, d_min=1.0)
+    w.crystal.symops = ["x,y,z", "-x+1/2,-y,-z+1/2"]
+    _compare(w, "HIP")
+
+
+def test_body_centred():
+    w = synth.small_structure("P212121", n_atoms=8, d_min=1.1)
+    w.crystal.symops = ["x,y,z", "-x,-y,z", "-x,y,-z", "x,-y,-z", "x+1/2,y+1/2,z+1/2", "-x+1/2,-y+1/2,z+1/2",
+                        "-x+1/2,y+1/2,-z+1/2", "x+1/2,-y+1/2,-z+1/2"]
+    w.crystal.atoms.multiplicity[:] = 8
+    _compare(w, "HIP")
+
+
+@pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
+def test_config0_full(mode):
+    """60-atom P2_1/c structure, all 15 300 unique reflections to 0.5 A."""
+    _compare(synth.config0(), mode)
+
+
+@pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
+def test_config1_reflection_subset(mode):
+    """5000-atom P2_1 2_1 2_1 structure, all atoms, a 3000-reflection subset spanning resolution."""
+    w = synth.config1()
+    rng = np.random.default_rng(5)
+    sel = np.sort(rng.choice(len(w.hkl), 3000, replace=False))
+    _compare(w, mode, hkl=w.hkl[sel], dtdf=w.dtdf[sel])
+
+
+def test_include_mask_anomalous_and_h000():
+    w = synth.small_structure("P21/c", n_atoms=12, d_min=1.2)
+    inc = np.ones(12, dtype=bool)
+    inc[[1, 5]] = False
+    rng = np.random.default_rng(8)
+    anom = rng.normal(size=12) * 0.3 + 1j * np.abs(rng.normal(size=12)) * 0.2
+    hkl = np.vstack([[[0, 0, 0]], w.hkl])
+    dtdf = np.concatenate([[0.7 - 0.2j], w.dtdf])
+    f, d = _compare(w, "HIP", include=inc, anomalous=anom, hkl=hkl, dtdf=dtdf)
+    assert np.all(d.atomic_position_derivatives[~inc] == 0)
+    assert np.all(d.occupancy_derivatives[~inc] == 0)
+
+
+def test_acentric_anomalous():
+    w = synth.small_structure("P212121", n_atoms=9, d_min=1.2)
+    anom = np.full(9, 0.25 + 0.4j)
+    _compare(w, "HIP", anomalous=anom)
+
+
+def test_cartesian_and_u_cart_conventions():
+    w = synth.small_structure("P21/c", n_atoms=10, d_min=1.2)
+    f2c, c2f = synth.cell_matrices(w.crystal.cell)
+    a = w.crystal.atoms.copy()
+    a.xyz = a.xyz @ f2c.T
+    J = hc_oracle.u_cif_to_cart_matrix(w.crystal.cell)
+    for i in range(a.n):
+        if a.adp_size[i] == 6:
+            a.adp[i] = J @ a.adp[i]
+    w.crystal.atoms = a
+    w.crystal.xyz_convention = "cartesian"
+    w.crystal.adp_convention = "U_cart"
+    calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+    f, d = calc.calculateStructureFactorsAndDerivatives(a, w.lcs, w.hkl, w.dtdf)
+    rf, rx, ru, ro = hc_oracle.sf_and_derivs(w.crystal, w.params, a, w.lcs, w.hkl, w.dtdf)
+    assert rel_err(f, rf) < 1e-10
+    assert rel_err(d.atomic_position_derivatives, rx) < 1e-10
+    assert rel_err(d.adp_derivatives, ru) < 1e-10
+    assert rel_err(d.occupancy_derivatives, ro) < 1e-10
+
+
+def test_f_only_and_selector():
+    w = synth.small_structure("P1", n_atoms=6, d_min=1.2)
+    calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+    f = calc.calculateStructureFactors(w.crystal.atoms, w.lcs, w.hkl)
+    rf = hc_oracle.sf_and_derivs(w.crystal, w.params, w.crystal.atoms, w.lcs, w.hkl, np.ones(len(w.hkl)))[0]
+    assert rel_err(f, rf) < 1e-10
+    f2, d = calc.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, w.hkl, w.dtdf, None,
+                                                         DerivativesSelector(True, False, False))
+    assert np.all(d.adp_derivatives == 0) and np.all(d.occupancy_derivatives == 0)
+    assert np.any(d.atomic_position_derivatives != 0)
+
+
+def test_empty_reflections_and_errors():
+    w = synth.small_structure("P1", n_atoms=4, d_min=1.5)
+    calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+    f, d = calc.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, np.zeros((0, 3), int),
+                                                        np.zeros(0, complex))
+    assert f.shape == (0,)
+    assert np.all(d.atomic_position_derivatives == 0)
+    with pytest.raises(RuntimeError):
+        calc.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, w.hkl, w.dtdf[:-1])
+    with pytest.raises(RuntimeError):
+        calc.setCalculationMode("NOPE")
+
+
+def test_determinism_full_config1():
+    """Full 5000-atom / 66 592-reflection workload: two runs are bitwise identical (fixed-order,
+    atomic-free reductions) and FP64 / mixed agree to the mixed tolerance."""
+    w = synth.config1()
+    calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+    f1, d1 = calc.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+    f2, d2 = calc.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+    assert np.array_equal(f1, f2)
+    assert np.array_equal(d1.adp_derivatives, d2.adp_derivatives)
+    calc.setCalculationMode("HIP_MIXED")
+    fm, dm = calc.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+    e = rel_err(fm, f1)
+    print("full config1 mixed vs fp64: F %.3e dxyz %.3e dadp %.3e" % (
+        e, rel_err(dm.atomic_position_derivatives, d1.atomic_position_derivatives),
+        rel_err(dm.adp_derivatives, d1.adp_derivatives)), flush=True)
+    assert e < 1e-6

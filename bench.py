@@ -45,7 +45,8 @@ def time_mode(calc, mode, steps, warmup):
     calc.setTiming(False)
     for _ in range(warmup):
         calc.run()
-    calc.fetch()
+    if warmup:
+        calc.fetch()
     calc.setTiming(True)
     main_ms = []
     t0 = time.perf_counter()
@@ -58,7 +59,7 @@ def time_mode(calc, mode, steps, warmup):
     return (t1 - t0) / steps, float(np.mean(main_ms)), info, f, d
 
 
-def cpu_baseline(w, budget_refl=8000):
+def cpu_baseline(w, budget_refl=16000):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import hc_oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -81,7 +82,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain read+write elementwise kernel over MIB MiB (PMC byte calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 2.0, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4, "launches": 3}))
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path shards by reflection; see DESIGN.md); "
               "run with --gpus 1", file=sys.stderr)

@@ -153,33 +153,46 @@ __global__ void k_hkl_prep(const int* __restrict__ hkl, const OpDev* __restrict_
     }
 }
 
-// S = f_core + Pval f_val(h/kappa);  G_l = 4 pi i^l-sign N_l g_l(n_l + 2, h/kappa', zeta)
-__global__ void k_radial_prep(const double* __restrict__ hlen, const TypeRec* __restrict__ types,
-                              const WfnRec* __restrict__ wfns, const TermRec* __restrict__ core,
+// f_core(|h|) per (wavefunction type, reflection): 4 pi sum_k c_k g_0(p_k + 2, |h|, e_k)
+__global__ void k_core_prep(const double* __restrict__ hlen, const WfnRec* __restrict__ wfns,
+                            const TermRec* __restrict__ core, int n_wfn, int n_hkl, double* __restrict__ fcore)
+{
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_hkl * n_wfn) return;
+    int w = idx / n_hkl, k = idx - w * n_hkl;       // wfn-major: a wave shares one term list
+    const WfnRec wf = wfns[w];
+    const double h = hlen[k];
+    double fc = 0.0;
+    for (int i = 0; i < wf.core_n; i++) {
+        const TermRec tr = core[wf.core_off + i];
+        fc += tr.c * g_function(0, tr.p + 2, h, tr.e);
+    }
+    fcore[idx] = 12.566370614359172953850 * fc;
+}
+
+// Per (reflection, type): S = f_core + Pval f_val(h/kappa) and
+// G_l = 4 pi (i^l sign) N_l g_l(n_l + 2, h/kappa', zeta), l <= max_l.
+__global__ void k_radial_prep(const double* __restrict__ hlen, const double* __restrict__ fcore,
+                              const TypeRec* __restrict__ types, const WfnRec* __restrict__ wfns,
                               const TermRec* __restrict__ val, int n_types, int n_hkl,
                               double* __restrict__ rad64, float* __restrict__ rad32)
 {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n_hkl * n_types) return;
-    int k = idx / n_types, t = idx - k * n_types;
+    int t = idx / n_hkl, k = idx - t * n_hkl;       // type-major: a wave shares one type
     const double four_pi = 12.566370614359172953850;
-    double h = hlen[k];
+    const double h = hlen[k];
     const TypeRec& ty = types[t];
-    const WfnRec& wf = wfns[ty.wfn];
-    double fc = 0.0;
-    for (int i = 0; i < wf.core_n; i++) {
-        const TermRec& tr = core[wf.core_off + i];
-        fc += tr.c * g_function(0, tr.p + 2, h, tr.e);
-    }
+    const WfnRec wf = wfns[ty.wfn];
     double fv = 0.0;
-    double hk = h / ty.kappa;
+    const double hk = h / ty.kappa;
     for (int i = 0; i < wf.val_n; i++) {
-        const TermRec& tr = val[wf.val_off + i];
+        const TermRec tr = val[wf.val_off + i];
         fv += tr.c * g_function(0, tr.p + 2, hk, tr.e);
     }
     double r[kRadRec] = {0, 0, 0, 0, 0, 0, 0, 0};
-    r[0] = four_pi * fc + ty.pval * four_pi * fv;
-    double hp = h / ty.kappa_p;
+    r[0] = fcore[(size_t)ty.wfn * n_hkl + k] + ty.pval * four_pi * fv;
+    const double hp = h / ty.kappa_p;
     for (int l = 0; l <= ty.max_l && l < ty.nl; l++) {
         int n = ty.n[l];
         double norm = 1.0;
@@ -189,45 +202,60 @@ __global__ void k_radial_prep(const double* __restrict__ hlen, const TypeRec* __
         // i^l: l = 0 (+, real), 1 (+, imag), 2 (-, real), 3 (-, imag), 4 (+, real)
         r[1 + l] = (l == 2 || l == 3) ? -g : g;
     }
-    size_t base = (size_t)idx * kRadRec;
+    size_t base = ((size_t)k * n_types + t) * kRadRec;
     if (rad64)
         for (int i = 0; i < kRadRec; i++) rad64[base + i] = r[i];
     if (rad32)
         for (int i = 0; i < kRadRec; i++) rad32[base + i] = (float)r[i];
 }
 
-__global__ void k_atom_prep(const AtomIn* __restrict__ in, const TypeRec* __restrict__ types,
-                            AtomRec* __restrict__ out, int n_slots)
+// One 64-lane workgroup per atom slot: lanes 0..49 evaluate the local-frame multipole expansion
+// f(u_p) = sum_j P_j d_j(L^T u_p) at the projection nodes, lanes 0..24 project it back onto the
+// global-frame harmonics.
+__global__ __launch_bounds__(64) void k_atom_prep(const AtomIn* __restrict__ in, const TypeRec* __restrict__ types,
+                                                  AtomRec* __restrict__ out, int n_slots)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_slots) return;
+    const int i = blockIdx.x;
+    const int p = threadIdx.x;
+    __shared__ double fsh[64];
     const AtomIn& a = in[i];
-    AtomRec r;
-    for (int j = 0; j < 3; j++) r.xf[j] = a.xf[j];
-    for (int j = 0; j < 6; j++) r.u[j] = a.u[j];
-    r.w = a.w;
-    r.anom[0] = a.anom[0];
-    r.anom[1] = a.anom[1];
-    r.type = a.type;
-    r.orig = a.orig;
-    for (int j = 0; j < 25; j++) r.plm[j] = 0.0;
-    const TypeRec& ty = types[a.type];
-    if (a.orig >= 0 && ty.max_l >= 0) {
-        const int nlm = (ty.max_l + 1) * (ty.max_l + 1);
-        for (int p = 0; p < 50; p++) {
-            double u0 = c_proj_dir[p][0], u1 = c_proj_dir[p][1], u2 = c_proj_dir[p][2];
-            // local coordinates: columns of the LCS are the local axes
-            double x = a.lcs[0] * u0 + a.lcs[3] * u1 + a.lcs[6] * u2;
-            double y = a.lcs[1] * u0 + a.lcs[4] * u1 + a.lcs[7] * u2;
-            double z = a.lcs[2] * u0 + a.lcs[5] * u1 + a.lcs[8] * u2;
-            double d[25];
-            ylm_density<double>(x, y, z, d);
-            double f = 0.0;
-            for (int j = 0; j < nlm; j++) f += ty.plm[j] * d[j];
-            for (int j = 0; j < nlm; j++) r.plm[j] += c_proj_w[p][j] * f;
-        }
+    AtomRec& r = out[i];
+    const bool real_atom = a.orig >= 0;
+    int max_l = -1;
+    if (real_atom) max_l = types[a.type].max_l;
+    double f = 0.0;
+    if (max_l >= 0 && p < 50) {
+        const double* P = types[a.type].plm;
+        const double u0 = c_proj_dir[p][0], u1 = c_proj_dir[p][1], u2 = c_proj_dir[p][2];
+        // local coordinates: columns of the LCS are the local axes
+        const double x = a.lcs[0] * u0 + a.lcs[3] * u1 + a.lcs[6] * u2;
+        const double y = a.lcs[1] * u0 + a.lcs[4] * u1 + a.lcs[7] * u2;
+        const double z = a.lcs[2] * u0 + a.lcs[5] * u1 + a.lcs[8] * u2;
+        double d[25];
+        ylm_density<double>(x, y, z, d);
+#pragma unroll
+        for (int j = 0; j < 25; j++) f += P[j] * d[j];
     }
-    out[i] = r;
+    fsh[p] = f;
+    __syncthreads();
+    if (p < 25) {
+        double s = 0.0;
+        const int l = p < 1 ? 0 : p < 4 ? 1 : p < 9 ? 2 : p < 16 ? 3 : 4;
+        if (l <= max_l)
+            for (int q = 0; q < 50; q++) s += c_proj_w[q][p] * fsh[q];
+        r.plm[p] = s;
+    } else if (p < 28) {
+        r.xf[p - 25] = a.xf[p - 25];
+    } else if (p < 34) {
+        r.u[p - 28] = a.u[p - 28];
+    } else if (p == 34) {
+        r.w = a.w;
+    } else if (p < 37) {
+        r.anom[p - 35] = a.anom[p - 35];
+    } else if (p == 37) {
+        r.type = a.type;
+        r.orig = a.orig;
+    }
 }
 
 // ------------------------------------------------------------------ main kernels
@@ -244,22 +272,55 @@ struct MainArgs {
     int n_hkl, n_ops, n_types, n_slots, n_waves, chunk;
 };
 
-__device__ __forceinline__ double wave_sum(double v)
+// Full-wave sums with DPP lane permutations (quad_perm xor 1, xor 2, row_half_mirror, row_mirror,
+// row_bcast:15, row_bcast:31); the total ends in lane 63 and is broadcast with readlane.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ int dpp_mov(int v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xF, false);
 }
 
-__device__ __forceinline__ float wave_sum(float v)
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_add(float v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+    return v + __builtin_bit_cast(float, dpp_mov<CTRL, ROW_MASK>(__builtin_bit_cast(int, v)));
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_add(double v)
+{
+    long long b = __builtin_bit_cast(long long, v);
+    int lo = dpp_mov<CTRL, ROW_MASK>((int)(b & 0xffffffff));
+    int hi = dpp_mov<CTRL, ROW_MASK>((int)(b >> 32));
+    long long r = ((long long)(unsigned)lo) | ((long long)hi << 32);
+    return v + __builtin_bit_cast(double, r);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+    v = dpp_add<0xB1, 0xF>(v);    // quad_perm [1,0,3,2]
+    v = dpp_add<0x4E, 0xF>(v);    // quad_perm [2,3,0,1]
+    v = dpp_add<0x141, 0xF>(v);   // row_half_mirror
+    v = dpp_add<0x140, 0xF>(v);   // row_mirror
+    v = dpp_add<0x142, 0xA>(v);   // row_bcast:15 into rows 1, 3
+    v = dpp_add<0x143, 0xC>(v);   // row_bcast:31 into rows 2, 3
+    return v;                     // complete in lane 63
+}
+
+// Wave-uniform FP64 record entries are fetched as 32-bit words so that type-based alias analysis
+// can separate them from the FP64 partial-sum stores and the loads stay on the scalar unit.
+__device__ __forceinline__ double ld_f64(const unsigned* __restrict__ p, int i)
+{
+    unsigned lo = p[2 * i], hi = p[2 * i + 1];
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
 template <int L, bool DERIV>
-__device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane, int k0, int k1)
+__device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane, int k0, int k1,
+                                          const unsigned* __restrict__ hop, const double* __restrict__ rad,
+                                          const unsigned* __restrict__ dtdf32, double* __restrict__ fpart,
+                                          double* __restrict__ dpart)
 {
     constexpr int NLM = (L + 1) * (L + 1);
     const int slot = wave * kWave + lane;
@@ -274,15 +335,16 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
     const double w = at.w, anr = at.anom[0], ani = at.anom[1];
     const int t = at.type;
     double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
-    const double* __restrict__ hop = static_cast<const double*>(a.hklop);
-    const double* __restrict__ rad = static_cast<const double*>(a.radial);
     for (int k = k0; k < k1; ++k) {
         const double* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
         const double S = rr[0] + anr, G0 = rr[1], G1 = rr[2], G2 = rr[3], G3 = rr[4], G4 = rr[5];
-        const double cr = a.dtdf[2 * k], ci = a.dtdf[2 * k + 1];
+        const double cr = ld_f64(dtdf32, 2 * k), ci = ld_f64(dtdf32, 2 * k + 1);
         double sr = 0, si = 0;
         for (int s = 0; s < a.n_ops; ++s) {
-            const double* r = hop + ((size_t)k * a.n_ops + s) * kHklopRec;
+            const unsigned* rw = hop + ((size_t)k * a.n_ops + s) * (2 * kHklopRec);
+            double r[kHklopRec];
+#pragma unroll
+            for (int j = 0; j < kHklopRec; j++) r[j] = ld_f64(rw, j);
             double A0 = P[0] * r[0];
             double A1 = P[1] * r[1] + P[2] * r[2] + P[3] * r[3];
             double A2 = 0;
@@ -318,14 +380,14 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
         }
         if (DERIV) docc += cr * sr - ci * si;
         double fr_w = wave_sum(w * sr), fi_w = wave_sum(w * si);
-        if (lane == 0) {
-            double* fp = a.fpart + ((size_t)wave * a.n_hkl + k) * 2;
+        if (lane == kWave - 1) {
+            double* fp = fpart + ((size_t)wave * a.n_hkl + k) * 2;
             fp[0] = fr_w;
             fp[1] = fi_w;
         }
     }
     if (DERIV) {
-        double* dp = a.dpart + ((size_t)blockIdx.y * a.n_slots + slot) * 10;
+        double* dp = dpart + ((size_t)blockIdx.y * a.n_slots + slot) * 10;
         dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
 #pragma unroll
         for (int j = 0; j < 6; j++) dp[3 + j] = du[j];
@@ -336,7 +398,10 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
 // Mixed precision: FP64 phase range reduction and FP64 accumulation across reflections, FP32
 // harmonics / radial / Debye-Waller / sincos / exp per term.
 template <int L, bool DERIV>
-__device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane, int k0, int k1)
+__device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane, int k0, int k1,
+                                           const float* __restrict__ hop, const float* __restrict__ rad,
+                                           const unsigned* __restrict__ phs32, const unsigned* __restrict__ dtdf32,
+                                           double* __restrict__ fpart, double* __restrict__ dpart)
 {
     constexpr int NLM = (L + 1) * (L + 1);
     constexpr int kFlush = 16;    // reflections accumulated in FP32 before an FP64 flush
@@ -352,19 +417,17 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
     const float w = (float)at.w, anr = (float)at.anom[0], ani = (float)at.anom[1];
     const int t = at.type;
     double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
-    const float* __restrict__ hop = static_cast<const float*>(a.hklop);
-    const float* __restrict__ rad = static_cast<const float*>(a.radial);
     for (int kb = k0; kb < k1; kb += kFlush) {
         const int kend = min(kb + kFlush, k1);
         float fx[3] = {0, 0, 0}, fu[6] = {0, 0, 0, 0, 0, 0}, focc = 0;
         for (int k = kb; k < kend; ++k) {
             const float* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
             const float S = rr[0] + anr, G0 = rr[1], G1 = rr[2], G2 = rr[3], G3 = rr[4], G4 = rr[5];
-            const float cr = (float)a.dtdf[2 * k], ci = (float)a.dtdf[2 * k + 1];
+            const float cr = (float)ld_f64(dtdf32, 2 * k), ci = (float)ld_f64(dtdf32, 2 * k + 1);
             float sr = 0, si = 0;
             for (int s = 0; s < a.n_ops; ++s) {
                 const float* r = hop + ((size_t)k * a.n_ops + s) * kHklopRec;
-                const double* pr = a.phase + ((size_t)k * a.n_ops + s) * 4;
+                const unsigned* pr = phs32 + ((size_t)k * a.n_ops + s) * 8;
                 float A0 = P[0] * r[0];
                 float A1 = P[1] * r[1] + P[2] * r[2] + P[3] * r[3];
                 float A2 = 0;
@@ -382,7 +445,7 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
                     fi += G3 * A3;
                 }
                 // FP64 range reduction of h.(R x + t) to [-1/2, 1/2) revolutions
-                double ph = pr[0] * x0 + pr[1] * x1 + pr[2] * x2 + pr[3];
+                double ph = ld_f64(pr, 0) * x0 + ld_f64(pr, 1) * x1 + ld_f64(pr, 2) * x2 + ld_f64(pr, 3);
                 float phf = (float)(ph - rint(ph));
                 float sn = __builtin_amdgcn_sinf(phf), cs = __builtin_amdgcn_cosf(phf);
                 float e = r[25] * u[0] + r[26] * u[1] + r[27] * u[2] + r[28] * u[3] + r[29] * u[4] + r[30] * u[5];
@@ -400,8 +463,8 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
             }
             if (DERIV) focc += cr * sr - ci * si;
             float fr_w = wave_sum(w * sr), fi_w = wave_sum(w * si);
-            if (lane == 0) {
-                double* fp = a.fpart + ((size_t)wave * a.n_hkl + k) * 2;
+            if (lane == kWave - 1) {
+                double* fp = fpart + ((size_t)wave * a.n_hkl + k) * 2;
                 fp[0] = fr_w;
                 fp[1] = fi_w;
             }
@@ -417,7 +480,7 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
     if (DERIV) {
         // U multipliers carried log2(e); undo for the derivative
         const double inv_log2e = 0.69314718055994530942;
-        double* dp = a.dpart + ((size_t)blockIdx.y * a.n_slots + slot) * 10;
+        double* dp = dpart + ((size_t)blockIdx.y * a.n_slots + slot) * 10;
         dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
 #pragma unroll
         for (int j = 0; j < 6; j++) dp[3 + j] = du[j] * inv_log2e;
@@ -434,12 +497,18 @@ __global__ __launch_bounds__(256) void k_main(MainArgs a)
     const int k0 = blockIdx.y * a.chunk;
     const int k1 = min(k0 + a.chunk, a.n_hkl);
     const int cls = __builtin_amdgcn_readfirstlane(a.wave_l[wave]);
+    const unsigned* dtdf32 = reinterpret_cast<const unsigned*>(a.dtdf);
     if (MIXED) {
-        if (cls <= 2) body_mixed<2, DERIV>(a, wave, lane, k0, k1);
-        else body_mixed<4, DERIV>(a, wave, lane, k0, k1);
+        const float* hop = static_cast<const float*>(a.hklop);
+        const float* rad = static_cast<const float*>(a.radial);
+        const unsigned* phs = reinterpret_cast<const unsigned*>(a.phase);
+        if (cls <= 2) body_mixed<2, DERIV>(a, wave, lane, k0, k1, hop, rad, phs, dtdf32, a.fpart, a.dpart);
+        else body_mixed<4, DERIV>(a, wave, lane, k0, k1, hop, rad, phs, dtdf32, a.fpart, a.dpart);
     } else {
-        if (cls <= 2) body_fp64<2, DERIV>(a, wave, lane, k0, k1);
-        else body_fp64<4, DERIV>(a, wave, lane, k0, k1);
+        const unsigned* hop = static_cast<const unsigned*>(a.hklop);
+        const double* rad = static_cast<const double*>(a.radial);
+        if (cls <= 2) body_fp64<2, DERIV>(a, wave, lane, k0, k1, hop, rad, dtdf32, a.fpart, a.dpart);
+        else body_fp64<4, DERIV>(a, wave, lane, k0, k1, hop, rad, dtdf32, a.fpart, a.dpart);
     }
 }
 
@@ -695,6 +764,7 @@ void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const doub
 void Engine::run(Precision prec, bool derivs)
 {
     HIP_CHECK(hipSetDevice(device_));
+    ran_ = true;
     const int n_ops = (int)ops_.size();
     const int n_types = std::max(model_.n_types, 1);
     const int n_hkl = n_hkl_;
@@ -729,14 +799,19 @@ void Engine::run(Precision prec, bool derivs)
                        mixed ? d_phase_.as<double>() : nullptr, hlen.as<double>());
     HIP_CHECK(hipGetLastError());
     if (model_.n_types > 0) {
+        d_fcore_.ensure((size_t)std::max(model_.n_wfn, 1) * n_hkl * sizeof(double));
+        int nw = n_hkl * model_.n_wfn;
+        hipLaunchKernelGGL(k_core_prep, dim3((nw + T - 1) / T), dim3(T), 0, stream_, hlen.as<double>(), wfns,
+                           d_core_.as<TermRec>(), model_.n_wfn, n_hkl, d_fcore_.as<double>());
+        HIP_CHECK(hipGetLastError());
         int nt = n_hkl * model_.n_types;
-        hipLaunchKernelGGL(k_radial_prep, dim3((nt + T - 1) / T), dim3(T), 0, stream_, hlen.as<double>(), types,
-                           wfns, d_core_.as<TermRec>(), d_val_.as<TermRec>(), model_.n_types, n_hkl,
+        hipLaunchKernelGGL(k_radial_prep, dim3((nt + T - 1) / T), dim3(T), 0, stream_, hlen.as<double>(),
+                           d_fcore_.as<double>(), types, wfns, d_val_.as<TermRec>(), model_.n_types, n_hkl,
                            mixed ? nullptr : d_radial_.as<double>(), mixed ? d_radial_.as<float>() : nullptr);
         HIP_CHECK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_atom_prep, dim3((n_slots + T - 1) / T), dim3(T), 0, stream_, ain, types,
-                       d_atoms_.as<AtomRec>(), n_slots);
+    hipLaunchKernelGGL(k_atom_prep, dim3(n_slots), dim3(64), 0, stream_, ain, types, d_atoms_.as<AtomRec>(),
+                       n_slots);
     HIP_CHECK(hipGetLastError());
     // launch geometry: enough waves to fill 256 CUs several times over
     int chunk = tune.hkl_chunk;
@@ -797,6 +872,7 @@ void Engine::run(Precision prec, bool derivs)
 void Engine::download(double* f, double* d10)
 {
     HIP_CHECK(hipSetDevice(device_));
+    if (!ran_) throw Error(1, "no results: run the calculation before fetching");
     if (f && n_hkl_ > 0)
         HIP_CHECK(hipMemcpyAsync(f, d_f_.p, (size_t)n_hkl_ * 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
     if (d10 && n_atoms_ > 0)

@@ -75,12 +75,13 @@ private:
     bool centric_origin_;
     double f2c_[9], c2f_[9];
     int n_atoms_ = 0, n_sorted_ = 0, n_hkl_ = 0;
+    bool ran_ = false;
     std::vector<int> order_;        // sorted slot -> caller atom index (-1 padding)
     // model tables
     DeviceBuffer d_core_, d_val_, d_type_, d_ops_;
     // per call
     DeviceBuffer d_hkl_, d_dtdf_, d_atoms_in_, d_atoms_, d_hklop_, d_hklop32_, d_phase_, d_radial_,
-        d_radial32_, d_fpart_, d_dpart_, d_f_, d_d_, d_order_;
+        d_radial32_, d_fpart_, d_dpart_, d_f_, d_d_, d_order_, d_fcore_;
 };
 
 }  // namespace discamb_amd

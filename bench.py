@@ -39,10 +39,11 @@ def algorithmic_flops(w):
     return tot * n_ops * w.hkl.shape[0]
 
 
-def time_mode(calc, mode, steps, warmup):
-    import ctypes  # noqa: F401
+def time_mode(calc, mode, steps, warmup, tune=True):
     calc.setCalculationMode(mode)
     calc.setTiming(False)
+    if tune:
+        calc.autotune()
     for _ in range(warmup):
         calc.run()
     if warmup:
@@ -59,7 +60,7 @@ def time_mode(calc, mode, steps, warmup):
     return (t1 - t0) / steps, float(np.mean(main_ms)), info, f, d
 
 
-def cpu_baseline(w, budget_refl=16000):
+def cpu_baseline(w, budget_refl=66592):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import hc_oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -71,8 +72,8 @@ def cpu_baseline(w, budget_refl=16000):
     dt = time.perf_counter() - t0
     terms = w.crystal.atoms.n * w.n_explicit_ops * len(sel)
     return {"value": terms / dt, "unit": "terms/s", "cores": threads, "kind": "port",
-            "sample": "FP64 OpenMP oracle (direct sincos/exp per term), same 5000-atom P212121 structure, "
-                      "%d random reflections of 66 592, F + xyz/U/occ derivatives, %.1f s" % (len(sel), dt)}
+            "sample": "FP64 OpenMP oracle (direct sincos/exp per term) on the same 5000-atom P212121 "
+                      "structure, %d of its 66 592 reflections, F + xyz/U/occ derivatives, %.1f s wall" % (len(sel), dt)}
 
 
 def main():
@@ -81,7 +82,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=0, help="fixed reflection chunk (disables the autotuner)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain read+write elementwise kernel over MIB MiB (PMC byte calibration)")
     args = ap.parse_args()
@@ -110,10 +111,11 @@ def main():
     flops = algorithmic_flops(w)
     print("workload %s: %d atoms x %d ops x %d hkl = %.3e terms" % (
         w.name, w.crystal.atoms.n, w.n_explicit_ops, w.hkl.shape[0], terms), file=sys.stderr, flush=True)
-    s64, k64, info64, f64, d64 = time_mode(calc, "HIP", args.steps, args.warmup)
+    tune = args.chunk == 0
+    s64, k64, info64, f64, d64 = time_mode(calc, "HIP", args.steps, args.warmup, tune)
     print("fp64: %.3f ms/step, main kernel %.3f ms, grid %s" % (s64 * 1e3, k64, info64["grid"]), file=sys.stderr,
           flush=True)
-    smx, kmx, infomx, fmx, dmx = time_mode(calc, "HIP_MIXED", args.steps, args.warmup)
+    smx, kmx, infomx, fmx, dmx = time_mode(calc, "HIP_MIXED", args.steps, args.warmup, tune)
     print("mixed: %.3f ms/step, main kernel %.3f ms, grid %s" % (smx * 1e3, kmx, infomx["grid"]),
           file=sys.stderr, flush=True)
 
@@ -154,7 +156,8 @@ def main():
                  "frac_of_fp32_peak": flops / (k64 * 1e-3) / FP32_PEAK},
         "speedup_mixed_vs_fp64": s64 / smx,
         "accuracy_mixed_vs_fp64": acc,
-        "launch": {"grid": infomx["grid"], "chunk": infomx["chunk"]},
+        "launch": {"grid": infomx["grid"], "chunk": infomx["chunk"], "fp64_chunk": info64["chunk"],
+                   "autotuned": tune},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w)

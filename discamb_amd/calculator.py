@@ -216,6 +216,13 @@ class HansenCoppensStructureFactorCalculator:
                                          _lib.ptr(du, ctypes.c_double), _lib.ptr(do, ctypes.c_double)))
         return f[:nh, 0] + 1j * f[:nh, 1], TargetFunctionAtomicParamDerivatives(dx[:n], du[:n], do[:n])
 
+    def autotune(self, derivativesSelector: DerivativesSelector = None):
+        """Pick the fastest launch shape for the uploaded inputs in the current mode."""
+        sel = (derivativesSelector or DerivativesSelector()).bits()
+        c = ctypes.c_int()
+        _lib.check(self._lib.dh_hc_autotune(self._h, sel, ctypes.byref(c)))
+        return c.value
+
     def setTiming(self, on=True):
         _lib.check(self._lib.dh_hc_set_timing(self._h, 1 if on else 0))
 

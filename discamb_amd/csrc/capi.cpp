@@ -375,6 +375,15 @@ int dh_hc_sf_and_derivs(dh_calculator* calc, const dh_atoms* atoms, int n_hkl, c
     });
 }
 
+int dh_hc_autotune(dh_calculator* calc, unsigned selector, int* chunk)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        int c = calc->c->engine->autotune(calc->c->prec, (selector & DH_D_ALL) != 0);
+        if (chunk) *chunk = c;
+    });
+}
+
 int dh_hc_set_timing(dh_calculator* calc, int on)
 {
     return guarded([&] {

@@ -761,29 +761,32 @@ void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const doub
     HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
-void Engine::run(Precision prec, bool derivs)
+__global__ void k_fill(double* __restrict__ p, size_t n, double v)
 {
-    HIP_CHECK(hipSetDevice(device_));
-    ran_ = true;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void Engine::fill(DeviceBuffer& b, size_t n, double v)
+{
+    if (n == 0) return;
+    const int T = 256;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + T - 1) / T)), dim3(T), 0, stream_, b.as<double>(), n, v);
+    HIP_CHECK(hipGetLastError());
+}
+
+void Engine::run_prep(Precision prec)
+{
     const int n_ops = (int)ops_.size();
     const int n_types = std::max(model_.n_types, 1);
     const int n_hkl = n_hkl_;
     const int n_slots = n_sorted_;
-    const int n_waves = n_slots / kWave;
     const bool mixed = prec == kMixed;
-    d_f_.ensure(std::max(n_hkl, 1) * 2 * sizeof(double));
-    d_d_.ensure(std::max(n_atoms_, 1) * 10 * sizeof(double));
-    HIP_CHECK(hipMemsetAsync(d_d_.p, 0, std::max(n_atoms_, 1) * 10 * sizeof(double), stream_));
-    if (n_hkl == 0) return;
-    if (n_waves == 0) {
-        HIP_CHECK(hipMemsetAsync(d_f_.p, 0, (size_t)n_hkl * 2 * sizeof(double), stream_));
-        return;
-    }
     const size_t n_rec = (size_t)n_hkl * n_ops;
     d_radial_.ensure((size_t)n_hkl * n_types * kRadRec * (mixed ? sizeof(float) : sizeof(double)));
     d_hklop_.ensure(n_rec * kHklopRec * (mixed ? sizeof(float) : sizeof(double)));
     if (mixed) d_phase_.ensure(n_rec * 4 * sizeof(double));
-    DeviceBuffer& hlen = d_radial32_;   // scratch for |h|
+    DeviceBuffer& hlen = d_radial32_;   // |h| per reflection
     hlen.ensure((size_t)n_hkl * sizeof(double));
     const OpDev* ops = d_ops_.as<OpDev>();
     const double* c2f = reinterpret_cast<const double*>(static_cast<const char*>(d_ops_.p) + n_ops * sizeof(OpDev));
@@ -791,8 +794,6 @@ void Engine::run(Precision prec, bool derivs)
     const WfnRec* wfns = reinterpret_cast<const WfnRec*>(static_cast<const char*>(d_type_.p) +
                                                          model_.n_types * sizeof(TypeRec));
     const AtomIn* ain = d_atoms_in_.as<AtomIn>();
-    const int* wave_cls = reinterpret_cast<const int*>(static_cast<const char*>(d_atoms_in_.p) +
-                                                       std::max(n_sorted_, 1) * sizeof(AtomIn));
     const int T = 256;
     hipLaunchKernelGGL(k_hkl_prep, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, d_hkl_.as<int>(), ops, n_ops,
                        n_hkl, c2f, mixed ? nullptr : d_hklop_.as<double>(), mixed ? d_hklop_.as<float>() : nullptr,
@@ -813,13 +814,41 @@ void Engine::run(Precision prec, bool derivs)
     hipLaunchKernelGGL(k_atom_prep, dim3(n_slots), dim3(64), 0, stream_, ain, types, d_atoms_.as<AtomRec>(),
                        n_slots);
     HIP_CHECK(hipGetLastError());
-    // launch geometry: enough waves to fill 256 CUs several times over
-    int chunk = tune.hkl_chunk;
-    if (chunk <= 0) {
-        const int target_waves = 8192;
-        int n_chunks = std::max(1, (target_waves + n_waves - 1) / n_waves);
-        chunk = std::max(16, (n_hkl + n_chunks - 1) / n_chunks);
-    }
+}
+
+static const void* main_kernel(bool mixed, bool derivs)
+{
+    if (mixed) return derivs ? (const void*)k_main<true, true> : (const void*)k_main<true, false>;
+    return derivs ? (const void*)k_main<false, true> : (const void*)k_main<false, false>;
+}
+
+int Engine::resident_waves(bool mixed, bool derivs)
+{
+    int key = (mixed ? 2 : 0) + (derivs ? 1 : 0);
+    if (resident_waves_[key] > 0) return resident_waves_[key];
+    int blocks = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, main_kernel(mixed, derivs), 256, 0));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, device_));
+    resident_waves_[key] = std::max(1, blocks) * kWavesPerBlock * prop.multiProcessorCount;
+    return resident_waves_[key];
+}
+
+int Engine::default_chunk(bool mixed, bool derivs) 
+{
+    // about four resident-capacity rounds of (atom wave, reflection chunk) work units
+    const int n_waves = n_sorted_ / kWave;
+    const double target = 4.0 * resident_waves(mixed, derivs);
+    int n_chunks = std::max(1, (int)std::lround(target / std::max(n_waves, 1)));
+    return std::max(8, (n_hkl_ + n_chunks - 1) / n_chunks);
+}
+
+void Engine::launch_main(Precision prec, bool derivs, int chunk)
+{
+    const bool mixed = prec == kMixed;
+    const int n_slots = n_sorted_;
+    const int n_waves = n_slots / kWave;
+    const int n_hkl = n_hkl_;
     const int n_chunks = (n_hkl + chunk - 1) / chunk;
     d_fpart_.ensure((size_t)n_waves * n_hkl * 2 * sizeof(double));
     d_dpart_.ensure((size_t)n_chunks * n_slots * 10 * sizeof(double));
@@ -827,24 +856,22 @@ void Engine::run(Precision prec, bool derivs)
     a.hklop = d_hklop_.p;
     a.phase = mixed ? d_phase_.as<double>() : nullptr;
     a.radial = d_radial_.p;
-    a.dtdf = nullptr;
+    a.dtdf = d_dtdf_.as<double>();    // [re, im] pairs followed by the centering multipliers
     a.atoms = d_atoms_.as<AtomRec>();
-    a.wave_l = wave_cls;
+    a.wave_l = reinterpret_cast<const int*>(static_cast<const char*>(d_atoms_in_.p) +
+                                            std::max(n_sorted_, 1) * sizeof(AtomIn));
     a.fpart = d_fpart_.as<double>();
     a.dpart = d_dpart_.as<double>();
     a.n_hkl = n_hkl;
-    a.n_ops = n_ops;
-    a.n_types = n_types;
+    a.n_ops = (int)ops_.size();
+    a.n_types = std::max(model_.n_types, 1);
     a.n_slots = n_slots;
     a.n_waves = n_waves;
     a.chunk = chunk;
-    // dtdf is stored as [re, im] pairs followed by the multipliers
-    a.dtdf = d_dtdf_.as<double>();
     dim3 grid((n_waves + kWavesPerBlock - 1) / kWavesPerBlock, n_chunks);
     last_grid_x = grid.x;
     last_grid_y = grid.y;
     last_chunk = chunk;
-    if (timing) HIP_CHECK(hipEventRecord(ev0_, stream_));
     if (mixed) {
         if (derivs) hipLaunchKernelGGL((k_main<true, true>), grid, dim3(256), 0, stream_, a);
         else hipLaunchKernelGGL((k_main<true, false>), grid, dim3(256), 0, stream_, a);
@@ -853,13 +880,50 @@ void Engine::run(Precision prec, bool derivs)
         else hipLaunchKernelGGL((k_main<false, false>), grid, dim3(256), 0, stream_, a);
     }
     HIP_CHECK(hipGetLastError());
+}
+
+int Engine::chunk_for(Precision prec, bool derivs)
+{
+    if (tune.hkl_chunk > 0) return tune.hkl_chunk;
+    auto it = tuned_.find(shape_key(prec, derivs));
+    if (it != tuned_.end()) return it->second;
+    return default_chunk(prec == kMixed, derivs);
+}
+
+long long Engine::shape_key(Precision prec, bool derivs) const
+{
+    return (((long long)n_sorted_ * 1000003LL + n_hkl_) * 64 + (long long)ops_.size()) * 4 +
+           (prec == kMixed ? 2 : 0) + (derivs ? 1 : 0);
+}
+
+void Engine::run(Precision prec, bool derivs)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    ran_ = true;
+    const int n_hkl = n_hkl_;
+    const int n_slots = n_sorted_;
+    const int n_waves = n_slots / kWave;
+    d_f_.ensure(std::max(n_hkl, 1) * 2 * sizeof(double));
+    d_d_.ensure(std::max(n_atoms_, 1) * 10 * sizeof(double));
+    fill(d_d_, (size_t)std::max(n_atoms_, 1) * 10, 0.0);
+    if (n_hkl == 0) return;
+    if (n_waves == 0) {
+        fill(d_f_, (size_t)n_hkl * 2, 0.0);
+        return;
+    }
+    run_prep(prec);
+    const int chunk = chunk_for(prec, derivs);
+    const int n_chunks = (n_hkl + chunk - 1) / chunk;
+    if (timing) HIP_CHECK(hipEventRecord(ev0_, stream_));
+    launch_main(prec, derivs, chunk);
     if (timing) HIP_CHECK(hipEventRecord(ev1_, stream_));
-    hipLaunchKernelGGL(k_reduce_f, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, a.fpart,
+    const int T = 256;
+    hipLaunchKernelGGL(k_reduce_f, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, d_fpart_.as<double>(),
                        d_dtdf_.as<double>() + 2 * (size_t)n_hkl, n_waves, n_hkl, centric_origin_ ? 1 : 0,
                        d_f_.as<double>());
     HIP_CHECK(hipGetLastError());
     if (derivs) {
-        hipLaunchKernelGGL(k_reduce_d, dim3((n_slots + T - 1) / T), dim3(T), 0, stream_, a.dpart,
+        hipLaunchKernelGGL(k_reduce_d, dim3((n_slots + T - 1) / T), dim3(T), 0, stream_, d_dpart_.as<double>(),
                            d_atoms_.as<AtomRec>(), n_chunks, n_slots, d_d_.as<double>());
         HIP_CHECK(hipGetLastError());
     }
@@ -867,6 +931,41 @@ void Engine::run(Precision prec, bool derivs)
         HIP_CHECK(hipEventSynchronize(ev1_));
         HIP_CHECK(hipEventElapsedTime(&last_main_ms, ev0_, ev1_));
     }
+}
+
+int Engine::autotune(Precision prec, bool derivs)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    const int n_waves = n_sorted_ / kWave;
+    if (n_hkl_ == 0 || n_waves == 0) return 0;
+    run_prep(prec);
+    const int cap = resident_waves(prec == kMixed, derivs);
+    std::vector<int> cands;
+    for (double rounds : {1.0, 2.0, 3.0, 4.0, 6.0, 8.0, 12.0, 16.0, 24.0}) {
+        int n_chunks = std::max(1, (int)std::lround(rounds * cap / n_waves));
+        int c = std::max(8, (n_hkl_ + n_chunks - 1) / n_chunks);
+        if (std::find(cands.begin(), cands.end(), c) == cands.end()) cands.push_back(c);
+    }
+    int best = cands.front();
+    float best_ms = 1e30f;
+    for (int c : cands) {
+        float ms_min = 1e30f;
+        for (int rep = 0; rep < 2; rep++) {
+            HIP_CHECK(hipEventRecord(ev0_, stream_));
+            launch_main(prec, derivs, c);
+            HIP_CHECK(hipEventRecord(ev1_, stream_));
+            HIP_CHECK(hipEventSynchronize(ev1_));
+            float ms = 0;
+            HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+            ms_min = std::min(ms_min, ms);
+        }
+        if (ms_min < best_ms) {
+            best_ms = ms_min;
+            best = c;
+        }
+    }
+    tuned_[shape_key(prec, derivs)] = best;
+    return best;
 }
 
 void Engine::download(double* f, double* d10)

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <vector>
 
 #include "hc_internal.hpp"
@@ -58,6 +59,9 @@ public:
     // dU_cart[6] (x 2 pi^2 w), docc (x mult/|G|)) back, in caller atom order; synchronises.
     void download(double* f, double* d10);
     void synchronize();
+    // Time the fused kernel over candidate reflection-chunk sizes on the uploaded inputs and keep
+    // the fastest for this (atoms, reflections, ops, precision) shape; returns the chunk chosen.
+    int autotune(Precision prec, bool derivs);
     hipStream_t stream() const { return stream_; }
     TuneParams tune;
     // last run: launch geometry of the main kernel
@@ -67,6 +71,15 @@ public:
     int n_ops() const { return (int)ops_.size(); }
 
 private:
+    void run_prep(Precision prec);
+    void launch_main(Precision prec, bool derivs, int chunk);
+    void fill(DeviceBuffer& b, size_t n, double v);
+    int resident_waves(bool mixed, bool derivs);
+    int default_chunk(bool mixed, bool derivs);
+    int chunk_for(Precision prec, bool derivs);
+    long long shape_key(Precision prec, bool derivs) const;
+    int resident_waves_[4] = {0, 0, 0, 0};
+    std::map<long long, int> tuned_;
     int device_;
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;

@@ -127,6 +127,10 @@ int dh_hc_upload(dh_calculator* calc, const dh_atoms* atoms, int n_hkl, const in
                  const double* dtarget_df);
 int dh_hc_run(dh_calculator* calc, unsigned selector);
 int dh_hc_fetch(dh_calculator* calc, double* f, double* d_xyz, double* d_adp, double* d_occ);
+/* Autotuner: on the uploaded inputs, time the fused kernel for candidate launch shapes (reflection
+ * chunk per workgroup column, derived from the device's resident-wave capacity) in the current
+ * calculation mode and keep the fastest for this workload shape.  Returns the chosen chunk. */
+int dh_hc_autotune(dh_calculator* calc, unsigned selector, int* chunk);
 /* Main-kernel time of the last dh_hc_run in ms (HIP events on the engine stream) and launch
  * geometry; enable with dh_hc_set_timing(calc, 1). */
 int dh_hc_set_timing(dh_calculator* calc, int on);

@@ -1,0 +1,123 @@
+// Test driver for the C++ drop-in class: reads a structure as a whitespace token stream (written
+// by tests/test_cpp_api.py), runs HansenCoppensStructureFactorCalculator on the GPU and prints
+// F and the derivatives.  Usage: hc_cpp_driver <input> <mode>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+
+#include "discamb_amd/HansenCoppensStructureFactorCalculator.hpp"
+
+using namespace discamb_amd;
+
+int main(int argc, char** argv)
+{
+    if (argc < 3) {
+        std::cerr << "usage: hc_cpp_driver <input> <mode>\n";
+        return 2;
+    }
+    std::ifstream in(argv[1]);
+    Crystal crystal;
+    HC_ModelParameters p;
+    in >> crystal.unitCell.a >> crystal.unitCell.b >> crystal.unitCell.c >> crystal.unitCell.alpha >>
+        crystal.unitCell.beta >> crystal.unitCell.gamma;
+    int n;
+    in >> n;
+    crystal.spaceGroupOperations.resize(n);
+    for (auto& s : crystal.spaceGroupOperations) in >> s;
+    in >> n;
+    p.wfn_parameters.resize(n);
+    for (auto& w : p.wfn_parameters) {
+        int norb;
+        in >> w.wfnAtomTypeName >> norb;
+        w.orbitals.resize(norb);
+        for (auto& o : w.orbitals) {
+            int nt;
+            in >> o.principal_number >> o.azimuthal_number >> nt;
+            o.coefficients.resize(nt);
+            o.exponents.resize(nt);
+            o.power_r.resize(nt);
+            for (int t = 0; t < nt; t++) in >> o.coefficients[t] >> o.exponents[t] >> o.power_r[t];
+        }
+        w.orbital_occupancy.resize(norb);
+        for (auto& v : w.orbital_occupancy) in >> v;
+        int k;
+        in >> k;
+        w.core_orbitals_indices.resize(k);
+        for (auto& v : w.core_orbitals_indices) in >> v;
+        in >> k;
+        w.valence_orbitals_indices.resize(k);
+        for (auto& v : w.valence_orbitals_indices) in >> v;
+        in >> w.deformation_valence_exponent >> k;
+        w.deformation_valence_power.resize(k);
+        for (auto& v : w.deformation_valence_power) in >> v;
+        double re, im;
+        in >> re >> im;
+        w.anomalous_scattering = {re, im};
+    }
+    in >> n;
+    p.type_parameters.resize(n);
+    for (auto& t : p.type_parameters) {
+        int nl;
+        in >> nl;
+        t.p_lm.resize(nl);
+        for (int l = 0; l < nl; l++) {
+            t.p_lm[l].resize(2 * l + 1);
+            for (auto& v : t.p_lm[l]) in >> v;
+        }
+        in >> t.p_val >> t.kappa_spherical_valence >> t.kappa_deformation_valence;
+    }
+    in >> n;
+    crystal.atoms.resize(n);
+    std::vector<Matrix3d> lcs(n);
+    std::vector<bool> include(n);
+    p.atom_to_wfn_map.resize(n);
+    p.atom_to_type_map.resize(n);
+    for (int i = 0; i < n; i++) {
+        AtomInCrystal& a = crystal.atoms[i];
+        int na, inc;
+        in >> p.atom_to_wfn_map[i] >> p.atom_to_type_map[i] >> a.coordinates.x >> a.coordinates.y >>
+            a.coordinates.z >> na;
+        a.adp.resize(na);
+        for (auto& v : a.adp) in >> v;
+        in >> a.occupancy >> a.multiplicity;
+        for (int j = 0; j < 9; j++) in >> lcs[i].m[j];
+        in >> inc;
+        include[i] = inc != 0;
+    }
+    in >> n;
+    std::vector<Vector3i> hkl(n);
+    std::vector<std::complex<double>> dtdf(n);
+    for (int k = 0; k < n; k++) {
+        double re, im;
+        in >> hkl[k].x >> hkl[k].y >> hkl[k].z >> re >> im;
+        dtdf[k] = {re, im};
+    }
+    try {
+        HansenCoppensStructureFactorCalculator calc(crystal, p);
+        calc.setCalculationMode(argv[2]);
+        std::vector<std::complex<double>> f;
+        std::vector<TargetFunctionAtomicParamDerivatives> d;
+        calc.calculateStructureFactorsAndDerivatives(crystal.atoms, lcs, hkl, f, d, dtdf, include);
+        std::cout << std::setprecision(17);
+        for (auto& v : f) std::cout << v.real() << " " << v.imag() << "\n";
+        for (auto& v : d) {
+            std::cout << v.atomic_position_derivatives.x << " " << v.atomic_position_derivatives.y << " "
+                      << v.atomic_position_derivatives.z << " " << v.adp_derivatives.size();
+            for (double u : v.adp_derivatives) std::cout << " " << u;
+            std::cout << " " << v.occupancy_derivatives << "\n";
+        }
+        // the reference error behaviour: inconsistent dT/dF size throws
+        bool threw = false;
+        try {
+            std::vector<std::complex<double>> bad(dtdf.begin(), dtdf.end() - 1);
+            calc.calculateStructureFactorsAndDerivatives(crystal.atoms, lcs, hkl, f, d, bad, include);
+        } catch (const Exception&) {
+            threw = true;
+        }
+        std::cerr << (threw ? "size check ok\n" : "size check missing\n");
+        return threw ? 0 : 1;
+    } catch (const Exception& e) {
+        std::cerr << "error " << e.code << ": " << e.what() << "\n";
+        return 1;
+    }
+}

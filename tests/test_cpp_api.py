@@ -1,0 +1,85 @@
+"""The C++ drop-in class (include/discamb_amd/HansenCoppensStructureFactorCalculator.hpp) end to end:
+a compiled driver builds Crystal / HC_ModelParameters / AtomInCrystal values, calls
+calculateStructureFactorsAndDerivatives on the GPU, and the result is compared with the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import hc_oracle
+from conftest import ROOT, rel_err
+from discamb_amd import synth
+
+pytestmark = pytest.mark.gpu
+DRIVER = os.path.join(ROOT, "tests", "cpp", "hc_cpp_driver")
+
+
+def write_input(w, path, include):
+    tok = []
+    tok += list(w.crystal.cell)
+    tok += [len(w.crystal.symops)] + [s.replace(" ", "") for s in w.crystal.symops]
+    tok += [len(w.params.wfn_parameters)]
+    for wf in w.params.wfn_parameters:
+        tok += [wf.label, len(wf.orbitals)]
+        for o in wf.orbitals:
+            tok += [o.n, o.l, len(o.coefficients)]
+            for c, e, p in zip(o.coefficients, o.exponents, o.powers):
+                tok += [repr(float(c)), repr(float(e)), p]
+        tok += list(wf.orbital_occupancy)
+        tok += [len(wf.core_orbitals_indices)] + list(wf.core_orbitals_indices)
+        tok += [len(wf.valence_orbitals_indices)] + list(wf.valence_orbitals_indices)
+        tok += [repr(wf.deformation_valence_exponent), len(wf.deformation_valence_power)]
+        tok += list(wf.deformation_valence_power)
+        tok += [0.0, 0.0]
+    tok += [len(w.params.type_parameters)]
+    for t in w.params.type_parameters:
+        tok += [len(t.p_lm)]
+        for row in t.p_lm:
+            tok += [repr(float(v)) for v in row]
+        tok += [repr(t.p_val), repr(t.kappa_spherical_valence), repr(t.kappa_deformation_valence)]
+    a = w.crystal.atoms
+    tok += [a.n]
+    for i in range(a.n):
+        tok += [w.params.atom_to_wfn_map[i], w.params.atom_to_type_map[i]]
+        tok += [repr(float(v)) for v in a.xyz[i]]
+        tok += [int(a.adp_size[i])] + [repr(float(v)) for v in a.adp[i, :a.adp_size[i]]]
+        tok += [repr(float(a.occupancy[i])), repr(float(a.multiplicity[i]))]
+        tok += [repr(float(v)) for v in w.lcs[i].reshape(-1)]
+        tok += [1 if include[i] else 0]
+    tok += [len(w.hkl)]
+    for h, d in zip(w.hkl, w.dtdf):
+        tok += [int(h[0]), int(h[1]), int(h[2]), repr(d.real), repr(d.imag)]
+    with open(path, "w") as f:
+        f.write(" ".join(str(t) for t in tok))
+
+
+@pytest.mark.parametrize("sg", ["P21/c", "P212121"])
+def test_cpp_drop_in_matches_oracle(tmp_path, sg):
+    if not os.path.exists(DRIVER):
+        pytest.fail("tests/cpp/hc_cpp_driver not built (run __graft_entry__.build())")
+    w = synth.small_structure(sg, n_atoms=9, d_min=1.1)
+    include = np.ones(9, dtype=bool)
+    include[3] = False
+    inp = tmp_path / "in.txt"
+    write_input(w, str(inp), include)
+    print("case cpp driver", sg, flush=True)
+    out = subprocess.run([DRIVER, str(inp), "HIP"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "size check ok" in out.stderr
+    lines = out.stdout.strip().split("\n")
+    m = len(w.hkl)
+    f = np.array([complex(*map(float, l.split())) for l in lines[:m]])
+    dx, du, do = [], np.zeros((9, 6)), []
+    for i, l in enumerate(lines[m:]):
+        v = [float(x) for x in l.split()]
+        dx.append(v[:3])
+        na = int(v[3])
+        du[i, :na] = v[4:4 + na]
+        do.append(v[4 + na])
+    rf, rx, ru, ro = hc_oracle.sf_and_derivs(w.crystal, w.params, w.crystal.atoms, w.lcs, w.hkl, w.dtdf,
+                                             include=include)
+    assert rel_err(f, rf) < 1e-10
+    assert rel_err(np.array(dx), rx) < 1e-10
+    assert rel_err(du, ru) < 1e-10
+    assert rel_err(np.array(do), ro) < 1e-10

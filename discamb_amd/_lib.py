@@ -73,6 +73,8 @@ SIGNATURES = {
     "dh_hc_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DhAtoms), ctypes.c_int, c_int_p, c_dbl_p]),
     "dh_hc_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint]),
     "dh_hc_fetch": (ctypes.c_int, [ctypes.c_void_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p]),
+    "dh_hc_set_type_parameters": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p]),
+    "dh_hc_fetch_type_derivatives": (ctypes.c_int, [ctypes.c_void_p, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p]),
     "dh_hc_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, c_int_p]),
     "dh_hc_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "dh_hc_last_timing": (ctypes.c_int, [ctypes.c_void_p, c_dbl_p, c_int_p, c_int_p, c_int_p]),

@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib
 from .model import (AtomSet, Crystal, DerivativesSelector, HC_ModelParameters,
-                    TargetFunctionAtomicParamDerivatives, parse_symop)
+                    TargetFunctionAtomicParamDerivatives, TypeParameterDerivatives, parse_symop)
 
 XYZ = {"fractional": 0, "cartesian": 1}
 ADP = {"U_cif": 0, "U_cart": 1}
@@ -110,6 +110,7 @@ class HansenCoppensStructureFactorCalculator:
         _lib.check(self._lib.dh_hc_create(ctypes.byref(desc), int(device), ctypes.byref(h)))
         self._h = h
         self.n_atoms = len(parameters.atom_to_type_map)
+        self.n_types = len(parameters.type_parameters)
         self._keep = None
 
     def __del__(self):
@@ -121,6 +122,26 @@ class HansenCoppensStructureFactorCalculator:
     # -- reference API -----------------------------------------------------------------------
     def setCalculationMode(self, mode: str):
         _lib.check(self._lib.dh_hc_set_calculation_mode(self._h, mode.encode()))
+
+    def setTypeParameters(self, plm, p_val, kappa, kappa_prime):
+        """Replace the atom-type parameters: plm (n_types, 25) in the l*l + l + m layout, p_val,
+        kappa, kappa_prime (n_types,)."""
+        nt = self.n_types
+        plm = np.ascontiguousarray(np.asarray(plm, dtype=np.float64).reshape(nt, 25))
+        pv, ka, kp = (np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(nt)) for v in (p_val, kappa, kappa_prime))
+        _lib.check(self._lib.dh_hc_set_type_parameters(self._h, nt, _lib.ptr(plm, ctypes.c_double),
+                                                       _lib.ptr(pv, ctypes.c_double), _lib.ptr(ka, ctypes.c_double),
+                                                       _lib.ptr(kp, ctypes.c_double)))
+
+    def typeDerivatives(self):
+        """Per-type multipole derivatives of the last call made with d_multipole."""
+        nt = self.n_types
+        plm = np.zeros((max(nt, 1), 25))
+        pv, ka, kp = np.zeros(max(nt, 1)), np.zeros(max(nt, 1)), np.zeros(max(nt, 1))
+        _lib.check(self._lib.dh_hc_fetch_type_derivatives(self._h, _lib.ptr(plm, ctypes.c_double),
+                                                          _lib.ptr(pv, ctypes.c_double), _lib.ptr(ka, ctypes.c_double),
+                                                          _lib.ptr(kp, ctypes.c_double)))
+        return TypeParameterDerivatives(plm[:nt], pv[:nt], ka[:nt], kp[:nt])
 
     def setTuning(self, hkl_chunk: int):
         _lib.check(self._lib.dh_hc_set_tuning(self._h, int(hkl_chunk)))

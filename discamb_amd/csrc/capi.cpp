@@ -33,6 +33,10 @@ struct Calculator {
     std::vector<double> occ;
     std::vector<double> d10;
     unsigned last_selector = 0;
+    std::vector<AtomType> types;
+    // per-type multipole derivatives of the last run with DH_D_MULTIPOLE
+    std::vector<double> d_plm, d_pval, d_kappa, d_kappa_prime;
+    bool have_mp = false;
 };
 
 static std::vector<EngineOp> make_ops(const Cell& c, const SpaceGroupInfo& g, bool centric_shift)
@@ -132,6 +136,7 @@ static Calculator* create(const dh_model_desc* d, int device)
     c->atom_type.assign(d->atom_to_type, d->atom_to_type + d->n_atoms);
     c->atom_wfn.assign(d->atom_to_wfn, d->atom_to_wfn + d->n_atoms);
     c->model = build_engine_model(wfn, types, c->atom_wfn, c->atom_type);
+    c->types = types;
     c->mult_w.resize(d->n_atoms);
     for (int a = 0; a < d->n_atoms; a++) c->mult_w[a] = d->atom_multiplicity[a] / double(c->sg.order);
     bool zero_t = c->sg.inv_t[0] == 0.0 && c->sg.inv_t[1] == 0.0 && c->sg.inv_t[2] == 0.0;
@@ -222,9 +227,11 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
 static void fetch(Calculator* c, double* f, double* d_xyz, double* d_adp, double* d_occ)
 {
     const int n = c->n_atoms;
-    c->d10.assign(10 * (size_t)std::max(n, 1), 0.0);
-    bool want_d = c->last_selector != 0;
+    const bool want_d = c->last_selector != 0;
+    const int nd = c->engine->nd();
+    c->d10.assign((size_t)std::max(n, 1) * nd, 0.0);
     c->engine->download(f, want_d ? c->d10.data() : nullptr);
+    c->have_mp = false;
     if (!want_d) {
         if (d_xyz) std::memset(d_xyz, 0, sizeof(double) * 3 * n);
         if (d_adp) std::memset(d_adp, 0, sizeof(double) * 6 * n);
@@ -233,7 +240,7 @@ static void fetch(Calculator* c, double* f, double* d_xyz, double* d_adp, double
     }
     const double two_pi = 2.0 * M_PI, two_pi2 = 2.0 * M_PI * M_PI;
     for (int i = 0; i < n; i++) {
-        const double* r = c->d10.data() + 10 * (size_t)i;
+        const double* r = c->d10.data() + (size_t)nd * i;
         double w = c->occ[i] * c->mult_w[i];
         double dxf[3] = {two_pi * w * r[0], two_pi * w * r[1], two_pi * w * r[2]};
         if (d_xyz) {
@@ -260,6 +267,28 @@ static void fetch(Calculator* c, double* f, double* d_xyz, double* d_adp, double
             }
         }
         if (d_occ) d_occ[i] = (c->last_selector & DH_D_OCC) ? c->mult_w[i] * r[9] : 0.0;
+    }
+    if ((c->last_selector & DH_D_MULTIPOLE) && nd >= 38) {
+        const int nt = c->model.n_types;
+        c->d_plm.assign(25 * (size_t)nt, 0.0);
+        c->d_pval.assign(nt, 0.0);
+        c->d_kappa.assign(nt, 0.0);
+        c->d_kappa_prime.assign(nt, 0.0);
+        for (int i = 0; i < n; i++) {
+            const double* r = c->d10.data() + (size_t)nd * i;
+            const double w = c->occ[i] * c->mult_w[i];
+            const int t = c->atom_type[i];
+            for (int j = 0; j < 25; j++) c->d_plm[25 * t + j] += w * r[10 + j];
+            c->d_pval[t] += w * r[35];
+            c->d_kappa[t] += w * r[36];
+            c->d_kappa_prime[t] += w * r[37];
+        }
+        // only entries present in the type's p_lm array are parameters
+        for (int t = 0; t < nt; t++)
+            for (int l = 0; l < 5; l++)
+                if (l >= c->types[t].n_l)
+                    for (int j = l * l; j < (l + 1) * (l + 1); j++) c->d_plm[25 * t + j] = 0.0;
+        c->have_mp = true;
     }
 }
 
@@ -349,8 +378,8 @@ int dh_hc_run(dh_calculator* calc, unsigned selector)
 {
     return guarded([&] {
         if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
-        calc->c->last_selector = selector & DH_D_ALL;
-        calc->c->engine->run(calc->c->prec, calc->c->last_selector != 0);
+        calc->c->last_selector = selector & (DH_D_ALL | DH_D_MULTIPOLE);
+        calc->c->engine->run(calc->c->prec, calc->c->last_selector != 0, (selector & DH_D_MULTIPOLE) != 0);
     });
 }
 
@@ -369,8 +398,8 @@ int dh_hc_sf_and_derivs(dh_calculator* calc, const dh_atoms* atoms, int n_hkl, c
     return guarded([&] {
         if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
         upload(calc->c, atoms, n_hkl, hkl, dtarget_df);
-        calc->c->last_selector = selector & DH_D_ALL;
-        calc->c->engine->run(calc->c->prec, calc->c->last_selector != 0);
+        calc->c->last_selector = selector & (DH_D_ALL | DH_D_MULTIPOLE);
+        calc->c->engine->run(calc->c->prec, calc->c->last_selector != 0, (selector & DH_D_MULTIPOLE) != 0);
         fetch(calc->c, f, d_xyz, d_adp, d_occ);
     });
 }
@@ -379,8 +408,44 @@ int dh_hc_autotune(dh_calculator* calc, unsigned selector, int* chunk)
 {
     return guarded([&] {
         if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
-        int c = calc->c->engine->autotune(calc->c->prec, (selector & DH_D_ALL) != 0);
+        int c = calc->c->engine->autotune(calc->c->prec, (selector & (DH_D_ALL | DH_D_MULTIPOLE)) != 0,
+                                          (selector & DH_D_MULTIPOLE) != 0);
         if (chunk) *chunk = c;
+    });
+}
+
+int dh_hc_set_type_parameters(dh_calculator* calc, int n_types, const double* plm, const double* p_val,
+                              const double* kappa, const double* kappa_prime)
+{
+    return guarded([&] {
+        if (!calc || !plm || !p_val || !kappa || !kappa_prime) throw Error(DH_E_ARGUMENT, "null argument");
+        Calculator* c = calc->c;
+        if (n_types != c->model.n_types) throw Error(DH_E_ARGUMENT, "number of atom types differs from the model");
+        std::vector<AtomType> t = c->types;
+        for (int i = 0; i < n_types; i++) {
+            for (int j = 0; j < 25; j++) t[i].plm[j] = plm[25 * i + j];
+            t[i].p_val = p_val[i];
+            t[i].kappa = kappa[i];
+            t[i].kappa_prime = kappa_prime[i];
+        }
+        c->engine->set_types(t);
+        c->types = t;
+        c->model = c->engine->model();
+    });
+}
+
+int dh_hc_fetch_type_derivatives(dh_calculator* calc, double* d_plm, double* d_p_val, double* d_kappa,
+                                 double* d_kappa_prime)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        Calculator* c = calc->c;
+        if (!c->have_mp) throw Error(DH_E_ARGUMENT, "no multipole derivatives: run with DH_D_MULTIPOLE and fetch first");
+        const size_t nt = c->model.n_types;
+        if (d_plm) std::memcpy(d_plm, c->d_plm.data(), 25 * nt * sizeof(double));
+        if (d_p_val) std::memcpy(d_p_val, c->d_pval.data(), nt * sizeof(double));
+        if (d_kappa) std::memcpy(d_kappa, c->d_kappa.data(), nt * sizeof(double));
+        if (d_kappa_prime) std::memcpy(d_kappa_prime, c->d_kappa_prime.data(), nt * sizeof(double));
     });
 }
 

@@ -52,4 +52,26 @@ __device__ __forceinline__ double g_function(int l, int n, double h, double z)
     return num / dn;
 }
 
+// g(l, n, h, z) and its derivative with respect to h (K = 2 pi h):
+// d/dK [P / d^n] = (P_K d - 2 n K P) / d^(n+1)
+__device__ __forceinline__ double g_function_dh(int l, int n, double h, double z, double* dg_dh)
+{
+    const double k = 6.283185307179586476925 * h;
+    const int off = c_goff[l][n];
+    const double d = k * k + z * z;
+    double acc = c_gcoef[off + n - 1], dacc = (n - 1) * c_gcoef[off + n - 1], zp = 1.0;
+    for (int i = n - 2; i >= 0; i--) {
+        zp *= z;
+        acc = acc * k + c_gcoef[off + i] * zp;
+        if (i >= 1) dacc = dacc * k + i * c_gcoef[off + i] * zp;
+    }
+    // dacc holds sum_i i c_i k^(i-1) z^(n-1-i) once the loop has consumed i = 1
+    double dn = 1.0;
+    for (int i = 0; i < n; i++) dn *= d;
+    const double g = acc / dn;
+    const double dgdk = (dacc * d - 2.0 * n * k * acc) / (dn * d);
+    *dg_dh = 6.283185307179586476925 * dgdk;
+    return g;
+}
+
 }  // namespace discamb_amd

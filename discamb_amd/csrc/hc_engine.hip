@@ -37,7 +37,7 @@ static constexpr int kWave = 64;
 static constexpr int kWavesPerBlock = 4;
 
 struct TypeRec {
-    int wfn, nl, max_l, pad;
+    int wfn, nl, max_l, lp;     // lp = min(4, n_l - 1, size(p_lm) - 1): l range of the multipole adjoint
     int n[5], pad2[3];
     double pval, kappa, kappa_p, zeta;
     double plm[25];
@@ -171,11 +171,13 @@ __global__ void k_core_prep(const double* __restrict__ hlen, const WfnRec* __res
 }
 
 // Per (reflection, type): S = f_core + Pval f_val(h/kappa) and
-// G_l = 4 pi (i^l sign) N_l g_l(n_l + 2, h/kappa', zeta), l <= max_l.
+// G_l = 4 pi (i^l sign) N_l g_l(n_l + 2, h/kappa', zeta), l <= lp.  With radx set, also the
+// multipole-adjoint record [f_val, dS/dkappa, dG_0..4/dkappa', 0].
 __global__ void k_radial_prep(const double* __restrict__ hlen, const double* __restrict__ fcore,
                               const TypeRec* __restrict__ types, const WfnRec* __restrict__ wfns,
                               const TermRec* __restrict__ val, int n_types, int n_hkl,
-                              double* __restrict__ rad64, float* __restrict__ rad32)
+                              double* __restrict__ rad64, float* __restrict__ rad32,
+                              double* __restrict__ radx64, float* __restrict__ radx32)
 {
     int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n_hkl * n_types) return;
@@ -184,29 +186,50 @@ __global__ void k_radial_prep(const double* __restrict__ hlen, const double* __r
     const double h = hlen[k];
     const TypeRec& ty = types[t];
     const WfnRec wf = wfns[ty.wfn];
-    double fv = 0.0;
+    const bool want_d = radx64 || radx32;
+    double fv = 0.0, dfv = 0.0;
     const double hk = h / ty.kappa;
     for (int i = 0; i < wf.val_n; i++) {
         const TermRec tr = val[wf.val_off + i];
-        fv += tr.c * g_function(0, tr.p + 2, hk, tr.e);
+        if (want_d) {
+            double dg;
+            fv += tr.c * g_function_dh(0, tr.p + 2, hk, tr.e, &dg);
+            dfv += tr.c * dg;
+        } else {
+            fv += tr.c * g_function(0, tr.p + 2, hk, tr.e);
+        }
     }
     double r[kRadRec] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double x[kRadRec] = {0, 0, 0, 0, 0, 0, 0, 0};
     r[0] = fcore[(size_t)ty.wfn * n_hkl + k] + ty.pval * four_pi * fv;
+    x[0] = four_pi * fv;
+    // d f_val(h/kappa) / d kappa = f_val'(h/kappa) * (-h / kappa^2)
+    x[1] = ty.pval * four_pi * dfv * (-h / (ty.kappa * ty.kappa));
     const double hp = h / ty.kappa_p;
-    for (int l = 0; l <= ty.max_l && l < ty.nl; l++) {
+    for (int l = 0; l <= ty.lp; l++) {
         int n = ty.n[l];
         double norm = 1.0;
         for (int q = 0; q < n + 3; q++) norm *= ty.zeta;
         for (int q = 2; q <= n + 2; q++) norm /= q;
-        double g = four_pi * norm * g_function(l, n + 2, hp, ty.zeta);
-        // i^l: l = 0 (+, real), 1 (+, imag), 2 (-, real), 3 (-, imag), 4 (+, real)
-        r[1 + l] = (l == 2 || l == 3) ? -g : g;
+        const double sgn = (l == 2 || l == 3) ? -1.0 : 1.0;   // i^l: +, +i, -, -i, +
+        if (want_d) {
+            double dg;
+            double g = g_function_dh(l, n + 2, hp, ty.zeta, &dg);
+            r[1 + l] = sgn * four_pi * norm * g;
+            x[2 + l] = sgn * four_pi * norm * dg * (-h / (ty.kappa_p * ty.kappa_p));
+        } else {
+            r[1 + l] = sgn * four_pi * norm * g_function(l, n + 2, hp, ty.zeta);
+        }
     }
     size_t base = ((size_t)k * n_types + t) * kRadRec;
     if (rad64)
         for (int i = 0; i < kRadRec; i++) rad64[base + i] = r[i];
     if (rad32)
         for (int i = 0; i < kRadRec; i++) rad32[base + i] = (float)r[i];
+    if (radx64)
+        for (int i = 0; i < kRadRec; i++) radx64[base + i] = x[i];
+    if (radx32)
+        for (int i = 0; i < kRadRec; i++) radx32[base + i] = (float)x[i];
 }
 
 // One 64-lane workgroup per atom slot: lanes 0..49 evaluate the local-frame multipole expansion
@@ -264,16 +287,22 @@ struct MainArgs {
     const void* hklop;        // double[36] or float[36] per (k, s)
     const double* phase;      // mixed path: double[4] per (k, s)
     const void* radial;       // [k][t][8]
+    const void* radx;         // multipole adjoint: [k][t][8] = f_val, dS/dkappa, dG_l/dkappa'
     const double* dtdf;       // [k][2]
     const AtomRec* atoms;     // sorted slots
     const int* wave_l;        // multipole class (2 or 4) per wave
     double* fpart;            // [wave][k][2]
-    double* dpart;            // [chunk][slot][10]
+    double* dpart;            // [chunk][slot][nd]
     int n_hkl, n_ops, n_types, n_slots, n_waves, chunk;
 };
 
+// per-atom partial layout: 0-2 d/dx_frac, 3-8 d/dU, 9 d/docc, then with the multipole adjoint
+// 10-34 d/dP_lm (global frame), 35 d/dPval, 36 d/dkappa, 37 d/dkappa'
+static constexpr int kNdBasic = 10;
+static constexpr int kNdMultipole = 38;
+
 // Full-wave sums with DPP lane permutations (quad_perm xor 1, xor 2, row_half_mirror, row_mirror,
-// row_bcast:15, row_bcast:31); the total ends in lane 63 and is broadcast with readlane.
+// row_bcast:15, row_bcast:31); the total ends in lane 63.
 template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ int dpp_mov(int v)
 {
@@ -316,11 +345,42 @@ __device__ __forceinline__ double ld_f64(const unsigned* __restrict__ p, int i)
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
-template <int L, bool DERIV>
+// One (atom, op, reflection) term in working precision R: the def-val contraction, phase,
+// Debye-Waller factor and X = f T exp(i phi).  Y: the 25 harmonics, m: 6 DW multipliers.
+template <int L, typename R>
+struct Term {
+    R A[5];
+    R Er, Ei;      // T exp(i phi)
+    R Xr, Xi;      // f T exp(i phi)
+};
+
+template <int L, typename R>
+__device__ __forceinline__ void contract(const R* P, const R* Y, R* A)
+{
+    constexpr int NLM = (L + 1) * (L + 1);
+    A[0] = P[0] * Y[0];
+    A[1] = P[1] * Y[1] + P[2] * Y[2] + P[3] * Y[3];
+    R a2 = 0;
+#pragma unroll
+    for (int j = 4; j < 9; j++) a2 += P[j] * Y[j];
+    A[2] = a2;
+    A[3] = A[4] = R(0);
+    if (L == 4) {
+        R a3 = 0, a4 = 0;
+#pragma unroll
+        for (int j = 9; j < 16; j++) a3 += P[j < NLM ? j : 0] * Y[j];
+#pragma unroll
+        for (int j = 16; j < 25; j++) a4 += P[j < NLM ? j : 0] * Y[j];
+        A[3] = a3;
+        A[4] = a4;
+    }
+}
+
+template <int L, bool DERIV, bool MP>
 __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane, int k0, int k1,
                                           const unsigned* __restrict__ hop, const double* __restrict__ rad,
-                                          const unsigned* __restrict__ dtdf32, double* __restrict__ fpart,
-                                          double* __restrict__ dpart)
+                                          const double* __restrict__ radx, const unsigned* __restrict__ dtdf32,
+                                          double* __restrict__ fpart, double* __restrict__ dpart)
 {
     constexpr int NLM = (L + 1) * (L + 1);
     const int slot = wave * kWave + lane;
@@ -335,31 +395,35 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
     const double w = at.w, anr = at.anom[0], ani = at.anom[1];
     const int t = at.type;
     double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
+    double B[MP ? NLM : 1];
+#pragma unroll
+    for (int j = 0; j < (MP ? NLM : 1); j++) B[j] = 0;
+    double dpv = 0, dka = 0, dkp = 0;
     for (int k = k0; k < k1; ++k) {
         const double* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
         const double S = rr[0] + anr, G0 = rr[1], G1 = rr[2], G2 = rr[3], G3 = rr[4], G4 = rr[5];
+        double fval = 0, dSk = 0, dG[5] = {0, 0, 0, 0, 0};
+        if (MP) {
+            const double* rx = radx + ((size_t)k * a.n_types + t) * kRadRec;
+            fval = rx[0];
+            dSk = rx[1];
+#pragma unroll
+            for (int l = 0; l < 5; l++) dG[l] = rx[2 + l];
+        }
         const double cr = ld_f64(dtdf32, 2 * k), ci = ld_f64(dtdf32, 2 * k + 1);
-        double sr = 0, si = 0;
+        double sr = 0, si = 0, Rh = 0;
         for (int s = 0; s < a.n_ops; ++s) {
             const unsigned* rw = hop + ((size_t)k * a.n_ops + s) * (2 * kHklopRec);
             double r[kHklopRec];
 #pragma unroll
             for (int j = 0; j < kHklopRec; j++) r[j] = ld_f64(rw, j);
-            double A0 = P[0] * r[0];
-            double A1 = P[1] * r[1] + P[2] * r[2] + P[3] * r[3];
-            double A2 = 0;
-#pragma unroll
-            for (int j = 4; j < 9; j++) A2 += P[j] * r[j];
-            double fr = S + G0 * A0 + G2 * A2;
-            double fi = ani * r[35] + G1 * A1;
+            double A[5];
+            contract<L, double>(P, r, A);
+            double fr = S + G0 * A[0] + G2 * A[2];
+            double fi = ani * r[35] + G1 * A[1];
             if (L == 4) {
-                double A3 = 0, A4 = 0;
-#pragma unroll
-                for (int j = 9; j < 16; j++) A3 += P[j < NLM ? j : 0] * r[j];
-#pragma unroll
-                for (int j = 16; j < 25; j++) A4 += P[j < NLM ? j : 0] * r[j];
-                fr += G4 * A4;
-                fi += G3 * A3;
+                fr += G4 * A[4];
+                fi += G3 * A[3];
             }
             double ph = r[31] * x0 + r[32] * x1 + r[33] * x2 + r[34];
             ph -= rint(ph);
@@ -367,7 +431,8 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
             sincospi(2.0 * ph, &sn, &cs);
             double e = r[25] * u[0] + r[26] * u[1] + r[27] * u[2] + r[28] * u[3] + r[29] * u[4] + r[30] * u[5];
             double T = exp(-e);
-            double Xr = T * (fr * cs - fi * sn), Xi = T * (fr * sn + fi * cs);
+            double Er = T * cs, Ei = T * sn;
+            double Xr = fr * Er - fi * Ei, Xi = fr * Ei + fi * Er;
             sr += Xr;
             si += Xi;
             if (DERIV) {
@@ -377,8 +442,23 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
 #pragma unroll
                 for (int j = 0; j < 3; j++) dx[j] -= r[31 + j] * zi;
             }
+            if (MP) {
+                double cer = cr * Er - ci * Ei, cei = cr * Ei + ci * Er;
+                Rh += cer;
+                double q[5] = {G0 * cer, -G1 * cei, G2 * cer, -G3 * cei, G4 * cer};
+#pragma unroll
+                for (int j = 0; j < NLM; j++) {
+                    const int l = j < 1 ? 0 : j < 4 ? 1 : j < 9 ? 2 : j < 16 ? 3 : 4;
+                    B[j] += q[l] * r[j];
+                }
+                dkp += (dG[0] * A[0] + dG[2] * A[2] + dG[4] * A[4]) * cer - (dG[1] * A[1] + dG[3] * A[3]) * cei;
+            }
         }
         if (DERIV) docc += cr * sr - ci * si;
+        if (MP) {
+            dpv += fval * Rh;
+            dka += dSk * Rh;
+        }
         double fr_w = wave_sum(w * sr), fi_w = wave_sum(w * si);
         if (lane == kWave - 1) {
             double* fp = fpart + ((size_t)wave * a.n_hkl + k) * 2;
@@ -387,21 +467,30 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
         }
     }
     if (DERIV) {
-        double* dp = dpart + ((size_t)blockIdx.y * a.n_slots + slot) * 10;
+        constexpr int ND = MP ? kNdMultipole : kNdBasic;
+        double* dp = dpart + ((size_t)blockIdx.y * a.n_slots + slot) * ND;
         dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
 #pragma unroll
         for (int j = 0; j < 6; j++) dp[3 + j] = du[j];
         dp[9] = docc;
+        if (MP) {
+#pragma unroll
+            for (int j = 0; j < 25; j++) dp[10 + j] = j < NLM ? B[j < NLM ? j : 0] : 0.0;
+            dp[35] = dpv;
+            dp[36] = dka;
+            dp[37] = dkp;
+        }
     }
 }
 
 // Mixed precision: FP64 phase range reduction and FP64 accumulation across reflections, FP32
 // harmonics / radial / Debye-Waller / sincos / exp per term.
-template <int L, bool DERIV>
+template <int L, bool DERIV, bool MP>
 __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane, int k0, int k1,
                                            const float* __restrict__ hop, const float* __restrict__ rad,
-                                           const unsigned* __restrict__ phs32, const unsigned* __restrict__ dtdf32,
-                                           double* __restrict__ fpart, double* __restrict__ dpart)
+                                           const float* __restrict__ radx, const unsigned* __restrict__ phs32,
+                                           const unsigned* __restrict__ dtdf32, double* __restrict__ fpart,
+                                           double* __restrict__ dpart)
 {
     constexpr int NLM = (L + 1) * (L + 1);
     constexpr int kFlush = 16;    // reflections accumulated in FP32 before an FP64 flush
@@ -417,32 +506,40 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
     const float w = (float)at.w, anr = (float)at.anom[0], ani = (float)at.anom[1];
     const int t = at.type;
     double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
+    double B[MP ? NLM : 1];
+#pragma unroll
+    for (int j = 0; j < (MP ? NLM : 1); j++) B[j] = 0;
+    double dpv = 0, dka = 0, dkp = 0;
     for (int kb = k0; kb < k1; kb += kFlush) {
         const int kend = min(kb + kFlush, k1);
         float fx[3] = {0, 0, 0}, fu[6] = {0, 0, 0, 0, 0, 0}, focc = 0;
+        float fB[MP ? NLM : 1];
+#pragma unroll
+        for (int j = 0; j < (MP ? NLM : 1); j++) fB[j] = 0;
+        float fpv = 0, fka = 0, fkp = 0;
         for (int k = kb; k < kend; ++k) {
             const float* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
             const float S = rr[0] + anr, G0 = rr[1], G1 = rr[2], G2 = rr[3], G3 = rr[4], G4 = rr[5];
+            float fval = 0, dSk = 0, dG[5] = {0, 0, 0, 0, 0};
+            if (MP) {
+                const float* rx = radx + ((size_t)k * a.n_types + t) * kRadRec;
+                fval = rx[0];
+                dSk = rx[1];
+#pragma unroll
+                for (int l = 0; l < 5; l++) dG[l] = rx[2 + l];
+            }
             const float cr = (float)ld_f64(dtdf32, 2 * k), ci = (float)ld_f64(dtdf32, 2 * k + 1);
-            float sr = 0, si = 0;
+            float sr = 0, si = 0, Rh = 0;
             for (int s = 0; s < a.n_ops; ++s) {
                 const float* r = hop + ((size_t)k * a.n_ops + s) * kHklopRec;
                 const unsigned* pr = phs32 + ((size_t)k * a.n_ops + s) * 8;
-                float A0 = P[0] * r[0];
-                float A1 = P[1] * r[1] + P[2] * r[2] + P[3] * r[3];
-                float A2 = 0;
-#pragma unroll
-                for (int j = 4; j < 9; j++) A2 += P[j] * r[j];
-                float fr = S + G0 * A0 + G2 * A2;
-                float fi = ani * r[35] + G1 * A1;
+                float A[5];
+                contract<L, float>(P, r, A);
+                float fr = S + G0 * A[0] + G2 * A[2];
+                float fi = ani * r[35] + G1 * A[1];
                 if (L == 4) {
-                    float A3 = 0, A4 = 0;
-#pragma unroll
-                    for (int j = 9; j < 16; j++) A3 += P[j < NLM ? j : 0] * r[j];
-#pragma unroll
-                    for (int j = 16; j < 25; j++) A4 += P[j < NLM ? j : 0] * r[j];
-                    fr += G4 * A4;
-                    fi += G3 * A3;
+                    fr += G4 * A[4];
+                    fi += G3 * A[3];
                 }
                 // FP64 range reduction of h.(R x + t) to [-1/2, 1/2) revolutions
                 double ph = ld_f64(pr, 0) * x0 + ld_f64(pr, 1) * x1 + ld_f64(pr, 2) * x2 + ld_f64(pr, 3);
@@ -450,7 +547,8 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
                 float sn = __builtin_amdgcn_sinf(phf), cs = __builtin_amdgcn_cosf(phf);
                 float e = r[25] * u[0] + r[26] * u[1] + r[27] * u[2] + r[28] * u[3] + r[29] * u[4] + r[30] * u[5];
                 float T = __builtin_amdgcn_exp2f(-e);
-                float Xr = T * (fr * cs - fi * sn), Xi = T * (fr * sn + fi * cs);
+                float Er = T * cs, Ei = T * sn;
+                float Xr = fr * Er - fi * Ei, Xi = fr * Ei + fi * Er;
                 sr += Xr;
                 si += Xi;
                 if (DERIV) {
@@ -460,8 +558,23 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
 #pragma unroll
                     for (int j = 0; j < 3; j++) fx[j] -= r[31 + j] * zi;
                 }
+                if (MP) {
+                    float cer = cr * Er - ci * Ei, cei = cr * Ei + ci * Er;
+                    Rh += cer;
+                    float q[5] = {G0 * cer, -G1 * cei, G2 * cer, -G3 * cei, G4 * cer};
+#pragma unroll
+                    for (int j = 0; j < NLM; j++) {
+                        const int l = j < 1 ? 0 : j < 4 ? 1 : j < 9 ? 2 : j < 16 ? 3 : 4;
+                        fB[j] += q[l] * r[j];
+                    }
+                    fkp += (dG[0] * A[0] + dG[2] * A[2] + dG[4] * A[4]) * cer - (dG[1] * A[1] + dG[3] * A[3]) * cei;
+                }
             }
             if (DERIV) focc += cr * sr - ci * si;
+            if (MP) {
+                fpv += fval * Rh;
+                fka += dSk * Rh;
+            }
             float fr_w = wave_sum(w * sr), fi_w = wave_sum(w * si);
             if (lane == kWave - 1) {
                 double* fp = fpart + ((size_t)wave * a.n_hkl + k) * 2;
@@ -476,19 +589,34 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
             for (int j = 0; j < 6; j++) du[j] += fu[j];
             docc += focc;
         }
+        if (MP) {
+#pragma unroll
+            for (int j = 0; j < NLM; j++) B[j] += fB[j];
+            dpv += fpv;
+            dka += fka;
+            dkp += fkp;
+        }
     }
     if (DERIV) {
         // U multipliers carried log2(e); undo for the derivative
         const double inv_log2e = 0.69314718055994530942;
-        double* dp = dpart + ((size_t)blockIdx.y * a.n_slots + slot) * 10;
+        constexpr int ND = MP ? kNdMultipole : kNdBasic;
+        double* dp = dpart + ((size_t)blockIdx.y * a.n_slots + slot) * ND;
         dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
 #pragma unroll
         for (int j = 0; j < 6; j++) dp[3 + j] = du[j] * inv_log2e;
         dp[9] = docc;
+        if (MP) {
+#pragma unroll
+            for (int j = 0; j < 25; j++) dp[10 + j] = j < NLM ? B[j < NLM ? j : 0] : 0.0;
+            dp[35] = dpv;
+            dp[36] = dka;
+            dp[37] = dkp;
+        }
     }
 }
 
-template <bool MIXED, bool DERIV>
+template <bool MIXED, bool DERIV, bool MP>
 __global__ __launch_bounds__(256) void k_main(MainArgs a)
 {
     const int wave = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x / kWave);
@@ -501,14 +629,16 @@ __global__ __launch_bounds__(256) void k_main(MainArgs a)
     if (MIXED) {
         const float* hop = static_cast<const float*>(a.hklop);
         const float* rad = static_cast<const float*>(a.radial);
+        const float* radx = static_cast<const float*>(a.radx);
         const unsigned* phs = reinterpret_cast<const unsigned*>(a.phase);
-        if (cls <= 2) body_mixed<2, DERIV>(a, wave, lane, k0, k1, hop, rad, phs, dtdf32, a.fpart, a.dpart);
-        else body_mixed<4, DERIV>(a, wave, lane, k0, k1, hop, rad, phs, dtdf32, a.fpart, a.dpart);
+        if (cls <= 2) body_mixed<2, DERIV, MP>(a, wave, lane, k0, k1, hop, rad, radx, phs, dtdf32, a.fpart, a.dpart);
+        else body_mixed<4, DERIV, MP>(a, wave, lane, k0, k1, hop, rad, radx, phs, dtdf32, a.fpart, a.dpart);
     } else {
         const unsigned* hop = static_cast<const unsigned*>(a.hklop);
         const double* rad = static_cast<const double*>(a.radial);
-        if (cls <= 2) body_fp64<2, DERIV>(a, wave, lane, k0, k1, hop, rad, dtdf32, a.fpart, a.dpart);
-        else body_fp64<4, DERIV>(a, wave, lane, k0, k1, hop, rad, dtdf32, a.fpart, a.dpart);
+        const double* radx = static_cast<const double*>(a.radx);
+        if (cls <= 2) body_fp64<2, DERIV, MP>(a, wave, lane, k0, k1, hop, rad, radx, dtdf32, a.fpart, a.dpart);
+        else body_fp64<4, DERIV, MP>(a, wave, lane, k0, k1, hop, rad, radx, dtdf32, a.fpart, a.dpart);
     }
 }
 
@@ -531,17 +661,56 @@ __global__ void k_reduce_f(const double* __restrict__ fpart, const double* __res
     f[2 * k + 1] = im * mult[k];
 }
 
+// d[orig][j] = sum over reflection chunks of the per-(chunk, slot) partials; one thread per
+// (slot, j), fixed chunk order.
 __global__ void k_reduce_d(const double* __restrict__ dpart, const AtomRec* __restrict__ atoms, int n_chunks,
-                           int n_slots, double* __restrict__ d)
+                           int n_slots, int nd, double* __restrict__ d)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_slots) return;
-    int orig = atoms[i].orig;
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_slots * nd) return;
+    const int i = idx / nd, j = idx - i * nd;
+    const int orig = atoms[i].orig;
     if (orig < 0) return;
-    double s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < n_chunks; c++)
-        for (int j = 0; j < 10; j++) s[j] += dpart[((size_t)c * n_slots + i) * 10 + j];
-    for (int j = 0; j < 10; j++) d[(size_t)orig * 10 + j] = s[j];
+    double s = 0.0;
+    for (int c = 0; c < n_chunks; c++) s += dpart[((size_t)c * n_slots + i) * nd + j];
+    d[(size_t)orig * nd + j] = s;
+}
+
+// dT/dP_lm in the atom's local frame from the global-frame derivative: P_glob = D P_loc with
+// D_ij = sum_p w_p d_i(u_p) d_j(L^T u_p) / int d_i^2, hence dT/dP_loc = D^T dT/dP_glob.
+__global__ __launch_bounds__(64) void k_plm_back(const AtomIn* __restrict__ in, int n_slots, int nd,
+                                                 double* __restrict__ d)
+{
+    const int i = blockIdx.x;
+    const int p = threadIdx.x;
+    __shared__ double v[25];
+    __shared__ double sp[64];
+    const AtomIn& a = in[i];
+    if (a.orig < 0) return;               // whole block exits together
+    double* row = d + (size_t)a.orig * nd + 10;
+    if (p < 25) v[p] = row[p];
+    __syncthreads();
+    double s = 0.0;
+    if (p < 50)
+        for (int q = 0; q < 25; q++) s += c_proj_w[p][q] * v[q];
+    sp[p] = s;
+    __syncthreads();
+    if (p < 25) {
+        double acc = 0.0;
+        for (int q = 0; q < 50; q++) {
+            const double u0 = c_proj_dir[q][0], u1 = c_proj_dir[q][1], u2 = c_proj_dir[q][2];
+            const double x = a.lcs[0] * u0 + a.lcs[3] * u1 + a.lcs[6] * u2;
+            const double y = a.lcs[1] * u0 + a.lcs[4] * u1 + a.lcs[7] * u2;
+            const double z = a.lcs[2] * u0 + a.lcs[5] * u1 + a.lcs[8] * u2;
+            double dl[25];
+            ylm_density<double>(x, y, z, dl);
+            double dj = 0.0;
+#pragma unroll
+            for (int j = 0; j < 25; j++) dj = (j == p) ? dl[j] : dj;
+            acc += dj * sp[q];
+        }
+        row[p] = acc;
+    }
 }
 
 // ------------------------------------------------------------------ host side
@@ -618,6 +787,56 @@ static void upload_constants(int device)
     if (device >= 0 && device < 64) g_constants_ready[device] = true;
 }
 
+static std::vector<TypeRec> type_records(const EngineModel& model)
+{
+    std::vector<TypeRec> tr(model.n_types);
+    for (int t = 0; t < model.n_types; t++) {
+        TypeRec& r = tr[t];
+        std::memset(&r, 0, sizeof(r));
+        r.wfn = model.type_wfn[t];
+        r.nl = model.dv_nl[r.wfn];
+        r.max_l = model.type_max_l[t];
+        r.lp = std::min(std::min(4, r.nl - 1), model.types[t].n_l - 1);
+        if (r.lp < r.max_l) r.lp = r.max_l;
+        for (int l = 0; l < 5; l++) r.n[l] = model.dv_n[5 * r.wfn + l];
+        r.pval = model.types[t].p_val;
+        r.kappa = model.types[t].kappa;
+        r.kappa_p = model.types[t].kappa_prime;
+        r.zeta = model.dv_zeta[r.wfn];
+        for (int l = 0; l <= r.max_l; l++)
+            for (int i = l * l; i < (l + 1) * (l + 1); i++) r.plm[i] = model.types[t].plm[i];
+    }
+    return tr;
+}
+
+int Engine::type_class(int t) const
+{
+    const int lp = std::max(model_.type_max_l[t],
+                            std::min(std::min(4, model_.dv_nl[model_.type_wfn[t]] - 1), model_.types[t].n_l - 1));
+    return lp <= 2 ? 2 : 4;
+}
+
+void Engine::set_types(const std::vector<AtomType>& types)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    if ((int)types.size() != model_.n_types) throw Error(1, "number of atom types differs from the model");
+    for (const AtomType& t : types)
+        if (!(t.kappa > 0) || !(t.kappa_prime > 0)) throw Error(2, "kappa must be positive");
+    model_.types = types;
+    for (int t = 0; t < model_.n_types; t++) {
+        int lp = -1;
+        for (int l = 0; l < types[t].n_l && l < 5; l++)
+            for (int i = l * l; i < (l + 1) * (l + 1); i++)
+                if (types[t].plm[i] != 0.0) lp = l;
+        int lw = std::min(4, model_.dv_nl[model_.type_wfn[t]] - 1);
+        model_.type_max_l[t] = std::min(lw, lp);
+    }
+    std::vector<TypeRec> tr = type_records(model_);
+    if (!tr.empty())
+        HIP_CHECK(hipMemcpyAsync(d_type_.p, tr.data(), tr.size() * sizeof(TypeRec), hipMemcpyHostToDevice, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
 Engine::Engine(int device, const EngineModel& model, const double* cell6, const std::vector<EngineOp>& ops,
                bool centric_origin)
     : device_(device), model_(model), ops_(ops), centric_origin_(centric_origin)
@@ -647,21 +866,7 @@ Engine::Engine(int device, const EngineModel& model, const double* cell6, const 
         for (size_t i = 0; i < model.valence[w].c.size(); i++)
             val.push_back({model.valence[w].c[i], model.valence[w].e[i], model.valence[w].p[i], 0});
     }
-    std::vector<TypeRec> tr(model.n_types);
-    for (int t = 0; t < model.n_types; t++) {
-        TypeRec& r = tr[t];
-        std::memset(&r, 0, sizeof(r));
-        r.wfn = model.type_wfn[t];
-        r.nl = model.dv_nl[r.wfn];
-        r.max_l = model.type_max_l[t];
-        for (int l = 0; l < 5; l++) r.n[l] = model.dv_n[5 * r.wfn + l];
-        r.pval = model.types[t].p_val;
-        r.kappa = model.types[t].kappa;
-        r.kappa_p = model.types[t].kappa_prime;
-        r.zeta = model.dv_zeta[r.wfn];
-        for (int l = 0; l <= r.max_l; l++)
-            for (int i = l * l; i < (l + 1) * (l + 1); i++) r.plm[i] = model.types[t].plm[i];
-    }
+    std::vector<TypeRec> tr = type_records(model);
     size_t bytes_ct = std::max<size_t>(1, core.size()) * sizeof(TermRec);
     size_t bytes_vt = std::max<size_t>(1, val.size()) * sizeof(TermRec);
     d_core_.ensure(bytes_ct);
@@ -707,7 +912,7 @@ void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const doub
     std::vector<int> idx;
     for (int i = 0; i < at.n; i++)
         if (at.include[i]) idx.push_back(i);
-    auto cls_of = [&](int i) { return model_.type_max_l[at.type[i]] <= 2 ? 2 : 4; };
+    auto cls_of = [&](int i) { return type_class(at.type[i]); };
     std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
         int cx = cls_of(x), cy = cls_of(y);
         if (cx != cy) return cx < cy;
@@ -775,7 +980,7 @@ void Engine::fill(DeviceBuffer& b, size_t n, double v)
     HIP_CHECK(hipGetLastError());
 }
 
-void Engine::run_prep(Precision prec)
+void Engine::run_prep(Precision prec, bool mp)
 {
     const int n_ops = (int)ops_.size();
     const int n_types = std::max(model_.n_types, 1);
@@ -806,9 +1011,12 @@ void Engine::run_prep(Precision prec)
                            d_core_.as<TermRec>(), model_.n_wfn, n_hkl, d_fcore_.as<double>());
         HIP_CHECK(hipGetLastError());
         int nt = n_hkl * model_.n_types;
+        if (mp) d_radx_.ensure((size_t)n_hkl * n_types * kRadRec * (mixed ? sizeof(float) : sizeof(double)));
         hipLaunchKernelGGL(k_radial_prep, dim3((nt + T - 1) / T), dim3(T), 0, stream_, hlen.as<double>(),
                            d_fcore_.as<double>(), types, wfns, d_val_.as<TermRec>(), model_.n_types, n_hkl,
-                           mixed ? nullptr : d_radial_.as<double>(), mixed ? d_radial_.as<float>() : nullptr);
+                           mixed ? nullptr : d_radial_.as<double>(), mixed ? d_radial_.as<float>() : nullptr,
+                           (mp && !mixed) ? d_radx_.as<double>() : nullptr,
+                           (mp && mixed) ? d_radx_.as<float>() : nullptr);
         HIP_CHECK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_atom_prep, dim3(n_slots), dim3(64), 0, stream_, ain, types, d_atoms_.as<AtomRec>(),
@@ -816,34 +1024,38 @@ void Engine::run_prep(Precision prec)
     HIP_CHECK(hipGetLastError());
 }
 
-static const void* main_kernel(bool mixed, bool derivs)
+static const void* main_kernel(bool mixed, bool derivs, bool mp)
 {
-    if (mixed) return derivs ? (const void*)k_main<true, true> : (const void*)k_main<true, false>;
-    return derivs ? (const void*)k_main<false, true> : (const void*)k_main<false, false>;
+    if (mixed) {
+        if (mp) return (const void*)k_main<true, true, true>;
+        return derivs ? (const void*)k_main<true, true, false> : (const void*)k_main<true, false, false>;
+    }
+    if (mp) return (const void*)k_main<false, true, true>;
+    return derivs ? (const void*)k_main<false, true, false> : (const void*)k_main<false, false, false>;
 }
 
-int Engine::resident_waves(bool mixed, bool derivs)
+int Engine::resident_waves(bool mixed, bool derivs, bool mp)
 {
-    int key = (mixed ? 2 : 0) + (derivs ? 1 : 0);
+    int key = (mixed ? 4 : 0) + (derivs ? 2 : 0) + (mp ? 1 : 0);
     if (resident_waves_[key] > 0) return resident_waves_[key];
     int blocks = 0;
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, main_kernel(mixed, derivs), 256, 0));
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, main_kernel(mixed, derivs, mp), 256, 0));
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, device_));
     resident_waves_[key] = std::max(1, blocks) * kWavesPerBlock * prop.multiProcessorCount;
     return resident_waves_[key];
 }
 
-int Engine::default_chunk(bool mixed, bool derivs) 
+int Engine::default_chunk(bool mixed, bool derivs, bool mp)
 {
     // about four resident-capacity rounds of (atom wave, reflection chunk) work units
     const int n_waves = n_sorted_ / kWave;
-    const double target = 4.0 * resident_waves(mixed, derivs);
+    const double target = 4.0 * resident_waves(mixed, derivs, mp);
     int n_chunks = std::max(1, (int)std::lround(target / std::max(n_waves, 1)));
     return std::max(8, (n_hkl_ + n_chunks - 1) / n_chunks);
 }
 
-void Engine::launch_main(Precision prec, bool derivs, int chunk)
+void Engine::launch_main(Precision prec, bool derivs, bool mp, int chunk)
 {
     const bool mixed = prec == kMixed;
     const int n_slots = n_sorted_;
@@ -851,11 +1063,13 @@ void Engine::launch_main(Precision prec, bool derivs, int chunk)
     const int n_hkl = n_hkl_;
     const int n_chunks = (n_hkl + chunk - 1) / chunk;
     d_fpart_.ensure((size_t)n_waves * n_hkl * 2 * sizeof(double));
-    d_dpart_.ensure((size_t)n_chunks * n_slots * 10 * sizeof(double));
+    const int nd = mp ? kNdMultipole : kNdBasic;
+    d_dpart_.ensure((size_t)n_chunks * n_slots * nd * sizeof(double));
     MainArgs a;
     a.hklop = d_hklop_.p;
     a.phase = mixed ? d_phase_.as<double>() : nullptr;
     a.radial = d_radial_.p;
+    a.radx = mp ? d_radx_.p : nullptr;
     a.dtdf = d_dtdf_.as<double>();    // [re, im] pairs followed by the centering multipliers
     a.atoms = d_atoms_.as<AtomRec>();
     a.wave_l = reinterpret_cast<const int*>(static_cast<const char*>(d_atoms_in_.p) +
@@ -873,49 +1087,53 @@ void Engine::launch_main(Precision prec, bool derivs, int chunk)
     last_grid_y = grid.y;
     last_chunk = chunk;
     if (mixed) {
-        if (derivs) hipLaunchKernelGGL((k_main<true, true>), grid, dim3(256), 0, stream_, a);
-        else hipLaunchKernelGGL((k_main<true, false>), grid, dim3(256), 0, stream_, a);
+        if (mp) hipLaunchKernelGGL((k_main<true, true, true>), grid, dim3(256), 0, stream_, a);
+        else if (derivs) hipLaunchKernelGGL((k_main<true, true, false>), grid, dim3(256), 0, stream_, a);
+        else hipLaunchKernelGGL((k_main<true, false, false>), grid, dim3(256), 0, stream_, a);
     } else {
-        if (derivs) hipLaunchKernelGGL((k_main<false, true>), grid, dim3(256), 0, stream_, a);
-        else hipLaunchKernelGGL((k_main<false, false>), grid, dim3(256), 0, stream_, a);
+        if (mp) hipLaunchKernelGGL((k_main<false, true, true>), grid, dim3(256), 0, stream_, a);
+        else if (derivs) hipLaunchKernelGGL((k_main<false, true, false>), grid, dim3(256), 0, stream_, a);
+        else hipLaunchKernelGGL((k_main<false, false, false>), grid, dim3(256), 0, stream_, a);
     }
     HIP_CHECK(hipGetLastError());
 }
 
-int Engine::chunk_for(Precision prec, bool derivs)
+int Engine::chunk_for(Precision prec, bool derivs, bool mp)
 {
     if (tune.hkl_chunk > 0) return tune.hkl_chunk;
-    auto it = tuned_.find(shape_key(prec, derivs));
+    auto it = tuned_.find(shape_key(prec, derivs, mp));
     if (it != tuned_.end()) return it->second;
-    return default_chunk(prec == kMixed, derivs);
+    return default_chunk(prec == kMixed, derivs, mp);
 }
 
-long long Engine::shape_key(Precision prec, bool derivs) const
+long long Engine::shape_key(Precision prec, bool derivs, bool mp) const
 {
-    return (((long long)n_sorted_ * 1000003LL + n_hkl_) * 64 + (long long)ops_.size()) * 4 +
-           (prec == kMixed ? 2 : 0) + (derivs ? 1 : 0);
+    return (((long long)n_sorted_ * 1000003LL + n_hkl_) * 64 + (long long)ops_.size()) * 8 +
+           (prec == kMixed ? 4 : 0) + (derivs ? 2 : 0) + (mp ? 1 : 0);
 }
 
-void Engine::run(Precision prec, bool derivs)
+void Engine::run(Precision prec, bool derivs, bool mp)
 {
     HIP_CHECK(hipSetDevice(device_));
     ran_ = true;
+    if (mp) derivs = true;
+    last_nd_ = mp ? kNdMultipole : kNdBasic;
     const int n_hkl = n_hkl_;
     const int n_slots = n_sorted_;
     const int n_waves = n_slots / kWave;
     d_f_.ensure(std::max(n_hkl, 1) * 2 * sizeof(double));
-    d_d_.ensure(std::max(n_atoms_, 1) * 10 * sizeof(double));
-    fill(d_d_, (size_t)std::max(n_atoms_, 1) * 10, 0.0);
+    d_d_.ensure((size_t)std::max(n_atoms_, 1) * last_nd_ * sizeof(double));
+    fill(d_d_, (size_t)std::max(n_atoms_, 1) * last_nd_, 0.0);
     if (n_hkl == 0) return;
     if (n_waves == 0) {
         fill(d_f_, (size_t)n_hkl * 2, 0.0);
         return;
     }
-    run_prep(prec);
-    const int chunk = chunk_for(prec, derivs);
+    run_prep(prec, mp);
+    const int chunk = chunk_for(prec, derivs, mp);
     const int n_chunks = (n_hkl + chunk - 1) / chunk;
     if (timing) HIP_CHECK(hipEventRecord(ev0_, stream_));
-    launch_main(prec, derivs, chunk);
+    launch_main(prec, derivs, mp, chunk);
     if (timing) HIP_CHECK(hipEventRecord(ev1_, stream_));
     const int T = 256;
     hipLaunchKernelGGL(k_reduce_f, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, d_fpart_.as<double>(),
@@ -923,9 +1141,15 @@ void Engine::run(Precision prec, bool derivs)
                        d_f_.as<double>());
     HIP_CHECK(hipGetLastError());
     if (derivs) {
-        hipLaunchKernelGGL(k_reduce_d, dim3((n_slots + T - 1) / T), dim3(T), 0, stream_, d_dpart_.as<double>(),
-                           d_atoms_.as<AtomRec>(), n_chunks, n_slots, d_d_.as<double>());
+        const int nd = last_nd_;
+        hipLaunchKernelGGL(k_reduce_d, dim3((n_slots * nd + T - 1) / T), dim3(T), 0, stream_, d_dpart_.as<double>(),
+                           d_atoms_.as<AtomRec>(), n_chunks, n_slots, nd, d_d_.as<double>());
         HIP_CHECK(hipGetLastError());
+        if (mp) {
+            hipLaunchKernelGGL(k_plm_back, dim3(n_slots), dim3(64), 0, stream_, d_atoms_in_.as<AtomIn>(), n_slots, nd,
+                               d_d_.as<double>());
+            HIP_CHECK(hipGetLastError());
+        }
     }
     if (timing) {
         HIP_CHECK(hipEventSynchronize(ev1_));
@@ -933,13 +1157,14 @@ void Engine::run(Precision prec, bool derivs)
     }
 }
 
-int Engine::autotune(Precision prec, bool derivs)
+int Engine::autotune(Precision prec, bool derivs, bool mp)
 {
     HIP_CHECK(hipSetDevice(device_));
+    if (mp) derivs = true;
     const int n_waves = n_sorted_ / kWave;
     if (n_hkl_ == 0 || n_waves == 0) return 0;
-    run_prep(prec);
-    const int cap = resident_waves(prec == kMixed, derivs);
+    run_prep(prec, mp);
+    const int cap = resident_waves(prec == kMixed, derivs, mp);
     std::vector<int> cands;
     for (double rounds : {1.0, 2.0, 3.0, 4.0, 6.0, 8.0, 12.0, 16.0, 24.0}) {
         int n_chunks = std::max(1, (int)std::lround(rounds * cap / n_waves));
@@ -952,7 +1177,7 @@ int Engine::autotune(Precision prec, bool derivs)
         float ms_min = 1e30f;
         for (int rep = 0; rep < 2; rep++) {
             HIP_CHECK(hipEventRecord(ev0_, stream_));
-            launch_main(prec, derivs, c);
+            launch_main(prec, derivs, mp, c);
             HIP_CHECK(hipEventRecord(ev1_, stream_));
             HIP_CHECK(hipEventSynchronize(ev1_));
             float ms = 0;
@@ -964,7 +1189,7 @@ int Engine::autotune(Precision prec, bool derivs)
             best = c;
         }
     }
-    tuned_[shape_key(prec, derivs)] = best;
+    tuned_[shape_key(prec, derivs, mp)] = best;
     return best;
 }
 
@@ -975,7 +1200,8 @@ void Engine::download(double* f, double* d10)
     if (f && n_hkl_ > 0)
         HIP_CHECK(hipMemcpyAsync(f, d_f_.p, (size_t)n_hkl_ * 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
     if (d10 && n_atoms_ > 0)
-        HIP_CHECK(hipMemcpyAsync(d10, d_d_.p, (size_t)n_atoms_ * 10 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        HIP_CHECK(hipMemcpyAsync(d10, d_d_.p, (size_t)n_atoms_ * last_nd_ * sizeof(double), hipMemcpyDeviceToHost,
+                                 stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
 }
 

@@ -53,15 +53,22 @@ public:
     ~Engine();
     // Upload one call's inputs; hkl as int triples, dtdf already conj * centering multiplier.
     void upload(const EngineAtoms& atoms, int n_hkl, const int* hkl, const double* dtdf);
-    // Run all kernels on the uploaded inputs (asynchronous on the engine stream).
-    void run(Precision prec, bool derivs);
-    // Copy F (2 n_hkl) and per-atom raw derivatives (10 per atom: dxyz_frac[3] (x 2 pi w),
-    // dU_cart[6] (x 2 pi^2 w), docc (x mult/|G|)) back, in caller atom order; synchronises.
-    void download(double* f, double* d10);
+    // Run all kernels on the uploaded inputs (asynchronous on the engine stream).  mp adds the
+    // multipole-parameter adjoint (implies derivs).
+    void run(Precision prec, bool derivs, bool mp = false);
+    // Copy F (2 n_hkl) and per-atom raw derivatives back, in caller atom order; synchronises.
+    // Per atom nd() values: dxyz_frac[3] (x 2 pi w), dU_cart[6] (x 2 pi^2 w), docc (x mult/|G|),
+    // and with the multipole adjoint dP_lm local frame [25], dPval, dkappa, dkappa' (all x w).
+    void download(double* f, double* d);
+    int nd() const { return last_nd_; }
+    // Replace the atom-type parameters (P_lm, Pval, kappa, kappa'); the multipole truncation
+    // max L is recomputed as the reference does on every call.
+    void set_types(const std::vector<AtomType>& types);
+    const EngineModel& model() const { return model_; }
     void synchronize();
     // Time the fused kernel over candidate reflection-chunk sizes on the uploaded inputs and keep
     // the fastest for this (atoms, reflections, ops, precision) shape; returns the chunk chosen.
-    int autotune(Precision prec, bool derivs);
+    int autotune(Precision prec, bool derivs, bool mp = false);
     hipStream_t stream() const { return stream_; }
     TuneParams tune;
     // last run: launch geometry of the main kernel
@@ -71,14 +78,16 @@ public:
     int n_ops() const { return (int)ops_.size(); }
 
 private:
-    void run_prep(Precision prec);
-    void launch_main(Precision prec, bool derivs, int chunk);
+    void run_prep(Precision prec, bool mp);
+    void launch_main(Precision prec, bool derivs, bool mp, int chunk);
     void fill(DeviceBuffer& b, size_t n, double v);
-    int resident_waves(bool mixed, bool derivs);
-    int default_chunk(bool mixed, bool derivs);
-    int chunk_for(Precision prec, bool derivs);
-    long long shape_key(Precision prec, bool derivs) const;
-    int resident_waves_[4] = {0, 0, 0, 0};
+    int resident_waves(bool mixed, bool derivs, bool mp);
+    int default_chunk(bool mixed, bool derivs, bool mp);
+    int chunk_for(Precision prec, bool derivs, bool mp);
+    long long shape_key(Precision prec, bool derivs, bool mp) const;
+    int type_class(int t) const;
+    int resident_waves_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int last_nd_ = 10;
     std::map<long long, int> tuned_;
     int device_;
     hipStream_t stream_ = nullptr;
@@ -94,7 +103,7 @@ private:
     DeviceBuffer d_core_, d_val_, d_type_, d_ops_;
     // per call
     DeviceBuffer d_hkl_, d_dtdf_, d_atoms_in_, d_atoms_, d_hklop_, d_hklop32_, d_phase_, d_radial_,
-        d_radial32_, d_fpart_, d_dpart_, d_f_, d_d_, d_order_, d_fcore_;
+        d_radial32_, d_fpart_, d_dpart_, d_f_, d_d_, d_order_, d_fcore_, d_radx_;
 };
 
 }  // namespace discamb_amd

@@ -138,9 +138,21 @@ class DerivativesSelector:
     d_adp: bool = True
     d_occ: bool = True
     d_anom: bool = False
+    d_multipole: bool = False      # P_lm, Pval, kappa, kappa' of the atom types (no reference counterpart)
 
     def bits(self):
-        return (1 if self.d_xyz else 0) | (2 if self.d_adp else 0) | (4 if self.d_occ else 0)
+        return ((1 if self.d_xyz else 0) | (2 if self.d_adp else 0) | (4 if self.d_occ else 0) |
+                (8 if self.d_multipole else 0))
+
+
+@dataclass
+class TypeParameterDerivatives:
+    """dT/d(atom-type parameter), summed over the atoms of each type: p_lm (n_types, 25) in the
+    l*l + l + m layout (local frame, density normalised), p_val, kappa, kappa_prime (n_types,)."""
+    p_lm: np.ndarray
+    p_val: np.ndarray
+    kappa: np.ndarray
+    kappa_prime: np.ndarray
 
 
 @dataclass

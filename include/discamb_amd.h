@@ -37,6 +37,9 @@ extern "C" {
 #define DH_D_ADP 2u
 #define DH_D_OCC 4u
 #define DH_D_ALL 7u
+/* multipole-model parameters of the atom types: P_lm (local frame), Pval, kappa, kappa'.  No
+ * counterpart in the reference, whose derivatives stop at xyz / ADP / occupancy. */
+#define DH_D_MULTIPOLE 8u
 
 /* xyz / ADP conventions (structural_parameters_convention) */
 #define DH_XYZ_FRACTIONAL 0
@@ -127,6 +130,14 @@ int dh_hc_upload(dh_calculator* calc, const dh_atoms* atoms, int n_hkl, const in
                  const double* dtarget_df);
 int dh_hc_run(dh_calculator* calc, unsigned selector);
 int dh_hc_fetch(dh_calculator* calc, double* f, double* d_xyz, double* d_adp, double* d_occ);
+/* Replace the atom-type parameters (HC_AtomTypeParameters: p_lm 25 per type in the l*l + l + m
+ * layout, p_val, kappa, kappa') between calls; the model's other data are kept. */
+int dh_hc_set_type_parameters(dh_calculator* calc, int n_types, const double* plm, const double* p_val,
+                              const double* kappa, const double* kappa_prime);
+/* After a run with DH_D_MULTIPOLE and dh_hc_fetch: dT/d(type parameter), summed over the atoms of
+ * each type; d_plm is 25 per type (entries beyond the type's p_lm size are 0). */
+int dh_hc_fetch_type_derivatives(dh_calculator* calc, double* d_plm, double* d_p_val, double* d_kappa,
+                                 double* d_kappa_prime);
 /* Autotuner: on the uploaded inputs, time the fused kernel for candidate launch shapes (reflection
  * chunk per workgroup column, derived from the device's resident-wave capacity) in the current
  * calculation mode and keep the fastest for this workload shape.  Returns the chosen chunk. */

@@ -17,7 +17,7 @@ DRIVER = os.path.join(ROOT, "tests", "cpp", "hc_cpp_driver")
 
 def write_input(w, path, include):
     tok = []
-    tok += list(w.crystal.cell)
+    tok += [repr(float(v)) for v in w.crystal.cell]
     tok += [len(w.crystal.symops)] + [s.replace(" ", "") for s in w.crystal.symops]
     tok += [len(w.params.wfn_parameters)]
     for wf in w.params.wfn_parameters:
@@ -29,7 +29,7 @@ def write_input(w, path, include):
         tok += list(wf.orbital_occupancy)
         tok += [len(wf.core_orbitals_indices)] + list(wf.core_orbitals_indices)
         tok += [len(wf.valence_orbitals_indices)] + list(wf.valence_orbitals_indices)
-        tok += [repr(wf.deformation_valence_exponent), len(wf.deformation_valence_power)]
+        tok += [repr(float(wf.deformation_valence_exponent)), len(wf.deformation_valence_power)]
         tok += list(wf.deformation_valence_power)
         tok += [0.0, 0.0]
     tok += [len(w.params.type_parameters)]
@@ -37,7 +37,7 @@ def write_input(w, path, include):
         tok += [len(t.p_lm)]
         for row in t.p_lm:
             tok += [repr(float(v)) for v in row]
-        tok += [repr(t.p_val), repr(t.kappa_spherical_valence), repr(t.kappa_deformation_valence)]
+        tok += [repr(float(t.p_val)), repr(float(t.kappa_spherical_valence)), repr(float(t.kappa_deformation_valence))]
     a = w.crystal.atoms
     tok += [a.n]
     for i in range(a.n):
@@ -49,7 +49,7 @@ def write_input(w, path, include):
         tok += [1 if include[i] else 0]
     tok += [len(w.hkl)]
     for h, d in zip(w.hkl, w.dtdf):
-        tok += [int(h[0]), int(h[1]), int(h[2]), repr(d.real), repr(d.imag)]
+        tok += [int(h[0]), int(h[1]), int(h[2]), repr(float(d.real)), repr(float(d.imag))]
     with open(path, "w") as f:
         f.write(" ".join(str(t) for t in tok))
 

@@ -83,6 +83,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0, help="fixed reflection chunk (disables the autotuner)")
+    ap.add_argument("--chunk-fp64", type=int, default=0, help="fixed reflection chunk for the FP64 path only")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain read+write elementwise kernel over MIB MiB (PMC byte calibration)")
     args = ap.parse_args()
@@ -104,17 +105,18 @@ def main():
 
     w = synth.config1()
     calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params, device=0)
-    if args.chunk:
-        calc.setTuning(args.chunk)
     calc.upload(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
     terms = w.terms
     flops = algorithmic_flops(w)
     print("workload %s: %d atoms x %d ops x %d hkl = %.3e terms" % (
         w.name, w.crystal.atoms.n, w.n_explicit_ops, w.hkl.shape[0], terms), file=sys.stderr, flush=True)
-    tune = args.chunk == 0
-    s64, k64, info64, f64, d64 = time_mode(calc, "HIP", args.steps, args.warmup, tune)
+    chunk64 = args.chunk_fp64 or args.chunk
+    calc.setTuning(chunk64)            # 0 = autotuned
+    s64, k64, info64, f64, d64 = time_mode(calc, "HIP", args.steps, args.warmup, chunk64 == 0)
+    calc.setTuning(args.chunk)
     print("fp64: %.3f ms/step, main kernel %.3f ms, grid %s" % (s64 * 1e3, k64, info64["grid"]), file=sys.stderr,
           flush=True)
+    tune = args.chunk == 0
     smx, kmx, infomx, fmx, dmx = time_mode(calc, "HIP_MIXED", args.steps, args.warmup, tune)
     print("mixed: %.3f ms/step, main kernel %.3f ms, grid %s" % (smx * 1e3, kmx, infomx["grid"]),
           file=sys.stderr, flush=True)
@@ -157,7 +159,7 @@ def main():
         "speedup_mixed_vs_fp64": s64 / smx,
         "accuracy_mixed_vs_fp64": acc,
         "launch": {"grid": infomx["grid"], "chunk": infomx["chunk"], "fp64_chunk": info64["chunk"],
-                   "autotuned": tune},
+                   "autotuned": tune, "fp64_autotuned": chunk64 == 0},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w)

@@ -35,6 +35,10 @@ static constexpr int kHklopRec = 36;    // doubles / floats per (reflection, op)
 static constexpr int kRadRec = 8;       // per (reflection, type) radial record
 static constexpr int kWave = 64;
 static constexpr int kWavesPerBlock = 4;
+// Workgroups are dealt to the 8 XCDs of an MI355X round-robin by workgroup id; each XCD has its
+// own L2.  The main kernel maps ids so that all atom blocks of one reflection chunk share an XCD.
+static constexpr int kXcds = 8;
+
 
 struct TypeRec {
     int wfn, nl, max_l, lp;     // lp = min(4, n_l - 1, size(p_lm) - 1): l range of the multipole adjoint
@@ -294,6 +298,9 @@ struct MainArgs {
     double* fpart;            // [wave][k][2]
     double* dpart;            // [chunk][slot][nd]
     int n_hkl, n_ops, n_types, n_slots, n_waves, chunk;
+    int grid_x;               // atom blocks per reflection chunk
+    int n_blocks;             // grid_x * number of chunks
+    int per_xcd;              // ceil(n_blocks / kXcds)
 };
 
 // per-atom partial layout: 0-2 d/dx_frac, 3-8 d/dU, 9 d/docc, then with the multipole adjoint
@@ -301,40 +308,108 @@ struct MainArgs {
 static constexpr int kNdBasic = 10;
 static constexpr int kNdMultipole = 38;
 
-// Full-wave sums with DPP lane permutations (quad_perm xor 1, xor 2, row_half_mirror, row_mirror,
-// row_bcast:15, row_bcast:31); the total ends in lane 63.
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ int dpp_mov(int v)
+// F staging.  Each lane deposits its weighted contribution to reflection kk of the current block
+// of kStage; the flush sums the 64 atoms of every staged reflection in a fixed order
+// (deterministic) and writes one FP64 partial per (wave, reflection).  In the flush lane
+// kk*8 + part sums atoms part*8 .. part*8+7 of reflection kk from LDS, then the eight parts of a
+// reflection (eight adjacent lanes) are combined with DPP.  Each group of eight atoms is padded so
+// that the 128-bit column reads of the flush fall on distinct LDS banks.  LDS accesses of one wave
+// complete in order; the fences keep the compiler from moving them across the staging boundary.
+static constexpr int kStage = 8;                               // reflections per staging block
+static constexpr int kGrpF32 = 10;                             // float2 slots per 8 atoms (2 pad)
+static constexpr int kGrpF64 = 9;                              // double2 slots per 8 atoms (1 pad)
+static constexpr int kStageBytesF32 = kStage * 8 * kGrpF32 * 8;    // per wave, mixed path
+static constexpr int kStageBytesF64 = kStage * 8 * kGrpF64 * 16;   // per wave, FP64 path
+
+__device__ __forceinline__ void lds_wave_sync()
 {
-    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xF, false);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int CTRL, int ROW_MASK>
+template <int CTRL>
 __device__ __forceinline__ float dpp_add(float v)
 {
-    return v + __builtin_bit_cast(float, dpp_mov<CTRL, ROW_MASK>(__builtin_bit_cast(int, v)));
+    return v + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF,
+                                                                      false));
 }
 
-template <int CTRL, int ROW_MASK>
+template <int CTRL>
 __device__ __forceinline__ double dpp_add(double v)
 {
-    long long b = __builtin_bit_cast(long long, v);
-    int lo = dpp_mov<CTRL, ROW_MASK>((int)(b & 0xffffffff));
-    int hi = dpp_mov<CTRL, ROW_MASK>((int)(b >> 32));
-    long long r = ((long long)(unsigned)lo) | ((long long)hi << 32);
-    return v + __builtin_bit_cast(double, r);
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const unsigned hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return v + __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// sum over the eight lanes 8j .. 8j+7 (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror)
 template <typename T>
-__device__ __forceinline__ T wave_sum(T v)
+__device__ __forceinline__ T octet_sum(T v)
 {
-    v = dpp_add<0xB1, 0xF>(v);    // quad_perm [1,0,3,2]
-    v = dpp_add<0x4E, 0xF>(v);    // quad_perm [2,3,0,1]
-    v = dpp_add<0x141, 0xF>(v);   // row_half_mirror
-    v = dpp_add<0x140, 0xF>(v);   // row_mirror
-    v = dpp_add<0x142, 0xA>(v);   // row_bcast:15 into rows 1, 3
-    v = dpp_add<0x143, 0xC>(v);   // row_bcast:31 into rows 2, 3
-    return v;                     // complete in lane 63
+    v = dpp_add<0xB1>(v);
+    v = dpp_add<0x4E>(v);
+    return dpp_add<0x141>(v);
+}
+
+__device__ __forceinline__ int stage_slot(int kk, int lane, int grp)
+{
+    return kk * 8 * grp + lane + (lane >> 3) * (grp - 8);
+}
+
+__device__ __forceinline__ void stage_f32(float2* buf, int kk, int lane, float re, float im)
+{
+    buf[stage_slot(kk, lane, kGrpF32)] = make_float2(re, im);
+}
+
+__device__ __forceinline__ void flush_f32(const float2* buf, int nk, int lane, double* __restrict__ fp)
+{
+    lds_wave_sync();
+    const int kk = lane >> 3, part = lane & 7;
+    const float4* seg = reinterpret_cast<const float4*>(buf + kk * 8 * kGrpF32 + part * kGrpF32);
+    float re = 0.f, im = 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const float4 v = seg[i];
+        re += v.x;
+        im += v.y;
+        re += v.z;
+        im += v.w;
+    }
+    re = octet_sum(re);
+    im = octet_sum(im);
+    if (part == 0 && kk < nk) {
+        fp[2 * kk] = re;
+        fp[2 * kk + 1] = im;
+    }
+    lds_wave_sync();
+}
+
+__device__ __forceinline__ void stage_f64(double2* buf, int kk, int lane, double re, double im)
+{
+    buf[stage_slot(kk, lane, kGrpF64)] = make_double2(re, im);
+}
+
+__device__ __forceinline__ void flush_f64(const double2* buf, int nk, int lane, double* __restrict__ fp)
+{
+    lds_wave_sync();
+    const int kk = lane >> 3, part = lane & 7;
+    const double2* seg = buf + kk * 8 * kGrpF64 + part * kGrpF64;
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const double2 v = seg[i];
+        re += v.x;
+        im += v.y;
+    }
+    re = octet_sum(re);
+    im = octet_sum(im);
+    if (part == 0 && kk < nk) {
+        fp[2 * kk] = re;
+        fp[2 * kk + 1] = im;
+    }
+    lds_wave_sync();
 }
 
 // Wave-uniform FP64 record entries are fetched as 32-bit words so that type-based alias analysis
@@ -377,10 +452,11 @@ __device__ __forceinline__ void contract(const R* P, const R* Y, R* A)
 }
 
 template <int L, bool DERIV, bool MP>
-__device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane, int k0, int k1,
+__device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane, int chunk_id, int k0, int k1,
                                           const unsigned* __restrict__ hop, const double* __restrict__ rad,
                                           const double* __restrict__ radx, const unsigned* __restrict__ dtdf32,
-                                          double* __restrict__ fpart, double* __restrict__ dpart)
+                                          double2* __restrict__ stage, double* __restrict__ fpart,
+                                          double* __restrict__ dpart)
 {
     constexpr int NLM = (L + 1) * (L + 1);
     const int slot = wave * kWave + lane;
@@ -399,7 +475,10 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
 #pragma unroll
     for (int j = 0; j < (MP ? NLM : 1); j++) B[j] = 0;
     double dpv = 0, dka = 0, dkp = 0;
-    for (int k = k0; k < k1; ++k) {
+    for (int kb = k0; kb < k1; kb += kStage) {
+    const int nk = min(kStage, k1 - kb);
+    for (int kk = 0; kk < nk; ++kk) {
+        const int k = kb + kk;
         const double* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
         const double S = rr[0] + anr, G0 = rr[1], G1 = rr[2], G2 = rr[3], G3 = rr[4], G4 = rr[5];
         double fval = 0, dSk = 0, dG[5] = {0, 0, 0, 0, 0};
@@ -425,8 +504,8 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
                 fr += G4 * A[4];
                 fi += G3 * A[3];
             }
-            double ph = r[31] * x0 + r[32] * x1 + r[33] * x2 + r[34];
-            ph -= rint(ph);
+            // h.(R x + t) reduced to [0, 1) revolutions
+            const double ph = __builtin_amdgcn_fract(fma(r[33], x2, fma(r[32], x1, fma(r[31], x0, r[34]))));
             double sn, cs;
             sincospi(2.0 * ph, &sn, &cs);
             double e = r[25] * u[0] + r[26] * u[1] + r[27] * u[2] + r[28] * u[3] + r[29] * u[4] + r[30] * u[5];
@@ -459,16 +538,13 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
             dpv += fval * Rh;
             dka += dSk * Rh;
         }
-        double fr_w = wave_sum(w * sr), fi_w = wave_sum(w * si);
-        if (lane == kWave - 1) {
-            double* fp = fpart + ((size_t)wave * a.n_hkl + k) * 2;
-            fp[0] = fr_w;
-            fp[1] = fi_w;
-        }
+        stage_f64(stage, kk, lane, w * sr, w * si);
+    }
+    flush_f64(stage, nk, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
     }
     if (DERIV) {
         constexpr int ND = MP ? kNdMultipole : kNdBasic;
-        double* dp = dpart + ((size_t)blockIdx.y * a.n_slots + slot) * ND;
+        double* dp = dpart + ((size_t)chunk_id * a.n_slots + slot) * ND;
         dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
 #pragma unroll
         for (int j = 0; j < 6; j++) dp[3 + j] = du[j];
@@ -486,14 +562,14 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
 // Mixed precision: FP64 phase range reduction and FP64 accumulation across reflections, FP32
 // harmonics / radial / Debye-Waller / sincos / exp per term.
 template <int L, bool DERIV, bool MP>
-__device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane, int k0, int k1,
+__device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane, int chunk_id, int k0, int k1,
                                            const float* __restrict__ hop, const float* __restrict__ rad,
                                            const float* __restrict__ radx, const unsigned* __restrict__ phs32,
-                                           const unsigned* __restrict__ dtdf32, double* __restrict__ fpart,
-                                           double* __restrict__ dpart)
+                                           const unsigned* __restrict__ dtdf32, float2* __restrict__ stage,
+                                           double* __restrict__ fpart, double* __restrict__ dpart)
 {
     constexpr int NLM = (L + 1) * (L + 1);
-    constexpr int kFlush = 16;    // reflections accumulated in FP32 before an FP64 flush
+    constexpr int kFlush = kStage;    // reflections accumulated in FP32 before an FP64 flush
     const int slot = wave * kWave + lane;
     const AtomRec& at = a.atoms[slot];
     const double x0 = at.xf[0], x1 = at.xf[1], x2 = at.xf[2];
@@ -518,6 +594,7 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
         for (int j = 0; j < (MP ? NLM : 1); j++) fB[j] = 0;
         float fpv = 0, fka = 0, fkp = 0;
         for (int k = kb; k < kend; ++k) {
+            const int kk = k - kb;
             const float* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
             const float S = rr[0] + anr, G0 = rr[1], G1 = rr[2], G2 = rr[3], G3 = rr[4], G4 = rr[5];
             float fval = 0, dSk = 0, dG[5] = {0, 0, 0, 0, 0};
@@ -541,9 +618,9 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
                     fr += G4 * A[4];
                     fi += G3 * A[3];
                 }
-                // FP64 range reduction of h.(R x + t) to [-1/2, 1/2) revolutions
-                double ph = ld_f64(pr, 0) * x0 + ld_f64(pr, 1) * x1 + ld_f64(pr, 2) * x2 + ld_f64(pr, 3);
-                float phf = (float)(ph - rint(ph));
+                // FP64 range reduction of h.(R x + t) to [0, 1) revolutions
+                const double ph = fma(ld_f64(pr, 2), x2, fma(ld_f64(pr, 1), x1, fma(ld_f64(pr, 0), x0, ld_f64(pr, 3))));
+                const float phf = (float)__builtin_amdgcn_fract(ph);
                 float sn = __builtin_amdgcn_sinf(phf), cs = __builtin_amdgcn_cosf(phf);
                 float e = r[25] * u[0] + r[26] * u[1] + r[27] * u[2] + r[28] * u[3] + r[29] * u[4] + r[30] * u[5];
                 float T = __builtin_amdgcn_exp2f(-e);
@@ -575,12 +652,7 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
                 fpv += fval * Rh;
                 fka += dSk * Rh;
             }
-            float fr_w = wave_sum(w * sr), fi_w = wave_sum(w * si);
-            if (lane == kWave - 1) {
-                double* fp = fpart + ((size_t)wave * a.n_hkl + k) * 2;
-                fp[0] = fr_w;
-                fp[1] = fi_w;
-            }
+            stage_f32(stage, kk, lane, w * sr, w * si);
         }
         if (DERIV) {
 #pragma unroll
@@ -596,12 +668,13 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
             dka += fka;
             dkp += fkp;
         }
+        flush_f32(stage, kend - kb, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
     }
     if (DERIV) {
         // U multipliers carried log2(e); undo for the derivative
         const double inv_log2e = 0.69314718055994530942;
         constexpr int ND = MP ? kNdMultipole : kNdBasic;
-        double* dp = dpart + ((size_t)blockIdx.y * a.n_slots + slot) * ND;
+        double* dp = dpart + ((size_t)chunk_id * a.n_slots + slot) * ND;
         dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
 #pragma unroll
         for (int j = 0; j < 6; j++) dp[3 + j] = du[j] * inv_log2e;
@@ -616,42 +689,74 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
     }
 }
 
+// One workgroup = kWavesPerBlock atom waves x one reflection chunk, over a 1-D grid of
+// kXcds * per_xcd workgroups: hardware id b runs on XCD b % kXcds and is mapped to logical block
+// (b % kXcds) * per_xcd + b / kXcds, so each XCD walks a contiguous range of chunks and every
+// record of a chunk is fetched into one L2 only.
 template <bool MIXED, bool DERIV, bool MP>
-__global__ __launch_bounds__(256) void k_main(MainArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && !MP ? 5 : 1))) void k_main(MainArgs a)
 {
-    const int wave = blockIdx.x * kWavesPerBlock + (int)(threadIdx.x / kWave);
+    constexpr int kStageBytes = MIXED ? kStageBytesF32 : kStageBytesF64;
+    __shared__ __align__(16) unsigned char stage_lds[kWavesPerBlock * kStageBytes];
+    const int b = blockIdx.x;
+    const int logical = (b % kXcds) * a.per_xcd + b / kXcds;
+    if (logical >= a.n_blocks) return;
+    const int chunk_id = logical / a.grid_x;
+    const int wib = (int)(threadIdx.x / kWave);
+    const int wave = (logical - chunk_id * a.grid_x) * kWavesPerBlock + wib;
     if (wave >= a.n_waves) return;
     const int lane = threadIdx.x & (kWave - 1);
-    const int k0 = blockIdx.y * a.chunk;
+    const int k0 = chunk_id * a.chunk;
     const int k1 = min(k0 + a.chunk, a.n_hkl);
     const int cls = __builtin_amdgcn_readfirstlane(a.wave_l[wave]);
     const unsigned* dtdf32 = reinterpret_cast<const unsigned*>(a.dtdf);
+    void* stage = stage_lds + wib * kStageBytes;
     if (MIXED) {
         const float* hop = static_cast<const float*>(a.hklop);
         const float* rad = static_cast<const float*>(a.radial);
         const float* radx = static_cast<const float*>(a.radx);
         const unsigned* phs = reinterpret_cast<const unsigned*>(a.phase);
-        if (cls <= 2) body_mixed<2, DERIV, MP>(a, wave, lane, k0, k1, hop, rad, radx, phs, dtdf32, a.fpart, a.dpart);
-        else body_mixed<4, DERIV, MP>(a, wave, lane, k0, k1, hop, rad, radx, phs, dtdf32, a.fpart, a.dpart);
+        float2* st = static_cast<float2*>(stage);
+        if (cls <= 2)
+            body_mixed<2, DERIV, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, phs, dtdf32, st, a.fpart, a.dpart);
+        else
+            body_mixed<4, DERIV, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, phs, dtdf32, st, a.fpart, a.dpart);
     } else {
         const unsigned* hop = static_cast<const unsigned*>(a.hklop);
         const double* rad = static_cast<const double*>(a.radial);
         const double* radx = static_cast<const double*>(a.radx);
-        if (cls <= 2) body_fp64<2, DERIV, MP>(a, wave, lane, k0, k1, hop, rad, radx, dtdf32, a.fpart, a.dpart);
-        else body_fp64<4, DERIV, MP>(a, wave, lane, k0, k1, hop, rad, radx, dtdf32, a.fpart, a.dpart);
+        double2* st = static_cast<double2*>(stage);
+        if (cls <= 2)
+            body_fp64<2, DERIV, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, dtdf32, st, a.fpart, a.dpart);
+        else
+            body_fp64<4, DERIV, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, dtdf32, st, a.fpart, a.dpart);
     }
 }
 
 // F(h) = mult(h) * sum_waves partial  (centric at origin: 2 Re, 0)
-__global__ void k_reduce_f(const double* __restrict__ fpart, const double* __restrict__ mult, int n_waves,
+// F(h) = mult(h) * sum_waves partial  (centric at origin: 2 Re, 0).  Loads are issued eight at a
+// time and added in wave order.
+__global__ void k_reduce_f(const double2* __restrict__ fpart, const double* __restrict__ mult, int n_waves,
                            int n_hkl, int centric_origin, double* __restrict__ f)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_hkl) return;
     double re = 0, im = 0;
-    for (int w = 0; w < n_waves; w++) {
-        re += fpart[((size_t)w * n_hkl + k) * 2];
-        im += fpart[((size_t)w * n_hkl + k) * 2 + 1];
+    int w = 0;
+    for (; w + 8 <= n_waves; w += 8) {
+        double2 v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = fpart[(size_t)(w + q) * n_hkl + k];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            re += v[q].x;
+            im += v[q].y;
+        }
+    }
+    for (; w < n_waves; w++) {
+        const double2 v = fpart[(size_t)w * n_hkl + k];
+        re += v.x;
+        im += v.y;
     }
     if (centric_origin) {
         re *= 2;
@@ -662,18 +767,28 @@ __global__ void k_reduce_f(const double* __restrict__ fpart, const double* __res
 }
 
 // d[orig][j] = sum over reflection chunks of the per-(chunk, slot) partials; one thread per
-// (slot, j), fixed chunk order.
+// (slot, j), chunks added in order, loads issued eight at a time.
 __global__ void k_reduce_d(const double* __restrict__ dpart, const AtomRec* __restrict__ atoms, int n_chunks,
                            int n_slots, int nd, double* __restrict__ d)
 {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n_slots * nd) return;
-    const int i = idx / nd, j = idx - i * nd;
+    const int i = idx / nd;
     const int orig = atoms[i].orig;
     if (orig < 0) return;
+    const size_t stride = (size_t)n_slots * nd;
+    const double* p = dpart + idx;
     double s = 0.0;
-    for (int c = 0; c < n_chunks; c++) s += dpart[((size_t)c * n_slots + i) * nd + j];
-    d[(size_t)orig * nd + j] = s;
+    int c = 0;
+    for (; c + 8 <= n_chunks; c += 8) {
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = p[(size_t)(c + q) * stride];
+#pragma unroll
+        for (int q = 0; q < 8; q++) s += v[q];
+    }
+    for (; c < n_chunks; c++) s += p[(size_t)c * stride];
+    d[(size_t)orig * nd + (idx - i * nd)] = s;
 }
 
 // dT/dP_lm in the atom's local frame from the global-frame derivative: P_glob = D P_loc with
@@ -1082,9 +1197,12 @@ void Engine::launch_main(Precision prec, bool derivs, bool mp, int chunk)
     a.n_slots = n_slots;
     a.n_waves = n_waves;
     a.chunk = chunk;
-    dim3 grid((n_waves + kWavesPerBlock - 1) / kWavesPerBlock, n_chunks);
-    last_grid_x = grid.x;
-    last_grid_y = grid.y;
+    a.grid_x = (n_waves + kWavesPerBlock - 1) / kWavesPerBlock;
+    a.n_blocks = a.grid_x * n_chunks;
+    a.per_xcd = (a.n_blocks + kXcds - 1) / kXcds;
+    dim3 grid(a.per_xcd * kXcds);
+    last_grid_x = a.grid_x;
+    last_grid_y = n_chunks;
     last_chunk = chunk;
     if (mixed) {
         if (mp) hipLaunchKernelGGL((k_main<true, true, true>), grid, dim3(256), 0, stream_, a);
@@ -1131,29 +1249,43 @@ void Engine::run(Precision prec, bool derivs, bool mp)
     }
     run_prep(prec, mp);
     const int chunk = chunk_for(prec, derivs, mp);
-    const int n_chunks = (n_hkl + chunk - 1) / chunk;
     if (timing) HIP_CHECK(hipEventRecord(ev0_, stream_));
     launch_main(prec, derivs, mp, chunk);
     if (timing) HIP_CHECK(hipEventRecord(ev1_, stream_));
-    const int T = 256;
-    hipLaunchKernelGGL(k_reduce_f, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, d_fpart_.as<double>(),
-                       d_dtdf_.as<double>() + 2 * (size_t)n_hkl, n_waves, n_hkl, centric_origin_ ? 1 : 0,
-                       d_f_.as<double>());
-    HIP_CHECK(hipGetLastError());
-    if (derivs) {
-        const int nd = last_nd_;
-        hipLaunchKernelGGL(k_reduce_d, dim3((n_slots * nd + T - 1) / T), dim3(T), 0, stream_, d_dpart_.as<double>(),
-                           d_atoms_.as<AtomRec>(), n_chunks, n_slots, nd, d_d_.as<double>());
-        HIP_CHECK(hipGetLastError());
-        if (mp) {
-            hipLaunchKernelGGL(k_plm_back, dim3(n_slots), dim3(64), 0, stream_, d_atoms_in_.as<AtomIn>(), n_slots, nd,
-                               d_d_.as<double>());
-            HIP_CHECK(hipGetLastError());
-        }
-    }
+    reduce(derivs, mp, chunk);
     if (timing) {
         HIP_CHECK(hipEventSynchronize(ev1_));
         HIP_CHECK(hipEventElapsedTime(&last_main_ms, ev0_, ev1_));
+    }
+}
+
+// Stage 2: F from the per-wave partials, derivatives from the per-chunk partials.
+void Engine::reduce(bool derivs, bool mp, int chunk)
+{
+    const int n_hkl = n_hkl_;
+    const int n_slots = n_sorted_;
+    const int n_waves = n_slots / kWave;
+    const int n_chunks = (n_hkl + chunk - 1) / chunk;
+    const int nd = mp ? kNdMultipole : kNdBasic;
+    // outputs sized here: the autotuner reaches this before any run() has allocated them
+    d_f_.ensure((size_t)std::max(n_hkl, 1) * 2 * sizeof(double));
+    if (derivs) d_d_.ensure((size_t)std::max(n_atoms_, 1) * nd * sizeof(double));
+    if (d_fpart_.bytes < (size_t)n_waves * n_hkl * 2 * sizeof(double) ||
+        (derivs && d_dpart_.bytes < (size_t)n_chunks * n_slots * nd * sizeof(double)))
+        throw Error(1, "internal: stage-2 partial buffers smaller than the launch geometry");
+    const int T = 64;
+    hipLaunchKernelGGL(k_reduce_f, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, d_fpart_.as<double2>(),
+                       d_dtdf_.as<double>() + 2 * (size_t)n_hkl, n_waves, n_hkl, centric_origin_ ? 1 : 0,
+                       d_f_.as<double>());
+    HIP_CHECK(hipGetLastError());
+    if (!derivs) return;
+    hipLaunchKernelGGL(k_reduce_d, dim3((n_slots * nd + T - 1) / T), dim3(T), 0, stream_, d_dpart_.as<double>(),
+                       d_atoms_.as<AtomRec>(), n_chunks, n_slots, nd, d_d_.as<double>());
+    HIP_CHECK(hipGetLastError());
+    if (mp) {
+        hipLaunchKernelGGL(k_plm_back, dim3(n_slots), dim3(64), 0, stream_, d_atoms_in_.as<AtomIn>(), n_slots, nd,
+                           d_d_.as<double>());
+        HIP_CHECK(hipGetLastError());
     }
 }
 
@@ -1178,6 +1310,7 @@ int Engine::autotune(Precision prec, bool derivs, bool mp)
         for (int rep = 0; rep < 2; rep++) {
             HIP_CHECK(hipEventRecord(ev0_, stream_));
             launch_main(prec, derivs, mp, c);
+            reduce(derivs, mp, c);
             HIP_CHECK(hipEventRecord(ev1_, stream_));
             HIP_CHECK(hipEventSynchronize(ev1_));
             float ms = 0;
@@ -1190,6 +1323,7 @@ int Engine::autotune(Precision prec, bool derivs, bool mp)
         }
     }
     tuned_[shape_key(prec, derivs, mp)] = best;
+    ran_ = false;     // the outputs hold a tuning launch, not a run() result
     return best;
 }
 

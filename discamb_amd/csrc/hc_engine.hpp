@@ -66,8 +66,9 @@ public:
     void set_types(const std::vector<AtomType>& types);
     const EngineModel& model() const { return model_; }
     void synchronize();
-    // Time the fused kernel over candidate reflection-chunk sizes on the uploaded inputs and keep
-    // the fastest for this (atoms, reflections, ops, precision) shape; returns the chunk chosen.
+    // Time the fused kernel plus its stage-2 reductions over candidate reflection-chunk sizes on
+    // the uploaded inputs and keep the fastest for this (atoms, reflections, ops, precision)
+    // shape; returns the chunk chosen.
     int autotune(Precision prec, bool derivs, bool mp = false);
     hipStream_t stream() const { return stream_; }
     TuneParams tune;
@@ -80,6 +81,7 @@ public:
 private:
     void run_prep(Precision prec, bool mp);
     void launch_main(Precision prec, bool derivs, bool mp, int chunk);
+    void reduce(bool derivs, bool mp, int chunk);
     void fill(DeviceBuffer& b, size_t n, double v);
     int resident_waves(bool mixed, bool derivs, bool mp);
     int default_chunk(bool mixed, bool derivs, bool mp);

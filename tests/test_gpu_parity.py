@@ -152,3 +152,28 @@ def test_determinism_full_config1():
         e, rel_err(dm.atomic_position_derivatives, d1.atomic_position_derivatives),
         rel_err(dm.adp_derivatives, d1.adp_derivatives)), flush=True)
     assert e < 1e-6
+
+
+@pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
+def test_autotune_before_first_run_and_launch_shapes(mode):
+    """The autotuner runs on freshly uploaded inputs (before any run), leaves no result to fetch,
+    and every reflection-chunk size (ragged last chunk, one chunk for everything) gives the
+    oracle's answer."""
+    w = synth.small_structure("P212121", n_atoms=150, d_min=0.8)
+    print("case autotune mode=%s atoms=%d hkl=%d" % (mode, w.crystal.atoms.n, len(w.hkl)), flush=True)
+    calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+    calc.setCalculationMode(mode)
+    calc.upload(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+    assert calc.autotune() > 0
+    with pytest.raises(RuntimeError):
+        calc.fetch()
+    rf, rx, ru, ro = hc_oracle.sf_and_derivs(w.crystal, w.params, w.crystal.atoms, w.lcs, w.hkl, w.dtdf, n_threads=16)
+    tf, td = TOL[mode]
+    for chunk in (0, 8, 13, 64, len(w.hkl)):
+        calc.setTuning(chunk)
+        calc.run()
+        f, d = calc.fetch()
+        errs = (rel_err(f, rf), rel_err(d.atomic_position_derivatives, rx), rel_err(d.adp_derivatives, ru),
+                rel_err(d.occupancy_derivatives, ro))
+        print("  chunk %d errors %s" % (chunk, errs), flush=True)
+        assert errs[0] <= tf and max(errs[1:]) <= td

@@ -68,6 +68,7 @@ SIGNATURES = {
     "dh_hc_destroy": (None, [ctypes.c_void_p]),
     "dh_hc_set_calculation_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "dh_hc_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "dh_hc_set_options": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint]),
     "dh_hc_sf_and_derivs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DhAtoms), ctypes.c_int, c_int_p, c_dbl_p,
                                            ctypes.c_uint, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p]),
     "dh_hc_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DhAtoms), ctypes.c_int, c_int_p, c_dbl_p]),

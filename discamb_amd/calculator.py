@@ -146,6 +146,11 @@ class HansenCoppensStructureFactorCalculator:
     def setTuning(self, hkl_chunk: int):
         _lib.check(self._lib.dh_hc_set_tuning(self._h, int(hkl_chunk)))
 
+    def setKernelOptions(self, general_rotations: bool = False):
+        """general_rotations: run the general-rotation kernels even for groups whose operations
+        are all diagonal sign matrices (those otherwise use the parity-folded kernels)."""
+        _lib.check(self._lib.dh_hc_set_options(self._h, 1 if general_rotations else 0))
+
     def engineInfo(self):
         n, o, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         ce = ctypes.create_string_buffer(2)

@@ -366,6 +366,14 @@ int dh_hc_set_tuning(dh_calculator* calc, int hkl_chunk)
     });
 }
 
+int dh_hc_set_options(dh_calculator* calc, unsigned flags)
+{
+    return guarded([&] {
+        if (!calc || (flags & ~DH_OPT_GENERAL_ROTATIONS)) throw Error(DH_E_ARGUMENT, "invalid options");
+        calc->c->engine->tune.general_path = (flags & DH_OPT_GENERAL_ROTATIONS) != 0;
+    });
+}
+
 int dh_hc_upload(dh_calculator* calc, const dh_atoms* atoms, int n_hkl, const int* hkl, const double* dtarget_df)
 {
     return guarded([&] {

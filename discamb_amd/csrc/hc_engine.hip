@@ -44,6 +44,8 @@ struct TypeRec {
     int wfn, nl, max_l, lp;     // lp = min(4, n_l - 1, size(p_lm) - 1): l range of the multipole adjoint
     int n[5], pad2[3];
     double pval, kappa, kappa_p, zeta;
+    double inv_kappa, inv_kappa_p;
+    double gscale[5];           // 4 pi (i^l sign) N_l: STO normalisation zeta^(n+3)/(n+2)!
     double plm[25];
 };
 
@@ -157,6 +159,39 @@ __global__ void k_hkl_prep(const int* __restrict__ hkl, const OpDev* __restrict_
     }
 }
 
+// Diagonal groups (mixed precision): one FP32 record per reflection [d_j(h) x 25, m(h) log2(e) x 6,
+// h x 3, pad x 2], an FP64 [h, 0] phase record, h.t_s per operation, and |h|.
+__global__ void k_hkl_prep_diag(const int* __restrict__ hkl, const OpDev* __restrict__ ops, int n_ops, int n_hkl,
+                                const double* __restrict__ c2f, float* __restrict__ rec32, double* __restrict__ hd,
+                                double* __restrict__ ht, double* __restrict__ hlen)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_hkl) return;
+    const int h0 = hkl[3 * k], h1 = hkl[3 * k + 1], h2 = hkl[3 * k + 2];
+    double hc[3];
+    for (int j = 0; j < 3; j++) hc[j] = h0 * c2f[j] + h1 * c2f[3 + j] + h2 * c2f[6 + j];
+    const double hl = sqrt(hc[0] * hc[0] + hc[1] * hc[1] + hc[2] * hc[2]);
+    hlen[k] = hl;
+    double y[25];
+    if (hl < 1e-10) {
+        for (int i = 0; i < 25; i++) y[i] = 0.0;
+    } else {
+        const double inv = 1.0 / hl;
+        ylm_density<double>(hc[0] * inv, hc[1] * inv, hc[2] * inv, y);
+    }
+    const double m[6] = {hc[0] * hc[0], hc[1] * hc[1], hc[2] * hc[2], 2 * hc[0] * hc[1], 2 * hc[0] * hc[2],
+                         2 * hc[1] * hc[2]};
+    float* r = rec32 + (size_t)k * kHklopRec;
+    for (int i = 0; i < 25; i++) r[i] = (float)y[i];
+    for (int i = 0; i < 6; i++) r[25 + i] = (float)(m[i] * 1.4426950408889634074);
+    r[31] = (float)h0; r[32] = (float)h1; r[33] = (float)h2;
+    r[34] = 0.f; r[35] = 0.f;
+    double* p = hd + (size_t)k * 4;
+    p[0] = h0; p[1] = h1; p[2] = h2; p[3] = 0.0;
+    for (int s = 0; s < n_ops; s++)
+        ht[(size_t)k * n_ops + s] = h0 * ops[s].tf[0] + h1 * ops[s].tf[1] + h2 * ops[s].tf[2];
+}
+
 // f_core(|h|) per (wavefunction type, reflection): 4 pi sum_k c_k g_0(p_k + 2, |h|, e_k)
 __global__ void k_core_prep(const double* __restrict__ hlen, const WfnRec* __restrict__ wfns,
                             const TermRec* __restrict__ core, int n_wfn, int n_hkl, double* __restrict__ fcore)
@@ -192,7 +227,7 @@ __global__ void k_radial_prep(const double* __restrict__ hlen, const double* __r
     const WfnRec wf = wfns[ty.wfn];
     const bool want_d = radx64 || radx32;
     double fv = 0.0, dfv = 0.0;
-    const double hk = h / ty.kappa;
+    const double hk = h * ty.inv_kappa;
     for (int i = 0; i < wf.val_n; i++) {
         const TermRec tr = val[wf.val_off + i];
         if (want_d) {
@@ -208,21 +243,17 @@ __global__ void k_radial_prep(const double* __restrict__ hlen, const double* __r
     r[0] = fcore[(size_t)ty.wfn * n_hkl + k] + ty.pval * four_pi * fv;
     x[0] = four_pi * fv;
     // d f_val(h/kappa) / d kappa = f_val'(h/kappa) * (-h / kappa^2)
-    x[1] = ty.pval * four_pi * dfv * (-h / (ty.kappa * ty.kappa));
-    const double hp = h / ty.kappa_p;
+    x[1] = ty.pval * four_pi * dfv * (-hk * ty.inv_kappa);
+    const double hp = h * ty.inv_kappa_p;
     for (int l = 0; l <= ty.lp; l++) {
-        int n = ty.n[l];
-        double norm = 1.0;
-        for (int q = 0; q < n + 3; q++) norm *= ty.zeta;
-        for (int q = 2; q <= n + 2; q++) norm /= q;
-        const double sgn = (l == 2 || l == 3) ? -1.0 : 1.0;   // i^l: +, +i, -, -i, +
+        const int n = ty.n[l];
         if (want_d) {
             double dg;
-            double g = g_function_dh(l, n + 2, hp, ty.zeta, &dg);
-            r[1 + l] = sgn * four_pi * norm * g;
-            x[2 + l] = sgn * four_pi * norm * dg * (-h / (ty.kappa_p * ty.kappa_p));
+            const double g = g_function_dh(l, n + 2, hp, ty.zeta, &dg);
+            r[1 + l] = ty.gscale[l] * g;
+            x[2 + l] = ty.gscale[l] * dg * (-hp * ty.inv_kappa_p);
         } else {
-            r[1 + l] = sgn * four_pi * norm * g_function(l, n + 2, hp, ty.zeta);
+            r[1 + l] = ty.gscale[l] * g_function(l, n + 2, hp, ty.zeta);
         }
     }
     size_t base = ((size_t)k * n_types + t) * kRadRec;
@@ -301,6 +332,10 @@ struct MainArgs {
     int grid_x;               // atom blocks per reflection chunk
     int n_blocks;             // grid_x * number of chunks
     int per_xcd;              // ceil(n_blocks / kXcds)
+    int diag_mask;            // diagonal groups: bit p set when an operation has sign pattern p
+    int diag_op[8];           // diagonal groups: operation index of sign pattern p (bit i: axis i flips)
+    float diag_anom[8];       // diagonal groups: anomalous sign of the operation with pattern p
+    const double* ht;         // diagonal groups: h.t_s per (reflection, operation)
 };
 
 // per-atom partial layout: 0-2 d/dx_frac, 3-8 d/dU, 9 d/docc, then with the multipole adjoint
@@ -689,13 +724,195 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
     }
 }
 
+// ------------------------------------------------------------------ diagonal point groups
+// When every explicit operation has a diagonal rotation (R = diag(sx, sy, sz), the same signs in
+// fractional and Cartesian axes: triclinic, monoclinic and orthorhombic groups) the rotated
+// reflection is h_s = (sx h, sy k, sz l).  Each density-normalised harmonic has a definite parity
+// in x, y and z, so d_j(h_s) = sign_j(s) d_j(h): the atom's P.d contraction is done once per
+// reflection as parity-class sums and every operation combines them with its own signs
+// (compile-time add/sub, one code path per sign pattern).  The Debye-Waller exponent splits the
+// same way into a diagonal part and three signed cross terms.  Records are per reflection:
+// [d_j(h) x 25, DW multipliers of h x 6, h x 3, pad] plus h.t_s per operation.  Mixed precision
+// only: in FP64 the class sums double the register footprint and cost more occupancy than the
+// saved contraction work.
+
+// parity bits of harmonic j: 1 = odd in x, 2 = odd in y, 4 = odd in z
+__host__ __device__ constexpr int harmonic_parity(int j)
+{
+    constexpr int t[25] = {0, 2, 4, 1, 3, 6, 0, 5, 0, 2, 7, 2, 4, 1, 4, 1, 3, 6, 3, 6, 0, 5, 0, 5, 0};
+    return t[j];
+}
+
+__host__ __device__ constexpr int harmonic_l(int j) { return j < 1 ? 0 : j < 4 ? 1 : j < 9 ? 2 : j < 16 ? 3 : 4; }
+
+__host__ __device__ constexpr bool class_present(int l, int key)
+{
+    for (int j = l * l; j < (l + 1) * (l + 1); j++)
+        if (harmonic_parity(j) == key) return true;
+    return false;
+}
+
+__host__ __device__ constexpr bool odd_under(int key, int pat)
+{
+    return (__builtin_popcount((unsigned)(key & pat)) & 1) != 0;
+}
+
+template <int L, typename R>
+__device__ __forceinline__ void class_sums(const R* P, const R* Y, R (&C)[5][8])
+{
+    constexpr int NLM = (L + 1) * (L + 1);
+#pragma unroll
+    for (int l = 0; l < 5; l++)
+#pragma unroll
+        for (int c = 0; c < 8; c++) C[l][c] = R(0);
+#pragma unroll
+    for (int j = 0; j < NLM; j++) C[harmonic_l(j)][harmonic_parity(j)] += P[j] * Y[j];
+}
+
+template <int L, int PAT, typename R>
+__device__ __forceinline__ void signed_contraction(const R (&C)[5][8], R* A)
+{
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+        R acc = R(0);
+        if (l <= L) {
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+                if (class_present(l, c)) acc = odd_under(c, PAT) ? acc - C[l][c] : acc + C[l][c];
+        }
+        A[l] = acc;
+    }
+}
+
+// per-reflection values shared by all operations of one atom (working precision R)
+template <typename R>
+struct DiagRefl {
+    R C[5][8];             // parity-class sums of P_j d_j(h)
+    R D, O3, O4, O5;       // DW exponent: diagonal part and the xy, xz, yz cross terms
+    R S, G0, G1, G2, G3, G4, cr, ci;
+};
+
+// Mixed precision: one operation with sign pattern PAT.
+template <int L, bool DERIV, int PAT>
+__device__ __forceinline__ void diag_term_mixed(const DiagRefl<float>& q, const float* __restrict__ r, double h0,
+                                                double h1, double h2, double ht, float an, float ani, double x0,
+                                                double x1, double x2, float& sr, float& si, float (&fx)[3],
+                                                float (&fu)[6])
+{
+    constexpr double sx = (PAT & 1) ? -1.0 : 1.0, sy = (PAT & 2) ? -1.0 : 1.0, sz = (PAT & 4) ? -1.0 : 1.0;
+    constexpr float fxy = float(sx * sy), fxz = float(sx * sz), fyz = float(sy * sz);
+    float A[5];
+    signed_contraction<L, PAT>(q.C, A);
+    float fr = q.S + q.G0 * A[0] + q.G2 * A[2];
+    float fi = ani * an + q.G1 * A[1];
+    if (L == 4) {
+        fr += q.G4 * A[4];
+        fi += q.G3 * A[3];
+    }
+    const double ph = __builtin_amdgcn_fract(fma(sz * h2, x2, fma(sy * h1, x1, fma(sx * h0, x0, ht))));
+    const float phf = (float)ph;
+    const float sn = __builtin_amdgcn_sinf(phf), cs = __builtin_amdgcn_cosf(phf);
+    const float e = q.D + fxy * q.O3 + fxz * q.O4 + fyz * q.O5;
+    const float T = __builtin_amdgcn_exp2f(-e);
+    const float Er = T * cs, Ei = T * sn;
+    const float Xr = fr * Er - fi * Ei, Xi = fr * Ei + fi * Er;
+    sr += Xr;
+    si += Xi;
+    if (DERIV) {
+        const float zr = q.cr * Xr - q.ci * Xi, zi = q.cr * Xi + q.ci * Xr;
+        fu[0] -= r[25] * zr;
+        fu[1] -= r[26] * zr;
+        fu[2] -= r[27] * zr;
+        fu[3] -= (fxy * r[28]) * zr;
+        fu[4] -= (fxz * r[29]) * zr;
+        fu[5] -= (fyz * r[30]) * zr;
+        fx[0] -= (float(sx) * r[31]) * zi;
+        fx[1] -= (float(sy) * r[32]) * zi;
+        fx[2] -= (float(sz) * r[33]) * zi;
+    }
+}
+
+template <int L, bool DERIV>
+__device__ __forceinline__ void body_mixed_diag(const MainArgs& a, int wave, int lane, int chunk_id, int k0, int k1,
+                                                const float* __restrict__ hrec, const float* __restrict__ rad,
+                                                const unsigned* __restrict__ hd32, const unsigned* __restrict__ ht32,
+                                                const unsigned* __restrict__ dtdf32, float2* __restrict__ stage,
+                                                double* __restrict__ fpart, double* __restrict__ dpart)
+{
+    constexpr int NLM = (L + 1) * (L + 1);
+    const int slot = wave * kWave + lane;
+    const AtomRec& at = a.atoms[slot];
+    const double x0 = at.xf[0], x1 = at.xf[1], x2 = at.xf[2];
+    float u[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) u[j] = (float)at.u[j];
+    float P[NLM];
+#pragma unroll
+    for (int j = 0; j < NLM; j++) P[j] = (float)at.plm[j];
+    const float w = (float)at.w, anr = (float)at.anom[0], ani = (float)at.anom[1];
+    const int t = at.type;
+    double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
+    for (int kb = k0; kb < k1; kb += kStage) {
+        const int kend = min(kb + kStage, k1);
+        float fx[3] = {0, 0, 0}, fu[6] = {0, 0, 0, 0, 0, 0}, focc = 0;
+        for (int k = kb; k < kend; ++k) {
+            const float* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
+            const float* r = hrec + (size_t)k * kHklopRec;
+            DiagRefl<float> q;
+            q.S = rr[0] + anr;
+            q.G0 = rr[1];
+            q.G1 = rr[2];
+            q.G2 = rr[3];
+            q.G3 = rr[4];
+            q.G4 = rr[5];
+            q.cr = (float)ld_f64(dtdf32, 2 * k);
+            q.ci = (float)ld_f64(dtdf32, 2 * k + 1);
+            class_sums<L, float>(P, r, q.C);
+            q.D = r[25] * u[0] + r[26] * u[1] + r[27] * u[2];
+            q.O3 = r[28] * u[3];
+            q.O4 = r[29] * u[4];
+            q.O5 = r[30] * u[5];
+            const unsigned* hd = hd32 + (size_t)k * 8;
+            const double h0 = ld_f64(hd, 0), h1 = ld_f64(hd, 1), h2 = ld_f64(hd, 2);
+            float sr = 0, si = 0;
+            const unsigned* htk = ht32 + (size_t)k * a.n_ops * 2;
+            // one uniform block per sign pattern present in the group (each at most once)
+#define DIAG_OP(PAT)                                                                                          \
+    if (a.diag_mask & (1 << PAT))                                                                             \
+        diag_term_mixed<L, DERIV, PAT>(q, r, h0, h1, h2, ld_f64(htk, a.diag_op[PAT]), a.diag_anom[PAT], ani,  \
+                                       x0, x1, x2, sr, si, fx, fu);
+            DIAG_OP(0) DIAG_OP(1) DIAG_OP(2) DIAG_OP(3) DIAG_OP(4) DIAG_OP(5) DIAG_OP(6) DIAG_OP(7)
+#undef DIAG_OP
+            if (DERIV) focc += q.cr * sr - q.ci * si;
+            stage_f32(stage, k - kb, lane, w * sr, w * si);
+        }
+        if (DERIV) {
+#pragma unroll
+            for (int j = 0; j < 3; j++) dx[j] += fx[j];
+#pragma unroll
+            for (int j = 0; j < 6; j++) du[j] += fu[j];
+            docc += focc;
+        }
+        flush_f32(stage, kend - kb, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
+    }
+    if (DERIV) {
+        const double inv_log2e = 0.69314718055994530942;
+        double* dp = dpart + ((size_t)chunk_id * a.n_slots + slot) * kNdBasic;
+        dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
+#pragma unroll
+        for (int j = 0; j < 6; j++) dp[3 + j] = du[j] * inv_log2e;
+        dp[9] = docc;
+    }
+}
+
 // One workgroup = kWavesPerBlock atom waves x one reflection chunk, over a 1-D grid of
 // kXcds * per_xcd workgroups: hardware id b runs on XCD b % kXcds and is mapped to logical block
 // (b % kXcds) * per_xcd + b / kXcds, so each XCD walks a contiguous range of chunks and every
 // record of a chunk is fetched into one L2 only.
-template <bool MIXED, bool DERIV, bool MP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && !MP ? 5 : 1))) void k_main(MainArgs a)
+template <bool MIXED, bool DERIV, bool MP, bool DIAG>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && !MP ? (DIAG ? 4 : 5) : 1))) void k_main(MainArgs a)
 {
+    static_assert(MIXED || !DIAG, "the parity-folded kernels are mixed precision only");
     constexpr int kStageBytes = MIXED ? kStageBytesF32 : kStageBytesF64;
     __shared__ __align__(16) unsigned char stage_lds[kWavesPerBlock * kStageBytes];
     const int b = blockIdx.x;
@@ -711,25 +928,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && !M
     const int cls = __builtin_amdgcn_readfirstlane(a.wave_l[wave]);
     const unsigned* dtdf32 = reinterpret_cast<const unsigned*>(a.dtdf);
     void* stage = stage_lds + wib * kStageBytes;
+    const unsigned* ht32 = reinterpret_cast<const unsigned*>(a.ht);
     if (MIXED) {
         const float* hop = static_cast<const float*>(a.hklop);
         const float* rad = static_cast<const float*>(a.radial);
         const float* radx = static_cast<const float*>(a.radx);
         const unsigned* phs = reinterpret_cast<const unsigned*>(a.phase);
         float2* st = static_cast<float2*>(stage);
-        if (cls <= 2)
+        if (DIAG) {
+            if (cls <= 2)
+                body_mixed_diag<2, DERIV>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, ht32, dtdf32, st, a.fpart,
+                                          a.dpart);
+            else
+                body_mixed_diag<4, DERIV>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, ht32, dtdf32, st, a.fpart,
+                                          a.dpart);
+        } else if (cls <= 2) {
             body_mixed<2, DERIV, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, phs, dtdf32, st, a.fpart, a.dpart);
-        else
+        } else {
             body_mixed<4, DERIV, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, phs, dtdf32, st, a.fpart, a.dpart);
+        }
     } else {
         const unsigned* hop = static_cast<const unsigned*>(a.hklop);
         const double* rad = static_cast<const double*>(a.radial);
         const double* radx = static_cast<const double*>(a.radx);
         double2* st = static_cast<double2*>(stage);
-        if (cls <= 2)
+        if (cls <= 2) {
             body_fp64<2, DERIV, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, dtdf32, st, a.fpart, a.dpart);
-        else
+        } else {
             body_fp64<4, DERIV, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, dtdf32, st, a.fpart, a.dpart);
+        }
     }
 }
 
@@ -918,6 +1145,15 @@ static std::vector<TypeRec> type_records(const EngineModel& model)
         r.kappa = model.types[t].kappa;
         r.kappa_p = model.types[t].kappa_prime;
         r.zeta = model.dv_zeta[r.wfn];
+        r.inv_kappa = 1.0 / r.kappa;
+        r.inv_kappa_p = 1.0 / r.kappa_p;
+        for (int l = 0; l < 5; l++) {
+            double norm = 1.0;
+            for (int q = 0; q < r.n[l] + 3; q++) norm *= r.zeta;
+            for (int q = 2; q <= r.n[l] + 2; q++) norm /= q;
+            const double sgn = (l == 2 || l == 3) ? -1.0 : 1.0;   // i^l: +, +i, -, -i, +
+            r.gscale[l] = sgn * 12.566370614359172953850 * norm;
+        }
         for (int l = 0; l <= r.max_l; l++)
             for (int i = l * l; i < (l + 1) * (l + 1); i++) r.plm[i] = model.types[t].plm[i];
     }
@@ -999,6 +1235,26 @@ Engine::Engine(int device, const EngineModel& model, const double* cell6, const 
         std::memcpy(od[s].tf, ops[s].tf, sizeof(od[s].tf));
         std::memcpy(od[s].rc, ops[s].rc, sizeof(od[s].rc));
         od[s].anom_sign = ops[s].anom_sign;
+    }
+    // every operation diagonal with equal fractional / Cartesian signs -> parity-class path
+    diag_ = !ops.empty() && ops.size() <= 8;
+    for (size_t s = 0; s < ops.size() && diag_; s++) {
+        int pat = 0;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                const int f = ops[s].rf[3 * i + j];
+                const double c = ops[s].rc[3 * i + j];
+                if (i != j) {
+                    if (f != 0 || std::fabs(c) > 1e-12) diag_ = false;
+                } else {
+                    if ((f != 1 && f != -1) || std::fabs(c - f) > 1e-12) diag_ = false;
+                    if (f == -1) pat |= 1 << i;
+                }
+            }
+        for (size_t q = 0; q < s && diag_; q++)
+            if (diag_pat_[q] == pat) diag_ = false;    // each sign pattern at most once
+        diag_pat_[s] = pat;
+        diag_anom_[s] = (float)ops[s].anom_sign;
     }
     d_ops_.ensure(od.size() * sizeof(OpDev) + 9 * sizeof(double));
     HIP_CHECK(hipMemcpy(d_ops_.p, od.data(), od.size() * sizeof(OpDev), hipMemcpyHostToDevice));
@@ -1115,9 +1371,16 @@ void Engine::run_prep(Precision prec, bool mp)
                                                          model_.n_types * sizeof(TypeRec));
     const AtomIn* ain = d_atoms_in_.as<AtomIn>();
     const int T = 256;
-    hipLaunchKernelGGL(k_hkl_prep, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, d_hkl_.as<int>(), ops, n_ops,
-                       n_hkl, c2f, mixed ? nullptr : d_hklop_.as<double>(), mixed ? d_hklop_.as<float>() : nullptr,
-                       mixed ? d_phase_.as<double>() : nullptr, hlen.as<double>());
+    if (use_diag(mixed, mp)) {
+        d_ht_.ensure(std::max<size_t>(n_rec, 1) * sizeof(double));
+        hipLaunchKernelGGL(k_hkl_prep_diag, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, d_hkl_.as<int>(), ops,
+                           n_ops, n_hkl, c2f, d_hklop_.as<float>(), d_phase_.as<double>(), d_ht_.as<double>(),
+                           hlen.as<double>());
+    } else {
+        hipLaunchKernelGGL(k_hkl_prep, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, d_hkl_.as<int>(), ops, n_ops,
+                           n_hkl, c2f, mixed ? nullptr : d_hklop_.as<double>(), mixed ? d_hklop_.as<float>() : nullptr,
+                           mixed ? d_phase_.as<double>() : nullptr, hlen.as<double>());
+    }
     HIP_CHECK(hipGetLastError());
     if (model_.n_types > 0) {
         d_fcore_.ensure((size_t)std::max(model_.n_wfn, 1) * n_hkl * sizeof(double));
@@ -1139,22 +1402,23 @@ void Engine::run_prep(Precision prec, bool mp)
     HIP_CHECK(hipGetLastError());
 }
 
-static const void* main_kernel(bool mixed, bool derivs, bool mp)
+static const void* main_kernel(bool mixed, bool derivs, bool mp, bool diag)
 {
     if (mixed) {
-        if (mp) return (const void*)k_main<true, true, true>;
-        return derivs ? (const void*)k_main<true, true, false> : (const void*)k_main<true, false, false>;
+        if (mp) return (const void*)k_main<true, true, true, false>;
+        if (diag) return derivs ? (const void*)k_main<true, true, false, true> : (const void*)k_main<true, false, false, true>;
+        return derivs ? (const void*)k_main<true, true, false, false> : (const void*)k_main<true, false, false, false>;
     }
-    if (mp) return (const void*)k_main<false, true, true>;
-    return derivs ? (const void*)k_main<false, true, false> : (const void*)k_main<false, false, false>;
+    if (mp) return (const void*)k_main<false, true, true, false>;
+    return derivs ? (const void*)k_main<false, true, false, false> : (const void*)k_main<false, false, false, false>;
 }
 
 int Engine::resident_waves(bool mixed, bool derivs, bool mp)
 {
-    int key = (mixed ? 4 : 0) + (derivs ? 2 : 0) + (mp ? 1 : 0);
+    const int key = (use_diag(mixed, mp) ? 8 : 0) + (mixed ? 4 : 0) + (derivs ? 2 : 0) + (mp ? 1 : 0);
     if (resident_waves_[key] > 0) return resident_waves_[key];
     int blocks = 0;
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, main_kernel(mixed, derivs, mp), 256, 0));
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, main_kernel(mixed, derivs, mp, use_diag(mixed, mp)), 256, 0));
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, device_));
     resident_waves_[key] = std::max(1, blocks) * kWavesPerBlock * prop.multiProcessorCount;
@@ -1204,16 +1468,22 @@ void Engine::launch_main(Precision prec, bool derivs, bool mp, int chunk)
     last_grid_x = a.grid_x;
     last_grid_y = n_chunks;
     last_chunk = chunk;
-    if (mixed) {
-        if (mp) hipLaunchKernelGGL((k_main<true, true, true>), grid, dim3(256), 0, stream_, a);
-        else if (derivs) hipLaunchKernelGGL((k_main<true, true, false>), grid, dim3(256), 0, stream_, a);
-        else hipLaunchKernelGGL((k_main<true, false, false>), grid, dim3(256), 0, stream_, a);
-    } else {
-        if (mp) hipLaunchKernelGGL((k_main<false, true, true>), grid, dim3(256), 0, stream_, a);
-        else if (derivs) hipLaunchKernelGGL((k_main<false, true, false>), grid, dim3(256), 0, stream_, a);
-        else hipLaunchKernelGGL((k_main<false, false, false>), grid, dim3(256), 0, stream_, a);
+    const bool diag = use_diag(mixed, mp);
+    a.ht = diag ? d_ht_.as<double>() : nullptr;
+    a.diag_mask = 0;
+    for (int p = 0; p < 8; p++) {
+        a.diag_op[p] = 0;
+        a.diag_anom[p] = 0.f;
     }
-    HIP_CHECK(hipGetLastError());
+    if (diag)
+        for (int s = 0; s < (int)ops_.size(); s++) {
+            a.diag_mask |= 1 << diag_pat_[s];
+            a.diag_op[diag_pat_[s]] = s;
+            a.diag_anom[diag_pat_[s]] = diag_anom_[s];
+        }
+    const void* kern = main_kernel(mixed, derivs, mp, diag);
+    void* args[] = {&a};
+    HIP_CHECK(hipLaunchKernel(kern, grid, dim3(256), args, 0, stream_));
 }
 
 int Engine::chunk_for(Precision prec, bool derivs, bool mp)
@@ -1226,8 +1496,8 @@ int Engine::chunk_for(Precision prec, bool derivs, bool mp)
 
 long long Engine::shape_key(Precision prec, bool derivs, bool mp) const
 {
-    return (((long long)n_sorted_ * 1000003LL + n_hkl_) * 64 + (long long)ops_.size()) * 8 +
-           (prec == kMixed ? 4 : 0) + (derivs ? 2 : 0) + (mp ? 1 : 0);
+    return (((long long)n_sorted_ * 1000003LL + n_hkl_) * 64 + (long long)ops_.size()) * 16 +
+           (use_diag(prec == kMixed, mp) ? 8 : 0) + (prec == kMixed ? 4 : 0) + (derivs ? 2 : 0) + (mp ? 1 : 0);
 }
 
 void Engine::run(Precision prec, bool derivs, bool mp)

@@ -30,6 +30,7 @@ struct DeviceBuffer {
 
 struct TuneParams {
     int hkl_chunk = 0;       // reflections per workgroup column; 0 = automatic
+    bool general_path = false;  // force the general-rotation kernels for diagonal groups (tests)
 };
 
 // Atom data for one call, in the order the caller supplies them (already converted to the
@@ -88,7 +89,13 @@ private:
     int chunk_for(Precision prec, bool derivs, bool mp);
     long long shape_key(Precision prec, bool derivs, bool mp) const;
     int type_class(int t) const;
-    int resident_waves_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // parity-folded kernels: mixed precision only (the FP64 variant needs twice the registers for
+    // its class sums and loses more to occupancy than it saves)
+    bool use_diag(bool mixed, bool mp) const { return diag_ && mixed && !mp && !tune.general_path; }
+    int resident_waves_[16] = {};
+    bool diag_ = false;               // every operation is a diagonal sign matrix
+    int diag_pat_[8] = {};
+    float diag_anom_[8] = {};
     int last_nd_ = 10;
     std::map<long long, int> tuned_;
     int device_;
@@ -105,7 +112,7 @@ private:
     DeviceBuffer d_core_, d_val_, d_type_, d_ops_;
     // per call
     DeviceBuffer d_hkl_, d_dtdf_, d_atoms_in_, d_atoms_, d_hklop_, d_hklop32_, d_phase_, d_radial_,
-        d_radial32_, d_fpart_, d_dpart_, d_f_, d_d_, d_order_, d_fcore_, d_radx_;
+        d_radial32_, d_ht_, d_fpart_, d_dpart_, d_f_, d_d_, d_order_, d_fcore_, d_radx_;
 };
 
 }  // namespace discamb_amd

@@ -24,6 +24,10 @@ SPACE_GROUPS = {
     "C2/c": ["x,y,z", "-x,y,-z+1/2", "x+1/2,y+1/2,z", "-x+1/2,y+1/2,-z+1/2", "-x,-y,-z", "x,-y,z+1/2",
              "-x+1/2,-y+1/2,-z", "x+1/2,-y+1/2,z+1/2"],
     "P212121": ["x,y,z", "-x+1/2,-y,z+1/2", "-x,y+1/2,-z+1/2", "x+1/2,-y+1/2,-z"],
+    # non-diagonal rotations (general-rotation kernels)
+    "P43212": ["x,y,z", "-x,-y,z+1/2", "-y+1/2,x+1/2,z+3/4", "y+1/2,-x+1/2,z+1/4", "-x+1/2,y+1/2,-z+3/4",
+               "x+1/2,-y+1/2,-z+1/4", "y,x,-z", "-y,-x,-z+1/2"],
+    "P31": ["x,y,z", "-y,x-y,z+1/3", "-x+y,-x,z+2/3"],
 }
 
 
@@ -236,6 +240,10 @@ def small_structure(sg, n_atoms=12, d_min=1.0, seed=7, cell=None, lmax_heavy=4):
             cell = (7.5, 8.0, 9.5, 92.0, 104.0, 88.0)
         elif sg in ("P21/c", "C2/c"):
             cell = (8.0, 9.0, 10.0, 90.0, 105.0, 90.0)
+        elif sg == "P43212":
+            cell = (8.5, 8.5, 10.0, 90.0, 90.0, 90.0)
+        elif sg == "P31":
+            cell = (8.5, 8.5, 10.0, 90.0, 90.0, 120.0)
         else:
             cell = (8.0, 9.0, 10.0, 90.0, 90.0, 90.0)
     els = ["C", "N", "O", "S", "H", "H"]

@@ -117,6 +117,11 @@ void dh_hc_destroy(dh_calculator* calc);
 int dh_hc_set_calculation_mode(dh_calculator* calc, const char* mode);
 /* Launch tuning: reflections per workgroup column (0 = automatic). */
 int dh_hc_set_tuning(dh_calculator* calc, int hkl_chunk);
+/* Kernel-path options (no reference counterpart; for tests and diagnosis).
+ * DH_OPT_GENERAL_ROTATIONS: use the general-rotation kernels even when every symmetry operation is
+ * a diagonal sign matrix (the engine otherwise folds such operations through harmonic parities). */
+#define DH_OPT_GENERAL_ROTATIONS 1u
+int dh_hc_set_options(dh_calculator* calc, unsigned flags);
 
 /* calculateStructureFactorsAndDerivatives: synchronous, host pointers. hkl is 3*n_hkl int,
  * dtarget_df 2*n_hkl (dT/dA + i dT/dB, not conjugated). selector: DH_D_* bits; 0 = F only. */

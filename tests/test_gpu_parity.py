@@ -16,13 +16,14 @@ pytestmark = pytest.mark.gpu
 TOL = {"HIP": (1e-10, 1e-10), "HIP_MIXED": (1e-6, 1e-5)}
 
 
-def _compare(w, mode, atoms=None, include=None, anomalous=None, hkl=None, dtdf=None, tol=None):
+def _compare(w, mode, atoms=None, include=None, anomalous=None, hkl=None, dtdf=None, tol=None, general=False):
     atoms = atoms or w.crystal.atoms
     hkl = w.hkl if hkl is None else hkl
     dtdf = w.dtdf if dtdf is None else dtdf
-    print("case %s mode=%s atoms=%d hkl=%d" % (w.name, mode, atoms.n, len(hkl)), flush=True)
+    print("case %s mode=%s general=%s atoms=%d hkl=%d" % (w.name, mode, general, atoms.n, len(hkl)), flush=True)
     calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
     calc.setCalculationMode(mode)
+    calc.setKernelOptions(general_rotations=general)
     f, d = calc.calculateStructureFactorsAndDerivatives(atoms, w.lcs, hkl, dtdf, include, anomalous=anomalous)
     rf, rx, ru, ro = hc_oracle.sf_and_derivs(w.crystal, w.params, atoms, w.lcs, hkl, dtdf, include=include,
                                              n_threads=16, anomalous=anomalous)
@@ -36,24 +37,29 @@ def _compare(w, mode, atoms=None, include=None, anomalous=None, hkl=None, dtdf=N
     return f, d
 
 
+# "general" forces the general-rotation kernels; triclinic, monoclinic and orthorhombic groups
+# otherwise run the parity-folded kernels for diagonal operations (P43212 and P31 always general).
+@pytest.mark.parametrize("general", [False, True])
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
 @pytest.mark.parametrize("sg", list(synth.SPACE_GROUPS))
-def test_space_groups(sg, mode):
-    _compare(synth.small_structure(sg, n_atoms=14, d_min=0.9), mode)
+def test_space_groups(sg, mode, general):
+    _compare(synth.small_structure(sg, n_atoms=14, d_min=0.9), mode, general=general)
 
 
-def test_shifted_inversion_centre():
+@pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
+def test_shifted_inversion_centre(mode):
     w = synth.small_structure("P-1", n_atoms=10, d_min=1.0)
     w.crystal.symops = ["x,y,z", "-x+1/2,-y,-z+1/2"]
-    _compare(w, "HIP")
+    _compare(w, mode)
 
 
-def test_body_centred():
+@pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
+def test_body_centred(mode):
     w = synth.small_structure("P212121", n_atoms=8, d_min=1.1)
     w.crystal.symops = ["x,y,z", "-x,-y,z", "-x,y,-z", "x,-y,-z", "x+1/2,y+1/2,z+1/2", "-x+1/2,-y+1/2,z+1/2",
                         "-x+1/2,y+1/2,-z+1/2", "x+1/2,-y+1/2,-z+1/2"]
     w.crystal.atoms.multiplicity[:] = 8
-    _compare(w, "HIP")
+    _compare(w, mode)
 
 
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])

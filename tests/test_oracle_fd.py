@@ -19,7 +19,7 @@ def target(w, atoms, dtdf):
     return float(np.sum((np.conj(dtdf) * F).real))
 
 
-@pytest.mark.parametrize("sg", ["P1", "P-1", "P21/c", "C2/c", "P212121", "P-1_shifted"])
+@pytest.mark.parametrize("sg", ["P1", "P-1", "P21/c", "C2/c", "P212121", "P-1_shifted", "P43212", "P31"])
 def test_oracle_derivatives_match_finite_differences(sg):
     shifted = sg == "P-1_shifted"
     w = synth.small_structure("P-1" if shifted else sg, n_atoms=5, d_min=1.6)
@@ -118,3 +118,15 @@ def test_deformation_density_fourier_transform():
             integrand = (radial[:, None] * r[:, None] ** 2) * phase * ang[None, :]
             val = np.sum(integrand * wa[None, :]) * dr
             assert abs(val - F[k]) < 2e-5 * max(1e-3, abs(F).max()), (l, k, val, F[k])
+
+
+@pytest.mark.parametrize("sg", list(synth.SPACE_GROUPS))
+def test_oracle_symmetry_folding_matches_the_full_group(sg):
+    """The reference's folding (coset representatives, 2 Re for centric-at-origin, centering
+    multipliers) against an explicit sum over every operation of the group."""
+    w = synth.small_structure(sg, n_atoms=6, d_min=1.3)
+    F, dx, du, do = hc_oracle.sf_and_derivs(w.crystal, w.params, w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+    G, gx, gu, go = hc_oracle.sf_and_derivs(w.crystal, w.params, w.crystal.atoms, w.lcs, w.hkl, w.dtdf,
+                                            mode="full_group")
+    assert rel_err(F, G) < 1e-13
+    assert rel_err(dx, gx) < 1e-12 and rel_err(du, gu) < 1e-12 and rel_err(do, go) < 1e-12

@@ -22,6 +22,21 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK = 157.3e12      # MI355X published FP32 vector peak (FLOP/s)
 FP64_PEAK = 78.6e12
+# HBM bytes per launch of the main kernels on this workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# passes scaled by a plain-copy calibration (profiles/*_hbm_traffic.md, "bench-traffic" lines)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_main_p212121_5000at_1.2A_hbm_traffic.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch recorded for `kernel` at this workload, or None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                if line.startswith("bench-traffic ") and line.split()[1] == kernel:
+                    return float(line.split()[2])
+    except OSError:
+        pass
+    return None
 
 
 def flops_per_term(L):
@@ -131,6 +146,7 @@ def main():
            "d_adp": rel(dmx.adp_derivatives, d64.adp_derivatives),
            "d_occ": rel(dmx.occupancy_derivatives, d64.occupancy_derivatives)}
     achieved = flops / (kmx * 1e-3)
+    kname = "k_main<mixed,deriv,diag>"    # P2_1 2_1 2_1: every operation is a diagonal sign matrix
     line = {
         "metric": "atom*symop*hkl terms/s for F+derivs (mixed precision)",
         "value": terms / smx,
@@ -149,8 +165,9 @@ def main():
                    "terms_per_step": terms, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "roof": "FP32 vector ALU (the path uses VALU, not matrix cores)",
                      "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK, "traffic": None,
-                     "kernel": "k_main<mixed,deriv>", "kernel_ms": kmx,
+                     "frac": achieved / FP32_PEAK, "traffic": measured_traffic(kname),
+                     "frac_of_unpacked_fp32": achieved / (FP32_PEAK / 2),
+                     "kernel": kname, "kernel_ms": kmx,
                      "flops_per_launch": flops,
                      "flops_per_term": "66 + 2(L+1)^2 + 2(L+1): 126 (L=4), 90 (L=2)"},
         "fp64": {"value": terms / s64, "unit": "terms/s", "ms_per_step": s64 * 1e3, "kernel_ms": k64,

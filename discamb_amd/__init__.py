@@ -7,9 +7,9 @@ same calculator interface; see include/discamb_amd.h for the C ABI.
 from .model import (AtomSet, Crystal, DerivativesSelector, HC_AtomTypeParameters, HC_ModelParameters,
                     HC_WfnType, SlaterOrbital, TargetFunctionAtomicParamDerivatives, TypeParameterDerivatives,
                     parse_symop, wfn_type_from_bank)
-from .calculator import HansenCoppensStructureFactorCalculator
+from .calculator import BatchCalculator, HansenCoppensStructureFactorCalculator
 
 __all__ = ["AtomSet", "Crystal", "DerivativesSelector", "HC_AtomTypeParameters", "HC_ModelParameters",
            "HC_WfnType", "SlaterOrbital", "TargetFunctionAtomicParamDerivatives", "TypeParameterDerivatives",
            "parse_symop",
-           "wfn_type_from_bank", "HansenCoppensStructureFactorCalculator"]
+           "wfn_type_from_bank", "HansenCoppensStructureFactorCalculator", "BatchCalculator"]

@@ -80,6 +80,12 @@ SIGNATURES = {
     "dh_hc_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "dh_hc_last_timing": (ctypes.c_int, [ctypes.c_void_p, c_dbl_p, c_int_p, c_int_p, c_int_p]),
     "dh_hc_engine_info": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p, c_int_p, ctypes.c_char_p]),
+    "dh_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "dh_batch_destroy": (None, [ctypes.c_void_p]),
+    "dh_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_int]),
+    "dh_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, c_int_p]),
+    "dh_batch_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "dh_batch_last_timing": (ctypes.c_int, [ctypes.c_void_p, c_dbl_p, c_int_p, c_int_p, c_int_p]),
 }
 
 _lib = None

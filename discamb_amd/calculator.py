@@ -258,3 +258,45 @@ class HansenCoppensStructureFactorCalculator:
         _lib.check(self._lib.dh_hc_last_timing(self._h, ctypes.byref(ms), ctypes.byref(gx), ctypes.byref(gy),
                                                ctypes.byref(ch)))
         return {"main_ms": ms.value, "grid": (gx.value, gy.value), "chunk": ch.value}
+
+
+class BatchCalculator:
+    """Many calculators computed as one step: the per-structure prep, a single persistent F +
+    adjoint kernel over every structure and the stage-2 reductions, optionally replayed from a
+    captured hipGraph.  Inputs are uploaded per calculator (``calc.upload``), results read per
+    calculator (``calc.fetch``).  No reference counterpart; it replaces a loop of
+    calculateStructureFactorsAndDerivatives calls over small structures."""
+
+    def __init__(self, calculators):
+        self._calcs = list(calculators)           # the batch borrows them
+        self._lib = _lib.load()
+        arr = (ctypes.c_void_p * len(self._calcs))(*[c._h.value for c in self._calcs])
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.dh_batch_create(arr, len(self._calcs), ctypes.byref(h)))
+        self._b = h
+
+    def __del__(self):
+        b = getattr(self, "_b", None)
+        if b is not None and b.value:
+            self._lib.dh_batch_destroy(b)
+            self._b = None
+
+    def run(self, derivativesSelector: DerivativesSelector = None, graph: bool = True):
+        sel = (derivativesSelector or DerivativesSelector()).bits()
+        _lib.check(self._lib.dh_batch_run(self._b, sel, 1 if graph else 0))
+
+    def autotune(self, derivativesSelector: DerivativesSelector = None):
+        sel = (derivativesSelector or DerivativesSelector()).bits()
+        c = ctypes.c_int()
+        _lib.check(self._lib.dh_batch_autotune(self._b, sel, ctypes.byref(c)))
+        return c.value
+
+    def setTuning(self, hkl_chunk: int):
+        _lib.check(self._lib.dh_batch_set_tuning(self._b, int(hkl_chunk)))
+
+    def lastTiming(self):
+        ms = ctypes.c_double()
+        ch, un, gr = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(self._lib.dh_batch_last_timing(self._b, ctypes.byref(ms), ctypes.byref(ch), ctypes.byref(un),
+                                                  ctypes.byref(gr)))
+        return {"step_ms": ms.value, "chunk": ch.value, "units": un.value, "grid": gr.value}

@@ -300,6 +300,11 @@ struct dh_calculator {
     Calculator* c;
 };
 
+struct dh_batch {
+    std::vector<Calculator*> calcs;
+    std::unique_ptr<BatchEngine> engine;
+};
+
 template <typename F>
 static int guarded(F&& f)
 {
@@ -485,6 +490,76 @@ int dh_hc_engine_info(dh_calculator* calc, int* n_ops, int* order, int* centro, 
         if (order) *order = calc->c->sg.order;
         if (centro) *centro = calc->c->sg.centro ? 1 : 0;
         if (centering) *centering = calc->c->sg.centering;
+    });
+}
+
+int dh_batch_create(dh_calculator* const* calcs, int n, dh_batch** out)
+{
+    return guarded([&] {
+        if (!out || !calcs || n <= 0) throw Error(DH_E_ARGUMENT, "invalid batch arguments");
+        *out = nullptr;
+        std::unique_ptr<dh_batch> b(new dh_batch);
+        std::vector<Engine*> eng;
+        for (int i = 0; i < n; i++) {
+            if (!calcs[i]) throw Error(DH_E_ARGUMENT, "null calculator in batch");
+            b->calcs.push_back(calcs[i]->c);
+            eng.push_back(calcs[i]->c->engine.get());
+        }
+        b->engine.reset(new BatchEngine(eng));
+        *out = b.release();
+    });
+}
+
+void dh_batch_destroy(dh_batch* batch) { delete batch; }
+
+static Precision batch_precision(dh_batch* b)
+{
+    const Precision p = b->calcs[0]->prec;
+    for (Calculator* c : b->calcs)
+        if (c->prec != p) throw Error(DH_E_ARGUMENT, "calculators of a batch must share the calculation mode");
+    return p;
+}
+
+int dh_batch_run(dh_batch* batch, unsigned selector, int use_graph)
+{
+    return guarded([&] {
+        if (!batch) throw Error(DH_E_ARGUMENT, "null batch");
+        if (selector & ~(unsigned)DH_D_ALL) throw Error(DH_E_ARGUMENT, "multipole derivatives are not batched");
+        const Precision p = batch_precision(batch);
+        batch->engine->run(p, selector != 0, use_graph != 0);
+        for (Calculator* c : batch->calcs) {
+            c->last_selector = selector;
+            c->have_mp = false;
+        }
+    });
+}
+
+int dh_batch_autotune(dh_batch* batch, unsigned selector, int* chunk)
+{
+    return guarded([&] {
+        if (!batch) throw Error(DH_E_ARGUMENT, "null batch");
+        if (selector & ~(unsigned)DH_D_ALL) throw Error(DH_E_ARGUMENT, "multipole derivatives are not batched");
+        const int c = batch->engine->autotune(batch_precision(batch), selector != 0);
+        if (chunk) *chunk = c;
+    });
+}
+
+int dh_batch_set_tuning(dh_batch* batch, int hkl_chunk)
+{
+    return guarded([&] {
+        if (!batch || hkl_chunk < 0) throw Error(DH_E_ARGUMENT, "invalid tuning");
+        batch->engine->hkl_chunk = hkl_chunk;
+    });
+}
+
+int dh_batch_last_timing(dh_batch* batch, double* step_ms, int* chunk, int* units, int* grid)
+{
+    return guarded([&] {
+        if (!batch) throw Error(DH_E_ARGUMENT, "null batch");
+        if (step_ms) *step_ms = batch->engine->last_ms;
+        if (chunk) *chunk = batch->engine->last_chunk;
+        if (units) *units = batch->engine->last_units;
+        if (grid) *grid = batch->engine->last_grid;
     });
 }
 

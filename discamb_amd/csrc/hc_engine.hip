@@ -107,12 +107,11 @@ struct OpDev {
     double anom_sign;
 };
 
-__global__ void k_hkl_prep(const int* __restrict__ hkl, const OpDev* __restrict__ ops, int n_ops, int n_hkl,
-                           const double* __restrict__ c2f, double* __restrict__ rec64, float* __restrict__ rec32,
-                           double* __restrict__ phase32, double* __restrict__ hlen)
+__device__ __forceinline__ void hkl_prep_item(int k, const int* __restrict__ hkl, const OpDev* __restrict__ ops,
+                                              int n_ops, const double* __restrict__ c2f, double* __restrict__ rec64,
+                                              float* __restrict__ rec32, double* __restrict__ phase32,
+                                              double* __restrict__ hlen)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_hkl) return;
     const int h0 = hkl[3 * k], h1 = hkl[3 * k + 1], h2 = hkl[3 * k + 2];
     double hc[3];
     for (int j = 0; j < 3; j++) hc[j] = h0 * c2f[j] + h1 * c2f[3 + j] + h2 * c2f[6 + j];
@@ -161,12 +160,12 @@ __global__ void k_hkl_prep(const int* __restrict__ hkl, const OpDev* __restrict_
 
 // Diagonal groups (mixed precision): one FP32 record per reflection [d_j(h) x 25, m(h) log2(e) x 6,
 // h x 3, pad x 2], an FP64 [h, 0] phase record, h.t_s per operation, and |h|.
-__global__ void k_hkl_prep_diag(const int* __restrict__ hkl, const OpDev* __restrict__ ops, int n_ops, int n_hkl,
-                                const double* __restrict__ c2f, float* __restrict__ rec32, double* __restrict__ hd,
-                                double* __restrict__ ht, double* __restrict__ hlen)
+__device__ __forceinline__ void hkl_prep_diag_item(int k, const int* __restrict__ hkl,
+                                                   const OpDev* __restrict__ ops, int n_ops,
+                                                   const double* __restrict__ c2f, float* __restrict__ rec32,
+                                                   double* __restrict__ hd, double* __restrict__ ht,
+                                                   double* __restrict__ hlen)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_hkl) return;
     const int h0 = hkl[3 * k], h1 = hkl[3 * k + 1], h2 = hkl[3 * k + 2];
     double hc[3];
     for (int j = 0; j < 3; j++) hc[j] = h0 * c2f[j] + h1 * c2f[3 + j] + h2 * c2f[6 + j];
@@ -193,11 +192,10 @@ __global__ void k_hkl_prep_diag(const int* __restrict__ hkl, const OpDev* __rest
 }
 
 // f_core(|h|) per (wavefunction type, reflection): 4 pi sum_k c_k g_0(p_k + 2, |h|, e_k)
-__global__ void k_core_prep(const double* __restrict__ hlen, const WfnRec* __restrict__ wfns,
-                            const TermRec* __restrict__ core, int n_wfn, int n_hkl, double* __restrict__ fcore)
+__device__ __forceinline__ void core_prep_item(int idx, const double* __restrict__ hlen,
+                                               const WfnRec* __restrict__ wfns, const TermRec* __restrict__ core,
+                                               int n_hkl, double* __restrict__ fcore)
 {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n_hkl * n_wfn) return;
     int w = idx / n_hkl, k = idx - w * n_hkl;       // wfn-major: a wave shares one term list
     const WfnRec wf = wfns[w];
     const double h = hlen[k];
@@ -212,14 +210,13 @@ __global__ void k_core_prep(const double* __restrict__ hlen, const WfnRec* __res
 // Per (reflection, type): S = f_core + Pval f_val(h/kappa) and
 // G_l = 4 pi (i^l sign) N_l g_l(n_l + 2, h/kappa', zeta), l <= lp.  With radx set, also the
 // multipole-adjoint record [f_val, dS/dkappa, dG_0..4/dkappa', 0].
-__global__ void k_radial_prep(const double* __restrict__ hlen, const double* __restrict__ fcore,
-                              const TypeRec* __restrict__ types, const WfnRec* __restrict__ wfns,
-                              const TermRec* __restrict__ val, int n_types, int n_hkl,
-                              double* __restrict__ rad64, float* __restrict__ rad32,
-                              double* __restrict__ radx64, float* __restrict__ radx32)
+__device__ __forceinline__ void radial_prep_item(int idx, const double* __restrict__ hlen,
+                                                 const double* __restrict__ fcore, const TypeRec* __restrict__ types,
+                                                 const WfnRec* __restrict__ wfns, const TermRec* __restrict__ val,
+                                                 int n_types, int n_hkl, double* __restrict__ rad64,
+                                                 float* __restrict__ rad32, double* __restrict__ radx64,
+                                                 float* __restrict__ radx32)
 {
-    int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n_hkl * n_types) return;
     int t = idx / n_hkl, k = idx - t * n_hkl;       // type-major: a wave shares one type
     const double four_pi = 12.566370614359172953850;
     const double h = hlen[k];
@@ -270,12 +267,10 @@ __global__ void k_radial_prep(const double* __restrict__ hlen, const double* __r
 // One 64-lane workgroup per atom slot: lanes 0..49 evaluate the local-frame multipole expansion
 // f(u_p) = sum_j P_j d_j(L^T u_p) at the projection nodes, lanes 0..24 project it back onto the
 // global-frame harmonics.
-__global__ __launch_bounds__(64) void k_atom_prep(const AtomIn* __restrict__ in, const TypeRec* __restrict__ types,
-                                                  AtomRec* __restrict__ out, int n_slots)
+// one 64-lane workgroup (lane p) per atom slot i; fsh: 64 doubles of LDS
+__device__ __forceinline__ void atom_prep_item(int i, int p, double* fsh, const AtomIn* __restrict__ in,
+                                               const TypeRec* __restrict__ types, AtomRec* __restrict__ out)
 {
-    const int i = blockIdx.x;
-    const int p = threadIdx.x;
-    __shared__ double fsh[64];
     const AtomIn& a = in[i];
     AtomRec& r = out[i];
     const bool real_atom = a.orig >= 0;
@@ -314,6 +309,47 @@ __global__ __launch_bounds__(64) void k_atom_prep(const AtomIn* __restrict__ in,
         r.type = a.type;
         r.orig = a.orig;
     }
+}
+
+__global__ void k_hkl_prep(const int* __restrict__ hkl, const OpDev* __restrict__ ops, int n_ops, int n_hkl,
+                           const double* __restrict__ c2f, double* __restrict__ rec64, float* __restrict__ rec32,
+                           double* __restrict__ phase32, double* __restrict__ hlen)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_hkl) hkl_prep_item(k, hkl, ops, n_ops, c2f, rec64, rec32, phase32, hlen);
+}
+
+__global__ void k_hkl_prep_diag(const int* __restrict__ hkl, const OpDev* __restrict__ ops, int n_ops, int n_hkl,
+                                const double* __restrict__ c2f, float* __restrict__ rec32, double* __restrict__ hd,
+                                double* __restrict__ ht, double* __restrict__ hlen)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_hkl) hkl_prep_diag_item(k, hkl, ops, n_ops, c2f, rec32, hd, ht, hlen);
+}
+
+__global__ void k_core_prep(const double* __restrict__ hlen, const WfnRec* __restrict__ wfns,
+                            const TermRec* __restrict__ core, int n_wfn, int n_hkl, double* __restrict__ fcore)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n_hkl * n_wfn) core_prep_item(idx, hlen, wfns, core, n_hkl, fcore);
+}
+
+__global__ void k_radial_prep(const double* __restrict__ hlen, const double* __restrict__ fcore,
+                              const TypeRec* __restrict__ types, const WfnRec* __restrict__ wfns,
+                              const TermRec* __restrict__ val, int n_types, int n_hkl,
+                              double* __restrict__ rad64, float* __restrict__ rad32,
+                              double* __restrict__ radx64, float* __restrict__ radx32)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n_hkl * n_types)
+        radial_prep_item(idx, hlen, fcore, types, wfns, val, n_types, n_hkl, rad64, rad32, radx64, radx32);
+}
+
+__global__ __launch_bounds__(64) void k_atom_prep(const AtomIn* __restrict__ in, const TypeRec* __restrict__ types,
+                                                  AtomRec* __restrict__ out, int n_slots)
+{
+    __shared__ double fsh[64];
+    atom_prep_item(blockIdx.x, threadIdx.x, fsh, in, types, out);
 }
 
 // ------------------------------------------------------------------ main kernels
@@ -909,25 +945,18 @@ __device__ __forceinline__ void body_mixed_diag(const MainArgs& a, int wave, int
 // kXcds * per_xcd workgroups: hardware id b runs on XCD b % kXcds and is mapped to logical block
 // (b % kXcds) * per_xcd + b / kXcds, so each XCD walks a contiguous range of chunks and every
 // record of a chunk is fetched into one L2 only.
+// One (atom block, reflection chunk) work unit: wave wib of the block runs atom wave
+// ablock * kWavesPerBlock + wib over the reflections of chunk chunk_id.
 template <bool MIXED, bool DERIV, bool MP, bool DIAG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && !MP ? (DIAG ? 4 : 5) : 1))) void k_main(MainArgs a)
+__device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int ablock, int wib, int lane, void* stage)
 {
     static_assert(MIXED || !DIAG, "the parity-folded kernels are mixed precision only");
-    constexpr int kStageBytes = MIXED ? kStageBytesF32 : kStageBytesF64;
-    __shared__ __align__(16) unsigned char stage_lds[kWavesPerBlock * kStageBytes];
-    const int b = blockIdx.x;
-    const int logical = (b % kXcds) * a.per_xcd + b / kXcds;
-    if (logical >= a.n_blocks) return;
-    const int chunk_id = logical / a.grid_x;
-    const int wib = (int)(threadIdx.x / kWave);
-    const int wave = (logical - chunk_id * a.grid_x) * kWavesPerBlock + wib;
+    const int wave = ablock * kWavesPerBlock + wib;
     if (wave >= a.n_waves) return;
-    const int lane = threadIdx.x & (kWave - 1);
     const int k0 = chunk_id * a.chunk;
     const int k1 = min(k0 + a.chunk, a.n_hkl);
     const int cls = __builtin_amdgcn_readfirstlane(a.wave_l[wave]);
     const unsigned* dtdf32 = reinterpret_cast<const unsigned*>(a.dtdf);
-    void* stage = stage_lds + wib * kStageBytes;
     const unsigned* ht32 = reinterpret_cast<const unsigned*>(a.ht);
     if (MIXED) {
         const float* hop = static_cast<const float*>(a.hklop);
@@ -960,14 +989,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && !M
     }
 }
 
-// F(h) = mult(h) * sum_waves partial  (centric at origin: 2 Re, 0)
+// One workgroup = kWavesPerBlock atom waves x one reflection chunk, over a 1-D grid of
+// kXcds * per_xcd workgroups: hardware id b runs on XCD b % kXcds and is mapped to logical block
+// (b % kXcds) * per_xcd + b / kXcds, so each XCD walks a contiguous range of chunks and every
+// record of a chunk is fetched into one L2 only.
+template <bool MIXED, bool DERIV, bool MP, bool DIAG>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && !MP ? (DIAG ? 4 : 5) : 1))) void k_main(MainArgs a)
+{
+    constexpr int kStageBytes = MIXED ? kStageBytesF32 : kStageBytesF64;
+    __shared__ __align__(16) unsigned char stage_lds[kWavesPerBlock * kStageBytes];
+    const int b = blockIdx.x;
+    const int logical = (b % kXcds) * a.per_xcd + b / kXcds;
+    if (logical >= a.n_blocks) return;
+    const int chunk_id = logical / a.grid_x;
+    const int wib = (int)(threadIdx.x / kWave);
+    main_unit<MIXED, DERIV, MP, DIAG>(a, chunk_id, logical - chunk_id * a.grid_x, wib, threadIdx.x & (kWave - 1),
+                                      stage_lds + wib * kStageBytes);
+}
+
 // F(h) = mult(h) * sum_waves partial  (centric at origin: 2 Re, 0).  Loads are issued eight at a
 // time and added in wave order.
-__global__ void k_reduce_f(const double2* __restrict__ fpart, const double* __restrict__ mult, int n_waves,
-                           int n_hkl, int centric_origin, double* __restrict__ f)
+__device__ __forceinline__ void reduce_f_item(int k, const double2* __restrict__ fpart,
+                                              const double* __restrict__ mult, int n_waves, int n_hkl,
+                                              int centric_origin, double* __restrict__ f)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_hkl) return;
     double re = 0, im = 0;
     int w = 0;
     for (; w + 8 <= n_waves; w += 8) {
@@ -993,13 +1038,19 @@ __global__ void k_reduce_f(const double2* __restrict__ fpart, const double* __re
     f[2 * k + 1] = im * mult[k];
 }
 
+__global__ void k_reduce_f(const double2* __restrict__ fpart, const double* __restrict__ mult, int n_waves,
+                           int n_hkl, int centric_origin, double* __restrict__ f)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_hkl) reduce_f_item(k, fpart, mult, n_waves, n_hkl, centric_origin, f);
+}
+
 // d[orig][j] = sum over reflection chunks of the per-(chunk, slot) partials; one thread per
 // (slot, j), chunks added in order, loads issued eight at a time.
-__global__ void k_reduce_d(const double* __restrict__ dpart, const AtomRec* __restrict__ atoms, int n_chunks,
-                           int n_slots, int nd, double* __restrict__ d)
+__device__ __forceinline__ void reduce_d_item(int idx, const double* __restrict__ dpart,
+                                              const AtomRec* __restrict__ atoms, int n_chunks, int n_slots, int nd,
+                                              double* __restrict__ d)
 {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n_slots * nd) return;
     const int i = idx / nd;
     const int orig = atoms[i].orig;
     if (orig < 0) return;
@@ -1016,6 +1067,13 @@ __global__ void k_reduce_d(const double* __restrict__ dpart, const AtomRec* __re
     }
     for (; c < n_chunks; c++) s += p[(size_t)c * stride];
     d[(size_t)orig * nd + (idx - i * nd)] = s;
+}
+
+__global__ void k_reduce_d(const double* __restrict__ dpart, const AtomRec* __restrict__ atoms, int n_chunks,
+                           int n_slots, int nd, double* __restrict__ d)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n_slots * nd) reduce_d_item(idx, dpart, atoms, n_chunks, n_slots, nd, d);
 }
 
 // dT/dP_lm in the atom's local frame from the global-frame derivative: P_glob = D P_loc with
@@ -1351,19 +1409,31 @@ void Engine::fill(DeviceBuffer& b, size_t n, double v)
     HIP_CHECK(hipGetLastError());
 }
 
+// every per-call table of the prep kernels (|h| lives in d_radial32_)
+void Engine::ensure_prep_buffers(Precision prec, bool mp)
+{
+    const int n_types = std::max(model_.n_types, 1);
+    const int n_hkl = n_hkl_;
+    const bool mixed = prec == kMixed;
+    const size_t n_rec = (size_t)n_hkl * ops_.size();
+    d_radial_.ensure((size_t)n_hkl * n_types * kRadRec * (mixed ? sizeof(float) : sizeof(double)));
+    d_hklop_.ensure(n_rec * kHklopRec * (mixed ? sizeof(float) : sizeof(double)));
+    if (mixed) d_phase_.ensure(n_rec * 4 * sizeof(double));
+    d_radial32_.ensure((size_t)std::max(n_hkl, 1) * sizeof(double));
+    if (use_diag(mixed, mp)) d_ht_.ensure(std::max<size_t>(n_rec, 1) * sizeof(double));
+    d_fcore_.ensure((size_t)std::max(model_.n_wfn, 1) * std::max(n_hkl, 1) * sizeof(double));
+    if (mp) d_radx_.ensure((size_t)n_hkl * n_types * kRadRec * (mixed ? sizeof(float) : sizeof(double)));
+}
+
 void Engine::run_prep(Precision prec, bool mp)
 {
+    ensure_prep_buffers(prec, mp);
     const int n_ops = (int)ops_.size();
     const int n_types = std::max(model_.n_types, 1);
     const int n_hkl = n_hkl_;
     const int n_slots = n_sorted_;
     const bool mixed = prec == kMixed;
-    const size_t n_rec = (size_t)n_hkl * n_ops;
-    d_radial_.ensure((size_t)n_hkl * n_types * kRadRec * (mixed ? sizeof(float) : sizeof(double)));
-    d_hklop_.ensure(n_rec * kHklopRec * (mixed ? sizeof(float) : sizeof(double)));
-    if (mixed) d_phase_.ensure(n_rec * 4 * sizeof(double));
     DeviceBuffer& hlen = d_radial32_;   // |h| per reflection
-    hlen.ensure((size_t)n_hkl * sizeof(double));
     const OpDev* ops = d_ops_.as<OpDev>();
     const double* c2f = reinterpret_cast<const double*>(static_cast<const char*>(d_ops_.p) + n_ops * sizeof(OpDev));
     const TypeRec* types = d_type_.as<TypeRec>();
@@ -1372,7 +1442,6 @@ void Engine::run_prep(Precision prec, bool mp)
     const AtomIn* ain = d_atoms_in_.as<AtomIn>();
     const int T = 256;
     if (use_diag(mixed, mp)) {
-        d_ht_.ensure(std::max<size_t>(n_rec, 1) * sizeof(double));
         hipLaunchKernelGGL(k_hkl_prep_diag, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, d_hkl_.as<int>(), ops,
                            n_ops, n_hkl, c2f, d_hklop_.as<float>(), d_phase_.as<double>(), d_ht_.as<double>(),
                            hlen.as<double>());
@@ -1383,13 +1452,11 @@ void Engine::run_prep(Precision prec, bool mp)
     }
     HIP_CHECK(hipGetLastError());
     if (model_.n_types > 0) {
-        d_fcore_.ensure((size_t)std::max(model_.n_wfn, 1) * n_hkl * sizeof(double));
         int nw = n_hkl * model_.n_wfn;
         hipLaunchKernelGGL(k_core_prep, dim3((nw + T - 1) / T), dim3(T), 0, stream_, hlen.as<double>(), wfns,
                            d_core_.as<TermRec>(), model_.n_wfn, n_hkl, d_fcore_.as<double>());
         HIP_CHECK(hipGetLastError());
         int nt = n_hkl * model_.n_types;
-        if (mp) d_radx_.ensure((size_t)n_hkl * n_types * kRadRec * (mixed ? sizeof(float) : sizeof(double)));
         hipLaunchKernelGGL(k_radial_prep, dim3((nt + T - 1) / T), dim3(T), 0, stream_, hlen.as<double>(),
                            d_fcore_.as<double>(), types, wfns, d_val_.as<TermRec>(), model_.n_types, n_hkl,
                            mixed ? nullptr : d_radial_.as<double>(), mixed ? d_radial_.as<float>() : nullptr,
@@ -1434,7 +1501,8 @@ int Engine::default_chunk(bool mixed, bool derivs, bool mp)
     return std::max(8, (n_hkl_ + n_chunks - 1) / n_chunks);
 }
 
-void Engine::launch_main(Precision prec, bool derivs, bool mp, int chunk)
+// Arguments of the main kernel for this structure (partial-sum buffers sized here).
+void Engine::main_args(Precision prec, bool derivs, bool mp, int chunk, MainArgs* out)
 {
     const bool mixed = prec == kMixed;
     const int n_slots = n_sorted_;
@@ -1444,7 +1512,8 @@ void Engine::launch_main(Precision prec, bool derivs, bool mp, int chunk)
     d_fpart_.ensure((size_t)n_waves * n_hkl * 2 * sizeof(double));
     const int nd = mp ? kNdMultipole : kNdBasic;
     d_dpart_.ensure((size_t)n_chunks * n_slots * nd * sizeof(double));
-    MainArgs a;
+    MainArgs& a = *out;
+    std::memset(&a, 0, sizeof(a));
     a.hklop = d_hklop_.p;
     a.phase = mixed ? d_phase_.as<double>() : nullptr;
     a.radial = d_radial_.p;
@@ -1464,10 +1533,6 @@ void Engine::launch_main(Precision prec, bool derivs, bool mp, int chunk)
     a.grid_x = (n_waves + kWavesPerBlock - 1) / kWavesPerBlock;
     a.n_blocks = a.grid_x * n_chunks;
     a.per_xcd = (a.n_blocks + kXcds - 1) / kXcds;
-    dim3 grid(a.per_xcd * kXcds);
-    last_grid_x = a.grid_x;
-    last_grid_y = n_chunks;
-    last_chunk = chunk;
     const bool diag = use_diag(mixed, mp);
     a.ht = diag ? d_ht_.as<double>() : nullptr;
     a.diag_mask = 0;
@@ -1481,7 +1546,17 @@ void Engine::launch_main(Precision prec, bool derivs, bool mp, int chunk)
             a.diag_op[diag_pat_[s]] = s;
             a.diag_anom[diag_pat_[s]] = diag_anom_[s];
         }
-    const void* kern = main_kernel(mixed, derivs, mp, diag);
+}
+
+void Engine::launch_main(Precision prec, bool derivs, bool mp, int chunk)
+{
+    MainArgs a;
+    main_args(prec, derivs, mp, chunk, &a);
+    dim3 grid(a.per_xcd * kXcds);
+    last_grid_x = a.grid_x;
+    last_grid_y = a.n_blocks / std::max(a.grid_x, 1);
+    last_chunk = chunk;
+    const void* kern = main_kernel(prec == kMixed, derivs, mp, use_diag(prec == kMixed, mp));
     void* args[] = {&a};
     HIP_CHECK(hipLaunchKernel(kern, grid, dim3(256), args, 0, stream_));
 }
@@ -1607,6 +1682,402 @@ void Engine::download(double* f, double* d10)
         HIP_CHECK(hipMemcpyAsync(d10, d_d_.p, (size_t)n_atoms_ * last_nd_ * sizeof(double), hipMemcpyDeviceToHost,
                                  stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+// ------------------------------------------------------------------ batched structures
+
+struct BatchItem {
+    // prep kernels
+    const int* hkl;
+    const OpDev* ops;
+    const double* c2f;
+    double* rec64;
+    float* rec32;
+    double* phase;
+    double* ht;
+    double* hlen;
+    const WfnRec* wfns;
+    const TermRec* core;
+    const TermRec* val;
+    const TypeRec* types;
+    double* fcore;
+    double* rad64;
+    float* rad32;
+    const AtomIn* ain;
+    AtomRec* atoms;
+    // stage 2
+    const double* mult;
+    double* f;
+    double* d;
+    int n_ops, n_hkl, n_wfn, n_types, n_slots, n_atoms, nd, n_chunks, centric_origin, diag;
+    MainArgs main;
+};
+
+// index spaces of the batched launches; offsets are prefix sums over structures
+enum BatchSpace { kSpFill = 0, kSpHkl, kSpCore, kSpRad, kSpSlot, kSpRedD, kSpUnit, kNumSpaces };
+
+// structure b with off[b] <= i < off[b + 1] (empty structures are skipped)
+__device__ __forceinline__ int find_item(const int* __restrict__ off, int nb, int i)
+{
+    int lo = 0, hi = nb;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_batch_fill(const BatchItem* __restrict__ it, const int* __restrict__ off, int nb, int total,
+                             int* __restrict__ counter)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) counter[0] = 0;         // work counter of the persistent main kernel
+    if (i >= total) return;
+    const int b = find_item(off, nb, i);
+    it[b].d[i - off[b]] = 0.0;
+}
+
+__global__ void k_batch_hkl_prep(const BatchItem* __restrict__ it, const int* __restrict__ off, int nb, int total)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int b = find_item(off, nb, i);
+    const BatchItem& q = it[b];
+    const int k = i - off[b];
+    if (q.diag) hkl_prep_diag_item(k, q.hkl, q.ops, q.n_ops, q.c2f, q.rec32, q.phase, q.ht, q.hlen);
+    else hkl_prep_item(k, q.hkl, q.ops, q.n_ops, q.c2f, q.rec64, q.rec32, q.phase, q.hlen);
+}
+
+__global__ void k_batch_core_prep(const BatchItem* __restrict__ it, const int* __restrict__ off, int nb, int total)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int b = find_item(off, nb, i);
+    const BatchItem& q = it[b];
+    core_prep_item(i - off[b], q.hlen, q.wfns, q.core, q.n_hkl, q.fcore);
+}
+
+__global__ void k_batch_radial_prep(const BatchItem* __restrict__ it, const int* __restrict__ off, int nb, int total)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int b = find_item(off, nb, i);
+    const BatchItem& q = it[b];
+    radial_prep_item(i - off[b], q.hlen, q.fcore, q.types, q.wfns, q.val, q.n_types, q.n_hkl, q.rad64, q.rad32,
+                     nullptr, nullptr);
+}
+
+__global__ __launch_bounds__(64) void k_batch_atom_prep(const BatchItem* __restrict__ it, const int* __restrict__ off,
+                                                        int nb)
+{
+    __shared__ double fsh[64];
+    const int b = find_item(off, nb, blockIdx.x);
+    const BatchItem& q = it[b];
+    atom_prep_item(blockIdx.x - off[b], threadIdx.x, fsh, q.ain, q.types, q.atoms);
+}
+
+// Persistent: every workgroup takes (structure, atom block, reflection chunk) units from the
+// device counter until none is left, so uneven structures share the machine without a tail per
+// structure.  Units are ordered costliest structure first.
+template <bool MIXED, bool DERIV>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 1))) void k_batch_main(
+    const BatchItem* __restrict__ it, const int* __restrict__ off, int nb, int total, int* __restrict__ counter)
+{
+    constexpr int kStageBytes = MIXED ? kStageBytesF32 : kStageBytesF64;
+    __shared__ __align__(16) unsigned char stage_lds[kWavesPerBlock * kStageBytes];
+    __shared__ int s_unit;
+    const int wib = (int)(threadIdx.x / kWave);
+    const int lane = threadIdx.x & (kWave - 1);
+    void* stage = stage_lds + wib * kStageBytes;
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = atomicAdd(counter, 1);
+        __syncthreads();
+        const int u = __builtin_amdgcn_readfirstlane(s_unit);
+        __syncthreads();
+        if (u >= total) break;
+        const int b = __builtin_amdgcn_readfirstlane(find_item(off, nb, u));
+        const MainArgs& a = it[b].main;
+        const int local = u - off[b];
+        const int chunk_id = local / a.grid_x;
+        const int ablock = local - chunk_id * a.grid_x;
+        if constexpr (MIXED) {
+            if (a.diag_mask) main_unit<true, DERIV, false, true>(a, chunk_id, ablock, wib, lane, stage);
+            else main_unit<true, DERIV, false, false>(a, chunk_id, ablock, wib, lane, stage);
+        } else {
+            main_unit<false, DERIV, false, false>(a, chunk_id, ablock, wib, lane, stage);
+        }
+    }
+}
+
+__global__ void k_batch_reduce_f(const BatchItem* __restrict__ it, const int* __restrict__ off, int nb, int total)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int b = find_item(off, nb, i);
+    const BatchItem& q = it[b];
+    reduce_f_item(i - off[b], reinterpret_cast<const double2*>(q.main.fpart), q.mult, q.main.n_waves, q.n_hkl,
+                  q.centric_origin, q.f);
+}
+
+__global__ void k_batch_reduce_d(const BatchItem* __restrict__ it, const int* __restrict__ off, int nb, int total)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int b = find_item(off, nb, i);
+    const BatchItem& q = it[b];
+    reduce_d_item(i - off[b], q.main.dpart, q.atoms, q.n_chunks, q.n_slots, q.nd, q.d);
+}
+
+void Engine::batch_item(Precision prec, bool derivs, int chunk, BatchItem* out)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    const bool mixed = prec == kMixed;
+    ensure_prep_buffers(prec, false);
+    BatchItem& q = *out;
+    std::memset(&q, 0, sizeof(q));
+    main_args(prec, derivs, false, chunk, &q.main);
+    const int n_ops = (int)ops_.size();
+    q.hkl = d_hkl_.as<int>();
+    q.ops = d_ops_.as<OpDev>();
+    q.c2f = reinterpret_cast<const double*>(static_cast<const char*>(d_ops_.p) + n_ops * sizeof(OpDev));
+    q.diag = use_diag(mixed, false) ? 1 : 0;
+    q.rec64 = mixed ? nullptr : d_hklop_.as<double>();
+    q.rec32 = mixed ? d_hklop_.as<float>() : nullptr;
+    q.phase = mixed ? d_phase_.as<double>() : nullptr;
+    q.ht = q.diag ? d_ht_.as<double>() : nullptr;
+    q.hlen = d_radial32_.as<double>();
+    q.types = d_type_.as<TypeRec>();
+    q.wfns = reinterpret_cast<const WfnRec*>(static_cast<const char*>(d_type_.p) + model_.n_types * sizeof(TypeRec));
+    q.core = d_core_.as<TermRec>();
+    q.val = d_val_.as<TermRec>();
+    q.fcore = d_fcore_.as<double>();
+    q.rad64 = mixed ? nullptr : d_radial_.as<double>();
+    q.rad32 = mixed ? d_radial_.as<float>() : nullptr;
+    q.ain = d_atoms_in_.as<AtomIn>();
+    q.atoms = d_atoms_.as<AtomRec>();
+    q.nd = kNdBasic;
+    d_f_.ensure((size_t)std::max(n_hkl_, 1) * 2 * sizeof(double));
+    d_d_.ensure((size_t)std::max(n_atoms_, 1) * q.nd * sizeof(double));
+    q.f = d_f_.as<double>();
+    q.d = d_d_.as<double>();
+    q.mult = d_dtdf_.as<double>() + 2 * (size_t)n_hkl_;
+    q.n_ops = n_ops;
+    q.n_hkl = n_hkl_;
+    q.n_wfn = model_.n_types > 0 ? model_.n_wfn : 0;
+    q.n_types = model_.n_types;
+    q.n_slots = n_sorted_;
+    q.n_atoms = n_atoms_;
+    q.n_chunks = n_hkl_ > 0 ? (n_hkl_ + chunk - 1) / chunk : 0;
+    q.centric_origin = centric_origin_ ? 1 : 0;
+}
+
+void Engine::mark_batched_run(bool derivs)
+{
+    ran_ = true;
+    last_nd_ = kNdBasic;
+    (void)derivs;
+}
+
+BatchEngine::BatchEngine(const std::vector<Engine*>& engines) : eng_(engines)
+{
+    if (eng_.empty()) throw Error(1, "empty batch");
+    device_ = eng_[0]->device();
+    for (Engine* e : eng_)
+        if (!e || e->device() != device_) throw Error(1, "every structure of a batch must use the same device");
+    HIP_CHECK(hipSetDevice(device_));
+    HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    HIP_CHECK(hipEventCreate(&ev0_));
+    HIP_CHECK(hipEventCreate(&ev1_));
+    d_counter_.ensure(256);
+}
+
+BatchEngine::~BatchEngine()
+{
+    drop_graph();
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void BatchEngine::drop_graph()
+{
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    exec_ = nullptr;
+    graph_key_.clear();
+}
+
+static const void* batch_main_kernel(bool mixed, bool derivs)
+{
+    if (mixed) return derivs ? (const void*)k_batch_main<true, true> : (const void*)k_batch_main<true, false>;
+    return derivs ? (const void*)k_batch_main<false, true> : (const void*)k_batch_main<false, false>;
+}
+
+int BatchEngine::chunk_for(Precision prec, bool derivs)
+{
+    if (hkl_chunk > 0) return hkl_chunk;
+    auto it = tuned_.find((prec == kMixed ? 2 : 0) + (derivs ? 1 : 0));
+    if (it != tuned_.end()) return it->second;
+    return 64;
+}
+
+// Build the per-structure descriptors and the prefix offsets of every index space; structures
+// are ordered by decreasing main-kernel work so the persistent kernel starts the long ones first.
+void BatchEngine::plan(Precision prec, bool derivs, int chunk)
+{
+    const int nb = (int)eng_.size();
+    std::vector<BatchItem> items(nb);
+    for (int b = 0; b < nb; b++) eng_[b]->batch_item(prec, derivs, chunk, &items[b]);
+    std::vector<int> order(nb);
+    for (int b = 0; b < nb; b++) order[b] = b;
+    auto work = [&](int b) {
+        const BatchItem& q = items[b];
+        return (double)q.n_slots * q.n_ops * q.n_hkl;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+    std::vector<BatchItem> sorted(nb);
+    for (int b = 0; b < nb; b++) sorted[b] = items[order[b]];
+    off_.assign((size_t)kNumSpaces * (nb + 1), 0);
+    total_.assign(kNumSpaces, 0);
+    for (int sp = 0; sp < kNumSpaces; sp++) {
+        long long acc = 0;
+        for (int b = 0; b < nb; b++) {
+            off_[(size_t)sp * (nb + 1) + b] = (int)acc;
+            const BatchItem& q = sorted[b];
+            long long n = 0;
+            switch (sp) {
+                case kSpFill: n = (long long)q.n_atoms * q.nd; break;
+                case kSpHkl: n = q.n_hkl; break;
+                case kSpCore: n = (long long)q.n_hkl * q.n_wfn; break;
+                case kSpRad: n = (long long)q.n_hkl * q.n_types; break;
+                case kSpSlot: n = q.n_slots; break;
+                case kSpRedD: n = derivs ? (long long)q.n_slots * q.nd : 0; break;
+                case kSpUnit: n = (q.main.n_waves > 0) ? q.main.n_blocks : 0; break;
+            }
+            acc += n;
+        }
+        if (acc > 0x7fffffffLL) throw Error(1, "batch too large for 32-bit index spaces");
+        off_[(size_t)sp * (nb + 1) + nb] = (int)acc;
+        total_[sp] = (int)acc;
+    }
+    items_.resize(sizeof(BatchItem) * nb);
+    std::memcpy(items_.data(), sorted.data(), items_.size());
+    d_items_.ensure(items_.size());
+    d_off_.ensure(off_.size() * sizeof(int));
+    HIP_CHECK(hipMemcpyAsync(d_items_.p, items_.data(), items_.size(), hipMemcpyHostToDevice, stream_));
+    HIP_CHECK(hipMemcpyAsync(d_off_.p, off_.data(), off_.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    last_chunk = chunk;
+    last_units = total_[kSpUnit];
+}
+
+void BatchEngine::launch(Precision prec, bool derivs)
+{
+    const bool mixed = prec == kMixed;
+    const int nb = (int)eng_.size();
+    const BatchItem* it = d_items_.as<BatchItem>();
+    const int* off = d_off_.as<int>();
+    auto o = [&](int sp) { return off + (size_t)sp * (nb + 1); };
+    auto blocks = [](int n, int t) { return dim3((unsigned)std::max(1, (n + t - 1) / t)); };
+    const int T = 256;
+    hipLaunchKernelGGL(k_batch_fill, blocks(total_[kSpFill], T), dim3(T), 0, stream_, it, o(kSpFill), nb,
+                       total_[kSpFill], d_counter_.as<int>());
+    if (total_[kSpHkl] > 0)
+        hipLaunchKernelGGL(k_batch_hkl_prep, blocks(total_[kSpHkl], T), dim3(T), 0, stream_, it, o(kSpHkl), nb,
+                           total_[kSpHkl]);
+    if (total_[kSpCore] > 0)
+        hipLaunchKernelGGL(k_batch_core_prep, blocks(total_[kSpCore], T), dim3(T), 0, stream_, it, o(kSpCore), nb,
+                           total_[kSpCore]);
+    if (total_[kSpRad] > 0)
+        hipLaunchKernelGGL(k_batch_radial_prep, blocks(total_[kSpRad], T), dim3(T), 0, stream_, it, o(kSpRad), nb,
+                           total_[kSpRad]);
+    if (total_[kSpSlot] > 0)
+        hipLaunchKernelGGL(k_batch_atom_prep, dim3(total_[kSpSlot]), dim3(64), 0, stream_, it, o(kSpSlot), nb);
+    HIP_CHECK(hipGetLastError());
+    if (total_[kSpUnit] > 0) {
+        const int grid = std::min(grid_[(mixed ? 2 : 0) + (derivs ? 1 : 0)], total_[kSpUnit]);
+        last_grid = grid;
+        const int nbv = nb, tot = total_[kSpUnit];
+        const int* ou = o(kSpUnit);
+        int* cnt = d_counter_.as<int>();
+        void* args[] = {(void*)&it, (void*)&ou, (void*)&nbv, (void*)&tot, (void*)&cnt};
+        HIP_CHECK(hipLaunchKernel(batch_main_kernel(mixed, derivs), dim3(grid), dim3(256), args, 0, stream_));
+    }
+    const int R = 64;
+    if (total_[kSpHkl] > 0)
+        hipLaunchKernelGGL(k_batch_reduce_f, blocks(total_[kSpHkl], R), dim3(R), 0, stream_, it, o(kSpHkl), nb,
+                           total_[kSpHkl]);
+    if (derivs && total_[kSpRedD] > 0)
+        hipLaunchKernelGGL(k_batch_reduce_d, blocks(total_[kSpRedD], R), dim3(R), 0, stream_, it, o(kSpRedD), nb,
+                           total_[kSpRedD]);
+    HIP_CHECK(hipGetLastError());
+}
+
+void BatchEngine::run(Precision prec, bool derivs, bool graph)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    int& g = grid_[(prec == kMixed ? 2 : 0) + (derivs ? 1 : 0)];
+    if (g == 0) {
+        // persistent grid: every workgroup the device holds at once
+        int per_cu = 0;
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_main_kernel(prec == kMixed, derivs), 256,
+                                                               0));
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, device_));
+        g = std::max(1, per_cu) * prop.multiProcessorCount;
+    }
+    const int chunk = chunk_for(prec, derivs);
+    plan(prec, derivs, chunk);
+    std::vector<unsigned char> key(items_);
+    key.insert(key.end(), reinterpret_cast<const unsigned char*>(off_.data()),
+               reinterpret_cast<const unsigned char*>(off_.data() + off_.size()));
+    key.push_back((unsigned char)(prec == kMixed ? 1 : 0));
+    key.push_back((unsigned char)(derivs ? 1 : 0));
+    if (graph) {
+        if (!exec_ || key != graph_key_) {
+            drop_graph();
+            hipGraph_t g = nullptr;
+            HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            launch(prec, derivs);
+            HIP_CHECK(hipStreamEndCapture(stream_, &g));
+            HIP_CHECK(hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(g);
+            graph_key_ = key;
+        }
+        HIP_CHECK(hipEventRecord(ev0_, stream_));
+        HIP_CHECK(hipGraphLaunch(exec_, stream_));
+        HIP_CHECK(hipEventRecord(ev1_, stream_));
+    } else {
+        HIP_CHECK(hipEventRecord(ev0_, stream_));
+        launch(prec, derivs);
+        HIP_CHECK(hipEventRecord(ev1_, stream_));
+    }
+    HIP_CHECK(hipEventSynchronize(ev1_));
+    HIP_CHECK(hipEventElapsedTime(&last_ms, ev0_, ev1_));
+    for (Engine* e : eng_) e->mark_batched_run(derivs);
+}
+
+int BatchEngine::autotune(Precision prec, bool derivs)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    const int saved = hkl_chunk;
+    int best = 64;
+    float best_ms = 1e30f;
+    for (int c : {16, 24, 32, 48, 64, 96, 128, 192, 256}) {
+        hkl_chunk = c;
+        float ms_min = 1e30f;
+        for (int rep = 0; rep < 2; rep++) {
+            run(prec, derivs, false);
+            ms_min = std::min(ms_min, last_ms);
+        }
+        if (ms_min < best_ms) {
+            best_ms = ms_min;
+            best = c;
+        }
+    }
+    hkl_chunk = saved;
+    tuned_[(prec == kMixed ? 2 : 0) + (derivs ? 1 : 0)] = best;
+    return best;
 }
 
 }  // namespace discamb_amd

@@ -11,6 +11,9 @@ namespace discamb_amd {
 
 enum Precision { kFp64 = 0, kMixed = 1 };
 
+struct BatchItem;      // per-structure arguments of the batched kernels (device layout)
+struct MainArgs;
+
 // One explicit operation as the engine uses it.
 struct EngineOp {
     int rf[9];          // fractional rotation
@@ -78,8 +81,17 @@ public:
     float last_main_ms = 0.f;     // main-kernel time measured with events when timing_ is on
     bool timing = false;
     int n_ops() const { return (int)ops_.size(); }
+    int device() const { return device_; }
+    int n_hkl() const { return n_hkl_; }
+    int n_slots() const { return n_sorted_; }
+    // Batched steps (BatchEngine): make every buffer of a run exist and describe this structure's
+    // part of the step; after the batched launch, mark the results as a completed run.
+    void batch_item(Precision prec, bool derivs, int chunk, BatchItem* out);
+    void mark_batched_run(bool derivs);
 
 private:
+    void ensure_prep_buffers(Precision prec, bool mp);
+    void main_args(Precision prec, bool derivs, bool mp, int chunk, MainArgs* a);
     void run_prep(Precision prec, bool mp);
     void launch_main(Precision prec, bool derivs, bool mp, int chunk);
     void reduce(bool derivs, bool mp, int chunk);
@@ -113,6 +125,45 @@ private:
     // per call
     DeviceBuffer d_hkl_, d_dtdf_, d_atoms_in_, d_atoms_, d_hklop_, d_hklop32_, d_phase_, d_radial_,
         d_radial32_, d_ht_, d_fpart_, d_dpart_, d_f_, d_d_, d_order_, d_fcore_, d_radx_;
+};
+
+// Many structures in one step: the per-structure prep kernels, the fused F + adjoint kernel and
+// the stage-2 reductions each run as ONE launch over all structures (flattened index spaces; the
+// main kernel is persistent and takes (structure, atom block, reflection chunk) units from a
+// device work counter).  Each structure keeps its own Engine (inputs uploaded with
+// Engine::upload, results read with Engine::download); the batch only borrows them.  All
+// engines must live on one device.
+class BatchEngine {
+public:
+    explicit BatchEngine(const std::vector<Engine*>& engines);
+    ~BatchEngine();
+    // One batched step on the engines' uploaded inputs; synchronous.  graph: the step is captured
+    // into a hipGraph on first use and replayed while the structures' buffers and launch shape
+    // stay the same.
+    void run(Precision prec, bool derivs, bool graph);
+    // Times the whole batched step over candidate reflection chunks; keeps the fastest.
+    int autotune(Precision prec, bool derivs);
+    int hkl_chunk = 0;             // fixed reflection chunk for every structure; 0 = autotuned/default
+    float last_ms = 0.f;           // device time of the last step (events around it)
+    int last_chunk = 0, last_units = 0, last_grid = 0;
+
+private:
+    void plan(Precision prec, bool derivs, int chunk);
+    void launch(Precision prec, bool derivs);
+    int chunk_for(Precision prec, bool derivs);
+    void drop_graph();
+    std::vector<Engine*> eng_;
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    DeviceBuffer d_items_, d_off_, d_counter_;
+    std::vector<unsigned char> items_;     // host image of the BatchItem array
+    std::vector<int> off_;                 // prefix offsets of every index space
+    std::vector<int> total_;
+    std::vector<unsigned char> graph_key_; // items_ + off_ + precision at capture time
+    hipGraphExec_t exec_ = nullptr;
+    std::map<long long, int> tuned_;
+    int grid_[4] = {0, 0, 0, 0};           // persistent main-kernel grid per (precision, derivatives)
 };
 
 }  // namespace discamb_amd

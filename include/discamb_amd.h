@@ -155,6 +155,27 @@ int dh_hc_last_timing(dh_calculator* calc, double* main_kernel_ms, int* grid_x, 
 int dh_hc_engine_info(dh_calculator* calc, int* n_explicit_ops, int* group_order, int* centrosymmetric,
                       char* centering);
 
+/* ---- Batches of structures (no reference counterpart: the reference computes one structure per
+ * call; a refinement of many small structures calls it once per structure).
+ * A batch borrows calculators created on one device; they must outlive it.  Inputs are uploaded
+ * per calculator with dh_hc_upload, dh_batch_run computes every structure in one launch sequence
+ * (per-structure prep, one persistent F + adjoint kernel over all structures, stage-2
+ * reductions), and results are read per calculator with dh_hc_fetch.  The precision is the
+ * calculators' calculation mode, which must agree.  selector: DH_D_XYZ | DH_D_ADP | DH_D_OCC bits
+ * (DH_D_MULTIPOLE is not batched).  use_graph != 0 captures the step into a hipGraph on first
+ * use and replays it while every structure keeps its buffer sizes. */
+typedef struct dh_batch dh_batch;
+int dh_batch_create(dh_calculator* const* calcs, int n, dh_batch** out);
+void dh_batch_destroy(dh_batch* batch);
+int dh_batch_run(dh_batch* batch, unsigned selector, int use_graph);
+/* time the whole batched step over reflection-chunk candidates; keeps and returns the fastest */
+int dh_batch_autotune(dh_batch* batch, unsigned selector, int* chunk);
+/* reflections per work unit for every structure (0 = autotuned or default) */
+int dh_batch_set_tuning(dh_batch* batch, int hkl_chunk);
+/* device time of the last dh_batch_run (events around the step), its chunk, work units and
+ * persistent grid size */
+int dh_batch_last_timing(dh_batch* batch, double* step_ms, int* chunk, int* units, int* grid);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,136 @@
+"""Batched structures (one launch sequence for many structures, optional hipGraph replay) against
+the same structures computed one at a time, and against the FP64 oracle.
+
+Tolerances (normwise max|d - d_ref| / max|d_ref| per output class):
+  FP64 batch vs oracle           1e-10
+  FP64 batch vs single calls     1e-12 (only the summation order of the chunks differs)
+  mixed batch vs mixed single    1e-6
+"""
+import numpy as np
+import pytest
+
+import hc_oracle
+from conftest import rel_err
+from discamb_amd import (BatchCalculator, DerivativesSelector, HansenCoppensStructureFactorCalculator, synth)
+
+pytestmark = pytest.mark.gpu
+
+I222 = ["x,y,z", "-x,-y,z", "-x,y,-z", "x,-y,-z", "x+1/2,y+1/2,z+1/2", "-x+1/2,-y+1/2,z+1/2",
+        "-x+1/2,y+1/2,-z+1/2", "x+1/2,-y+1/2,-z+1/2"]
+
+
+def _structures():
+    """Uneven structures over diagonal and general groups, shifted inversion and I-centring."""
+    out = []
+    specs = [("P1", 5, 1.2, 1), ("P-1", 23, 1.0, 2), ("P21/c", 40, 0.9, 3), ("C2/c", 12, 1.0, 4),
+             ("P212121", 70, 1.1, 5), ("P43212", 17, 1.0, 6), ("P31", 9, 0.9, 7), ("P212121", 130, 1.3, 8)]
+    for sg, n, dmin, seed in specs:
+        out.append(synth.small_structure(sg, n_atoms=n, d_min=dmin, seed=seed))
+    w = synth.small_structure("P-1", n_atoms=11, d_min=1.1, seed=9)
+    w.crystal.symops = ["x,y,z", "-x+1/2,-y,-z+1/2"]
+    out.append(w)
+    w = synth.small_structure("P212121", n_atoms=8, d_min=1.1, seed=10)
+    w.crystal.symops = I222
+    w.crystal.atoms.multiplicity[:] = 8
+    out.append(w)
+    return out
+
+
+def _calcs(ws, mode):
+    calcs = []
+    for w in ws:
+        c = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+        c.setCalculationMode(mode)
+        calcs.append(c)
+    return calcs
+
+
+def _errs(f, d, rf, rd):
+    return (rel_err(f, rf), rel_err(d.atomic_position_derivatives, rd.atomic_position_derivatives),
+            rel_err(d.adp_derivatives, rd.adp_derivatives), rel_err(d.occupancy_derivatives, rd.occupancy_derivatives))
+
+
+@pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
+def test_batch_matches_single_calls_and_oracle(mode):
+    ws = _structures()
+    calcs = _calcs(ws, mode)
+    single = []
+    for w, c in zip(ws, calcs):
+        single.append(c.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, w.hkl, w.dtdf))
+    batch = BatchCalculator(calcs)
+    for w, c in zip(ws, calcs):
+        c.upload(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+    for chunk in (0, 16, 40):
+        batch.setTuning(chunk)
+        print("case batch mode=%s structures=%d chunk=%d" % (mode, len(ws), chunk), flush=True)
+        batch.run(graph=False)
+        tol = 1e-12 if mode == "HIP" else 1e-6
+        for i, (w, c) in enumerate(zip(ws, calcs)):
+            f, d = c.fetch()
+            e = _errs(f, d, *single[i])
+            print("  %s vs single %s" % (w.name, e), flush=True)
+            assert max(e) <= tol, (w.name, e)
+    if mode == "HIP":
+        for w, c in zip(ws, calcs):
+            f, d = c.fetch()
+            rf, rx, ru, ro = hc_oracle.sf_and_derivs(w.crystal, w.params, w.crystal.atoms, w.lcs, w.hkl, w.dtdf,
+                                                     n_threads=16)
+            e = (rel_err(f, rf), rel_err(d.atomic_position_derivatives, rx), rel_err(d.adp_derivatives, ru),
+                 rel_err(d.occupancy_derivatives, ro))
+            print("  %s vs oracle %s" % (w.name, e), flush=True)
+            assert max(e) <= 1e-10
+
+
+@pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
+def test_batch_graph_replays_follow_new_inputs(mode):
+    """A captured step replayed after new coordinates are uploaded gives, at every replay, what an
+    eager batched step gives on the same inputs."""
+    ws = _structures()[:6]
+    calcs = _calcs(ws, mode)
+    batch = BatchCalculator(calcs)
+    rng = np.random.default_rng(3)
+    for rep in range(3):
+        atoms = []
+        for w, c in zip(ws, calcs):
+            a = w.crystal.atoms.copy()
+            if rep:
+                a.xyz = a.xyz + rng.normal(0, 0.002, a.xyz.shape)
+            atoms.append(a)
+            c.upload(a, w.lcs, w.hkl, w.dtdf)
+        print("case graph replay %d mode=%s" % (rep, mode), flush=True)
+        batch.run(graph=False)
+        eager = [c.fetch() for c in calcs]
+        batch.run(graph=True)
+        for i, c in enumerate(calcs):
+            f, d = c.fetch()
+            e = _errs(f, d, *eager[i])
+            assert max(e) == 0.0, (rep, ws[i].name, e)
+
+
+def test_batch_autotune_and_selector():
+    ws = _structures()[:5]
+    calcs = _calcs(ws, "HIP_MIXED")
+    batch = BatchCalculator(calcs)
+    for w, c in zip(ws, calcs):
+        c.upload(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+    assert batch.autotune() > 0
+    batch.run(DerivativesSelector(True, False, False))
+    f, d = calcs[2].fetch()
+    assert np.all(d.adp_derivatives == 0) and np.any(d.atomic_position_derivatives != 0)
+    t = batch.lastTiming()
+    assert t["step_ms"] > 0 and t["units"] > 0
+
+
+def test_batch_errors():
+    ws = _structures()[:2]
+    calcs = _calcs(ws, "HIP")
+    calcs[1].setCalculationMode("HIP_MIXED")
+    batch = BatchCalculator(calcs)
+    for w, c in zip(ws, calcs):
+        c.upload(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+    with pytest.raises(RuntimeError):
+        batch.run()
+    calcs[1].setCalculationMode("HIP")
+    with pytest.raises(RuntimeError):
+        batch.run(DerivativesSelector(d_multipole=True))
+    batch.run()

@@ -91,6 +91,7 @@ public:
 
 private:
     void ensure_prep_buffers(Precision prec, bool mp);
+    void plan_tables();
     void main_args(Precision prec, bool derivs, bool mp, int chunk, MainArgs* a);
     void run_prep(Precision prec, bool mp);
     void launch_main(Precision prec, bool derivs, bool mp, int chunk);
@@ -124,7 +125,9 @@ private:
     DeviceBuffer d_core_, d_val_, d_type_, d_ops_;
     // per call
     DeviceBuffer d_hkl_, d_dtdf_, d_atoms_in_, d_atoms_, d_hklop_, d_hklop32_, d_phase_, d_radial_,
-        d_radial32_, d_ht_, d_fpart_, d_dpart_, d_f_, d_d_, d_order_, d_fcore_, d_radx_;
+        d_radial32_, d_ht_, d_fpart_, d_dpart_, d_f_, d_d_, d_order_, d_radx_, d_tab_, d_tabrec_;
+    double h_max_ = 0.0;            // largest |h| of the uploaded reflections
+    int n_tab_nodes_ = 0;           // form-factor table nodes over all wavefunctions
 };
 
 // Many structures in one step: the per-structure prep kernels, the fused F + adjoint kernel and

@@ -374,8 +374,11 @@ int dh_hc_set_tuning(dh_calculator* calc, int hkl_chunk)
 int dh_hc_set_options(dh_calculator* calc, unsigned flags)
 {
     return guarded([&] {
-        if (!calc || (flags & ~DH_OPT_GENERAL_ROTATIONS)) throw Error(DH_E_ARGUMENT, "invalid options");
+        const unsigned all = DH_OPT_GENERAL_ROTATIONS | DH_OPT_RADIAL_RECORDS | DH_OPT_RADIAL_TABLES;
+        if (!calc || (flags & ~all) || ((flags & DH_OPT_RADIAL_RECORDS) && (flags & DH_OPT_RADIAL_TABLES)))
+            throw Error(DH_E_ARGUMENT, "invalid options");
         calc->c->engine->tune.general_path = (flags & DH_OPT_GENERAL_ROTATIONS) != 0;
+        calc->c->engine->tune.radial_mode = (flags & DH_OPT_RADIAL_RECORDS) ? 1 : (flags & DH_OPT_RADIAL_TABLES) ? 2 : 0;
     });
 }
 

@@ -51,6 +51,9 @@ struct TypeRec {
 
 struct WfnRec {
     int core_off, core_n, val_off, val_n;
+    int n[5], pad;              // deformation-valence STO powers n_l
+    double zeta;                // deformation-valence exponent
+    double gscale[5];           // 4 pi (i^l sign) N_l: STO normalisation zeta^(n+3)/(n+2)!
 };
 
 struct TermRec {
@@ -76,6 +79,8 @@ struct AtomRec {
     double plm[25];
     int type;
     int orig;
+    double pval, inv_kappa, inv_kappa_p;    // type parameters for on-the-fly radial functions
+    int wfn, pad;
 };
 
 void DeviceBuffer::ensure(size_t n)
@@ -191,13 +196,17 @@ __device__ __forceinline__ void hkl_prep_diag_item(int k, const int* __restrict_
         ht[(size_t)k * n_ops + s] = h0 * ops[s].tf[0] + h1 * ops[s].tf[1] + h2 * ops[s].tf[2];
 }
 
-// Spherical core and valence form factors, 4 pi sum_k c_k g_0(p_k + 2, s, e_k), are smooth in s
-// and shared by every atom type of a wavefunction: they are tabulated per wavefunction on a
-// uniform grid (value and d/ds at every node) and read back by cubic Hermite interpolation.  With
+// Radial functions of the model are smooth in s and shared by every atom type of a wavefunction:
+// core and valence form factors 4 pi sum_k c_k g_0(p_k + 2, s, e_k) and the deformation-valence
+// Fourier-Bessel transforms G_l(s) = 4 pi (i^l sign) N_l g_l(n_l + 2, s, zeta), l = 0..4.  They are
+// tabulated per wavefunction on a uniform grid (value and d/ds at every node) and read back by
+// cubic Hermite interpolation: core at s = |h|, valence at |h| / kappa, G_l at |h| / kappa'.  With
 // a step of kTabStep = 5e-4 / A the interpolation error is below 1e-13 of f(0) for H..S (measured
 // against the closed forms), far inside the FP64 parity bound.  Node i of wavefunction w sits at
-// s = i * kTabStep; each node holds {core f, core f', valence f, valence f'}.
+// s = i * kTabStep and holds 16 values {core f, f', valence f, f', G_0 f, f', ..., G_4 f, f', 0, 0}
+// (FP64 table, and an FP32 copy for the mixed-precision kernels).
 static constexpr double kTabStep = 5e-4;
+static constexpr int kTabNode = 16;
 
 struct TabRec {
     int off, n;        // first node, node count
@@ -205,12 +214,14 @@ struct TabRec {
 
 __device__ __forceinline__ void tab_build_item(int j, const TabRec* __restrict__ tabs, int n_wfn,
                                                const WfnRec* __restrict__ wfns, const TermRec* __restrict__ core,
-                                               const TermRec* __restrict__ val, double4* __restrict__ tab)
+                                               const TermRec* __restrict__ val, double* __restrict__ tab,
+                                               float* __restrict__ tab32)
 {
     int w = 0;
     while (w + 1 < n_wfn && tabs[w + 1].off <= j) w++;
     const double sv = (j - tabs[w].off) * kTabStep;
-    const WfnRec wf = wfns[w];
+    const WfnRec& wf = wfns[w];
+    double v[kTabNode];
     double fc = 0.0, dc = 0.0, fv = 0.0, dv = 0.0;
     for (int i = 0; i < wf.core_n; i++) {
         const TermRec tr = core[wf.core_off + i];
@@ -225,23 +236,65 @@ __device__ __forceinline__ void tab_build_item(int j, const TabRec* __restrict__
         dv += tr.c * dg;
     }
     const double four_pi = 12.566370614359172953850;
-    tab[j] = make_double4(four_pi * fc, four_pi * dc, four_pi * fv, four_pi * dv);
+    v[0] = four_pi * fc;
+    v[1] = four_pi * dc;
+    v[2] = four_pi * fv;
+    v[3] = four_pi * dv;
+    for (int l = 0; l < 5; l++) {
+        double dg = 0.0, g = 0.0;
+        if (wf.n[l] >= 0) g = g_function_dh(l, wf.n[l] + 2, sv, wf.zeta, &dg);
+        v[4 + 2 * l] = wf.gscale[l] * g;
+        v[5 + 2 * l] = wf.gscale[l] * dg;
+    }
+    v[14] = v[15] = 0.0;
+    for (int q = 0; q < kTabNode; q++) {
+        tab[(size_t)j * kTabNode + q] = v[q];
+        tab32[(size_t)j * kTabNode + q] = (float)v[q];
+    }
 }
 
-// cubic Hermite interpolation of component pair (f, f') = (.x, .y) (core) or (.z, .w) (valence)
-template <bool VALENCE>
-__device__ __forceinline__ double tab_lookup(const double4* __restrict__ tab, TabRec tr, double sv)
+// Hermite basis at x = s / kTabStep: node index i (clamped to the table) and weights for
+// (f_i, f_{i+1}, f'_i, f'_{i+1}) with the step folded into the derivative weights.
+template <typename R>
+__device__ __forceinline__ int hermite(R x, int n, R (&wt)[4])
 {
-    const double x = sv * (1.0 / kTabStep);
-    int i = (int)x;
-    i = min(i, tr.n - 2);
-    const double t = x - i;
-    const double4 a = tab[tr.off + i], b = tab[tr.off + i + 1];
-    const double ya = VALENCE ? a.z : a.x, da = VALENCE ? a.w : a.y;
-    const double yb = VALENCE ? b.z : b.x, db = VALENCE ? b.w : b.y;
-    const double u = 1.0 - t;
-    const double h00 = (1.0 + 2.0 * t) * u * u, h10 = t * u * u, h01 = t * t * (3.0 - 2.0 * t), h11 = -t * t * u;
-    return h00 * ya + h01 * yb + kTabStep * (h10 * da + h11 * db);
+    const int i = min((int)x, n - 2);
+    const R t = x - (R)i;
+    const R u = R(1) - t, tt = t * t;
+    wt[1] = tt * (R(3) - R(2) * t);
+    wt[0] = R(1) - wt[1];
+    wt[2] = R(kTabStep) * t * u * u;
+    wt[3] = -R(kTabStep) * tt * u;
+    return i;
+}
+
+// component pair (c, c + 1) of the table at x = s / kTabStep
+template <typename R>
+__device__ __forceinline__ R tab_at(const R* __restrict__ tab, TabRec tr, R x, int c)
+{
+    R wt[4];
+    const int i = hermite<R>(x, tr.n, wt);
+    const R* a = tab + (size_t)(tr.off + i) * kTabNode + c;
+    return wt[0] * a[0] + wt[1] * a[kTabNode] + wt[2] * a[1] + wt[3] * a[kTabNode + 1];
+}
+
+// On-the-fly radial factors of one atom at |h| = h: S = f_core + Pval f_val + f' and G_0..G_L.
+template <int L, typename R>
+__device__ __forceinline__ void radial_otf(const R* __restrict__ tab, TabRec tr, R h, R pval, R inv_kappa,
+                                           R inv_kappa_p, R anr, R& S, R (&G)[5])
+{
+    constexpr R inv_step = R(1.0 / kTabStep);
+    S = tab_at<R>(tab, tr, h * inv_step, 0) + pval * tab_at<R>(tab, tr, h * (inv_kappa * inv_step), 2) + anr;
+    R wt[4];
+    const int i = hermite<R>(h * (inv_kappa_p * inv_step), tr.n, wt);
+    const R* a = tab + (size_t)(tr.off + i) * kTabNode + 4;
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+        G[l] = R(0);
+        if (l <= L)
+            G[l] = wt[0] * a[2 * l] + wt[1] * a[kTabNode + 2 * l] + wt[2] * a[2 * l + 1] +
+                   wt[3] * a[kTabNode + 2 * l + 1];
+    }
 }
 
 // Per (reflection, type): S = f_core + Pval f_val(h/kappa) and
@@ -249,7 +302,7 @@ __device__ __forceinline__ double tab_lookup(const double4* __restrict__ tab, Ta
 // adjoint) f_val and its derivative are summed in closed form and the record
 // [f_val, dS/dkappa, dG_0..4/dkappa', 0] is written as well.
 __device__ __forceinline__ void radial_prep_item(int idx, const double* __restrict__ hlen,
-                                                 const double4* __restrict__ tab, const TabRec* __restrict__ tabs,
+                                                 const double* __restrict__ tab, const TabRec* __restrict__ tabs,
                                                  const TypeRec* __restrict__ types, const WfnRec* __restrict__ wfns,
                                                  const TermRec* __restrict__ val, int n_types, int n_hkl,
                                                  double* __restrict__ rad64, float* __restrict__ rad32,
@@ -274,11 +327,11 @@ __device__ __forceinline__ void radial_prep_item(int idx, const double* __restri
         fv *= four_pi;
         dfv *= four_pi;
     } else {
-        fv = tab_lookup<true>(tab, tr, hk);
+        fv = tab_at<double>(tab, tr, hk * (1.0 / kTabStep), 2);
     }
     double r[kRadRec] = {0, 0, 0, 0, 0, 0, 0, 0};
     double x[kRadRec] = {0, 0, 0, 0, 0, 0, 0, 0};
-    r[0] = tab_lookup<false>(tab, tr, h) + ty.pval * fv;
+    r[0] = tab_at<double>(tab, tr, h * (1.0 / kTabStep), 0) + ty.pval * fv;
     x[0] = fv;
     // d f_val(h/kappa) / d kappa = f_val'(h/kappa) * (-h / kappa^2)
     x[1] = ty.pval * dfv * (-hk * ty.inv_kappa);
@@ -349,6 +402,13 @@ __device__ __forceinline__ void atom_prep_item(int i, int p, double* fsh, const 
     } else if (p == 37) {
         r.type = a.type;
         r.orig = a.orig;
+    } else if (p == 38) {
+        const TypeRec& ty = types[a.type];
+        r.pval = ty.pval;
+        r.inv_kappa = ty.inv_kappa;
+        r.inv_kappa_p = ty.inv_kappa_p;
+        r.wfn = ty.wfn;
+        r.pad = 0;
     }
 }
 
@@ -370,13 +430,13 @@ __global__ void k_hkl_prep_diag(const int* __restrict__ hkl, const OpDev* __rest
 
 __global__ void k_tab_build(const TabRec* __restrict__ tabs, int n_wfn, const WfnRec* __restrict__ wfns,
                             const TermRec* __restrict__ core, const TermRec* __restrict__ val, int n_nodes,
-                            double4* __restrict__ tab)
+                            double* __restrict__ tab, float* __restrict__ tab32)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n_nodes) tab_build_item(j, tabs, n_wfn, wfns, core, val, tab);
+    if (j < n_nodes) tab_build_item(j, tabs, n_wfn, wfns, core, val, tab, tab32);
 }
 
-__global__ void k_radial_prep(const double* __restrict__ hlen, const double4* __restrict__ tab,
+__global__ void k_radial_prep(const double* __restrict__ hlen, const double* __restrict__ tab,
                               const TabRec* __restrict__ tabs, const TypeRec* __restrict__ types,
                               const WfnRec* __restrict__ wfns, const TermRec* __restrict__ val, int n_types,
                               int n_hkl, double* __restrict__ rad64, float* __restrict__ rad32,
@@ -414,7 +474,28 @@ struct MainArgs {
     int diag_op[8];           // diagonal groups: operation index of sign pattern p (bit i: axis i flips)
     float diag_anom[8];       // diagonal groups: anomalous sign of the operation with pattern p
     const double* ht;         // diagonal groups: h.t_s per (reflection, operation)
+    const void* tab;          // on-the-fly radial functions: per-wavefunction table (FP32 mixed / FP64)
+    const TabRec* tabs;       // node range of each wavefunction in tab
+    const double* hlen;       // |h| per reflection
 };
+
+// Pointers read out of argument structs in memory (the batched kernels) are generic to the
+// compiler, which then emits flat loads with full waits and no scalar loads.  Every buffer here is
+// device global memory: gmem() re-types a pointer as global (address space 1) and cmem() a
+// pointer to data no kernel of the step writes while the main kernel runs as constant (address
+// space 4), so uniform reads become scalar loads again.  The integer round trip keeps the
+// compiler from folding the casts away.
+template <typename T>
+__device__ __forceinline__ T* gmem(T* p)
+{
+    return (T*)((__attribute__((address_space(1))) T*)(unsigned long long)p);
+}
+
+template <typename T>
+__device__ __forceinline__ const T* cmem(const T* p)
+{
+    return (const T*)((const __attribute__((address_space(4))) T*)(unsigned long long)p);
+}
 
 // per-atom partial layout: 0-2 d/dx_frac, 3-8 d/dU, 9 d/docc, then with the multipole adjoint
 // 10-34 d/dP_lm (global frame), 35 d/dPval, 36 d/dkappa, 37 d/dkappa'
@@ -564,7 +645,7 @@ __device__ __forceinline__ void contract(const R* P, const R* Y, R* A)
     }
 }
 
-template <int L, bool DERIV, bool MP>
+template <int L, bool DERIV, bool MP, bool OTF>
 __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane, int chunk_id, int k0, int k1,
                                           const unsigned* __restrict__ hop, const double* __restrict__ rad,
                                           const double* __restrict__ radx, const unsigned* __restrict__ dtdf32,
@@ -573,7 +654,7 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
 {
     constexpr int NLM = (L + 1) * (L + 1);
     const int slot = wave * kWave + lane;
-    const AtomRec& at = a.atoms[slot];
+    const AtomRec& at = cmem(a.atoms)[slot];
     const double x0 = at.xf[0], x1 = at.xf[1], x2 = at.xf[2];
     double u[6];
 #pragma unroll
@@ -583,6 +664,16 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
     for (int j = 0; j < NLM; j++) P[j] = at.plm[j];
     const double w = at.w, anr = at.anom[0], ani = at.anom[1];
     const int t = at.type;
+    TabRec tr{0, 2};
+    double pv = 0, ik = 0, ikp = 0;
+    if (OTF) {
+        tr = cmem(a.tabs)[at.wfn];
+        pv = at.pval;
+        ik = at.inv_kappa;
+        ikp = at.inv_kappa_p;
+    }
+    const double* tab = cmem(static_cast<const double*>(a.tab));
+    const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
     double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
     double B[MP ? NLM : 1];
 #pragma unroll
@@ -592,8 +683,15 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
     const int nk = min(kStage, k1 - kb);
     for (int kk = 0; kk < nk; ++kk) {
         const int k = kb + kk;
-        const double* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
-        const double S = rr[0] + anr, G0 = rr[1], G1 = rr[2], G2 = rr[3], G3 = rr[4], G4 = rr[5];
+        double S, G0, G1, G2, G3, G4;
+        if (OTF) {
+            double G[5];
+            radial_otf<L, double>(tab, tr, ld_f64(hl32, k), pv, ik, ikp, anr, S, G);
+            G0 = G[0]; G1 = G[1]; G2 = G[2]; G3 = G[3]; G4 = G[4];
+        } else {
+            const double* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
+            S = rr[0] + anr; G0 = rr[1]; G1 = rr[2]; G2 = rr[3]; G3 = rr[4]; G4 = rr[5];
+        }
         double fval = 0, dSk = 0, dG[5] = {0, 0, 0, 0, 0};
         if (MP) {
             const double* rx = radx + ((size_t)k * a.n_types + t) * kRadRec;
@@ -674,7 +772,7 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
 
 // Mixed precision: FP64 phase range reduction and FP64 accumulation across reflections, FP32
 // harmonics / radial / Debye-Waller / sincos / exp per term.
-template <int L, bool DERIV, bool MP>
+template <int L, bool DERIV, bool MP, bool OTF>
 __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane, int chunk_id, int k0, int k1,
                                            const float* __restrict__ hop, const float* __restrict__ rad,
                                            const float* __restrict__ radx, const unsigned* __restrict__ phs32,
@@ -684,7 +782,7 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
     constexpr int NLM = (L + 1) * (L + 1);
     constexpr int kFlush = kStage;    // reflections accumulated in FP32 before an FP64 flush
     const int slot = wave * kWave + lane;
-    const AtomRec& at = a.atoms[slot];
+    const AtomRec& at = cmem(a.atoms)[slot];
     const double x0 = at.xf[0], x1 = at.xf[1], x2 = at.xf[2];
     float u[6];
 #pragma unroll
@@ -694,6 +792,16 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
     for (int j = 0; j < NLM; j++) P[j] = (float)at.plm[j];
     const float w = (float)at.w, anr = (float)at.anom[0], ani = (float)at.anom[1];
     const int t = at.type;
+    TabRec tr{0, 2};
+    float pv = 0, ik = 0, ikp = 0;
+    if (OTF) {
+        tr = cmem(a.tabs)[at.wfn];
+        pv = (float)at.pval;
+        ik = (float)at.inv_kappa;
+        ikp = (float)at.inv_kappa_p;
+    }
+    const float* tab = cmem(static_cast<const float*>(a.tab));
+    const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
     double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
     double B[MP ? NLM : 1];
 #pragma unroll
@@ -708,8 +816,15 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
         float fpv = 0, fka = 0, fkp = 0;
         for (int k = kb; k < kend; ++k) {
             const int kk = k - kb;
-            const float* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
-            const float S = rr[0] + anr, G0 = rr[1], G1 = rr[2], G2 = rr[3], G3 = rr[4], G4 = rr[5];
+            float S, G0, G1, G2, G3, G4;
+            if (OTF) {
+                float G[5];
+                radial_otf<L, float>(tab, tr, (float)ld_f64(hl32, k), pv, ik, ikp, anr, S, G);
+                G0 = G[0]; G1 = G[1]; G2 = G[2]; G3 = G[3]; G4 = G[4];
+            } else {
+                const float* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
+                S = rr[0] + anr; G0 = rr[1]; G1 = rr[2]; G2 = rr[3]; G3 = rr[4]; G4 = rr[5];
+            }
             float fval = 0, dSk = 0, dG[5] = {0, 0, 0, 0, 0};
             if (MP) {
                 const float* rx = radx + ((size_t)k * a.n_types + t) * kRadRec;
@@ -910,7 +1025,7 @@ __device__ __forceinline__ void diag_term_mixed(const DiagRefl<float>& q, const 
     }
 }
 
-template <int L, bool DERIV>
+template <int L, bool DERIV, bool OTF>
 __device__ __forceinline__ void body_mixed_diag(const MainArgs& a, int wave, int lane, int chunk_id, int k0, int k1,
                                                 const float* __restrict__ hrec, const float* __restrict__ rad,
                                                 const unsigned* __restrict__ hd32, const unsigned* __restrict__ ht32,
@@ -919,7 +1034,7 @@ __device__ __forceinline__ void body_mixed_diag(const MainArgs& a, int wave, int
 {
     constexpr int NLM = (L + 1) * (L + 1);
     const int slot = wave * kWave + lane;
-    const AtomRec& at = a.atoms[slot];
+    const AtomRec& at = cmem(a.atoms)[slot];
     const double x0 = at.xf[0], x1 = at.xf[1], x2 = at.xf[2];
     float u[6];
 #pragma unroll
@@ -929,20 +1044,36 @@ __device__ __forceinline__ void body_mixed_diag(const MainArgs& a, int wave, int
     for (int j = 0; j < NLM; j++) P[j] = (float)at.plm[j];
     const float w = (float)at.w, anr = (float)at.anom[0], ani = (float)at.anom[1];
     const int t = at.type;
+    TabRec tr{0, 2};
+    float pv = 0, ik = 0, ikp = 0;
+    if (OTF) {
+        tr = cmem(a.tabs)[at.wfn];
+        pv = (float)at.pval;
+        ik = (float)at.inv_kappa;
+        ikp = (float)at.inv_kappa_p;
+    }
+    const float* tab = cmem(static_cast<const float*>(a.tab));
+    const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
     double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
     for (int kb = k0; kb < k1; kb += kStage) {
         const int kend = min(kb + kStage, k1);
         float fx[3] = {0, 0, 0}, fu[6] = {0, 0, 0, 0, 0, 0}, focc = 0;
         for (int k = kb; k < kend; ++k) {
-            const float* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
             const float* r = hrec + (size_t)k * kHklopRec;
             DiagRefl<float> q;
-            q.S = rr[0] + anr;
-            q.G0 = rr[1];
-            q.G1 = rr[2];
-            q.G2 = rr[3];
-            q.G3 = rr[4];
-            q.G4 = rr[5];
+            if (OTF) {
+                float G[5];
+                radial_otf<L, float>(tab, tr, (float)ld_f64(hl32, k), pv, ik, ikp, anr, q.S, G);
+                q.G0 = G[0]; q.G1 = G[1]; q.G2 = G[2]; q.G3 = G[3]; q.G4 = G[4];
+            } else {
+                const float* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
+                q.S = rr[0] + anr;
+                q.G0 = rr[1];
+                q.G1 = rr[2];
+                q.G2 = rr[3];
+                q.G3 = rr[4];
+                q.G4 = rr[5];
+            }
             q.cr = (float)ld_f64(dtdf32, 2 * k);
             q.ci = (float)ld_f64(dtdf32, 2 * k + 1);
             class_sums<L, float>(P, r, q.C);
@@ -983,50 +1114,49 @@ __device__ __forceinline__ void body_mixed_diag(const MainArgs& a, int wave, int
     }
 }
 
-// One workgroup = kWavesPerBlock atom waves x one reflection chunk, over a 1-D grid of
-// kXcds * per_xcd workgroups: hardware id b runs on XCD b % kXcds and is mapped to logical block
-// (b % kXcds) * per_xcd + b / kXcds, so each XCD walks a contiguous range of chunks and every
-// record of a chunk is fetched into one L2 only.
 // One (atom block, reflection chunk) work unit: wave wib of the block runs atom wave
 // ablock * kWavesPerBlock + wib over the reflections of chunk chunk_id.
-template <bool MIXED, bool DERIV, bool MP, bool DIAG>
+template <bool MIXED, bool DERIV, bool MP, bool DIAG, bool OTF>
 __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int ablock, int wib, int lane, void* stage)
 {
     static_assert(MIXED || !DIAG, "the parity-folded kernels are mixed precision only");
+    static_assert(!(MP && OTF), "the multipole adjoint uses radial records");
     const int wave = ablock * kWavesPerBlock + wib;
     if (wave >= a.n_waves) return;
     const int k0 = chunk_id * a.chunk;
     const int k1 = min(k0 + a.chunk, a.n_hkl);
-    const int cls = __builtin_amdgcn_readfirstlane(a.wave_l[wave]);
-    const unsigned* dtdf32 = reinterpret_cast<const unsigned*>(a.dtdf);
-    const unsigned* ht32 = reinterpret_cast<const unsigned*>(a.ht);
+    const int cls = __builtin_amdgcn_readfirstlane(cmem(a.wave_l)[wave]);
+    const unsigned* dtdf32 = cmem(reinterpret_cast<const unsigned*>(a.dtdf));
+    const unsigned* ht32 = cmem(reinterpret_cast<const unsigned*>(a.ht));
+    double* fpart = gmem(a.fpart);
+    double* dpart = gmem(a.dpart);
     if (MIXED) {
-        const float* hop = static_cast<const float*>(a.hklop);
-        const float* rad = static_cast<const float*>(a.radial);
-        const float* radx = static_cast<const float*>(a.radx);
-        const unsigned* phs = reinterpret_cast<const unsigned*>(a.phase);
+        const float* hop = cmem(static_cast<const float*>(a.hklop));
+        const float* rad = cmem(static_cast<const float*>(a.radial));
+        const float* radx = cmem(static_cast<const float*>(a.radx));
+        const unsigned* phs = cmem(reinterpret_cast<const unsigned*>(a.phase));
         float2* st = static_cast<float2*>(stage);
         if (DIAG) {
             if (cls <= 2)
-                body_mixed_diag<2, DERIV>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, ht32, dtdf32, st, a.fpart,
-                                          a.dpart);
+                body_mixed_diag<2, DERIV, OTF>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, ht32, dtdf32, st, fpart,
+                                          dpart);
             else
-                body_mixed_diag<4, DERIV>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, ht32, dtdf32, st, a.fpart,
-                                          a.dpart);
+                body_mixed_diag<4, DERIV, OTF>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, ht32, dtdf32, st, fpart,
+                                          dpart);
         } else if (cls <= 2) {
-            body_mixed<2, DERIV, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, phs, dtdf32, st, a.fpart, a.dpart);
+            body_mixed<2, DERIV, MP, OTF>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, phs, dtdf32, st, fpart, dpart);
         } else {
-            body_mixed<4, DERIV, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, phs, dtdf32, st, a.fpart, a.dpart);
+            body_mixed<4, DERIV, MP, OTF>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, phs, dtdf32, st, fpart, dpart);
         }
     } else {
-        const unsigned* hop = static_cast<const unsigned*>(a.hklop);
-        const double* rad = static_cast<const double*>(a.radial);
-        const double* radx = static_cast<const double*>(a.radx);
+        const unsigned* hop = cmem(static_cast<const unsigned*>(a.hklop));
+        const double* rad = cmem(static_cast<const double*>(a.radial));
+        const double* radx = cmem(static_cast<const double*>(a.radx));
         double2* st = static_cast<double2*>(stage);
         if (cls <= 2) {
-            body_fp64<2, DERIV, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, dtdf32, st, a.fpart, a.dpart);
+            body_fp64<2, DERIV, MP, OTF>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, dtdf32, st, fpart, dpart);
         } else {
-            body_fp64<4, DERIV, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, dtdf32, st, a.fpart, a.dpart);
+            body_fp64<4, DERIV, MP, OTF>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, dtdf32, st, fpart, dpart);
         }
     }
 }
@@ -1035,8 +1165,8 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
 // kXcds * per_xcd workgroups: hardware id b runs on XCD b % kXcds and is mapped to logical block
 // (b % kXcds) * per_xcd + b / kXcds, so each XCD walks a contiguous range of chunks and every
 // record of a chunk is fetched into one L2 only.
-template <bool MIXED, bool DERIV, bool MP, bool DIAG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && !MP ? (DIAG ? 4 : 5) : 1))) void k_main(MainArgs a)
+template <bool MIXED, bool DERIV, bool MP, bool DIAG, bool OTF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && !MP ? (DIAG || OTF ? 4 : 5) : 1))) void k_main(MainArgs a)
 {
     constexpr int kStageBytes = MIXED ? kStageBytesF32 : kStageBytesF64;
     __shared__ __align__(16) unsigned char stage_lds[kWavesPerBlock * kStageBytes];
@@ -1045,8 +1175,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && !M
     if (logical >= a.n_blocks) return;
     const int chunk_id = logical / a.grid_x;
     const int wib = (int)(threadIdx.x / kWave);
-    main_unit<MIXED, DERIV, MP, DIAG>(a, chunk_id, logical - chunk_id * a.grid_x, wib, threadIdx.x & (kWave - 1),
-                                      stage_lds + wib * kStageBytes);
+    main_unit<MIXED, DERIV, MP, DIAG, OTF>(a, chunk_id, logical - chunk_id * a.grid_x, wib, threadIdx.x & (kWave - 1),
+                                           stage_lds + wib * kStageBytes);
 }
 
 // F(h) = mult(h) * sum_waves partial  (centric at origin: 2 Re, 0).  Loads are issued eight at a
@@ -1229,6 +1359,16 @@ static void upload_constants(int device)
     if (device >= 0 && device < 64) g_constants_ready[device] = true;
 }
 
+// 4 pi (i^l sign) N for the deformation-valence STO r^n exp(-zeta r): N = zeta^(n+3) / (n+2)!
+static double def_val_scale(double zeta, int n, int l)
+{
+    double norm = 1.0;
+    for (int q = 0; q < n + 3; q++) norm *= zeta;
+    for (int q = 2; q <= n + 2; q++) norm /= q;
+    const double sgn = (l == 2 || l == 3) ? -1.0 : 1.0;   // i^l: +, +i, -, -i, +
+    return sgn * 12.566370614359172953850 * norm;
+}
+
 static std::vector<TypeRec> type_records(const EngineModel& model)
 {
     std::vector<TypeRec> tr(model.n_types);
@@ -1247,13 +1387,7 @@ static std::vector<TypeRec> type_records(const EngineModel& model)
         r.zeta = model.dv_zeta[r.wfn];
         r.inv_kappa = 1.0 / r.kappa;
         r.inv_kappa_p = 1.0 / r.kappa_p;
-        for (int l = 0; l < 5; l++) {
-            double norm = 1.0;
-            for (int q = 0; q < r.n[l] + 3; q++) norm *= r.zeta;
-            for (int q = 2; q <= r.n[l] + 2; q++) norm /= q;
-            const double sgn = (l == 2 || l == 3) ? -1.0 : 1.0;   // i^l: +, +i, -, -i, +
-            r.gscale[l] = sgn * 12.566370614359172953850 * norm;
-        }
+        for (int l = 0; l < 5; l++) r.gscale[l] = def_val_scale(r.zeta, r.n[l], l);
         for (int l = 0; l <= r.max_l; l++)
             for (int i = l * l; i < (l + 1) * (l + 1); i++) r.plm[i] = model.types[t].plm[i];
     }
@@ -1297,7 +1431,7 @@ void Engine::plan_tables()
     std::vector<double> kmin(nw, 1.0);
     for (int t = 0; t < model_.n_types; t++) {
         const int w = model_.type_wfn[t];
-        kmin[w] = std::min(kmin[w], model_.types[t].kappa);
+        kmin[w] = std::min(kmin[w], std::min(model_.types[t].kappa, model_.types[t].kappa_prime));
     }
     std::vector<TabRec> tr(nw);
     int nodes = 0;
@@ -1308,7 +1442,8 @@ void Engine::plan_tables()
         nodes += tr[w].n;
     }
     n_tab_nodes_ = nodes;
-    d_tab_.ensure((size_t)nodes * sizeof(double4));
+    d_tab_.ensure((size_t)nodes * kTabNode * sizeof(double));
+    d_tab32_.ensure((size_t)nodes * kTabNode * sizeof(float));
     d_tabrec_.ensure(tr.size() * sizeof(TabRec));
     HIP_CHECK(hipMemcpy(d_tabrec_.p, tr.data(), tr.size() * sizeof(TabRec), hipMemcpyHostToDevice));
 }
@@ -1339,6 +1474,13 @@ Engine::Engine(int device, const EngineModel& model, const double* cell6, const 
             core.push_back({model.core[w].c[i], model.core[w].e[i], model.core[w].p[i], 0});
         wr[w].val_off = (int)val.size();
         wr[w].val_n = (int)model.valence[w].c.size();
+        wr[w].zeta = model.dv_zeta[w];
+        wr[w].pad = 0;
+        for (int l = 0; l < 5; l++) {
+            const bool has = l < model.dv_nl[w];
+            wr[w].n[l] = has ? model.dv_n[5 * w + l] : -1;
+            wr[w].gscale[l] = has ? def_val_scale(model.dv_zeta[w], model.dv_n[5 * w + l], l) : 0.0;
+        }
         for (size_t i = 0; i < model.valence[w].c.size(); i++)
             val.push_back({model.valence[w].c[i], model.valence[w].e[i], model.valence[w].p[i], 0});
     }
@@ -1494,7 +1636,7 @@ void Engine::ensure_prep_buffers(Precision prec, bool mp)
     const int n_hkl = n_hkl_;
     const bool mixed = prec == kMixed;
     const size_t n_rec = (size_t)n_hkl * ops_.size();
-    d_radial_.ensure((size_t)n_hkl * n_types * kRadRec * (mixed ? sizeof(float) : sizeof(double)));
+    if (!use_otf(mp)) d_radial_.ensure((size_t)n_hkl * n_types * kRadRec * (mixed ? sizeof(float) : sizeof(double)));
     d_hklop_.ensure(n_rec * kHklopRec * (mixed ? sizeof(float) : sizeof(double)));
     if (mixed) d_phase_.ensure(n_rec * 4 * sizeof(double));
     d_radial32_.ensure((size_t)std::max(n_hkl, 1) * sizeof(double));
@@ -1531,11 +1673,13 @@ void Engine::run_prep(Precision prec, bool mp)
     if (model_.n_types > 0) {
         hipLaunchKernelGGL(k_tab_build, dim3((n_tab_nodes_ + T - 1) / T), dim3(T), 0, stream_,
                            d_tabrec_.as<TabRec>(), model_.n_wfn, wfns, d_core_.as<TermRec>(), d_val_.as<TermRec>(),
-                           n_tab_nodes_, d_tab_.as<double4>());
+                           n_tab_nodes_, d_tab_.as<double>(), d_tab32_.as<float>());
         HIP_CHECK(hipGetLastError());
+    }
+    if (model_.n_types > 0 && !use_otf(mp)) {
         int nt = n_hkl * model_.n_types;
         hipLaunchKernelGGL(k_radial_prep, dim3((nt + T - 1) / T), dim3(T), 0, stream_, hlen.as<double>(),
-                           d_tab_.as<double4>(), d_tabrec_.as<TabRec>(), types, wfns, d_val_.as<TermRec>(),
+                           d_tab_.as<double>(), d_tabrec_.as<TabRec>(), types, wfns, d_val_.as<TermRec>(),
                            model_.n_types, n_hkl,
                            mixed ? nullptr : d_radial_.as<double>(), mixed ? d_radial_.as<float>() : nullptr,
                            (mp && !mixed) ? d_radx_.as<double>() : nullptr,
@@ -1547,23 +1691,31 @@ void Engine::run_prep(Precision prec, bool mp)
     HIP_CHECK(hipGetLastError());
 }
 
-static const void* main_kernel(bool mixed, bool derivs, bool mp, bool diag)
+static const void* main_kernel(bool mixed, bool derivs, bool mp, bool diag, bool otf)
 {
+#define KM(M, D, P, G, O) (const void*)k_main<M, D, P, G, O>
+    if (mp) return mixed ? KM(true, true, true, false, false) : KM(false, true, true, false, false);
     if (mixed) {
-        if (mp) return (const void*)k_main<true, true, true, false>;
-        if (diag) return derivs ? (const void*)k_main<true, true, false, true> : (const void*)k_main<true, false, false, true>;
-        return derivs ? (const void*)k_main<true, true, false, false> : (const void*)k_main<true, false, false, false>;
+        if (diag) {
+            if (otf) return derivs ? KM(true, true, false, true, true) : KM(true, false, false, true, true);
+            return derivs ? KM(true, true, false, true, false) : KM(true, false, false, true, false);
+        }
+        if (otf) return derivs ? KM(true, true, false, false, true) : KM(true, false, false, false, true);
+        return derivs ? KM(true, true, false, false, false) : KM(true, false, false, false, false);
     }
-    if (mp) return (const void*)k_main<false, true, true, false>;
-    return derivs ? (const void*)k_main<false, true, false, false> : (const void*)k_main<false, false, false, false>;
+    if (otf) return derivs ? KM(false, true, false, false, true) : KM(false, false, false, false, true);
+    return derivs ? KM(false, true, false, false, false) : KM(false, false, false, false, false);
+#undef KM
 }
 
 int Engine::resident_waves(bool mixed, bool derivs, bool mp)
 {
-    const int key = (use_diag(mixed, mp) ? 8 : 0) + (mixed ? 4 : 0) + (derivs ? 2 : 0) + (mp ? 1 : 0);
+    const int key = (use_otf(mp) ? 16 : 0) + (use_diag(mixed, mp) ? 8 : 0) + (mixed ? 4 : 0) + (derivs ? 2 : 0) +
+                    (mp ? 1 : 0);
     if (resident_waves_[key] > 0) return resident_waves_[key];
     int blocks = 0;
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, main_kernel(mixed, derivs, mp, use_diag(mixed, mp)), 256, 0));
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, main_kernel(mixed, derivs, mp, use_diag(mixed, mp), use_otf(mp)),
+                                                               256, 0));
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, device_));
     resident_waves_[key] = std::max(1, blocks) * kWavesPerBlock * prop.multiProcessorCount;
@@ -1613,6 +1765,9 @@ void Engine::main_args(Precision prec, bool derivs, bool mp, int chunk, MainArgs
     a.per_xcd = (a.n_blocks + kXcds - 1) / kXcds;
     const bool diag = use_diag(mixed, mp);
     a.ht = diag ? d_ht_.as<double>() : nullptr;
+    a.tab = mixed ? d_tab32_.p : d_tab_.p;
+    a.tabs = d_tabrec_.as<TabRec>();
+    a.hlen = d_radial32_.as<double>();
     a.diag_mask = 0;
     for (int p = 0; p < 8; p++) {
         a.diag_op[p] = 0;
@@ -1634,7 +1789,7 @@ void Engine::launch_main(Precision prec, bool derivs, bool mp, int chunk)
     last_grid_x = a.grid_x;
     last_grid_y = a.n_blocks / std::max(a.grid_x, 1);
     last_chunk = chunk;
-    const void* kern = main_kernel(prec == kMixed, derivs, mp, use_diag(prec == kMixed, mp));
+    const void* kern = main_kernel(prec == kMixed, derivs, mp, use_diag(prec == kMixed, mp), use_otf(mp));
     void* args[] = {&a};
     HIP_CHECK(hipLaunchKernel(kern, grid, dim3(256), args, 0, stream_));
 }
@@ -1649,8 +1804,9 @@ int Engine::chunk_for(Precision prec, bool derivs, bool mp)
 
 long long Engine::shape_key(Precision prec, bool derivs, bool mp) const
 {
-    return (((long long)n_sorted_ * 1000003LL + n_hkl_) * 64 + (long long)ops_.size()) * 16 +
-           (use_diag(prec == kMixed, mp) ? 8 : 0) + (prec == kMixed ? 4 : 0) + (derivs ? 2 : 0) + (mp ? 1 : 0);
+    return (((long long)n_sorted_ * 1000003LL + n_hkl_) * 64 + (long long)ops_.size()) * 32 +
+           (use_otf(mp) ? 16 : 0) + (use_diag(prec == kMixed, mp) ? 8 : 0) + (prec == kMixed ? 4 : 0) +
+           (derivs ? 2 : 0) + (mp ? 1 : 0);
 }
 
 void Engine::run(Precision prec, bool derivs, bool mp)
@@ -1779,7 +1935,8 @@ struct BatchItem {
     const TermRec* val;
     const TypeRec* types;
     const TabRec* tabs;
-    double4* tab;
+    double* tab;
+    float* tab32;
     double* rad64;
     float* rad32;
     const AtomIn* ain;
@@ -1793,7 +1950,7 @@ struct BatchItem {
 };
 
 // index spaces of the batched launches; offsets are prefix sums over structures
-enum BatchSpace { kSpFill = 0, kSpHkl, kSpTab, kSpRad, kSpSlot, kSpRedD, kSpUnit, kNumSpaces };
+enum BatchSpace { kSpFill = 0, kSpHkl, kSpTab, kSpSlot, kSpRedD, kSpUnit, kNumSpaces };
 
 // structure b with off[b] <= i < off[b + 1] (empty structures are skipped)
 __device__ __forceinline__ int find_item(const int* __restrict__ off, int nb, int i)
@@ -1834,18 +1991,9 @@ __global__ void k_batch_tab_build(const BatchItem* __restrict__ it, const int* _
     if (i >= total) return;
     const int b = find_item(off, nb, i);
     const BatchItem& q = it[b];
-    tab_build_item(i - off[b], q.tabs, q.n_wfn, q.wfns, q.core, q.val, q.tab);
+    tab_build_item(i - off[b], q.tabs, q.n_wfn, q.wfns, q.core, q.val, q.tab, q.tab32);
 }
 
-__global__ void k_batch_radial_prep(const BatchItem* __restrict__ it, const int* __restrict__ off, int nb, int total)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int b = find_item(off, nb, i);
-    const BatchItem& q = it[b];
-    radial_prep_item(i - off[b], q.hlen, q.tab, q.tabs, q.types, q.wfns, q.val, q.n_types, q.n_hkl, q.rad64, q.rad32,
-                     nullptr, nullptr);
-}
 
 __global__ __launch_bounds__(64) void k_batch_atom_prep(const BatchItem* __restrict__ it, const int* __restrict__ off,
                                                         int nb)
@@ -1858,7 +2006,8 @@ __global__ __launch_bounds__(64) void k_batch_atom_prep(const BatchItem* __restr
 
 // Persistent: every workgroup takes (structure, atom block, reflection chunk) units from the
 // device counter until none is left, so uneven structures share the machine without a tail per
-// structure.  Units are ordered costliest structure first.
+// structure.  Units are ordered costliest structure first.  Radial factors are evaluated on the
+// fly from each structure's per-wavefunction tables (no per-(reflection, type) records).
 template <bool MIXED, bool DERIV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 1))) void k_batch_main(
     const BatchItem* __restrict__ it, const int* __restrict__ off, int nb, int total, int* __restrict__ counter)
@@ -1881,10 +2030,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 :
         const int chunk_id = local / a.grid_x;
         const int ablock = local - chunk_id * a.grid_x;
         if constexpr (MIXED) {
-            if (a.diag_mask) main_unit<true, DERIV, false, true>(a, chunk_id, ablock, wib, lane, stage);
-            else main_unit<true, DERIV, false, false>(a, chunk_id, ablock, wib, lane, stage);
+            if (a.diag_mask) main_unit<true, DERIV, false, true, true>(a, chunk_id, ablock, wib, lane, stage);
+            else main_unit<true, DERIV, false, false, true>(a, chunk_id, ablock, wib, lane, stage);
         } else {
-            main_unit<false, DERIV, false, false>(a, chunk_id, ablock, wib, lane, stage);
+            main_unit<false, DERIV, false, false, true>(a, chunk_id, ablock, wib, lane, stage);
         }
     }
 }
@@ -1931,7 +2080,8 @@ void Engine::batch_item(Precision prec, bool derivs, int chunk, BatchItem* out)
     q.core = d_core_.as<TermRec>();
     q.val = d_val_.as<TermRec>();
     q.tabs = d_tabrec_.as<TabRec>();
-    q.tab = d_tab_.as<double4>();
+    q.tab = d_tab_.as<double>();
+    q.tab32 = d_tab32_.as<float>();
     q.n_tab = model_.n_types > 0 ? n_tab_nodes_ : 0;
     q.rad64 = mixed ? nullptr : d_radial_.as<double>();
     q.rad32 = mixed ? d_radial_.as<float>() : nullptr;
@@ -2030,7 +2180,6 @@ void BatchEngine::plan(Precision prec, bool derivs, int chunk)
                 case kSpFill: n = (long long)q.n_atoms * q.nd; break;
                 case kSpHkl: n = q.n_hkl; break;
                 case kSpTab: n = q.n_tab; break;
-                case kSpRad: n = (long long)q.n_hkl * q.n_types; break;
                 case kSpSlot: n = q.n_slots; break;
                 case kSpRedD: n = derivs ? (long long)q.n_slots * q.nd : 0; break;
                 case kSpUnit: n = (q.main.n_waves > 0) ? q.main.n_blocks : 0; break;
@@ -2069,9 +2218,6 @@ void BatchEngine::launch(Precision prec, bool derivs)
     if (total_[kSpTab] > 0)
         hipLaunchKernelGGL(k_batch_tab_build, blocks(total_[kSpTab], T), dim3(T), 0, stream_, it, o(kSpTab), nb,
                            total_[kSpTab]);
-    if (total_[kSpRad] > 0)
-        hipLaunchKernelGGL(k_batch_radial_prep, blocks(total_[kSpRad], T), dim3(T), 0, stream_, it, o(kSpRad), nb,
-                           total_[kSpRad]);
     if (total_[kSpSlot] > 0)
         hipLaunchKernelGGL(k_batch_atom_prep, dim3(total_[kSpSlot]), dim3(64), 0, stream_, it, o(kSpSlot), nb);
     HIP_CHECK(hipGetLastError());

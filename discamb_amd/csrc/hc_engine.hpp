@@ -34,6 +34,7 @@ struct DeviceBuffer {
 struct TuneParams {
     int hkl_chunk = 0;       // reflections per workgroup column; 0 = automatic
     bool general_path = false;  // force the general-rotation kernels for diagonal groups (tests)
+    int radial_mode = 0;        // 0 automatic, 1 per-(reflection, type) records, 2 on-the-fly tables
 };
 
 // Atom data for one call, in the order the caller supplies them (already converted to the
@@ -105,7 +106,16 @@ private:
     // parity-folded kernels: mixed precision only (the FP64 variant needs twice the registers for
     // its class sums and loses more to occupancy than it saves)
     bool use_diag(bool mixed, bool mp) const { return diag_ && mixed && !mp && !tune.general_path; }
-    int resident_waves_[16] = {};
+    // radial factors evaluated per (atom, reflection) from the per-wavefunction tables instead of
+    // per-(reflection, type) records: chosen when types are not shared by many atoms (the records
+    // would then cost more to write and read than the lookups), never with the multipole adjoint
+    bool use_otf(bool mp) const
+    {
+        if (mp) return false;
+        if (tune.radial_mode != 0) return tune.radial_mode == 2;
+        return (long long)model_.n_types * 8 > (long long)n_sorted_;
+    }
+    int resident_waves_[32] = {};
     bool diag_ = false;               // every operation is a diagonal sign matrix
     int diag_pat_[8] = {};
     float diag_anom_[8] = {};
@@ -125,7 +135,7 @@ private:
     DeviceBuffer d_core_, d_val_, d_type_, d_ops_;
     // per call
     DeviceBuffer d_hkl_, d_dtdf_, d_atoms_in_, d_atoms_, d_hklop_, d_hklop32_, d_phase_, d_radial_,
-        d_radial32_, d_ht_, d_fpart_, d_dpart_, d_f_, d_d_, d_order_, d_radx_, d_tab_, d_tabrec_;
+        d_radial32_, d_ht_, d_fpart_, d_dpart_, d_f_, d_d_, d_order_, d_radx_, d_tab_, d_tab32_, d_tabrec_;
     double h_max_ = 0.0;            // largest |h| of the uploaded reflections
     int n_tab_nodes_ = 0;           // form-factor table nodes over all wavefunctions
 };

@@ -249,3 +249,54 @@ def small_structure(sg, n_atoms=12, d_min=1.0, seed=7, cell=None, lmax_heavy=4):
     els = ["C", "N", "O", "S", "H", "H"]
     comp = [els[i % len(els)] for i in range(n_atoms)]
     return build_workload("small_%s" % sg, rng, cell, sg, comp, d_min, None, lmax_heavy)
+
+
+BATCH_GROUPS = ["P1", "P-1", "P21/c", "C2/c", "P212121"]
+_LAUE_ORDER = {"P1": 2, "P-1": 2, "P21/c": 4, "C2/c": 4, "P212121": 8}
+
+
+def config4(n_structures=256, seed=4000, max_refl=40000, min_refl=2000):
+    """Uneven small-molecule structures for batched refinement steps.
+
+    Atoms per asymmetric unit log-uniform in [20, 400] (composition C14 N4 O6 H36 scaled), space
+    group uniform over BATCH_GROUPS, cell edges U(6, 25) A (triclinic angles U(80, 100), monoclinic
+    beta U(95, 110)), d_min U(0.5, 0.9) A, parameters as in config0 (one type per atom).  Two
+    adjustments keep the structures physical and the reflection counts in the 2k-40k range: a cell
+    that cannot hold the atoms at 11 A^3 per atom is scaled up isotropically, and d_min is raised
+    (lowered, not below 0.5 A) when the expected unique-reflection count exceeds max_refl (falls
+    below min_refl)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for b in range(n_structures):
+        n = int(round(np.exp(rng.uniform(np.log(20), np.log(400)))))
+        sg = BATCH_GROUPS[rng.integers(len(BATCH_GROUPS))]
+        edges = rng.uniform(6.0, 25.0, 3)
+        if sg in ("P1", "P-1"):
+            ang = rng.uniform(80.0, 100.0, 3)
+        elif sg in ("P21/c", "C2/c"):
+            ang = np.array([90.0, rng.uniform(95.0, 110.0), 90.0])
+        else:
+            ang = np.array([90.0, 90.0, 90.0])
+        cell = tuple(edges) + tuple(ang)
+        f2c, _ = cell_matrices(cell)
+        vol = abs(np.linalg.det(f2c))
+        need = 11.0 * n * len(SPACE_GROUPS[sg])
+        if vol < need:
+            edges = edges * (need / vol) ** (1.0 / 3.0)
+            cell = tuple(edges) + tuple(ang)
+            vol = need
+        d_min = rng.uniform(0.5, 0.9)
+        est = 4.18879 * vol / d_min ** 3 / _LAUE_ORDER[sg]
+        if est > max_refl:
+            d_min = (4.18879 * vol / (_LAUE_ORDER[sg] * max_refl)) ** (1.0 / 3.0)
+        elif est < min_refl:
+            d_min = max(0.5, (4.18879 * vol / (_LAUE_ORDER[sg] * min_refl)) ** (1.0 / 3.0))
+        comp = []
+        for el, k in (("C", 14), ("N", 4), ("O", 6)):
+            comp += [el] * int(round(n * k / 60.0))
+        comp += ["H"] * max(0, n - len(comp))
+        comp = comp[:n]
+        sub = np.random.Generator(np.random.PCG64(seed * 1000 + b))
+        out.append(build_workload("batch%03d_%s_%dat_%.2fA" % (b, sg.replace("/", ""), n, d_min), sub, cell, sg, comp,
+                                  d_min))
+    return out

@@ -121,6 +121,11 @@ int dh_hc_set_tuning(dh_calculator* calc, int hkl_chunk);
  * DH_OPT_GENERAL_ROTATIONS: use the general-rotation kernels even when every symmetry operation is
  * a diagonal sign matrix (the engine otherwise folds such operations through harmonic parities). */
 #define DH_OPT_GENERAL_ROTATIONS 1u
+/* DH_OPT_RADIAL_RECORDS / DH_OPT_RADIAL_TABLES: radial factors from per-(reflection, atom type)
+ * records, or evaluated per (atom, reflection) from per-wavefunction tables; by default the
+ * engine takes tables when atom types are not shared by at least eight atoms on average. */
+#define DH_OPT_RADIAL_RECORDS 2u
+#define DH_OPT_RADIAL_TABLES 4u
 int dh_hc_set_options(dh_calculator* calc, unsigned flags);
 
 /* calculateStructureFactorsAndDerivatives: synchronous, host pointers. hkl is 3*n_hkl int,

@@ -16,14 +16,16 @@ pytestmark = pytest.mark.gpu
 TOL = {"HIP": (1e-10, 1e-10), "HIP_MIXED": (1e-6, 1e-5)}
 
 
-def _compare(w, mode, atoms=None, include=None, anomalous=None, hkl=None, dtdf=None, tol=None, general=False):
+def _compare(w, mode, atoms=None, include=None, anomalous=None, hkl=None, dtdf=None, tol=None, general=False,
+             radial="auto"):
     atoms = atoms or w.crystal.atoms
     hkl = w.hkl if hkl is None else hkl
     dtdf = w.dtdf if dtdf is None else dtdf
-    print("case %s mode=%s general=%s atoms=%d hkl=%d" % (w.name, mode, general, atoms.n, len(hkl)), flush=True)
+    print("case %s mode=%s general=%s radial=%s atoms=%d hkl=%d" % (w.name, mode, general, radial, atoms.n, len(hkl)),
+          flush=True)
     calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
     calc.setCalculationMode(mode)
-    calc.setKernelOptions(general_rotations=general)
+    calc.setKernelOptions(general_rotations=general, radial=radial)
     f, d = calc.calculateStructureFactorsAndDerivatives(atoms, w.lcs, hkl, dtdf, include, anomalous=anomalous)
     rf, rx, ru, ro = hc_oracle.sf_and_derivs(w.crystal, w.params, atoms, w.lcs, hkl, dtdf, include=include,
                                              n_threads=16, anomalous=anomalous)
@@ -39,11 +41,14 @@ def _compare(w, mode, atoms=None, include=None, anomalous=None, hkl=None, dtdf=N
 
 # "general" forces the general-rotation kernels; triclinic, monoclinic and orthorhombic groups
 # otherwise run the parity-folded kernels for diagonal operations (P43212 and P31 always general).
+# radial: per-(reflection, type) records or per-atom lookups in per-wavefunction tables (the
+# default for these one-type-per-atom structures)
+@pytest.mark.parametrize("radial", ["auto", "records"])
 @pytest.mark.parametrize("general", [False, True])
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
 @pytest.mark.parametrize("sg", list(synth.SPACE_GROUPS))
-def test_space_groups(sg, mode, general):
-    _compare(synth.small_structure(sg, n_atoms=14, d_min=0.9), mode, general=general)
+def test_space_groups(sg, mode, general, radial):
+    _compare(synth.small_structure(sg, n_atoms=14, d_min=0.9), mode, general=general, radial=radial)
 
 
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
@@ -68,13 +73,14 @@ def test_config0_full(mode):
     _compare(synth.config0(), mode)
 
 
+@pytest.mark.parametrize("radial", ["auto", "tables"])
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
-def test_config1_reflection_subset(mode):
+def test_config1_reflection_subset(mode, radial):
     """5000-atom P2_1 2_1 2_1 structure, all atoms, a 3000-reflection subset spanning resolution."""
     w = synth.config1()
     rng = np.random.default_rng(5)
     sel = np.sort(rng.choice(len(w.hkl), 3000, replace=False))
-    _compare(w, mode, hkl=w.hkl[sel], dtdf=w.dtdf[sel])
+    _compare(w, mode, hkl=w.hkl[sel], dtdf=w.dtdf[sel], radial=radial)
 
 
 def test_include_mask_anomalous_and_h000():

@@ -1504,7 +1504,8 @@ Engine::Engine(int device, const EngineModel& model, const double* cell6, const 
         od[s].anom_sign = ops[s].anom_sign;
     }
     // every operation diagonal with equal fractional / Cartesian signs -> parity-class path
-    diag_ = !ops.empty() && ops.size() <= 8;
+    // (with a single operation the direct contraction is cheaper than the class sums)
+    diag_ = ops.size() >= 2 && ops.size() <= 8;
     for (size_t s = 0; s < ops.size() && diag_; s++) {
         int pat = 0;
         for (int i = 0; i < 3; i++)
@@ -1593,23 +1594,39 @@ void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const doub
         HIP_CHECK(hipMemcpyAsync(static_cast<char*>(d_atoms_in_.p) + ain.size() * sizeof(AtomIn), wave_cls.data(),
                                  wave_cls.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
     d_atoms_.ensure(ain.size() * sizeof(AtomRec));
-    // reflections; |h| range for the form-factor tables
-    double h2max = 0.0;
+    // Reflections are processed in order of increasing |h| (stable; hkl_perm_[i] = caller index of
+    // position i): consecutive reflections of a wave then read neighbouring nodes of the radial
+    // tables.  |h| range for the form-factor tables.
+    std::vector<double> h2(n_hkl);
     for (int k = 0; k < n_hkl; k++) {
-        double h2 = 0.0;
+        double v = 0.0;
         for (int j = 0; j < 3; j++) {
             const double hc = hkl[3 * k] * c2f_[j] + hkl[3 * k + 1] * c2f_[3 + j] + hkl[3 * k + 2] * c2f_[6 + j];
-            h2 += hc * hc;
+            v += hc * hc;
         }
-        h2max = std::max(h2max, h2);
+        h2[k] = v;
     }
-    h_max_ = std::sqrt(h2max);
+    hkl_perm_.resize(n_hkl);
+    for (int k = 0; k < n_hkl; k++) hkl_perm_[k] = k;
+    std::stable_sort(hkl_perm_.begin(), hkl_perm_.end(), [&](int x, int y) { return h2[x] < h2[y]; });
+    h_max_ = n_hkl > 0 ? std::sqrt(h2[hkl_perm_[n_hkl - 1]]) : 0.0;
     plan_tables();
+    std::vector<int> hkl_s(3 * (size_t)std::max(n_hkl, 1));
+    std::vector<double> dtdf_s(3 * (size_t)std::max(n_hkl, 1));
+    for (int i = 0; i < n_hkl; i++) {
+        const int k = hkl_perm_[i];
+        for (int j = 0; j < 3; j++) hkl_s[3 * i + j] = hkl[3 * k + j];
+        dtdf_s[2 * i] = dtdf[2 * k];
+        dtdf_s[2 * i + 1] = dtdf[2 * k + 1];
+        dtdf_s[2 * (size_t)n_hkl + i] = dtdf[2 * (size_t)n_hkl + k];
+    }
     d_hkl_.ensure(std::max(n_hkl, 1) * 3 * sizeof(int));
     d_dtdf_.ensure(std::max(n_hkl, 1) * 3 * sizeof(double));   // dtdf (2) + mult (1)
     if (n_hkl > 0) {
-        HIP_CHECK(hipMemcpyAsync(d_hkl_.p, hkl, (size_t)n_hkl * 3 * sizeof(int), hipMemcpyHostToDevice, stream_));
-        HIP_CHECK(hipMemcpyAsync(d_dtdf_.p, dtdf, (size_t)n_hkl * 3 * sizeof(double), hipMemcpyHostToDevice, stream_));
+        HIP_CHECK(hipMemcpyAsync(d_hkl_.p, hkl_s.data(), (size_t)n_hkl * 3 * sizeof(int), hipMemcpyHostToDevice,
+                                 stream_));
+        HIP_CHECK(hipMemcpyAsync(d_dtdf_.p, dtdf_s.data(), (size_t)n_hkl * 3 * sizeof(double), hipMemcpyHostToDevice,
+                                 stream_));
     }
     // keep host staging alive until the copies complete
     HIP_CHECK(hipStreamSynchronize(stream_));
@@ -1910,12 +1927,21 @@ void Engine::download(double* f, double* d10)
 {
     HIP_CHECK(hipSetDevice(device_));
     if (!ran_) throw Error(1, "no results: run the calculation before fetching");
-    if (f && n_hkl_ > 0)
-        HIP_CHECK(hipMemcpyAsync(f, d_f_.p, (size_t)n_hkl_ * 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    std::vector<double> fs;
+    if (f && n_hkl_ > 0) {
+        fs.resize(2 * (size_t)n_hkl_);
+        HIP_CHECK(hipMemcpyAsync(fs.data(), d_f_.p, (size_t)n_hkl_ * 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                 stream_));
+    }
     if (d10 && n_atoms_ > 0)
         HIP_CHECK(hipMemcpyAsync(d10, d_d_.p, (size_t)n_atoms_ * last_nd_ * sizeof(double), hipMemcpyDeviceToHost,
                                  stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
+    if (f)
+        for (int i = 0; i < n_hkl_; i++) {
+            f[2 * (size_t)hkl_perm_[i]] = fs[2 * (size_t)i];
+            f[2 * (size_t)hkl_perm_[i] + 1] = fs[2 * (size_t)i + 1];
+        }
 }
 
 // ------------------------------------------------------------------ batched structures
@@ -1950,7 +1976,8 @@ struct BatchItem {
 };
 
 // index spaces of the batched launches; offsets are prefix sums over structures
-enum BatchSpace { kSpFill = 0, kSpHkl, kSpTab, kSpSlot, kSpRedD, kSpUnit, kNumSpaces };
+// units are split by kernel flavour: parity-folded (kSpUnitD) and general rotations (kSpUnitG)
+enum BatchSpace { kSpFill = 0, kSpHkl, kSpTab, kSpSlot, kSpRedD, kSpUnitD, kSpUnitG, kNumSpaces };
 
 // structure b with off[b] <= i < off[b + 1] (empty structures are skipped)
 __device__ __forceinline__ int find_item(const int* __restrict__ off, int nb, int i)
@@ -1968,7 +1995,7 @@ __global__ void k_batch_fill(const BatchItem* __restrict__ it, const int* __rest
                              int* __restrict__ counter)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) counter[0] = 0;         // work counter of the persistent main kernel
+    if (i == 0) counter[0] = counter[1] = 0;    // work counters of the persistent main kernels
     if (i >= total) return;
     const int b = find_item(off, nb, i);
     it[b].d[i - off[b]] = 0.0;
@@ -2008,7 +2035,7 @@ __global__ __launch_bounds__(64) void k_batch_atom_prep(const BatchItem* __restr
 // device counter until none is left, so uneven structures share the machine without a tail per
 // structure.  Units are ordered costliest structure first.  Radial factors are evaluated on the
 // fly from each structure's per-wavefunction tables (no per-(reflection, type) records).
-template <bool MIXED, bool DERIV>
+template <bool MIXED, bool DERIV, bool DIAG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 1))) void k_batch_main(
     const BatchItem* __restrict__ it, const int* __restrict__ off, int nb, int total, int* __restrict__ counter)
 {
@@ -2029,12 +2056,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 :
         const int local = u - off[b];
         const int chunk_id = local / a.grid_x;
         const int ablock = local - chunk_id * a.grid_x;
-        if constexpr (MIXED) {
-            if (a.diag_mask) main_unit<true, DERIV, false, true, true>(a, chunk_id, ablock, wib, lane, stage);
-            else main_unit<true, DERIV, false, false, true>(a, chunk_id, ablock, wib, lane, stage);
-        } else {
-            main_unit<false, DERIV, false, false, true>(a, chunk_id, ablock, wib, lane, stage);
-        }
+        main_unit<MIXED, DERIV, false, DIAG, true>(a, chunk_id, ablock, wib, lane, stage);
     }
 }
 
@@ -2138,10 +2160,13 @@ void BatchEngine::drop_graph()
     graph_key_.clear();
 }
 
-static const void* batch_main_kernel(bool mixed, bool derivs)
+static const void* batch_main_kernel(bool mixed, bool derivs, bool diag)
 {
-    if (mixed) return derivs ? (const void*)k_batch_main<true, true> : (const void*)k_batch_main<true, false>;
-    return derivs ? (const void*)k_batch_main<false, true> : (const void*)k_batch_main<false, false>;
+    if (mixed) {
+        if (diag) return derivs ? (const void*)k_batch_main<true, true, true> : (const void*)k_batch_main<true, false, true>;
+        return derivs ? (const void*)k_batch_main<true, true, false> : (const void*)k_batch_main<true, false, false>;
+    }
+    return derivs ? (const void*)k_batch_main<false, true, false> : (const void*)k_batch_main<false, false, false>;
 }
 
 int BatchEngine::chunk_for(Precision prec, bool derivs)
@@ -2182,7 +2207,8 @@ void BatchEngine::plan(Precision prec, bool derivs, int chunk)
                 case kSpTab: n = q.n_tab; break;
                 case kSpSlot: n = q.n_slots; break;
                 case kSpRedD: n = derivs ? (long long)q.n_slots * q.nd : 0; break;
-                case kSpUnit: n = (q.main.n_waves > 0) ? q.main.n_blocks : 0; break;
+                case kSpUnitD: n = (q.main.n_waves > 0 && q.main.diag_mask) ? q.main.n_blocks : 0; break;
+                case kSpUnitG: n = (q.main.n_waves > 0 && !q.main.diag_mask) ? q.main.n_blocks : 0; break;
             }
             acc += n;
         }
@@ -2198,7 +2224,7 @@ void BatchEngine::plan(Precision prec, bool derivs, int chunk)
     HIP_CHECK(hipMemcpyAsync(d_off_.p, off_.data(), off_.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
     last_chunk = chunk;
-    last_units = total_[kSpUnit];
+    last_units = total_[kSpUnitD] + total_[kSpUnitG];
 }
 
 void BatchEngine::launch(Precision prec, bool derivs)
@@ -2221,14 +2247,18 @@ void BatchEngine::launch(Precision prec, bool derivs)
     if (total_[kSpSlot] > 0)
         hipLaunchKernelGGL(k_batch_atom_prep, dim3(total_[kSpSlot]), dim3(64), 0, stream_, it, o(kSpSlot), nb);
     HIP_CHECK(hipGetLastError());
-    if (total_[kSpUnit] > 0) {
-        const int grid = std::min(grid_[(mixed ? 2 : 0) + (derivs ? 1 : 0)], total_[kSpUnit]);
-        last_grid = grid;
-        const int nbv = nb, tot = total_[kSpUnit];
-        const int* ou = o(kSpUnit);
-        int* cnt = d_counter_.as<int>();
+    last_grid = 0;
+    for (int flavour = 0; flavour < 2; flavour++) {
+        const int sp = flavour == 0 ? kSpUnitD : kSpUnitG;
+        if (total_[sp] == 0) continue;
+        const bool diag = flavour == 0;
+        const int grid = std::min(grid_[(diag ? 4 : 0) + (mixed ? 2 : 0) + (derivs ? 1 : 0)], total_[sp]);
+        last_grid = std::max(last_grid, grid);
+        const int nbv = nb, tot = total_[sp];
+        const int* ou = o(sp);
+        int* cnt = d_counter_.as<int>() + flavour;
         void* args[] = {(void*)&it, (void*)&ou, (void*)&nbv, (void*)&tot, (void*)&cnt};
-        HIP_CHECK(hipLaunchKernel(batch_main_kernel(mixed, derivs), dim3(grid), dim3(256), args, 0, stream_));
+        HIP_CHECK(hipLaunchKernel(batch_main_kernel(mixed, derivs, diag), dim3(grid), dim3(256), args, 0, stream_));
     }
     const int R = 64;
     if (total_[kSpHkl] > 0)
@@ -2243,15 +2273,17 @@ void BatchEngine::launch(Precision prec, bool derivs)
 void BatchEngine::run(Precision prec, bool derivs, bool graph)
 {
     HIP_CHECK(hipSetDevice(device_));
-    int& g = grid_[(prec == kMixed ? 2 : 0) + (derivs ? 1 : 0)];
-    if (g == 0) {
-        // persistent grid: every workgroup the device holds at once
-        int per_cu = 0;
-        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_main_kernel(prec == kMixed, derivs), 256,
-                                                               0));
-        hipDeviceProp_t prop;
-        HIP_CHECK(hipGetDeviceProperties(&prop, device_));
-        g = std::max(1, per_cu) * prop.multiProcessorCount;
+    for (int diag = 0; diag < 2; diag++) {
+        int& g = grid_[(diag ? 4 : 0) + (prec == kMixed ? 2 : 0) + (derivs ? 1 : 0)];
+        if (g == 0) {
+            // persistent grid: every workgroup the device holds at once
+            int per_cu = 0;
+            HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, batch_main_kernel(prec == kMixed, derivs, diag && prec == kMixed), 256, 0));
+            hipDeviceProp_t prop;
+            HIP_CHECK(hipGetDeviceProperties(&prop, device_));
+            g = std::max(1, per_cu) * prop.multiProcessorCount;
+        }
     }
     const int chunk = chunk_for(prec, derivs);
     plan(prec, derivs, chunk);

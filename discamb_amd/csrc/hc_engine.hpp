@@ -131,6 +131,7 @@ private:
     int n_atoms_ = 0, n_sorted_ = 0, n_hkl_ = 0;
     bool ran_ = false;
     std::vector<int> order_;        // sorted slot -> caller atom index (-1 padding)
+    std::vector<int> hkl_perm_;     // device reflection position -> caller reflection index
     // model tables
     DeviceBuffer d_core_, d_val_, d_type_, d_ops_;
     // per call
@@ -176,7 +177,7 @@ private:
     std::vector<unsigned char> graph_key_; // items_ + off_ + precision at capture time
     hipGraphExec_t exec_ = nullptr;
     std::map<long long, int> tuned_;
-    int grid_[4] = {0, 0, 0, 0};           // persistent main-kernel grid per (precision, derivatives)
+    int grid_[8] = {};                     // persistent main-kernel grid per (flavour, precision, derivatives)
 };
 
 }  // namespace discamb_amd

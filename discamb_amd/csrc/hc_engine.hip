@@ -196,6 +196,32 @@ __device__ __forceinline__ void hkl_prep_diag_item(int k, const int* __restrict_
         ht[(size_t)k * n_ops + s] = h0 * ops[s].tf[0] + h1 * ops[s].tf[1] + h2 * ops[s].tf[2];
 }
 
+// DPP lane sums (quad_perm xor 1, xor 2, row_half_mirror): total of eight adjacent lanes in each.
+template <int CTRL>
+__device__ __forceinline__ float dpp_add(float v)
+{
+    return v + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF,
+                                                                      false));
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_add(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const unsigned hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return v + __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// sum over the eight lanes 8j .. 8j+7 (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror)
+template <typename T>
+__device__ __forceinline__ T octet_sum(T v)
+{
+    v = dpp_add<0xB1>(v);
+    v = dpp_add<0x4E>(v);
+    return dpp_add<0x141>(v);
+}
+
 // Radial functions of the model are smooth in s and shared by every atom type of a wavefunction:
 // core and valence form factors 4 pi sum_k c_k g_0(p_k + 2, s, e_k) and the deformation-valence
 // Fourier-Bessel transforms G_l(s) = 4 pi (i^l sign) N_l g_l(n_l + 2, s, zeta), l = 0..4.  They are
@@ -212,44 +238,47 @@ struct TabRec {
     int off, n;        // first node, node count
 };
 
-__device__ __forceinline__ void tab_build_item(int j, const TabRec* __restrict__ tabs, int n_wfn,
+// Node j of the tables, one 64-lane wave: lanes 8c .. 8c+7 build component c (0 core, 1 valence,
+// 2..6 G_0..G_4, 7 padding), value and d/ds; the Slater terms of a component are spread over its
+// eight lanes and summed with a fixed-order DPP reduction.
+__device__ __forceinline__ void tab_build_node(int j, int lane, const TabRec* __restrict__ tabs, int n_wfn,
                                                const WfnRec* __restrict__ wfns, const TermRec* __restrict__ core,
                                                const TermRec* __restrict__ val, double* __restrict__ tab,
                                                float* __restrict__ tab32)
 {
+    const int c = lane >> 3, q = lane & 7;
     int w = 0;
     while (w + 1 < n_wfn && tabs[w + 1].off <= j) w++;
     const double sv = (j - tabs[w].off) * kTabStep;
     const WfnRec& wf = wfns[w];
-    double v[kTabNode];
-    double fc = 0.0, dc = 0.0, fv = 0.0, dv = 0.0;
-    for (int i = 0; i < wf.core_n; i++) {
-        const TermRec tr = core[wf.core_off + i];
-        double dg;
-        fc += tr.c * g_function_dh(0, tr.p + 2, sv, tr.e, &dg);
-        dc += tr.c * dg;
+    double f = 0.0, d = 0.0;
+    if (c < 2) {
+        const TermRec* terms = c == 0 ? core + wf.core_off : val + wf.val_off;
+        const int nt = c == 0 ? wf.core_n : wf.val_n;
+        for (int i = q; i < nt; i += 8) {
+            const TermRec tr = terms[i];
+            double dg;
+            f += tr.c * g_function_dh(0, tr.p + 2, sv, tr.e, &dg);
+            d += tr.c * dg;
+        }
+        const double four_pi = 12.566370614359172953850;
+        f *= four_pi;
+        d *= four_pi;
+    } else if (c < 7 && q == 0) {
+        const int l = c - 2;
+        if (wf.n[l] >= 0) {
+            double dg;
+            f = wf.gscale[l] * g_function_dh(l, wf.n[l] + 2, sv, wf.zeta, &dg);
+            d = wf.gscale[l] * dg;
+        }
     }
-    for (int i = 0; i < wf.val_n; i++) {
-        const TermRec tr = val[wf.val_off + i];
-        double dg;
-        fv += tr.c * g_function_dh(0, tr.p + 2, sv, tr.e, &dg);
-        dv += tr.c * dg;
-    }
-    const double four_pi = 12.566370614359172953850;
-    v[0] = four_pi * fc;
-    v[1] = four_pi * dc;
-    v[2] = four_pi * fv;
-    v[3] = four_pi * dv;
-    for (int l = 0; l < 5; l++) {
-        double dg = 0.0, g = 0.0;
-        if (wf.n[l] >= 0) g = g_function_dh(l, wf.n[l] + 2, sv, wf.zeta, &dg);
-        v[4 + 2 * l] = wf.gscale[l] * g;
-        v[5 + 2 * l] = wf.gscale[l] * dg;
-    }
-    v[14] = v[15] = 0.0;
-    for (int q = 0; q < kTabNode; q++) {
-        tab[(size_t)j * kTabNode + q] = v[q];
-        tab32[(size_t)j * kTabNode + q] = (float)v[q];
+    f = octet_sum(f);
+    d = octet_sum(d);
+    if (q == 0) {
+        tab[(size_t)j * kTabNode + 2 * c] = f;
+        tab[(size_t)j * kTabNode + 2 * c + 1] = d;
+        tab32[(size_t)j * kTabNode + 2 * c] = (float)f;
+        tab32[(size_t)j * kTabNode + 2 * c + 1] = (float)d;
     }
 }
 
@@ -336,16 +365,21 @@ __device__ __forceinline__ void radial_prep_item(int idx, const double* __restri
     // d f_val(h/kappa) / d kappa = f_val'(h/kappa) * (-h / kappa^2)
     x[1] = ty.pval * dfv * (-hk * ty.inv_kappa);
     const double hp = h * ty.inv_kappa_p;
-    for (int l = 0; l <= ty.lp; l++) {
-        const int n = ty.n[l];
-        if (want_d) {
+    if (want_d) {
+        for (int l = 0; l <= ty.lp; l++) {
             double dg;
-            const double g = g_function_dh(l, n + 2, hp, ty.zeta, &dg);
+            const double g = g_function_dh(l, ty.n[l] + 2, hp, ty.zeta, &dg);
             r[1 + l] = ty.gscale[l] * g;
             x[2 + l] = ty.gscale[l] * dg * (-hp * ty.inv_kappa_p);
-        } else {
-            r[1 + l] = ty.gscale[l] * g_function(l, n + 2, hp, ty.zeta);
         }
+    } else {
+        // G_l from the wavefunction's table (same Hermite basis for every l)
+        double wt[4];
+        const int i = hermite<double>(hp * (1.0 / kTabStep), tr.n, wt);
+        const double* a = tab + (size_t)(tr.off + i) * kTabNode + 4;
+        for (int l = 0; l <= ty.lp; l++)
+            r[1 + l] = wt[0] * a[2 * l] + wt[1] * a[kTabNode + 2 * l] + wt[2] * a[2 * l + 1] +
+                       wt[3] * a[kTabNode + 2 * l + 1];
     }
     size_t base = ((size_t)k * n_types + t) * kRadRec;
     if (rad64)
@@ -428,12 +462,12 @@ __global__ void k_hkl_prep_diag(const int* __restrict__ hkl, const OpDev* __rest
     if (k < n_hkl) hkl_prep_diag_item(k, hkl, ops, n_ops, c2f, rec32, hd, ht, hlen);
 }
 
-__global__ void k_tab_build(const TabRec* __restrict__ tabs, int n_wfn, const WfnRec* __restrict__ wfns,
+__global__ __launch_bounds__(64) void k_tab_build(const TabRec* __restrict__ tabs, int n_wfn,
+                                                  const WfnRec* __restrict__ wfns,
                             const TermRec* __restrict__ core, const TermRec* __restrict__ val, int n_nodes,
                             double* __restrict__ tab, float* __restrict__ tab32)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n_nodes) tab_build_item(j, tabs, n_wfn, wfns, core, val, tab, tab32);
+    tab_build_node(blockIdx.x, threadIdx.x, tabs, n_wfn, wfns, core, val, tab, tab32);
 }
 
 __global__ void k_radial_prep(const double* __restrict__ hlen, const double* __restrict__ tab,
@@ -520,31 +554,6 @@ __device__ __forceinline__ void lds_wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <int CTRL>
-__device__ __forceinline__ float dpp_add(float v)
-{
-    return v + __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF,
-                                                                      false));
-}
-
-template <int CTRL>
-__device__ __forceinline__ double dpp_add(double v)
-{
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
-    const unsigned hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
-    return v + __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
-
-// sum over the eight lanes 8j .. 8j+7 (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror)
-template <typename T>
-__device__ __forceinline__ T octet_sum(T v)
-{
-    v = dpp_add<0xB1>(v);
-    v = dpp_add<0x4E>(v);
-    return dpp_add<0x141>(v);
 }
 
 __device__ __forceinline__ int stage_slot(int kk, int lane, int grp)
@@ -1688,7 +1697,7 @@ void Engine::run_prep(Precision prec, bool mp)
     }
     HIP_CHECK(hipGetLastError());
     if (model_.n_types > 0) {
-        hipLaunchKernelGGL(k_tab_build, dim3((n_tab_nodes_ + T - 1) / T), dim3(T), 0, stream_,
+        hipLaunchKernelGGL(k_tab_build, dim3(n_tab_nodes_), dim3(64), 0, stream_,
                            d_tabrec_.as<TabRec>(), model_.n_wfn, wfns, d_core_.as<TermRec>(), d_val_.as<TermRec>(),
                            n_tab_nodes_, d_tab_.as<double>(), d_tab32_.as<float>());
         HIP_CHECK(hipGetLastError());
@@ -2012,13 +2021,12 @@ __global__ void k_batch_hkl_prep(const BatchItem* __restrict__ it, const int* __
     else hkl_prep_item(k, q.hkl, q.ops, q.n_ops, q.c2f, q.rec64, q.rec32, q.phase, q.hlen);
 }
 
-__global__ void k_batch_tab_build(const BatchItem* __restrict__ it, const int* __restrict__ off, int nb, int total)
+__global__ __launch_bounds__(64) void k_batch_tab_build(const BatchItem* __restrict__ it, const int* __restrict__ off,
+                                                        int nb)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const int b = find_item(off, nb, i);
+    const int b = find_item(off, nb, blockIdx.x);
     const BatchItem& q = it[b];
-    tab_build_item(i - off[b], q.tabs, q.n_wfn, q.wfns, q.core, q.val, q.tab, q.tab32);
+    tab_build_node(blockIdx.x - off[b], threadIdx.x, q.tabs, q.n_wfn, q.wfns, q.core, q.val, q.tab, q.tab32);
 }
 
 
@@ -2242,8 +2250,7 @@ void BatchEngine::launch(Precision prec, bool derivs)
         hipLaunchKernelGGL(k_batch_hkl_prep, blocks(total_[kSpHkl], T), dim3(T), 0, stream_, it, o(kSpHkl), nb,
                            total_[kSpHkl]);
     if (total_[kSpTab] > 0)
-        hipLaunchKernelGGL(k_batch_tab_build, blocks(total_[kSpTab], T), dim3(T), 0, stream_, it, o(kSpTab), nb,
-                           total_[kSpTab]);
+        hipLaunchKernelGGL(k_batch_tab_build, dim3(total_[kSpTab]), dim3(64), 0, stream_, it, o(kSpTab), nb);
     if (total_[kSpSlot] > 0)
         hipLaunchKernelGGL(k_batch_atom_prep, dim3(total_[kSpSlot]), dim3(64), 0, stream_, it, o(kSpSlot), nb);
     HIP_CHECK(hipGetLastError());

@@ -75,7 +75,7 @@ def time_mode(calc, mode, steps, warmup, tune=True):
     return (t1 - t0) / steps, float(np.mean(main_ms)), info, f, d
 
 
-def cpu_baseline(w, budget_refl=66592):
+def cpu_baseline(w, budget_refl=66592, label="the same 5000-atom P212121 structure"):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import hc_oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -87,8 +87,8 @@ def cpu_baseline(w, budget_refl=66592):
     dt = time.perf_counter() - t0
     terms = w.crystal.atoms.n * w.n_explicit_ops * len(sel)
     return {"value": terms / dt, "unit": "terms/s", "cores": threads, "kind": "port",
-            "sample": "FP64 OpenMP oracle (direct sincos/exp per term) on the same 5000-atom P212121 "
-                      "structure, %d of its 66 592 reflections, F + xyz/U/occ derivatives, %.1f s wall" % (len(sel), dt)}
+            "sample": "FP64 OpenMP oracle (direct sincos/exp per term) on %s, %d of its %d reflections, "
+                      "F + xyz/U/occ derivatives, %.1f s wall" % (label, len(sel), len(w.hkl), dt)}
 
 
 def bench_batch(args):
@@ -202,9 +202,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0, help="fixed reflection chunk (disables the autotuner)")
     ap.add_argument("--chunk-fp64", type=int, default=0, help="fixed reflection chunk for the FP64 path only")
-    ap.add_argument("--workload", default="p212121_5000", choices=["p212121_5000", "batch256"],
-                    help="p212121_5000: one 5000-atom structure (headline); batch256: 256 uneven structures per "
-                         "batched step")
+    ap.add_argument("--workload", default="p212121_5000", choices=["p212121_5000", "p212121_20000", "batch256"],
+                    help="p212121_5000: one 5000-atom structure at 1.2 A (headline); p212121_20000: 20 000 atoms "
+                         "at 1.5 A; batch256: 256 uneven structures per batched step")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain read+write elementwise kernel over MIB MiB (PMC byte calibration)")
     args = ap.parse_args()
@@ -226,7 +226,18 @@ def main():
         return bench_batch(args)
     from discamb_amd import HansenCoppensStructureFactorCalculator, synth
 
-    w = synth.config1()
+    if args.workload == "p212121_20000":
+        w = synth.config2_large()
+        wl_text = ("P212121 20000 atoms (protein composition), 40 multipole types, cell 80x90x110 A, %d unique hkl "
+                   "to 1.5 A, 4 explicit ops" % len(w.hkl))
+        seed_text = "synthetic, seeded (numpy PCG64 seed 2000)"
+        cpu_label, cpu_refl = "the same 20000-atom P212121 structure", 16648
+    else:
+        w = synth.config1()
+        wl_text = ("P212121 5000 atoms (C1575 N425 O475 S50 H2475), 40 multipole types, cell 50x60x70 A, 66592 "
+                   "unique hkl to 1.2 A, 4 explicit ops")
+        seed_text = "synthetic, seeded (numpy PCG64 seed 1000)"
+        cpu_label, cpu_refl = "the same 5000-atom P212121 structure", 66592
     calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params, device=0)
     calc.upload(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
     terms = w.terms
@@ -267,13 +278,11 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 terms, f64 phase reduction and accumulation",
-        "data": "synthetic, seeded (numpy PCG64 seed 1000)",
-        "config": {"workload": "P212121 5000 atoms (C1575 N425 O475 S50 H2475), 40 multipole types, "
-                               "cell 50x60x70 A, 66592 unique hkl to 1.2 A, 4 explicit ops",
-                   "terms_per_step": terms, "parallelism": "single GPU"},
+        "data": seed_text,
+        "config": {"workload": wl_text, "terms_per_step": terms, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "roof": "FP32 vector ALU (the path uses VALU, not matrix cores)",
                      "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK, "traffic": measured_traffic(kname),
+                     "frac": achieved / FP32_PEAK, "traffic": measured_traffic(kname) if args.workload == "p212121_5000" else None,
                      "frac_of_unpacked_fp32": achieved / (FP32_PEAK / 2),
                      "kernel": kname, "kernel_ms": kmx,
                      "flops_per_launch": flops,
@@ -287,7 +296,7 @@ def main():
                    "autotuned": tune, "fp64_autotuned": chunk64 == 0},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(w)
+        line["cpu_baseline"] = cpu_baseline(w, cpu_refl, cpu_label)
     print(json.dumps(line), flush=True)
 
 

@@ -1451,6 +1451,8 @@ void Engine::plan_tables()
         nodes += tr[w].n;
     }
     n_tab_nodes_ = nodes;
+    tab_nodes_.resize(nw);
+    for (int w = 0; w < nw; w++) tab_nodes_[w] = tr[w].n;
     d_tab_.ensure((size_t)nodes * kTabNode * sizeof(double));
     d_tab32_.ensure((size_t)nodes * kTabNode * sizeof(float));
     d_tabrec_.ensure(tr.size() * sizeof(TabRec));
@@ -2021,13 +2023,6 @@ __global__ void k_batch_hkl_prep(const BatchItem* __restrict__ it, const int* __
     else hkl_prep_item(k, q.hkl, q.ops, q.n_ops, q.c2f, q.rec64, q.rec32, q.phase, q.hlen);
 }
 
-__global__ __launch_bounds__(64) void k_batch_tab_build(const BatchItem* __restrict__ it, const int* __restrict__ off,
-                                                        int nb)
-{
-    const int b = find_item(off, nb, blockIdx.x);
-    const BatchItem& q = it[b];
-    tab_build_node(blockIdx.x - off[b], threadIdx.x, q.tabs, q.n_wfn, q.wfns, q.core, q.val, q.tab, q.tab32);
-}
 
 
 __global__ __launch_bounds__(64) void k_batch_atom_prep(const BatchItem* __restrict__ it, const int* __restrict__ off,
@@ -2201,6 +2196,7 @@ void BatchEngine::plan(Precision prec, bool derivs, int chunk)
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
     std::vector<BatchItem> sorted(nb);
     for (int b = 0; b < nb; b++) sorted[b] = items[order[b]];
+    share_tables(prec, order, sorted);
     off_.assign((size_t)kNumSpaces * (nb + 1), 0);
     total_.assign(kNumSpaces, 0);
     for (int sp = 0; sp < kNumSpaces; sp++) {
@@ -2235,6 +2231,103 @@ void BatchEngine::plan(Precision prec, bool derivs, int chunk)
     last_units = total_[kSpUnitD] + total_[kSpUnitG];
 }
 
+// Radial tables depend only on the wavefunction data (Slater terms, deformation-valence exponent
+// and powers), not on the structure: one table per distinct wavefunction of the batch, as long as
+// the longest grid any structure needs, replaces the per-structure tables.  Each structure gets
+// its own small TabRec array mapping its wavefunction indices into the shared table.
+void BatchEngine::share_tables(Precision prec, const std::vector<int>& order, std::vector<BatchItem>& items)
+{
+    const int nb = (int)eng_.size();
+    std::map<std::vector<double>, int> index;
+    std::vector<WfnRec> wr;
+    std::vector<TermRec> core, val;
+    std::vector<int> nodes;
+    std::vector<std::vector<int>> local(nb);
+    for (int b = 0; b < nb; b++) {
+        const Engine& e = *eng_[order[b]];
+        const EngineModel& m = e.model();
+        const int nw = e.model().n_types > 0 ? m.n_wfn : 0;
+        for (int w = 0; w < nw; w++) {
+            std::vector<double> key;
+            key.push_back(m.dv_zeta[w]);
+            for (int l = 0; l < 5; l++) key.push_back(l < m.dv_nl[w] ? m.dv_n[5 * w + l] : -1);
+            for (const SlaterDensity* d : {&m.core[w], &m.valence[w]}) {
+                key.push_back((double)d->c.size());
+                for (size_t i = 0; i < d->c.size(); i++) {
+                    key.push_back(d->c[i]);
+                    key.push_back(d->e[i]);
+                    key.push_back(d->p[i]);
+                }
+            }
+            auto it = index.find(key);
+            int g;
+            if (it == index.end()) {
+                g = (int)wr.size();
+                index[key] = g;
+                WfnRec r;
+                std::memset(&r, 0, sizeof(r));
+                r.core_off = (int)core.size();
+                r.core_n = (int)m.core[w].c.size();
+                for (size_t i = 0; i < m.core[w].c.size(); i++)
+                    core.push_back({m.core[w].c[i], m.core[w].e[i], m.core[w].p[i], 0});
+                r.val_off = (int)val.size();
+                r.val_n = (int)m.valence[w].c.size();
+                for (size_t i = 0; i < m.valence[w].c.size(); i++)
+                    val.push_back({m.valence[w].c[i], m.valence[w].e[i], m.valence[w].p[i], 0});
+                r.zeta = m.dv_zeta[w];
+                for (int l = 0; l < 5; l++) {
+                    const bool has = l < m.dv_nl[w];
+                    r.n[l] = has ? m.dv_n[5 * w + l] : -1;
+                    r.gscale[l] = has ? def_val_scale(m.dv_zeta[w], m.dv_n[5 * w + l], l) : 0.0;
+                }
+                wr.push_back(r);
+                nodes.push_back(0);
+            } else {
+                g = it->second;
+            }
+            nodes[g] = std::max(nodes[g], e.tab_nodes(w));
+            local[b].push_back(g);
+        }
+    }
+    std::vector<TabRec> gtab(wr.size());
+    int total = 0;
+    for (size_t g = 0; g < wr.size(); g++) {
+        gtab[g].off = total;
+        gtab[g].n = nodes[g];
+        total += nodes[g];
+    }
+    n_shared_nodes_ = total;
+    n_shared_wfn_ = (int)wr.size();
+    std::vector<TabRec> per;      // every structure's map, concatenated
+    std::vector<size_t> start(nb);
+    for (int b = 0; b < nb; b++) {
+        start[b] = per.size();
+        for (int g : local[b]) per.push_back(gtab[g]);
+        if (local[b].empty()) per.push_back(TabRec{0, 2});
+    }
+    d_gwfn_.ensure(std::max<size_t>(wr.size(), 1) * sizeof(WfnRec));
+    d_gcore_.ensure(std::max<size_t>(core.size(), 1) * sizeof(TermRec));
+    d_gval_.ensure(std::max<size_t>(val.size(), 1) * sizeof(TermRec));
+    d_gtabs_.ensure(std::max<size_t>(gtab.size(), 1) * sizeof(TabRec));
+    d_ltabs_.ensure(std::max<size_t>(per.size(), 1) * sizeof(TabRec));
+    d_gtab_.ensure((size_t)std::max(total, 1) * kTabNode * sizeof(double));
+    d_gtab32_.ensure((size_t)std::max(total, 1) * kTabNode * sizeof(float));
+    auto up = [&](DeviceBuffer& d, const void* src, size_t bytes) {
+        if (bytes) HIP_CHECK(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, stream_));
+    };
+    up(d_gwfn_, wr.data(), wr.size() * sizeof(WfnRec));
+    up(d_gcore_, core.data(), core.size() * sizeof(TermRec));
+    up(d_gval_, val.data(), val.size() * sizeof(TermRec));
+    up(d_gtabs_, gtab.data(), gtab.size() * sizeof(TabRec));
+    up(d_ltabs_, per.data(), per.size() * sizeof(TabRec));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    for (int b = 0; b < nb; b++) {
+        items[b].main.tabs = d_ltabs_.as<TabRec>() + start[b];
+        items[b].main.tab = prec == kMixed ? d_gtab32_.p : d_gtab_.p;
+        items[b].n_tab = 0;        // no per-structure table build
+    }
+}
+
 void BatchEngine::launch(Precision prec, bool derivs)
 {
     const bool mixed = prec == kMixed;
@@ -2249,8 +2342,10 @@ void BatchEngine::launch(Precision prec, bool derivs)
     if (total_[kSpHkl] > 0)
         hipLaunchKernelGGL(k_batch_hkl_prep, blocks(total_[kSpHkl], T), dim3(T), 0, stream_, it, o(kSpHkl), nb,
                            total_[kSpHkl]);
-    if (total_[kSpTab] > 0)
-        hipLaunchKernelGGL(k_batch_tab_build, dim3(total_[kSpTab]), dim3(64), 0, stream_, it, o(kSpTab), nb);
+    if (n_shared_nodes_ > 0)
+        hipLaunchKernelGGL(k_tab_build, dim3(n_shared_nodes_), dim3(64), 0, stream_, d_gtabs_.as<TabRec>(),
+                           n_shared_wfn_, d_gwfn_.as<WfnRec>(), d_gcore_.as<TermRec>(), d_gval_.as<TermRec>(),
+                           n_shared_nodes_, d_gtab_.as<double>(), d_gtab32_.as<float>());
     if (total_[kSpSlot] > 0)
         hipLaunchKernelGGL(k_batch_atom_prep, dim3(total_[kSpSlot]), dim3(64), 0, stream_, it, o(kSpSlot), nb);
     HIP_CHECK(hipGetLastError());

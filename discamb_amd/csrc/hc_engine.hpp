@@ -89,6 +89,7 @@ public:
     // part of the step; after the batched launch, mark the results as a completed run.
     void batch_item(Precision prec, bool derivs, int chunk, BatchItem* out);
     void mark_batched_run(bool derivs);
+    int tab_nodes(int wfn) const { return wfn < (int)tab_nodes_.size() ? tab_nodes_[wfn] : 3; }
 
 private:
     void ensure_prep_buffers(Precision prec, bool mp);
@@ -132,6 +133,7 @@ private:
     bool ran_ = false;
     std::vector<int> order_;        // sorted slot -> caller atom index (-1 padding)
     std::vector<int> hkl_perm_;     // device reflection position -> caller reflection index
+    std::vector<int> tab_nodes_;    // radial-table node count per wavefunction
     // model tables
     DeviceBuffer d_core_, d_val_, d_type_, d_ops_;
     // per call
@@ -163,6 +165,7 @@ public:
 
 private:
     void plan(Precision prec, bool derivs, int chunk);
+    void share_tables(Precision prec, const std::vector<int>& order, std::vector<BatchItem>& items);
     void launch(Precision prec, bool derivs);
     int chunk_for(Precision prec, bool derivs);
     void drop_graph();
@@ -171,6 +174,9 @@ private:
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     DeviceBuffer d_items_, d_off_, d_counter_;
+    // radial tables shared by the structures of the batch (one per distinct wavefunction)
+    DeviceBuffer d_gwfn_, d_gcore_, d_gval_, d_gtabs_, d_ltabs_, d_gtab_, d_gtab32_;
+    int n_shared_nodes_ = 0, n_shared_wfn_ = 0;
     std::vector<unsigned char> items_;     // host image of the BatchItem array
     std::vector<int> off_;                 // prefix offsets of every index space
     std::vector<int> total_;

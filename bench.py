@@ -194,6 +194,94 @@ def bench_batch(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_per_hkl(args):
+    """Per-reflection derivative rows (the per-hkl path): F and dF/d(xyz, ADP, occ) of every atom
+    for every reflection, FP64, written by k_jac into device memory (dh_hc_jacobian_rows_dev).  Two
+    structures: the 60-atom P2_1/c small molecule (all its reflections in one launch) and the
+    5000-atom P2_1 2_1 2_1 structure (a block of 2400 reflections, 1.9 GB of rows, per launch)."""
+    import ctypes
+
+    from discamb_amd import HansenCoppensStructureFactorCalculator, _lib, synth
+
+    hip = ctypes.CDLL("libamdhip64.so.7")      # the runtime libdiscamb_amd.so itself uses
+    lib = _lib.load()
+    out = {}
+    for key, w, m in (("p21c_60", synth.config0(), None), ("p212121_5000", synth.config1(), 2400)):
+        n = w.crystal.atoms.n
+        m = len(w.hkl) if m is None else m
+        c = HansenCoppensStructureFactorCalculator(w.crystal, w.params, device=0)
+        c.jacobianUpload(w.crystal.atoms, w.lcs, w.hkl)
+        fb, rb = 16 * m, 16 * m * 10 * n
+        pf, pr = ctypes.c_void_p(), ctypes.c_void_p()
+        if hip.hipMalloc(ctypes.byref(pf), ctypes.c_size_t(fb)) or hip.hipMalloc(ctypes.byref(pr), ctypes.c_size_t(rb)):
+            raise RuntimeError("hipMalloc failed")
+        for _ in range(args.warmup):
+            _lib.check(lib.dh_hc_jacobian_rows_dev(c._h, 0, m, pf, pr, None))
+        hip.hipDeviceSynchronize()
+        c.setTiming(True)
+        kms = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _lib.check(lib.dh_hc_jacobian_rows_dev(c._h, 0, m, pf, pr, None))
+            kms.append(c.lastTiming()["main_ms"])
+        hip.hipDeviceSynchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        c.setTiming(False)
+        info = c.lastTiming()
+        hip.hipFree(pf)
+        hip.hipFree(pr)
+        # host delivery: the same rows copied block by block into pageable host memory
+        th = time.perf_counter()
+        c.jacobianRows(0, m)
+        host = time.perf_counter() - th
+        kms = float(np.mean(kms))
+        out[key] = {"atoms": n, "reflections": m, "explicit_ops": w.n_explicit_ops, "kernel_ms": kms,
+                    "wall_ms": wall * 1e3, "bytes_written": rb + fb, "pairs": n * m,
+                    "hbm_GBps": (rb + fb) / (kms * 1e-3) / 1e9, "host_rows_s": host,
+                    "host_GBps": (rb + fb) / host / 1e9, "chunk": info["chunk"], "grid": info["grid"][0]}
+        print("per-hkl %s: %d atoms x %d reflections, k_jac %.3f ms (%.0f GB/s of rows), wall %.3f ms, host "
+              "copy-out %.1f ms" % (key, n, m, kms, out[key]["hbm_GBps"], wall * 1e3, host * 1e3),
+              file=sys.stderr, flush=True)
+    big, small = out["p212121_5000"], out["p21c_60"]
+    line = {
+        "metric": "per-reflection derivative rows: (atom, reflection) pairs/s, F + dF/d(xyz, U, occ)",
+        "value": big["pairs"] / (big["kernel_ms"] * 1e-3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": big["kernel_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic, seeded (numpy PCG64 seeds 1000 and 0)",
+        "config": {"workload": "P212121 5000 atoms (the headline structure), rows of 2400 of its reflections per "
+                               "launch (10 complex FP64 derivatives per atom and reflection, 1.92 GB)",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "roof": "HBM writes of the rows (algorithmic bytes = 160 B per (atom, "
+                                             "reflection) + 16 B of F per reflection)",
+                     "achieved": big["hbm_GBps"], "peak": 8000.0, "unit": "GB/s",
+                     "frac": big["hbm_GBps"] / 8000.0, "traffic": None, "kernel": "k_jac<acentric>",
+                     "kernel_ms": big["kernel_ms"], "bytes_per_launch": big["bytes_written"]},
+        "small_molecule": small,
+        "large_block": big,
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import hc_oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        w = synth.config0()
+        t0 = time.perf_counter()
+        hc_oracle.per_hkl(w.crystal, w.params, w.crystal.atoms, w.lcs, w.hkl, n_threads=threads)
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": w.crystal.atoms.n * len(w.hkl) / dt, "unit": "pairs/s", "cores": threads,
+                                "kind": "port",
+                                "sample": "FP64 OpenMP per-hkl oracle on the 60-atom P21/c structure, all %d "
+                                          "reflections, %.2f s wall" % (len(w.hkl), dt)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,9 +290,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0, help="fixed reflection chunk (disables the autotuner)")
     ap.add_argument("--chunk-fp64", type=int, default=0, help="fixed reflection chunk for the FP64 path only")
-    ap.add_argument("--workload", default="p212121_5000", choices=["p212121_5000", "p212121_20000", "batch256"],
+    ap.add_argument("--workload", default="p212121_5000",
+                    choices=["p212121_5000", "p212121_20000", "batch256", "per_hkl"],
                     help="p212121_5000: one 5000-atom structure at 1.2 A (headline); p212121_20000: 20 000 atoms "
-                         "at 1.5 A; batch256: 256 uneven structures per batched step")
+                         "at 1.5 A; batch256: 256 uneven structures per batched step; per_hkl: per-reflection "
+                         "derivative rows")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain read+write elementwise kernel over MIB MiB (PMC byte calibration)")
     args = ap.parse_args()
@@ -224,6 +314,8 @@ def main():
         sys.exit(2)
     if args.workload == "batch256":
         return bench_batch(args)
+    if args.workload == "per_hkl":
+        return bench_per_hkl(args)
     from discamb_amd import HansenCoppensStructureFactorCalculator, synth
 
     if args.workload == "p212121_20000":

@@ -5,11 +5,11 @@ A from-scratch HIP implementation of DiSCaMB's
 same calculator interface; see include/discamb_amd.h for the C ABI.
 """
 from .model import (AtomSet, Crystal, DerivativesSelector, HC_AtomTypeParameters, HC_ModelParameters,
-                    HC_WfnType, SlaterOrbital, TargetFunctionAtomicParamDerivatives, TypeParameterDerivatives,
-                    parse_symop, wfn_type_from_bank)
+                    HC_WfnType, SfDerivativesAtHkl, SlaterOrbital, TargetFunctionAtomicParamDerivatives,
+                    TypeParameterDerivatives, parse_symop, wfn_type_from_bank)
 from .calculator import BatchCalculator, HansenCoppensStructureFactorCalculator
 
 __all__ = ["AtomSet", "Crystal", "DerivativesSelector", "HC_AtomTypeParameters", "HC_ModelParameters",
-           "HC_WfnType", "SlaterOrbital", "TargetFunctionAtomicParamDerivatives", "TypeParameterDerivatives",
+           "HC_WfnType", "SfDerivativesAtHkl", "SlaterOrbital", "TargetFunctionAtomicParamDerivatives", "TypeParameterDerivatives",
            "parse_symop",
            "wfn_type_from_bank", "HansenCoppensStructureFactorCalculator", "BatchCalculator"]

@@ -59,6 +59,10 @@ class DhAtoms(ctypes.Structure):
     ]
 
 
+# int (*)(void* user, int hkl_index, const double* f, const double* row)
+PER_HKL_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, c_dbl_p, c_dbl_p)
+JAC_PLANES = 10
+
 # every symbol include/discamb_amd.h declares, with (restype, argtypes)
 SIGNATURES = {
     "dh_last_error": (ctypes.c_char_p, []),
@@ -80,6 +84,12 @@ SIGNATURES = {
     "dh_hc_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "dh_hc_last_timing": (ctypes.c_int, [ctypes.c_void_p, c_dbl_p, c_int_p, c_int_p, c_int_p]),
     "dh_hc_engine_info": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p, c_int_p, ctypes.c_char_p]),
+    "dh_hc_per_hkl": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DhAtoms), ctypes.c_int, c_int_p, PER_HKL_FN,
+                                     ctypes.c_void_p]),
+    "dh_hc_jacobian_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DhAtoms), ctypes.c_int, c_int_p]),
+    "dh_hc_jacobian_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_dbl_p, c_dbl_p]),
+    "dh_hc_jacobian_rows_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p]),
     "dh_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     "dh_batch_destroy": (None, [ctypes.c_void_p]),
     "dh_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_int]),

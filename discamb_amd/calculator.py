@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .model import (AtomSet, Crystal, DerivativesSelector, HC_ModelParameters,
+from .model import (AtomSet, Crystal, DerivativesSelector, HC_ModelParameters, SfDerivativesAtHkl,
                     TargetFunctionAtomicParamDerivatives, TypeParameterDerivatives, parse_symop)
 
 XYZ = {"fractional": 0, "cartesian": 1}
@@ -197,7 +197,14 @@ class HansenCoppensStructureFactorCalculator:
                                                 dTarget_dF, countAtomContribution=None,
                                                 derivativesSelector: DerivativesSelector = None,
                                                 anomalous=None):
-        """Returns (f (n_hkl,) complex, TargetFunctionAtomicParamDerivatives)."""
+        """Returns (f (n_hkl,) complex, TargetFunctionAtomicParamDerivatives).
+
+        With an object that has ``onPerHklCalculation(hkl_idx, f, SfDerivativesAtHkl)`` in place of
+        dTarget_dF this is the reference's per-hkl overload (HansenCoppensStructureFactorCalculator
+        .h:206-242): the processor is called once per reflection, in order, and nothing is
+        returned."""
+        if hasattr(dTarget_dF, "onPerHklCalculation"):
+            return self._per_hkl(atoms, localCoordinateSystems, hkl, dTarget_dF, countAtomContribution, anomalous)
         sel = (derivativesSelector or DerivativesSelector()).bits()
         a, keep = self._atoms(atoms, localCoordinateSystems, countAtomContribution, anomalous)
         hkl_i, dd = self._hkl(hkl, dTarget_dF)
@@ -212,6 +219,57 @@ class HansenCoppensStructureFactorCalculator:
             _lib.ptr(do, ctypes.c_double)))
         fc = f[:nh, 0] + 1j * f[:nh, 1]
         return fc, TargetFunctionAtomicParamDerivatives(dx[:n], du[:n], do[:n])
+
+    # -- per-reflection derivatives (the per-hkl path) -------------------------------------------
+    def _per_hkl(self, atoms, lcs, hkl, processor, include, anomalous):
+        a, keep = self._atoms(atoms, lcs, include, anomalous)
+        hkl_i = np.ascontiguousarray(np.asarray(hkl).reshape(-1, 3), dtype=np.int32)
+        n = atoms.n
+        failure = []
+
+        def cb(user, k, f, row):
+            try:
+                fv = complex(f[0], f[1])
+                r = np.ctypeslib.as_array(row, shape=(_lib.JAC_PLANES * n * 2,)).view(np.complex128)
+                processor.onPerHklCalculation(int(k), fv, SfDerivativesAtHkl(r.reshape(_lib.JAC_PLANES, n)))
+                return 0
+            except BaseException as e:      # re-raised after the library has stopped
+                failure.append(e)
+                return 1
+
+        fn = _lib.PER_HKL_FN(cb)
+        rc = self._lib.dh_hc_per_hkl(self._h, ctypes.byref(a), hkl_i.shape[0], _lib.ptr(hkl_i, ctypes.c_int), fn,
+                                     None)
+        if failure:
+            raise failure[0]
+        _lib.check(rc)
+
+    def jacobianUpload(self, atoms: AtomSet, localCoordinateSystems, hkl, countAtomContribution=None,
+                       anomalous=None):
+        """Inputs of the per-reflection rows (no dT/dF); reflections stay in the given order."""
+        a, keep = self._atoms(atoms, localCoordinateSystems, countAtomContribution, anomalous)
+        hkl_i = np.ascontiguousarray(np.asarray(hkl).reshape(-1, 3), dtype=np.int32)
+        _lib.check(self._lib.dh_hc_jacobian_upload(self._h, ctypes.byref(a), hkl_i.shape[0],
+                                                   _lib.ptr(hkl_i, ctypes.c_int)))
+        self._jac_shape = (hkl_i.shape[0], atoms.n)
+
+    def jacobianRows(self, k0: int = 0, k1: int = None):
+        """(F (k1-k0,) complex, rows (k1-k0, 10, n) complex) of the uploaded reflections [k0, k1)."""
+        nh, n = self._jac_shape
+        k1 = nh if k1 is None else k1
+        m = max(k1 - k0, 0)
+        f = np.zeros(max(m, 1), dtype=np.complex128)
+        rows = np.zeros((max(m, 1), _lib.JAC_PLANES, max(n, 1)), dtype=np.complex128)
+        _lib.check(self._lib.dh_hc_jacobian_rows(self._h, int(k0), int(k1),
+                                                 f.ctypes.data_as(_lib.c_dbl_p), rows.ctypes.data_as(_lib.c_dbl_p)))
+        return f[:m], rows[:m, :, :n]
+
+    def calculateJacobian(self, atoms: AtomSet, localCoordinateSystems, hkl, countAtomContribution=None,
+                          anomalous=None):
+        """All per-reflection rows at once: (F (n_hkl,), rows (n_hkl, 10, n)) complex, planes
+        dF/dxyz (3), dF/dADP (6; U_iso in plane 3), dF/docc, in the calculator's conventions."""
+        self.jacobianUpload(atoms, localCoordinateSystems, hkl, countAtomContribution, anomalous)
+        return self.jacobianRows()
 
     def calculateStructureFactors(self, atoms: AtomSet, localCoordinateSystems, hkl, countAtomContribution=None,
                                   anomalous=None):

@@ -16,6 +16,30 @@ namespace discamb_amd {
 
 static thread_local std::string g_last_error;
 
+// Page-locked host memory (hipHostMalloc), grown on demand.
+struct PinnedBuffer {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t n)
+    {
+        if (n <= bytes && p) return;
+        release();
+        if (hipHostMalloc(&p, std::max<size_t>(n, 256), hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            throw Error(DH_E_MEMORY, "pinned host allocation failed");
+        }
+        bytes = std::max<size_t>(n, 256);
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    ~PinnedBuffer() { release(); }
+    double* d() const { return static_cast<double*>(p); }
+};
+
 struct Calculator {
     Cell cell;
     SpaceGroupInfo sg;
@@ -37,6 +61,16 @@ struct Calculator {
     // per-type multipole derivatives of the last run with DH_D_MULTIPOLE
     std::vector<double> d_plm, d_pval, d_kappa, d_kappa_prime;
     bool have_mp = false;
+    // per-reflection rows: device blocks and pinned host staging of dh_hc_per_hkl / _rows
+    bool rows_uploaded = false;
+    DeviceBuffer rows_dev[2], f_dev[2];
+    PinnedBuffer rows_host[2], f_host[2];
+    hipEvent_t rows_ev[2] = {nullptr, nullptr}, ev_in = nullptr, ev_out = nullptr;
+    ~Calculator()
+    {
+        for (hipEvent_t e : {rows_ev[0], rows_ev[1], ev_in, ev_out})
+            if (e) (void)hipEventDestroy(e);
+    }
 };
 
 static std::vector<EngineOp> make_ops(const Cell& c, const SpaceGroupInfo& g, bool centric_shift)
@@ -147,17 +181,21 @@ static Calculator* create(const dh_model_desc* d, int device)
     return c.release();
 }
 
-static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl, const double* dtdf)
+// rows: inputs of the per-reflection derivative rows (no dT/dF; reflections kept in caller order)
+static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl, const double* dtdf,
+                   bool rows = false)
 {
     if (!at) throw Error(DH_E_ARGUMENT, "null atoms");
     const int n = at->n_atoms;
     if (n != (int)c->atom_type.size())
         throw Error(DH_E_ARGUMENT, "number of atoms differs from the model's atom-to-type map");
     if (n_hkl < 0) throw Error(DH_E_ARGUMENT, "negative number of reflections");
-    if (n_hkl > 0 && (!hkl || !dtdf))
+    if (n_hkl > 0 && !hkl) throw Error(DH_E_ARGUMENT, "null Miller indices");
+    if (n_hkl > 0 && !dtdf && !rows)
         throw Error(DH_E_ARGUMENT,
                     "Inconsistent size of hkl vectors set and (d target/ d F) table when calculating multipolar "
                     "structure factors.");
+    c->rows_uploaded = false;
     const double two_pi2 = 2.0 * M_PI * M_PI;
     EngineAtoms ea;
     ea.n = n;
@@ -209,7 +247,7 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
     std::vector<double> dd(3 * (size_t)n_hkl);
     for (int k = 0; k < n_hkl; k++) {
         double m = centering_multiplier(c->sg.centering, c->sg.obverse, hkl + 3 * k);
-        double re = dtdf[2 * k] * m, im = -dtdf[2 * k + 1] * m;
+        double re = rows ? 0.0 : dtdf[2 * k] * m, im = rows ? 0.0 : -dtdf[2 * k + 1] * m;
         if (c->centric_origin) {
             // F = 2 Re(sum) and derivatives 2 Re(c) Re(.)  (Engine3.cpp:1924-1958)
             re *= 2.0;
@@ -219,9 +257,106 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
         dd[2 * k + 1] = im;
         dd[2 * (size_t)n_hkl + k] = m;
     }
-    c->engine->upload(ea, n_hkl, hkl, dd.data());
+    c->engine->upload(ea, n_hkl, hkl, dd.data(), rows);
     c->n_atoms = n;
     c->n_hkl = n_hkl;
+    if (!rows) return;
+    // derivative conventions of the rows (WrapperForSingleHklResultProcessor::convert,
+    // HansenCoppensStructureFactorCalculator.h:512-531): d/dx_cart = c2f^T d/dx_frac and the
+    // U_cart -> U_cif chain rule, both linear, applied inside the kernel through its records
+    JacConv jc;
+    for (int i = 0; i < 3; i++)
+        for (int m = 0; m < 3; m++)
+            jc.xyz_m[3 * i + m] = c->d_xyz_conv == DH_XYZ_CARTESIAN ? c->cell.c2f[3 * m + i] : (i == m ? 1.0 : 0.0);
+    for (int j = 0; j < 6; j++) {
+        double e[6] = {0, 0, 0, 0, 0, 0}, o[6];
+        e[j] = 1.0;
+        if (c->d_adp_conv == DH_ADP_U_CART) std::memcpy(o, e, sizeof(o));
+        else c->cell.d_ucart_to_d_ucif(e, o);
+        for (int i = 0; i < 6; i++) jc.adp_m[6 * i + j] = o[i];
+    }
+    jc.mult_w = c->mult_w;
+    jc.adp_kind = c->adp_size;
+    c->engine->set_jacobian_conventions(jc);
+    c->rows_uploaded = true;
+}
+
+static constexpr size_t kRowBlockBytes = 64u << 20;   // device / pinned staging per block
+
+static int rows_per_block(const Calculator* c)
+{
+    const size_t row = ((size_t)DH_JAC_PLANES * c->n_atoms * 2 + 2) * sizeof(double);
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(c->n_hkl, 1), kRowBlockBytes / row));
+}
+
+static void check_rows(Calculator* c, int k0, int k1)
+{
+    if (!c->rows_uploaded) throw Error(DH_E_ARGUMENT, "per-reflection rows: call dh_hc_jacobian_upload first");
+    if (k0 < 0 || k1 < k0 || k1 > c->n_hkl) throw Error(DH_E_ARGUMENT, "reflection range outside the upload");
+}
+
+// Rows [k0, k1) into host memory, through the calculator's device blocks.
+static void rows_to_host(Calculator* c, int k0, int k1, double* f, double* rows)
+{
+    check_rows(c, k0, k1);
+    const int B = rows_per_block(c);
+    const size_t row = (size_t)DH_JAC_PLANES * c->n_atoms * 2;
+    hipStream_t st = c->engine->stream();
+    for (int b0 = k0; b0 < k1; b0 += B) {
+        const int b1 = std::min(k1, b0 + B), n = b1 - b0;
+        c->rows_dev[0].ensure((size_t)n * row * sizeof(double));
+        c->f_dev[0].ensure((size_t)n * 2 * sizeof(double));
+        c->engine->jacobian(b0, b1, c->f_dev[0].as<double>(), c->rows_dev[0].as<double>());
+        if (f)
+            HIP_CHECK(hipMemcpyAsync(f + 2 * (size_t)(b0 - k0), c->f_dev[0].p, (size_t)n * 2 * sizeof(double),
+                                     hipMemcpyDeviceToHost, st));
+        if (rows)
+            HIP_CHECK(hipMemcpyAsync(rows + row * (b0 - k0), c->rows_dev[0].p, (size_t)n * row * sizeof(double),
+                                     hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+    }
+}
+
+// The per-hkl driver: blocks of rows computed on the GPU while the host hands the previous block
+// to the callback, reflection by reflection in caller order.
+static void per_hkl(Calculator* c, dh_per_hkl_fn fn, void* user)
+{
+    if (!fn) throw Error(DH_E_ARGUMENT, "null per-reflection callback");
+    const int n_hkl = c->n_hkl;
+    if (n_hkl == 0) return;
+    const int B = rows_per_block(c);
+    const size_t row = (size_t)DH_JAC_PLANES * c->n_atoms * 2;
+    hipStream_t st = c->engine->stream();
+    for (int i = 0; i < 2; i++) {
+        if (!c->rows_ev[i]) HIP_CHECK(hipEventCreateWithFlags(&c->rows_ev[i], hipEventDisableTiming));
+        c->rows_dev[i].ensure((size_t)B * row * sizeof(double));
+        c->f_dev[i].ensure((size_t)B * 2 * sizeof(double));
+        c->rows_host[i].ensure((size_t)B * row * sizeof(double));
+        c->f_host[i].ensure((size_t)B * 2 * sizeof(double));
+    }
+    const int nb = (n_hkl + B - 1) / B;
+    for (int b = 0; b <= nb; b++) {
+        if (b < nb) {
+            const int q = b & 1, b0 = b * B, b1 = std::min(n_hkl, b0 + B), n = b1 - b0;
+            c->engine->jacobian(b0, b1, c->f_dev[q].as<double>(), c->rows_dev[q].as<double>());
+            HIP_CHECK(hipMemcpyAsync(c->f_host[q].p, c->f_dev[q].p, (size_t)n * 2 * sizeof(double),
+                                     hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipMemcpyAsync(c->rows_host[q].p, c->rows_dev[q].p, (size_t)n * row * sizeof(double),
+                                     hipMemcpyDeviceToHost, st));
+            HIP_CHECK(hipEventRecord(c->rows_ev[q], st));
+        }
+        if (b > 0) {
+            const int q = (b - 1) & 1, b0 = (b - 1) * B, b1 = std::min(n_hkl, b0 + B);
+            HIP_CHECK(hipEventSynchronize(c->rows_ev[q]));
+            for (int k = b0; k < b1; k++) {
+                const int rc = fn(user, k, c->f_host[q].d() + 2 * (size_t)(k - b0), c->rows_host[q].d() + row * (k - b0));
+                if (rc != 0) {
+                    HIP_CHECK(hipStreamSynchronize(st));
+                    throw Error(DH_E_CALLBACK, "per-reflection callback returned " + std::to_string(rc));
+                }
+            }
+        }
+    }
 }
 
 static void fetch(Calculator* c, double* f, double* d_xyz, double* d_adp, double* d_occ)
@@ -493,6 +628,54 @@ int dh_hc_engine_info(dh_calculator* calc, int* n_ops, int* order, int* centro, 
         if (order) *order = calc->c->sg.order;
         if (centro) *centro = calc->c->sg.centro ? 1 : 0;
         if (centering) *centering = calc->c->sg.centering;
+    });
+}
+
+int dh_hc_per_hkl(dh_calculator* calc, const dh_atoms* atoms, int n_hkl, const int* hkl, dh_per_hkl_fn fn,
+                  void* user)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        if (!fn) throw Error(DH_E_ARGUMENT, "null per-reflection callback");
+        upload(calc->c, atoms, n_hkl, hkl, nullptr, true);
+        per_hkl(calc->c, fn, user);
+    });
+}
+
+int dh_hc_jacobian_upload(dh_calculator* calc, const dh_atoms* atoms, int n_hkl, const int* hkl)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        upload(calc->c, atoms, n_hkl, hkl, nullptr, true);
+    });
+}
+
+int dh_hc_jacobian_rows(dh_calculator* calc, int k0, int k1, double* f, double* rows)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        rows_to_host(calc->c, k0, k1, f, rows);
+    });
+}
+
+int dh_hc_jacobian_rows_dev(dh_calculator* calc, int k0, int k1, void* f_dev, void* rows_dev, void* stream)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        Calculator* c = calc->c;
+        check_rows(c, k0, k1);
+        if (k1 == k0) return;
+        if (!f_dev || !rows_dev) throw Error(DH_E_ARGUMENT, "null output pointer");
+        hipStream_t caller = static_cast<hipStream_t>(stream), st = c->engine->stream();
+        if (!c->ev_in) HIP_CHECK(hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
+        if (!c->ev_out) HIP_CHECK(hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming));
+        // the engine stream starts after the caller's earlier work and the caller's stream resumes
+        // after the rows are written
+        HIP_CHECK(hipEventRecord(c->ev_in, caller));
+        HIP_CHECK(hipStreamWaitEvent(st, c->ev_in, 0));
+        c->engine->jacobian(k0, k1, static_cast<double*>(f_dev), static_cast<double*>(rows_dev));
+        HIP_CHECK(hipEventRecord(c->ev_out, st));
+        HIP_CHECK(hipStreamWaitEvent(caller, c->ev_out, 0));
     });
 }
 

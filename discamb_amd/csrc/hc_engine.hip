@@ -24,12 +24,6 @@
 
 namespace discamb_amd {
 
-#define HIP_CHECK(x)                                                                              \
-    do {                                                                                          \
-        hipError_t e_ = (x);                                                                      \
-        if (e_ != hipSuccess)                                                                     \
-            throw Error(3, std::string("HIP error ") + hipGetErrorString(e_) + " at " #x);       \
-    } while (0)
 
 static constexpr int kHklopRec = 36;    // doubles / floats per (reflection, op) record
 static constexpr int kRadRec = 8;       // per (reflection, type) radial record
@@ -1188,6 +1182,247 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && !M
                                            stage_lds + wib * kStageBytes);
 }
 
+// ------------------------------------------------------------------ per-reflection derivatives
+//
+// The per-hkl path (HansenCoppensStructureFactorCalculator.h:384-545 over
+// HansenCoppens_SF_Engine.h:512-780): for every reflection k of a block and every atom a the
+// kernel writes dF_k/dp_a for the atom's 10 parameters (xyz, 6 ADP, occupancy) as complex FP64
+// values, already scaled and converted to the caller's conventions, and F_k through the same
+// per-wave staging and stage-2 reduction as the summed path.  The output is ~160 bytes per
+// (atom, reflection) against ~130 FP64 flops per (atom, operation, reflection) term, so the
+// kernel is bound by its HBM writes, not by arithmetic; it computes in FP64 in every mode.
+//
+// Work unit = (atom wave, reflection chunk of the block); each of the four waves of a workgroup
+// takes its own unit, atom wave fastest, so the waves of a workgroup share reflection records
+// and write neighbouring atoms of the same rows.
+
+struct JacAtom {
+    double mult_w;     // m / |G|: occupancy-derivative weight
+    int adp_kind;      // 0 (no ADP), 1 (U_iso), 6
+    int pad;
+};
+
+struct JacArgs {
+    double* rows;          // block rows: [k - k0][plane][n_atoms] complex (re, im)
+    double* f;             // block F: [k - k0] complex
+    const JacAtom* atoms;  // caller atom order
+    const double* jrec;    // per (reflection, operation): kJacRec doubles, see k_jac_rec
+    int n_atoms, k0, k1, chunk, n_units;
+};
+
+static constexpr int kJacPlanes = 10;
+static constexpr int kJacRec = 12;
+
+// one complex entry of a per-reflection row; the rows are written once and not read back by
+// this kernel, so the stores are marked non-temporal (streamed past the caches' retention)
+__device__ __forceinline__ void put_row(double* p, double re, double im)
+{
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    const d2v v = {re, im};
+    __builtin_nontemporal_store(v, reinterpret_cast<d2v*>(p));
+}
+
+// Per (reflection, operation): the multipliers of the xyz and ADP derivatives already carried into
+// the caller's conventions, so that the main kernel needs no conversion:
+//   [0..2] xyz_m . h_s (fractional h_s; xyz_m = identity for fractional derivatives, c2f^T for
+//          Cartesian ones),  [3..8] adp_m . m(h_s) (m = Debye-Waller multipliers of U_cart;
+//          adp_m = identity for U_cart derivatives, the U_cart -> U_cif chain rule otherwise).
+__global__ void k_jac_rec(const double* __restrict__ rec64, int n, const double* __restrict__ conv,
+                          double* __restrict__ jrec)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* r = rec64 + (size_t)i * kHklopRec;
+    double* o = jrec + (size_t)i * kJacRec;
+    for (int a = 0; a < 3; a++) o[a] = conv[3 * a] * r[31] + conv[3 * a + 1] * r[32] + conv[3 * a + 2] * r[33];
+    for (int a = 0; a < 6; a++) {
+        double v = 0.0;
+        for (int b = 0; b < 6; b++) v += conv[9 + 6 * a + b] * r[25 + b];
+        o[3 + a] = v;
+    }
+    o[9] = o[10] = o[11] = 0.0;
+}
+
+// CENTRIC: centre of symmetry at the origin with only the coset representatives explicit; the
+// inversion partner is folded in per operation (f'' kept unconjugated, as the per-hkl engine does):
+//   F part   X = 2 T [(f0 + f') cos phi + Re(f_def e^{i phi})] + 2 i T f'' cos phi
+//   xyz part Y = -2 T f'' sin phi + 2 i T [(f0 + f') sin phi + Im(f_def e^{i phi})]
+// Otherwise every explicit operation (inversion partners of a shifted centre included) adds
+// X = Y = f T e^{i phi} with f = f0 + f' + f_def + i f''.
+template <int L, bool CENTRIC, bool OTF>
+__device__ __forceinline__ void body_jac(const MainArgs& a, const JacArgs& jb, int wave, int lane, int k0, int k1,
+                                         const unsigned* __restrict__ hop, const unsigned* __restrict__ jrec,
+                                         const double* __restrict__ rad, const unsigned* __restrict__ mult32,
+                                         double2* __restrict__ stage, double* __restrict__ fpart,
+                                         double* __restrict__ rows)
+{
+    constexpr int NLM = (L + 1) * (L + 1);
+    const int slot = wave * kWave + lane;
+    const AtomRec& at = cmem(a.atoms)[slot];
+    const double x0 = at.xf[0], x1 = at.xf[1], x2 = at.xf[2];
+    double u[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) u[j] = at.u[j];
+    double P[NLM];
+#pragma unroll
+    for (int j = 0; j < NLM; j++) P[j] = at.plm[j];
+    const double w = at.w, anr = at.anom[0], ani = at.anom[1];
+    const int t = at.type;
+    const int orig = at.orig;
+    double mw = 0.0;
+    int kind = 0;
+    if (orig >= 0) {
+        const JacAtom ja = gmem(jb.atoms)[orig];
+        mw = ja.mult_w;
+        kind = ja.adp_kind;
+    }
+    TabRec tr{0, 2};
+    double pv = 0, ik = 0, ikp = 0;
+    if (OTF) {
+        tr = cmem(a.tabs)[at.wfn];
+        pv = at.pval;
+        ik = at.inv_kappa;
+        ikp = at.inv_kappa_p;
+    }
+    const double* tab = cmem(static_cast<const double*>(a.tab));
+    const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
+    const size_t plane = (size_t)jb.n_atoms * 2;
+    for (int kb = k0; kb < k1; kb += kStage) {
+    const int nk = min(kStage, k1 - kb);
+    for (int kk = 0; kk < nk; ++kk) {
+        const int k = kb + kk;
+        const double hl = ld_f64(hl32, k);
+        double S, G0, G1, G2, G3, G4;
+        if (OTF) {
+            double G[5];
+            radial_otf<L, double>(tab, tr, hl, pv, ik, ikp, anr, S, G);
+            G0 = G[0]; G1 = G[1]; G2 = G[2]; G3 = G[3]; G4 = G[4];
+        } else {
+            const double* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
+            S = rr[0] + anr; G0 = rr[1]; G1 = rr[2]; G2 = rr[3]; G3 = rr[4]; G4 = rr[5];
+        }
+        double s0r = 0, s0i = 0, sxr[3] = {0, 0, 0}, sxi[3] = {0, 0, 0};
+        double sur[6] = {0, 0, 0, 0, 0, 0}, sui[6] = {0, 0, 0, 0, 0, 0};
+        for (int s = 0; s < a.n_ops; ++s) {
+            const unsigned* rw = hop + ((size_t)k * a.n_ops + s) * (2 * kHklopRec);
+            double r[kHklopRec];
+#pragma unroll
+            for (int j = 0; j < kHklopRec; j++) r[j] = ld_f64(rw, j);
+            const unsigned* qw = jrec + ((size_t)k * a.n_ops + s) * (2 * kJacRec);
+            double q[9];
+#pragma unroll
+            for (int j = 0; j < 9; j++) q[j] = ld_f64(qw, j);
+            double A[5];
+            contract<L, double>(P, r, A);
+            double fr = S + G0 * A[0] + G2 * A[2];
+            double fi = ani + G1 * A[1];
+            if (L == 4) {
+                fr += G4 * A[4];
+                fi += G3 * A[3];
+            }
+            const double ph = __builtin_amdgcn_fract(fma(r[33], x2, fma(r[32], x1, fma(r[31], x0, r[34]))));
+            double sn, cs;
+            sincospi(2.0 * ph, &sn, &cs);
+            const double e = r[25] * u[0] + r[26] * u[1] + r[27] * u[2] + r[28] * u[3] + r[29] * u[4] + r[30] * u[5];
+            const double T = exp(-e);
+            const double Er = T * cs, Ei = T * sn;
+            const double Xr = fr * Er - fi * Ei, Xi = fr * Ei + fi * Er;
+            double gr, gi, yr, yi;
+            if (CENTRIC) {
+                gr = 2.0 * (Xr + ani * Ei);
+                gi = 2.0 * ani * Er;
+                yr = -2.0 * ani * Ei;
+                yi = 2.0 * (Xi - ani * Er);
+            } else {
+                gr = Xr;
+                gi = Xi;
+                yr = Xr;
+                yi = Xi;
+            }
+            s0r += gr;
+            s0i += gi;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                sxr[j] += q[j] * yr;
+                sxi[j] += q[j] * yi;
+            }
+#pragma unroll
+            for (int j = 0; j < 6; j++) {
+                sur[j] -= q[3 + j] * gr;
+                sui[j] -= q[3 + j] * gi;
+            }
+        }
+        stage_f64(stage, kk, lane, w * s0r, w * s0i);
+        if (orig >= 0) {
+            const double c = ld_f64(mult32, k);
+            const double cx = 6.283185307179586477 * w * c, cu = 19.739208802178717238 * w * c, co = mw * c;
+            double* o = rows + ((size_t)(k - jb.k0) * kJacPlanes * jb.n_atoms + orig) * 2;
+            // xyz: 2 pi i w c sum h'_s Y_s
+#pragma unroll
+            for (int i = 0; i < 3; i++) put_row(o + i * plane, -cx * sxi[i], cx * sxr[i]);
+            // ADP: 2 pi^2 w c sum (-m'(h_s)) X_s; U_iso: 2 pi^2 w c (-|h|^2) sum X_s
+            double vr[6], vi[6];
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                vr[i] = kind == 6 ? cu * sur[i] : 0.0;
+                vi[i] = kind == 6 ? cu * sui[i] : 0.0;
+            }
+            if (kind == 1) {
+                vr[0] = -cu * hl * hl * s0r;
+                vi[0] = -cu * hl * hl * s0i;
+            }
+#pragma unroll
+            for (int i = 0; i < 6; i++) put_row(o + (3 + i) * plane, vr[i], vi[i]);
+            put_row(o + 9 * plane, co * s0r, co * s0i);
+        }
+    }
+    flush_f64(stage, nk, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
+    }
+}
+
+template <bool CENTRIC, bool OTF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_jac(MainArgs a, JacArgs jb)
+{
+    __shared__ __align__(16) unsigned char stage_lds[kWavesPerBlock * kStageBytesF64];
+    const int wib = (int)(threadIdx.x / kWave);
+    const int unit = blockIdx.x * kWavesPerBlock + wib;
+    if (unit >= jb.n_units) return;
+    const int wave = unit % a.n_waves;
+    const int chunk_id = unit / a.n_waves;
+    const int k0 = jb.k0 + chunk_id * jb.chunk;
+    const int k1 = min(k0 + jb.chunk, jb.k1);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int cls = __builtin_amdgcn_readfirstlane(cmem(a.wave_l)[wave]);
+    const unsigned* hop = cmem(static_cast<const unsigned*>(a.hklop));
+    const unsigned* jrec = cmem(reinterpret_cast<const unsigned*>(jb.jrec));
+    const double* rad = cmem(static_cast<const double*>(a.radial));
+    const unsigned* mult32 = cmem(reinterpret_cast<const unsigned*>(a.dtdf + 2 * (size_t)a.n_hkl));
+    double2* st = reinterpret_cast<double2*>(stage_lds + wib * kStageBytesF64);
+    if (cls <= 2)
+        body_jac<2, CENTRIC, OTF>(a, jb, wave, lane, k0, k1, hop, jrec, rad, mult32, st, gmem(a.fpart),
+                                  gmem(jb.rows));
+    else
+        body_jac<4, CENTRIC, OTF>(a, jb, wave, lane, k0, k1, hop, jrec, rad, mult32, st, gmem(a.fpart),
+                                  gmem(jb.rows));
+}
+
+// F of a block of reflections [k0, k0 + n): mult(h) * sum_waves partial, into f[k - k0].
+__global__ void k_reduce_f_rows(const double2* __restrict__ fpart, const double* __restrict__ mult, int n_waves,
+                                int n_hkl, int k0, int n, double* __restrict__ f)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k = k0 + i;
+    double re = 0, im = 0;
+    for (int w = 0; w < n_waves; w++) {
+        const double2 v = fpart[(size_t)w * n_hkl + k];
+        re += v.x;
+        im += v.y;
+    }
+    f[2 * i] = re * mult[k];
+    f[2 * i + 1] = im * mult[k];
+}
+
 // F(h) = mult(h) * sum_waves partial  (centric at origin: 2 Re, 0).  Loads are issued eight at a
 // time and added in wave order.
 __device__ __forceinline__ void reduce_f_item(int k, const double2* __restrict__ fpart,
@@ -1550,12 +1785,16 @@ Engine::~Engine()
 
 void Engine::synchronize() { HIP_CHECK(hipStreamSynchronize(stream_)); }
 
-void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const double* dtdf)
+void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const double* dtdf, bool rows_layout)
 {
+    const bool sort_hkl = !rows_layout;
     HIP_CHECK(hipSetDevice(device_));
     if (n_hkl < 0 || at.n < 0) throw Error(1, "negative size");
     n_atoms_ = at.n;
     n_hkl_ = n_hkl;
+    jac_ok_ = false;
+    jac_prepped_ = false;
+    rows_layout_ = rows_layout;
     for (int i = 0; i < at.n; i++)
         if (at.type[i] < 0 || at.type[i] >= model_.n_types) throw Error(1, "atom type index out of range");
     // sort included atoms by multipole class, then type; pad each class to whole waves
@@ -1563,23 +1802,41 @@ void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const doub
     for (int i = 0; i < at.n; i++)
         if (at.include[i]) idx.push_back(i);
     auto cls_of = [&](int i) { return type_class(at.type[i]); };
-    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
-        int cx = cls_of(x), cy = cls_of(y);
-        if (cx != cy) return cx < cy;
-        return at.type[x] < at.type[y];
-    });
     order_.clear();
     std::vector<int> wave_cls;
-    for (int pass = 2; pass <= 4; pass += 2) {
-        size_t start = order_.size();
-        for (int i : idx)
-            if (cls_of(i) == pass) order_.push_back(i);
-        size_t cnt = order_.size() - start;
-        size_t padded = (cnt + kWave - 1) / kWave * kWave;
-        while (order_.size() < start + padded) order_.push_back(-1);
-        for (size_t wv = 0; wv < padded / kWave; wv++) wave_cls.push_back(pass);
+    if (rows_layout) {
+        // caller order (included atoms), so that a wave's 64 atoms are adjacent in every
+        // per-reflection row and its stores are contiguous; a wave takes the larger multipole
+        // class of its atoms (the l > 2 coefficients of a class-2 type are zero)
+        for (int i = 0; i < at.n; i++)
+            if (at.include[i]) order_.push_back(i);
+        while (order_.size() % kWave) order_.push_back(-1);
+        for (size_t wv = 0; wv < order_.size() / kWave; wv++) {
+            int c = 2;
+            for (int l = 0; l < kWave; l++) {
+                const int i = order_[wv * kWave + l];
+                if (i >= 0) c = std::max(c, cls_of(i));
+            }
+            wave_cls.push_back(c);
+        }
+    } else {
+        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+            int cx = cls_of(x), cy = cls_of(y);
+            if (cx != cy) return cx < cy;
+            return at.type[x] < at.type[y];
+        });
+        for (int pass = 2; pass <= 4; pass += 2) {
+            size_t start = order_.size();
+            for (int i : idx)
+                if (cls_of(i) == pass) order_.push_back(i);
+            size_t cnt = order_.size() - start;
+            size_t padded = (cnt + kWave - 1) / kWave * kWave;
+            while (order_.size() < start + padded) order_.push_back(-1);
+            for (size_t wv = 0; wv < padded / kWave; wv++) wave_cls.push_back(pass);
+        }
     }
     n_sorted_ = (int)order_.size();
+    n_included_ = (int)idx.size();
     std::vector<AtomIn> ain(std::max(n_sorted_, 1));
     for (int s = 0; s < n_sorted_; s++) {
         AtomIn& r = ain[s];
@@ -1619,8 +1876,10 @@ void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const doub
     }
     hkl_perm_.resize(n_hkl);
     for (int k = 0; k < n_hkl; k++) hkl_perm_[k] = k;
-    std::stable_sort(hkl_perm_.begin(), hkl_perm_.end(), [&](int x, int y) { return h2[x] < h2[y]; });
-    h_max_ = n_hkl > 0 ? std::sqrt(h2[hkl_perm_[n_hkl - 1]]) : 0.0;
+    if (sort_hkl)
+        std::stable_sort(hkl_perm_.begin(), hkl_perm_.end(), [&](int x, int y) { return h2[x] < h2[y]; });
+    h_max_ = 0.0;
+    for (int k = 0; k < n_hkl; k++) h_max_ = std::max(h_max_, std::sqrt(h2[k]));
     plan_tables();
     std::vector<int> hkl_s(3 * (size_t)std::max(n_hkl, 1));
     std::vector<double> dtdf_s(3 * (size_t)std::max(n_hkl, 1));
@@ -1649,11 +1908,13 @@ __global__ void k_fill(double* __restrict__ p, size_t n, double v)
     if (i < n) p[i] = v;
 }
 
-void Engine::fill(DeviceBuffer& b, size_t n, double v)
+void Engine::fill(DeviceBuffer& b, size_t n, double v) { fill(b.as<double>(), n, v); }
+
+void Engine::fill(double* p, size_t n, double v)
 {
     if (n == 0) return;
     const int T = 256;
-    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + T - 1) / T)), dim3(T), 0, stream_, b.as<double>(), n, v);
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + T - 1) / T)), dim3(T), 0, stream_, p, n, v);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -1674,6 +1935,7 @@ void Engine::ensure_prep_buffers(Precision prec, bool mp)
 
 void Engine::run_prep(Precision prec, bool mp)
 {
+    jac_prepped_ = false;
     ensure_prep_buffers(prec, mp);
     const int n_ops = (int)ops_.size();
     const int n_types = std::max(model_.n_types, 1);
@@ -1769,7 +2031,7 @@ void Engine::main_args(Precision prec, bool derivs, bool mp, int chunk, MainArgs
     const int n_chunks = (n_hkl + chunk - 1) / chunk;
     d_fpart_.ensure((size_t)n_waves * n_hkl * 2 * sizeof(double));
     const int nd = mp ? kNdMultipole : kNdBasic;
-    d_dpart_.ensure((size_t)n_chunks * n_slots * nd * sizeof(double));
+    if (derivs || mp) d_dpart_.ensure((size_t)n_chunks * n_slots * nd * sizeof(double));
     MainArgs& a = *out;
     std::memset(&a, 0, sizeof(a));
     a.hklop = d_hklop_.p;
@@ -1955,6 +2217,108 @@ void Engine::download(double* f, double* d10)
         }
 }
 
+// ------------------------------------------------------------------ per-reflection rows
+
+void Engine::set_jacobian_conventions(const JacConv& conv)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    if ((int)conv.mult_w.size() != n_atoms_ || (int)conv.adp_kind.size() != n_atoms_)
+        throw Error(1, "internal: per-atom data of the per-reflection rows do not match the upload");
+    if (!rows_layout_) throw Error(1, "internal: per-reflection rows need the caller's reflection and atom order");
+    std::vector<double> m(45);
+    std::memcpy(m.data(), conv.xyz_m, 9 * sizeof(double));
+    std::memcpy(m.data() + 9, conv.adp_m, 36 * sizeof(double));
+    d_jconv_.ensure(m.size() * sizeof(double));
+    HIP_CHECK(hipMemcpyAsync(d_jconv_.p, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+    std::vector<JacAtom> ja(std::max(n_atoms_, 1));
+    for (int i = 0; i < n_atoms_; i++) {
+        ja[i].mult_w = conv.mult_w[i];
+        ja[i].adp_kind = conv.adp_kind[i];
+        ja[i].pad = 0;
+    }
+    d_jatoms_.ensure(ja.size() * sizeof(JacAtom));
+    HIP_CHECK(hipMemcpyAsync(d_jatoms_.p, ja.data(), ja.size() * sizeof(JacAtom), hipMemcpyHostToDevice, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    jac_ok_ = true;
+}
+
+static const void* jac_kernel(bool centric, bool otf)
+{
+    if (centric) return otf ? (const void*)k_jac<true, true> : (const void*)k_jac<true, false>;
+    return otf ? (const void*)k_jac<false, true> : (const void*)k_jac<false, false>;
+}
+
+// Reflections per work unit: enough units for two rounds of the resident waves, at most 64.
+int Engine::jac_chunk(int n_rows)
+{
+    if (tune.hkl_chunk > 0) return tune.hkl_chunk;
+    if (jac_resident_ == 0) {
+        int blocks = 0;
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, jac_kernel(centric_origin_, use_otf(false)),
+                                                               256, 0));
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, device_));
+        jac_resident_ = std::max(1, blocks) * kWavesPerBlock * prop.multiProcessorCount;
+    }
+    const long long work = (long long)n_rows * std::max(n_sorted_ / kWave, 1);
+    return (int)std::max(1LL, std::min(64LL, work / (2LL * jac_resident_)));
+}
+
+void Engine::jacobian(int k0, int k1, double* f_dev, double* rows_dev)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    if (!jac_ok_) throw Error(1, "per-reflection derivatives: no upload for them (dh_hc_jacobian_upload)");
+    if (k0 < 0 || k1 > n_hkl_ || k0 > k1) throw Error(1, "reflection range outside the uploaded reflections");
+    const int n = k1 - k0;
+    if (n == 0) return;
+    if (!f_dev || !rows_dev) throw Error(1, "null output pointer");
+    const int n_waves = n_sorted_ / kWave;
+    if (n_waves == 0 || n_included_ < n_atoms_) fill(rows_dev, (size_t)n * kJacPlanes * n_atoms_ * 2, 0.0);
+    if (n_waves == 0) {
+        fill(f_dev, (size_t)n * 2, 0.0);
+        return;
+    }
+    if (!jac_prepped_) {
+        run_prep(kFp64, false);
+        const int n_rec = n_hkl_ * (int)ops_.size();
+        d_jrec_.ensure((size_t)std::max(n_rec, 1) * kJacRec * sizeof(double));
+        hipLaunchKernelGGL(k_jac_rec, dim3((n_rec + 255) / 256), dim3(256), 0, stream_, d_hklop_.as<double>(), n_rec,
+                           d_jconv_.as<double>(), d_jrec_.as<double>());
+        HIP_CHECK(hipGetLastError());
+        jac_prepped_ = true;
+    }
+    const int chunk = jac_chunk(n);
+    MainArgs a;
+    main_args(kFp64, false, false, chunk, &a);
+    JacArgs jb;
+    std::memset(&jb, 0, sizeof(jb));
+    jb.rows = rows_dev;
+    jb.f = f_dev;
+    jb.atoms = d_jatoms_.as<JacAtom>();
+    jb.n_atoms = n_atoms_;
+    jb.k0 = k0;
+    jb.k1 = k1;
+    jb.jrec = d_jrec_.as<double>();
+    jb.chunk = chunk;
+    jb.n_units = n_waves * ((n + chunk - 1) / chunk);
+    last_chunk = chunk;
+    last_grid_x = (jb.n_units + kWavesPerBlock - 1) / kWavesPerBlock;
+    last_grid_y = 1;
+    void* args[] = {&a, &jb};
+    if (timing) HIP_CHECK(hipEventRecord(ev0_, stream_));
+    HIP_CHECK(hipLaunchKernel(jac_kernel(centric_origin_, use_otf(false)), dim3(last_grid_x), dim3(256), args, 0,
+                              stream_));
+    if (timing) HIP_CHECK(hipEventRecord(ev1_, stream_));
+    const int T = 64;
+    hipLaunchKernelGGL(k_reduce_f_rows, dim3((n + T - 1) / T), dim3(T), 0, stream_, d_fpart_.as<double2>(),
+                       d_dtdf_.as<double>() + 2 * (size_t)n_hkl_, n_waves, n_hkl_, k0, n, f_dev);
+    HIP_CHECK(hipGetLastError());
+    if (timing) {
+        HIP_CHECK(hipEventSynchronize(ev1_));
+        HIP_CHECK(hipEventElapsedTime(&last_main_ms, ev0_, ev1_));
+    }
+}
+
 // ------------------------------------------------------------------ batched structures
 
 struct BatchItem {
@@ -2086,6 +2450,7 @@ void Engine::batch_item(Precision prec, bool derivs, int chunk, BatchItem* out)
 {
     HIP_CHECK(hipSetDevice(device_));
     const bool mixed = prec == kMixed;
+    jac_prepped_ = false;     // the batched prep rewrites this engine's tables
     ensure_prep_buffers(prec, false);
     BatchItem& q = *out;
     std::memset(&q, 0, sizeof(q));

@@ -9,6 +9,13 @@
 
 namespace discamb_amd {
 
+#define HIP_CHECK(x)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess)                                                                     \
+            throw Error(3, std::string("HIP error ") + hipGetErrorString(e_) + " at " #x);       \
+    } while (0)
+
 enum Precision { kFp64 = 0, kMixed = 1 };
 
 struct BatchItem;      // per-structure arguments of the batched kernels (device layout)
@@ -51,13 +58,32 @@ struct EngineAtoms {
     std::vector<unsigned char> include;
 };
 
+// Per-reflection derivative rows (the per-hkl path): the caller's derivative conventions and the
+// per-atom data the rows need, in caller atom order.
+struct JacConv {
+    double xyz_m[9];              // caller xyz derivative = xyz_m . d/dx_frac (row-major)
+    double adp_m[36];             // caller ADP derivative = adp_m . d/dU_cart (row-major)
+    std::vector<double> mult_w;   // m / |G|
+    std::vector<int> adp_kind;    // 0 (no ADP), 1 (U_iso), 6
+};
+
 class Engine {
 public:
     Engine(int device, const EngineModel& model, const double* cell6, const std::vector<EngineOp>& ops,
            bool centric_origin);
     ~Engine();
-    // Upload one call's inputs; hkl as int triples, dtdf already conj * centering multiplier.
-    void upload(const EngineAtoms& atoms, int n_hkl, const int* hkl, const double* dtdf);
+    // Upload one call's inputs; hkl as int triples, dtdf already conj * centering multiplier
+    // followed by the n_hkl centering multipliers.  By default reflections are processed in order
+    // of increasing |h| and atoms grouped by multipole class and type (results are returned in
+    // caller order either way); rows_layout keeps the caller's order of both on the device, which
+    // the per-reflection rows need.
+    void upload(const EngineAtoms& atoms, int n_hkl, const int* hkl, const double* dtdf, bool rows_layout = false);
+    // Per-reflection derivative rows after an upload with rows_layout: set the conventions,
+    // then compute F and the rows of reflections [k0, k1) into device memory, asynchronously on
+    // the engine stream.  rows: [k - k0][10 planes][n_atoms] complex; f: [k - k0] complex.
+    void set_jacobian_conventions(const JacConv& conv);
+    void jacobian(int k0, int k1, double* f_dev, double* rows_dev);
+    int n_atoms() const { return n_atoms_; }
     // Run all kernels on the uploaded inputs (asynchronous on the engine stream).  mp adds the
     // multipole-parameter adjoint (implies derivs).
     void run(Precision prec, bool derivs, bool mp = false);
@@ -99,6 +125,8 @@ private:
     void launch_main(Precision prec, bool derivs, bool mp, int chunk);
     void reduce(bool derivs, bool mp, int chunk);
     void fill(DeviceBuffer& b, size_t n, double v);
+    void fill(double* p, size_t n, double v);
+    int jac_chunk(int n_rows);
     int resident_waves(bool mixed, bool derivs, bool mp);
     int default_chunk(bool mixed, bool derivs, bool mp);
     int chunk_for(Precision prec, bool derivs, bool mp);
@@ -139,6 +167,13 @@ private:
     // per call
     DeviceBuffer d_hkl_, d_dtdf_, d_atoms_in_, d_atoms_, d_hklop_, d_hklop32_, d_phase_, d_radial_,
         d_radial32_, d_ht_, d_fpart_, d_dpart_, d_f_, d_d_, d_order_, d_radx_, d_tab_, d_tab32_, d_tabrec_;
+    // per-reflection rows
+    bool jac_ok_ = false;           // conventions set for the current upload
+    bool jac_prepped_ = false;      // the prep tables hold the FP64 records of the current upload
+    bool rows_layout_ = false;
+    int n_included_ = 0;
+    int jac_resident_ = 0;
+    DeviceBuffer d_jatoms_, d_jconv_, d_jrec_;
     double h_max_ = 0.0;            // largest |h| of the uploaded reflections
     int n_tab_nodes_ = 0;           // form-factor table nodes over all wavefunctions
 };

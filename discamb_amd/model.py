@@ -163,6 +163,35 @@ class TargetFunctionAtomicParamDerivatives:
     occupancy_derivatives: np.ndarray
 
 
+class SfDerivativesAtHkl:
+    """dF/dp of one reflection for every atom (SF_CalcDataTypes.h:49-55), complex:
+    atomicPostionDerivatives (n, 3), adpDerivatives (n, 6) (U_iso in column 0, unused columns 0),
+    occupancyDerivatives (n,), anomalousScatteringDerivatives (empty: the per-hkl engine leaves it
+    unset).  The field names keep the reference's spelling.  Instances handed to a per-hkl
+    processor view library memory that the next reflection overwrites: copy what must outlive
+    the callback."""
+
+    __slots__ = ("row", "anomalousScatteringDerivatives")
+
+    def __init__(self, row):
+        self.row = row                               # (10, n) complex
+        self.anomalousScatteringDerivatives = np.zeros(0, dtype=np.complex128)
+
+    @property
+    def atomicPostionDerivatives(self):
+        return self.row[0:3].T
+
+    atomicPositionDerivatives = atomicPostionDerivatives
+
+    @property
+    def adpDerivatives(self):
+        return self.row[3:9].T
+
+    @property
+    def occupancyDerivatives(self):
+        return self.row[9]
+
+
 def _bank():
     global _BANK
     if _BANK is None:

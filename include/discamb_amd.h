@@ -27,6 +27,7 @@ extern "C" {
 #define DH_E_DEVICE 3     /* HIP runtime failure */
 #define DH_E_MEMORY 4     /* device allocation failure */
 #define DH_E_NO_DEVICE 5  /* no usable GPU */
+#define DH_E_CALLBACK 6   /* a per-reflection callback asked to stop (non-zero return) */
 
 /* engine precision (setCalculationMode strings "HIP" and "HIP_MIXED") */
 #define DH_PRECISION_FP64 0
@@ -159,6 +160,38 @@ int dh_hc_last_timing(dh_calculator* calc, double* main_kernel_ms, int* grid_x, 
 /* explicit operations used by the engine (after inversion / centering folding) */
 int dh_hc_engine_info(dh_calculator* calc, int* n_explicit_ops, int* group_order, int* centrosymmetric,
                       char* centering);
+
+/* ---- Per-reflection derivatives: the reference's per-hkl overload
+ *   template<T> calculateStructureFactorsAndDerivatives(atoms, lcs, hkl, T& singleHklResultsProcessor
+ *       [, countAtomContribution])     HansenCoppensStructureFactorCalculator.h:206-242, 384-545
+ * which runs HansenCoppens_SF_Engine::calculateSF<T> (HansenCoppens_SF_Engine.h:512-780) and calls
+ *   singleHklResultsProcessor.onPerHklCalculation(hkl_idx, F, SfDerivativesAtHkl)
+ * once per reflection in input order.  Each reflection's derivatives form one row of
+ * DH_JAC_PLANES planes of n_atoms complex values (re, im interleaved), in the calculator's
+ * derivative conventions (as the reference's WrapperForSingleHklResultProcessor converts them):
+ *   plane 0-2  dF/dx, dF/dy, dF/dz          (SfDerivativesAtHkl::atomicPostionDerivatives)
+ *   plane 3-8  dF/dU11 .. dF/dU23           (adpDerivatives; U_iso atoms: plane 3 only, 4-8 zero;
+ *                                            atoms without ADPs: zero)
+ *   plane 9    dF/docc                      (occupancyDerivatives)
+ * F follows the per-hkl engine, which differs from the summed one for centrosymmetric groups
+ * with f'' != 0: it keeps f'' unconjugated for the inversion partner.  Atoms excluded by
+ * countAtomContribution (dh_atoms.include) have zero entries and no part in F.  The rows are
+ * computed in FP64 in every calculation mode. */
+#define DH_JAC_PLANES 10
+/* Called once per reflection, hkl_index = 0 .. n_hkl-1 in order; f = 2 doubles, row =
+ * 2 * DH_JAC_PLANES * n_atoms doubles, valid during the call only.  Return 0 to continue; any
+ * other value stops the calculation, which then returns DH_E_CALLBACK. */
+typedef int (*dh_per_hkl_fn)(void* user, int hkl_index, const double* f, const double* row);
+int dh_hc_per_hkl(dh_calculator* calc, const dh_atoms* atoms, int n_hkl, const int* hkl, dh_per_hkl_fn fn,
+                  void* user);
+/* The same rows in bulk: upload once, then fetch any reflection range [k0, k1) into host memory
+ * (f: 2 (k1-k0) doubles, rows: (k1-k0) rows; either may be NULL) ... */
+int dh_hc_jacobian_upload(dh_calculator* calc, const dh_atoms* atoms, int n_hkl, const int* hkl);
+int dh_hc_jacobian_rows(dh_calculator* calc, int k0, int k1, double* f, double* rows);
+/* ... or into device memory (HIP device pointers, same layout), ordered after the work already
+ * queued on `stream` (a hipStream_t; NULL = the null stream), which waits for the rows before its
+ * later work; returns without waiting on the host. */
+int dh_hc_jacobian_rows_dev(dh_calculator* calc, int k0, int k1, void* f_dev, void* rows_dev, void* stream);
 
 /* ---- Batches of structures (no reference counterpart: the reference computes one structure per
  * call; a refinement of many small structures calls it once per structure).

@@ -324,6 +324,190 @@ int hc_oracle_engine(
     return 0;
 }
 
+/* Per-reflection structure factors and derivatives: the per-hkl engine that the reference's
+ * template calculateStructureFactorsAndDerivatives<T>(atoms, lcs, hkl, processor[, include])
+ * dispatches to (HansenCoppens_SF_Engine.h:512-780, Engine v1, not Engine3).  For every
+ * reflection k it returns F_k and, per atom a, the complex derivatives
+ *   rows[((k * 10 + plane) * n_atoms + a) * 2 + {re, im}]
+ *   plane 0-2: dF/dx_cart   = w c 2 pi i sum_s h_s Y_s            (.h:767-770)
+ *   plane 3-8: dF/dU_cart   = 2 pi^2 w c sum_s (-m_j(h_s)) X_s     (.h:728-738, 755-762)
+ *              U_iso atoms: plane 3 = 2 pi^2 w c sum_s (-|h|^2) X_s; atoms without ADP: 0
+ *   plane 9:   dF/docc      = (m/|G|) c sum_s X_s                 (.h:741)
+ * with w = occ m/|G| and c the centering multiplier (.h:637), X_s the transformed atomic form
+ * factor of operation s and Y_s its position-derivative multiplier:
+ *   acentric              X = T (f + f_def) e^{i phi},  Y = X                             (.h:690-699)
+ *   centre at t_inv != 0  t1 = (f + f_def) e^{i phi}, t2 = (f + conj f_def) e^{-i phi} e^{2 pi i h.t_inv},
+ *                         X = T (t1 + t2),  Y = T (t1 - t2)                               (.h:704-714)
+ *   centre at the origin  X = 2 T (f cos phi + Re(f_def e^{i phi})),
+ *                         Y = 2 T i (f sin phi + Im(f_def e^{i phi}))                      (.h:716-724)
+ * where f = f_core + Pval f_val + f' + i f'' is NOT conjugated for the inversion partner (Engine3
+ * conjugates it - the two engines differ for centrosymmetric groups with f'' != 0).  The reference
+ * divides by |h| for h = 0 here; this restatement sets the deformation term of h = 0 to zero, as
+ * Engine3 does.  Excluded atoms keep zero derivatives and do not enter F (.h:622-623).
+ * Inputs as hc_oracle_engine, plus mult = the centering multiplier of every reflection. */
+int hc_oracle_per_hkl(
+    const oracle_model *mdl,
+    int n_atoms, const int *atom_wfn, const int *atom_type, const double *xyz, const int *adp_n,
+    const double *adp, const double *occ, const double *mult_w, const double *lcs,
+    const double *anom_atom, const unsigned char *include,
+    int n_ops, const double *rot, const double *trans, int centro, const double *inv_t,
+    int n_hkl, const double *hv, const double *mult,
+    double *f_out, double *rows, int n_threads)
+{
+    const double two_pi = 2.0 * M_PI, two_pi2 = 2.0 * M_PI * M_PI;
+    int *type_wfn = (int *)calloc(mdl->n_types > 0 ? mdl->n_types : 1, sizeof(int));
+    int *type_max_l = (int *)calloc(mdl->n_types > 0 ? mdl->n_types : 1, sizeof(int));
+    for (int a = 0; a < n_atoms; a++) type_wfn[atom_type[a]] = atom_wfn[a];
+    /* pre_hkl_loop_sf_calc (HansenCoppens_SF_Engine.cpp:160-221) */
+    for (int t = 0; t < mdl->n_types; t++) {
+        int lp = -1;
+        for (int l = 0; l < mdl->plm_nl[t]; l++)
+            for (int i = l * l; i < (l + 1) * (l + 1); i++)
+                if (mdl->plm[25 * t + i] != 0.0) lp = l;
+        int lw = mdl->dv_nl[type_wfn[t]] - 1;
+        int ml = lw < 4 ? lw : 4;
+        type_max_l[t] = ml < lp ? ml : lp;
+    }
+    int centric_origin = centro && inv_t[0] == 0.0 && inv_t[1] == 0.0 && inv_t[2] == 0.0;
+    int centric_shift = centro && !centric_origin;
+    int *adp_off = (int *)malloc(sizeof(int) * (n_atoms + 1));
+    int n_adp_tot = 0;
+    for (int a = 0; a < n_atoms; a++) { adp_off[a] = n_adp_tot; n_adp_tot += adp_n[a]; }
+    if (n_threads < 1) n_threads = 1;
+#ifndef _OPENMP
+    n_threads = 1;
+#endif
+#pragma omp parallel num_threads(n_threads)
+    {
+        double *fcore = (double *)malloc(sizeof(double) * (mdl->n_wfn + 1));
+        double *fval = (double *)malloc(sizeof(double) * (mdl->n_types + 1));
+        double *gl = (double *)malloc(sizeof(double) * 5 * (mdl->n_types + 1));
+#pragma omp for schedule(dynamic, 16)
+        for (int k = 0; k < n_hkl; k++) {
+            const double *h = hv + 3 * k;
+            double hl2 = h[0] * h[0] + h[1] * h[1] + h[2] * h[2];
+            double hl = sqrt(hl2);
+            int h000 = hl < 1e-10;
+            double *row = rows + (size_t)k * 10 * n_atoms * 2;
+            memset(row, 0, sizeof(double) * 10 * (size_t)n_atoms * 2);
+            /* pre_atom_loop_sf_calc (HansenCoppens_SF_Engine.cpp:281-377) */
+            for (int w = 0; w < mdl->n_wfn; w++)
+                fcore[w] = sph_density_sf(mdl->core_c + mdl->core_off[w], mdl->core_e + mdl->core_off[w],
+                                          mdl->core_p + mdl->core_off[w],
+                                          mdl->core_off[w + 1] - mdl->core_off[w], hl);
+            for (int t = 0; t < mdl->n_types; t++) {
+                int w = type_wfn[t];
+                fval[t] = sph_density_sf(mdl->val_c + mdl->val_off[w], mdl->val_e + mdl->val_off[w],
+                                         mdl->val_p + mdl->val_off[w],
+                                         mdl->val_off[w + 1] - mdl->val_off[w], hl / mdl->kappa[t]);
+                for (int l = 0; l < 5; l++) {
+                    gl[5 * t + l] = 0.0;
+                    if (l < mdl->dv_nl[w]) {
+                        int nn = mdl->dv_n[5 * w + l];
+                        double zeta = mdl->dv_zeta[w];
+                        double norm = pow(zeta, nn + 3);
+                        for (int q = 2; q <= nn + 2; q++) norm /= q;
+                        gl[5 * t + l] = norm * g_fun(l, nn + 2, hl / mdl->kappa_p[t], zeta);
+                    }
+                }
+            }
+            double inv_re = 1.0, inv_im = 0.0;
+            if (centric_shift) {
+                double ph = two_pi * (h[0] * inv_t[0] + h[1] * inv_t[1] + h[2] * inv_t[2]);
+                inv_re = cos(ph);
+                inv_im = sin(ph);
+            }
+            const double c = mult[k];
+            double fr_tot = 0.0, fi_tot = 0.0;
+            for (int a = 0; a < n_atoms; a++) {
+                if (!include[a]) continue;
+                int w = atom_wfn[a], t = atom_type[a];
+                double weight = occ[a] * mult_w[a] * c;
+                double f_re = fcore[w] + fval[t] * mdl->pval[t];
+                double f_im = 0.0;
+                if (anom_atom) { f_re += anom_atom[2 * a]; f_im += anom_atom[2 * a + 1]; }
+                else { f_re += mdl->anom[2 * w]; f_im += mdl->anom[2 * w + 1]; }
+                const double *x = xyz + 3 * a;
+                const double *L = lcs + 9 * a;
+                const double *U = adp + adp_off[a];
+                int nu = adp_n[a];
+                double sr = 0, si = 0, xr[3] = {0, 0, 0}, xi[3] = {0, 0, 0};
+                double ur[6] = {0, 0, 0, 0, 0, 0}, ui[6] = {0, 0, 0, 0, 0, 0};
+                for (int s = 0; s < n_ops; s++) {
+                    const double *R = rot + 9 * s, *tt = trans + 3 * s;
+                    double hs[3];
+                    for (int j = 0; j < 3; j++) hs[j] = h[0] * R[j] + h[1] * R[3 + j] + h[2] * R[6 + j];
+                    double phase = two_pi * (hs[0] * x[0] + hs[1] * x[1] + hs[2] * x[2] +
+                                             h[0] * tt[0] + h[1] * tt[1] + h[2] * tt[2]);
+                    double pc = cos(phase), ps = sin(phase);
+                    double dre = 0.0, dim = 0.0;
+                    if (!h000) {
+                        double u[3] = {hs[0] / hl, hs[1] / hl, hs[2] / hl};
+                        double ul[3];
+                        for (int j = 0; j < 3; j++) ul[j] = L[j] * u[0] + L[3 + j] * u[1] + L[6 + j] * u[2];
+                        def_val(mdl->plm + 25 * t, type_max_l[t], gl + 5 * t, ul, &dre, &dim);
+                    }
+                    double m6[6] = {hs[0] * hs[0], hs[1] * hs[1], hs[2] * hs[2],
+                                    2 * hs[0] * hs[1], 2 * hs[0] * hs[2], 2 * hs[1] * hs[2]};
+                    double T = 1.0;
+                    if (nu == 6)
+                        T = exp(-(m6[0] * U[0] + m6[1] * U[1] + m6[2] * U[2] + m6[3] * U[3] + m6[4] * U[4] +
+                                  m6[5] * U[5]));
+                    else if (nu == 1)
+                        T = exp(-hl2 * U[0]);
+                    /* (f + f_def) e^{i phi} and f_def e^{i phi} */
+                    double er = (f_re + dre) * pc - (f_im + dim) * ps, ei = (f_re + dre) * ps + (f_im + dim) * pc;
+                    double Xr, Xi, Yr, Yi;
+                    if (!centro) {
+                        Xr = T * er; Xi = T * ei; Yr = Xr; Yi = Xi;
+                    } else if (centric_shift) {
+                        /* (f + conj f_def) e^{-i phi} e^{2 pi i h.t_inv} */
+                        double br = (f_re + dre) * pc + (f_im - dim) * ps, bi = (f_im - dim) * pc - (f_re + dre) * ps;
+                        double t2r = br * inv_re - bi * inv_im, t2i = br * inv_im + bi * inv_re;
+                        Xr = T * (er + t2r); Xi = T * (ei + t2i);
+                        Yr = T * (er - t2r); Yi = T * (ei - t2i);
+                    } else {
+                        double dpr = dre * pc - dim * ps, dpi = dre * ps + dim * pc;   /* f_def e^{i phi} */
+                        Xr = 2 * T * (f_re * pc + dpr); Xi = 2 * T * f_im * pc;
+                        /* 2 T i (f sin phi + i Im(f_def e^{i phi}) / i) */
+                        Yr = -2 * T * f_im * ps; Yi = 2 * T * (f_re * ps + dpi);
+                    }
+                    sr += Xr; si += Xi;
+                    for (int j = 0; j < 3; j++) { xr[j] += hs[j] * Yr; xi[j] += hs[j] * Yi; }
+                    if (nu == 1) { ur[0] -= hl2 * Xr; ui[0] -= hl2 * Xi; }
+                    else if (nu == 6)
+                        for (int j = 0; j < 6; j++) { ur[j] -= m6[j] * Xr; ui[j] -= m6[j] * Xi; }
+                }
+                fr_tot += weight * sr;
+                fi_tot += weight * si;
+                /* xyz: w 2 pi i sum h_s Y_s */
+                for (int j = 0; j < 3; j++) {
+                    double *o = row + ((size_t)j * n_atoms + a) * 2;
+                    o[0] = -two_pi * weight * xi[j];
+                    o[1] = two_pi * weight * xr[j];
+                }
+                for (int j = 0; j < nu; j++) {
+                    double *o = row + ((size_t)(3 + j) * n_atoms + a) * 2;
+                    o[0] = two_pi2 * weight * ur[j];
+                    o[1] = two_pi2 * weight * ui[j];
+                }
+                double *o = row + ((size_t)9 * n_atoms + a) * 2;
+                o[0] = sr * mult_w[a] * c;
+                o[1] = si * mult_w[a] * c;
+            }
+            f_out[2 * k] = fr_tot;
+            f_out[2 * k + 1] = fi_tot;
+        }
+        free(fcore);
+        free(fval);
+        free(gl);
+    }
+    free(adp_off);
+    free(type_wfn);
+    free(type_max_l);
+    return 0;
+}
+
 /* test hooks: the oracle's own g-function and harmonics */
 double hc_oracle_g(int l, int n, double h, double z) { return g_fun(l, n, h, z); }
 void hc_oracle_ylm(const double *u, double *d25) { ylm_density(u, d25); }

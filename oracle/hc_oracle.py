@@ -164,10 +164,8 @@ def u_cif_to_cart_matrix(cell):
     return J
 
 
-def sf_and_derivs(crystal, params, atoms, lcs, hkl, dtdf, include=None, n_threads=1, mode="reference",
-                  anomalous=None):
-    """Returns (F complex (m,), d_xyz (n,3), d_adp (n,6) (or U_iso in column 0), d_occ (n,)); derivative
-    conventions follow the crystal's input conventions."""
+def _engine_inputs(crystal, params, atoms, lcs, hkl, include, mode, anomalous):
+    """Everything the C restatement takes, in the engine's units (prepareDataForEngine_part)."""
     hkl = np.ascontiguousarray(np.asarray(hkl, dtype=np.int64).reshape(-1, 3))
     m = hkl.shape[0]
     n = atoms.n
@@ -182,18 +180,20 @@ def sf_and_derivs(crystal, params, atoms, lcs, hkl, dtdf, include=None, n_thread
         ops = subset
         centro_flag, tinv = (1 if centro else 0), t_inv
         mult = centering_mult(letter, obverse, hkl)
-    rot = np.ascontiguousarray(np.array([f2c @ R @ c2f for R, t in ops]).reshape(-1))
-    trans = np.ascontiguousarray(np.array([f2c @ t for R, t in ops]).reshape(-1))
-    inv_cart = np.ascontiguousarray(f2c @ tinv)
+    x = {}
+    x["rot"] = np.ascontiguousarray(np.array([f2c @ R @ c2f for R, t in ops]).reshape(-1))
+    x["trans"] = np.ascontiguousarray(np.array([f2c @ t for R, t in ops]).reshape(-1))
+    x["n_ops"], x["centro"] = len(ops), centro_flag
+    x["inv_cart"] = np.ascontiguousarray(f2c @ tinv)
     # model
     wfn = params.wfn_parameters
     core, val = [sphere_density(w, False) for w in wfn], [sphere_density(w, True) for w in wfn]
 
     def flat(parts):
-        off = np.cumsum([0] + [len(x[0]) for x in parts]).astype(np.int32)
-        c = np.array(sum([x[0] for x in parts], []) + [0.0])
-        e = np.array(sum([x[1] for x in parts], []) + [1.0])
-        p = np.array(sum([x[2] for x in parts], []) + [0], dtype=np.int32)
+        off = np.cumsum([0] + [len(q[0]) for q in parts]).astype(np.int32)
+        c = np.array(sum([q[0] for q in parts], []) + [0.0])
+        e = np.array(sum([q[1] for q in parts], []) + [1.0])
+        p = np.array(sum([q[2] for q in parts], []) + [0], dtype=np.int32)
         return off, c, e, p
 
     co, cc, ce, cp = flat(core)
@@ -212,14 +212,14 @@ def sf_and_derivs(crystal, params, atoms, lcs, hkl, dtdf, include=None, n_thread
     kapp = np.array([t.kappa_deformation_valence for t in tp])
     plm = np.ascontiguousarray(np.array([t.plm25() for t in tp]))
     plm_nl = np.array([len(t.p_lm) for t in tp], dtype=np.int32)
-    keep = [co, cc, ce, cp, vo, vc, ve, vp, dv_zeta, dv_n, dv_nl, anom, pval, kap, kapp, plm, plm_nl]
-    md = _Model(len(wfn), co.ctypes.data, vo.ctypes.data, cc.ctypes.data, ce.ctypes.data, cp.ctypes.data,
-                vc.ctypes.data, ve.ctypes.data, vp.ctypes.data, dv_zeta.ctypes.data, dv_n.ctypes.data,
-                dv_nl.ctypes.data, anom.ctypes.data, len(tp), pval.ctypes.data, kap.ctypes.data,
-                kapp.ctypes.data, plm.ctypes.data, plm_nl.ctypes.data)
+    x["keep"] = [co, cc, ce, cp, vo, vc, ve, vp, dv_zeta, dv_n, dv_nl, anom, pval, kap, kapp, plm, plm_nl]
+    x["model"] = _Model(len(wfn), co.ctypes.data, vo.ctypes.data, cc.ctypes.data, ce.ctypes.data, cp.ctypes.data,
+                        vc.ctypes.data, ve.ctypes.data, vp.ctypes.data, dv_zeta.ctypes.data, dv_n.ctypes.data,
+                        dv_nl.ctypes.data, anom.ctypes.data, len(tp), pval.ctypes.data, kap.ctypes.data,
+                        kapp.ctypes.data, plm.ctypes.data, plm_nl.ctypes.data)
     # atoms (prepareDataForEngine_part)
     xyz = atoms.xyz if crystal.xyz_convention == "fractional" else atoms.xyz @ c2f.T
-    xyz_cart = np.ascontiguousarray(xyz @ f2c.T)
+    x["xyz_cart"] = np.ascontiguousarray(xyz @ f2c.T)
     J = u_cif_to_cart_matrix(cell)
     adp_n = np.ascontiguousarray(atoms.adp_size.astype(np.int32))
     adp_flat = []
@@ -229,19 +229,38 @@ def sf_and_derivs(crystal, params, atoms, lcs, hkl, dtdf, include=None, n_thread
             adp_flat += list(2 * np.pi ** 2 * u)
         elif adp_n[i] == 1:
             adp_flat.append(2 * np.pi ** 2 * atoms.adp[i, 0])
-    adp_flat = np.array(adp_flat + [0.0])
-    occ = np.ascontiguousarray(atoms.occupancy, dtype=np.float64)
-    mult_w = np.ascontiguousarray(crystal.atoms.multiplicity / float(order))
-    lcs_f = np.ascontiguousarray(np.asarray(lcs, dtype=np.float64).reshape(n, 9))
-    an_atom = None
+    x["adp_n"], x["adp_flat"] = adp_n, np.array(adp_flat + [0.0])
+    x["occ"] = np.ascontiguousarray(atoms.occupancy, dtype=np.float64)
+    x["mult_w"] = np.ascontiguousarray(crystal.atoms.multiplicity / float(order))
+    x["lcs"] = np.ascontiguousarray(np.asarray(lcs, dtype=np.float64).reshape(n, 9))
+    x["anom_atom"] = None
     if anomalous is not None:
         a = np.asarray(anomalous, dtype=np.complex128)
-        an_atom = np.ascontiguousarray(np.stack([a.real, a.imag], 1))
-    inc = np.ones(n, dtype=np.uint8) if include is None else np.asarray(include, dtype=np.uint8)
-    a_wfn = np.ascontiguousarray(np.array(params.atom_to_wfn_map, dtype=np.int32))
-    a_type = np.ascontiguousarray(np.array(params.atom_to_type_map, dtype=np.int32))
-    # reflections
-    h_cart = np.ascontiguousarray(hkl.astype(np.float64) @ c2f)
+        x["anom_atom"] = np.ascontiguousarray(np.stack([a.real, a.imag], 1))
+    x["include"] = np.ones(n, dtype=np.uint8) if include is None else np.ascontiguousarray(include, dtype=np.uint8)
+    x["a_wfn"] = np.ascontiguousarray(np.array(params.atom_to_wfn_map, dtype=np.int32))
+    x["a_type"] = np.ascontiguousarray(np.array(params.atom_to_type_map, dtype=np.int32))
+    x["h_cart"] = np.ascontiguousarray(hkl.astype(np.float64) @ c2f)
+    x["mult"], x["m"], x["n"], x["f2c"], x["J"] = np.ascontiguousarray(mult, dtype=np.float64), m, n, f2c, J
+    return x
+
+
+def _atom_args(x):
+    P = ctypes.c_void_p
+    return [ctypes.byref(x["model"]), ctypes.c_int(x["n"]), P(x["a_wfn"].ctypes.data), P(x["a_type"].ctypes.data),
+            P(x["xyz_cart"].ctypes.data), P(x["adp_n"].ctypes.data), P(x["adp_flat"].ctypes.data),
+            P(x["occ"].ctypes.data), P(x["mult_w"].ctypes.data), P(x["lcs"].ctypes.data),
+            P(x["anom_atom"].ctypes.data) if x["anom_atom"] is not None else P(0), P(x["include"].ctypes.data),
+            ctypes.c_int(x["n_ops"]), P(x["rot"].ctypes.data), P(x["trans"].ctypes.data), ctypes.c_int(x["centro"]),
+            P(x["inv_cart"].ctypes.data)]
+
+
+def sf_and_derivs(crystal, params, atoms, lcs, hkl, dtdf, include=None, n_threads=1, mode="reference",
+                  anomalous=None):
+    """Returns (F complex (m,), d_xyz (n,3), d_adp (n,6) (or U_iso in column 0), d_occ (n,)); derivative
+    conventions follow the crystal's input conventions."""
+    x = _engine_inputs(crystal, params, atoms, lcs, hkl, include, mode, anomalous)
+    m, n, f2c, J, adp_n, mult = x["m"], x["n"], x["f2c"], x["J"], x["adp_n"], x["mult"]
     d = np.asarray(dtdf, dtype=np.complex128) * 1.0
     d = np.conj(d) * mult
     dd = np.ascontiguousarray(np.stack([d.real, d.imag], 1))
@@ -250,14 +269,9 @@ def sf_and_derivs(crystal, params, atoms, lcs, hkl, dtdf, include=None, n_thread
     du = np.zeros(max(1, int(adp_n.sum())))
     do = np.zeros(n)
     P = ctypes.c_void_p
-    rc = lib().hc_oracle_engine(
-        ctypes.byref(md), ctypes.c_int(n), P(a_wfn.ctypes.data), P(a_type.ctypes.data), P(xyz_cart.ctypes.data),
-        P(adp_n.ctypes.data), P(adp_flat.ctypes.data), P(occ.ctypes.data), P(mult_w.ctypes.data),
-        P(lcs_f.ctypes.data), P(an_atom.ctypes.data) if an_atom is not None else P(0), P(inc.ctypes.data),
-        ctypes.c_int(len(ops)), P(rot.ctypes.data), P(trans.ctypes.data), ctypes.c_int(centro_flag),
-        P(inv_cart.ctypes.data), ctypes.c_int(m), P(h_cart.ctypes.data), P(dd.ctypes.data), P(fo.ctypes.data),
-        P(dx.ctypes.data), P(du.ctypes.data), P(do.ctypes.data), ctypes.c_int(n_threads))
-    del keep
+    rc = lib().hc_oracle_engine(*_atom_args(x), ctypes.c_int(m), P(x["h_cart"].ctypes.data), P(dd.ctypes.data),
+                                P(fo.ctypes.data), P(dx.ctypes.data), P(du.ctypes.data), P(do.ctypes.data),
+                                ctypes.c_int(n_threads))
     if rc != 0:
         raise RuntimeError("oracle engine failed")
     F = (fo[:, 0] + 1j * fo[:, 1]) * mult
@@ -274,3 +288,29 @@ def sf_and_derivs(crystal, params, atoms, lcs, hkl, dtdf, include=None, n_thread
             d_adp[i, 0] = du[off]
             off += 1
     return F, d_xyz, d_adp, do
+
+
+def per_hkl(crystal, params, atoms, lcs, hkl, include=None, n_threads=1, anomalous=None):
+    """Per-reflection F and derivatives of the reference's per-hkl path (HansenCoppens_SF_Engine.h:
+    512-780 behind HansenCoppensStructureFactorCalculator.h:384-545).  Returns (F (m,) complex,
+    rows (m, 10, n) complex): planes dF/dxyz (3), dF/dU (6; U_iso in plane 3), dF/docc, converted to
+    the crystal's conventions as WrapperForSingleHklResultProcessor::convert does (.h:512-531)."""
+    x = _engine_inputs(crystal, params, atoms, lcs, hkl, include, "reference", anomalous)
+    m, n, f2c, J, adp_n = x["m"], x["n"], x["f2c"], x["J"], x["adp_n"]
+    fo = np.zeros((m, 2))
+    rows = np.zeros((m, 10, max(n, 1), 2))
+    P = ctypes.c_void_p
+    rc = lib().hc_oracle_per_hkl(*_atom_args(x), ctypes.c_int(m), P(x["h_cart"].ctypes.data),
+                                 P(x["mult"].ctypes.data), P(fo.ctypes.data), P(rows.ctypes.data),
+                                 ctypes.c_int(n_threads))
+    if rc != 0:
+        raise RuntimeError("oracle per-hkl engine failed")
+    F = fo[:, 0] + 1j * fo[:, 1]
+    R = rows[..., 0] + 1j * rows[..., 1]
+    R = R[:, :, :n]
+    if crystal.xyz_convention == "fractional":
+        R[:, 0:3, :] = np.einsum("ji,mjn->min", f2c, R[:, 0:3, :])
+    if crystal.adp_convention == "U_cif":
+        aniso = adp_n == 6
+        R[:, 3:9, aniso] = np.einsum("ji,mjn->min", J, R[:, 3:9, aniso])
+    return F, R

@@ -1,6 +1,8 @@
 // Test driver for the C++ drop-in class: reads a structure as a whitespace token stream (written
 // by tests/test_cpp_api.py), runs HansenCoppensStructureFactorCalculator on the GPU and prints
-// F and the derivatives.  Usage: hc_cpp_driver <input> <mode>
+// F and the derivatives.  Usage: hc_cpp_driver <input> <mode>; mode "per_hkl" runs the per-hkl
+// overload with a gradient-collecting processor and prints the same quantities.
+#include <array>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
@@ -8,6 +10,30 @@
 #include "discamb_amd/HansenCoppensStructureFactorCalculator.hpp"
 
 using namespace discamb_amd;
+
+// A per-hkl processor in the style of the reference's gradient collectors: keeps F and contracts
+// each reflection's dF/dp with dT/dF into dT/dp = sum_h Re(conj(dT/dF_h) dF_h/dp).
+struct GradientCollector {
+    const std::vector<std::complex<double>>* dtdf = nullptr;
+    std::vector<std::complex<double>> f;
+    std::vector<std::array<double, 10>> g;
+    int next = 0;
+    bool ordered = true;
+    int throw_at = -1;
+    void onPerHklCalculation(int k, std::complex<double>& F, SfDerivativesAtHkl& d)
+    {
+        if (k == throw_at) throw std::runtime_error("processor stop");
+        ordered = ordered && k == next;
+        next = k + 1;
+        f[k] = F;
+        const std::complex<double> c = std::conj((*dtdf)[k]);
+        for (size_t a = 0; a < g.size(); a++) {
+            for (int j = 0; j < 3; j++) g[a][j] += (c * d.atomicPostionDerivatives[a][j]).real();
+            for (size_t j = 0; j < d.adpDerivatives[a].size(); j++) g[a][3 + j] += (c * d.adpDerivatives[a][j]).real();
+            g[a][9] += (c * d.occupancyDerivatives[a]).real();
+        }
+    }
+};
 
 int main(int argc, char** argv)
 {
@@ -94,6 +120,33 @@ int main(int argc, char** argv)
     }
     try {
         HansenCoppensStructureFactorCalculator calc(crystal, p);
+        if (std::string(argv[2]) == "per_hkl") {
+            GradientCollector gc;
+            gc.dtdf = &dtdf;
+            gc.f.resize(hkl.size());
+            gc.g.assign(crystal.atoms.size(), {});
+            calc.calculateStructureFactorsAndDerivatives(crystal.atoms, lcs, hkl, gc, include);
+            std::cout << std::setprecision(17);
+            for (auto& v : gc.f) std::cout << v.real() << " " << v.imag() << "\n";
+            for (size_t a = 0; a < gc.g.size(); a++) {
+                const size_t na = crystal.atoms[a].adp.size();
+                std::cout << gc.g[a][0] << " " << gc.g[a][1] << " " << gc.g[a][2] << " " << na;
+                for (size_t j = 0; j < na; j++) std::cout << " " << gc.g[a][3 + j];
+                std::cout << " " << gc.g[a][9] << "\n";
+            }
+            // an exception thrown by the processor reaches the caller
+            GradientCollector bad = gc;
+            bad.throw_at = 3;
+            bool threw = false;
+            try {
+                calc.calculateStructureFactorsAndDerivatives(crystal.atoms, lcs, hkl, bad, include);
+            } catch (const std::runtime_error& e) {
+                threw = std::string(e.what()) == "processor stop";
+            }
+            std::cerr << (gc.ordered && gc.next == (int)hkl.size() ? "order ok\n" : "order wrong\n");
+            std::cerr << (threw ? "rethrow ok\n" : "rethrow missing\n");
+            return gc.ordered && threw ? 0 : 1;
+        }
         calc.setCalculationMode(argv[2]);
         std::vector<std::complex<double>> f;
         std::vector<TargetFunctionAtomicParamDerivatives> d;

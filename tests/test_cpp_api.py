@@ -54,8 +54,12 @@ def write_input(w, path, include):
         f.write(" ".join(str(t) for t in tok))
 
 
-@pytest.mark.parametrize("sg", ["P21/c", "P212121"])
-def test_cpp_drop_in_matches_oracle(tmp_path, sg):
+@pytest.mark.parametrize("sg,mode", [("P21/c", "HIP"), ("P212121", "HIP"), ("P21/c", "per_hkl"),
+                                     ("P212121", "per_hkl")])
+def test_cpp_drop_in_matches_oracle(tmp_path, sg, mode):
+    """mode per_hkl: the per-hkl template overload with a processor that contracts every
+    reflection's SfDerivativesAtHkl with dT/dF; without f'' the contraction equals the summed
+    derivatives."""
     if not os.path.exists(DRIVER):
         pytest.fail("tests/cpp/hc_cpp_driver not built (run __graft_entry__.build())")
     w = synth.small_structure(sg, n_atoms=9, d_min=1.1)
@@ -63,10 +67,13 @@ def test_cpp_drop_in_matches_oracle(tmp_path, sg):
     include[3] = False
     inp = tmp_path / "in.txt"
     write_input(w, str(inp), include)
-    print("case cpp driver", sg, flush=True)
-    out = subprocess.run([DRIVER, str(inp), "HIP"], capture_output=True, text=True, timeout=120)
+    print("case cpp driver", sg, mode, flush=True)
+    out = subprocess.run([DRIVER, str(inp), mode], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
-    assert "size check ok" in out.stderr
+    if mode == "per_hkl":
+        assert "order ok" in out.stderr and "rethrow ok" in out.stderr
+    else:
+        assert "size check ok" in out.stderr
     lines = out.stdout.strip().split("\n")
     m = len(w.hkl)
     f = np.array([complex(*map(float, l.split())) for l in lines[:m]])

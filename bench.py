@@ -25,12 +25,13 @@ FP64_PEAK = 78.6e12
 # HBM bytes per launch of the main kernels on this workload, from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # passes scaled by a plain-copy calibration (profiles/*_hbm_traffic.md, "bench-traffic" lines)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_main_p212121_5000at_1.2A_hbm_traffic.md")
+TRAFFIC_FILE_ROWS = os.path.join(ROOT, "profiles", "k_jac_rows_hbm_traffic.md")
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, path=TRAFFIC_FILE):
     """HBM bytes per launch recorded for `kernel` at this workload, or None."""
     try:
-        with open(TRAFFIC_FILE) as fh:
+        with open(path) as fh:
             for line in fh:
                 if line.startswith("bench-traffic ") and line.split()[1] == kernel:
                     return float(line.split()[2])
@@ -262,7 +263,8 @@ def bench_per_hkl(args):
         "roofline": {"bound": "hbm", "roof": "HBM writes of the rows (algorithmic bytes = 160 B per (atom, "
                                              "reflection) + 16 B of F per reflection)",
                      "achieved": big["hbm_GBps"], "peak": 8000.0, "unit": "GB/s",
-                     "frac": big["hbm_GBps"] / 8000.0, "traffic": None, "kernel": "k_jac<acentric>",
+                     "frac": big["hbm_GBps"] / 8000.0,
+                     "traffic": measured_traffic("k_jac<acentric>", TRAFFIC_FILE_ROWS), "kernel": "k_jac<acentric>",
                      "kernel_ms": big["kernel_ms"], "bytes_per_launch": big["bytes_written"]},
         "small_molecule": small,
         "large_block": big,

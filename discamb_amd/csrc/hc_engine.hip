@@ -1384,7 +1384,9 @@ template <bool CENTRIC, bool OTF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_jac(MainArgs a, JacArgs jb)
 {
     __shared__ __align__(16) unsigned char stage_lds[kWavesPerBlock * kStageBytesF64];
-    const int wib = (int)(threadIdx.x / kWave);
+    // wave-uniform (the compiler cannot see that threadIdx.x / 64 is), so that the reflection
+    // range and every record address derived from it stay in scalar registers
+    const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
     const int unit = blockIdx.x * kWavesPerBlock + wib;
     if (unit >= jb.n_units) return;
     const int wave = unit % a.n_waves;
@@ -2248,7 +2250,9 @@ static const void* jac_kernel(bool centric, bool otf)
     return otf ? (const void*)k_jac<false, true> : (const void*)k_jac<false, false>;
 }
 
-// Reflections per work unit: enough units for two rounds of the resident waves, at most 64.
+// Reflections per work unit: one staging block (8), or fewer when that would leave resident wave
+// slots idle.  Longer chunks lose more to the last, partly filled round of units than they save
+// in per-unit set-up (measured on 60- and 5000-atom structures).
 int Engine::jac_chunk(int n_rows)
 {
     if (tune.hkl_chunk > 0) return tune.hkl_chunk;
@@ -2261,7 +2265,7 @@ int Engine::jac_chunk(int n_rows)
         jac_resident_ = std::max(1, blocks) * kWavesPerBlock * prop.multiProcessorCount;
     }
     const long long work = (long long)n_rows * std::max(n_sorted_ / kWave, 1);
-    return (int)std::max(1LL, std::min(64LL, work / (2LL * jac_resident_)));
+    return (int)std::max(1LL, std::min((long long)kStage, work / jac_resident_));
 }
 
 void Engine::jacobian(int k0, int k1, double* f_dev, double* rows_dev)

@@ -90,6 +90,8 @@ SIGNATURES = {
     "dh_hc_jacobian_rows": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, c_dbl_p, c_dbl_p]),
     "dh_hc_jacobian_rows_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                                ctypes.c_void_p, ctypes.c_void_p]),
+    "dh_hc_form_factors": (ctypes.c_int, [ctypes.c_void_p, c_dbl_p, c_u8_p, ctypes.c_int, c_int_p, c_dbl_p]),
+    "dh_hc_form_factors_cart": (ctypes.c_int, [ctypes.c_void_p, c_dbl_p, c_u8_p, ctypes.c_int, c_dbl_p, c_dbl_p]),
     "dh_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     "dh_batch_destroy": (None, [ctypes.c_void_p]),
     "dh_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_int]),

@@ -271,6 +271,34 @@ class HansenCoppensStructureFactorCalculator:
         self.jacobianUpload(atoms, localCoordinateSystems, hkl, countAtomContribution, anomalous)
         return self.jacobianRows()
 
+    # -- form-factor route -------------------------------------------------------------------------
+    def _form_factors(self, fn, lcs, h, dtype, countAtom):
+        n = self.n_atoms
+        h = np.asarray(h)
+        single = h.ndim == 1
+        h = np.ascontiguousarray(h.reshape(-1, 3), dtype=dtype)
+        m = h.shape[0]
+        lcs_a = np.ascontiguousarray(np.broadcast_to(np.eye(3), (n, 3, 3)) if lcs is None else
+                                     np.asarray(lcs, dtype=np.float64).reshape(n, 3, 3))
+        inc = None if countAtom is None else np.ascontiguousarray(np.asarray(countAtom, dtype=bool).astype(np.uint8))
+        f = np.zeros((max(m, 1), max(n, 1)), dtype=np.complex128)
+        _lib.check(fn(self._h, lcs_a.ctypes.data_as(_lib.c_dbl_p), None if inc is None else _lib.ptr(inc, ctypes.c_ubyte),
+                      m, h.ctypes.data_as(_lib.c_int_p if dtype == np.int32 else _lib.c_dbl_p),
+                      f.ctypes.data_as(_lib.c_dbl_p)))
+        f = f[:m, :n]
+        return f[0] if single else f
+
+    def calculateFormFactors(self, localCoordinateSystems, hkl, countAtomContribution=None):
+        """Atomic form factors (HansenCoppensStructureFactorCalculator.h:171-175): hkl (3,) gives
+        f (n_atoms,) complex as the reference; hkl (m, 3) gives (m, n_atoms) for m reflections."""
+        return self._form_factors(self._lib.dh_hc_form_factors, localCoordinateSystems, hkl, np.int32,
+                                  countAtomContribution)
+
+    def calculateFormFactorsCart(self, localCoordinateSystems, h_cart, countAtomContribution=None):
+        """As calculateFormFactors for Cartesian reciprocal vectors (1/Angstrom)."""
+        return self._form_factors(self._lib.dh_hc_form_factors_cart, localCoordinateSystems, h_cart, np.float64,
+                                  countAtomContribution)
+
     def calculateStructureFactors(self, atoms: AtomSet, localCoordinateSystems, hkl, countAtomContribution=None,
                                   anomalous=None):
         nh = np.asarray(hkl).reshape(-1, 3).shape[0]

@@ -679,6 +679,49 @@ int dh_hc_jacobian_rows_dev(dh_calculator* calc, int k0, int k1, void* f_dev, vo
     });
 }
 
+static void form_factors(Calculator* c, const double* lcs, const unsigned char* include, int n_h, const double* h_cart,
+                         double* f)
+{
+    if (n_h < 0) throw Error(DH_E_ARGUMENT, "negative number of reflections");
+    if (n_h > 0 && (!h_cart || !f)) throw Error(DH_E_ARGUMENT, "null argument");
+    const int n = (int)c->atom_type.size();
+    if (n_h == 0 || n == 0) return;
+    std::vector<double> frames(9 * (size_t)n);
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < 9; j++) frames[9 * (size_t)i + j] = lcs ? lcs[9 * (size_t)i + j] : (j % 4 == 0 ? 1.0 : 0.0);
+    c->engine->form_factors(n, c->atom_type.data(), frames.data(), n_h, h_cart, f);
+    if (include)   // countAtom: excluded atoms get 0 (HansenCoppens_SF_Engine.cpp:467-471)
+        for (int k = 0; k < n_h; k++)
+            for (int i = 0; i < n; i++)
+                if (!include[i]) f[2 * ((size_t)k * n + i)] = f[2 * ((size_t)k * n + i) + 1] = 0.0;
+}
+
+int dh_hc_form_factors(dh_calculator* calc, const double* lcs, const unsigned char* include, int n_h, const int* hkl,
+                       double* f)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        if (n_h > 0 && !hkl) throw Error(DH_E_ARGUMENT, "null Miller indices");
+        Calculator* c = calc->c;
+        // h_cart = h a* + k b* + l c* (ReciprocalLatticeUnitCell::fractionalToCartesian)
+        std::vector<double> hc(3 * (size_t)std::max(n_h, 0));
+        for (int k = 0; k < n_h; k++)
+            for (int j = 0; j < 3; j++)
+                hc[3 * (size_t)k + j] = hkl[3 * k] * c->cell.c2f[j] + hkl[3 * k + 1] * c->cell.c2f[3 + j] +
+                                        hkl[3 * k + 2] * c->cell.c2f[6 + j];
+        form_factors(c, lcs, include, n_h, hc.data(), f);
+    });
+}
+
+int dh_hc_form_factors_cart(dh_calculator* calc, const double* lcs, const unsigned char* include, int n_h,
+                            const double* h_cart, double* f)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        form_factors(calc->c, lcs, include, n_h, h_cart, f);
+    });
+}
+
 int dh_batch_create(dh_calculator* const* calcs, int n, dh_batch** out)
 {
     return guarded([&] {

@@ -1425,6 +1425,123 @@ __global__ void k_reduce_f_rows(const double2* __restrict__ fpart, const double*
     f[2 * i + 1] = im * mult[k];
 }
 
+// ------------------------------------------------------------------ form-factor route
+//
+// Atomic form factors f_a(h) = f_core(|h|) + Pval f_val(|h| / kappa) + f_def(h) for a list of
+// Cartesian h (HansenCoppensStructureFactorCalculator::calculateFormFactors(Cart),
+// .cpp:1274-1333, over HansenCoppens_SF_Engine::calculateFormFactors, HansenCoppens_SF_Engine.cpp:
+// 383-495).  The spherical terms come, as in the reference, from cubic splines on a 0.001 A^-1
+// grid of 10 000 nodes (setDataForCpuEngineCallsForSingleHkl, .cpp:1244-1272); f_def uses the
+// closed-form g-functions.  No anomalous term.
+
+static constexpr double kSplineStep = 0.001;
+static constexpr int kSplineNodes = 10000;
+
+// Grid values: spline s < n_wfn is the core density of wavefunction s, spline n_wfn + t the
+// valence term Pval_t f_val(h / kappa_t) of type t, both at h = i * step.
+__global__ void k_spline_values(const TypeRec* __restrict__ types, const WfnRec* __restrict__ wfns,
+                                const TermRec* __restrict__ core, const TermRec* __restrict__ val, int n_wfn,
+                                int n_types, double* __restrict__ out)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (n_wfn + n_types) * kSplineNodes) return;
+    const int sp = idx / kSplineNodes, i = idx - sp * kSplineNodes;
+    const double h = i * kSplineStep;
+    double v = 0.0;
+    if (sp < n_wfn) {
+        const WfnRec w = wfns[sp];
+        for (int q = 0; q < w.core_n; q++) {
+            const TermRec tc = core[w.core_off + q];
+            v += tc.c * g_function(0, tc.p + 2, h, tc.e);
+        }
+        v *= 12.566370614359172953850;
+    } else {
+        const TypeRec& ty = types[sp - n_wfn];
+        const WfnRec w = wfns[ty.wfn];
+        const double hk = h / ty.kappa;
+        for (int q = 0; q < w.val_n; q++) {
+            const TermRec tv = val[w.val_off + q];
+            v += tv.c * g_function(0, tv.p + 2, hk, tv.e);
+        }
+        v *= 12.566370614359172953850 * ty.pval;
+    }
+    out[idx] = v;
+}
+
+// NaturalCubicSpline::evaluate (NaturalCubicSpline.cpp): zero outside the grid and on its last
+// node; c holds 4 coefficients per segment.
+__device__ __forceinline__ double spline_at(const double* __restrict__ c, double x)
+{
+    const double u = x / kSplineStep;
+    if (u < 0.0) return 0.0;
+    const int i = (int)u;
+    if (i >= kSplineNodes) return 0.0;
+    const double t = (x - i * kSplineStep) / kSplineStep;
+    const double* q = c + 4 * (size_t)i;
+    return q[0] + t * q[1] + t * t * q[2] + t * t * t * q[3];
+}
+
+// Per (h, type): [S = core + valence spline, G_0..G_4 (4 pi i^l-sign N_l g_l(n_l + 2, |h| / kappa',
+// zeta))]; per h: the 25 harmonics of h/|h| (zero for |h| < 1e-10).
+__global__ void k_ff_prep(const double* __restrict__ hc, int n_h, const TypeRec* __restrict__ types, int n_types,
+                          int n_wfn, const double* __restrict__ spl, double* __restrict__ rec,
+                          double* __restrict__ hrec)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_h * n_types) return;
+    const int k = idx / n_types, t = idx - k * n_types;
+    const double h0 = hc[3 * k], h1 = hc[3 * k + 1], h2 = hc[3 * k + 2];
+    const double hl = sqrt(h0 * h0 + h1 * h1 + h2 * h2);
+    if (t == 0) {
+        double y[25];
+        if (hl < 1e-10) {
+            for (int i = 0; i < 25; i++) y[i] = 0.0;
+        } else {
+            ylm_density<double>(h0 / hl, h1 / hl, h2 / hl, y);
+        }
+        for (int i = 0; i < 25; i++) hrec[(size_t)k * 26 + i] = y[i];
+        hrec[(size_t)k * 26 + 25] = hl;
+    }
+    const TypeRec& ty = types[t];
+    double r[kRadRec] = {0, 0, 0, 0, 0, 0, 0, 0};
+    r[0] = spline_at(spl + (size_t)ty.wfn * kSplineNodes * 4, hl) +
+           spline_at(spl + (size_t)(n_wfn + t) * kSplineNodes * 4, hl);
+    const double hp = hl / ty.kappa_p;
+    for (int l = 0; l <= ty.lp; l++) r[1 + l] = ty.gscale[l] * g_function(l, ty.n[l] + 2, hp, ty.zeta);
+    for (int i = 0; i < kRadRec; i++) rec[((size_t)k * n_types + t) * kRadRec + i] = r[i];
+}
+
+// lane = atom (caller order), work unit = (atom wave, chunk of h); out[(k - k0) * n_atoms + atom]
+__global__ __launch_bounds__(256) void k_ff(const AtomRec* __restrict__ atoms, int n_waves, int n_atoms,
+                                            const double* __restrict__ hrec, const double* __restrict__ rec,
+                                            int n_types, int k0, int k1, int chunk, int n_units,
+                                            double* __restrict__ out)
+{
+    const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int unit = blockIdx.x * kWavesPerBlock + wib;
+    if (unit >= n_units) return;
+    const int wave = unit % n_waves;
+    const int ka = k0 + (unit / n_waves) * chunk, kb = min(ka + chunk, k1);
+    const int slot = wave * kWave + (int)(threadIdx.x & (kWave - 1));
+    const AtomRec& at = atoms[slot];
+    double P[25];
+#pragma unroll
+    for (int j = 0; j < 25; j++) P[j] = at.plm[j];
+    const int t = at.type, orig = at.orig;
+    const unsigned* h32 = cmem(reinterpret_cast<const unsigned*>(hrec));
+    for (int k = ka; k < kb; k++) {
+        double y[25];
+#pragma unroll
+        for (int j = 0; j < 25; j++) y[j] = ld_f64(h32 + (size_t)k * 52, j);
+        double A[5];
+        contract<4, double>(P, y, A);
+        const double* rr = rec + ((size_t)k * n_types + t) * kRadRec;
+        const double fr = rr[0] + rr[1] * A[0] + rr[3] * A[2] + rr[5] * A[4];
+        const double fi = rr[2] * A[1] + rr[4] * A[3];
+        if (orig >= 0) put_row(out + ((size_t)(k - k0) * n_atoms + orig) * 2, fr, fi);
+    }
+}
+
 // F(h) = mult(h) * sum_waves partial  (centric at origin: 2 Re, 0).  Loads are issued eight at a
 // time and added in wave order.
 __device__ __forceinline__ void reduce_f_item(int k, const double2* __restrict__ fpart,
@@ -1667,6 +1784,8 @@ void Engine::set_types(const std::vector<AtomType>& types)
         HIP_CHECK(hipMemcpyAsync(d_type_.p, tr.data(), tr.size() * sizeof(TypeRec), hipMemcpyHostToDevice, stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
     plan_tables();
+    jac_prepped_ = false;      // radial records depend on the type parameters
+    splines_ready_ = false;    // the valence splines carry Pval and kappa
 }
 
 // Node counts of the per-wavefunction form-factor tables: core at s = |h|, valence at
@@ -2320,6 +2439,117 @@ void Engine::jacobian(int k0, int k1, double* f_dev, double* rows_dev)
     if (timing) {
         HIP_CHECK(hipEventSynchronize(ev1_));
         HIP_CHECK(hipEventElapsedTime(&last_main_ms, ev0_, ev1_));
+    }
+}
+
+// ------------------------------------------------------------------ form-factor route (host)
+
+// NaturalCubicSpline::set (NaturalCubicSpline.cpp) on the grid values of every spline: the
+// tridiagonal system of the derivatives x_i (rows 1 4 1, first and last rows with 4 on the
+// diagonal as in the reference), solved by forward elimination and back substitution, then
+// segment coefficients [y_i, x_i, 3(y_{i+1} - y_i) - 2 x_i - x_{i+1}, 2(y_i - y_{i+1}) + x_i + x_{i+1}];
+// the last node's segment stays zero.
+static void spline_coefficients(const double* y, int n, double* coeff)
+{
+    std::vector<double> b(n, 4.0), d(n), x(n);
+    d[0] = 3.0 * (y[1] - y[0]);
+    for (int i = 1; i < n - 1; i++) d[i] = 3.0 * (y[i + 1] - y[i - 1]);
+    d[n - 1] = 3.0 * (y[n - 1] - y[n - 2]);
+    for (int i = 1; i < n; i++) {
+        const double w = 1.0 / b[i - 1];
+        b[i] -= w;
+        d[i] -= w * d[i - 1];
+    }
+    x[n - 1] = d[n - 1] / b[n - 1];
+    for (int i = n - 2; i >= 0; i--) x[i] = (d[i] - x[i + 1]) / b[i];
+    for (int i = 0; i < n - 1; i++) {
+        double* c = coeff + 4 * (size_t)i;
+        c[0] = y[i];
+        c[1] = x[i];
+        c[2] = 3.0 * (y[i + 1] - y[i]) - 2.0 * x[i] - x[i + 1];
+        c[3] = 2.0 * (y[i] - y[i + 1]) + x[i] + x[i + 1];
+    }
+    for (int j = 0; j < 4; j++) coeff[4 * (size_t)(n - 1) + j] = 0.0;
+}
+
+void Engine::build_splines()
+{
+    const int n_spl = model_.n_wfn + model_.n_types;
+    const size_t nv = (size_t)n_spl * kSplineNodes;
+    DeviceBuffer vals;
+    vals.ensure(nv * sizeof(double));
+    const TypeRec* types = d_type_.as<TypeRec>();
+    const WfnRec* wfns = reinterpret_cast<const WfnRec*>(static_cast<const char*>(d_type_.p) +
+                                                         model_.n_types * sizeof(TypeRec));
+    hipLaunchKernelGGL(k_spline_values, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, stream_, types, wfns,
+                       d_core_.as<TermRec>(), d_val_.as<TermRec>(), model_.n_wfn, model_.n_types, vals.as<double>());
+    HIP_CHECK(hipGetLastError());
+    std::vector<double> y(nv), c(nv * 4);
+    HIP_CHECK(hipMemcpyAsync(y.data(), vals.p, nv * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    for (int sp = 0; sp < n_spl; sp++)
+        spline_coefficients(y.data() + (size_t)sp * kSplineNodes, kSplineNodes, c.data() + (size_t)sp * kSplineNodes * 4);
+    d_spline_.ensure(c.size() * sizeof(double));
+    HIP_CHECK(hipMemcpyAsync(d_spline_.p, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    splines_ready_ = true;
+}
+
+void Engine::form_factors(int n_atoms, const int* type, const double* lcs, int n_h, const double* h_cart,
+                          double* f)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    if (n_atoms <= 0 || n_h <= 0) return;
+    if (model_.n_types <= 0) throw Error(1, "form factors need atom types");
+    for (int i = 0; i < n_atoms; i++)
+        if (type[i] < 0 || type[i] >= model_.n_types) throw Error(1, "atom type index out of range");
+    if (!splines_ready_) build_splines();
+    // atoms in caller order, padded to whole waves; P_lm to the global frame by k_atom_prep
+    const int n_slots = (n_atoms + kWave - 1) / kWave * kWave, n_waves = n_slots / kWave;
+    std::vector<AtomIn> ain(n_slots);
+    for (int s = 0; s < n_slots; s++) {
+        AtomIn& r = ain[s];
+        std::memset(&r, 0, sizeof(r));
+        if (s < n_atoms) {
+            r.orig = s;
+            r.type = type[s];
+            for (int j = 0; j < 9; j++) r.lcs[j] = lcs[9 * (size_t)s + j];
+        } else {
+            r.orig = -1;
+            r.lcs[0] = r.lcs[4] = r.lcs[8] = 1.0;
+        }
+    }
+    d_ff_ain_.ensure(ain.size() * sizeof(AtomIn));
+    d_ff_atoms_.ensure(ain.size() * sizeof(AtomRec));
+    HIP_CHECK(hipMemcpyAsync(d_ff_ain_.p, ain.data(), ain.size() * sizeof(AtomIn), hipMemcpyHostToDevice, stream_));
+    hipLaunchKernelGGL(k_atom_prep, dim3(n_slots), dim3(64), 0, stream_, d_ff_ain_.as<AtomIn>(), d_type_.as<TypeRec>(),
+                       d_ff_atoms_.as<AtomRec>(), n_slots);
+    HIP_CHECK(hipGetLastError());
+    // blocks of reflections bounded by a 64 MB output buffer
+    const size_t row = (size_t)n_atoms * 2 * sizeof(double);
+    const int B = (int)std::max<size_t>(1, std::min<size_t>((size_t)n_h, (64u << 20) / row));
+    const int nt = model_.n_types;
+    d_ff_h_.ensure((size_t)B * 3 * sizeof(double));
+    d_ff_rec_.ensure((size_t)B * nt * kRadRec * sizeof(double));
+    d_ff_hrec_.ensure((size_t)B * 26 * sizeof(double));
+    d_ff_out_.ensure((size_t)B * row);
+    for (int b0 = 0; b0 < n_h; b0 += B) {
+        const int n = std::min(B, n_h - b0);
+        HIP_CHECK(hipMemcpyAsync(d_ff_h_.p, h_cart + 3 * (size_t)b0, (size_t)n * 3 * sizeof(double),
+                                 hipMemcpyHostToDevice, stream_));
+        hipLaunchKernelGGL(k_ff_prep, dim3((n * nt + 255) / 256), dim3(256), 0, stream_, d_ff_h_.as<double>(), n,
+                           d_type_.as<TypeRec>(), nt, model_.n_wfn, d_spline_.as<double>(), d_ff_rec_.as<double>(),
+                           d_ff_hrec_.as<double>());
+        HIP_CHECK(hipGetLastError());
+        const int chunk = 8;
+        const int n_units = n_waves * ((n + chunk - 1) / chunk);
+        hipLaunchKernelGGL(k_ff, dim3((n_units + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256), 0, stream_,
+                           d_ff_atoms_.as<AtomRec>(), n_waves, n_atoms, d_ff_hrec_.as<double>(), d_ff_rec_.as<double>(),
+                           nt, 0, n, chunk, n_units, d_ff_out_.as<double>());
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(f + (size_t)b0 * n_atoms * 2, d_ff_out_.p, (size_t)n * row, hipMemcpyDeviceToHost,
+                                 stream_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
     }
 }
 

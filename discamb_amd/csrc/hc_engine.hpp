@@ -84,6 +84,9 @@ public:
     void set_jacobian_conventions(const JacConv& conv);
     void jacobian(int k0, int k1, double* f_dev, double* rows_dev);
     int n_atoms() const { return n_atoms_; }
+    // Form-factor route: f[k][atom] (complex, caller atom order) for n_h Cartesian h vectors and
+    // the given local frames and atom types; synchronous, host pointers.
+    void form_factors(int n_atoms, const int* type, const double* lcs, int n_h, const double* h_cart, double* f);
     // Run all kernels on the uploaded inputs (asynchronous on the engine stream).  mp adds the
     // multipole-parameter adjoint (implies derivs).
     void run(Precision prec, bool derivs, bool mp = false);
@@ -125,6 +128,7 @@ private:
     void launch_main(Precision prec, bool derivs, bool mp, int chunk);
     void reduce(bool derivs, bool mp, int chunk);
     void fill(DeviceBuffer& b, size_t n, double v);
+    void build_splines();
     void fill(double* p, size_t n, double v);
     int jac_chunk(int n_rows);
     int resident_waves(bool mixed, bool derivs, bool mp);
@@ -174,6 +178,9 @@ private:
     int n_included_ = 0;
     int jac_resident_ = 0;
     DeviceBuffer d_jatoms_, d_jconv_, d_jrec_;
+    // form-factor route
+    bool splines_ready_ = false;
+    DeviceBuffer d_spline_, d_ff_ain_, d_ff_atoms_, d_ff_h_, d_ff_rec_, d_ff_hrec_, d_ff_out_;
     double h_max_ = 0.0;            // largest |h| of the uploaded reflections
     int n_tab_nodes_ = 0;           // form-factor table nodes over all wavefunctions
 };

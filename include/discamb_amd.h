@@ -193,6 +193,21 @@ int dh_hc_jacobian_rows(dh_calculator* calc, int k0, int k1, double* f, double* 
  * later work; returns without waiting on the host. */
 int dh_hc_jacobian_rows_dev(dh_calculator* calc, int k0, int k1, void* f_dev, void* rows_dev, void* stream);
 
+/* ---- Form-factor route: atomic form factors of every atom for a list of reflections,
+ *   HansenCoppensStructureFactorCalculator::calculateFormFactors(lcs, const Vector3i& hkl, f, countAtom)
+ *   ::calculateFormFactorsCart(lcs, const Vector3d& h_cart, f, countAtom)
+ *       (HansenCoppensStructureFactorCalculator.h:171-181, .cpp:1274-1333; engine
+ *        HansenCoppens_SF_Engine::calculateFormFactors, HansenCoppens_SF_Engine.cpp:383-495)
+ * which the reference computes one reflection per call.  f = f_core(|h|) + Pval f_val(|h|/kappa)
+ * (cubic splines on a 0.001 A^-1 grid, as the reference) + the deformation-valence term in the
+ * atom's local frame; no anomalous term; excluded atoms 0.  lcs: 9 n_atoms (NULL = identity
+ * frames), include: n_atoms or NULL (all).  Output f: n_h * n_atoms complex values (re, im),
+ * reflection-major, atoms in model order.  Synchronous, host pointers, any calculation mode. */
+int dh_hc_form_factors(dh_calculator* calc, const double* lcs, const unsigned char* include, int n_h, const int* hkl,
+                       double* f);
+int dh_hc_form_factors_cart(dh_calculator* calc, const double* lcs, const unsigned char* include, int n_h,
+                            const double* h_cart, double* f);
+
 /* ---- Batches of structures (no reference counterpart: the reference computes one structure per
  * call; a refinement of many small structures calls it once per structure).
  * A batch borrows calculators created on one device; they must outlive it.  Inputs are uploaded

@@ -451,6 +451,44 @@ public:
         detail::check(rc);
     }
 
+    // Form-factor route (HansenCoppensStructureFactorCalculator.h:171-181): atomic form factors of
+    // every atom for one reflection (Miller indices or Cartesian 1/Angstrom), as the reference, and
+    // for many reflections in one call (f[k * n_atoms + atom]).
+    void calculateFormFactors(const std::vector<Matrix3d>& localCoordinateSystems, const Vector3i& hkl,
+                              std::vector<std::complex<double>>& f, const std::vector<bool>& countAtom) const
+    {
+        calculateFormFactors(localCoordinateSystems, std::vector<Vector3i>(1, hkl), f, countAtom);
+    }
+
+    void calculateFormFactorsCart(const std::vector<Matrix3d>& localCoordinateSystems, const Vector3d& h_cart,
+                                  std::vector<std::complex<double>>& f, const std::vector<bool>& countAtom) const
+    {
+        calculateFormFactorsCart(localCoordinateSystems, std::vector<Vector3d>(1, h_cart), f, countAtom);
+    }
+
+    void calculateFormFactors(const std::vector<Matrix3d>& localCoordinateSystems, const std::vector<Vector3i>& hkl,
+                              std::vector<std::complex<double>>& f, const std::vector<bool>& countAtom) const
+    {
+        std::vector<int> h = detail::flat_hkl(hkl);
+        form_factors(localCoordinateSystems, countAtom, hkl.size(), f,
+                     [&](const double* l, const unsigned char* inc, double* out) {
+                         return dh_hc_form_factors(mCalc, l, inc, (int)hkl.size(), h.data(), out);
+                     });
+    }
+
+    void calculateFormFactorsCart(const std::vector<Matrix3d>& localCoordinateSystems,
+                                  const std::vector<Vector3d>& h_cart, std::vector<std::complex<double>>& f,
+                                  const std::vector<bool>& countAtom) const
+    {
+        std::vector<double> h(3 * h_cart.size() + 3);
+        for (size_t k = 0; k < h_cart.size(); k++)
+            for (int j = 0; j < 3; j++) h[3 * k + j] = h_cart[k][j];
+        form_factors(localCoordinateSystems, countAtom, h_cart.size(), f,
+                     [&](const double* l, const unsigned char* inc, double* out) {
+                         return dh_hc_form_factors_cart(mCalc, l, inc, (int)h_cart.size(), h.data(), out);
+                     });
+    }
+
     void calculateStructureFactors(const std::vector<AtomInCrystal>& atoms,
                                    const std::vector<Matrix3d>& localCoordinateSystems,
                                    const std::vector<Vector3i>& hkl, std::vector<std::complex<double>>& f,
@@ -472,6 +510,23 @@ public:
     }
 
 private:
+    template <typename Call>
+    void form_factors(const std::vector<Matrix3d>& lcs, const std::vector<bool>& countAtom, size_t n_h,
+                      std::vector<std::complex<double>>& f, Call call) const
+    {
+        const size_t n = mAtomCount;
+        if (lcs.size() != n || countAtom.size() != n) throw Exception(DH_E_ARGUMENT, "inconsistent number of atoms");
+        std::vector<double> l(9 * n + 9), out(2 * n * n_h + 2);
+        std::vector<unsigned char> inc(n + 1);
+        for (size_t i = 0; i < n; i++) {
+            for (int j = 0; j < 9; j++) l[9 * i + j] = lcs[i].m[j];
+            inc[i] = countAtom[i] ? 1 : 0;
+        }
+        detail::check(call(l.data(), inc.data(), out.data()));
+        f.resize(n * n_h);
+        for (size_t i = 0; i < n * n_h; i++) f[i] = {out[2 * i], out[2 * i + 1]};
+    }
+
     dh_calculator* mCalc = nullptr;
     size_t mAtomCount;
 };

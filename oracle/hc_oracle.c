@@ -508,6 +508,66 @@ int hc_oracle_per_hkl(
     return 0;
 }
 
+/* Spherical Slater density form factor sum_k c_k 4 pi g(0, p_k + 2, h, e_k) at m values of h
+ * (scatteringSphericalDensity, SlaterTypeOrbitalScattering.cpp:8-21). */
+void hc_oracle_sph_density(const double *c, const double *e, const int *p, int n_terms, const double *h, int m,
+                           double *out)
+{
+    for (int i = 0; i < m; i++) out[i] = sph_density_sf(c, e, p, n_terms, h[i]);
+}
+
+/* Deformation-valence part of the atomic form factors of the form-factor route
+ * (HansenCoppens_SF_Engine::calculateFormFactors, HansenCoppens_SF_Engine.cpp:383-495): for
+ * every Cartesian h and atom, 4 pi sum_l i^l N_l g_l(h / kappa') sum_m P_lm d_lm(L^T h/|h|),
+ * zero at |h| < 1e-10 and for excluded atoms.  out: [k][atom] complex. */
+void hc_oracle_def_val_ff(const oracle_model *mdl, int n_atoms, const int *atom_wfn, const int *atom_type,
+                          const double *lcs, const unsigned char *include, int n_h, const double *hv, double *out)
+{
+    int *type_wfn = (int *)calloc(mdl->n_types > 0 ? mdl->n_types : 1, sizeof(int));
+    int *type_max_l = (int *)calloc(mdl->n_types > 0 ? mdl->n_types : 1, sizeof(int));
+    for (int a = 0; a < n_atoms; a++) type_wfn[atom_type[a]] = atom_wfn[a];
+    for (int t = 0; t < mdl->n_types; t++) {
+        int lp = -1;
+        for (int l = 0; l < mdl->plm_nl[t]; l++)
+            for (int i = l * l; i < (l + 1) * (l + 1); i++)
+                if (mdl->plm[25 * t + i] != 0.0) lp = l;
+        int lw = mdl->dv_nl[type_wfn[t]] - 1;
+        int ml = lw < 4 ? lw : 4;
+        type_max_l[t] = ml < lp ? ml : lp;
+    }
+    double *gl = (double *)malloc(sizeof(double) * 5 * (mdl->n_types + 1));
+    for (int k = 0; k < n_h; k++) {
+        const double *h = hv + 3 * k;
+        double hl = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+        for (int t = 0; t < mdl->n_types; t++) {
+            int w = type_wfn[t];
+            for (int l = 0; l < 5; l++) {
+                gl[5 * t + l] = 0.0;
+                if (l < mdl->dv_nl[w]) {
+                    int nn = mdl->dv_n[5 * w + l];
+                    double zeta = mdl->dv_zeta[w];
+                    double norm = pow(zeta, nn + 3);
+                    for (int q = 2; q <= nn + 2; q++) norm /= q;
+                    gl[5 * t + l] = norm * g_fun(l, nn + 2, hl / mdl->kappa_p[t], zeta);
+                }
+            }
+        }
+        for (int a = 0; a < n_atoms; a++) {
+            double *o = out + ((size_t)k * n_atoms + a) * 2;
+            o[0] = o[1] = 0.0;
+            if (!include[a] || hl < 1e-10) continue;
+            const double *L = lcs + 9 * a;
+            double u[3] = {h[0] / hl, h[1] / hl, h[2] / hl}, ul[3];
+            for (int j = 0; j < 3; j++) ul[j] = L[j] * u[0] + L[3 + j] * u[1] + L[6 + j] * u[2];
+            int t = atom_type[a];
+            def_val(mdl->plm + 25 * t, type_max_l[t], gl + 5 * t, ul, &o[0], &o[1]);
+        }
+    }
+    free(gl);
+    free(type_wfn);
+    free(type_max_l);
+}
+
 /* test hooks: the oracle's own g-function and harmonics */
 double hc_oracle_g(int l, int n, double h, double z) { return g_fun(l, n, h, z); }
 void hc_oracle_ylm(const double *u, double *d25) { ylm_density(u, d25); }

@@ -314,3 +314,92 @@ def per_hkl(crystal, params, atoms, lcs, hkl, include=None, n_threads=1, anomalo
         aniso = adp_n == 6
         R[:, 3:9, aniso] = np.einsum("ji,mjn->min", J, R[:, 3:9, aniso])
     return F, R
+
+
+# ------------------------------------------------------------------------------ form-factor route
+SPLINE_STEP, SPLINE_NODES = 0.001, 10000     # HansenCoppensStructureFactorCalculator.cpp:1250-1251
+
+
+def spline_coefficients(y):
+    """NaturalCubicSpline::set (src/MathUtilities/NaturalCubicSpline.cpp) restated: derivatives x
+    from the tridiagonal system with rows (1, 4, 1) - 4 on the diagonal in the first and last row
+    too - and right-hand side 3 (y_{i+1} - y_{i-1}) (one-sided at the ends); segment i is
+    y_i + t x_i + t^2 (3 (y_{i+1} - y_i) - 2 x_i - x_{i+1}) + t^3 (2 (y_i - y_{i+1}) + x_i + x_{i+1});
+    the last node's segment is zero."""
+    from scipy.linalg import solve_banded
+    y = np.asarray(y, dtype=np.float64)
+    n = len(y)
+    d = np.empty(n)
+    d[0] = 3 * (y[1] - y[0])
+    d[1:-1] = 3 * (y[2:] - y[:-2])
+    d[-1] = 3 * (y[-1] - y[-2])
+    ab = np.zeros((3, n))
+    ab[0, 1:] = 1.0
+    ab[1, :] = 4.0
+    ab[2, :-1] = 1.0
+    x = solve_banded((1, 1), ab, d)
+    c = np.zeros((n, 4))
+    c[:-1, 0] = y[:-1]
+    c[:-1, 1] = x[:-1]
+    c[:-1, 2] = 3 * (y[1:] - y[:-1]) - 2 * x[:-1] - x[1:]
+    c[:-1, 3] = 2 * (y[:-1] - y[1:]) + x[:-1] + x[1:]
+    return c
+
+
+def spline_eval(c, x, step=SPLINE_STEP):
+    """NaturalCubicSpline::evaluate: zero below the grid and from the last node on."""
+    x = np.asarray(x, dtype=np.float64)
+    u = x / step
+    i = np.floor(u).astype(np.int64)
+    ok = (u >= 0) & (i < len(c))
+    i = np.clip(i, 0, len(c) - 1)
+    t = (x - i * step) / step
+    q = c[i]
+    v = q[..., 0] + t * q[..., 1] + t * t * q[..., 2] + t * t * t * q[..., 3]
+    return np.where(ok, v, 0.0)
+
+
+def _sph_density(parts, h):
+    c, e, p = parts
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    out = np.zeros(len(h))
+    if len(c):
+        cc, ee = np.ascontiguousarray(c, dtype=np.float64), np.ascontiguousarray(e, dtype=np.float64)
+        pp = np.ascontiguousarray(p, dtype=np.int32)
+        P = ctypes.c_void_p
+        lib().hc_oracle_sph_density(P(cc.ctypes.data), P(ee.ctypes.data), P(pp.ctypes.data), ctypes.c_int(len(cc)),
+                                    P(h.ctypes.data), ctypes.c_int(len(h)), P(out.ctypes.data))
+    return out
+
+
+def form_factors(crystal, params, lcs, h_cart, include=None):
+    """Atomic form factors of the form-factor route (HansenCoppensStructureFactorCalculator::
+    calculateFormFactorsCart, .cpp:1305-1333, with the splines of setDataForCpuEngineCallsForSingleHkl,
+    .cpp:1244-1272, and HansenCoppens_SF_Engine::calculateFormFactors, HansenCoppens_SF_Engine.cpp:
+    383-495): f[k, a] = spline_core[wfn(type)](|h|) + spline_val[type](|h|) + f_def, zero for
+    excluded atoms.  h_cart (m, 3) in 1/Angstrom."""
+    h = np.ascontiguousarray(np.asarray(h_cart, dtype=np.float64).reshape(-1, 3))
+    m, n = h.shape[0], len(params.atom_to_type_map)
+    wfn, tp = params.wfn_parameters, params.type_parameters
+    type_wfn = np.zeros(len(tp), dtype=int)
+    for a in range(n):
+        type_wfn[params.atom_to_type_map[a]] = params.atom_to_wfn_map[a]
+    grid = np.arange(SPLINE_NODES) * SPLINE_STEP
+    core = [spline_coefficients(_sph_density(sphere_density(w, False), grid)) for w in wfn]
+    val = [spline_coefficients(tp[t].p_val * _sph_density(sphere_density(wfn[type_wfn[t]], True),
+                                                            grid / tp[t].kappa_spherical_valence))
+           for t in range(len(tp))]
+    hl = np.sqrt((h * h).sum(1))
+    S = np.stack([spline_eval(core[type_wfn[t]], hl) + spline_eval(val[t], hl) for t in range(len(tp))], 1)
+    # deformation valence (the pinned def_val of the C restatement)
+    x = _engine_inputs(crystal, params, crystal.atoms, lcs, np.zeros((1, 3), dtype=np.int64), include,
+                       "reference", None)
+    dv = np.zeros((m, max(n, 1), 2))
+    P = ctypes.c_void_p
+    lib().hc_oracle_def_val_ff(ctypes.byref(x["model"]), ctypes.c_int(n), P(x["a_wfn"].ctypes.data),
+                               P(x["a_type"].ctypes.data), P(x["lcs"].ctypes.data), P(x["include"].ctypes.data),
+                               ctypes.c_int(m), P(h.ctypes.data), P(dv.ctypes.data))
+    f = S[:, params.atom_to_type_map] + dv[:, :n, 0] + 1j * dv[:, :n, 1]
+    inc = np.ones(n, dtype=bool) if include is None else np.asarray(include, dtype=bool)
+    f[:, ~inc] = 0.0
+    return f

@@ -7,6 +7,7 @@
 #include "discamb/Scattering/SlaterTypeOrbitalScattering.h"
 #include "discamb/MathUtilities/real_spherical_harmonics.h"
 #include "discamb/MathUtilities/SphConverter.h"
+#include "discamb/MathUtilities/NaturalCubicSpline.h"
 #include "discamb/HC_Model/HC_WfnData.h"
 #include "discamb/HC_Model/ClementiRoettiData.h"
 #include "discamb/HC_Model/DeformationValenceParameters.h"
@@ -24,6 +25,13 @@ using namespace discamb;
 namespace spc = structural_parameters_convention;
 
 extern "C" {
+
+// NaturalCubicSpline(values, step, start) evaluated at m points   (NaturalCubicSpline.cpp)
+void ref_spline(const double* values, int n, double step, double start, const double* x, int m, double* out)
+{
+    NaturalCubicSpline s(std::vector<double>(values, values + n), step, start);
+    for (int i = 0; i < m; i++) out[i] = s(x[i]);
+}
 
 // sto_scattering::gFunction<l>(n, h, Z)   (SlaterTypeOrbitalScattering.h:51-183)
 double ref_g_function(int l, int n, double h, double z) { return sto_scattering::gFunction(l, n, h, z); }

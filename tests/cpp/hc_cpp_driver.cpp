@@ -120,6 +120,20 @@ int main(int argc, char** argv)
     }
     try {
         HansenCoppensStructureFactorCalculator calc(crystal, p);
+        if (std::string(argv[2]) == "form_factors") {
+            // one reflection per call (the reference's signature) and all in one call
+            std::cout << std::setprecision(17);
+            std::vector<std::complex<double>> all, one;
+            calc.calculateFormFactors(lcs, hkl, all, include);
+            bool same = true;
+            for (size_t k = 0; k < hkl.size(); k++) {
+                calc.calculateFormFactors(lcs, hkl[k], one, include);
+                for (size_t a = 0; a < one.size(); a++) same = same && one[a] == all[k * one.size() + a];
+            }
+            for (auto& v : all) std::cout << v.real() << " " << v.imag() << "\n";
+            std::cerr << (same ? "single ok\n" : "single differs\n");
+            return same ? 0 : 1;
+        }
         if (std::string(argv[2]) == "per_hkl") {
             GradientCollector gc;
             gc.dtdf = &dtdf;

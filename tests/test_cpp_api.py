@@ -90,3 +90,23 @@ def test_cpp_drop_in_matches_oracle(tmp_path, sg, mode):
     assert rel_err(np.array(dx), rx) < 1e-10
     assert rel_err(du, ru) < 1e-10
     assert rel_err(np.array(do), ro) < 1e-10
+
+
+def test_cpp_form_factors_match_oracle(tmp_path):
+    """calculateFormFactors through the C++ class: one reflection per call and the batched form."""
+    if not os.path.exists(DRIVER):
+        pytest.fail("tests/cpp/hc_cpp_driver not built (run __graft_entry__.build())")
+    w = synth.small_structure("P21/c", n_atoms=9, d_min=1.1)
+    include = np.ones(9, dtype=bool)
+    include[3] = False
+    inp = tmp_path / "in.txt"
+    write_input(w, str(inp), include)
+    print("case cpp driver form factors", flush=True)
+    out = subprocess.run([DRIVER, str(inp), "form_factors"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "single ok" in out.stderr
+    v = np.array([complex(*map(float, l.split())) for l in out.stdout.strip().split("\n")])
+    f = v.reshape(len(w.hkl), 9)
+    c2f = np.linalg.inv(hc_oracle.cell_f2c(w.crystal.cell))
+    ref = hc_oracle.form_factors(w.crystal, w.params, w.lcs, w.hkl.astype(np.float64) @ c2f, include=include)
+    assert rel_err(f, ref) < 1e-10

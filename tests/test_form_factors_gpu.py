@@ -1,0 +1,68 @@
+"""Form-factor route (calculateFormFactors / calculateFormFactorsCart) on the GPU against the oracle.
+
+Tolerance: normwise max|f - f_ref| / max|f_ref| <= 1e-10 (FP64; both sides use the reference's
+10 000-node cubic splines for the spherical terms, built from closed-form grid values).
+"""
+import numpy as np
+import pytest
+
+import hc_oracle
+from conftest import rel_err
+from discamb_amd import HansenCoppensStructureFactorCalculator, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("sg", ["P1", "P21/c", "C2/c", "P43212"])
+def test_form_factors_match_oracle(sg):
+    w = synth.small_structure(sg, n_atoms=10, d_min=0.8, seed=12)
+    c2f = np.linalg.inv(hc_oracle.cell_f2c(w.crystal.cell))
+    print("case form factors %s atoms=%d hkl=%d" % (sg, w.crystal.atoms.n, len(w.hkl)), flush=True)
+    calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+    f = calc.calculateFormFactors(w.lcs, w.hkl)
+    ref = hc_oracle.form_factors(w.crystal, w.params, w.lcs, w.hkl.astype(np.float64) @ c2f)
+    assert f.shape == (len(w.hkl), w.crystal.atoms.n)
+    assert rel_err(f, ref) < 1e-10
+    # one reflection per call, as the reference
+    f1 = calc.calculateFormFactors(w.lcs, w.hkl[7])
+    assert f1.shape == (w.crystal.atoms.n,) and np.array_equal(f1, f[7])
+
+
+def test_form_factors_cartesian_edges_mask_and_new_types():
+    w = synth.small_structure("P21/c", n_atoms=7, d_min=1.0, seed=4)
+    rng = np.random.default_rng(9)
+    h = rng.normal(size=(300, 3)) * 0.8
+    h[0] = 0.0                                   # (000): no deformation term
+    h[1] = [0.0, 0.0, 10.5]                      # beyond the spline grid: spherical terms 0
+    h[2] = [1e-4, 0.0, 0.0]                      # first spline segment
+    inc = np.ones(7, dtype=bool)
+    inc[3] = False
+    calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+    print("case form factors cartesian", flush=True)
+    f = calc.calculateFormFactorsCart(w.lcs, h, countAtomContribution=inc)
+    ref = hc_oracle.form_factors(w.crystal, w.params, w.lcs, h, include=inc)
+    assert rel_err(f, ref) < 1e-10
+    assert np.all(f[:, 3] == 0) and np.all(f[0].imag == 0)
+    # new type parameters rebuild the valence splines
+    tp = w.params.type_parameters
+    for t in tp:
+        t.kappa_spherical_valence *= 1.03
+        t.p_val *= 0.97
+    calc.setTypeParameters(np.array([t.plm25() for t in tp]), [t.p_val for t in tp],
+                           [t.kappa_spherical_valence for t in tp], [t.kappa_deformation_valence for t in tp])
+    f2 = calc.calculateFormFactorsCart(w.lcs, h, countAtomContribution=inc)
+    ref2 = hc_oracle.form_factors(w.crystal, w.params, w.lcs, h, include=inc)
+    assert rel_err(f2, ref2) < 1e-10 and rel_err(f2, f) > 1e-4
+
+
+def test_form_factors_large_blocks():
+    """5000 atoms x 2000 reflections: three 64 MB output blocks."""
+    w = synth.config1()
+    hkl = w.hkl[::33][:2000]
+    calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+    print("case form factors 5000 atoms x %d" % len(hkl), flush=True)
+    f = calc.calculateFormFactors(w.lcs, hkl)
+    sel = np.r_[0:40, 800:840, 1960:2000]
+    c2f = np.linalg.inv(hc_oracle.cell_f2c(w.crystal.cell))
+    ref = hc_oracle.form_factors(w.crystal, w.params, w.lcs, hkl[sel].astype(np.float64) @ c2f)
+    assert rel_err(f[sel], ref) < 1e-10

@@ -43,6 +43,8 @@ class DhModelDesc(ctypes.Structure):
         ("atom_to_wfn", c_int_p),
         ("atom_to_type", c_int_p),
         ("atom_multiplicity", c_dbl_p),
+        ("wfn_label", ctypes.POINTER(ctypes.c_char_p)),
+        ("wfn_atomic_number", c_int_p),
     ]
 
 
@@ -72,6 +74,7 @@ SIGNATURES = {
     "dh_hc_destroy": (None, [ctypes.c_void_p]),
     "dh_hc_set_calculation_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "dh_hc_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "dh_hc_set_electron_scattering": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "dh_hc_set_options": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint]),
     "dh_hc_sf_and_derivs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DhAtoms), ctypes.c_int, c_int_p, c_dbl_p,
                                            ctypes.c_uint, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p]),

@@ -92,6 +92,12 @@ def _model_desc(crystal: Crystal, params: HC_ModelParameters, d_xyz="fractional"
     d.atom_to_type = _lib.ptr(keep.arr(list(params.atom_to_type_map) + [0], np.int32), ctypes.c_int)
     d.atom_multiplicity = _lib.ptr(keep.arr(np.append(crystal.atoms.multiplicity, 0.0), np.float64),
                                    ctypes.c_double)
+    labels = (ctypes.c_char_p * (len(params.wfn_parameters) + 1))(
+        *[w.label.encode() for w in params.wfn_parameters], None)
+    keep.items.append(labels)
+    d.wfn_label = ctypes.cast(labels, ctypes.POINTER(ctypes.c_char_p))
+    d.wfn_atomic_number = _lib.ptr(keep.arr([int(w.atomic_number) for w in params.wfn_parameters] + [0], np.int32),
+                                   ctypes.c_int)
     return d, keep
 
 
@@ -122,6 +128,10 @@ class HansenCoppensStructureFactorCalculator:
     # -- reference API -----------------------------------------------------------------------
     def setCalculationMode(self, mode: str):
         _lib.check(self._lib.dh_hc_set_calculation_mode(self._h, mode.encode()))
+
+    def setElectronScattering(self, electronScattering: bool):
+        """Mott-Bethe electron structure factors for the summed path (needs HC_WfnType.atomic_number)."""
+        _lib.check(self._lib.dh_hc_set_electron_scattering(self._h, 1 if electronScattering else 0))
 
     def setTypeParameters(self, plm, p_val, kappa, kappa_prime):
         """Replace the atom-type parameters: plm (n_types, 25) in the l*l + l + m layout, p_val,

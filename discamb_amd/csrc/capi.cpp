@@ -51,6 +51,18 @@ struct Calculator {
     bool centric_origin = false;
     Precision prec = kFp64;
     std::unique_ptr<Engine> engine;
+    // independent-atom model ("CPU_IAM"): its own engine (Gaussian form factors, inversion
+    // partners explicit with f'' unconjugated, as HansenCoppens_SF_Engine::calculateSF_IAM)
+    bool iam = false;
+    std::unique_ptr<Engine> iam_engine;
+    std::vector<int> iam_atom_type;
+    std::vector<std::string> wfn_label;
+    std::vector<int> wfn_z;
+    std::vector<std::complex<double>> wfn_anom;
+    int device = 0;
+    bool electron = false;
+    Engine& eng() { return iam ? *iam_engine : *engine; }
+    bool eng_centric_origin() const { return !iam && centric_origin; }
     // state of the last upload
     int n_atoms = 0, n_hkl = 0;
     std::vector<int> adp_size;
@@ -73,7 +85,10 @@ struct Calculator {
     }
 };
 
-static std::vector<EngineOp> make_ops(const Cell& c, const SpaceGroupInfo& g, bool centric_shift)
+// explicit operations; with expand_centre the inversion partners (-R, -t + t_inv) are listed
+// too, their anomalous term conjugated (partner_anom = -1, Engine3) or not (+1, Engine v1)
+static std::vector<EngineOp> make_ops(const Cell& c, const SpaceGroupInfo& g, bool expand_centre,
+                                      double partner_anom = -1.0)
 {
     std::vector<EngineOp> ops;
     auto cart = [&](const int* rf, double* rc) {
@@ -99,9 +114,9 @@ static std::vector<EngineOp> make_ops(const Cell& c, const SpaceGroupInfo& g, bo
         e.anom_sign = 1.0;
         ops.push_back(e);
     }
-    if (centric_shift) {
-        // inversion partners (-R, -t + t_inv); the reference's term2 conjugates the whole
-        // atomic form factor (HansenCoppens_SF_Engine3.cpp:2811-2813), hence anom_sign -1
+    if (expand_centre) {
+        // inversion partners (-R, -t + t_inv); Engine3's term2 conjugates the whole atomic form
+        // factor (HansenCoppens_SF_Engine3.cpp:2811-2813), hence anom_sign -1 there
         size_t n = ops.size();
         for (size_t s = 0; s < n; s++) {
             EngineOp e = ops[s];
@@ -110,7 +125,7 @@ static std::vector<EngineOp> make_ops(const Cell& c, const SpaceGroupInfo& g, bo
                 e.rc[i] = -e.rc[i];
             }
             for (int i = 0; i < 3; i++) e.tf[i] = -ops[s].tf[i] + g.inv_t[i];
-            e.anom_sign = -1.0;
+            e.anom_sign = partner_anom;
             ops.push_back(e);
         }
     }
@@ -178,7 +193,24 @@ static Calculator* create(const dh_model_desc* d, int device)
     bool centric_shift = c->sg.centro && !zero_t;
     std::vector<EngineOp> eops = make_ops(c->cell, c->sg, centric_shift);
     c->engine.reset(new Engine(device, c->model, d->cell, eops, c->centric_origin));
+    c->device = device;
+    for (int w = 0; w < d->n_wfn; w++) {
+        c->wfn_label.push_back(d->wfn_label && d->wfn_label[w] ? std::string(d->wfn_label[w]) : std::string());
+        c->wfn_z.push_back(d->wfn_atomic_number ? d->wfn_atomic_number[w] : 0);
+        c->wfn_anom.push_back(wfn[w].anomalous);
+    }
     return c.release();
+}
+
+static void ensure_iam_engine(Calculator* c)
+{
+    if (c->iam_engine) return;
+    for (const std::string& l : c->wfn_label)
+        if (l.empty()) throw Error(DH_E_MODEL, "CPU_IAM needs the wavefunction type labels (element symbols)");
+    EngineModel m = build_iam_model(c->wfn_label, c->wfn_anom, c->atom_wfn, c->iam_atom_type);
+    const bool centro = c->sg.centro;
+    std::vector<EngineOp> ops = make_ops(c->cell, c->sg, centro, +1.0);
+    c->iam_engine.reset(new Engine(c->device, m, c->cell.par, ops, false));
 }
 
 // rows: inputs of the per-reflection derivative rows (no dT/dF; reflections kept in caller order)
@@ -205,7 +237,7 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
     ea.weight.resize(n);
     ea.anom.resize(2 * n);
     ea.lcs.resize(9 * n);
-    ea.type = c->atom_type;
+    ea.type = c->iam ? c->iam_atom_type : c->atom_type;
     ea.include.resize(n);
     c->adp_size.resize(n);
     c->occ.resize(n);
@@ -248,7 +280,7 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
     for (int k = 0; k < n_hkl; k++) {
         double m = centering_multiplier(c->sg.centering, c->sg.obverse, hkl + 3 * k);
         double re = rows ? 0.0 : dtdf[2 * k] * m, im = rows ? 0.0 : -dtdf[2 * k + 1] * m;
-        if (c->centric_origin) {
+        if (c->eng_centric_origin()) {
             // F = 2 Re(sum) and derivatives 2 Re(c) Re(.)  (Engine3.cpp:1924-1958)
             re *= 2.0;
             im = 0.0;
@@ -257,7 +289,7 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
         dd[2 * k + 1] = im;
         dd[2 * (size_t)n_hkl + k] = m;
     }
-    c->engine->upload(ea, n_hkl, hkl, dd.data(), rows);
+    c->eng().upload(ea, n_hkl, hkl, dd.data(), rows);
     c->n_atoms = n;
     c->n_hkl = n_hkl;
     if (!rows) return;
@@ -277,7 +309,7 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
     }
     jc.mult_w = c->mult_w;
     jc.adp_kind = c->adp_size;
-    c->engine->set_jacobian_conventions(jc);
+    c->eng().set_jacobian_conventions(jc);
     c->rows_uploaded = true;
 }
 
@@ -301,12 +333,12 @@ static void rows_to_host(Calculator* c, int k0, int k1, double* f, double* rows)
     check_rows(c, k0, k1);
     const int B = rows_per_block(c);
     const size_t row = (size_t)DH_JAC_PLANES * c->n_atoms * 2;
-    hipStream_t st = c->engine->stream();
+    hipStream_t st = c->eng().stream();
     for (int b0 = k0; b0 < k1; b0 += B) {
         const int b1 = std::min(k1, b0 + B), n = b1 - b0;
         c->rows_dev[0].ensure((size_t)n * row * sizeof(double));
         c->f_dev[0].ensure((size_t)n * 2 * sizeof(double));
-        c->engine->jacobian(b0, b1, c->f_dev[0].as<double>(), c->rows_dev[0].as<double>());
+        c->eng().jacobian(b0, b1, c->f_dev[0].as<double>(), c->rows_dev[0].as<double>());
         if (f)
             HIP_CHECK(hipMemcpyAsync(f + 2 * (size_t)(b0 - k0), c->f_dev[0].p, (size_t)n * 2 * sizeof(double),
                                      hipMemcpyDeviceToHost, st));
@@ -326,7 +358,7 @@ static void per_hkl(Calculator* c, dh_per_hkl_fn fn, void* user)
     if (n_hkl == 0) return;
     const int B = rows_per_block(c);
     const size_t row = (size_t)DH_JAC_PLANES * c->n_atoms * 2;
-    hipStream_t st = c->engine->stream();
+    hipStream_t st = c->eng().stream();
     for (int i = 0; i < 2; i++) {
         if (!c->rows_ev[i]) HIP_CHECK(hipEventCreateWithFlags(&c->rows_ev[i], hipEventDisableTiming));
         c->rows_dev[i].ensure((size_t)B * row * sizeof(double));
@@ -338,7 +370,7 @@ static void per_hkl(Calculator* c, dh_per_hkl_fn fn, void* user)
     for (int b = 0; b <= nb; b++) {
         if (b < nb) {
             const int q = b & 1, b0 = b * B, b1 = std::min(n_hkl, b0 + B), n = b1 - b0;
-            c->engine->jacobian(b0, b1, c->f_dev[q].as<double>(), c->rows_dev[q].as<double>());
+            c->eng().jacobian(b0, b1, c->f_dev[q].as<double>(), c->rows_dev[q].as<double>());
             HIP_CHECK(hipMemcpyAsync(c->f_host[q].p, c->f_dev[q].p, (size_t)n * 2 * sizeof(double),
                                      hipMemcpyDeviceToHost, st));
             HIP_CHECK(hipMemcpyAsync(c->rows_host[q].p, c->rows_dev[q].p, (size_t)n * row * sizeof(double),
@@ -363,9 +395,9 @@ static void fetch(Calculator* c, double* f, double* d_xyz, double* d_adp, double
 {
     const int n = c->n_atoms;
     const bool want_d = c->last_selector != 0;
-    const int nd = c->engine->nd();
+    const int nd = c->eng().nd();
     c->d10.assign((size_t)std::max(n, 1) * nd, 0.0);
-    c->engine->download(f, want_d ? c->d10.data() : nullptr);
+    c->eng().download(f, want_d ? c->d10.data() : nullptr);
     c->have_mp = false;
     if (!want_d) {
         if (d_xyz) std::memset(d_xyz, 0, sizeof(double) * 3 * n);
@@ -425,6 +457,34 @@ static void fetch(Calculator* c, double* f, double* d_xyz, double* d_adp, double
                     for (int j = l * l; j < (l + 1) * (l + 1); j++) c->d_plm[25 * t + j] = 0.0;
         c->have_mp = true;
     }
+}
+
+static void check_selector(Calculator* c, unsigned selector)
+{
+    if (c->iam && (selector & DH_D_MULTIPOLE))
+        throw Error(DH_E_ARGUMENT, "multipole-parameter derivatives need a multipole model (not CPU_IAM)");
+}
+
+// Electron scattering per atom type: Z of the type's wavefunction and the electron form factor at
+// zero angle from the "electron-IT" table (HansenCoppens_SF_Engine3::electronScatteringAt000,
+// HansenCoppens_SF_Engine3.cpp:952-970).
+static void set_electron(Calculator* c, bool on)
+{
+    std::vector<double> z_f0;
+    if (on) {
+        for (int t = 0; t < c->model.n_types; t++) {
+            const int w = c->model.type_wfn[t];
+            const int z = w < (int)c->wfn_z.size() ? c->wfn_z[w] : 0;
+            if (z <= 0) throw Error(DH_E_MODEL, "electron scattering needs the atomic number of every wavefunction type");
+            int n = 0;
+            const NGaussEntry* e = ngauss_lookup(element_symbol(z), "electron-IT", &n);
+            if (!e) throw Error(DH_E_MODEL, "no electron form factor for element " + element_symbol(z));
+            z_f0.push_back((double)z);
+            z_f0.push_back(ngauss_value(*e, n, 0.0));
+        }
+    }
+    c->engine->set_electron(on, z_f0);
+    c->electron = on;
 }
 
 }  // namespace discamb_amd
@@ -492,9 +552,30 @@ int dh_hc_set_calculation_mode(dh_calculator* calc, const char* mode)
     return guarded([&] {
         if (!calc || !mode) throw Error(DH_E_ARGUMENT, "null argument");
         std::string m(mode);
-        if (m == "HIP" || m == "GPU" || m == "CPU") calc->c->prec = kFp64;
-        else if (m == "HIP_MIXED") calc->c->prec = kMixed;
-        else throw Error(DH_E_ARGUMENT, "unknown calculation mode for multipolar structure factor engine");
+        Calculator* c = calc->c;
+        if (m == "HIP" || m == "GPU" || m == "CPU" || m == "CPU_v4") {
+            c->prec = kFp64;
+            c->iam = false;
+        } else if (m == "HIP_MIXED") {
+            c->prec = kMixed;
+            c->iam = false;
+        } else if (m == "CPU_IAM" || m == "HIP_IAM" || m == "HIP_IAM_MIXED") {
+            ensure_iam_engine(c);
+            c->iam_engine->tune = c->engine->tune;
+            c->iam_engine->timing = c->engine->timing;
+            c->prec = m == "HIP_IAM_MIXED" ? kMixed : kFp64;
+            c->iam = true;
+        } else
+            throw Error(DH_E_ARGUMENT, "unknown calculation mode for multipolar structure factor engine");
+        c->rows_uploaded = false;
+    });
+}
+
+int dh_hc_set_electron_scattering(dh_calculator* calc, int on)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        set_electron(calc->c, on != 0);
     });
 }
 
@@ -503,6 +584,7 @@ int dh_hc_set_tuning(dh_calculator* calc, int hkl_chunk)
     return guarded([&] {
         if (!calc || hkl_chunk < 0) throw Error(DH_E_ARGUMENT, "invalid tuning");
         calc->c->engine->tune.hkl_chunk = hkl_chunk;
+        if (calc->c->iam_engine) calc->c->iam_engine->tune.hkl_chunk = hkl_chunk;
     });
 }
 
@@ -512,8 +594,11 @@ int dh_hc_set_options(dh_calculator* calc, unsigned flags)
         const unsigned all = DH_OPT_GENERAL_ROTATIONS | DH_OPT_RADIAL_RECORDS | DH_OPT_RADIAL_TABLES;
         if (!calc || (flags & ~all) || ((flags & DH_OPT_RADIAL_RECORDS) && (flags & DH_OPT_RADIAL_TABLES)))
             throw Error(DH_E_ARGUMENT, "invalid options");
-        calc->c->engine->tune.general_path = (flags & DH_OPT_GENERAL_ROTATIONS) != 0;
-        calc->c->engine->tune.radial_mode = (flags & DH_OPT_RADIAL_RECORDS) ? 1 : (flags & DH_OPT_RADIAL_TABLES) ? 2 : 0;
+        for (Engine* e : {calc->c->engine.get(), calc->c->iam_engine.get()}) {
+            if (!e) continue;
+            e->tune.general_path = (flags & DH_OPT_GENERAL_ROTATIONS) != 0;
+            e->tune.radial_mode = (flags & DH_OPT_RADIAL_RECORDS) ? 1 : (flags & DH_OPT_RADIAL_TABLES) ? 2 : 0;
+        }
     });
 }
 
@@ -530,7 +615,8 @@ int dh_hc_run(dh_calculator* calc, unsigned selector)
     return guarded([&] {
         if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
         calc->c->last_selector = selector & (DH_D_ALL | DH_D_MULTIPOLE);
-        calc->c->engine->run(calc->c->prec, calc->c->last_selector != 0, (selector & DH_D_MULTIPOLE) != 0);
+        check_selector(calc->c, selector);
+        calc->c->eng().run(calc->c->prec, calc->c->last_selector != 0, (selector & DH_D_MULTIPOLE) != 0);
     });
 }
 
@@ -550,7 +636,8 @@ int dh_hc_sf_and_derivs(dh_calculator* calc, const dh_atoms* atoms, int n_hkl, c
         if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
         upload(calc->c, atoms, n_hkl, hkl, dtarget_df);
         calc->c->last_selector = selector & (DH_D_ALL | DH_D_MULTIPOLE);
-        calc->c->engine->run(calc->c->prec, calc->c->last_selector != 0, (selector & DH_D_MULTIPOLE) != 0);
+        check_selector(calc->c, selector);
+        calc->c->eng().run(calc->c->prec, calc->c->last_selector != 0, (selector & DH_D_MULTIPOLE) != 0);
         fetch(calc->c, f, d_xyz, d_adp, d_occ);
     });
 }
@@ -559,7 +646,8 @@ int dh_hc_autotune(dh_calculator* calc, unsigned selector, int* chunk)
 {
     return guarded([&] {
         if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
-        int c = calc->c->engine->autotune(calc->c->prec, (selector & (DH_D_ALL | DH_D_MULTIPOLE)) != 0,
+        check_selector(calc->c, selector);
+        int c = calc->c->eng().autotune(calc->c->prec, (selector & (DH_D_ALL | DH_D_MULTIPOLE)) != 0,
                                           (selector & DH_D_MULTIPOLE) != 0);
         if (chunk) *chunk = c;
     });
@@ -605,6 +693,7 @@ int dh_hc_set_timing(dh_calculator* calc, int on)
     return guarded([&] {
         if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
         calc->c->engine->timing = on != 0;
+        if (calc->c->iam_engine) calc->c->iam_engine->timing = on != 0;
     });
 }
 
@@ -612,7 +701,7 @@ int dh_hc_last_timing(dh_calculator* calc, double* ms, int* gx, int* gy, int* ch
 {
     return guarded([&] {
         if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
-        Engine& e = *calc->c->engine;
+        Engine& e = calc->c->eng();
         if (ms) *ms = e.last_main_ms;
         if (gx) *gx = e.last_grid_x;
         if (gy) *gy = e.last_grid_y;
@@ -624,7 +713,7 @@ int dh_hc_engine_info(dh_calculator* calc, int* n_ops, int* order, int* centro, 
 {
     return guarded([&] {
         if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
-        if (n_ops) *n_ops = calc->c->engine->n_ops();
+        if (n_ops) *n_ops = calc->c->eng().n_ops();
         if (order) *order = calc->c->sg.order;
         if (centro) *centro = calc->c->sg.centro ? 1 : 0;
         if (centering) *centering = calc->c->sg.centering;
@@ -666,14 +755,14 @@ int dh_hc_jacobian_rows_dev(dh_calculator* calc, int k0, int k1, void* f_dev, vo
         check_rows(c, k0, k1);
         if (k1 == k0) return;
         if (!f_dev || !rows_dev) throw Error(DH_E_ARGUMENT, "null output pointer");
-        hipStream_t caller = static_cast<hipStream_t>(stream), st = c->engine->stream();
+        hipStream_t caller = static_cast<hipStream_t>(stream), st = c->eng().stream();
         if (!c->ev_in) HIP_CHECK(hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
         if (!c->ev_out) HIP_CHECK(hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming));
         // the engine stream starts after the caller's earlier work and the caller's stream resumes
         // after the rows are written
         HIP_CHECK(hipEventRecord(c->ev_in, caller));
         HIP_CHECK(hipStreamWaitEvent(st, c->ev_in, 0));
-        c->engine->jacobian(k0, k1, static_cast<double*>(f_dev), static_cast<double*>(rows_dev));
+        c->eng().jacobian(k0, k1, static_cast<double*>(f_dev), static_cast<double*>(rows_dev));
         HIP_CHECK(hipEventRecord(c->ev_out, st));
         HIP_CHECK(hipStreamWaitEvent(caller, c->ev_out, 0));
     });
@@ -689,6 +778,9 @@ static void form_factors(Calculator* c, const double* lcs, const unsigned char* 
     std::vector<double> frames(9 * (size_t)n);
     for (int i = 0; i < n; i++)
         for (int j = 0; j < 9; j++) frames[9 * (size_t)i + j] = lcs ? lcs[9 * (size_t)i + j] : (j % 4 == 0 ? 1.0 : 0.0);
+    if (c->iam)   // HansenCoppensStructureFactorCalculator.cpp:1246-1247
+        throw Error(DH_E_ARGUMENT, "using functionality available only for CPU calcluations of multipole model form "
+                                   "factors");
     c->engine->form_factors(n, c->atom_type.data(), frames.data(), n_h, h_cart, f);
     if (include)   // countAtom: excluded atoms get 0 (HansenCoppens_SF_Engine.cpp:467-471)
         for (int k = 0; k < n_h; k++)
@@ -731,6 +823,8 @@ int dh_batch_create(dh_calculator* const* calcs, int n, dh_batch** out)
         std::vector<Engine*> eng;
         for (int i = 0; i < n; i++) {
             if (!calcs[i]) throw Error(DH_E_ARGUMENT, "null calculator in batch");
+            if (calcs[i]->c->iam || calcs[i]->c->electron)
+                throw Error(DH_E_ARGUMENT, "batches take multipole-model X-ray calculators (not CPU_IAM or electron)");
             b->calcs.push_back(calcs[i]->c);
             eng.push_back(calcs[i]->c->engine.get());
         }
@@ -744,8 +838,11 @@ void dh_batch_destroy(dh_batch* batch) { delete batch; }
 static Precision batch_precision(dh_batch* b)
 {
     const Precision p = b->calcs[0]->prec;
-    for (Calculator* c : b->calcs)
+    for (Calculator* c : b->calcs) {
         if (c->prec != p) throw Error(DH_E_ARGUMENT, "calculators of a batch must share the calculation mode");
+        if (c->iam || c->electron)
+            throw Error(DH_E_ARGUMENT, "batches take multipole-model X-ray calculators (not CPU_IAM or electron)");
+    }
     return p;
 }
 

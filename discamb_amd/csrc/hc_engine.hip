@@ -249,15 +249,23 @@ __device__ __forceinline__ void tab_build_node(int j, int lane, const TabRec* __
     if (c < 2) {
         const TermRec* terms = c == 0 ? core + wf.core_off : val + wf.val_off;
         const int nt = c == 0 ? wf.core_n : wf.val_n;
+        double fg = 0.0, dgs = 0.0;
         for (int i = q; i < nt; i += 8) {
             const TermRec tr = terms[i];
-            double dg;
-            f += tr.c * g_function_dh(0, tr.p + 2, sv, tr.e, &dg);
-            d += tr.c * dg;
+            if (tr.p < 0) {
+                // independent-atom model: Gaussian term a exp(-b s^2 / 4) of an N-Gaussian fit
+                const double g = tr.c * exp(-0.25 * tr.e * sv * sv);
+                fg += g;
+                dgs += g * (-0.5 * tr.e * sv);
+            } else {
+                double dg;
+                f += tr.c * g_function_dh(0, tr.p + 2, sv, tr.e, &dg);
+                d += tr.c * dg;
+            }
         }
         const double four_pi = 12.566370614359172953850;
-        f *= four_pi;
-        d *= four_pi;
+        f = f * four_pi + fg;
+        d = d * four_pi + dgs;
     } else if (c < 7 && q == 0) {
         const int l = c - 2;
         if (wf.n[l] >= 0) {
@@ -324,12 +332,17 @@ __device__ __forceinline__ void radial_otf(const R* __restrict__ tab, TabRec tr,
 // G_l = 4 pi (i^l sign) N_l g_l(n_l + 2, h/kappa', zeta), l <= lp.  With radx set (multipole
 // adjoint) f_val and its derivative are summed in closed form and the record
 // [f_val, dS/dkappa, dG_0..4/dkappa', 0] is written as well.
+// Electron scattering (Mott-Bethe, HansenCoppens_SF_Engine3.cpp:1387-1398): the X-ray form factor
+// without its anomalous term becomes 0.023934 (Z - f_x) / (h^2 / 4) and, at h = 0, the type's
+// electron form factor at zero angle; ez holds (Z, f_e(0)) per type.  The anomalous term, added in
+// the main kernels, is unchanged.
 __device__ __forceinline__ void radial_prep_item(int idx, const double* __restrict__ hlen,
                                                  const double* __restrict__ tab, const TabRec* __restrict__ tabs,
                                                  const TypeRec* __restrict__ types, const WfnRec* __restrict__ wfns,
                                                  const TermRec* __restrict__ val, int n_types, int n_hkl,
                                                  double* __restrict__ rad64, float* __restrict__ rad32,
-                                                 double* __restrict__ radx64, float* __restrict__ radx32)
+                                                 double* __restrict__ radx64, float* __restrict__ radx32,
+                                                 const double* __restrict__ ez)
 {
     int t = idx / n_hkl, k = idx - t * n_hkl;       // type-major: a wave shares one type
     const double four_pi = 12.566370614359172953850;
@@ -374,6 +387,17 @@ __device__ __forceinline__ void radial_prep_item(int idx, const double* __restri
         for (int l = 0; l <= ty.lp; l++)
             r[1 + l] = wt[0] * a[2 * l] + wt[1] * a[kTabNode + 2 * l] + wt[2] * a[2 * l + 1] +
                        wt[3] * a[kTabNode + 2 * l + 1];
+    }
+    if (ez) {
+        if (h < 1e-10) {
+            for (int i = 0; i < kRadRec; i++) r[i] = x[i] = 0.0;
+            r[0] = ez[2 * t + 1];
+        } else {
+            const double b = -0.023934 / (0.25 * h * h);
+            r[0] = -b * ez[2 * t] + b * r[0];
+            for (int i = 1; i < kRadRec; i++) r[i] *= b;
+            for (int i = 0; i < kRadRec; i++) x[i] *= b;
+        }
     }
     size_t base = ((size_t)k * n_types + t) * kRadRec;
     if (rad64)
@@ -468,11 +492,11 @@ __global__ void k_radial_prep(const double* __restrict__ hlen, const double* __r
                               const TabRec* __restrict__ tabs, const TypeRec* __restrict__ types,
                               const WfnRec* __restrict__ wfns, const TermRec* __restrict__ val, int n_types,
                               int n_hkl, double* __restrict__ rad64, float* __restrict__ rad32,
-                              double* __restrict__ radx64, float* __restrict__ radx32)
+                              double* __restrict__ radx64, float* __restrict__ radx32, const double* __restrict__ ez)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx < n_hkl * n_types)
-        radial_prep_item(idx, hlen, tab, tabs, types, wfns, val, n_types, n_hkl, rad64, rad32, radx64, radx32);
+        radial_prep_item(idx, hlen, tab, tabs, types, wfns, val, n_types, n_hkl, rad64, rad32, radx64, radx32, ez);
 }
 
 __global__ __launch_bounds__(64) void k_atom_prep(const AtomIn* __restrict__ in, const TypeRec* __restrict__ types,
@@ -2054,7 +2078,7 @@ void Engine::ensure_prep_buffers(Precision prec, bool mp)
     if (mp) d_radx_.ensure((size_t)n_hkl * n_types * kRadRec * (mixed ? sizeof(float) : sizeof(double)));
 }
 
-void Engine::run_prep(Precision prec, bool mp)
+void Engine::run_prep(Precision prec, bool mp, bool electron)
 {
     jac_prepped_ = false;
     ensure_prep_buffers(prec, mp);
@@ -2094,7 +2118,8 @@ void Engine::run_prep(Precision prec, bool mp)
                            model_.n_types, n_hkl,
                            mixed ? nullptr : d_radial_.as<double>(), mixed ? d_radial_.as<float>() : nullptr,
                            (mp && !mixed) ? d_radx_.as<double>() : nullptr,
-                           (mp && mixed) ? d_radx_.as<float>() : nullptr);
+                           (mp && mixed) ? d_radx_.as<float>() : nullptr,
+                           electron && electron_ ? d_ez_.as<double>() : nullptr);
         HIP_CHECK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_atom_prep, dim3(n_slots), dim3(64), 0, stream_, ain, types, d_atoms_.as<AtomRec>(),
@@ -2340,6 +2365,18 @@ void Engine::download(double* f, double* d10)
 
 // ------------------------------------------------------------------ per-reflection rows
 
+void Engine::set_electron(bool on, const std::vector<double>& z_f0)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    if (on) {
+        if ((int)z_f0.size() != 2 * model_.n_types) throw Error(1, "internal: electron data per type");
+        d_ez_.ensure(std::max<size_t>(z_f0.size(), 1) * sizeof(double));
+        HIP_CHECK(hipMemcpy(d_ez_.p, z_f0.data(), z_f0.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    electron_ = on;
+    jac_prepped_ = false;
+}
+
 void Engine::set_jacobian_conventions(const JacConv& conv)
 {
     HIP_CHECK(hipSetDevice(device_));
@@ -2402,7 +2439,7 @@ void Engine::jacobian(int k0, int k1, double* f_dev, double* rows_dev)
         return;
     }
     if (!jac_prepped_) {
-        run_prep(kFp64, false);
+        run_prep(kFp64, false, false);     // the per-hkl engine has no electron-scattering mode
         const int n_rec = n_hkl_ * (int)ops_.size();
         d_jrec_.ensure((size_t)std::max(n_rec, 1) * kJacRec * sizeof(double));
         hipLaunchKernelGGL(k_jac_rec, dim3((n_rec + 255) / 256), dim3(256), 0, stream_, d_hklop_.as<double>(), n_rec,

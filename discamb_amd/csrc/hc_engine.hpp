@@ -84,6 +84,10 @@ public:
     void set_jacobian_conventions(const JacConv& conv);
     void jacobian(int k0, int k1, double* f_dev, double* rows_dev);
     int n_atoms() const { return n_atoms_; }
+    // Electron scattering (Mott-Bethe) for the summed path: z_f0 = (Z, f_e(0)) per atom type.
+    // The radial factors then come from per-(reflection, type) records.
+    void set_electron(bool on, const std::vector<double>& z_f0);
+    bool electron() const { return electron_; }
     // Form-factor route: f[k][atom] (complex, caller atom order) for n_h Cartesian h vectors and
     // the given local frames and atom types; synchronous, host pointers.
     void form_factors(int n_atoms, const int* type, const double* lcs, int n_h, const double* h_cart, double* f);
@@ -124,7 +128,7 @@ private:
     void ensure_prep_buffers(Precision prec, bool mp);
     void plan_tables();
     void main_args(Precision prec, bool derivs, bool mp, int chunk, MainArgs* a);
-    void run_prep(Precision prec, bool mp);
+    void run_prep(Precision prec, bool mp, bool electron = true);
     void launch_main(Precision prec, bool derivs, bool mp, int chunk);
     void reduce(bool derivs, bool mp, int chunk);
     void fill(DeviceBuffer& b, size_t n, double v);
@@ -144,7 +148,7 @@ private:
     // would then cost more to write and read than the lookups), never with the multipole adjoint
     bool use_otf(bool mp) const
     {
-        if (mp) return false;
+        if (mp || electron_) return false;
         if (tune.radial_mode != 0) return tune.radial_mode == 2;
         return (long long)model_.n_types * 8 > (long long)n_sorted_;
     }
@@ -178,6 +182,8 @@ private:
     int n_included_ = 0;
     int jac_resident_ = 0;
     DeviceBuffer d_jatoms_, d_jconv_, d_jrec_;
+    bool electron_ = false;
+    DeviceBuffer d_ez_;
     // form-factor route
     bool splines_ready_ = false;
     DeviceBuffer d_spline_, d_ff_ain_, d_ff_atoms_, d_ff_h_, d_ff_rec_, d_ff_hrec_, d_ff_out_;

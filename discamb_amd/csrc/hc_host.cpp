@@ -2,8 +2,11 @@
 #include "hc_internal.hpp"
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstring>
+
+#include "ngauss_tables.h"
 
 namespace discamb_amd {
 
@@ -292,6 +295,101 @@ EngineModel build_engine_model(const std::vector<WfnType>& wfn, const std::vecto
         int lw = std::min(4, m.dv_nl[m.type_wfn[t]] - 1);
         m.type_max_l[t] = std::min(lw, lp);
     }
+    return m;
+}
+
+// ---------------------------------------------------------------- N-Gaussian form factors
+
+template <size_t N>
+static const NGaussEntry* find_entry(const NGaussEntry (&t)[N], const std::string& label)
+{
+    for (size_t i = 0; i < N; i++)
+        if (label == t[i].label) return &t[i];
+    return nullptr;
+}
+
+// n_gaussian_form_factors_table::getFormFactor (NGaussianFormFactorsTable.cpp:3176-3217): exact
+// label match in the named table, else the alternative spelling of an ion ("O-" -> "O1-").
+const NGaussEntry* ngauss_lookup(const std::string& label, const std::string& table, int* n_gauss)
+{
+    const NGaussEntry* e = nullptr;
+    int n = 5;
+    if (table == "Waasmeier-Kirfel") e = find_entry(kNGauss_Waasmeier_Kirfel, label);
+    else if (table == "IT92") {
+        e = find_entry(kNGauss_X_Ray_It95, label);
+        n = 4;
+    } else if (table == "electron-IT") e = find_entry(kNGauss_Electron_IT, label);
+    else if (table == "electron-cctbx") e = find_entry(kNGauss_Electron_cctbx, label);
+    else throw Error(1, "unknown form-factor table " + table);
+    if (!e && label.size() > 1 && (label.back() == '-' || label.back() == '+') &&
+        !std::isdigit((unsigned char)label[label.size() - 2]))
+        return ngauss_lookup(label.substr(0, label.size() - 1) + "1" + label.back(), table, n_gauss);
+    if (n_gauss) *n_gauss = n;
+    return e;
+}
+
+double ngauss_value(const NGaussEntry& e, int n, double h)
+{
+    double r = 0.0;
+    const double x2 = h * h / 4;
+    for (int i = 0; i < n; i++) r += e.a[i] * std::exp(-e.b[i] * x2);
+    return r + e.c;
+}
+
+static const char* const kElements[] = {
+    "H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne", "Na", "Mg", "Al", "Si", "P", "S", "Cl", "Ar", "K", "Ca",
+    "Sc", "Ti", "V", "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Ga", "Ge", "As", "Se", "Br", "Kr", "Rb", "Sr", "Y",
+    "Zr", "Nb", "Mo", "Tc", "Ru", "Rh", "Pd", "Ag", "Cd", "In", "Sn", "Sb", "Te", "I", "Xe", "Cs", "Ba", "La", "Ce",
+    "Pr", "Nd", "Pm", "Sm", "Eu", "Gd", "Tb", "Dy", "Ho", "Er", "Tm", "Yb", "Lu", "Hf", "Ta", "W", "Re", "Os", "Ir",
+    "Pt", "Au", "Hg", "Tl", "Pb", "Bi", "Po", "At", "Rn", "Fr", "Ra", "Ac", "Th", "Pa", "U", "Np", "Pu", "Am", "Cm",
+    "Bk", "Cf"};
+
+std::string element_symbol(int z)
+{
+    if (z < 1 || z > (int)(sizeof(kElements) / sizeof(kElements[0])))
+        throw Error(2, "atomic number out of range: " + std::to_string(z));
+    return kElements[z - 1];
+}
+
+// The independent-atom model of HansenCoppens_SF_Engine::calculateSF_IAM
+// (HansenCoppens_SF_Engine.cpp:229-278): per wavefunction type the Waasmaier-Kirfel form factor
+// of its label plus its anomalous term, no valence or deformation terms.  Each wavefunction gets
+// its own atom type (so per-type radial records follow the element); the Gaussians are stored as
+// core "terms" with power -1.
+EngineModel build_iam_model(const std::vector<std::string>& labels, const std::vector<std::complex<double>>& anom,
+                            const std::vector<int>& atom_wfn, std::vector<int>& atom_type)
+{
+    EngineModel m;
+    m.n_wfn = (int)labels.size();
+    m.n_types = m.n_wfn;
+    m.core.resize(m.n_wfn);
+    m.valence.resize(m.n_wfn);
+    for (int w = 0; w < m.n_wfn; w++) {
+        int n = 0;
+        const NGaussEntry* e = ngauss_lookup(labels[w], "Waasmeier-Kirfel", &n);
+        if (!e)
+            throw Error(2, "request for Gaussian type atomic form factor parameter for unknown atom type: " + labels[w]);
+        for (int i = 0; i < n; i++) {
+            m.core[w].c.push_back(e->a[i]);
+            m.core[w].e.push_back(e->b[i]);
+            m.core[w].p.push_back(-1);
+        }
+        m.core[w].c.push_back(e->c);
+        m.core[w].e.push_back(0.0);
+        m.core[w].p.push_back(-1);
+        m.dv_zeta.push_back(1.0);
+        m.dv_nl.push_back(0);
+        for (int l = 0; l < 5; l++) m.dv_n.push_back(0);
+        m.anom.push_back(anom[w].real());
+        m.anom.push_back(anom[w].imag());
+    }
+    m.types.assign(m.n_types, AtomType());
+    m.type_wfn.resize(m.n_types);
+    for (int t = 0; t < m.n_types; t++) m.type_wfn[t] = t;
+    m.type_max_l.assign(m.n_types, -1);
+    atom_type = atom_wfn;
+    for (int w : atom_wfn)
+        if (w < 0 || w >= m.n_wfn) throw Error(1, "atom wavefunction index out of range");
     return m;
 }
 

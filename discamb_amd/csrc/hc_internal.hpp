@@ -98,7 +98,17 @@ struct EngineModel {
 };
 
 void build_sphere_density(const WfnType& w, bool valence, SlaterDensity& out);
+
+// N-Gaussian atomic form factors (generated tables, ngauss_tables.h)
+struct NGaussEntry;
+const NGaussEntry* ngauss_lookup(const std::string& label, const std::string& table, int* n_gauss);
+double ngauss_value(const NGaussEntry& e, int n, double h);
+std::string element_symbol(int z);
 EngineModel build_engine_model(const std::vector<WfnType>& wfn, const std::vector<AtomType>& types,
                                const std::vector<int>& atom_wfn, const std::vector<int>& atom_type);
+// Independent-atom model (Waasmaier-Kirfel Gaussians per wavefunction label); atom_type is set to
+// the atom-to-wavefunction map.
+EngineModel build_iam_model(const std::vector<std::string>& labels, const std::vector<std::complex<double>>& anom,
+                            const std::vector<int>& atom_wfn, std::vector<int>& atom_type);
 
 }  // namespace discamb_amd

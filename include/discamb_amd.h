@@ -88,6 +88,11 @@ typedef struct {
     const int* atom_to_wfn;
     const int* atom_to_type;
     const double* atom_multiplicity;  /* AtomInCrystal::multiplicity (|G| for a general position) */
+    /* optional (may be NULL): HC_WfnType::wfnAtomTypeName and ::atomic_number per wavefunction
+     * type; the labels select the Gaussian form factors of the CPU_IAM mode, the atomic numbers
+     * are needed for electron scattering */
+    const char* const* wfn_label;
+    const int* wfn_atomic_number;
 } dh_model_desc;
 
 /* Per-call atom parameters (vector<AtomInCrystal> + vector<Matrix3d> local frames). */
@@ -113,9 +118,19 @@ int dh_device_count(int* n);
 /* HansenCoppensStructureFactorCalculator(crystal, parameters) on GPU `device`. */
 int dh_hc_create(const dh_model_desc* model, int device, dh_calculator** out);
 void dh_hc_destroy(dh_calculator* calc);
-/* setCalculationMode: "HIP" (FP64 drop-in; "GPU" and "CPU" are accepted as aliases so reference
- * configuration strings keep working - the computation always runs on the GPU) or "HIP_MIXED". */
+/* setCalculationMode (HansenCoppensStructureFactorCalculator.cpp:350-372): "HIP" (FP64 drop-in;
+ * "GPU", "CPU" and "CPU_v4" are accepted as aliases so reference configuration strings keep
+ * working - the computation always runs on the GPU), "HIP_MIXED", and the independent-atom model
+ * "CPU_IAM" (alias "HIP_IAM"; "HIP_IAM_MIXED" for mixed precision): Waasmaier-Kirfel Gaussian form
+ * factors of the wavefunction labels plus f' + i f'', no multipoles, with f'' unconjugated for
+ * inversion partners as the reference's calculateSF_IAM (HansenCoppens_SF_Engine.cpp:229-278). */
 int dh_hc_set_calculation_mode(dh_calculator* calc, const char* mode);
+/* setElectronScattering (HansenCoppensStructureFactorCalculator.h:63, .cpp:510-514): multipole-model
+ * structure factors for electrons by the Mott-Bethe formula, f_e = 0.023934 (Z - f_x) / (|h|^2 / 4)
+ * + f' + i f'' and the "electron-IT" Gaussian value at h = 0 (HansenCoppens_SF_Engine3.cpp:1387-1398).
+ * Needs dh_model_desc.wfn_atomic_number.  As in the reference it affects the summed path only (not
+ * the per-reflection rows, the form factors or CPU_IAM); batches do not take it. */
+int dh_hc_set_electron_scattering(dh_calculator* calc, int on);
 /* Launch tuning: reflections per workgroup column (0 = automatic). */
 int dh_hc_set_tuning(dh_calculator* calc, int hkl_chunk);
 /* Kernel-path options (no reference counterpart; for tests and diagnosis).

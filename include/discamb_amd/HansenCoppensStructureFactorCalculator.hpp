@@ -361,6 +361,16 @@ public:
         d.atom_to_wfn = a2w.data();
         d.atom_to_type = a2t.data();
         d.atom_multiplicity = mult.data();
+        std::vector<const char*> labels;
+        std::vector<int> z;
+        for (const HC_WfnType& w : parameters.wfn_parameters) {
+            labels.push_back(w.wfnAtomTypeName.c_str());
+            z.push_back(w.atomic_number);
+        }
+        labels.push_back(nullptr);
+        z.push_back(0);
+        d.wfn_label = labels.data();
+        d.wfn_atomic_number = z.data();
         detail::check(dh_hc_create(&d, device, &mCalc));
     }
 
@@ -368,8 +378,13 @@ public:
     HansenCoppensStructureFactorCalculator(const HansenCoppensStructureFactorCalculator&) = delete;
     HansenCoppensStructureFactorCalculator& operator=(const HansenCoppensStructureFactorCalculator&) = delete;
 
-    // "HIP" (FP64 drop-in; "GPU" / "CPU" accepted as aliases) or "HIP_MIXED"
+    // "HIP" (FP64 drop-in; "GPU" / "CPU" / "CPU_v4" accepted as aliases), "HIP_MIXED", "CPU_IAM"
+    // (independent-atom model; "HIP_IAM_MIXED" in mixed precision)
     void setCalculationMode(const std::string& mode) { detail::check(dh_hc_set_calculation_mode(mCalc, mode.c_str())); }
+    void setElectronScattering(bool electronScattering)
+    {
+        detail::check(dh_hc_set_electron_scattering(mCalc, electronScattering ? 1 : 0));
+    }
 
     void calculateStructureFactorsAndDerivatives(const std::vector<AtomInCrystal>& atoms,
                                                  const std::vector<Matrix3d>& localCoordinateSystems,

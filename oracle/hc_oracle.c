@@ -50,6 +50,33 @@ typedef struct {
     const int *plm_nl;                   /* n_types: size of p_lm (number of l values) */
 } oracle_model;
 
+/* Optional replacements used by the independent-atom and electron-scattering restatements:
+ *   core override  - f_core of wavefunction w at reflection k taken from core_f[k * n_wfn + w]
+ *                    (the N-Gaussian form factors of HansenCoppens_SF_Engine::calculateSF_IAM);
+ *   electron       - Mott-Bethe transform of every term's atomic form factor
+ *                    (HansenCoppens_SF_Engine3.cpp:1387-1398) with per-atom Z and f_e(0). */
+static const double *g_core_f = NULL;
+static int g_core_nwfn = 0;
+static const double *g_el_z = NULL, *g_el_f0 = NULL;
+
+void hc_oracle_set_core_override(const double *core_f, int n_wfn) { g_core_f = core_f; g_core_nwfn = n_wfn; }
+void hc_oracle_set_electron(const double *z, const double *f0) { g_el_z = z; g_el_f0 = f0; }
+
+static void electron_ff(int a, double hl2, int h000, double anr, double ani, double *ar, double *ai)
+{
+    if (!g_el_z) return;
+    if (h000) {
+        *ar = g_el_f0[a] + anr;
+        *ai = ani;
+        return;
+    }
+    double s2 = 0.25 * hl2;
+    double nuc = 0.023934 * g_el_z[a] / s2;
+    double er = -0.023934 * (*ar - anr) / s2, ei = -0.023934 * (*ai - ani) / s2;
+    *ar = nuc + er + anr;
+    *ai = ei + ani;
+}
+
 /* int_0^inf r^n exp(-Z r) j_l(K r) dr, K = 2 pi h */
 static double g_fun(int l, int n, double h, double z)
 {
@@ -166,9 +193,10 @@ int hc_oracle_engine(
             int h000 = hl < 1e-10;
             double cr = dtdf[2 * k], ci = dtdf[2 * k + 1];
             for (int w = 0; w < mdl->n_wfn; w++)
-                fcore[w] = sph_density_sf(mdl->core_c + mdl->core_off[w], mdl->core_e + mdl->core_off[w],
-                                          mdl->core_p + mdl->core_off[w],
-                                          mdl->core_off[w + 1] - mdl->core_off[w], hl);
+                fcore[w] = g_core_f ? g_core_f[(size_t)k * g_core_nwfn + w]
+                                    : sph_density_sf(mdl->core_c + mdl->core_off[w], mdl->core_e + mdl->core_off[w],
+                                                     mdl->core_p + mdl->core_off[w],
+                                                     mdl->core_off[w + 1] - mdl->core_off[w], hl);
             for (int t = 0; t < mdl->n_types; t++) {
                 int w = type_wfn[t];
                 fval[t] = sph_density_sf(mdl->val_c + mdl->val_off[w], mdl->val_e + mdl->val_off[w],
@@ -222,6 +250,8 @@ int hc_oracle_engine(
                         def_val(mdl->plm + 25 * t, type_max_l[t], gl + 5 * t, ul, &dre, &dim);
                     }
                     double ar = f_re + dre, ai = f_im + dim;
+                    electron_ff(a, hl2, h000, anom_atom ? anom_atom[2 * a] : mdl->anom[2 * w],
+                                anom_atom ? anom_atom[2 * a + 1] : mdl->anom[2 * w + 1], &ar, &ai);
                     double m6[6] = {hs[0] * hs[0], hs[1] * hs[1], hs[2] * hs[2],
                                     2 * hs[0] * hs[1], 2 * hs[0] * hs[2], 2 * hs[1] * hs[2]};
                     double T = 1.0;
@@ -392,9 +422,10 @@ int hc_oracle_per_hkl(
             memset(row, 0, sizeof(double) * 10 * (size_t)n_atoms * 2);
             /* pre_atom_loop_sf_calc (HansenCoppens_SF_Engine.cpp:281-377) */
             for (int w = 0; w < mdl->n_wfn; w++)
-                fcore[w] = sph_density_sf(mdl->core_c + mdl->core_off[w], mdl->core_e + mdl->core_off[w],
-                                          mdl->core_p + mdl->core_off[w],
-                                          mdl->core_off[w + 1] - mdl->core_off[w], hl);
+                fcore[w] = g_core_f ? g_core_f[(size_t)k * g_core_nwfn + w]
+                                    : sph_density_sf(mdl->core_c + mdl->core_off[w], mdl->core_e + mdl->core_off[w],
+                                                     mdl->core_p + mdl->core_off[w],
+                                                     mdl->core_off[w + 1] - mdl->core_off[w], hl);
             for (int t = 0; t < mdl->n_types; t++) {
                 int w = type_wfn[t];
                 fval[t] = sph_density_sf(mdl->val_c + mdl->val_off[w], mdl->val_e + mdl->val_off[w],

@@ -403,3 +403,109 @@ def form_factors(crystal, params, lcs, h_cart, include=None):
     inc = np.ones(n, dtype=bool) if include is None else np.asarray(include, dtype=bool)
     f[:, ~inc] = 0.0
     return f
+
+
+# ------------------------------------------------------------------------------ N-Gaussian tables
+_NGAUSS = None
+
+
+def ngauss_tables():
+    """The N-Gaussian tables as data (discamb_amd/data/ngaussian_tables.json)."""
+    global _NGAUSS
+    if _NGAUSS is None:
+        import json
+        path = os.path.join(HERE, "..", "discamb_amd", "data", "ngaussian_tables.json")
+        _NGAUSS = {name: {e["label"]: e for e in entries} for name, entries in json.load(open(path))["tables"].items()}
+    return _NGAUSS
+
+
+def ngauss(label, table, h):
+    """n_gaussian_form_factors_table::getFormFactor(label, table).calculate_h(h)
+    (NGaussianFormFactorsTable.cpp:3176-3217, NGaussianFormFactor.cpp:115-127): exact label, else
+    the ion spelling with '1' before the sign; None if the table has neither."""
+    t = ngauss_tables()[table]
+    e = t.get(label)
+    if e is None and len(label) > 1 and label[-1] in "+-" and not label[-2].isdigit():
+        return ngauss(label[:-1] + "1" + label[-1], table, h)
+    if e is None:
+        return None
+    h = np.asarray(h, dtype=np.float64)
+    x2 = h * h / 4
+    return sum(a * np.exp(-b * x2) for a, b in zip(e["a"], e["b"])) + e["c"]
+
+
+ELEMENTS = ("H He Li Be B C N O F Ne Na Mg Al Si P S Cl Ar K Ca Sc Ti V Cr Mn Fe Co Ni Cu Zn Ga Ge As Se Br Kr "
+            "Rb Sr Y Zr Nb Mo Tc Ru Rh Pd Ag Cd In Sn Sb Te I Xe Cs Ba La Ce Pr Nd Pm Sm Eu Gd Tb Dy Ho Er Tm Yb "
+            "Lu Hf Ta W Re Os Ir Pt Au Hg Tl Pb Bi Po At Rn Fr Ra Ac Th Pa U Np Pu Am Cm Bk Cf").split()
+
+
+def _iam_params(params):
+    """The model the reference's calculateSF_IAM passes to its engine: one type per wavefunction
+    with no valence or multipole terms (HansenCoppens_SF_Engine.cpp:229-278)."""
+    from discamb_amd.model import HC_AtomTypeParameters, HC_ModelParameters
+    types = [HC_AtomTypeParameters([[0.0]], 0.0, 1.0, 1.0) for _ in params.wfn_parameters]
+    return HC_ModelParameters(params.wfn_parameters, types, list(params.atom_to_wfn_map),
+                              list(params.atom_to_wfn_map))
+
+
+def _iam_core(crystal, params, hkl):
+    c2f = np.linalg.inv(cell_f2c(crystal.cell))
+    hl = np.sqrt(((np.asarray(hkl, dtype=np.float64).reshape(-1, 3) @ c2f) ** 2).sum(1))
+    cols = []
+    for w in params.wfn_parameters:
+        f = ngauss(w.label, "Waasmeier-Kirfel", hl)
+        if f is None:
+            raise ValueError("no Gaussian form factor for " + w.label)
+        cols.append(f)
+    return np.ascontiguousarray(np.stack(cols, 1))
+
+
+def iam_per_hkl(crystal, params, atoms, hkl, include=None, anomalous=None, n_threads=1):
+    """Per-reflection F and derivative rows of the independent-atom model (the per-hkl engine with
+    the Waasmaier-Kirfel form factors, calculateStructureFactorsAndDerivatives<T> in CPU_IAM mode)."""
+    core = _iam_core(crystal, params, hkl)
+    lcs = np.broadcast_to(np.eye(3), (atoms.n, 3, 3))
+    lib().hc_oracle_set_core_override(ctypes.c_void_p(core.ctypes.data), ctypes.c_int(core.shape[1]))
+    try:
+        return per_hkl(crystal, _iam_params(params), atoms, lcs, hkl, include, n_threads, anomalous)
+    finally:
+        lib().hc_oracle_set_core_override(ctypes.c_void_p(0), ctypes.c_int(0))
+
+
+def iam_sf_and_derivs(crystal, params, atoms, hkl, dtdf, include=None, anomalous=None, n_threads=1):
+    """CPU_IAM calculateStructureFactorsAndDerivatives (HansenCoppensStructureFactorCalculator.cpp:
+    478-499 over HansenCoppens_SF_Engine::calculateSF): F and dT/dp = sum_h Re(conj(dT/dF_h) dF_h/dp)
+    - the engine's summed paths are the contraction of its per-hkl derivatives."""
+    F, R = iam_per_hkl(crystal, params, atoms, hkl, include, anomalous, n_threads)
+    g = np.conj(np.asarray(dtdf, dtype=np.complex128))[:, None]
+    n = atoms.n
+    dx = np.stack([np.real(g * R[:, j]).sum(0) for j in range(3)], 1)
+    du = np.stack([np.real(g * R[:, 3 + j]).sum(0) for j in range(6)], 1)
+    do = np.real(g * R[:, 9]).sum(0)
+    return F, dx[:n], du[:n], do[:n]
+
+
+def iam_full_group_f(crystal, params, atoms, hkl, anomalous=None):
+    """Explicit sum over every operation of the group with the IAM form factors (no folding)."""
+    core = _iam_core(crystal, params, hkl)
+    lcs = np.broadcast_to(np.eye(3), (atoms.n, 3, 3))
+    lib().hc_oracle_set_core_override(ctypes.c_void_p(core.ctypes.data), ctypes.c_int(core.shape[1]))
+    try:
+        return sf_and_derivs(crystal, _iam_params(params), atoms, lcs, hkl, np.ones(len(core)), mode="full_group",
+                             anomalous=anomalous)[0]
+    finally:
+        lib().hc_oracle_set_core_override(ctypes.c_void_p(0), ctypes.c_int(0))
+
+
+def electron_sf_and_derivs(crystal, params, atoms, lcs, hkl, dtdf, include=None, n_threads=1, mode="reference",
+                           anomalous=None):
+    """sf_and_derivs with setElectronScattering(true): every term's atomic form factor through
+    the Mott-Bethe formula with Z of the atom's wavefunction and, at h = 0, the "electron-IT"
+    Gaussian value (HansenCoppens_SF_Engine3.cpp:952-970, 1387-1398)."""
+    z = np.array([params.wfn_parameters[w].atomic_number for w in params.atom_to_wfn_map], dtype=np.float64)
+    f0 = np.array([ngauss(ELEMENTS[int(v) - 1], "electron-IT", 0.0) for v in z], dtype=np.float64)
+    lib().hc_oracle_set_electron(ctypes.c_void_p(z.ctypes.data), ctypes.c_void_p(f0.ctypes.data))
+    try:
+        return sf_and_derivs(crystal, params, atoms, lcs, hkl, dtdf, include, n_threads, mode, anomalous)
+    finally:
+        lib().hc_oracle_set_electron(ctypes.c_void_p(0), ctypes.c_void_p(0))

@@ -8,6 +8,7 @@
 #include "discamb/MathUtilities/real_spherical_harmonics.h"
 #include "discamb/MathUtilities/SphConverter.h"
 #include "discamb/MathUtilities/NaturalCubicSpline.h"
+#include "discamb/Scattering/NGaussianFormFactorsTable.h"
 #include "discamb/HC_Model/HC_WfnData.h"
 #include "discamb/HC_Model/ClementiRoettiData.h"
 #include "discamb/HC_Model/DeformationValenceParameters.h"
@@ -31,6 +32,16 @@ void ref_spline(const double* values, int n, double step, double start, const do
 {
     NaturalCubicSpline s(std::vector<double>(values, values + n), step, start);
     for (int i = 0; i < m; i++) out[i] = s(x[i]);
+}
+
+// n_gaussian_form_factors_table::getFormFactor(label, table).calculate_h(h) at m values of |h|;
+// returns 0 if the table has no entry for the label (hasFormFactor)
+int ref_ngauss(const char* label, const char* table, const double* h, int m, double* out)
+{
+    if (!n_gaussian_form_factors_table::hasFormFactor(label, table)) return 0;
+    NGaussianFormFactor f = n_gaussian_form_factors_table::getFormFactor(label, table);
+    for (int i = 0; i < m; i++) out[i] = f.calculate_h(h[i]);
+    return 1;
 }
 
 // sto_scattering::gFunction<l>(n, h, Z)   (SlaterTypeOrbitalScattering.h:51-183)

@@ -284,6 +284,95 @@ def bench_per_hkl(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_variant(args):
+    """The 5000-atom P2_1 2_1 2_1 structure in the independent-atom model (CPU_IAM / HIP_IAM_MIXED)
+    or with electron scattering (Mott-Bethe), mixed and FP64, or its form factors for every
+    reflection (calculateFormFactors)."""
+    from discamb_amd import HansenCoppensStructureFactorCalculator, synth
+
+    w = synth.config1()
+    calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params, device=0)
+    terms = w.terms
+    wl = ("P212121 5000 atoms (C1575 N425 O475 S50 H2475), cell 50x60x70 A, 66592 unique hkl to 1.2 A, "
+          "4 explicit ops")
+    if args.workload == "form_factors":
+        f = calc.calculateFormFactors(w.lcs, w.hkl[:200])          # splines and set-up
+        calc.setTiming(True)
+        t0 = time.perf_counter()
+        f = calc.calculateFormFactors(w.lcs, w.hkl)
+        wall = time.perf_counter() - t0
+        kms = calc.lastTiming()["main_ms"]
+        n, m = w.crystal.atoms.n, len(w.hkl)
+        bytes_out = 16.0 * n * m
+        line = {"metric": "atomic form factors per second (form-factor route, FP64)",
+                "value": n * m / (kms * 1e-3), "unit": "form factors/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+                "ms_per_step": kms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                "dtype": "f64", "data": "synthetic, seeded (numpy PCG64 seed 1000)",
+                "config": {"workload": wl + "; f of every atom for every reflection (%.2f GB)" % (bytes_out / 1e9),
+                           "parallelism": "single GPU"},
+                "roofline": {"bound": "hbm", "roof": "HBM writes of the form factors (16 B per atom and reflection)",
+                             "achieved": bytes_out / (kms * 1e-3) / 1e9, "peak": 8000.0, "unit": "GB/s",
+                             "frac": bytes_out / (kms * 1e-3) / 8e12, "traffic": None, "kernel": "k_ff",
+                             "kernel_ms": kms},
+                "host_delivery": {"wall_s": wall, "GBps": bytes_out / wall / 1e9,
+                                  "note": "256 MB device blocks copied to pageable host memory"}}
+        if not args.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import hc_oracle
+            c2f = np.linalg.inv(hc_oracle.cell_f2c(w.crystal.cell))
+            sel = w.hkl[:2000].astype(np.float64) @ c2f
+            t0 = time.perf_counter()
+            hc_oracle.form_factors(w.crystal, w.params, w.lcs, sel)
+            dt = time.perf_counter() - t0
+            line["cpu_baseline"] = {"value": n * len(sel) / dt, "unit": "form factors/s", "cores": 1, "kind": "port",
+                                    "sample": "oracle (numpy splines + C deformation term) on 2000 reflections x "
+                                              "5000 atoms, %.1f s" % dt}
+        print(json.dumps(line), flush=True)
+        return
+    calc.upload(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+    if args.workload == "iam":
+        modes = ("HIP_IAM_MIXED", "CPU_IAM")
+        label = "independent-atom model (Waasmaier-Kirfel form factors)"
+    else:
+        calc.setElectronScattering(True)
+        calc.upload(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+        modes = ("HIP_MIXED", "HIP")
+        label = "electron scattering (Mott-Bethe on the multipole model)"
+    res = {}
+    for mode in modes:
+        calc.setCalculationMode(mode)
+        calc.upload(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+        res[mode] = time_mode(calc, mode, args.steps, args.warmup, True)
+        print("%s %s: %.3f ms/step, kernel %.3f ms" % (args.workload, mode, res[mode][0] * 1e3, res[mode][1]),
+              file=sys.stderr, flush=True)
+    mx, fp = res[modes[0]], res[modes[1]]
+    line = {"metric": "atom*symop*hkl terms/s for F+derivs (mixed precision), " + label,
+            "value": terms / mx[0], "unit": "terms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": mx[0] * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 terms, f64 phase reduction and accumulation",
+            "data": "synthetic, seeded (numpy PCG64 seed 1000)",
+            "config": {"workload": wl, "terms_per_step": terms, "parallelism": "single GPU"},
+            "kernel_ms": mx[1], "launch": mx[2],
+            "fp64": {"value": terms / fp[0], "ms_per_step": fp[0] * 1e3, "kernel_ms": fp[1]},
+            "accuracy_mixed_vs_fp64": {"max_dF_over_maxF": float(np.abs(mx[3] - fp[3]).max() / np.abs(fp[3]).max())}}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import hc_oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        k = 300 if args.workload == "iam" else 4000
+        t0 = time.perf_counter()
+        if args.workload == "iam":
+            hc_oracle.iam_sf_and_derivs(w.crystal, w.params, w.crystal.atoms, w.hkl[:k], w.dtdf[:k], n_threads=threads)
+        else:
+            hc_oracle.electron_sf_and_derivs(w.crystal, w.params, w.crystal.atoms, w.lcs, w.hkl[:k], w.dtdf[:k],
+                                             n_threads=threads)
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": w.crystal.atoms.n * w.n_explicit_ops * k / dt, "unit": "terms/s",
+                                "cores": threads, "kind": "port",
+                                "sample": "FP64 OpenMP oracle, same structure, first %d reflections, %.1f s" % (k, dt)}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,10 +382,12 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="fixed reflection chunk (disables the autotuner)")
     ap.add_argument("--chunk-fp64", type=int, default=0, help="fixed reflection chunk for the FP64 path only")
     ap.add_argument("--workload", default="p212121_5000",
-                    choices=["p212121_5000", "p212121_20000", "batch256", "per_hkl"],
+                    choices=["p212121_5000", "p212121_20000", "batch256", "per_hkl", "iam", "electron",
+                             "form_factors"],
                     help="p212121_5000: one 5000-atom structure at 1.2 A (headline); p212121_20000: 20 000 atoms "
                          "at 1.5 A; batch256: 256 uneven structures per batched step; per_hkl: per-reflection "
-                         "derivative rows")
+                         "derivative rows; iam / electron: the headline structure in those modes; form_factors: "
+                         "its atomic form factors")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain read+write elementwise kernel over MIB MiB (PMC byte calibration)")
     args = ap.parse_args()
@@ -318,6 +409,8 @@ def main():
         return bench_batch(args)
     if args.workload == "per_hkl":
         return bench_per_hkl(args)
+    if args.workload in ("iam", "electron", "form_factors"):
+        return bench_variant(args)
     from discamb_amd import HansenCoppensStructureFactorCalculator, synth
 
     if args.workload == "p212121_20000":

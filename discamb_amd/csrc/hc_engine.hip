@@ -2562,14 +2562,15 @@ void Engine::form_factors(int n_atoms, const int* type, const double* lcs, int n
     hipLaunchKernelGGL(k_atom_prep, dim3(n_slots), dim3(64), 0, stream_, d_ff_ain_.as<AtomIn>(), d_type_.as<TypeRec>(),
                        d_ff_atoms_.as<AtomRec>(), n_slots);
     HIP_CHECK(hipGetLastError());
-    // blocks of reflections bounded by a 64 MB output buffer
+    // blocks of reflections bounded by a 256 MB output buffer
     const size_t row = (size_t)n_atoms * 2 * sizeof(double);
-    const int B = (int)std::max<size_t>(1, std::min<size_t>((size_t)n_h, (64u << 20) / row));
+    const int B = (int)std::max<size_t>(1, std::min<size_t>((size_t)n_h, (256u << 20) / row));
     const int nt = model_.n_types;
     d_ff_h_.ensure((size_t)B * 3 * sizeof(double));
     d_ff_rec_.ensure((size_t)B * nt * kRadRec * sizeof(double));
     d_ff_hrec_.ensure((size_t)B * 26 * sizeof(double));
     d_ff_out_.ensure((size_t)B * row);
+    float ff_ms = 0.f;
     for (int b0 = 0; b0 < n_h; b0 += B) {
         const int n = std::min(B, n_h - b0);
         HIP_CHECK(hipMemcpyAsync(d_ff_h_.p, h_cart + 3 * (size_t)b0, (size_t)n * 3 * sizeof(double),
@@ -2580,13 +2581,26 @@ void Engine::form_factors(int n_atoms, const int* type, const double* lcs, int n
         HIP_CHECK(hipGetLastError());
         const int chunk = 8;
         const int n_units = n_waves * ((n + chunk - 1) / chunk);
+        if (timing) HIP_CHECK(hipEventRecord(ev0_, stream_));
         hipLaunchKernelGGL(k_ff, dim3((n_units + kWavesPerBlock - 1) / kWavesPerBlock), dim3(256), 0, stream_,
                            d_ff_atoms_.as<AtomRec>(), n_waves, n_atoms, d_ff_hrec_.as<double>(), d_ff_rec_.as<double>(),
                            nt, 0, n, chunk, n_units, d_ff_out_.as<double>());
         HIP_CHECK(hipGetLastError());
+        if (timing) HIP_CHECK(hipEventRecord(ev1_, stream_));
         HIP_CHECK(hipMemcpyAsync(f + (size_t)b0 * n_atoms * 2, d_ff_out_.p, (size_t)n * row, hipMemcpyDeviceToHost,
                                  stream_));
         HIP_CHECK(hipStreamSynchronize(stream_));
+        if (timing) {
+            float ms = 0.f;
+            HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+            ff_ms += ms;
+        }
+    }
+    if (timing) {
+        last_main_ms = ff_ms;
+        last_chunk = 8;
+        last_grid_x = (n_waves * ((B + 7) / 8) + kWavesPerBlock - 1) / kWavesPerBlock;
+        last_grid_y = (n_h + B - 1) / B;     // blocks of reflections
     }
 }
 

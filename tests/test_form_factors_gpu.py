@@ -56,13 +56,13 @@ def test_form_factors_cartesian_edges_mask_and_new_types():
 
 
 def test_form_factors_large_blocks():
-    """5000 atoms x 2000 reflections: three 64 MB output blocks."""
+    """5000 atoms x 8000 reflections: three 256 MB output blocks."""
     w = synth.config1()
-    hkl = w.hkl[::33][:2000]
+    hkl = w.hkl[::8][:8000]
     calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
     print("case form factors 5000 atoms x %d" % len(hkl), flush=True)
     f = calc.calculateFormFactors(w.lcs, hkl)
-    sel = np.r_[0:40, 800:840, 1960:2000]
+    sel = np.r_[0:40, 3340:3380, 7960:8000]
     c2f = np.linalg.inv(hc_oracle.cell_f2c(w.crystal.cell))
     ref = hc_oracle.form_factors(w.crystal, w.params, w.lcs, hkl[sel].astype(np.float64) @ c2f)
     assert rel_err(f[sel], ref) < 1e-10

@@ -373,6 +373,76 @@ def bench_variant(args):
     print(json.dumps(line), flush=True)
 
 
+def neighbour_frames(w):
+    """One local coordinate system per atom of the structure, defined from its two nearest
+    neighbours in the asymmetric unit (x along the nearest, y towards the second nearest), the
+    kind of definition a TAAM assignment produces."""
+    from scipy.spatial import cKDTree
+    from discamb_amd import LcsDefinition
+    from discamb_amd.synth import cell_matrices
+
+    cart = w.crystal.atoms.xyz @ cell_matrices(w.crystal.cell)[0].T
+    _, nb = cKDTree(cart).query(cart, k=3)
+    return [LcsDefinition(i, "average_direction", [(int(nb[i, 1]), "x,y,z")], "average_direction",
+                          [(int(nb[i, 2]), "x,y,z")], 0, 1) for i in range(len(cart))]
+
+
+def bench_taam(args):
+    """Whole calculateStructureFactorsAndDerivatives calls on the 5000-atom structure (atoms, 66 592
+    reflections and dT/dF uploaded, results downloaded, every call) with (a) frames given, (b) frames
+    recomputed from the positions on the host at every call (AnyHcCalculator without frozen frames),
+    (c) (b) plus the chain rule dT/dL : dL/dxyz (multipole adjoint kernel, per-frame forward-mode
+    derivatives on host threads).  Positions move by 1e-4 between calls."""
+    from discamb_amd import HansenCoppensStructureFactorCalculator, synth
+
+    w = synth.config1()
+    defs = neighbour_frames(w)
+    terms = w.terms
+    res = {}
+    for mode in ("HIP_MIXED", "HIP"):
+        calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params, device=0)
+        calc.setCalculationMode(mode)
+        calc.upload(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+        calc.autotune()
+        for variant in ("given", "recomputed", "chain"):
+            if variant == "recomputed":
+                calc.setLocalCoordinateSystems(defs)
+            elif variant == "chain":
+                calc.setLocalCoordinateSystems(defs, chainRule=True)
+            a = w.crystal.atoms.copy()
+            rng = np.random.default_rng(5)
+            for _ in range(args.warmup):
+                calc.calculateStructureFactorsAndDerivatives(a, w.lcs, w.hkl, w.dtdf)
+            calc.setTiming(True)
+            ms, kms = [], []
+            for _ in range(args.steps):
+                a.xyz = a.xyz + rng.normal(0, 1e-4, a.xyz.shape)
+                t0 = time.perf_counter()
+                calc.calculateStructureFactorsAndDerivatives(a, w.lcs, w.hkl, w.dtdf)
+                ms.append((time.perf_counter() - t0) * 1e3)
+                kms.append(calc.lastTiming()["main_ms"])
+            calc.setTiming(False)
+            res[(mode, variant)] = (float(np.median(ms)), float(np.mean(kms)))
+            print("taam %s %s: %.3f ms/call (median), main kernel %.3f ms" % (mode, variant, res[(mode, variant)][0],
+                                                                          res[(mode, variant)][1]),
+                  file=sys.stderr, flush=True)
+        calc.setLocalCoordinateSystems(None)
+    mx = res[("HIP_MIXED", "chain")]
+    wl = ("P212121 5000 atoms (C1575 N425 O475 S50 H2475), cell 50x60x70 A, 66592 unique hkl to 1.2 A, "
+          "4 explicit ops; one two-neighbour local frame per atom")
+    line = {"metric": "atom*symop*hkl terms/s per whole call (F + derivs, frames recomputed, chain rule; mixed)",
+            "value": terms / (mx[0] * 1e-3), "unit": "terms/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": mx[0], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32 terms, f64 phase reduction and accumulation",
+            "data": "synthetic, seeded (numpy PCG64 seed 1000)",
+            "config": {"workload": wl, "terms_per_step": terms, "parallelism": "single GPU",
+                       "timed": "host call incl. uploads of atoms/hkl/dT/dF and downloads"},
+            "calls_ms": {"%s %s" % k: {"call_ms": v[0], "main_kernel_ms": v[1]} for k, v in res.items()}}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(w, budget_refl=4000)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -383,11 +453,12 @@ def main():
     ap.add_argument("--chunk-fp64", type=int, default=0, help="fixed reflection chunk for the FP64 path only")
     ap.add_argument("--workload", default="p212121_5000",
                     choices=["p212121_5000", "p212121_20000", "batch256", "per_hkl", "iam", "electron",
-                             "form_factors"],
+                             "form_factors", "taam"],
                     help="p212121_5000: one 5000-atom structure at 1.2 A (headline); p212121_20000: 20 000 atoms "
                          "at 1.5 A; batch256: 256 uneven structures per batched step; per_hkl: per-reflection "
                          "derivative rows; iam / electron: the headline structure in those modes; form_factors: "
-                         "its atomic form factors")
+                         "its atomic form factors; taam: whole calls with local frames recomputed from the "
+                         "positions and the chain rule through them")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain read+write elementwise kernel over MIB MiB (PMC byte calibration)")
     args = ap.parse_args()
@@ -411,6 +482,8 @@ def main():
         return bench_per_hkl(args)
     if args.workload in ("iam", "electron", "form_factors"):
         return bench_variant(args)
+    if args.workload == "taam":
+        return bench_taam(args)
     from discamb_amd import HansenCoppensStructureFactorCalculator, synth
 
     if args.workload == "p212121_20000":

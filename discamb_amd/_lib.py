@@ -61,6 +61,18 @@ class DhAtoms(ctypes.Structure):
     ]
 
 
+class DhAtomRef(ctypes.Structure):
+    _fields_ = [("atom", ctypes.c_int), ("rot", ctypes.c_int * 9), ("trans", ctypes.c_double * 3)]
+
+
+class DhLcsDef(ctypes.Structure):
+    _fields_ = [("central", DhAtomRef), ("direction1_type", ctypes.c_int), ("direction2_type", ctypes.c_int),
+                ("n_ref1", ctypes.c_int), ("n_ref2", ctypes.c_int), ("n_chirality", ctypes.c_int),
+                ("ref1", ctypes.POINTER(DhAtomRef)), ("ref2", ctypes.POINTER(DhAtomRef)),
+                ("chirality", ctypes.POINTER(DhAtomRef)), ("coordinate_1", ctypes.c_int),
+                ("coordinate_2", ctypes.c_int), ("is_r", ctypes.c_int)]
+
+
 # int (*)(void* user, int hkl_index, const double* f, const double* row)
 PER_HKL_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, c_dbl_p, c_dbl_p)
 JAC_PLANES = 10
@@ -75,6 +87,7 @@ SIGNATURES = {
     "dh_hc_set_calculation_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "dh_hc_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "dh_hc_set_electron_scattering": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "dh_hc_set_iam_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "dh_hc_set_options": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint]),
     "dh_hc_sf_and_derivs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DhAtoms), ctypes.c_int, c_int_p, c_dbl_p,
                                            ctypes.c_uint, c_dbl_p, c_dbl_p, c_dbl_p, c_dbl_p]),
@@ -95,6 +108,9 @@ SIGNATURES = {
                                                ctypes.c_void_p, ctypes.c_void_p]),
     "dh_hc_form_factors": (ctypes.c_int, [ctypes.c_void_p, c_dbl_p, c_u8_p, ctypes.c_int, c_int_p, c_dbl_p]),
     "dh_hc_form_factors_cart": (ctypes.c_int, [ctypes.c_void_p, c_dbl_p, c_u8_p, ctypes.c_int, c_dbl_p, c_dbl_p]),
+    "dh_lcs_compute": (ctypes.c_int, [c_dbl_p, ctypes.c_int, c_dbl_p, ctypes.c_int, ctypes.POINTER(DhLcsDef), c_dbl_p]),
+    "dh_hc_set_lcs_definitions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(DhLcsDef), ctypes.c_int]),
+    "dh_hc_last_lcs": (ctypes.c_int, [ctypes.c_void_p, c_dbl_p]),
     "dh_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     "dh_batch_destroy": (None, [ctypes.c_void_p]),
     "dh_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_int]),

@@ -101,6 +101,53 @@ def _model_desc(crystal: Crystal, params: HC_ModelParameters, d_xyz="fractional"
     return d, keep
 
 
+_LCS_KIND = {"any_orthogonal": 0, "average_direction": 1, "average_position": 2}
+
+
+def _lcs_defs(defs):
+    """ctypes array of dh_lcs_def (and the arrays it points into) from LcsDefinition objects."""
+    keep = []
+
+    def ref(r):
+        idx, op = (r, "x,y,z") if isinstance(r, (int, np.integer)) else r
+        R, t = parse_symop(op)
+        a = _lib.DhAtomRef()
+        a.atom = int(idx)
+        a.rot = (ctypes.c_int * 9)(*[int(v) for v in np.asarray(R).reshape(-1)])
+        a.trans = (ctypes.c_double * 3)(*[float(v) for v in t])
+        return a
+
+    def refs(lst):
+        arr = (_lib.DhAtomRef * max(len(lst), 1))(*[ref(r) for r in lst])
+        keep.append(arr)
+        return ctypes.cast(arr, ctypes.POINTER(_lib.DhAtomRef))
+
+    out = (_lib.DhLcsDef * max(len(defs), 1))()
+    for i, d in enumerate(defs):
+        o = out[i]
+        o.central = ref((d.central, d.central_op))
+        o.direction1_type = _LCS_KIND[d.direction1_type]
+        o.direction2_type = _LCS_KIND[d.direction2_type]
+        o.n_ref1, o.n_ref2, o.n_chirality = len(d.ref1), len(d.ref2), len(d.chirality)
+        o.ref1, o.ref2, o.chirality = refs(d.ref1), refs(d.ref2), refs(d.chirality)
+        o.coordinate_1, o.coordinate_2 = int(d.coordinate_1), int(d.coordinate_2)
+        o.is_r = 1 if d.is_r else 0
+    return out, keep
+
+
+def compute_lcs(cell, xyz_frac, definitions):
+    """Local frames (n, 3, 3), axes as columns, of the definitions for atoms at fractional xyz
+    (LocalCoordinateSystemCalculator::calculate); host code, no GPU needed."""
+    lib = _lib.load()
+    arr, keep = _lcs_defs(definitions)
+    x = np.ascontiguousarray(np.asarray(xyz_frac, dtype=np.float64).reshape(-1, 3))
+    c6 = np.ascontiguousarray(np.asarray(cell, dtype=np.float64))
+    out = np.zeros((max(len(definitions), 1), 9))
+    _lib.check(lib.dh_lcs_compute(_lib.ptr(c6, ctypes.c_double), x.shape[0], _lib.ptr(x, ctypes.c_double),
+                                  len(definitions), arr, _lib.ptr(out, ctypes.c_double)))
+    return out[:len(definitions)].reshape(-1, 3, 3)
+
+
 class HansenCoppensStructureFactorCalculator:
     """F(h) and dT/dp for the Hansen-Coppens multipole model on one GPU."""
 
@@ -129,9 +176,28 @@ class HansenCoppensStructureFactorCalculator:
     def setCalculationMode(self, mode: str):
         _lib.check(self._lib.dh_hc_set_calculation_mode(self._h, mode.encode()))
 
+    def setLocalCoordinateSystems(self, definitions, chainRule: bool = False):
+        """One LcsDefinition per atom (or None to detach): the frames are then recomputed from the
+        positions at every call and the localCoordinateSystems argument is ignored (TAAM,
+        AnyHcCalculator with unfrozen frames).  chainRule adds dT/dL : dL/dxyz to the xyz
+        derivatives."""
+        defs = definitions or []
+        arr, keep = _lcs_defs(defs)
+        _lib.check(self._lib.dh_hc_set_lcs_definitions(self._h, len(defs), arr, 1 if chainRule else 0))
+
+    def lastLocalCoordinateSystems(self):
+        out = np.zeros((max(self.n_atoms, 1), 9))
+        _lib.check(self._lib.dh_hc_last_lcs(self._h, _lib.ptr(out, ctypes.c_double)))
+        return out[:self.n_atoms].reshape(-1, 3, 3)
+
     def setElectronScattering(self, electronScattering: bool):
         """Mott-Bethe electron structure factors for the summed path (needs HC_WfnType.atomic_number)."""
         _lib.check(self._lib.dh_hc_set_electron_scattering(self._h, 1 if electronScattering else 0))
+
+    def setIamTable(self, table: str):
+        """N-Gaussian table of the CPU_IAM mode: "Waasmeier-Kirfel" (default), "IT92" (the table of
+        AnyIamCalculator), "electron-IT" or "electron-cctbx"."""
+        _lib.check(self._lib.dh_hc_set_iam_table(self._h, table.encode()))
 
     def setTypeParameters(self, plm, p_val, kappa, kappa_prime):
         """Replace the atom-type parameters: plm (n_types, 25) in the l*l + l + m layout, p_val,

@@ -3,14 +3,18 @@
 // (src/Scattering/HansenCoppensStructureFactorCalculator.cpp:380-508) around the HIP engine.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <memory>
 #include <string>
+#include <thread>
 
 #include "../../include/discamb_amd.h"
 #include "hc_engine.hpp"
 #include "hc_internal.hpp"
+#include "hc_lcs.hpp"
 
 namespace discamb_amd {
 
@@ -40,12 +44,35 @@ struct PinnedBuffer {
     double* d() const { return static_cast<double*>(p); }
 };
 
+// f(a0, a1) over [0, n) split across up to 16 host threads (at least 256 atoms each)
+template <typename F>
+static void parallel_atoms(int n, F&& f)
+{
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nt = std::max(1, std::min({hw > 0 ? (int)hw : 1, n / 256, 16}));
+    std::vector<std::exception_ptr> err(nt);
+    auto run = [&](int t) {
+        try {
+            f((int)((long)n * t / nt), (int)((long)n * (t + 1) / nt));
+        } catch (...) {
+            err[t] = std::current_exception();
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; t++) pool.emplace_back(run, t);
+    run(0);
+    for (std::thread& th : pool) th.join();
+    for (const std::exception_ptr& e : err)
+        if (e) std::rethrow_exception(e);
+}
+
 struct Calculator {
     Cell cell;
     SpaceGroupInfo sg;
     EngineModel model;
     std::vector<double> mult_w;      // multiplicity / |G|
     std::vector<int> atom_type, atom_wfn;
+    std::vector<int> eng_atom_type;  // engine type per atom (atom_type unless split by core wavefunction)
     int xyz_conv = DH_XYZ_FRACTIONAL, adp_conv = DH_ADP_U_CIF, d_xyz_conv = DH_XYZ_FRACTIONAL,
         d_adp_conv = DH_ADP_U_CIF;
     bool centric_origin = false;
@@ -56,11 +83,16 @@ struct Calculator {
     bool iam = false;
     std::unique_ptr<Engine> iam_engine;
     std::vector<int> iam_atom_type;
+    std::string iam_table = "Waasmeier-Kirfel";
     std::vector<std::string> wfn_label;
     std::vector<int> wfn_z;
     std::vector<std::complex<double>> wfn_anom;
     int device = 0;
     bool electron = false;
+    // local frames recomputed from the positions at every upload (and the chain rule through them)
+    std::vector<LcsDef> lcs_defs;
+    bool lcs_chain = false;
+    std::vector<double> last_lcs, last_xyz_frac;
     Engine& eng() { return iam ? *iam_engine : *engine; }
     bool eng_centric_origin() const { return !iam && centric_origin; }
     // state of the last upload
@@ -184,7 +216,7 @@ static Calculator* create(const dh_model_desc* d, int device)
     }
     c->atom_type.assign(d->atom_to_type, d->atom_to_type + d->n_atoms);
     c->atom_wfn.assign(d->atom_to_wfn, d->atom_to_wfn + d->n_atoms);
-    c->model = build_engine_model(wfn, types, c->atom_wfn, c->atom_type);
+    c->model = build_engine_model(wfn, types, c->atom_wfn, c->atom_type, &c->eng_atom_type);
     c->types = types;
     c->mult_w.resize(d->n_atoms);
     for (int a = 0; a < d->n_atoms; a++) c->mult_w[a] = d->atom_multiplicity[a] / double(c->sg.order);
@@ -207,7 +239,7 @@ static void ensure_iam_engine(Calculator* c)
     if (c->iam_engine) return;
     for (const std::string& l : c->wfn_label)
         if (l.empty()) throw Error(DH_E_MODEL, "CPU_IAM needs the wavefunction type labels (element symbols)");
-    EngineModel m = build_iam_model(c->wfn_label, c->wfn_anom, c->atom_wfn, c->iam_atom_type);
+    EngineModel m = build_iam_model(c->wfn_label, c->wfn_anom, c->atom_wfn, c->iam_atom_type, c->iam_table);
     const bool centro = c->sg.centro;
     std::vector<EngineOp> ops = make_ops(c->cell, c->sg, centro, +1.0);
     c->iam_engine.reset(new Engine(c->device, m, c->cell.par, ops, false));
@@ -237,7 +269,7 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
     ea.weight.resize(n);
     ea.anom.resize(2 * n);
     ea.lcs.resize(9 * n);
-    ea.type = c->iam ? c->iam_atom_type : c->atom_type;
+    ea.type = c->iam ? c->iam_atom_type : c->eng_atom_type;
     ea.include.resize(n);
     c->adp_size.resize(n);
     c->occ.resize(n);
@@ -275,6 +307,15 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
         for (int j = 0; j < 9; j++) ea.lcs[9 * i + j] = at->lcs ? at->lcs[9 * i + j] : (j % 4 == 0 ? 1.0 : 0.0);
         ea.include[i] = at->include ? at->include[i] : 1;
     }
+    if (!c->lcs_defs.empty()) {
+        // AnyHcCalculator::update (AnyHcCalculator.cpp:317-330): frames from the current positions
+        parallel_atoms(n, [&](int a0, int a1) {
+            for (int i = a0; i < a1; i++)
+                lcs_matrix(c->lcs_defs[i], ea.xyz_frac.data(), c->cell.f2c, &ea.lcs[9 * (size_t)i]);
+        });
+    }
+    c->last_lcs = ea.lcs;
+    c->last_xyz_frac = ea.xyz_frac;
     // centering multipliers, dT/dF <- conj(dT/dF) * mult  (.cpp:393-404)
     std::vector<double> dd(3 * (size_t)n_hkl);
     for (int k = 0; k < n_hkl; k++) {
@@ -406,10 +447,36 @@ static void fetch(Calculator* c, double* f, double* d_xyz, double* d_adp, double
         return;
     }
     const double two_pi = 2.0 * M_PI, two_pi2 = 2.0 * M_PI * M_PI;
+    // chain rule through the recomputed frames: dT/dx_frac += w_a dT/dL_a : dL_a/dx_frac
+    std::vector<double> chain;
+    if (c->lcs_chain && !c->lcs_defs.empty() && (c->last_selector & DH_D_XYZ)) {
+        std::vector<double> g(9 * (size_t)n);
+        c->eng().download_lcs_grad(g.data());
+        chain.assign(3 * (size_t)n, 0.0);
+        // per-frame contributions on host threads, then one serial scatter
+        std::vector<int> who((size_t)n * kLcsMaxAtoms), cnt(n);
+        std::vector<double> part((size_t)n * kLcsMaxAtoms * 3);
+        auto work = [&](int a0, int a1) {
+            for (int a = a0; a < a1; a++) {
+                double gw[9];
+                const double w = c->occ[a] * c->mult_w[a];
+                for (int k = 0; k < 9; k++) gw[k] = w * g[9 * (size_t)a + k];
+                cnt[a] = lcs_chain(c->lcs_defs[a], c->last_xyz_frac.data(), c->cell.f2c, gw,
+                                   &who[(size_t)a * kLcsMaxAtoms], &part[(size_t)a * kLcsMaxAtoms * 3]);
+            }
+        };
+        parallel_atoms(n, work);
+        for (int a = 0; a < n; a++)
+            for (int s = 0; s < cnt[a]; s++)
+                for (int j = 0; j < 3; j++)
+                    chain[3 * (size_t)who[(size_t)a * kLcsMaxAtoms + s] + j] += part[((size_t)a * kLcsMaxAtoms + s) * 3 + j];
+    }
     for (int i = 0; i < n; i++) {
         const double* r = c->d10.data() + (size_t)nd * i;
         double w = c->occ[i] * c->mult_w[i];
         double dxf[3] = {two_pi * w * r[0], two_pi * w * r[1], two_pi * w * r[2]};
+        if (!chain.empty())
+            for (int j = 0; j < 3; j++) dxf[j] += chain[3 * (size_t)i + j];
         if (d_xyz) {
             if (c->d_xyz_conv == DH_XYZ_CARTESIAN) {
                 // d/dx_cart = c2f^T d/dx_frac
@@ -436,7 +503,7 @@ static void fetch(Calculator* c, double* f, double* d_xyz, double* d_adp, double
         if (d_occ) d_occ[i] = (c->last_selector & DH_D_OCC) ? c->mult_w[i] * r[9] : 0.0;
     }
     if ((c->last_selector & DH_D_MULTIPOLE) && nd >= 38) {
-        const int nt = c->model.n_types;
+        const int nt = (int)c->types.size();
         c->d_plm.assign(25 * (size_t)nt, 0.0);
         c->d_pval.assign(nt, 0.0);
         c->d_kappa.assign(nt, 0.0);
@@ -459,6 +526,47 @@ static void fetch(Calculator* c, double* f, double* d_xyz, double* d_adp, double
     }
 }
 
+// the multipole-adjoint kernels run when asked for, and for the chain rule through the frames
+static bool run_mp(Calculator* c, unsigned selector)
+{
+    const bool chain = c->lcs_chain && !c->lcs_defs.empty() && (selector & DH_D_XYZ) && !c->iam;
+    c->eng().lcs_grad = chain;
+    return (selector & DH_D_MULTIPOLE) != 0 || chain;
+}
+
+static LcsRef to_ref(const dh_atom_ref& r)
+{
+    LcsRef o;
+    o.atom = r.atom;
+    for (int i = 0; i < 9; i++) o.rot[i] = r.rot[i];
+    for (int i = 0; i < 3; i++) o.trans[i] = r.trans[i];
+    return o;
+}
+
+static std::vector<LcsDef> to_defs(int n, const dh_lcs_def* defs, int n_atoms)
+{
+    if (n < 0 || (n > 0 && !defs)) throw Error(DH_E_ARGUMENT, "invalid local coordinate system definitions");
+    std::vector<LcsDef> out(n);
+    for (int i = 0; i < n; i++) {
+        const dh_lcs_def& d = defs[i];
+        LcsDef& o = out[i];
+        o.central = to_ref(d.central);
+        o.dir1 = d.direction1_type;
+        o.dir2 = d.direction2_type;
+        if (d.n_ref1 < 0 || d.n_ref2 < 0 || (d.n_ref1 > 0 && !d.ref1) || (d.n_ref2 > 0 && !d.ref2) ||
+            (d.n_chirality > 0 && !d.chirality))
+            throw Error(DH_E_ARGUMENT, "invalid local coordinate system definitions");
+        for (int k = 0; k < d.n_ref1; k++) o.ref1.push_back(to_ref(d.ref1[k]));
+        for (int k = 0; k < d.n_ref2; k++) o.ref2.push_back(to_ref(d.ref2[k]));
+        for (int k = 0; k < d.n_chirality; k++) o.chirality.push_back(to_ref(d.chirality[k]));
+        o.c1 = d.coordinate_1;
+        o.c2 = d.coordinate_2;
+        o.is_r = d.is_r != 0;
+        lcs_validate(o, n_atoms);
+    }
+    return out;
+}
+
 static void check_selector(Calculator* c, unsigned selector)
 {
     if (c->iam && (selector & DH_D_MULTIPOLE))
@@ -473,7 +581,7 @@ static void set_electron(Calculator* c, bool on)
     std::vector<double> z_f0;
     if (on) {
         for (int t = 0; t < c->model.n_types; t++) {
-            const int w = c->model.type_wfn[t];
+            const int w = c->model.wfn_origin[c->model.type_wfn[t]];
             const int z = w < (int)c->wfn_z.size() ? c->wfn_z[w] : 0;
             if (z <= 0) throw Error(DH_E_MODEL, "electron scattering needs the atomic number of every wavefunction type");
             int n = 0;
@@ -571,6 +679,56 @@ int dh_hc_set_calculation_mode(dh_calculator* calc, const char* mode)
     });
 }
 
+int dh_hc_set_iam_table(dh_calculator* calc, const char* table)
+{
+    return guarded([&] {
+        if (!calc || !table) throw Error(DH_E_ARGUMENT, "null argument");
+        Calculator* c = calc->c;
+        const std::string t(table);
+        ngauss_lookup("H", t, nullptr);     // rejects an unknown table name
+        if (t == c->iam_table) return;
+        c->iam_table = t;
+        c->iam_engine.reset();
+        if (c->iam) {
+            ensure_iam_engine(c);
+            c->iam_engine->tune = c->engine->tune;
+            c->iam_engine->timing = c->engine->timing;
+        }
+        c->rows_uploaded = false;
+    });
+}
+
+int dh_lcs_compute(const double cell[6], int n_atoms, const double* xyz_frac, int n, const dh_lcs_def* defs, double* lcs)
+{
+    return guarded([&] {
+        if (!cell || (n > 0 && (!xyz_frac || !lcs))) throw Error(DH_E_ARGUMENT, "null argument");
+        Cell c;
+        c.set(cell);
+        std::vector<LcsDef> d = to_defs(n, defs, n_atoms);
+        for (int i = 0; i < n; i++) lcs_matrix(d[i], xyz_frac, c.f2c, lcs + 9 * (size_t)i);
+    });
+}
+
+int dh_hc_set_lcs_definitions(dh_calculator* calc, int n, const dh_lcs_def* defs, int chain_rule)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        Calculator* c = calc->c;
+        if (n != 0 && n != (int)c->atom_type.size())
+            throw Error(DH_E_ARGUMENT, "one local coordinate system definition per atom is needed");
+        c->lcs_defs = to_defs(n, defs, (int)c->atom_type.size());
+        c->lcs_chain = chain_rule != 0 && n > 0;
+    });
+}
+
+int dh_hc_last_lcs(dh_calculator* calc, double* lcs)
+{
+    return guarded([&] {
+        if (!calc || !lcs) throw Error(DH_E_ARGUMENT, "null argument");
+        std::memcpy(lcs, calc->c->last_lcs.data(), calc->c->last_lcs.size() * sizeof(double));
+    });
+}
+
 int dh_hc_set_electron_scattering(dh_calculator* calc, int on)
 {
     return guarded([&] {
@@ -616,7 +774,7 @@ int dh_hc_run(dh_calculator* calc, unsigned selector)
         if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
         calc->c->last_selector = selector & (DH_D_ALL | DH_D_MULTIPOLE);
         check_selector(calc->c, selector);
-        calc->c->eng().run(calc->c->prec, calc->c->last_selector != 0, (selector & DH_D_MULTIPOLE) != 0);
+        calc->c->eng().run(calc->c->prec, calc->c->last_selector != 0, run_mp(calc->c, calc->c->last_selector));
     });
 }
 
@@ -637,7 +795,7 @@ int dh_hc_sf_and_derivs(dh_calculator* calc, const dh_atoms* atoms, int n_hkl, c
         upload(calc->c, atoms, n_hkl, hkl, dtarget_df);
         calc->c->last_selector = selector & (DH_D_ALL | DH_D_MULTIPOLE);
         check_selector(calc->c, selector);
-        calc->c->eng().run(calc->c->prec, calc->c->last_selector != 0, (selector & DH_D_MULTIPOLE) != 0);
+        calc->c->eng().run(calc->c->prec, calc->c->last_selector != 0, run_mp(calc->c, calc->c->last_selector));
         fetch(calc->c, f, d_xyz, d_adp, d_occ);
     });
 }
@@ -648,7 +806,7 @@ int dh_hc_autotune(dh_calculator* calc, unsigned selector, int* chunk)
         if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
         check_selector(calc->c, selector);
         int c = calc->c->eng().autotune(calc->c->prec, (selector & (DH_D_ALL | DH_D_MULTIPOLE)) != 0,
-                                          (selector & DH_D_MULTIPOLE) != 0);
+                                          run_mp(calc->c, selector));
         if (chunk) *chunk = c;
     });
 }
@@ -659,7 +817,7 @@ int dh_hc_set_type_parameters(dh_calculator* calc, int n_types, const double* pl
     return guarded([&] {
         if (!calc || !plm || !p_val || !kappa || !kappa_prime) throw Error(DH_E_ARGUMENT, "null argument");
         Calculator* c = calc->c;
-        if (n_types != c->model.n_types) throw Error(DH_E_ARGUMENT, "number of atom types differs from the model");
+        if (n_types != (int)c->types.size()) throw Error(DH_E_ARGUMENT, "number of atom types differs from the model");
         std::vector<AtomType> t = c->types;
         for (int i = 0; i < n_types; i++) {
             for (int j = 0; j < 25; j++) t[i].plm[j] = plm[25 * i + j];
@@ -667,7 +825,9 @@ int dh_hc_set_type_parameters(dh_calculator* calc, int n_types, const double* pl
             t[i].kappa = kappa[i];
             t[i].kappa_prime = kappa_prime[i];
         }
-        c->engine->set_types(t);
+        std::vector<AtomType> et(c->model.n_types);
+        for (int e = 0; e < c->model.n_types; e++) et[e] = t[c->model.type_origin[e]];
+        c->engine->set_types(et);
         c->types = t;
         c->model = c->engine->model();
     });
@@ -680,7 +840,7 @@ int dh_hc_fetch_type_derivatives(dh_calculator* calc, double* d_plm, double* d_p
         if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
         Calculator* c = calc->c;
         if (!c->have_mp) throw Error(DH_E_ARGUMENT, "no multipole derivatives: run with DH_D_MULTIPOLE and fetch first");
-        const size_t nt = c->model.n_types;
+        const size_t nt = c->types.size();
         if (d_plm) std::memcpy(d_plm, c->d_plm.data(), 25 * nt * sizeof(double));
         if (d_p_val) std::memcpy(d_p_val, c->d_pval.data(), nt * sizeof(double));
         if (d_kappa) std::memcpy(d_kappa, c->d_kappa.data(), nt * sizeof(double));
@@ -823,8 +983,9 @@ int dh_batch_create(dh_calculator* const* calcs, int n, dh_batch** out)
         std::vector<Engine*> eng;
         for (int i = 0; i < n; i++) {
             if (!calcs[i]) throw Error(DH_E_ARGUMENT, "null calculator in batch");
-            if (calcs[i]->c->iam || calcs[i]->c->electron)
-                throw Error(DH_E_ARGUMENT, "batches take multipole-model X-ray calculators (not CPU_IAM or electron)");
+            if (calcs[i]->c->iam || calcs[i]->c->electron || calcs[i]->c->lcs_chain)
+                throw Error(DH_E_ARGUMENT, "batches take multipole-model X-ray calculators (not CPU_IAM, electron "
+                                           "or the local-frame chain rule)");
             b->calcs.push_back(calcs[i]->c);
             eng.push_back(calcs[i]->c->engine.get());
         }
@@ -840,8 +1001,9 @@ static Precision batch_precision(dh_batch* b)
     const Precision p = b->calcs[0]->prec;
     for (Calculator* c : b->calcs) {
         if (c->prec != p) throw Error(DH_E_ARGUMENT, "calculators of a batch must share the calculation mode");
-        if (c->iam || c->electron)
-            throw Error(DH_E_ARGUMENT, "batches take multipole-model X-ray calculators (not CPU_IAM or electron)");
+        if (c->iam || c->electron || c->lcs_chain)
+            throw Error(DH_E_ARGUMENT, "batches take multipole-model X-ray calculators (not CPU_IAM, electron or the "
+                                       "local-frame chain rule)");
     }
     return p;
 }

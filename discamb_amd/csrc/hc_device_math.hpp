@@ -34,6 +34,28 @@ __device__ __forceinline__ void ylm_density(T x, T y, T z, T* d)
     for (int i = 0; i < 25; i++) d[i] *= T(c_density_norm[i]);
 }
 
+// Gradients (d/dx, d/dy, d/dz) of the 25 density-normalised harmonic polynomials of ylm_density,
+// taken as polynomials in (x, y, z).
+__device__ __forceinline__ void ylm_density_grad(double x, double y, double z, double (*g)[3])
+{
+    const double xx = x * x, yy = y * y, zz = z * z;
+    const double q[25][3] = {
+        {0, 0, 0},
+        {0, 1, 0}, {0, 0, 1}, {1, 0, 0},
+        {y, x, 0}, {0, z, y}, {-2 * x, -2 * y, 4 * z}, {z, 0, x}, {2 * x, -2 * y, 0},
+        {6 * x * y, 3 * xx - 3 * yy, 0}, {y * z, x * z, x * y}, {-2 * x * y, 4 * zz - xx - 3 * yy, 8 * y * z},
+        {-6 * x * z, -6 * y * z, 6 * zz - 3 * xx - 3 * yy}, {4 * zz - 3 * xx - yy, -2 * x * y, 8 * x * z},
+        {2 * x * z, -2 * y * z, xx - yy}, {3 * xx - 3 * yy, -6 * x * y, 0},
+        {3 * xx * y - y * yy, x * xx - 3 * x * yy, 0}, {6 * x * y * z, 3 * xx * z - 3 * yy * z, 3 * xx * y - y * yy},
+        {y * (7 * zz - 1), x * (7 * zz - 1), 14 * x * y * z}, {0, 7 * z * zz - 3 * z, 21 * y * zz - 3 * y},
+        {0, 0, 140 * z * zz - 60 * z}, {7 * z * zz - 3 * z, 0, 21 * x * zz - 3 * x},
+        {2 * x * (7 * zz - 1), -2 * y * (7 * zz - 1), 14 * z * (xx - yy)},
+        {3 * xx * z - 3 * yy * z, -6 * x * y * z, x * xx - 3 * x * yy},
+        {4 * x * xx - 12 * x * yy, -12 * xx * y + 4 * y * yy, 0}};
+    for (int i = 0; i < 25; i++)
+        for (int c = 0; c < 3; c++) g[i][c] = q[i][c] * c_density_norm[i];
+}
+
 // int_0^inf r^n exp(-z r) j_l(k r) dr with k = 2 pi h (FP64)
 __device__ __forceinline__ double g_function(int l, int n, double h, double z)
 {

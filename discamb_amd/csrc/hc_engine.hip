@@ -1672,6 +1672,53 @@ __global__ __launch_bounds__(64) void k_plm_back(const AtomIn* __restrict__ in, 
     }
 }
 
+// dT/dL for every atom's local frame L (columns = local axes), from the global-frame multipole
+// adjoint B = dT/dP_glob (weight excluded) before k_plm_back maps it to the local frame:
+// P_glob,i = sum_q W_qi f(L^T u_q) with f(v) = sum_j P_loc,j d_j(v) (the projection of
+// atom_prep_item), hence dT/dL_ab = sum_q s_q u_q,a (grad f)_b(L^T u_q), s_q = sum_{i, l <= max_l}
+// W_qi B_i.  One 64-lane workgroup per atom slot, lane q = quadrature node; lane sums in order.
+__global__ __launch_bounds__(64) void k_lcs_grad(const AtomIn* __restrict__ in, const TypeRec* __restrict__ types,
+                                                 int nd, const double* __restrict__ d, double* __restrict__ g)
+{
+    const int i = blockIdx.x;
+    const int p = threadIdx.x;
+    __shared__ double part[9][64];
+    const AtomIn& a = in[i];
+    if (a.orig < 0) return;               // whole block exits together
+    const TypeRec& ty = types[a.type];
+    double G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (p < 50 && ty.max_l >= 0) {
+        const double* B = d + (size_t)a.orig * nd + 10;
+        double s = 0.0;
+        for (int j = 0; j < 25; j++) {
+            const int l = j < 1 ? 0 : j < 4 ? 1 : j < 9 ? 2 : j < 16 ? 3 : 4;
+            if (l <= ty.max_l) s += c_proj_w[p][j] * B[j];
+        }
+        const double u0 = c_proj_dir[p][0], u1 = c_proj_dir[p][1], u2 = c_proj_dir[p][2];
+        const double x = a.lcs[0] * u0 + a.lcs[3] * u1 + a.lcs[6] * u2;
+        const double y = a.lcs[1] * u0 + a.lcs[4] * u1 + a.lcs[7] * u2;
+        const double z = a.lcs[2] * u0 + a.lcs[5] * u1 + a.lcs[8] * u2;
+        double gy[25][3];
+        ylm_density_grad(x, y, z, gy);
+        double gf[3] = {0, 0, 0};
+        for (int j = 0; j < 25; j++) {
+            const int l = j < 1 ? 0 : j < 4 ? 1 : j < 9 ? 2 : j < 16 ? 3 : 4;
+            if (l <= ty.max_l)
+                for (int c = 0; c < 3; c++) gf[c] += ty.plm[j] * gy[j][c];
+        }
+        const double u[3] = {u0, u1, u2};
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) G[3 * r + c] = s * u[r] * gf[c];
+    }
+    for (int k = 0; k < 9; k++) part[k][p] = G[k];
+    __syncthreads();
+    if (p < 9) {
+        double acc = 0.0;
+        for (int q = 0; q < 50; q++) acc += part[p][q];
+        g[(size_t)a.orig * 9 + p] = acc;
+    }
+}
+
 // ------------------------------------------------------------------ host side
 
 static bool g_constants_ready[64] = {false};
@@ -2298,10 +2345,25 @@ void Engine::reduce(bool derivs, bool mp, int chunk)
                        d_atoms_.as<AtomRec>(), n_chunks, n_slots, nd, d_d_.as<double>());
     HIP_CHECK(hipGetLastError());
     if (mp) {
+        if (lcs_grad) {
+            d_lcsg_.ensure((size_t)std::max(n_atoms_, 1) * 9 * sizeof(double));
+            fill(d_lcsg_, (size_t)std::max(n_atoms_, 1) * 9, 0.0);
+            hipLaunchKernelGGL(k_lcs_grad, dim3(n_slots), dim3(64), 0, stream_, d_atoms_in_.as<AtomIn>(),
+                               d_type_.as<TypeRec>(), nd, d_d_.as<double>(), d_lcsg_.as<double>());
+            HIP_CHECK(hipGetLastError());
+        }
         hipLaunchKernelGGL(k_plm_back, dim3(n_slots), dim3(64), 0, stream_, d_atoms_in_.as<AtomIn>(), n_slots, nd,
                            d_d_.as<double>());
         HIP_CHECK(hipGetLastError());
     }
+}
+
+void Engine::download_lcs_grad(double* g)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    if (!ran_ || !lcs_grad || last_nd_ < kNdMultipole) throw Error(1, "internal: no local-frame gradient computed");
+    HIP_CHECK(hipMemcpyAsync(g, d_lcsg_.p, (size_t)n_atoms_ * 9 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
 }
 
 int Engine::autotune(Precision prec, bool derivs, bool mp)

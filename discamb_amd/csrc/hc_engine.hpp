@@ -88,6 +88,10 @@ public:
     // The radial factors then come from per-(reflection, type) records.
     void set_electron(bool on, const std::vector<double>& z_f0);
     bool electron() const { return electron_; }
+    // With the multipole adjoint: also dT/dL (9 per atom, local axes as columns, atom weight not
+    // applied) for the chain rule through recomputed local frames; read with download_lcs_grad.
+    bool lcs_grad = false;
+    void download_lcs_grad(double* g);
     // Form-factor route: f[k][atom] (complex, caller atom order) for n_h Cartesian h vectors and
     // the given local frames and atom types; synchronous, host pointers.
     void form_factors(int n_atoms, const int* type, const double* lcs, int n_h, const double* h_cart, double* f);
@@ -183,7 +187,7 @@ private:
     int jac_resident_ = 0;
     DeviceBuffer d_jatoms_, d_jconv_, d_jrec_;
     bool electron_ = false;
-    DeviceBuffer d_ez_;
+    DeviceBuffer d_ez_, d_lcsg_;
     // form-factor route
     bool splines_ready_ = false;
     DeviceBuffer d_spline_, d_ff_ain_, d_ff_atoms_, d_ff_h_, d_ff_rec_, d_ff_hrec_, d_ff_out_;

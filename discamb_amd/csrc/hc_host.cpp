@@ -5,6 +5,7 @@
 #include <cctype>
 #include <cmath>
 #include <cstring>
+#include <map>
 
 #include "ngauss_tables.h"
 
@@ -245,7 +246,8 @@ void build_sphere_density(const WfnType& w, bool valence, SlaterDensity& out)
 }
 
 EngineModel build_engine_model(const std::vector<WfnType>& wfn, const std::vector<AtomType>& types,
-                               const std::vector<int>& atom_wfn, const std::vector<int>& atom_type)
+                               const std::vector<int>& atom_wfn, const std::vector<int>& atom_type,
+                               std::vector<int>* engine_atom_type)
 {
     EngineModel m;
     m.n_wfn = (int)wfn.size();
@@ -286,15 +288,55 @@ EngineModel build_engine_model(const std::vector<WfnType>& wfn, const std::vecto
             throw Error(1, "atom type or wavefunction index out of range");
         m.type_wfn[atom_type[a]] = atom_wfn[a];
     }
+    m.n_model_types = m.n_types;
+    for (int t = 0; t < m.n_types; t++) m.type_origin.push_back(t);
+    for (int w = 0; w < m.n_wfn; w++) m.wfn_origin.push_back(w);
+    // core of the atom's wavefunction, valence and deformation terms of the type's
+    // (HansenCoppens_SF_Engine3.cpp:218-224, 1295-1300)
+    std::map<std::pair<int, int>, int> pair_wfn, pair_type;
+    std::vector<int> eat(atom_type);
+    for (size_t a = 0; a < atom_wfn.size(); a++) {
+        const int t = atom_type[a], cw = atom_wfn[a], vw = m.type_wfn[t];
+        if (cw == vw) continue;
+        auto pw = pair_wfn.find({cw, vw});
+        int ew;
+        if (pw == pair_wfn.end()) {
+            ew = m.n_wfn++;
+            pair_wfn[{cw, vw}] = ew;
+            m.core.push_back(m.core[cw]);
+            m.valence.push_back(m.valence[vw]);
+            m.dv_zeta.push_back(m.dv_zeta[vw]);
+            m.dv_nl.push_back(m.dv_nl[vw]);
+            for (int l = 0; l < 5; l++) m.dv_n.push_back(m.dv_n[5 * vw + l]);
+            m.anom.push_back(m.anom[2 * cw]);
+            m.anom.push_back(m.anom[2 * cw + 1]);
+            m.wfn_origin.push_back(cw);
+        } else {
+            ew = pw->second;
+        }
+        auto pt = pair_type.find({t, cw});
+        if (pt == pair_type.end()) {
+            const int et = m.n_types++;
+            pair_type[{t, cw}] = et;
+            m.types.push_back(m.types[t]);
+            m.type_wfn.push_back(ew);
+            m.type_origin.push_back(t);
+            eat[a] = et;
+        } else {
+            eat[a] = pt->second;
+        }
+    }
     m.type_max_l.resize(m.n_types);
     for (int t = 0; t < m.n_types; t++) {
+        const AtomType& ty = m.types[t];
         int lp = -1;
-        for (int l = 0; l < types[t].n_l && l < 5; l++)
+        for (int l = 0; l < ty.n_l && l < 5; l++)
             for (int i = l * l; i < (l + 1) * (l + 1); i++)
-                if (types[t].plm[i] != 0.0) lp = l;
+                if (ty.plm[i] != 0.0) lp = l;
         int lw = std::min(4, m.dv_nl[m.type_wfn[t]] - 1);
         m.type_max_l[t] = std::min(lw, lp);
     }
+    if (engine_atom_type) *engine_atom_type = eat;
     return m;
 }
 
@@ -357,7 +399,7 @@ std::string element_symbol(int z)
 // its own atom type (so per-type radial records follow the element); the Gaussians are stored as
 // core "terms" with power -1.
 EngineModel build_iam_model(const std::vector<std::string>& labels, const std::vector<std::complex<double>>& anom,
-                            const std::vector<int>& atom_wfn, std::vector<int>& atom_type)
+                            const std::vector<int>& atom_wfn, std::vector<int>& atom_type, const std::string& table)
 {
     EngineModel m;
     m.n_wfn = (int)labels.size();
@@ -366,7 +408,7 @@ EngineModel build_iam_model(const std::vector<std::string>& labels, const std::v
     m.valence.resize(m.n_wfn);
     for (int w = 0; w < m.n_wfn; w++) {
         int n = 0;
-        const NGaussEntry* e = ngauss_lookup(labels[w], "Waasmeier-Kirfel", &n);
+        const NGaussEntry* e = ngauss_lookup(labels[w], table, &n);
         if (!e)
             throw Error(2, "request for Gaussian type atomic form factor parameter for unknown atom type: " + labels[w]);
         for (int i = 0; i < n; i++) {
@@ -387,6 +429,11 @@ EngineModel build_iam_model(const std::vector<std::string>& labels, const std::v
     m.type_wfn.resize(m.n_types);
     for (int t = 0; t < m.n_types; t++) m.type_wfn[t] = t;
     m.type_max_l.assign(m.n_types, -1);
+    m.n_model_types = m.n_types;
+    for (int t = 0; t < m.n_types; t++) {
+        m.type_origin.push_back(t);
+        m.wfn_origin.push_back(t);
+    }
     atom_type = atom_wfn;
     for (int w : atom_wfn)
         if (w < 0 || w >= m.n_wfn) throw Error(1, "atom wavefunction index out of range");

@@ -95,6 +95,13 @@ struct EngineModel {
     std::vector<AtomType> types;
     std::vector<int> type_wfn;                    // type -> wfn (last atom wins, as Engine3)
     std::vector<int> type_max_l;                  // min(4, n_dv - 1, highest non-zero l)
+    // Engine3 takes the core term from the atom's own wavefunction but the valence and
+    // deformation terms from the wavefunction of the type's last atom.  Atoms whose wavefunction
+    // differs from that one get an engine type of their own (appended after the model's types,
+    // same parameters) over an engine wavefunction combining their core with the type's valence.
+    int n_model_types = 0;
+    std::vector<int> type_origin;                 // engine type -> model type
+    std::vector<int> wfn_origin;                  // engine wfn -> model wfn of its core, f', f'', Z
 };
 
 void build_sphere_density(const WfnType& w, bool valence, SlaterDensity& out);
@@ -104,11 +111,14 @@ struct NGaussEntry;
 const NGaussEntry* ngauss_lookup(const std::string& label, const std::string& table, int* n_gauss);
 double ngauss_value(const NGaussEntry& e, int n, double h);
 std::string element_symbol(int z);
+// engine_atom_type: the engine type of every atom (the model type unless split as above)
 EngineModel build_engine_model(const std::vector<WfnType>& wfn, const std::vector<AtomType>& types,
-                               const std::vector<int>& atom_wfn, const std::vector<int>& atom_type);
-// Independent-atom model (Waasmaier-Kirfel Gaussians per wavefunction label); atom_type is set to
-// the atom-to-wavefunction map.
+                               const std::vector<int>& atom_wfn, const std::vector<int>& atom_type,
+                               std::vector<int>* engine_atom_type = nullptr);
+// Independent-atom model (N-Gaussians of the named table per wavefunction label, Waasmaier-Kirfel
+// by default); atom_type is set to the atom-to-wavefunction map.
 EngineModel build_iam_model(const std::vector<std::string>& labels, const std::vector<std::complex<double>>& anom,
-                            const std::vector<int>& atom_wfn, std::vector<int>& atom_type);
+                            const std::vector<int>& atom_wfn, std::vector<int>& atom_type,
+                            const std::string& table = "Waasmeier-Kirfel");
 
 }  // namespace discamb_amd

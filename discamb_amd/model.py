@@ -163,6 +163,28 @@ class TargetFunctionAtomicParamDerivatives:
     occupancy_derivatives: np.ndarray
 
 
+@dataclass
+class LcsDefinition:
+    """Definition of one atom's local coordinate system from atomic positions
+    (LocalCoordinateSystem<AtomInCrystalID>, LocalCoordinateSystem.h:20-55).  Every reference atom is
+    (index in the asymmetric unit, symmetry operation applied to it).  direction types:
+    "average_direction" (mean of the unit vectors from the central atom to the reference atoms),
+    "average_position" (from the central atom to the reference atoms' centroid), "any_orthogonal".
+    coordinate_1 / coordinate_2: axis (0 X, 1 Y, 2 Z) along direction 1 / in the plane of both
+    directions; is_r: right-handed; chirality: three atoms whose triple product fixes the sign of
+    the third axis (or empty)."""
+    central: int
+    direction1_type: str
+    ref1: List[tuple]
+    direction2_type: str
+    ref2: List[tuple]
+    coordinate_1: int = 0
+    coordinate_2: int = 1
+    is_r: bool = True
+    chirality: List[tuple] = field(default_factory=list)
+    central_op: str = "x,y,z"
+
+
 class SfDerivativesAtHkl:
     """dF/dp of one reflection for every atom (SF_CalcDataTypes.h:49-55), complex:
     atomicPostionDerivatives (n, 3), adpDerivatives (n, 6) (U_iso in column 0, unused columns 0),

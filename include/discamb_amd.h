@@ -131,6 +131,13 @@ int dh_hc_set_calculation_mode(dh_calculator* calc, const char* mode);
  * Needs dh_model_desc.wfn_atomic_number.  As in the reference it affects the summed path only (not
  * the per-reflection rows, the form factors or CPU_IAM); batches do not take it. */
 int dh_hc_set_electron_scattering(dh_calculator* calc, int on);
+/* N-Gaussian table of the independent-atom model: "Waasmeier-Kirfel" (default, the table of
+ * CPU_IAM), "IT92" (AnyIamCalculator's default, AnyIamCalculator.cpp:101-108), "electron-IT" or
+ * "electron-cctbx" (n_gaussian_form_factors_table, NGaussianFormFactorsTable.cpp).  In IAM mode
+ * the terms are summed over every symmetry operation with f'' unconjugated, which is also what
+ * AnyScattererStructureFactorCalculator computes (AnyScattererStructureFactorCalculator.cpp:
+ * 614-832), so an IAM calculator with the "IT92" table stands in for AnyIamCalculator. */
+int dh_hc_set_iam_table(dh_calculator* calc, const char* table);
 /* Launch tuning: reflections per workgroup column (0 = automatic). */
 int dh_hc_set_tuning(dh_calculator* calc, int hkl_chunk);
 /* Kernel-path options (no reference counterpart; for tests and diagnosis).
@@ -222,6 +229,41 @@ int dh_hc_form_factors(dh_calculator* calc, const double* lcs, const unsigned ch
                        double* f);
 int dh_hc_form_factors_cart(dh_calculator* calc, const double* lcs, const unsigned char* include, int n_h,
                             const double* h_cart, double* f);
+
+/* ---- Local coordinate systems recomputed from the atomic positions (the TAAM route:
+ * LocalCoordinateSystem<AtomInCrystalID> definitions, LocalCoordinateSystem.h:20-55, evaluated by
+ * LocalCoordinateSystemCalculator::calculate, LocalCoordinateSystemCalculator.cpp:194-238, every
+ * step unless frozen, AnyHcCalculator.cpp:317-330).  A reference atom is an atom of the model under
+ * a fractional symmetry operation. */
+#define DH_LCS_ANY_ORTHOGONAL 0
+#define DH_LCS_AVERAGE_DIRECTION 1
+#define DH_LCS_AVERAGE_POSITION 2
+typedef struct {
+    int atom;
+    int rot[9];       /* fractional rotation, row-major */
+    double trans[3];  /* fractional translation */
+} dh_atom_ref;
+typedef struct {
+    dh_atom_ref central;
+    int direction1_type, direction2_type;     /* DH_LCS_* */
+    int n_ref1, n_ref2, n_chirality;          /* n_chirality: 0 or 3 */
+    const dh_atom_ref* ref1;
+    const dh_atom_ref* ref2;
+    const dh_atom_ref* chirality;
+    int coordinate_1, coordinate_2;           /* 0 X, 1 Y, 2 Z: axis along direction 1, axis in the plane */
+    int is_r;                                 /* right-handed */
+} dh_lcs_def;
+/* The frames of n definitions for atoms at fractional positions xyz (3 per atom): lcs 9n, row-major
+ * 3x3 with the local axes as columns (the layout dh_atoms.lcs takes). */
+int dh_lcs_compute(const double cell[6], int n_atoms, const double* xyz_frac, int n, const dh_lcs_def* defs,
+                   double* lcs);
+/* Attach one definition per atom (n = number of atoms; n = 0 detaches): every following upload
+ * computes the frames from the uploaded positions and ignores dh_atoms.lcs.  chain_rule != 0 adds
+ * dT/dL : dL/dxyz to the xyz derivatives (the reference treats frames as constant,
+ * HansenCoppens_SF_Engine3.cpp:1030-1509); it runs the multipole-adjoint kernels for that. */
+int dh_hc_set_lcs_definitions(dh_calculator* calc, int n, const dh_lcs_def* defs, int chain_rule);
+/* the frames used by the last upload (9 per atom) */
+int dh_hc_last_lcs(dh_calculator* calc, double* lcs);
 
 /* ---- Batches of structures (no reference counterpart: the reference computes one structure per
  * call; a refinement of many small structures calls it once per structure).

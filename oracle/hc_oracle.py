@@ -448,22 +448,23 @@ def _iam_params(params):
                               list(params.atom_to_wfn_map))
 
 
-def _iam_core(crystal, params, hkl):
+def _iam_core(crystal, params, hkl, table="Waasmeier-Kirfel"):
     c2f = np.linalg.inv(cell_f2c(crystal.cell))
     hl = np.sqrt(((np.asarray(hkl, dtype=np.float64).reshape(-1, 3) @ c2f) ** 2).sum(1))
     cols = []
     for w in params.wfn_parameters:
-        f = ngauss(w.label, "Waasmeier-Kirfel", hl)
+        f = ngauss(w.label, table, hl)
         if f is None:
             raise ValueError("no Gaussian form factor for " + w.label)
         cols.append(f)
     return np.ascontiguousarray(np.stack(cols, 1))
 
 
-def iam_per_hkl(crystal, params, atoms, hkl, include=None, anomalous=None, n_threads=1):
+def iam_per_hkl(crystal, params, atoms, hkl, include=None, anomalous=None, n_threads=1, table="Waasmeier-Kirfel"):
     """Per-reflection F and derivative rows of the independent-atom model (the per-hkl engine with
-    the Waasmaier-Kirfel form factors, calculateStructureFactorsAndDerivatives<T> in CPU_IAM mode)."""
-    core = _iam_core(crystal, params, hkl)
+    the Waasmaier-Kirfel form factors, calculateStructureFactorsAndDerivatives<T> in CPU_IAM mode;
+    another N-Gaussian table by name)."""
+    core = _iam_core(crystal, params, hkl, table)
     lcs = np.broadcast_to(np.eye(3), (atoms.n, 3, 3))
     lib().hc_oracle_set_core_override(ctypes.c_void_p(core.ctypes.data), ctypes.c_int(core.shape[1]))
     try:
@@ -472,11 +473,12 @@ def iam_per_hkl(crystal, params, atoms, hkl, include=None, anomalous=None, n_thr
         lib().hc_oracle_set_core_override(ctypes.c_void_p(0), ctypes.c_int(0))
 
 
-def iam_sf_and_derivs(crystal, params, atoms, hkl, dtdf, include=None, anomalous=None, n_threads=1):
+def iam_sf_and_derivs(crystal, params, atoms, hkl, dtdf, include=None, anomalous=None, n_threads=1,
+                      table="Waasmeier-Kirfel"):
     """CPU_IAM calculateStructureFactorsAndDerivatives (HansenCoppensStructureFactorCalculator.cpp:
     478-499 over HansenCoppens_SF_Engine::calculateSF): F and dT/dp = sum_h Re(conj(dT/dF_h) dF_h/dp)
     - the engine's summed paths are the contraction of its per-hkl derivatives."""
-    F, R = iam_per_hkl(crystal, params, atoms, hkl, include, anomalous, n_threads)
+    F, R = iam_per_hkl(crystal, params, atoms, hkl, include, anomalous, n_threads, table)
     g = np.conj(np.asarray(dtdf, dtype=np.complex128))[:, None]
     n = atoms.n
     dx = np.stack([np.real(g * R[:, j]).sum(0) for j in range(3)], 1)
@@ -485,9 +487,9 @@ def iam_sf_and_derivs(crystal, params, atoms, hkl, dtdf, include=None, anomalous
     return F, dx[:n], du[:n], do[:n]
 
 
-def iam_full_group_f(crystal, params, atoms, hkl, anomalous=None):
+def iam_full_group_f(crystal, params, atoms, hkl, anomalous=None, table="Waasmeier-Kirfel"):
     """Explicit sum over every operation of the group with the IAM form factors (no folding)."""
-    core = _iam_core(crystal, params, hkl)
+    core = _iam_core(crystal, params, hkl, table)
     lcs = np.broadcast_to(np.eye(3), (atoms.n, 3, 3))
     lib().hc_oracle_set_core_override(ctypes.c_void_p(core.ctypes.data), ctypes.c_int(core.shape[1]))
     try:
@@ -509,3 +511,59 @@ def electron_sf_and_derivs(crystal, params, atoms, lcs, hkl, dtdf, include=None,
         return sf_and_derivs(crystal, params, atoms, lcs, hkl, dtdf, include, n_threads, mode, anomalous)
     finally:
         lib().hc_oracle_set_electron(ctypes.c_void_p(0), ctypes.c_void_p(0))
+
+
+# ------------------------------------------------------------------------------ local coordinate systems
+def lcs_matrix(cell, xyz_frac, d):
+    """LocalCoordinateSystemCalculator::calculate restated (LocalCoordinateSystemCalculator.cpp:54-238,
+    CrossProductLcs.cpp:10-62): the local axes (as matrix columns) of definition d (a
+    discamb_amd.LcsDefinition) for atoms at fractional positions xyz_frac."""
+    f2c = cell_f2c(cell)
+    xyz_frac = np.asarray(xyz_frac, dtype=np.float64)
+
+    def pos(ref):
+        R, t = parse_op(ref[1])
+        return f2c @ (R @ xyz_frac[ref[0]] + t)
+
+    r0 = pos((d.central, d.central_op))
+
+    def direction(refs, kind):
+        if kind == "average_direction":
+            v = np.zeros(3)
+            for r in refs:
+                u = pos(r) - r0
+                v += u / np.sqrt(u @ u)
+        elif kind == "average_position":
+            v = np.mean([pos(r) for r in refs], axis=0) - r0
+        else:
+            raise ValueError(kind)
+        return v / np.sqrt(v @ v)
+
+    def any_orthogonal(v):
+        rx, ry = np.cross(v, [1.0, 0, 0]), np.cross(v, [0, 1.0, 0])
+        r = rx if rx @ rx > ry @ ry else ry
+        return r / np.sqrt(r @ r)
+
+    t1, t2 = d.direction1_type, d.direction2_type
+    if t1 == "any_orthogonal" and t2 == "any_orthogonal":
+        return np.eye(3)
+    if t1 == "any_orthogonal":
+        r2 = direction(d.ref2, t2)
+        r1 = any_orthogonal(r2)
+    else:
+        r1 = direction(d.ref1, t1)
+        r2 = any_orthogonal(r1) if t2 == "any_orthogonal" else direction(d.ref2, t2)
+    c1, c2 = d.coordinate_1 + 1, d.coordinate_2 + 1
+    c3 = 6 - c1 - c2
+    s = (1 - 2 * ((2 + c2 - c1) % 3)) * (1.0 if d.is_r else -1.0)
+    a0 = r1 / np.sqrt(r1 @ r1)
+    a2 = s * np.cross(a0, r2)
+    a2 /= np.sqrt(a2 @ a2)
+    a1 = s * np.cross(a2, a0)
+    axes = [None, None, None]
+    axes[c1 - 1], axes[c2 - 1], axes[c3 - 1] = a0, a1, a2
+    if len(d.chirality) == 3:
+        p = [pos(r) - r0 for r in d.chirality]
+        if np.cross(p[0], p[1]) @ p[2] < 0:
+            axes[3 - d.coordinate_1 - d.coordinate_2] = -axes[3 - d.coordinate_1 - d.coordinate_2]
+    return np.stack(axes, 1)

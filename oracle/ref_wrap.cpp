@@ -9,6 +9,7 @@
 #include "discamb/MathUtilities/SphConverter.h"
 #include "discamb/MathUtilities/NaturalCubicSpline.h"
 #include "discamb/Scattering/NGaussianFormFactorsTable.h"
+#include "discamb/AtomTyping/LocalCoordinateSystemCalculator.h"
 #include "discamb/HC_Model/HC_WfnData.h"
 #include "discamb/HC_Model/ClementiRoettiData.h"
 #include "discamb/HC_Model/DeformationValenceParameters.h"
@@ -42,6 +43,38 @@ int ref_ngauss(const char* label, const char* table, const double* h, int m, dou
     NGaussianFormFactor f = n_gaussian_form_factors_table::getFormFactor(label, table);
     for (int i = 0; i < m; i++) out[i] = f.calculate_h(h[i]);
     return 1;
+}
+
+// LocalCoordinateSystemCalculator::calculate (LocalCoordinateSystemCalculator.cpp:194-238) for one
+// definition: atoms at fractional xyz; reference atoms are (index, operation string) pairs; direction
+// types 0 ANY_ORTHOGONAL, 1 AVERAGE_DIRECTION, 2 AVERAGE_POSITION; m9 = local axes as columns.
+int ref_lcs(const double* cell6, int n_atoms, const double* xyz, int central, const char* central_op, int t1,
+            int n1, const int* r1, const char* const* op1, int t2, int n2, const int* r2, const char* const* op2,
+            int c1, int c2, int is_r, int nch, const int* ch, const char* const* opch, double* m9)
+{
+    Crystal c;
+    c.unitCell.set(cell6[0], cell6[1], cell6[2], cell6[3], cell6[4], cell6[5]);
+    c.atoms.resize(n_atoms);
+    for (int i = 0; i < n_atoms; i++) c.atoms[i].coordinates = Vector3d(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    const LcsDirectionType types[3] = {LcsDirectionType::ANY_ORTHOGONAL, LcsDirectionType::AVERAGE_DIRECTION,
+                                       LcsDirectionType::AVERAGE_POSITION};
+    LocalCoordinateSystem<AtomInCrystalID> lcs;
+    lcs.centralAtom = AtomInCrystalID(central, SpaceGroupOperation(std::string(central_op)));
+    lcs.direction1_type = types[t1];
+    lcs.direction2_type = types[t2];
+    for (int i = 0; i < n1; i++) lcs.refPoint_1.push_back(AtomInCrystalID(r1[i], SpaceGroupOperation(std::string(op1[i]))));
+    for (int i = 0; i < n2; i++) lcs.refPoint_2.push_back(AtomInCrystalID(r2[i], SpaceGroupOperation(std::string(op2[i]))));
+    for (int i = 0; i < nch; i++) lcs.chirality.push_back(AtomInCrystalID(ch[i], SpaceGroupOperation(std::string(opch[i]))));
+    lcs.coordinate_1 = c1;
+    lcs.coordinate_2 = c2;
+    lcs.isR = is_r != 0;
+    LocalCoordinateSystemCalculator calc(lcs, c);
+    Matrix3d m;
+    bool same = true;
+    calc.calculate(m, c, same);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) m9[3 * i + j] = m(i, j);
+    return same ? 1 : 0;
 }
 
 // sto_scattering::gFunction<l>(n, h, Z)   (SlaterTypeOrbitalScattering.h:51-183)

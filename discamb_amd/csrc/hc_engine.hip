@@ -2056,40 +2056,48 @@ void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const doub
     d_atoms_.ensure(ain.size() * sizeof(AtomRec));
     // Reflections are processed in order of increasing |h| (stable; hkl_perm_[i] = caller index of
     // position i): consecutive reflections of a wave then read neighbouring nodes of the radial
-    // tables.  |h| range for the form-factor tables.
-    std::vector<double> h2(n_hkl);
-    for (int k = 0; k < n_hkl; k++) {
-        double v = 0.0;
-        for (int j = 0; j < 3; j++) {
-            const double hc = hkl[3 * k] * c2f_[j] + hkl[3 * k + 1] * c2f_[3 + j] + hkl[3 * k + 2] * c2f_[6 + j];
-            v += hc * hc;
+    // tables.  |h| range for the form-factor tables.  A refinement passes the same list every
+    // call: then the order, the tables' range and the device copy of the indices are kept.
+    const bool same_hkl = sort_hkl == last_sorted_ && (size_t)3 * n_hkl == last_hkl_.size() &&
+                          (n_hkl == 0 || std::memcmp(last_hkl_.data(), hkl, (size_t)3 * n_hkl * sizeof(int)) == 0);
+    d_hkl_.ensure(std::max(n_hkl, 1) * 3 * sizeof(int));
+    d_dtdf_.ensure(std::max(n_hkl, 1) * 3 * sizeof(double));   // dtdf (2) + mult (1)
+    std::vector<int> hkl_s;
+    if (!same_hkl) {
+        std::vector<std::pair<double, int>> key(n_hkl);
+        for (int k = 0; k < n_hkl; k++) {
+            double v = 0.0;
+            for (int j = 0; j < 3; j++) {
+                const double hc = hkl[3 * k] * c2f_[j] + hkl[3 * k + 1] * c2f_[3 + j] + hkl[3 * k + 2] * c2f_[6 + j];
+                v += hc * hc;
+            }
+            key[k] = {v, k};
         }
-        h2[k] = v;
+        h_max_ = 0.0;
+        for (int k = 0; k < n_hkl; k++) h_max_ = std::max(h_max_, std::sqrt(key[k].first));
+        if (sort_hkl) std::sort(key.begin(), key.end());     // (|h|^2, index): the stable order
+        hkl_perm_.resize(n_hkl);
+        for (int k = 0; k < n_hkl; k++) hkl_perm_[k] = key[k].second;
+        plan_tables();
+        hkl_s.resize(3 * (size_t)std::max(n_hkl, 1));
+        for (int i = 0; i < n_hkl; i++)
+            for (int j = 0; j < 3; j++) hkl_s[3 * i + j] = hkl[3 * hkl_perm_[i] + j];
+        last_hkl_.assign(hkl, hkl + 3 * (size_t)n_hkl);
+        last_sorted_ = sort_hkl;
+        if (n_hkl > 0)
+            HIP_CHECK(hipMemcpyAsync(d_hkl_.p, hkl_s.data(), (size_t)n_hkl * 3 * sizeof(int), hipMemcpyHostToDevice,
+                                     stream_));
     }
-    hkl_perm_.resize(n_hkl);
-    for (int k = 0; k < n_hkl; k++) hkl_perm_[k] = k;
-    if (sort_hkl)
-        std::stable_sort(hkl_perm_.begin(), hkl_perm_.end(), [&](int x, int y) { return h2[x] < h2[y]; });
-    h_max_ = 0.0;
-    for (int k = 0; k < n_hkl; k++) h_max_ = std::max(h_max_, std::sqrt(h2[k]));
-    plan_tables();
-    std::vector<int> hkl_s(3 * (size_t)std::max(n_hkl, 1));
     std::vector<double> dtdf_s(3 * (size_t)std::max(n_hkl, 1));
     for (int i = 0; i < n_hkl; i++) {
         const int k = hkl_perm_[i];
-        for (int j = 0; j < 3; j++) hkl_s[3 * i + j] = hkl[3 * k + j];
         dtdf_s[2 * i] = dtdf[2 * k];
         dtdf_s[2 * i + 1] = dtdf[2 * k + 1];
         dtdf_s[2 * (size_t)n_hkl + i] = dtdf[2 * (size_t)n_hkl + k];
     }
-    d_hkl_.ensure(std::max(n_hkl, 1) * 3 * sizeof(int));
-    d_dtdf_.ensure(std::max(n_hkl, 1) * 3 * sizeof(double));   // dtdf (2) + mult (1)
-    if (n_hkl > 0) {
-        HIP_CHECK(hipMemcpyAsync(d_hkl_.p, hkl_s.data(), (size_t)n_hkl * 3 * sizeof(int), hipMemcpyHostToDevice,
-                                 stream_));
+    if (n_hkl > 0)
         HIP_CHECK(hipMemcpyAsync(d_dtdf_.p, dtdf_s.data(), (size_t)n_hkl * 3 * sizeof(double), hipMemcpyHostToDevice,
                                  stream_));
-    }
     // keep host staging alive until the copies complete
     HIP_CHECK(hipStreamSynchronize(stream_));
 }

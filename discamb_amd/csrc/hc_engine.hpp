@@ -173,6 +173,8 @@ private:
     bool ran_ = false;
     std::vector<int> order_;        // sorted slot -> caller atom index (-1 padding)
     std::vector<int> hkl_perm_;     // device reflection position -> caller reflection index
+    std::vector<int> last_hkl_;     // reflections of the last upload (unchanged lists skip the sort)
+    bool last_sorted_ = false;
     std::vector<int> tab_nodes_;    // radial-table node count per wavefunction
     // model tables
     DeviceBuffer d_core_, d_val_, d_type_, d_ops_;

@@ -672,7 +672,7 @@ __device__ __forceinline__ void contract(const R* P, const R* Y, R* A)
     }
 }
 
-template <int L, bool DERIV, bool MP, bool OTF>
+template <int L, bool DERIV, int MP, bool OTF>
 __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane, int chunk_id, int k0, int k1,
                                           const unsigned* __restrict__ hop, const double* __restrict__ rad,
                                           const double* __restrict__ radx, const unsigned* __restrict__ dtdf32,
@@ -680,6 +680,8 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
                                           double* __restrict__ dpart)
 {
     constexpr int NLM = (L + 1) * (L + 1);
+    constexpr bool SC = MP == 1;    // multipole pass 1: F, xyz/U/occ and the Pval, kappa, kappa' sums
+    constexpr bool BO = MP == 2;    // multipole pass 2: the global-frame B_lm sums only
     const int slot = wave * kWave + lane;
     const AtomRec& at = cmem(a.atoms)[slot];
     const double x0 = at.xf[0], x1 = at.xf[1], x2 = at.xf[2];
@@ -702,9 +704,9 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
     const double* tab = cmem(static_cast<const double*>(a.tab));
     const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
     double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
-    double B[MP ? NLM : 1];
+    double B[BO ? NLM : 1];
 #pragma unroll
-    for (int j = 0; j < (MP ? NLM : 1); j++) B[j] = 0;
+    for (int j = 0; j < (BO ? NLM : 1); j++) B[j] = 0;
     double dpv = 0, dka = 0, dkp = 0;
     for (int kb = k0; kb < k1; kb += kStage) {
     const int nk = min(kStage, k1 - kb);
@@ -720,7 +722,7 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
             S = rr[0] + anr; G0 = rr[1]; G1 = rr[2]; G2 = rr[3]; G3 = rr[4]; G4 = rr[5];
         }
         double fval = 0, dSk = 0, dG[5] = {0, 0, 0, 0, 0};
-        if (MP) {
+        if (SC) {
             const double* rx = radx + ((size_t)k * a.n_types + t) * kRadRec;
             fval = rx[0];
             dSk = rx[1];
@@ -734,13 +736,16 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
             double r[kHklopRec];
 #pragma unroll
             for (int j = 0; j < kHklopRec; j++) r[j] = ld_f64(rw, j);
-            double A[5];
-            contract<L, double>(P, r, A);
-            double fr = S + G0 * A[0] + G2 * A[2];
-            double fi = ani * r[35] + G1 * A[1];
-            if (L == 4) {
-                fr += G4 * A[4];
-                fi += G3 * A[3];
+            double A[5] = {0, 0, 0, 0, 0};
+            double fr = 0, fi = 0;
+            if (!BO) {
+                contract<L, double>(P, r, A);
+                fr = S + G0 * A[0] + G2 * A[2];
+                fi = ani * r[35] + G1 * A[1];
+                if (L == 4) {
+                    fr += G4 * A[4];
+                    fi += G3 * A[3];
+                }
             }
             // h.(R x + t) reduced to [0, 1) revolutions
             const double ph = __builtin_amdgcn_fract(fma(r[33], x2, fma(r[32], x1, fma(r[31], x0, r[34]))));
@@ -749,47 +754,56 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
             double e = r[25] * u[0] + r[26] * u[1] + r[27] * u[2] + r[28] * u[3] + r[29] * u[4] + r[30] * u[5];
             double T = exp(-e);
             double Er = T * cs, Ei = T * sn;
-            double Xr = fr * Er - fi * Ei, Xi = fr * Ei + fi * Er;
-            sr += Xr;
-            si += Xi;
-            if (DERIV) {
-                double zr = cr * Xr - ci * Xi, zi = cr * Xi + ci * Xr;
+            if (!BO) {
+                double Xr = fr * Er - fi * Ei, Xi = fr * Ei + fi * Er;
+                sr += Xr;
+                si += Xi;
+                if (DERIV) {
+                    double zr = cr * Xr - ci * Xi, zi = cr * Xi + ci * Xr;
 #pragma unroll
-                for (int j = 0; j < 6; j++) du[j] -= r[25 + j] * zr;
+                    for (int j = 0; j < 6; j++) du[j] -= r[25 + j] * zr;
 #pragma unroll
-                for (int j = 0; j < 3; j++) dx[j] -= r[31 + j] * zi;
+                    for (int j = 0; j < 3; j++) dx[j] -= r[31 + j] * zi;
+                }
             }
             if (MP) {
                 double cer = cr * Er - ci * Ei, cei = cr * Ei + ci * Er;
-                Rh += cer;
-                double q[5] = {G0 * cer, -G1 * cei, G2 * cer, -G3 * cei, G4 * cer};
+                if (SC) {
+                    Rh += cer;
+                    dkp += (dG[0] * A[0] + dG[2] * A[2] + dG[4] * A[4]) * cer - (dG[1] * A[1] + dG[3] * A[3]) * cei;
+                } else {
+                    double q[5] = {G0 * cer, -G1 * cei, G2 * cer, -G3 * cei, G4 * cer};
 #pragma unroll
-                for (int j = 0; j < NLM; j++) {
-                    const int l = j < 1 ? 0 : j < 4 ? 1 : j < 9 ? 2 : j < 16 ? 3 : 4;
-                    B[j] += q[l] * r[j];
+                    for (int j = 0; j < NLM; j++) {
+                        const int l = j < 1 ? 0 : j < 4 ? 1 : j < 9 ? 2 : j < 16 ? 3 : 4;
+                        B[j] += q[l] * r[j];
+                    }
                 }
-                dkp += (dG[0] * A[0] + dG[2] * A[2] + dG[4] * A[4]) * cer - (dG[1] * A[1] + dG[3] * A[3]) * cei;
             }
         }
-        if (DERIV) docc += cr * sr - ci * si;
-        if (MP) {
+        if (DERIV && !BO) docc += cr * sr - ci * si;
+        if (SC) {
             dpv += fval * Rh;
             dka += dSk * Rh;
         }
-        stage_f64(stage, kk, lane, w * sr, w * si);
+        if (!BO) stage_f64(stage, kk, lane, w * sr, w * si);
     }
-    flush_f64(stage, nk, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
+    if (!BO) flush_f64(stage, nk, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
     }
     if (DERIV) {
         constexpr int ND = MP ? kNdMultipole : kNdBasic;
         double* dp = dpart + ((size_t)chunk_id * a.n_slots + slot) * ND;
-        dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
+        if (!BO) {
+            dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
 #pragma unroll
-        for (int j = 0; j < 6; j++) dp[3 + j] = du[j];
-        dp[9] = docc;
-        if (MP) {
+            for (int j = 0; j < 6; j++) dp[3 + j] = du[j];
+            dp[9] = docc;
+        }
+        if (BO) {
 #pragma unroll
             for (int j = 0; j < 25; j++) dp[10 + j] = j < NLM ? B[j < NLM ? j : 0] : 0.0;
+        }
+        if (SC) {
             dp[35] = dpv;
             dp[36] = dka;
             dp[37] = dkp;
@@ -799,7 +813,7 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
 
 // Mixed precision: FP64 phase range reduction and FP64 accumulation across reflections, FP32
 // harmonics / radial / Debye-Waller / sincos / exp per term.
-template <int L, bool DERIV, bool MP, bool OTF>
+template <int L, bool DERIV, int MP, bool OTF>
 __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane, int chunk_id, int k0, int k1,
                                            const float* __restrict__ hop, const float* __restrict__ rad,
                                            const float* __restrict__ radx, const unsigned* __restrict__ phs32,
@@ -808,6 +822,8 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
 {
     constexpr int NLM = (L + 1) * (L + 1);
     constexpr int kFlush = kStage;    // reflections accumulated in FP32 before an FP64 flush
+    constexpr bool SC = MP == 1;      // multipole pass 1: F, xyz/U/occ and the Pval, kappa, kappa' sums
+    constexpr bool BO = MP == 2;      // multipole pass 2: the global-frame B_lm sums only
     const int slot = wave * kWave + lane;
     const AtomRec& at = cmem(a.atoms)[slot];
     const double x0 = at.xf[0], x1 = at.xf[1], x2 = at.xf[2];
@@ -830,16 +846,16 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
     const float* tab = cmem(static_cast<const float*>(a.tab));
     const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
     double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
-    double B[MP ? NLM : 1];
+    double B[BO ? NLM : 1];
 #pragma unroll
-    for (int j = 0; j < (MP ? NLM : 1); j++) B[j] = 0;
+    for (int j = 0; j < (BO ? NLM : 1); j++) B[j] = 0;
     double dpv = 0, dka = 0, dkp = 0;
     for (int kb = k0; kb < k1; kb += kFlush) {
         const int kend = min(kb + kFlush, k1);
         float fx[3] = {0, 0, 0}, fu[6] = {0, 0, 0, 0, 0, 0}, focc = 0;
-        float fB[MP ? NLM : 1];
+        float fB[BO ? NLM : 1];
 #pragma unroll
-        for (int j = 0; j < (MP ? NLM : 1); j++) fB[j] = 0;
+        for (int j = 0; j < (BO ? NLM : 1); j++) fB[j] = 0;
         float fpv = 0, fka = 0, fkp = 0;
         for (int k = kb; k < kend; ++k) {
             const int kk = k - kb;
@@ -853,7 +869,7 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
                 S = rr[0] + anr; G0 = rr[1]; G1 = rr[2]; G2 = rr[3]; G3 = rr[4]; G4 = rr[5];
             }
             float fval = 0, dSk = 0, dG[5] = {0, 0, 0, 0, 0};
-            if (MP) {
+            if (SC) {
                 const float* rx = radx + ((size_t)k * a.n_types + t) * kRadRec;
                 fval = rx[0];
                 dSk = rx[1];
@@ -865,13 +881,16 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
             for (int s = 0; s < a.n_ops; ++s) {
                 const float* r = hop + ((size_t)k * a.n_ops + s) * kHklopRec;
                 const unsigned* pr = phs32 + ((size_t)k * a.n_ops + s) * 8;
-                float A[5];
-                contract<L, float>(P, r, A);
-                float fr = S + G0 * A[0] + G2 * A[2];
-                float fi = ani * r[35] + G1 * A[1];
-                if (L == 4) {
-                    fr += G4 * A[4];
-                    fi += G3 * A[3];
+                float A[5] = {0, 0, 0, 0, 0};
+                float fr = 0, fi = 0;
+                if (!BO) {
+                    contract<L, float>(P, r, A);
+                    fr = S + G0 * A[0] + G2 * A[2];
+                    fi = ani * r[35] + G1 * A[1];
+                    if (L == 4) {
+                        fr += G4 * A[4];
+                        fi += G3 * A[3];
+                    }
                 }
                 // FP64 range reduction of h.(R x + t) to [0, 1) revolutions
                 const double ph = fma(ld_f64(pr, 2), x2, fma(ld_f64(pr, 1), x1, fma(ld_f64(pr, 0), x0, ld_f64(pr, 3))));
@@ -880,63 +899,75 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
                 float e = r[25] * u[0] + r[26] * u[1] + r[27] * u[2] + r[28] * u[3] + r[29] * u[4] + r[30] * u[5];
                 float T = __builtin_amdgcn_exp2f(-e);
                 float Er = T * cs, Ei = T * sn;
-                float Xr = fr * Er - fi * Ei, Xi = fr * Ei + fi * Er;
-                sr += Xr;
-                si += Xi;
-                if (DERIV) {
-                    float zr = cr * Xr - ci * Xi, zi = cr * Xi + ci * Xr;
+                if (!BO) {
+                    float Xr = fr * Er - fi * Ei, Xi = fr * Ei + fi * Er;
+                    sr += Xr;
+                    si += Xi;
+                    if (DERIV) {
+                        float zr = cr * Xr - ci * Xi, zi = cr * Xi + ci * Xr;
 #pragma unroll
-                    for (int j = 0; j < 6; j++) fu[j] -= r[25 + j] * zr;
+                        for (int j = 0; j < 6; j++) fu[j] -= r[25 + j] * zr;
 #pragma unroll
-                    for (int j = 0; j < 3; j++) fx[j] -= r[31 + j] * zi;
+                        for (int j = 0; j < 3; j++) fx[j] -= r[31 + j] * zi;
+                    }
                 }
                 if (MP) {
                     float cer = cr * Er - ci * Ei, cei = cr * Ei + ci * Er;
-                    Rh += cer;
-                    float q[5] = {G0 * cer, -G1 * cei, G2 * cer, -G3 * cei, G4 * cer};
+                    if (SC) {
+                        Rh += cer;
+                        fkp += (dG[0] * A[0] + dG[2] * A[2] + dG[4] * A[4]) * cer -
+                               (dG[1] * A[1] + dG[3] * A[3]) * cei;
+                    } else {
+                        float q[5] = {G0 * cer, -G1 * cei, G2 * cer, -G3 * cei, G4 * cer};
 #pragma unroll
-                    for (int j = 0; j < NLM; j++) {
-                        const int l = j < 1 ? 0 : j < 4 ? 1 : j < 9 ? 2 : j < 16 ? 3 : 4;
-                        fB[j] += q[l] * r[j];
+                        for (int j = 0; j < NLM; j++) {
+                            const int l = j < 1 ? 0 : j < 4 ? 1 : j < 9 ? 2 : j < 16 ? 3 : 4;
+                            fB[j] += q[l] * r[j];
+                        }
                     }
-                    fkp += (dG[0] * A[0] + dG[2] * A[2] + dG[4] * A[4]) * cer - (dG[1] * A[1] + dG[3] * A[3]) * cei;
                 }
             }
-            if (DERIV) focc += cr * sr - ci * si;
-            if (MP) {
+            if (DERIV && !BO) focc += cr * sr - ci * si;
+            if (SC) {
                 fpv += fval * Rh;
                 fka += dSk * Rh;
             }
-            stage_f32(stage, kk, lane, w * sr, w * si);
+            if (!BO) stage_f32(stage, kk, lane, w * sr, w * si);
         }
-        if (DERIV) {
+        if (DERIV && !BO) {
 #pragma unroll
             for (int j = 0; j < 3; j++) dx[j] += fx[j];
 #pragma unroll
             for (int j = 0; j < 6; j++) du[j] += fu[j];
             docc += focc;
         }
-        if (MP) {
+        if (BO) {
 #pragma unroll
             for (int j = 0; j < NLM; j++) B[j] += fB[j];
+        }
+        if (SC) {
             dpv += fpv;
             dka += fka;
             dkp += fkp;
         }
-        flush_f32(stage, kend - kb, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
+        if (!BO) flush_f32(stage, kend - kb, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
     }
     if (DERIV) {
         // U multipliers carried log2(e); undo for the derivative
         const double inv_log2e = 0.69314718055994530942;
         constexpr int ND = MP ? kNdMultipole : kNdBasic;
         double* dp = dpart + ((size_t)chunk_id * a.n_slots + slot) * ND;
-        dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
+        if (!BO) {
+            dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
 #pragma unroll
-        for (int j = 0; j < 6; j++) dp[3 + j] = du[j] * inv_log2e;
-        dp[9] = docc;
-        if (MP) {
+            for (int j = 0; j < 6; j++) dp[3 + j] = du[j] * inv_log2e;
+            dp[9] = docc;
+        }
+        if (BO) {
 #pragma unroll
             for (int j = 0; j < 25; j++) dp[10 + j] = j < NLM ? B[j < NLM ? j : 0] : 0.0;
+        }
+        if (SC) {
             dp[35] = dpv;
             dp[36] = dka;
             dp[37] = dkp;
@@ -969,6 +1000,14 @@ __host__ __device__ constexpr bool class_present(int l, int key)
 {
     for (int j = l * l; j < (l + 1) * (l + 1); j++)
         if (harmonic_parity(j) == key) return true;
+    return false;
+}
+
+template <int L>
+__host__ __device__ constexpr bool class_used(int key)
+{
+    for (int l = 0; l <= L; l++)
+        if (class_present(l, key)) return true;
     return false;
 }
 
@@ -1010,14 +1049,16 @@ struct DiagRefl {
     R C[5][8];             // parity-class sums of P_j d_j(h)
     R D, O3, O4, O5;       // DW exponent: diagonal part and the xy, xz, yz cross terms
     R S, G0, G1, G2, G3, G4, cr, ci;
+    R dG[5];               // d G_l / d kappa' (first multipole pass only)
 };
 
-// Mixed precision: one operation with sign pattern PAT.
-template <int L, bool DERIV, int PAT>
+// Mixed precision: one operation with sign pattern PAT.  SC adds the sums of the first
+// multipole pass: Rh = sum Re(c E) and the kappa' sum.
+template <int L, bool DERIV, int PAT, bool SC = false>
 __device__ __forceinline__ void diag_term_mixed(const DiagRefl<float>& q, const float* __restrict__ r, double h0,
                                                 double h1, double h2, double ht, float an, float ani, double x0,
                                                 double x1, double x2, float& sr, float& si, float (&fx)[3],
-                                                float (&fu)[6])
+                                                float (&fu)[6], float& Rh, float& fkp)
 {
     constexpr double sx = (PAT & 1) ? -1.0 : 1.0, sy = (PAT & 2) ? -1.0 : 1.0, sz = (PAT & 4) ? -1.0 : 1.0;
     constexpr float fxy = float(sx * sy), fxz = float(sx * sz), fyz = float(sy * sz);
@@ -1050,16 +1091,48 @@ __device__ __forceinline__ void diag_term_mixed(const DiagRefl<float>& q, const 
         fx[1] -= (float(sy) * r[32]) * zi;
         fx[2] -= (float(sz) * r[33]) * zi;
     }
+    if (SC) {
+        const float cer = q.cr * Er - q.ci * Ei, cei = q.cr * Ei + q.ci * Er;
+        Rh += cer;
+        fkp += (q.dG[0] * A[0] + q.dG[2] * A[2] + q.dG[4] * A[4]) * cer - (q.dG[1] * A[1] + q.dG[3] * A[3]) * cei;
+    }
 }
 
-template <int L, bool DERIV, bool OTF>
+// Second multipole pass, one operation with sign pattern PAT: V[c] += sign_c(PAT) Re(c E) for the
+// even parity classes (even l) and sign_c(PAT) Im(c E) for the odd ones, so that afterwards
+// B_j += G'_l d_j(h) V[class(j)] sums every operation at once.
+template <int L, int PAT>
+__device__ __forceinline__ void diag_b_term(const DiagRefl<float>& q, double h0, double h1, double h2, double ht,
+                                            double x0, double x1, double x2, float (&V)[8])
+{
+    constexpr double sx = (PAT & 1) ? -1.0 : 1.0, sy = (PAT & 2) ? -1.0 : 1.0, sz = (PAT & 4) ? -1.0 : 1.0;
+    constexpr float fxy = float(sx * sy), fxz = float(sx * sz), fyz = float(sy * sz);
+    const double ph = __builtin_amdgcn_fract(fma(sz * h2, x2, fma(sy * h1, x1, fma(sx * h0, x0, ht))));
+    const float phf = (float)ph;
+    const float sn = __builtin_amdgcn_sinf(phf), cs = __builtin_amdgcn_cosf(phf);
+    const float e = q.D + fxy * q.O3 + fxz * q.O4 + fyz * q.O5;
+    const float T = __builtin_amdgcn_exp2f(-e);
+    const float Er = T * cs, Ei = T * sn;
+    const float cer = q.cr * Er - q.ci * Ei, cei = q.cr * Ei + q.ci * Er;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        if (!class_used<L>(c)) continue;
+        const float v = (__builtin_popcount((unsigned)c) & 1) ? cei : cer;
+        V[c] = odd_under(c, PAT) ? V[c] - v : V[c] + v;
+    }
+}
+
+template <int L, bool DERIV, int MP, bool OTF>
 __device__ __forceinline__ void body_mixed_diag(const MainArgs& a, int wave, int lane, int chunk_id, int k0, int k1,
                                                 const float* __restrict__ hrec, const float* __restrict__ rad,
-                                                const unsigned* __restrict__ hd32, const unsigned* __restrict__ ht32,
-                                                const unsigned* __restrict__ dtdf32, float2* __restrict__ stage,
-                                                double* __restrict__ fpart, double* __restrict__ dpart)
+                                                const float* __restrict__ radx, const unsigned* __restrict__ hd32,
+                                                const unsigned* __restrict__ ht32, const unsigned* __restrict__ dtdf32,
+                                                float2* __restrict__ stage, double* __restrict__ fpart,
+                                                double* __restrict__ dpart)
 {
     constexpr int NLM = (L + 1) * (L + 1);
+    constexpr bool SC = MP == 1;    // multipole pass 1: F, xyz/U/occ and the Pval, kappa, kappa' sums
+    constexpr bool BO = MP == 2;    // multipole pass 2: the global-frame B_lm sums only
     const int slot = wave * kWave + lane;
     const AtomRec& at = cmem(a.atoms)[slot];
     const double x0 = at.xf[0], x1 = at.xf[1], x2 = at.xf[2];
@@ -1082,9 +1155,17 @@ __device__ __forceinline__ void body_mixed_diag(const MainArgs& a, int wave, int
     const float* tab = cmem(static_cast<const float*>(a.tab));
     const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
     double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
+    double B[BO ? NLM : 1];
+#pragma unroll
+    for (int j = 0; j < (BO ? NLM : 1); j++) B[j] = 0;
+    double dpv = 0, dka = 0, dkp = 0;
     for (int kb = k0; kb < k1; kb += kStage) {
         const int kend = min(kb + kStage, k1);
         float fx[3] = {0, 0, 0}, fu[6] = {0, 0, 0, 0, 0, 0}, focc = 0;
+        float fB[BO ? NLM : 1];
+#pragma unroll
+        for (int j = 0; j < (BO ? NLM : 1); j++) fB[j] = 0;
+        float fpv = 0, fka = 0, fkp = 0;
         for (int k = kb; k < kend; ++k) {
             const float* r = hrec + (size_t)k * kHklopRec;
             DiagRefl<float> q;
@@ -1101,49 +1182,95 @@ __device__ __forceinline__ void body_mixed_diag(const MainArgs& a, int wave, int
                 q.G3 = rr[4];
                 q.G4 = rr[5];
             }
+            float fval = 0, dSk = 0;
+#pragma unroll
+            for (int l = 0; l < 5; l++) q.dG[l] = 0;
+            if (SC) {
+                const float* rx = radx + ((size_t)k * a.n_types + t) * kRadRec;
+                fval = rx[0];
+                dSk = rx[1];
+#pragma unroll
+                for (int l = 0; l < 5; l++) q.dG[l] = rx[2 + l];
+            }
             q.cr = (float)ld_f64(dtdf32, 2 * k);
             q.ci = (float)ld_f64(dtdf32, 2 * k + 1);
-            class_sums<L, float>(P, r, q.C);
+            if (!BO) class_sums<L, float>(P, r, q.C);
             q.D = r[25] * u[0] + r[26] * u[1] + r[27] * u[2];
             q.O3 = r[28] * u[3];
             q.O4 = r[29] * u[4];
             q.O5 = r[30] * u[5];
             const unsigned* hd = hd32 + (size_t)k * 8;
             const double h0 = ld_f64(hd, 0), h1 = ld_f64(hd, 1), h2 = ld_f64(hd, 2);
-            float sr = 0, si = 0;
             const unsigned* htk = ht32 + (size_t)k * a.n_ops * 2;
-            // one uniform block per sign pattern present in the group (each at most once)
+            if (BO) {
+                float V[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#define DIAG_OP(PAT)                                                                                          \
+    if (a.diag_mask & (1 << PAT)) diag_b_term<L, PAT>(q, h0, h1, h2, ld_f64(htk, a.diag_op[PAT]), x0, x1, x2, V);
+                DIAG_OP(0) DIAG_OP(1) DIAG_OP(2) DIAG_OP(3) DIAG_OP(4) DIAG_OP(5) DIAG_OP(6) DIAG_OP(7)
+#undef DIAG_OP
+                const float Gs[5] = {q.G0, -q.G1, q.G2, -q.G3, q.G4};
+#pragma unroll
+                for (int j = 0; j < NLM; j++) fB[j] += (Gs[harmonic_l(j)] * r[j]) * V[harmonic_parity(j)];
+            } else {
+                float sr = 0, si = 0, Rh = 0;
+                // one uniform block per sign pattern present in the group (each at most once)
 #define DIAG_OP(PAT)                                                                                          \
     if (a.diag_mask & (1 << PAT))                                                                             \
-        diag_term_mixed<L, DERIV, PAT>(q, r, h0, h1, h2, ld_f64(htk, a.diag_op[PAT]), a.diag_anom[PAT], ani,  \
-                                       x0, x1, x2, sr, si, fx, fu);
-            DIAG_OP(0) DIAG_OP(1) DIAG_OP(2) DIAG_OP(3) DIAG_OP(4) DIAG_OP(5) DIAG_OP(6) DIAG_OP(7)
+        diag_term_mixed<L, DERIV, PAT, SC>(q, r, h0, h1, h2, ld_f64(htk, a.diag_op[PAT]), a.diag_anom[PAT], ani, \
+                                           x0, x1, x2, sr, si, fx, fu, Rh, fkp);
+                DIAG_OP(0) DIAG_OP(1) DIAG_OP(2) DIAG_OP(3) DIAG_OP(4) DIAG_OP(5) DIAG_OP(6) DIAG_OP(7)
 #undef DIAG_OP
-            if (DERIV) focc += q.cr * sr - q.ci * si;
-            stage_f32(stage, k - kb, lane, w * sr, w * si);
+                if (DERIV) focc += q.cr * sr - q.ci * si;
+                if (SC) {
+                    fpv += fval * Rh;
+                    fka += dSk * Rh;
+                }
+                stage_f32(stage, k - kb, lane, w * sr, w * si);
+            }
         }
-        if (DERIV) {
+        if (DERIV && !BO) {
 #pragma unroll
             for (int j = 0; j < 3; j++) dx[j] += fx[j];
 #pragma unroll
             for (int j = 0; j < 6; j++) du[j] += fu[j];
             docc += focc;
         }
-        flush_f32(stage, kend - kb, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
+        if (BO) {
+#pragma unroll
+            for (int j = 0; j < NLM; j++) B[j] += fB[j];
+        }
+        if (SC) {
+            dpv += fpv;
+            dka += fka;
+            dkp += fkp;
+        }
+        if (!BO) flush_f32(stage, kend - kb, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
     }
     if (DERIV) {
         const double inv_log2e = 0.69314718055994530942;
-        double* dp = dpart + ((size_t)chunk_id * a.n_slots + slot) * kNdBasic;
-        dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
+        constexpr int ND = MP ? kNdMultipole : kNdBasic;
+        double* dp = dpart + ((size_t)chunk_id * a.n_slots + slot) * ND;
+        if (!BO) {
+            dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
 #pragma unroll
-        for (int j = 0; j < 6; j++) dp[3 + j] = du[j] * inv_log2e;
-        dp[9] = docc;
+            for (int j = 0; j < 6; j++) dp[3 + j] = du[j] * inv_log2e;
+            dp[9] = docc;
+        }
+        if (BO) {
+#pragma unroll
+            for (int j = 0; j < 25; j++) dp[10 + j] = j < NLM ? B[j < NLM ? j : 0] : 0.0;
+        }
+        if (SC) {
+            dp[35] = dpv;
+            dp[36] = dka;
+            dp[37] = dkp;
+        }
     }
 }
 
 // One (atom block, reflection chunk) work unit: wave wib of the block runs atom wave
 // ablock * kWavesPerBlock + wib over the reflections of chunk chunk_id.
-template <bool MIXED, bool DERIV, bool MP, bool DIAG, bool OTF>
+template <bool MIXED, bool DERIV, int MP, bool DIAG, bool OTF>
 __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int ablock, int wib, int lane, void* stage)
 {
     static_assert(MIXED || !DIAG, "the parity-folded kernels are mixed precision only");
@@ -1165,11 +1292,11 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
         float2* st = static_cast<float2*>(stage);
         if (DIAG) {
             if (cls <= 2)
-                body_mixed_diag<2, DERIV, OTF>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, ht32, dtdf32, st, fpart,
-                                          dpart);
+                body_mixed_diag<2, DERIV, MP, OTF>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, phs, ht32, dtdf32,
+                                                   st, fpart, dpart);
             else
-                body_mixed_diag<4, DERIV, OTF>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, ht32, dtdf32, st, fpart,
-                                          dpart);
+                body_mixed_diag<4, DERIV, MP, OTF>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, phs, ht32, dtdf32,
+                                                   st, fpart, dpart);
         } else if (cls <= 2) {
             body_mixed<2, DERIV, MP, OTF>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, phs, dtdf32, st, fpart, dpart);
         } else {
@@ -1192,7 +1319,10 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
 // kXcds * per_xcd workgroups: hardware id b runs on XCD b % kXcds and is mapped to logical block
 // (b % kXcds) * per_xcd + b / kXcds, so each XCD walks a contiguous range of chunks and every
 // record of a chunk is fetched into one L2 only.
-template <bool MIXED, bool DERIV, bool MP, bool DIAG, bool OTF>
+// MP: 0 no multipole adjoint; 1 and 2 its two passes over the same work units (1: F, the
+// xyz/U/occupancy derivatives and the Pval, kappa, kappa' sums; 2: the 25 global-frame B_lm sums),
+// split so that neither pass holds both the atom's P_lm and the B_lm accumulators.
+template <bool MIXED, bool DERIV, int MP, bool DIAG, bool OTF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && !MP ? (DIAG || OTF ? 4 : 5) : 1))) void k_main(MainArgs a)
 {
     constexpr int kStageBytes = MIXED ? kStageBytesF32 : kStageBytesF64;
@@ -2182,20 +2312,24 @@ void Engine::run_prep(Precision prec, bool mp, bool electron)
     HIP_CHECK(hipGetLastError());
 }
 
-static const void* main_kernel(bool mixed, bool derivs, bool mp, bool diag, bool otf)
+// mp_pass: 1 or 2 selects the pass of the multipole adjoint (0 without it)
+static const void* main_kernel(bool mixed, bool derivs, int mp_pass, bool diag, bool otf)
 {
 #define KM(M, D, P, G, O) (const void*)k_main<M, D, P, G, O>
-    if (mp) return mixed ? KM(true, true, true, false, false) : KM(false, true, true, false, false);
+    if (mp_pass == 1) return mixed ? (diag ? KM(true, true, 1, true, false) : KM(true, true, 1, false, false))
+                                   : KM(false, true, 1, false, false);
+    if (mp_pass == 2) return mixed ? (diag ? KM(true, true, 2, true, false) : KM(true, true, 2, false, false))
+                                   : KM(false, true, 2, false, false);
     if (mixed) {
         if (diag) {
-            if (otf) return derivs ? KM(true, true, false, true, true) : KM(true, false, false, true, true);
-            return derivs ? KM(true, true, false, true, false) : KM(true, false, false, true, false);
+            if (otf) return derivs ? KM(true, true, 0, true, true) : KM(true, false, 0, true, true);
+            return derivs ? KM(true, true, 0, true, false) : KM(true, false, 0, true, false);
         }
-        if (otf) return derivs ? KM(true, true, false, false, true) : KM(true, false, false, false, true);
-        return derivs ? KM(true, true, false, false, false) : KM(true, false, false, false, false);
+        if (otf) return derivs ? KM(true, true, 0, false, true) : KM(true, false, 0, false, true);
+        return derivs ? KM(true, true, 0, false, false) : KM(true, false, 0, false, false);
     }
-    if (otf) return derivs ? KM(false, true, false, false, true) : KM(false, false, false, false, true);
-    return derivs ? KM(false, true, false, false, false) : KM(false, false, false, false, false);
+    if (otf) return derivs ? KM(false, true, 0, false, true) : KM(false, false, 0, false, true);
+    return derivs ? KM(false, true, 0, false, false) : KM(false, false, 0, false, false);
 #undef KM
 }
 
@@ -2205,7 +2339,8 @@ int Engine::resident_waves(bool mixed, bool derivs, bool mp)
                     (mp ? 1 : 0);
     if (resident_waves_[key] > 0) return resident_waves_[key];
     int blocks = 0;
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, main_kernel(mixed, derivs, mp, use_diag(mixed, mp), use_otf(mp)),
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, main_kernel(mixed, derivs, mp ? 1 : 0, use_diag(mixed, mp),
+                                                                                use_otf(mp)),
                                                                256, 0));
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, device_));
@@ -2280,9 +2415,11 @@ void Engine::launch_main(Precision prec, bool derivs, bool mp, int chunk)
     last_grid_x = a.grid_x;
     last_grid_y = a.n_blocks / std::max(a.grid_x, 1);
     last_chunk = chunk;
-    const void* kern = main_kernel(prec == kMixed, derivs, mp, use_diag(prec == kMixed, mp), use_otf(mp));
     void* args[] = {&a};
-    HIP_CHECK(hipLaunchKernel(kern, grid, dim3(256), args, 0, stream_));
+    for (int pass = mp ? 1 : 0; pass <= (mp ? 2 : 0); pass++) {
+        const void* kern = main_kernel(prec == kMixed, derivs, pass, use_diag(prec == kMixed, mp), use_otf(mp));
+        HIP_CHECK(hipLaunchKernel(kern, grid, dim3(256), args, 0, stream_));
+    }
 }
 
 int Engine::chunk_for(Precision prec, bool derivs, bool mp)
@@ -2778,7 +2915,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 :
         const int local = u - off[b];
         const int chunk_id = local / a.grid_x;
         const int ablock = local - chunk_id * a.grid_x;
-        main_unit<MIXED, DERIV, false, DIAG, true>(a, chunk_id, ablock, wib, lane, stage);
+        main_unit<MIXED, DERIV, 0, DIAG, true>(a, chunk_id, ablock, wib, lane, stage);
     }
 }
 

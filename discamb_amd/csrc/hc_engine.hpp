@@ -146,7 +146,7 @@ private:
     int type_class(int t) const;
     // parity-folded kernels: mixed precision only (the FP64 variant needs twice the registers for
     // its class sums and loses more to occupancy than it saves)
-    bool use_diag(bool mixed, bool mp) const { return diag_ && mixed && !mp && !tune.general_path; }
+    bool use_diag(bool mixed, bool mp) const { return diag_ && mixed && !tune.general_path; }
     // radial factors evaluated per (atom, reflection) from the per-wavefunction tables instead of
     // per-(reflection, type) records: chosen when types are not shared by many atoms (the records
     // would then cost more to write and read than the lookups), never with the multipole adjoint

@@ -224,9 +224,15 @@ __device__ __forceinline__ T octet_sum(T v)
 // a step of kTabStep = 5e-4 / A the interpolation error is below 1e-13 of f(0) for H..S (measured
 // against the closed forms), far inside the FP64 parity bound.  Node i of wavefunction w sits at
 // s = i * kTabStep and holds 16 values {core f, f', valence f, f', G_0 f, f', ..., G_4 f, f', 0, 0}
-// (FP64 table, and an FP32 copy for the mixed-precision kernels).
+// (FP64 table).  The mixed-precision kernels read an FP32 table on a grid kLinRatio times finer
+// by linear interpolation instead: node i holds {f(s_i), f(s_i+1) - f(s_i)} for the same seven
+// functions, so a lookup is one FMA per function.  At 1.25e-4 / A the interpolation error for
+// H..S is <= 3e-8 of f(0) for the spherical terms and <= 1e-7 of max |G_l| for the deformation
+// terms (measured against the closed forms for s <= 3.2 / A), inside the mixed-precision bound.
 static constexpr double kTabStep = 5e-4;
 static constexpr int kTabNode = 16;
+static constexpr int kLinRatio = 4;
+static constexpr double kLinStep = kTabStep / kLinRatio;
 
 struct TabRec {
     int off, n;        // first node, node count
@@ -237,8 +243,7 @@ struct TabRec {
 // eight lanes and summed with a fixed-order DPP reduction.
 __device__ __forceinline__ void tab_build_node(int j, int lane, const TabRec* __restrict__ tabs, int n_wfn,
                                                const WfnRec* __restrict__ wfns, const TermRec* __restrict__ core,
-                                               const TermRec* __restrict__ val, double* __restrict__ tab,
-                                               float* __restrict__ tab32)
+                                               const TermRec* __restrict__ val, double* __restrict__ tab)
 {
     const int c = lane >> 3, q = lane & 7;
     int w = 0;
@@ -279,8 +284,6 @@ __device__ __forceinline__ void tab_build_node(int j, int lane, const TabRec* __
     if (q == 0) {
         tab[(size_t)j * kTabNode + 2 * c] = f;
         tab[(size_t)j * kTabNode + 2 * c + 1] = d;
-        tab32[(size_t)j * kTabNode + 2 * c] = (float)f;
-        tab32[(size_t)j * kTabNode + 2 * c + 1] = (float)d;
     }
 }
 
@@ -314,6 +317,26 @@ template <int L, typename R>
 __device__ __forceinline__ void radial_otf(const R* __restrict__ tab, TabRec tr, R h, R pval, R inv_kappa,
                                            R inv_kappa_p, R anr, R& S, R (&G)[5])
 {
+    if constexpr (sizeof(R) == sizeof(float)) {
+        // FP32 linear tables (k_lin_build): value + t * difference
+        constexpr float inv_lstep = float(1.0 / kLinStep);
+        const int nl = kLinRatio * tr.n;
+        const R* base = tab + (size_t)kLinRatio * tr.off * kTabNode;
+        auto at = [&](R x, int c) {
+            const int i = min((int)x, nl - 2);
+            const R t = x - (R)i;
+            const R* a = base + (size_t)i * kTabNode + c;
+            return a[0] + t * a[1];
+        };
+        S = at(h * inv_lstep, 0) + pval * at(h * (inv_kappa * inv_lstep), 2) + anr;
+        const R x = h * (inv_kappa_p * inv_lstep);
+        const int i = min((int)x, nl - 2);
+        const R t = x - (R)i;
+        const R* a = base + (size_t)i * kTabNode + 4;
+#pragma unroll
+        for (int l = 0; l < 5; l++) G[l] = l <= L ? a[2 * l] + t * a[2 * l + 1] : R(0);
+        return;
+    }
     constexpr R inv_step = R(1.0 / kTabStep);
     S = tab_at<R>(tab, tr, h * inv_step, 0) + pval * tab_at<R>(tab, tr, h * (inv_kappa * inv_step), 2) + anr;
     R wt[4];
@@ -483,9 +506,32 @@ __global__ void k_hkl_prep_diag(const int* __restrict__ hkl, const OpDev* __rest
 __global__ __launch_bounds__(64) void k_tab_build(const TabRec* __restrict__ tabs, int n_wfn,
                                                   const WfnRec* __restrict__ wfns,
                             const TermRec* __restrict__ core, const TermRec* __restrict__ val, int n_nodes,
-                            double* __restrict__ tab, float* __restrict__ tab32)
+                            double* __restrict__ tab)
 {
-    tab_build_node(blockIdx.x, threadIdx.x, tabs, n_wfn, wfns, core, val, tab, tab32);
+    tab_build_node(blockIdx.x, threadIdx.x, tabs, n_wfn, wfns, core, val, tab);
+}
+
+// The FP32 linear tables from the FP64 Hermite tables: one thread per (fine node, function);
+// wavefunction w's fine nodes are [kLinRatio * off, kLinRatio * (off + n)).
+__global__ void k_lin_build(const TabRec* __restrict__ tabs, int n_wfn, const double* __restrict__ tab, int n_nodes,
+                            float* __restrict__ lin)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_nodes * kLinRatio * 8) return;
+    const int j = idx >> 3, c = idx & 7;
+    int w = 0;
+    while (w + 1 < n_wfn && kLinRatio * tabs[w + 1].off <= j) w++;
+    const TabRec tr = tabs[w];
+    float* out = lin + (size_t)j * kTabNode + 2 * c;
+    if (c == 7) {
+        out[0] = out[1] = 0.0f;
+        return;
+    }
+    const double x = (double)(j - kLinRatio * tr.off) / kLinRatio;       // in coarse-node units
+    const double f0 = tab_at<double>(tab, tr, x, 2 * c);
+    const double f1 = tab_at<double>(tab, tr, x + 1.0 / kLinRatio, 2 * c);
+    out[0] = (float)f0;
+    out[1] = (float)(f1 - f0);
 }
 
 __global__ void k_radial_prep(const double* __restrict__ hlen, const double* __restrict__ tab,
@@ -2011,7 +2057,7 @@ void Engine::plan_tables()
     tab_nodes_.resize(nw);
     for (int w = 0; w < nw; w++) tab_nodes_[w] = tr[w].n;
     d_tab_.ensure((size_t)nodes * kTabNode * sizeof(double));
-    d_tab32_.ensure((size_t)nodes * kTabNode * sizeof(float));
+    d_tab32_.ensure((size_t)nodes * kLinRatio * kTabNode * sizeof(float));
     d_tabrec_.ensure(tr.size() * sizeof(TabRec));
     HIP_CHECK(hipMemcpy(d_tabrec_.p, tr.data(), tr.size() * sizeof(TabRec), hipMemcpyHostToDevice));
 }
@@ -2293,8 +2339,14 @@ void Engine::run_prep(Precision prec, bool mp, bool electron)
     if (model_.n_types > 0) {
         hipLaunchKernelGGL(k_tab_build, dim3(n_tab_nodes_), dim3(64), 0, stream_,
                            d_tabrec_.as<TabRec>(), model_.n_wfn, wfns, d_core_.as<TermRec>(), d_val_.as<TermRec>(),
-                           n_tab_nodes_, d_tab_.as<double>(), d_tab32_.as<float>());
+                           n_tab_nodes_, d_tab_.as<double>());
         HIP_CHECK(hipGetLastError());
+        if (mixed && use_otf(mp)) {
+            const int nl = n_tab_nodes_ * kLinRatio * 8;
+            hipLaunchKernelGGL(k_lin_build, dim3((nl + T - 1) / T), dim3(T), 0, stream_, d_tabrec_.as<TabRec>(),
+                               model_.n_wfn, d_tab_.as<double>(), n_tab_nodes_, d_tab32_.as<float>());
+            HIP_CHECK(hipGetLastError());
+        }
     }
     if (model_.n_types > 0 && !use_otf(mp)) {
         int nt = n_hkl * model_.n_types;
@@ -3168,7 +3220,7 @@ void BatchEngine::share_tables(Precision prec, const std::vector<int>& order, st
     d_gtabs_.ensure(std::max<size_t>(gtab.size(), 1) * sizeof(TabRec));
     d_ltabs_.ensure(std::max<size_t>(per.size(), 1) * sizeof(TabRec));
     d_gtab_.ensure((size_t)std::max(total, 1) * kTabNode * sizeof(double));
-    d_gtab32_.ensure((size_t)std::max(total, 1) * kTabNode * sizeof(float));
+    d_gtab32_.ensure((size_t)std::max(total, 1) * kLinRatio * kTabNode * sizeof(float));
     auto up = [&](DeviceBuffer& d, const void* src, size_t bytes) {
         if (bytes) HIP_CHECK(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, stream_));
     };
@@ -3199,10 +3251,15 @@ void BatchEngine::launch(Precision prec, bool derivs)
     if (total_[kSpHkl] > 0)
         hipLaunchKernelGGL(k_batch_hkl_prep, blocks(total_[kSpHkl], T), dim3(T), 0, stream_, it, o(kSpHkl), nb,
                            total_[kSpHkl]);
-    if (n_shared_nodes_ > 0)
+    if (n_shared_nodes_ > 0) {
         hipLaunchKernelGGL(k_tab_build, dim3(n_shared_nodes_), dim3(64), 0, stream_, d_gtabs_.as<TabRec>(),
                            n_shared_wfn_, d_gwfn_.as<WfnRec>(), d_gcore_.as<TermRec>(), d_gval_.as<TermRec>(),
-                           n_shared_nodes_, d_gtab_.as<double>(), d_gtab32_.as<float>());
+                           n_shared_nodes_, d_gtab_.as<double>());
+        if (prec == kMixed)
+            hipLaunchKernelGGL(k_lin_build, blocks(n_shared_nodes_ * kLinRatio * 8, T), dim3(T), 0, stream_,
+                               d_gtabs_.as<TabRec>(), n_shared_wfn_, d_gtab_.as<double>(), n_shared_nodes_,
+                               d_gtab32_.as<float>());
+    }
     if (total_[kSpSlot] > 0)
         hipLaunchKernelGGL(k_batch_atom_prep, dim3(total_[kSpSlot]), dim3(64), 0, stream_, it, o(kSpSlot), nb);
     HIP_CHECK(hipGetLastError());

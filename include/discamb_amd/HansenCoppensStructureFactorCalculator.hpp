@@ -126,6 +126,29 @@ struct DerivativesSelector {
     bool d_xyz = true, d_adp = true, d_occ = true, d_anom = false;
 };
 
+// An atom of the asymmetric unit under a symmetry operation (AtomInCrystalID).
+struct AtomInCrystalID {
+    int index = 0;
+    std::string operation = "x,y,z";
+};
+
+enum class LcsDirectionType { ANY_ORTHOGONAL = DH_LCS_ANY_ORTHOGONAL, AVERAGE_DIRECTION = DH_LCS_AVERAGE_DIRECTION,
+                              AVERAGE_POSITION = DH_LCS_AVERAGE_POSITION };
+
+// Definition of one atom's local coordinate system from atomic positions
+// (LocalCoordinateSystem<AtomInCrystalID>, AtomTyping/LocalCoordinateSystem.h:14-45): X, Y, Z =
+// coordinate 0, 1, 2; coordinate_1 along direction 1, coordinate_2 in the plane of both
+// directions; chirality empty or three atoms.
+struct LocalCoordinateSystem {
+    bool isR = true;
+    LcsDirectionType direction1_type = LcsDirectionType::ANY_ORTHOGONAL;
+    LcsDirectionType direction2_type = LcsDirectionType::ANY_ORTHOGONAL;
+    AtomInCrystalID centralAtom;
+    std::vector<AtomInCrystalID> refPoint_1, refPoint_2;
+    int coordinate_1 = 0, coordinate_2 = 1;
+    std::vector<AtomInCrystalID> chirality;
+};
+
 namespace detail {
 inline void check(int rc)
 {
@@ -165,6 +188,21 @@ inline void parse_component(const std::string& s, int* row, double* t)
         i = j;
     }
 }
+// "-x+1/2,y,-z" -> fractional rotation (row-major) and translation
+inline void parse_operation(const std::string& op, int* rot, double* tr)
+{
+    std::string parts[3];
+    int k = 0;
+    for (char ch : op) {
+        if (ch == ',') {
+            if (++k > 2) throw Exception(DH_E_MODEL, "invalid symmetry operation " + op);
+        } else
+            parts[k] += ch;
+    }
+    if (k != 2) throw Exception(DH_E_MODEL, "invalid symmetry operation " + op);
+    for (int r = 0; r < 3; r++) parse_component(parts[r], rot + 3 * r, tr + r);
+}
+
 // AtomInCrystal values, local frames and countAtomContribution flattened for the C ABI.
 struct PackedAtoms {
     std::vector<double> xyz, adp, occ, lcs;
@@ -259,22 +297,11 @@ public:
         std::vector<int> rot;
         std::vector<double> tr;
         for (const std::string& op : crystal.spaceGroupOperations) {
-            std::string parts[3];
-            int k = 0;
-            for (char ch : op) {
-                if (ch == ',') {
-                    if (++k > 2) throw Exception(DH_E_MODEL, "invalid symmetry operation " + op);
-                } else
-                    parts[k] += ch;
-            }
-            if (k != 2) throw Exception(DH_E_MODEL, "invalid symmetry operation " + op);
-            for (int r = 0; r < 3; r++) {
-                int row[3];
-                double t;
-                detail::parse_component(parts[r], row, &t);
-                rot.insert(rot.end(), row, row + 3);
-                tr.push_back(t);
-            }
+            int r9[9];
+            double t3[3];
+            detail::parse_operation(op, r9, t3);
+            rot.insert(rot.end(), r9, r9 + 9);
+            tr.insert(tr.end(), t3, t3 + 3);
         }
         // flatten the model
         std::vector<int> wo{0}, to{0}, nl, tp, occ, kind, dvo{0}, dvp;
@@ -384,6 +411,58 @@ public:
     void setElectronScattering(bool electronScattering)
     {
         detail::check(dh_hc_set_electron_scattering(mCalc, electronScattering ? 1 : 0));
+    }
+    // N-Gaussian table of the CPU_IAM mode: "Waasmeier-Kirfel" (default), "IT92", "electron-IT",
+    // "electron-cctbx"
+    void setIamTable(const std::string& table) { detail::check(dh_hc_set_iam_table(mCalc, table.c_str())); }
+
+    // One definition per atom (empty: detach).  Every later call recomputes the frames from the
+    // positions it is given and ignores its localCoordinateSystems argument (AnyHcCalculator
+    // without "frozen lcs"); chainRule adds dT/dL : dL/dxyz to the position derivatives.
+    void setLocalCoordinateSystems(const std::vector<LocalCoordinateSystem>& lcs, bool chainRule = false)
+    {
+        std::vector<std::vector<dh_atom_ref>> refs;
+        refs.reserve(3 * lcs.size() + 1);
+        auto conv = [](const AtomInCrystalID& a) {
+            dh_atom_ref r{};
+            r.atom = a.index;
+            detail::parse_operation(a.operation, r.rot, r.trans);
+            return r;
+        };
+        auto list = [&](const std::vector<AtomInCrystalID>& v) {
+            refs.emplace_back();
+            for (const AtomInCrystalID& a : v) refs.back().push_back(conv(a));
+            refs.back().push_back(dh_atom_ref{});
+            return refs.back().data();
+        };
+        std::vector<dh_lcs_def> defs(lcs.size() + 1);
+        for (size_t i = 0; i < lcs.size(); i++) {
+            const LocalCoordinateSystem& l = lcs[i];
+            dh_lcs_def& d = defs[i];
+            d.central = conv(l.centralAtom);
+            d.direction1_type = (int)l.direction1_type;
+            d.direction2_type = (int)l.direction2_type;
+            d.n_ref1 = (int)l.refPoint_1.size();
+            d.n_ref2 = (int)l.refPoint_2.size();
+            d.n_chirality = (int)l.chirality.size();
+            d.ref1 = list(l.refPoint_1);
+            d.ref2 = list(l.refPoint_2);
+            d.chirality = list(l.chirality);
+            d.coordinate_1 = l.coordinate_1;
+            d.coordinate_2 = l.coordinate_2;
+            d.is_r = l.isR ? 1 : 0;
+        }
+        detail::check(dh_hc_set_lcs_definitions(mCalc, (int)lcs.size(), defs.data(), chainRule ? 1 : 0));
+    }
+    // the frames the last call used
+    std::vector<Matrix3d> lastLocalCoordinateSystems() const
+    {
+        std::vector<double> m(9 * mAtomCount + 9);
+        detail::check(dh_hc_last_lcs(mCalc, m.data()));
+        std::vector<Matrix3d> out(mAtomCount);
+        for (size_t i = 0; i < mAtomCount; i++)
+            for (int j = 0; j < 9; j++) out[i].m[j] = m[9 * i + j];
+        return out;
     }
 
     void calculateStructureFactorsAndDerivatives(const std::vector<AtomInCrystal>& atoms,

@@ -1,6 +1,8 @@
 // Test driver for the C++ drop-in class: reads a structure as a whitespace token stream (written
 // by tests/test_cpp_api.py), runs HansenCoppensStructureFactorCalculator on the GPU and prints
-// F and the derivatives.  Usage: hc_cpp_driver <input> <mode>; mode "per_hkl" runs the per-hkl
+// F and the derivatives.  Usage: hc_cpp_driver <input> <mode>; mode "lcs" reads local-frame
+// definitions after the reflections and runs with frames recomputed from the positions and the
+// chain rule; mode "per_hkl" runs the per-hkl
 // overload with a gradient-collecting processor and prints the same quantities.
 #include <array>
 #include <fstream>
@@ -160,6 +162,45 @@ int main(int argc, char** argv)
             std::cerr << (gc.ordered && gc.next == (int)hkl.size() ? "order ok\n" : "order wrong\n");
             std::cerr << (threw ? "rethrow ok\n" : "rethrow missing\n");
             return gc.ordered && threw ? 0 : 1;
+        }
+        if (std::string(argv[2]) == "lcs") {
+            // frames from positions with the chain rule: definitions follow the reflections
+            auto ref = [&](AtomInCrystalID& r) { in >> r.index >> r.operation; };
+            auto refs = [&](std::vector<AtomInCrystalID>& v) {
+                int m;
+                in >> m;
+                v.resize(m);
+                for (auto& r : v) ref(r);
+            };
+            int nd;
+            in >> nd;
+            std::vector<LocalCoordinateSystem> defs(nd);
+            for (auto& d : defs) {
+                int t1, t2, isr;
+                ref(d.centralAtom);
+                in >> t1;
+                refs(d.refPoint_1);
+                in >> t2;
+                refs(d.refPoint_2);
+                in >> d.coordinate_1 >> d.coordinate_2 >> isr;
+                refs(d.chirality);
+                d.direction1_type = (LcsDirectionType)t1;
+                d.direction2_type = (LcsDirectionType)t2;
+                d.isR = isr != 0;
+            }
+            calc.setLocalCoordinateSystems(defs, true);
+            std::vector<std::complex<double>> f;
+            std::vector<TargetFunctionAtomicParamDerivatives> d;
+            calc.calculateStructureFactorsAndDerivatives(crystal.atoms, lcs, hkl, f, d, dtdf, include);
+            std::cout << std::setprecision(17);
+            for (const Matrix3d& m : calc.lastLocalCoordinateSystems()) {
+                for (int j = 0; j < 9; j++) std::cout << m.m[j] << (j < 8 ? " " : "\n");
+            }
+            for (auto& v : f) std::cout << v.real() << " " << v.imag() << "\n";
+            for (auto& v : d)
+                std::cout << v.atomic_position_derivatives.x << " " << v.atomic_position_derivatives.y << " "
+                          << v.atomic_position_derivatives.z << "\n";
+            return 0;
         }
         calc.setCalculationMode(argv[2]);
         std::vector<std::complex<double>> f;

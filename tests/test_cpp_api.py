@@ -110,3 +110,42 @@ def test_cpp_form_factors_match_oracle(tmp_path):
     c2f = np.linalg.inv(hc_oracle.cell_f2c(w.crystal.cell))
     ref = hc_oracle.form_factors(w.crystal, w.params, w.lcs, w.hkl.astype(np.float64) @ c2f, include=include)
     assert rel_err(f, ref) < 1e-10
+
+
+def test_cpp_local_frames_from_positions(tmp_path):
+    """setLocalCoordinateSystems through the C++ class with the chain rule: the frames equal the
+    oracle's restatement, F the oracle's with those frames, and the position derivatives the
+    Python binding's (whose chain rule is checked against finite differences in test_lcs_gpu.py)."""
+    if not os.path.exists(DRIVER):
+        pytest.fail("tests/cpp/hc_cpp_driver not built (run __graft_entry__.build())")
+    from discamb_amd import HansenCoppensStructureFactorCalculator
+    from test_lcs_gpu import definitions
+    w = synth.small_structure("P21/c", n_atoms=8, d_min=1.2, seed=3)
+    include = np.ones(8, dtype=bool)
+    defs = definitions(8, 5)
+    inp = tmp_path / "in.txt"
+    write_input(w, str(inp), include)
+    kind = {"any_orthogonal": 0, "average_direction": 1, "average_position": 2}
+    tok = [len(defs)]
+    refs = lambda lst: [len(lst)] + [x for r in lst for x in (r[0], r[1].replace(" ", ""))]
+    for d in defs:
+        tok += [d.central, d.central_op, kind[d.direction1_type]] + refs(d.ref1)
+        tok += [kind[d.direction2_type]] + refs(d.ref2)
+        tok += [d.coordinate_1, d.coordinate_2, 1 if d.is_r else 0] + refs(d.chirality)
+    with open(inp, "a") as f:
+        f.write(" " + " ".join(str(t) for t in tok))
+    print("case cpp driver lcs", flush=True)
+    out = subprocess.run([DRIVER, str(inp), "lcs"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.strip().split("\n")
+    n, m = 8, len(w.hkl)
+    L = np.array([[float(x) for x in l.split()] for l in lines[:n]]).reshape(n, 3, 3)
+    f = np.array([complex(*map(float, l.split())) for l in lines[n:n + m]])
+    dx = np.array([[float(x) for x in l.split()] for l in lines[n + m:]])
+    ref_L = np.array([hc_oracle.lcs_matrix(w.crystal.cell, w.crystal.atoms.xyz, d) for d in defs])
+    assert np.abs(L - ref_L).max() < 1e-13
+    assert rel_err(f, hc_oracle.sf_and_derivs(w.crystal, w.params, w.crystal.atoms, ref_L, w.hkl, w.dtdf)[0]) < 1e-10
+    c = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+    c.setLocalCoordinateSystems(defs, chainRule=True)
+    _, d = c.calculateStructureFactorsAndDerivatives(w.crystal.atoms, None, w.hkl, w.dtdf)
+    assert rel_err(dx, d.atomic_position_derivatives) < 1e-12

@@ -197,8 +197,9 @@ def bench_batch(args):
 
 def bench_per_hkl(args):
     """Per-reflection derivative rows (the per-hkl path): F and dF/d(xyz, ADP, occ) of every atom
-    for every reflection, FP64, written by k_jac into device memory (dh_hc_jacobian_rows_dev).  Two
-    structures: the 60-atom P2_1/c small molecule (all its reflections in one launch) and the
+    for every reflection, FP64 rows, written by k_jac into device memory (dh_hc_jacobian_rows_dev),
+    with FP64 terms (HIP, the value) and with mixed-precision terms (HIP_MIXED, reported beside).
+    Two structures: the 60-atom P2_1/c small molecule (all its reflections in one launch) and the
     5000-atom P2_1 2_1 2_1 structure (a block of 2400 reflections, 1.9 GB of rows, per launch)."""
     import ctypes
 
@@ -207,10 +208,13 @@ def bench_per_hkl(args):
     hip = ctypes.CDLL("libamdhip64.so.7")      # the runtime libdiscamb_amd.so itself uses
     lib = _lib.load()
     out = {}
-    for key, w, m in (("p21c_60", synth.config0(), None), ("p212121_5000", synth.config1(), 2400)):
+    for key, w, m, mode in (("p21c_60", synth.config0(), None, "HIP"), ("p212121_5000", synth.config1(), 2400, "HIP"),
+                            ("p21c_60_mixed", synth.config0(), None, "HIP_MIXED"),
+                            ("p212121_5000_mixed", synth.config1(), 2400, "HIP_MIXED")):
         n = w.crystal.atoms.n
         m = len(w.hkl) if m is None else m
         c = HansenCoppensStructureFactorCalculator(w.crystal, w.params, device=0)
+        c.setCalculationMode(mode)
         c.jacobianUpload(w.crystal.atoms, w.lcs, w.hkl)
         fb, rb = 16 * m, 16 * m * 10 * n
         pf, pr = ctypes.c_void_p(), ctypes.c_void_p()
@@ -268,6 +272,9 @@ def bench_per_hkl(args):
                      "kernel_ms": big["kernel_ms"], "bytes_per_launch": big["bytes_written"]},
         "small_molecule": small,
         "large_block": big,
+        "mixed": {"note": "HIP_MIXED: FP32 terms with FP64 phase reduction, FP64 rows",
+                  "small_molecule": out["p21c_60_mixed"], "large_block": out["p212121_5000_mixed"],
+                  "hbm_frac_large_block": out["p212121_5000_mixed"]["hbm_GBps"] / 8000.0},
     }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -379,7 +379,7 @@ static void rows_to_host(Calculator* c, int k0, int k1, double* f, double* rows)
         const int b1 = std::min(k1, b0 + B), n = b1 - b0;
         c->rows_dev[0].ensure((size_t)n * row * sizeof(double));
         c->f_dev[0].ensure((size_t)n * 2 * sizeof(double));
-        c->eng().jacobian(b0, b1, c->f_dev[0].as<double>(), c->rows_dev[0].as<double>());
+        c->eng().jacobian(b0, b1, c->f_dev[0].as<double>(), c->rows_dev[0].as<double>(), c->prec);
         if (f)
             HIP_CHECK(hipMemcpyAsync(f + 2 * (size_t)(b0 - k0), c->f_dev[0].p, (size_t)n * 2 * sizeof(double),
                                      hipMemcpyDeviceToHost, st));
@@ -411,7 +411,7 @@ static void per_hkl(Calculator* c, dh_per_hkl_fn fn, void* user)
     for (int b = 0; b <= nb; b++) {
         if (b < nb) {
             const int q = b & 1, b0 = b * B, b1 = std::min(n_hkl, b0 + B), n = b1 - b0;
-            c->eng().jacobian(b0, b1, c->f_dev[q].as<double>(), c->rows_dev[q].as<double>());
+            c->eng().jacobian(b0, b1, c->f_dev[q].as<double>(), c->rows_dev[q].as<double>(), c->prec);
             HIP_CHECK(hipMemcpyAsync(c->f_host[q].p, c->f_dev[q].p, (size_t)n * 2 * sizeof(double),
                                      hipMemcpyDeviceToHost, st));
             HIP_CHECK(hipMemcpyAsync(c->rows_host[q].p, c->rows_dev[q].p, (size_t)n * row * sizeof(double),
@@ -922,7 +922,7 @@ int dh_hc_jacobian_rows_dev(dh_calculator* calc, int k0, int k1, void* f_dev, vo
         // after the rows are written
         HIP_CHECK(hipEventRecord(c->ev_in, caller));
         HIP_CHECK(hipStreamWaitEvent(st, c->ev_in, 0));
-        c->eng().jacobian(k0, k1, static_cast<double*>(f_dev), static_cast<double*>(rows_dev));
+        c->eng().jacobian(k0, k1, static_cast<double*>(f_dev), static_cast<double*>(rows_dev), c->prec);
         HIP_CHECK(hipEventRecord(c->ev_out, st));
         HIP_CHECK(hipStreamWaitEvent(caller, c->ev_out, 0));
     });

@@ -1390,7 +1390,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && !M
 // values, already scaled and converted to the caller's conventions, and F_k through the same
 // per-wave staging and stage-2 reduction as the summed path.  The output is ~160 bytes per
 // (atom, reflection) against ~130 FP64 flops per (atom, operation, reflection) term, so the
-// kernel is bound by its HBM writes, not by arithmetic; it computes in FP64 in every mode.
+// kernel sits near the HBM-write roof; the mixed-precision variant (FP32 terms, FP64 phase
+// reduction and rows) takes the term work off that path.
 //
 // Work unit = (atom wave, reflection chunk of the block); each of the four waves of a workgroup
 // takes its own unit, atom wave fastest, so the waves of a workgroup share reflection records
@@ -1438,6 +1439,25 @@ __global__ void k_jac_rec(const double* __restrict__ rec64, int n, const double*
     for (int a = 0; a < 6; a++) {
         double v = 0.0;
         for (int b = 0; b < 6; b++) v += conv[9 + 6 * a + b] * r[25 + b];
+        o[3 + a] = v;
+    }
+    o[9] = o[10] = o[11] = 0.0;
+}
+
+// The same from the mixed-precision records (Debye-Waller multipliers carry log2(e)).
+__global__ void k_jac_rec32(const float* __restrict__ rec32, int n, const double* __restrict__ conv,
+                            double* __restrict__ jrec)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* r = rec32 + (size_t)i * kHklopRec;
+    double* o = jrec + (size_t)i * kJacRec;
+    const double inv_log2e = 0.69314718055994530942;
+    for (int a = 0; a < 3; a++)
+        o[a] = conv[3 * a] * (double)r[31] + conv[3 * a + 1] * (double)r[32] + conv[3 * a + 2] * (double)r[33];
+    for (int a = 0; a < 6; a++) {
+        double v = 0.0;
+        for (int b = 0; b < 6; b++) v += conv[9 + 6 * a + b] * ((double)r[25 + b] * inv_log2e);
         o[3 + a] = v;
     }
     o[9] = o[10] = o[11] = 0.0;
@@ -1580,10 +1600,143 @@ __device__ __forceinline__ void body_jac(const MainArgs& a, const JacArgs& jb, i
     }
 }
 
-template <bool CENTRIC, bool OTF>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_jac(MainArgs a, JacArgs jb)
+// Mixed precision (HIP_MIXED): the terms as in body_mixed (FP64 phase reduction, FP32 harmonics,
+// radial factors, Debye-Waller, sincos and the sums over operations), the rows' scale factors and
+// the stored rows in FP64.
+template <int L, bool CENTRIC, bool OTF>
+__device__ __forceinline__ void body_jac_mixed(const MainArgs& a, const JacArgs& jb, int wave, int lane, int k0,
+                                               int k1, const float* __restrict__ hop,
+                                               const unsigned* __restrict__ phs32, const unsigned* __restrict__ jrec,
+                                               const float* __restrict__ rad, const unsigned* __restrict__ mult32,
+                                               float2* __restrict__ stage, double* __restrict__ fpart,
+                                               double* __restrict__ rows)
 {
-    __shared__ __align__(16) unsigned char stage_lds[kWavesPerBlock * kStageBytesF64];
+    constexpr int NLM = (L + 1) * (L + 1);
+    const int slot = wave * kWave + lane;
+    const AtomRec& at = cmem(a.atoms)[slot];
+    const double x0 = at.xf[0], x1 = at.xf[1], x2 = at.xf[2];
+    float u[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) u[j] = (float)at.u[j];
+    float P[NLM];
+#pragma unroll
+    for (int j = 0; j < NLM; j++) P[j] = (float)at.plm[j];
+    const double w = at.w;
+    const float anr = (float)at.anom[0], ani = (float)at.anom[1];
+    const int t = at.type;
+    const int orig = at.orig;
+    double mw = 0.0;
+    int kind = 0;
+    if (orig >= 0) {
+        const JacAtom ja = gmem(jb.atoms)[orig];
+        mw = ja.mult_w;
+        kind = ja.adp_kind;
+    }
+    TabRec tr{0, 2};
+    float pv = 0, ik = 0, ikp = 0;
+    if (OTF) {
+        tr = cmem(a.tabs)[at.wfn];
+        pv = (float)at.pval;
+        ik = (float)at.inv_kappa;
+        ikp = (float)at.inv_kappa_p;
+    }
+    const float* tab = cmem(static_cast<const float*>(a.tab));
+    const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
+    const size_t plane = (size_t)jb.n_atoms * 2;
+    for (int kb = k0; kb < k1; kb += kStage) {
+        const int nk = min(kStage, k1 - kb);
+        for (int kk = 0; kk < nk; ++kk) {
+            const int k = kb + kk;
+            const double hl = ld_f64(hl32, k);
+            float S, G0, G1, G2, G3, G4;
+            if (OTF) {
+                float G[5];
+                radial_otf<L, float>(tab, tr, (float)hl, pv, ik, ikp, anr, S, G);
+                G0 = G[0]; G1 = G[1]; G2 = G[2]; G3 = G[3]; G4 = G[4];
+            } else {
+                const float* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
+                S = rr[0] + anr; G0 = rr[1]; G1 = rr[2]; G2 = rr[3]; G3 = rr[4]; G4 = rr[5];
+            }
+            float s0r = 0, s0i = 0, sxr[3] = {0, 0, 0}, sxi[3] = {0, 0, 0};
+            float sur[6] = {0, 0, 0, 0, 0, 0}, sui[6] = {0, 0, 0, 0, 0, 0};
+            for (int s = 0; s < a.n_ops; ++s) {
+                const float* r = hop + ((size_t)k * a.n_ops + s) * kHklopRec;
+                const unsigned* pr = phs32 + ((size_t)k * a.n_ops + s) * 8;
+                const unsigned* qw = jrec + ((size_t)k * a.n_ops + s) * (2 * kJacRec);
+                float q[9];
+#pragma unroll
+                for (int j = 0; j < 9; j++) q[j] = (float)ld_f64(qw, j);
+                float A[5];
+                contract<L, float>(P, r, A);
+                float fr = S + G0 * A[0] + G2 * A[2];
+                float fi = ani + G1 * A[1];
+                if (L == 4) {
+                    fr += G4 * A[4];
+                    fi += G3 * A[3];
+                }
+                // FP64 range reduction of h.(R x + t) to [0, 1) revolutions
+                const double ph = fma(ld_f64(pr, 2), x2, fma(ld_f64(pr, 1), x1, fma(ld_f64(pr, 0), x0, ld_f64(pr, 3))));
+                const float phf = (float)__builtin_amdgcn_fract(ph);
+                const float sn = __builtin_amdgcn_sinf(phf), cs = __builtin_amdgcn_cosf(phf);
+                const float e = r[25] * u[0] + r[26] * u[1] + r[27] * u[2] + r[28] * u[3] + r[29] * u[4] + r[30] * u[5];
+                const float T = __builtin_amdgcn_exp2f(-e);
+                const float Er = T * cs, Ei = T * sn;
+                const float Xr = fr * Er - fi * Ei, Xi = fr * Ei + fi * Er;
+                float gr, gi, yr, yi;
+                if (CENTRIC) {
+                    gr = 2.0f * (Xr + ani * Ei);
+                    gi = 2.0f * ani * Er;
+                    yr = -2.0f * ani * Ei;
+                    yi = 2.0f * (Xi - ani * Er);
+                } else {
+                    gr = Xr;
+                    gi = Xi;
+                    yr = Xr;
+                    yi = Xi;
+                }
+                s0r += gr;
+                s0i += gi;
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    sxr[j] += q[j] * yr;
+                    sxi[j] += q[j] * yi;
+                }
+#pragma unroll
+                for (int j = 0; j < 6; j++) {
+                    sur[j] -= q[3 + j] * gr;
+                    sui[j] -= q[3 + j] * gi;
+                }
+            }
+            stage_f32(stage, kk, lane, (float)w * s0r, (float)w * s0i);
+            if (orig >= 0) {
+                const double c = ld_f64(mult32, k);
+                const double cx = 6.283185307179586477 * w * c, cu = 19.739208802178717238 * w * c, co = mw * c;
+                double* o = rows + ((size_t)(k - jb.k0) * kJacPlanes * jb.n_atoms + orig) * 2;
+#pragma unroll
+                for (int i = 0; i < 3; i++) put_row(o + i * plane, -cx * sxi[i], cx * sxr[i]);
+                double vr[6], vi[6];
+#pragma unroll
+                for (int i = 0; i < 6; i++) {
+                    vr[i] = kind == 6 ? cu * sur[i] : 0.0;
+                    vi[i] = kind == 6 ? cu * sui[i] : 0.0;
+                }
+                if (kind == 1) {
+                    vr[0] = -cu * hl * hl * s0r;
+                    vi[0] = -cu * hl * hl * s0i;
+                }
+#pragma unroll
+                for (int i = 0; i < 6; i++) put_row(o + (3 + i) * plane, vr[i], vi[i]);
+                put_row(o + 9 * plane, co * s0r, co * s0i);
+            }
+        }
+        flush_f32(stage, nk, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
+    }
+}
+
+template <bool CENTRIC, bool OTF, bool MIXED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 4 : 2))) void k_jac(MainArgs a, JacArgs jb)
+{
+    __shared__ __align__(16) unsigned char stage_lds[kWavesPerBlock * (MIXED ? kStageBytesF32 : kStageBytesF64)];
     // wave-uniform (the compiler cannot see that threadIdx.x / 64 is), so that the reflection
     // range and every record address derived from it stay in scalar registers
     const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
@@ -1595,10 +1748,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int k1 = min(k0 + jb.chunk, jb.k1);
     const int lane = threadIdx.x & (kWave - 1);
     const int cls = __builtin_amdgcn_readfirstlane(cmem(a.wave_l)[wave]);
-    const unsigned* hop = cmem(static_cast<const unsigned*>(a.hklop));
     const unsigned* jrec = cmem(reinterpret_cast<const unsigned*>(jb.jrec));
-    const double* rad = cmem(static_cast<const double*>(a.radial));
     const unsigned* mult32 = cmem(reinterpret_cast<const unsigned*>(a.dtdf + 2 * (size_t)a.n_hkl));
+    if (MIXED) {
+        const float* hop = cmem(static_cast<const float*>(a.hklop));
+        const unsigned* phs = cmem(reinterpret_cast<const unsigned*>(a.phase));
+        const float* rad = cmem(static_cast<const float*>(a.radial));
+        float2* st = reinterpret_cast<float2*>(stage_lds + wib * kStageBytesF32);
+        if (cls <= 2)
+            body_jac_mixed<2, CENTRIC, OTF>(a, jb, wave, lane, k0, k1, hop, phs, jrec, rad, mult32, st,
+                                            gmem(a.fpart), gmem(jb.rows));
+        else
+            body_jac_mixed<4, CENTRIC, OTF>(a, jb, wave, lane, k0, k1, hop, phs, jrec, rad, mult32, st,
+                                            gmem(a.fpart), gmem(jb.rows));
+        return;
+    }
+    const unsigned* hop = cmem(static_cast<const unsigned*>(a.hklop));
+    const double* rad = cmem(static_cast<const double*>(a.radial));
     double2* st = reinterpret_cast<double2*>(stage_lds + wib * kStageBytesF64);
     if (cls <= 2)
         body_jac<2, CENTRIC, OTF>(a, jb, wave, lane, k0, k1, hop, jrec, rad, mult32, st, gmem(a.fpart),
@@ -2309,7 +2475,7 @@ void Engine::ensure_prep_buffers(Precision prec, bool mp)
     if (mp) d_radx_.ensure((size_t)n_hkl * n_types * kRadRec * (mixed ? sizeof(float) : sizeof(double)));
 }
 
-void Engine::run_prep(Precision prec, bool mp, bool electron)
+void Engine::run_prep(Precision prec, bool mp, bool electron, bool general)
 {
     jac_prepped_ = false;
     ensure_prep_buffers(prec, mp);
@@ -2326,7 +2492,7 @@ void Engine::run_prep(Precision prec, bool mp, bool electron)
                                                          model_.n_types * sizeof(TypeRec));
     const AtomIn* ain = d_atoms_in_.as<AtomIn>();
     const int T = 256;
-    if (use_diag(mixed, mp)) {
+    if (use_diag(mixed, mp) && !general) {
         hipLaunchKernelGGL(k_hkl_prep_diag, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, d_hkl_.as<int>(), ops,
                            n_ops, n_hkl, c2f, d_hklop_.as<float>(), d_phase_.as<double>(), d_ht_.as<double>(),
                            hlen.as<double>());
@@ -2659,31 +2825,36 @@ void Engine::set_jacobian_conventions(const JacConv& conv)
     jac_ok_ = true;
 }
 
-static const void* jac_kernel(bool centric, bool otf)
+static const void* jac_kernel(bool centric, bool otf, bool mixed)
 {
-    if (centric) return otf ? (const void*)k_jac<true, true> : (const void*)k_jac<true, false>;
-    return otf ? (const void*)k_jac<false, true> : (const void*)k_jac<false, false>;
+    if (mixed) {
+        if (centric) return otf ? (const void*)k_jac<true, true, true> : (const void*)k_jac<true, false, true>;
+        return otf ? (const void*)k_jac<false, true, true> : (const void*)k_jac<false, false, true>;
+    }
+    if (centric) return otf ? (const void*)k_jac<true, true, false> : (const void*)k_jac<true, false, false>;
+    return otf ? (const void*)k_jac<false, true, false> : (const void*)k_jac<false, false, false>;
 }
 
 // Reflections per work unit: one staging block (8), or fewer when that would leave resident wave
 // slots idle.  Longer chunks lose more to the last, partly filled round of units than they save
 // in per-unit set-up (measured on 60- and 5000-atom structures).
-int Engine::jac_chunk(int n_rows)
+int Engine::jac_chunk(int n_rows, Precision prec)
 {
     if (tune.hkl_chunk > 0) return tune.hkl_chunk;
-    if (jac_resident_ == 0) {
+    const int m = prec == kMixed ? 1 : 0;
+    if (jac_resident_[m] == 0) {
         int blocks = 0;
-        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, jac_kernel(centric_origin_, use_otf(false)),
-                                                               256, 0));
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &blocks, jac_kernel(centric_origin_, use_otf(false), prec == kMixed), 256, 0));
         hipDeviceProp_t prop;
         HIP_CHECK(hipGetDeviceProperties(&prop, device_));
-        jac_resident_ = std::max(1, blocks) * kWavesPerBlock * prop.multiProcessorCount;
+        jac_resident_[m] = std::max(1, blocks) * kWavesPerBlock * prop.multiProcessorCount;
     }
     const long long work = (long long)n_rows * std::max(n_sorted_ / kWave, 1);
-    return (int)std::max(1LL, std::min((long long)kStage, work / jac_resident_));
+    return (int)std::max(1LL, std::min((long long)kStage, work / jac_resident_[m]));
 }
 
-void Engine::jacobian(int k0, int k1, double* f_dev, double* rows_dev)
+void Engine::jacobian(int k0, int k1, double* f_dev, double* rows_dev, Precision prec)
 {
     HIP_CHECK(hipSetDevice(device_));
     if (!jac_ok_) throw Error(1, "per-reflection derivatives: no upload for them (dh_hc_jacobian_upload)");
@@ -2697,18 +2868,24 @@ void Engine::jacobian(int k0, int k1, double* f_dev, double* rows_dev)
         fill(f_dev, (size_t)n * 2, 0.0);
         return;
     }
-    if (!jac_prepped_) {
-        run_prep(kFp64, false, false);     // the per-hkl engine has no electron-scattering mode
+    const bool mixed = prec == kMixed;
+    if (!jac_prepped_ || jac_prec_ != prec) {
+        run_prep(prec, false, false, true);     // the per-hkl engine has no electron-scattering mode
         const int n_rec = n_hkl_ * (int)ops_.size();
         d_jrec_.ensure((size_t)std::max(n_rec, 1) * kJacRec * sizeof(double));
-        hipLaunchKernelGGL(k_jac_rec, dim3((n_rec + 255) / 256), dim3(256), 0, stream_, d_hklop_.as<double>(), n_rec,
-                           d_jconv_.as<double>(), d_jrec_.as<double>());
+        if (mixed)
+            hipLaunchKernelGGL(k_jac_rec32, dim3((n_rec + 255) / 256), dim3(256), 0, stream_, d_hklop_.as<float>(),
+                               n_rec, d_jconv_.as<double>(), d_jrec_.as<double>());
+        else
+            hipLaunchKernelGGL(k_jac_rec, dim3((n_rec + 255) / 256), dim3(256), 0, stream_, d_hklop_.as<double>(),
+                               n_rec, d_jconv_.as<double>(), d_jrec_.as<double>());
         HIP_CHECK(hipGetLastError());
         jac_prepped_ = true;
+        jac_prec_ = prec;
     }
-    const int chunk = jac_chunk(n);
+    const int chunk = jac_chunk(n, prec);
     MainArgs a;
-    main_args(kFp64, false, false, chunk, &a);
+    main_args(prec, false, false, chunk, &a);
     JacArgs jb;
     std::memset(&jb, 0, sizeof(jb));
     jb.rows = rows_dev;
@@ -2725,8 +2902,8 @@ void Engine::jacobian(int k0, int k1, double* f_dev, double* rows_dev)
     last_grid_y = 1;
     void* args[] = {&a, &jb};
     if (timing) HIP_CHECK(hipEventRecord(ev0_, stream_));
-    HIP_CHECK(hipLaunchKernel(jac_kernel(centric_origin_, use_otf(false)), dim3(last_grid_x), dim3(256), args, 0,
-                              stream_));
+    HIP_CHECK(hipLaunchKernel(jac_kernel(centric_origin_, use_otf(false), mixed), dim3(last_grid_x), dim3(256), args,
+                              0, stream_));
     if (timing) HIP_CHECK(hipEventRecord(ev1_, stream_));
     const int T = 64;
     hipLaunchKernelGGL(k_reduce_f_rows, dim3((n + T - 1) / T), dim3(T), 0, stream_, d_fpart_.as<double2>(),

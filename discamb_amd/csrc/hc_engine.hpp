@@ -82,7 +82,8 @@ public:
     // then compute F and the rows of reflections [k0, k1) into device memory, asynchronously on
     // the engine stream.  rows: [k - k0][10 planes][n_atoms] complex; f: [k - k0] complex.
     void set_jacobian_conventions(const JacConv& conv);
-    void jacobian(int k0, int k1, double* f_dev, double* rows_dev);
+    // prec: kFp64, or kMixed for the mixed-precision terms (FP64 rows either way)
+    void jacobian(int k0, int k1, double* f_dev, double* rows_dev, Precision prec = kFp64);
     int n_atoms() const { return n_atoms_; }
     // Electron scattering (Mott-Bethe) for the summed path: z_f0 = (Z, f_e(0)) per atom type.
     // The radial factors then come from per-(reflection, type) records.
@@ -132,13 +133,14 @@ private:
     void ensure_prep_buffers(Precision prec, bool mp);
     void plan_tables();
     void main_args(Precision prec, bool derivs, bool mp, int chunk, MainArgs* a);
-    void run_prep(Precision prec, bool mp, bool electron = true);
+    // general: the per-(reflection, operation) records even where the parity-folded ones apply
+    void run_prep(Precision prec, bool mp, bool electron = true, bool general = false);
     void launch_main(Precision prec, bool derivs, bool mp, int chunk);
     void reduce(bool derivs, bool mp, int chunk);
     void fill(DeviceBuffer& b, size_t n, double v);
     void build_splines();
     void fill(double* p, size_t n, double v);
-    int jac_chunk(int n_rows);
+    int jac_chunk(int n_rows, Precision prec);
     int resident_waves(bool mixed, bool derivs, bool mp);
     int default_chunk(bool mixed, bool derivs, bool mp);
     int chunk_for(Precision prec, bool derivs, bool mp);
@@ -183,10 +185,11 @@ private:
         d_radial32_, d_ht_, d_fpart_, d_dpart_, d_f_, d_d_, d_order_, d_radx_, d_tab_, d_tab32_, d_tabrec_;
     // per-reflection rows
     bool jac_ok_ = false;           // conventions set for the current upload
-    bool jac_prepped_ = false;      // the prep tables hold the FP64 records of the current upload
+    bool jac_prepped_ = false;      // the prep tables hold the general records of the current upload
+    Precision jac_prec_ = kFp64;    // ... in this precision
     bool rows_layout_ = false;
     int n_included_ = 0;
-    int jac_resident_ = 0;
+    int jac_resident_[2] = {0, 0}; // resident waves of the FP64 / mixed rows kernel
     DeviceBuffer d_jatoms_, d_jconv_, d_jrec_;
     bool electron_ = false;
     DeviceBuffer d_ez_, d_lcsg_;

@@ -198,7 +198,8 @@ int dh_hc_engine_info(dh_calculator* calc, int* n_explicit_ops, int* group_order
  * F follows the per-hkl engine, which differs from the summed one for centrosymmetric groups
  * with f'' != 0: it keeps f'' unconjugated for the inversion partner.  Atoms excluded by
  * countAtomContribution (dh_atoms.include) have zero entries and no part in F.  The rows are
- * computed in FP64 in every calculation mode. */
+ * FP64; their terms are FP64 in the FP64 modes and FP32 with an FP64 phase reduction in
+ * HIP_MIXED / HIP_IAM_MIXED (1e-6 / 1e-5 of the FP64 rows). */
 #define DH_JAC_PLANES 10
 /* Called once per reflection, hkl_index = 0 .. n_hkl-1 in order; f = 2 doubles, row =
  * 2 * DH_JAC_PLANES * n_atoms doubles, valid during the call only.  Return 0 to continue; any

@@ -1,9 +1,10 @@
 """Per-reflection derivative rows (the reference's per-hkl path) on the GPU against the oracle.
 
 Tolerances (normwise max|d - d_ref| / max|d_ref| per quantity): F and every derivative class
-1e-10 against the FP64 oracle; sub-ranges, the callback driver, the device-pointer entry and the
-mixed calculation mode must reproduce the full host result exactly (the rows do not depend on the
-blocking, and are computed in FP64 in every mode).
+1e-10 against the FP64 oracle; sub-ranges, the callback driver and the device-pointer entry must
+reproduce the full host result exactly (the rows do not depend on the blocking).  In the mixed
+calculation mode the terms are FP32 (FP64 phase reduction, FP64 rows): 1e-6 for F, 1e-5 for the
+derivative classes.
 """
 import numpy as np
 import pytest
@@ -84,20 +85,45 @@ def test_rows_excluded_atoms_blocks_and_device_entry():
         c.jacobianRows(0, m + 1)
 
 
-def test_rows_are_fp64_in_mixed_mode_and_follow_new_inputs():
+def _mixed_errs(F, R, rF, rR):
+    return {"F": rel_err(F, rF), "xyz": rel_err(R[:, 0:3], rR[:, 0:3]), "adp": rel_err(R[:, 3:9], rR[:, 3:9]),
+            "occ": rel_err(R[:, 9], rR[:, 9])}
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_mixed_mode_rows_match_oracle(case):
+    w = _structure(case, n_atoms=12, d_min=1.0, seed=7)
+    an = _anomalous(w.crystal.atoms.n, 3)
+    c = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+    c.setCalculationMode("HIP_MIXED")
+    print("case mixed rows %s" % case, flush=True)
+    F, R = c.calculateJacobian(w.crystal.atoms, w.lcs, w.hkl, anomalous=an)
+    rF, rR = hc_oracle.per_hkl(w.crystal, w.params, w.crystal.atoms, w.lcs, w.hkl, anomalous=an)
+    e = _mixed_errs(F, R, rF, rR)
+    print("  ", e, flush=True)
+    assert e["F"] < 1e-6 and max(e.values()) < 1e-5
+
+
+def test_mixed_rows_follow_mode_switches_and_new_inputs():
     w = _structure("P212121", n_atoms=12, d_min=1.0, seed=2)
     c = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
     F, R = c.calculateJacobian(w.crystal.atoms, w.lcs, w.hkl)
     c.setCalculationMode("HIP_MIXED")
-    # a summed mixed-precision run in between rewrites the engine's tables
+    # a summed mixed-precision run in between rewrites the engine's tables (parity-folded records)
     c.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
     F2, R2 = c.calculateJacobian(w.crystal.atoms, w.lcs, w.hkl)
-    assert np.array_equal(F, F2) and np.array_equal(R, R2)
+    e = _mixed_errs(F2, R2, F, R)
+    assert e["F"] < 1e-6 and max(e.values()) < 1e-5 and e["F"] > 0
     a = w.crystal.atoms.copy()
     a.xyz = a.xyz + 0.01
     F3, R3 = c.calculateJacobian(a, w.lcs, w.hkl)
     rF, rR = hc_oracle.per_hkl(w.crystal, w.params, a, w.lcs, w.hkl)
-    _check_rows(F3, R3, rF, rR, a.adp_size)
+    e = _mixed_errs(F3, R3, rF, rR)
+    assert e["F"] < 1e-6 and max(e.values()) < 1e-5
+    # back to FP64: the FP64 rows again, exactly
+    c.setCalculationMode("HIP")
+    F4, R4 = c.calculateJacobian(w.crystal.atoms, w.lcs, w.hkl)
+    assert np.array_equal(F, F4) and np.array_equal(R, R4)
 
 
 class _Collector:

@@ -274,7 +274,8 @@ def bench_per_hkl(args):
         "large_block": big,
         "mixed": {"note": "HIP_MIXED: FP32 terms with FP64 phase reduction, FP64 rows",
                   "small_molecule": out["p21c_60_mixed"], "large_block": out["p212121_5000_mixed"],
-                  "hbm_frac_large_block": out["p212121_5000_mixed"]["hbm_GBps"] / 8000.0},
+                  "hbm_frac_large_block": out["p212121_5000_mixed"]["hbm_GBps"] / 8000.0,
+                  "traffic_large_block": measured_traffic("k_jac<acentric,mixed>", TRAFFIC_FILE_ROWS)},
     }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

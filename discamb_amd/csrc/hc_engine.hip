@@ -20,6 +20,7 @@
 #include <numeric>
 
 #include "hc_device_math.hpp"
+#include "hc_fp64_trig.hpp"
 #include "hc_engine.hpp"
 
 namespace discamb_amd {
@@ -796,9 +797,9 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
             // h.(R x + t) reduced to [0, 1) revolutions
             const double ph = __builtin_amdgcn_fract(fma(r[33], x2, fma(r[32], x1, fma(r[31], x0, r[34]))));
             double sn, cs;
-            sincospi(2.0 * ph, &sn, &cs);
+            sincos_rev(ph, &sn, &cs);
             double e = r[25] * u[0] + r[26] * u[1] + r[27] * u[2] + r[28] * u[3] + r[29] * u[4] + r[30] * u[5];
-            double T = exp(-e);
+            double T = exp_neg(e);
             double Er = T * cs, Ei = T * sn;
             if (!BO) {
                 double Xr = fr * Er - fi * Ei, Xi = fr * Ei + fi * Er;
@@ -1542,9 +1543,9 @@ __device__ __forceinline__ void body_jac(const MainArgs& a, const JacArgs& jb, i
             }
             const double ph = __builtin_amdgcn_fract(fma(r[33], x2, fma(r[32], x1, fma(r[31], x0, r[34]))));
             double sn, cs;
-            sincospi(2.0 * ph, &sn, &cs);
+            sincos_rev(ph, &sn, &cs);
             const double e = r[25] * u[0] + r[26] * u[1] + r[27] * u[2] + r[28] * u[3] + r[29] * u[4] + r[30] * u[5];
-            const double T = exp(-e);
+            const double T = exp_neg(e);
             const double Er = T * cs, Ei = T * sn;
             const double Xr = fr * Er - fi * Ei, Xi = fr * Ei + fi * Er;
             double gr, gi, yr, yi;

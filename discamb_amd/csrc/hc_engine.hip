@@ -3125,7 +3125,7 @@ __global__ __launch_bounds__(64) void k_batch_atom_prep(const BatchItem* __restr
 // structure.  Units are ordered costliest structure first.  Radial factors are evaluated on the
 // fly from each structure's per-wavefunction tables (no per-(reflection, type) records).
 template <bool MIXED, bool DERIV, bool DIAG>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 1))) void k_batch_main(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 2))) void k_batch_main(
     const BatchItem* __restrict__ it, const int* __restrict__ off, int nb, int total, int* __restrict__ counter)
 {
     constexpr int kStageBytes = MIXED ? kStageBytesF32 : kStageBytesF64;

@@ -44,12 +44,13 @@ struct PinnedBuffer {
     double* d() const { return static_cast<double*>(p); }
 };
 
-// f(a0, a1) over [0, n) split across up to 16 host threads (at least 256 atoms each)
+// f(a0, a1) over [0, n) split across up to 16 host threads, at least min_each atoms per thread
+// (a thread start costs about as much as a few hundred frame derivatives)
 template <typename F>
-static void parallel_atoms(int n, F&& f)
+static void parallel_atoms(int n, int min_each, F&& f)
 {
     const unsigned hw = std::thread::hardware_concurrency();
-    const int nt = std::max(1, std::min({hw > 0 ? (int)hw : 1, n / 256, 16}));
+    const int nt = std::max(1, std::min({hw > 0 ? (int)hw : 1, n / std::max(min_each, 1), 16}));
     std::vector<std::exception_ptr> err(nt);
     auto run = [&](int t) {
         try {
@@ -309,7 +310,7 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
     }
     if (!c->lcs_defs.empty()) {
         // AnyHcCalculator::update (AnyHcCalculator.cpp:317-330): frames from the current positions
-        parallel_atoms(n, [&](int a0, int a1) {
+        parallel_atoms(n, 4096, [&](int a0, int a1) {
             for (int i = a0; i < a1; i++)
                 lcs_matrix(c->lcs_defs[i], ea.xyz_frac.data(), c->cell.f2c, &ea.lcs[9 * (size_t)i]);
         });
@@ -465,7 +466,7 @@ static void fetch(Calculator* c, double* f, double* d_xyz, double* d_adp, double
                                    &who[(size_t)a * kLcsMaxAtoms], &part[(size_t)a * kLcsMaxAtoms * 3]);
             }
         };
-        parallel_atoms(n, work);
+        parallel_atoms(n, 512, work);
         for (int a = 0; a < n; a++)
             for (int s = 0; s < cnt[a]; s++)
                 for (int j = 0; j < 3; j++)

@@ -532,6 +532,7 @@ static bool run_mp(Calculator* c, unsigned selector)
 {
     const bool chain = c->lcs_chain && !c->lcs_defs.empty() && (selector & DH_D_XYZ) && !c->iam;
     c->eng().lcs_grad = chain;
+    c->eng().mp_scalars = (selector & DH_D_MULTIPOLE) != 0;
     return (selector & DH_D_MULTIPOLE) != 0 || chain;
 }
 

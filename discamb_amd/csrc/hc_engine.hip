@@ -813,7 +813,7 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
                     for (int j = 0; j < 3; j++) dx[j] -= r[31 + j] * zi;
                 }
             }
-            if (MP) {
+            if (SC || BO) {
                 double cer = cr * Er - ci * Ei, cei = cr * Ei + ci * Er;
                 if (SC) {
                     Rh += cer;
@@ -855,6 +855,7 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
             dp[36] = dka;
             dp[37] = dkp;
         }
+        if (MP == 3) dp[35] = dp[36] = dp[37] = 0.0;
     }
 }
 
@@ -958,7 +959,7 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
                         for (int j = 0; j < 3; j++) fx[j] -= r[31 + j] * zi;
                     }
                 }
-                if (MP) {
+                if (SC || BO) {
                     float cer = cr * Er - ci * Ei, cei = cr * Ei + ci * Er;
                     if (SC) {
                         Rh += cer;
@@ -1019,6 +1020,7 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
             dp[36] = dka;
             dp[37] = dkp;
         }
+        if (MP == 3) dp[35] = dp[36] = dp[37] = 0.0;
     }
 }
 
@@ -1312,6 +1314,7 @@ __device__ __forceinline__ void body_mixed_diag(const MainArgs& a, int wave, int
             dp[36] = dka;
             dp[37] = dkp;
         }
+        if (MP == 3) dp[35] = dp[36] = dp[37] = 0.0;
     }
 }
 
@@ -1368,9 +1371,11 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
 // record of a chunk is fetched into one L2 only.
 // MP: 0 no multipole adjoint; 1 and 2 its two passes over the same work units (1: F, the
 // xyz/U/occupancy derivatives and the Pval, kappa, kappa' sums; 2: the 25 global-frame B_lm sums),
-// split so that neither pass holds both the atom's P_lm and the B_lm accumulators.
+// split so that neither pass holds both the atom's P_lm and the B_lm accumulators; 3 replaces 1
+// when only the B_lm are wanted (the chain rule through the local frames): the plain pass in the
+// multipole row layout.
 template <bool MIXED, bool DERIV, int MP, bool DIAG, bool OTF>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && !MP ? (DIAG || OTF ? 4 : 5) : 1))) void k_main(MainArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && (MP == 0 || MP == 3) ? (DIAG || OTF ? 4 : 5) : (!MIXED && MP == 2 ? 3 : 1)))) void k_main(MainArgs a)
 {
     constexpr int kStageBytes = MIXED ? kStageBytesF32 : kStageBytesF64;
     __shared__ __align__(16) unsigned char stage_lds[kWavesPerBlock * kStageBytes];
@@ -2539,6 +2544,8 @@ static const void* main_kernel(bool mixed, bool derivs, int mp_pass, bool diag, 
                                    : KM(false, true, 1, false, false);
     if (mp_pass == 2) return mixed ? (diag ? KM(true, true, 2, true, false) : KM(true, true, 2, false, false))
                                    : KM(false, true, 2, false, false);
+    if (mp_pass == 3) return mixed ? (diag ? KM(true, true, 3, true, false) : KM(true, true, 3, false, false))
+                                   : KM(false, true, 3, false, false);
     if (mixed) {
         if (diag) {
             if (otf) return derivs ? KM(true, true, 0, true, true) : KM(true, false, 0, true, true);
@@ -2635,8 +2642,9 @@ void Engine::launch_main(Precision prec, bool derivs, bool mp, int chunk)
     last_grid_y = a.n_blocks / std::max(a.grid_x, 1);
     last_chunk = chunk;
     void* args[] = {&a};
-    for (int pass = mp ? 1 : 0; pass <= (mp ? 2 : 0); pass++) {
-        const void* kern = main_kernel(prec == kMixed, derivs, pass, use_diag(prec == kMixed, mp), use_otf(mp));
+    const int passes[2] = {mp ? (mp_scalars ? 1 : 3) : 0, 2};
+    for (int p = 0; p < (mp ? 2 : 1); p++) {
+        const void* kern = main_kernel(prec == kMixed, derivs, passes[p], use_diag(prec == kMixed, mp), use_otf(mp));
         HIP_CHECK(hipLaunchKernel(kern, grid, dim3(256), args, 0, stream_));
     }
 }

@@ -92,6 +92,9 @@ public:
     // With the multipole adjoint: also dT/dL (9 per atom, local axes as columns, atom weight not
     // applied) for the chain rule through recomputed local frames; read with download_lcs_grad.
     bool lcs_grad = false;
+    // With the multipole adjoint: whether the Pval / kappa / kappa' sums (and the P_lm derivatives)
+    // are wanted as well; without them only the B_lm pass is added to the plain one.
+    bool mp_scalars = true;
     void download_lcs_grad(double* g);
     // Form-factor route: f[k][atom] (complex, caller atom order) for n_h Cartesian h vectors and
     // the given local frames and atom types; synchronous, host pointers.

@@ -180,6 +180,8 @@ private:
     std::vector<int> hkl_perm_;     // device reflection position -> caller reflection index
     std::vector<int> last_hkl_;     // reflections of the last upload (unchanged lists skip the sort)
     bool last_sorted_ = false;
+    bool prep_valid_ = false;       // the prep buffers hold the records of prep_key_ (run_prep)
+    int prep_key_ = -1;
     std::vector<int> tab_nodes_;    // radial-table node count per wavefunction
     // model tables
     DeviceBuffer d_core_, d_val_, d_type_, d_ops_;

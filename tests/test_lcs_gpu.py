@@ -92,3 +92,48 @@ def test_chain_rule_matches_finite_differences(sg, mode):
     f2, d2 = c.calculateStructureFactorsAndDerivatives(a, None, w.hkl, w.dtdf, None,
                                                       DerivativesSelector(False, True, True))
     assert rel_err(d2.adp_derivatives, ru) < tol
+
+
+def test_chain_rule_with_cartesian_conventions():
+    """Cartesian input coordinates and U_cart: frames from the converted positions, position
+    derivatives in Cartesian coordinates, against central differences in Cartesian coordinates."""
+    w = synth.small_structure("P21/c", n_atoms=6, d_min=1.4, seed=11)
+    f2c, c2f = synth.cell_matrices(w.crystal.cell)
+    a = w.crystal.atoms.copy()
+    a.xyz = a.xyz @ f2c.T
+    for i in range(a.n):
+        if a.adp_size[i] == 6:
+            u = a.adp[i]
+            ucif = np.array([[u[0], u[3], u[4]], [u[3], u[1], u[5]], [u[4], u[5], u[2]]])
+            n = np.diag(np.linalg.norm(c2f, axis=1))     # lengths of the reciprocal axes
+            uc = f2c @ n @ ucif @ n @ f2c.T
+            a.adp[i] = [uc[0, 0], uc[1, 1], uc[2, 2], uc[0, 1], uc[0, 2], uc[1, 2]]
+    cr = synth.Crystal(w.crystal.cell, w.crystal.symops, a, "cartesian", "U_cart")
+    defs = definitions(6, 4)
+    c = HansenCoppensStructureFactorCalculator(cr, w.params)
+    c.setLocalCoordinateSystems(defs, chainRule=True)
+    print("case lcs chain rule cartesian", flush=True)
+    f, d = c.calculateStructureFactorsAndDerivatives(a, None, w.hkl, w.dtdf)
+
+    def frames(atoms):
+        frac = atoms.xyz @ c2f.T
+        return np.array([hc_oracle.lcs_matrix(cr.cell, frac, dd) for dd in defs])
+
+    rf = hc_oracle.sf_and_derivs(cr, w.params, a, frames(a), w.hkl, w.dtdf)[0]
+    assert rel_err(f, rf) < 1e-10
+
+    def T(atoms):
+        F = hc_oracle.sf_and_derivs(cr, w.params, atoms, frames(atoms), w.hkl, w.dtdf)[0]
+        return float(np.sum((np.conj(w.dtdf) * F).real))
+
+    h = 1e-6
+    fd = np.zeros((a.n, 3))
+    for i in range(a.n):
+        for j in range(3):
+            a1, a2 = a.copy(), a.copy()
+            a1.xyz[i, j] += h
+            a2.xyz[i, j] -= h
+            fd[i, j] = (T(a1) - T(a2)) / (2 * h)
+    e = rel_err(d.atomic_position_derivatives, fd)
+    print("   chain rule vs finite differences (Cartesian) %.2e" % e, flush=True)
+    assert e < 1e-6

@@ -546,7 +546,7 @@ def main():
         print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4, "launches": 3}))
         return
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the path shards by reflection; see DESIGN.md); "
+        print("bench.py: measures one GPU only (the path would shard by reflection); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "batch256":

@@ -148,3 +148,19 @@ def test_electron_and_iam_are_refused_in_batches():
     c.setCalculationMode("CPU_IAM")
     with pytest.raises(RuntimeError):
         BatchCalculator([c])
+
+
+def test_iam_tables_by_name():
+    """setIamTable: IT92 (AnyIamCalculator's table) and electron-IT against the oracle with the same
+    table; an unknown name is refused."""
+    w = _structure("P21/c", n_atoms=8)
+    c = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+    c.setCalculationMode("CPU_IAM")
+    for table in ("IT92", "electron-IT", "Waasmeier-Kirfel"):
+        print("case iam table %s" % table, flush=True)
+        c.setIamTable(table)
+        f, d = c.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+        ref = hc_oracle.iam_sf_and_derivs(w.crystal, w.params, w.crystal.atoms, w.hkl, w.dtdf, table=table)
+        assert max(_errs(f, d, *ref).values()) < 1e-10
+    with pytest.raises(RuntimeError):
+        c.setIamTable("no such table")

@@ -76,6 +76,21 @@ def time_mode(calc, mode, steps, warmup, tune=True):
     return (t1 - t0) / steps, float(np.mean(main_ms)), info, f, d
 
 
+def time_host_calls(calc, w, calls=10):
+    """Whole calls from host buffers: upload (atoms and dT/dF; the unchanged reflection list stays on the
+    device), run, download of F and derivatives.  The mode and tuning are whatever `calc` last used."""
+    calc.setTiming(False)
+    calc.upload(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+    calc.run()
+    calc.fetch()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        calc.upload(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+        calc.run()
+        calc.fetch()
+    return (time.perf_counter() - t0) / calls
+
+
 def cpu_baseline(w, budget_refl=66592, label="the same 5000-atom P212121 structure"):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import hc_oracle
@@ -589,6 +604,8 @@ def main():
     smx, kmx, infomx, fmx, dmx = time_mode(calc, "HIP_MIXED", args.steps, args.warmup, tune)
     print("mixed: %.3f ms/step, main kernel %.3f ms, grid %s" % (smx * 1e3, kmx, infomx["grid"]),
           file=sys.stderr, flush=True)
+    host_mx = time_host_calls(calc, w)
+    print("mixed, whole call from host buffers: %.3f ms" % (host_mx * 1e3), file=sys.stderr, flush=True)
 
     def rel(a, b):
         return float(np.abs(a - b).max() / np.abs(b).max())
@@ -625,6 +642,9 @@ def main():
         "fp64": {"value": terms / s64, "unit": "terms/s", "ms_per_step": s64 * 1e3, "kernel_ms": k64,
                  "achieved_tflops": flops / (k64 * 1e-3) / 1e12, "frac_of_fp64_peak": flops / (k64 * 1e-3) / FP64_PEAK,
                  "frac_of_fp32_peak": flops / (k64 * 1e-3) / FP32_PEAK},
+        "host_call": {"ms": host_mx * 1e3, "terms_per_s": terms / host_mx,
+                      "timed": "mixed; upload of atoms and dT/dF (reflections unchanged, kept on the device), run, "
+                               "download of F and derivatives"},
         "speedup_mixed_vs_fp64": s64 / smx,
         "accuracy_mixed_vs_fp64": acc,
         "launch": {"grid": infomx["grid"], "chunk": infomx["chunk"], "fp64_chunk": info64["chunk"],

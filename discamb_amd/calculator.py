@@ -262,12 +262,11 @@ class HansenCoppensStructureFactorCalculator:
     @staticmethod
     def _hkl(hkl, dtdf):
         hkl = np.ascontiguousarray(np.asarray(hkl).reshape(-1, 3), dtype=np.int32)
-        d = np.asarray(dtdf, dtype=np.complex128).reshape(-1)
+        d = np.ascontiguousarray(np.asarray(dtdf, dtype=np.complex128).reshape(-1))
         if d.shape[0] != hkl.shape[0]:
             raise _lib.DhError(1, "Inconsistent size of hkl vectors set and (d target/ d F) table when "
                                   "calculating multipolar structure factors.")
-        dd = np.ascontiguousarray(np.stack([d.real, d.imag], 1))
-        return hkl, dd
+        return hkl, d.view(np.float64)          # (re, im) interleaved, no copy
 
     def calculateStructureFactorsAndDerivatives(self, atoms: AtomSet, localCoordinateSystems, hkl,
                                                 dTarget_dF, countAtomContribution=None,
@@ -285,16 +284,15 @@ class HansenCoppensStructureFactorCalculator:
         a, keep = self._atoms(atoms, localCoordinateSystems, countAtomContribution, anomalous)
         hkl_i, dd = self._hkl(hkl, dTarget_dF)
         nh, n = hkl_i.shape[0], atoms.n
-        f = np.zeros((max(nh, 1), 2))
+        f = np.zeros(max(nh, 1), np.complex128)       # written through its (re, im) float64 view
         dx = np.zeros((max(n, 1), 3))
         du = np.zeros((max(n, 1), 6))
         do = np.zeros(max(n, 1))
         _lib.check(self._lib.dh_hc_sf_and_derivs(
             self._h, ctypes.byref(a), nh, _lib.ptr(hkl_i, ctypes.c_int), _lib.ptr(dd, ctypes.c_double), sel,
-            _lib.ptr(f, ctypes.c_double), _lib.ptr(dx, ctypes.c_double), _lib.ptr(du, ctypes.c_double),
-            _lib.ptr(do, ctypes.c_double)))
-        fc = f[:nh, 0] + 1j * f[:nh, 1]
-        return fc, TargetFunctionAtomicParamDerivatives(dx[:n], du[:n], do[:n])
+            _lib.ptr(f.view(np.float64), ctypes.c_double), _lib.ptr(dx, ctypes.c_double),
+            _lib.ptr(du, ctypes.c_double), _lib.ptr(do, ctypes.c_double)))
+        return f[:nh], TargetFunctionAtomicParamDerivatives(dx[:n], du[:n], do[:n])
 
     # -- per-reflection derivatives (the per-hkl path) -------------------------------------------
     def _per_hkl(self, atoms, lcs, hkl, processor, include, anomalous):
@@ -399,13 +397,14 @@ class HansenCoppensStructureFactorCalculator:
 
     def fetch(self):
         nh, n = self._nh, self._n
-        f = np.zeros((max(nh, 1), 2))
+        f = np.zeros(max(nh, 1), np.complex128)       # written through its (re, im) float64 view
         dx = np.zeros((max(n, 1), 3))
         du = np.zeros((max(n, 1), 6))
         do = np.zeros(max(n, 1))
-        _lib.check(self._lib.dh_hc_fetch(self._h, _lib.ptr(f, ctypes.c_double), _lib.ptr(dx, ctypes.c_double),
-                                         _lib.ptr(du, ctypes.c_double), _lib.ptr(do, ctypes.c_double)))
-        return f[:nh, 0] + 1j * f[:nh, 1], TargetFunctionAtomicParamDerivatives(dx[:n], du[:n], do[:n])
+        _lib.check(self._lib.dh_hc_fetch(self._h, _lib.ptr(f.view(np.float64), ctypes.c_double),
+                                         _lib.ptr(dx, ctypes.c_double), _lib.ptr(du, ctypes.c_double),
+                                         _lib.ptr(do, ctypes.c_double)))
+        return f[:nh], TargetFunctionAtomicParamDerivatives(dx[:n], du[:n], do[:n])
 
     def autotune(self, derivativesSelector: DerivativesSelector = None):
         """Pick the fastest launch shape for the uploaded inputs in the current mode."""

@@ -98,6 +98,26 @@ void DeviceBuffer::release()
     bytes = 0;
 }
 
+void HostBuffer::ensure(size_t n)
+{
+    if (n <= bytes && p) return;
+    release();
+    size_t want = std::max<size_t>(n, 4096);
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+        p = nullptr;
+        bytes = 0;
+        throw Error(4, "pinned host allocation failed");
+    }
+    bytes = want;
+}
+
+void HostBuffer::release()
+{
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
 // ------------------------------------------------------------------ prep kernels
 
 struct OpDev {
@@ -2378,7 +2398,15 @@ void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const doub
     }
     n_sorted_ = (int)order_.size();
     n_included_ = (int)idx.size();
-    std::vector<AtomIn> ain(std::max(n_sorted_, 1));
+    // page-locked staging: [atoms | wave classes | dT/dF and multipliers | sorted reflections]
+    const size_t n_ain = std::max(n_sorted_, 1);
+    const size_t off_cls = n_ain * sizeof(AtomIn);
+    const size_t in_bytes = off_cls + wave_cls.size() * sizeof(int);
+    const size_t off_dtdf = (in_bytes + 255) / 256 * 256;
+    const size_t off_hkl = off_dtdf + 3 * (size_t)std::max(n_hkl, 1) * sizeof(double);
+    h_up_.ensure(off_hkl + 3 * (size_t)std::max(n_hkl, 1) * sizeof(int));
+    AtomIn* ain = h_up_.at<AtomIn>(0);
+    std::memset(ain, 0, sizeof(AtomIn));
     for (int s = 0; s < n_sorted_; s++) {
         AtomIn& r = ain[s];
         std::memset(&r, 0, sizeof(r));
@@ -2397,12 +2425,10 @@ void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const doub
         for (int j = 0; j < 9; j++) r.lcs[j] = at.lcs[9 * i + j];
         r.type = at.type[i];
     }
-    d_atoms_in_.ensure(ain.size() * sizeof(AtomIn) + wave_cls.size() * sizeof(int) + 64);
-    HIP_CHECK(hipMemcpyAsync(d_atoms_in_.p, ain.data(), ain.size() * sizeof(AtomIn), hipMemcpyHostToDevice, stream_));
-    if (!wave_cls.empty())
-        HIP_CHECK(hipMemcpyAsync(static_cast<char*>(d_atoms_in_.p) + ain.size() * sizeof(AtomIn), wave_cls.data(),
-                                 wave_cls.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
-    d_atoms_.ensure(ain.size() * sizeof(AtomRec));
+    if (!wave_cls.empty()) std::memcpy(h_up_.at<int>(off_cls), wave_cls.data(), wave_cls.size() * sizeof(int));
+    d_atoms_in_.ensure(in_bytes + 64);
+    HIP_CHECK(hipMemcpyAsync(d_atoms_in_.p, h_up_.p, in_bytes, hipMemcpyHostToDevice, stream_));
+    d_atoms_.ensure(n_ain * sizeof(AtomRec));
     // Reflections are processed in order of increasing |h| (stable; hkl_perm_[i] = caller index of
     // position i): consecutive reflections of a wave then read neighbouring nodes of the radial
     // tables.  |h| range for the form-factor tables.  A refinement passes the same list every
@@ -2411,7 +2437,7 @@ void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const doub
                           (n_hkl == 0 || std::memcmp(last_hkl_.data(), hkl, (size_t)3 * n_hkl * sizeof(int)) == 0);
     d_hkl_.ensure(std::max(n_hkl, 1) * 3 * sizeof(int));
     d_dtdf_.ensure(std::max(n_hkl, 1) * 3 * sizeof(double));   // dtdf (2) + mult (1)
-    std::vector<int> hkl_s;
+    int* hkl_s = h_up_.at<int>(off_hkl);
     if (!same_hkl) {
         prep_valid_ = false;
         std::vector<std::pair<double, int>> key(n_hkl);
@@ -2429,16 +2455,15 @@ void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const doub
         hkl_perm_.resize(n_hkl);
         for (int k = 0; k < n_hkl; k++) hkl_perm_[k] = key[k].second;
         plan_tables();
-        hkl_s.resize(3 * (size_t)std::max(n_hkl, 1));
         for (int i = 0; i < n_hkl; i++)
             for (int j = 0; j < 3; j++) hkl_s[3 * i + j] = hkl[3 * hkl_perm_[i] + j];
         last_hkl_.assign(hkl, hkl + 3 * (size_t)n_hkl);
         last_sorted_ = sort_hkl;
         if (n_hkl > 0)
-            HIP_CHECK(hipMemcpyAsync(d_hkl_.p, hkl_s.data(), (size_t)n_hkl * 3 * sizeof(int), hipMemcpyHostToDevice,
+            HIP_CHECK(hipMemcpyAsync(d_hkl_.p, hkl_s, (size_t)n_hkl * 3 * sizeof(int), hipMemcpyHostToDevice,
                                      stream_));
     }
-    std::vector<double> dtdf_s(3 * (size_t)std::max(n_hkl, 1));
+    double* dtdf_s = h_up_.at<double>(off_dtdf);
     for (int i = 0; i < n_hkl; i++) {
         const int k = hkl_perm_[i];
         dtdf_s[2 * i] = dtdf[2 * k];
@@ -2446,7 +2471,7 @@ void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const doub
         dtdf_s[2 * (size_t)n_hkl + i] = dtdf[2 * (size_t)n_hkl + k];
     }
     if (n_hkl > 0)
-        HIP_CHECK(hipMemcpyAsync(d_dtdf_.p, dtdf_s.data(), (size_t)n_hkl * 3 * sizeof(double), hipMemcpyHostToDevice,
+        HIP_CHECK(hipMemcpyAsync(d_dtdf_.p, dtdf_s, (size_t)n_hkl * 3 * sizeof(double), hipMemcpyHostToDevice,
                                  stream_));
     // keep host staging alive until the copies complete
     HIP_CHECK(hipStreamSynchronize(stream_));
@@ -2795,16 +2820,18 @@ void Engine::download(double* f, double* d10)
 {
     HIP_CHECK(hipSetDevice(device_));
     if (!ran_) throw Error(1, "no results: run the calculation before fetching");
-    std::vector<double> fs;
-    if (f && n_hkl_ > 0) {
-        fs.resize(2 * (size_t)n_hkl_);
-        HIP_CHECK(hipMemcpyAsync(fs.data(), d_f_.p, (size_t)n_hkl_ * 2 * sizeof(double), hipMemcpyDeviceToHost,
-                                 stream_));
-    }
+    // through page-locked staging: [F (sorted order) | per-atom derivative sums]
+    const size_t f_bytes = (size_t)n_hkl_ * 2 * sizeof(double);
+    const size_t d_bytes = (size_t)n_atoms_ * last_nd_ * sizeof(double);
+    const size_t off_d = (f_bytes + 255) / 256 * 256;
+    h_down_.ensure(off_d + d_bytes);
+    const double* fs = h_down_.at<double>(0);
+    if (f && n_hkl_ > 0)
+        HIP_CHECK(hipMemcpyAsync(h_down_.p, d_f_.p, f_bytes, hipMemcpyDeviceToHost, stream_));
     if (d10 && n_atoms_ > 0)
-        HIP_CHECK(hipMemcpyAsync(d10, d_d_.p, (size_t)n_atoms_ * last_nd_ * sizeof(double), hipMemcpyDeviceToHost,
-                                 stream_));
+        HIP_CHECK(hipMemcpyAsync(h_down_.at<char>(off_d), d_d_.p, d_bytes, hipMemcpyDeviceToHost, stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
+    if (d10 && n_atoms_ > 0) std::memcpy(d10, h_down_.at<char>(off_d), d_bytes);
     if (f)
         for (int i = 0; i < n_hkl_; i++) {
             f[2 * (size_t)hkl_perm_[i]] = fs[2 * (size_t)i];

@@ -38,6 +38,19 @@ struct DeviceBuffer {
     template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Page-locked host staging for the per-call copies (grown on demand, kept by the engine).
+struct HostBuffer {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t n);
+    void release();
+    ~HostBuffer() { release(); }
+    template <typename T> T* at(size_t byte_offset) const
+    {
+        return reinterpret_cast<T*>(static_cast<char*>(p) + byte_offset);
+    }
+};
+
 struct TuneParams {
     int hkl_chunk = 0;       // reflections per workgroup column; 0 = automatic
     bool general_path = false;  // force the general-rotation kernels for diagonal groups (tests)
@@ -188,6 +201,7 @@ private:
     // per call
     DeviceBuffer d_hkl_, d_dtdf_, d_atoms_in_, d_atoms_, d_hklop_, d_hklop32_, d_phase_, d_radial_,
         d_radial32_, d_ht_, d_fpart_, d_dpart_, d_f_, d_d_, d_order_, d_radx_, d_tab_, d_tab32_, d_tabrec_;
+    HostBuffer h_up_, h_down_;      // pinned staging of upload() and download()
     // per-reflection rows
     bool jac_ok_ = false;           // conventions set for the current upload
     bool jac_prepped_ = false;      // the prep tables hold the general records of the current upload

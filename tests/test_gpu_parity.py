@@ -189,3 +189,34 @@ def test_autotune_before_first_run_and_launch_shapes(mode):
                 rel_err(d.occupancy_derivatives, ro))
         print("  chunk %d errors %s" % (chunk, errs), flush=True)
         assert errs[0] <= tf and max(errs[1:]) <= td
+
+
+@pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
+def test_one_calculator_across_changing_reflection_lists(mode):
+    """One calculator called with reflection lists that grow, shrink, reorder, empty and repeat
+    (its host staging, |h| order and reflection tables are reused or rebuilt between calls) gives
+    bitwise the results of a fresh calculator for each list."""
+    w = synth.small_structure("P212121", n_atoms=40, d_min=0.9)
+    rng = np.random.default_rng(7)
+    n = len(w.hkl)
+    sub = rng.permutation(n)[: n // 3]
+    lists = [(w.hkl, w.dtdf), (w.hkl[sub], w.dtdf[sub]), (w.hkl, w.dtdf), (w.hkl, w.dtdf * 2.0),
+             (w.hkl[:0], w.dtdf[:0]), (w.hkl[::-1], w.dtdf[::-1]), (w.hkl, w.dtdf * 2.0)]
+    calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+    calc.setCalculationMode(mode)
+    for i, (h, g) in enumerate(lists):
+        print("case reflection-list sequence mode=%s call=%d hkl=%d" % (mode, i, len(h)), flush=True)
+        f, d = calc.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, h, g)
+        fresh = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+        fresh.setCalculationMode(mode)
+        rf, rd = fresh.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, h, g)
+        assert f.shape == (len(h),)
+        assert np.array_equal(f, rf)
+        assert np.array_equal(d.atomic_position_derivatives, rd.atomic_position_derivatives)
+        assert np.array_equal(d.adp_derivatives, rd.adp_derivatives)
+        assert np.array_equal(d.occupancy_derivatives, rd.occupancy_derivatives)
+    # the last list against the oracle as well
+    rf, rx, ru, ro = hc_oracle.sf_and_derivs(w.crystal, w.params, w.crystal.atoms, w.lcs, w.hkl, w.dtdf * 2.0,
+                                             n_threads=16)
+    tf, td = TOL[mode]
+    assert rel_err(f, rf) <= tf and rel_err(d.adp_derivatives, ru) <= td

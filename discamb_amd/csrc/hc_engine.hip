@@ -2789,27 +2789,33 @@ int Engine::autotune(Precision prec, bool derivs, bool mp)
         int c = std::max(8, (n_hkl_ + n_chunks - 1) / n_chunks);
         if (std::find(cands.begin(), cands.end(), c) == cands.end()) cands.push_back(c);
     }
-    int best = cands.front();
-    float best_ms = 1e30f;
+    auto time_once = [&](int c) {
+        HIP_CHECK(hipEventRecord(ev0_, stream_));
+        launch_main(prec, derivs, mp, c);
+        reduce(derivs, mp, c);
+        HIP_CHECK(hipEventRecord(ev1_, stream_));
+        HIP_CHECK(hipEventSynchronize(ev1_));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+        return ms;
+    };
     // one untimed launch first (clocks and caches), then the best of three per candidate
-    launch_main(prec, derivs, mp, cands.front());
-    reduce(derivs, mp, cands.front());
-    for (int c : cands) {
-        float ms_min = 1e30f;
-        for (int rep = 0; rep < 3; rep++) {
-            HIP_CHECK(hipEventRecord(ev0_, stream_));
-            launch_main(prec, derivs, mp, c);
-            reduce(derivs, mp, c);
-            HIP_CHECK(hipEventRecord(ev1_, stream_));
-            HIP_CHECK(hipEventSynchronize(ev1_));
-            float ms = 0;
-            HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
-            ms_min = std::min(ms_min, ms);
+    (void)time_once(cands.front());
+    std::vector<float> ms_min(cands.size(), 1e30f);
+    for (size_t i = 0; i < cands.size(); i++)
+        for (int rep = 0; rep < 3; rep++) ms_min[i] = std::min(ms_min[i], time_once(cands[i]));
+    std::vector<size_t> rank(cands.size());
+    for (size_t i = 0; i < rank.size(); i++) rank[i] = i;
+    std::sort(rank.begin(), rank.end(), [&](size_t x, size_t y) { return ms_min[x] < ms_min[y]; });
+    int best = cands[rank[0]];
+    if (rank.size() > 1) {
+        // the two leaders again, interleaved, best of five more each (they often differ by noise)
+        const size_t a = rank[0], b = rank[1];
+        for (int rep = 0; rep < 5; rep++) {
+            ms_min[a] = std::min(ms_min[a], time_once(cands[a]));
+            ms_min[b] = std::min(ms_min[b], time_once(cands[b]));
         }
-        if (ms_min < best_ms) {
-            best_ms = ms_min;
-            best = c;
-        }
+        best = ms_min[b] < ms_min[a] ? cands[b] : cands[a];
     }
     tuned_[shape_key(prec, derivs, mp)] = best;
     ran_ = false;     // the outputs hold a tuning launch, not a run() result

@@ -250,6 +250,8 @@ static void ensure_iam_engine(Calculator* c)
 static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl, const double* dtdf,
                    bool rows = false)
 {
+    // whatever fails below, the engine keeps no inputs of an earlier call to run on
+    c->eng().invalidate_inputs();
     if (!at) throw Error(DH_E_ARGUMENT, "null atoms");
     const int n = at->n_atoms;
     if (n != (int)c->atom_type.size())

@@ -2348,16 +2348,35 @@ void Engine::synchronize() { HIP_CHECK(hipStreamSynchronize(stream_)); }
 
 void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const double* dtdf, bool rows_layout)
 {
-    const bool sort_hkl = !rows_layout;
-    HIP_CHECK(hipSetDevice(device_));
-    if (n_hkl < 0 || at.n < 0) throw Error(1, "negative size");
-    n_atoms_ = at.n;
-    n_hkl_ = n_hkl;
+    // No result of an earlier run survives a new upload, and nothing runs until an upload has
+    // completed: a failed upload leaves the engine without inputs (and without reusable prep).
+    uploaded_ = false;
+    ran_ = false;
     jac_ok_ = false;
     jac_prepped_ = false;
-    rows_layout_ = rows_layout;
+    HIP_CHECK(hipSetDevice(device_));
+    if (n_hkl < 0 || at.n < 0) throw Error(1, "negative size");
     for (int i = 0; i < at.n; i++)
         if (at.type[i] < 0 || at.type[i] >= model_.n_types) throw Error(1, "atom type index out of range");
+    try {
+        upload_checked(at, n_hkl, hkl, dtdf, rows_layout);
+    } catch (...) {
+        prep_valid_ = false;
+        last_hkl_.clear();
+        hkl_perm_.clear();
+        n_atoms_ = n_hkl_ = n_sorted_ = n_included_ = 0;
+        order_.clear();
+        throw;
+    }
+    uploaded_ = true;
+}
+
+void Engine::upload_checked(const EngineAtoms& at, int n_hkl, const int* hkl, const double* dtdf, bool rows_layout)
+{
+    const bool sort_hkl = !rows_layout;
+    n_atoms_ = at.n;
+    n_hkl_ = n_hkl;
+    rows_layout_ = rows_layout;
     // sort included atoms by multipole class, then type; pad each class to whole waves
     std::vector<int> idx;
     for (int i = 0; i < at.n; i++)
@@ -2704,6 +2723,7 @@ long long Engine::shape_key(Precision prec, bool derivs, bool mp) const
 void Engine::run(Precision prec, bool derivs, bool mp)
 {
     HIP_CHECK(hipSetDevice(device_));
+    if (!uploaded_) throw Error(1, "no inputs: upload them before running the calculation");
     ran_ = true;
     if (mp) derivs = true;
     last_nd_ = mp ? kNdMultipole : kNdBasic;
@@ -2778,6 +2798,7 @@ void Engine::download_lcs_grad(double* g)
 int Engine::autotune(Precision prec, bool derivs, bool mp)
 {
     HIP_CHECK(hipSetDevice(device_));
+    if (!uploaded_) throw Error(1, "no inputs: upload them before tuning");
     if (mp) derivs = true;
     const int n_waves = n_sorted_ / kWave;
     if (n_hkl_ == 0 || n_waves == 0) return 0;
@@ -3228,6 +3249,7 @@ __global__ void k_batch_reduce_d(const BatchItem* __restrict__ it, const int* __
 void Engine::batch_item(Precision prec, bool derivs, int chunk, BatchItem* out)
 {
     HIP_CHECK(hipSetDevice(device_));
+    if (!uploaded_) throw Error(1, "batched step: a structure has no uploaded inputs");
     const bool mixed = prec == kMixed;
     jac_prepped_ = false;     // the batched prep rewrites this engine's tables
     prep_valid_ = false;

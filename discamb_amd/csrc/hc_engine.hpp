@@ -91,6 +91,14 @@ public:
     // caller order either way); rows_layout keeps the caller's order of both on the device, which
     // the per-reflection rows need.
     void upload(const EngineAtoms& atoms, int n_hkl, const int* hkl, const double* dtdf, bool rows_layout = false);
+    // Forget the inputs of the last upload (and its results): run() then refuses until the next
+    // upload completes.  The reflection-dependent prep stays valid for its reflection list.
+    void invalidate_inputs()
+    {
+        uploaded_ = false;
+        ran_ = false;
+        jac_ok_ = false;
+    }
     // Per-reflection derivative rows after an upload with rows_layout: set the conventions,
     // then compute F and the rows of reflections [k0, k1) into device memory, asynchronously on
     // the engine stream.  rows: [k - k0][10 planes][n_atoms] complex; f: [k - k0] complex.
@@ -146,6 +154,7 @@ public:
     int tab_nodes(int wfn) const { return wfn < (int)tab_nodes_.size() ? tab_nodes_[wfn] : 3; }
 
 private:
+    void upload_checked(const EngineAtoms& atoms, int n_hkl, const int* hkl, const double* dtdf, bool rows_layout);
     void ensure_prep_buffers(Precision prec, bool mp);
     void plan_tables();
     void main_args(Precision prec, bool derivs, bool mp, int chunk, MainArgs* a);
@@ -189,6 +198,7 @@ private:
     double f2c_[9], c2f_[9];
     int n_atoms_ = 0, n_sorted_ = 0, n_hkl_ = 0;
     bool ran_ = false;
+    bool uploaded_ = false;         // the last upload completed (run() needs one)
     std::vector<int> order_;        // sorted slot -> caller atom index (-1 padding)
     std::vector<int> hkl_perm_;     // device reflection position -> caller reflection index
     std::vector<int> last_hkl_;     // reflections of the last upload (unchanged lists skip the sort)

@@ -220,3 +220,83 @@ def test_one_calculator_across_changing_reflection_lists(mode):
                                              n_threads=16)
     tf, td = TOL[mode]
     assert rel_err(f, rf) <= tf and rel_err(d.adp_derivatives, ru) <= td
+
+
+def _type_arrays(params):
+    tp = params.type_parameters
+    return (np.array([t.plm25() for t in tp]), [t.p_val for t in tp], [t.kappa_spherical_valence for t in tp],
+            [t.kappa_deformation_valence for t in tp])
+
+
+@pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
+def test_one_calculator_across_changing_model_state(mode):
+    """With the reflection list held fixed (so the reflection records, radial tables and radial
+    records are kept between calls), every change of the model state the cached prep depends on -
+    new type parameters (kappa, kappa', Pval, P_lm), electron scattering on and off, forced kernel
+    paths - gives bitwise the result of a fresh calculator set up the same way."""
+    import copy
+    w = synth.small_structure("P212121", n_atoms=40, d_min=0.9)
+    base = _type_arrays(w.params)
+    p2 = copy.deepcopy(w.params)
+    rng = np.random.default_rng(12)
+    for t in p2.type_parameters:
+        t.kappa_spherical_valence *= 1.03
+        t.kappa_deformation_valence *= 0.97
+        t.p_val *= 0.98
+        for row in t.p_lm:
+            for m in range(len(row)):
+                row[m] += rng.normal() * 0.02
+    changed = _type_arrays(p2)
+    # (type parameters, electron scattering, kernel options) before each call
+    states = [(base, False, {}), (changed, False, {}), (changed, True, {}), (changed, False, {}),
+              (changed, False, {"general_rotations": True}), (changed, False, {"radial": "records"}),
+              (changed, False, {"radial": "tables"}), (base, False, {}), (base, True, {"radial": "records"})]
+    calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+    calc.setCalculationMode(mode)
+    for i, (types, electron, opts) in enumerate(states):
+        print("case model-state sequence mode=%s call=%d electron=%s options=%s" % (mode, i, electron, opts),
+              flush=True)
+        for c in (calc,):
+            c.setTypeParameters(*types)
+            c.setElectronScattering(electron)
+            c.setKernelOptions(**opts)
+        f, d = calc.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+        fresh = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+        fresh.setCalculationMode(mode)
+        fresh.setTypeParameters(*types)
+        fresh.setElectronScattering(electron)
+        fresh.setKernelOptions(**opts)
+        rf, rd = fresh.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+        assert np.array_equal(f, rf), i
+        assert np.array_equal(d.atomic_position_derivatives, rd.atomic_position_derivatives), i
+        assert np.array_equal(d.adp_derivatives, rd.adp_derivatives), i
+        assert np.array_equal(d.occupancy_derivatives, rd.occupancy_derivatives), i
+    # the changed parameters against the oracle as well (X-ray, default kernels)
+    calc.setTypeParameters(*changed)
+    calc.setElectronScattering(False)
+    calc.setKernelOptions()
+    f, d = calc.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+    rf, rx, ru, ro = hc_oracle.sf_and_derivs(w.crystal, p2, w.crystal.atoms, w.lcs, w.hkl, w.dtdf, n_threads=16)
+    tf, td = TOL[mode]
+    assert rel_err(f, rf) <= tf and rel_err(d.adp_derivatives, ru) <= td
+
+
+def test_failed_upload_leaves_no_stale_inputs():
+    """An upload that fails (here: too few dT/dF values) leaves the calculator without inputs:
+    run() and fetch() refuse until an upload succeeds, and the next good call matches a fresh
+    calculator bitwise."""
+    w = synth.small_structure("P21/c", n_atoms=12, d_min=1.0)
+    calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+    f0, d0 = calc.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+    bad = w.crystal.atoms.copy()
+    bad.adp_size = bad.adp_size.copy()
+    bad.adp_size[0] = 3
+    with pytest.raises(RuntimeError):
+        calc.upload(bad, w.lcs, w.hkl[:10], w.dtdf[:10])
+    with pytest.raises(RuntimeError):
+        calc.run()
+    with pytest.raises(RuntimeError):
+        calc.fetch()
+    f1, d1 = calc.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+    assert np.array_equal(f0, f1)
+    assert np.array_equal(d0.adp_derivatives, d1.adp_derivatives)

@@ -61,6 +61,20 @@ class DhAtoms(ctypes.Structure):
     ]
 
 
+class DhAtomsDev(ctypes.Structure):
+    """dh_atoms_dev: device addresses (integers) of the per-atom arrays."""
+    _fields_ = [
+        ("n_atoms", ctypes.c_int),
+        ("xyz", ctypes.c_void_p),
+        ("adp_size", ctypes.c_void_p),
+        ("adp", ctypes.c_void_p),
+        ("occupancy", ctypes.c_void_p),
+        ("lcs", ctypes.c_void_p),
+        ("anomalous", ctypes.c_void_p),
+        ("include", ctypes.c_void_p),
+    ]
+
+
 class DhAtomRef(ctypes.Structure):
     _fields_ = [("atom", ctypes.c_int), ("rot", ctypes.c_int * 9), ("trans", ctypes.c_double * 3)]
 
@@ -100,6 +114,10 @@ SIGNATURES = {
     "dh_hc_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "dh_hc_last_timing": (ctypes.c_int, [ctypes.c_void_p, c_dbl_p, c_int_p, c_int_p, c_int_p]),
     "dh_hc_engine_info": (ctypes.c_int, [ctypes.c_void_p, c_int_p, c_int_p, c_int_p, ctypes.c_char_p]),
+    "dh_hc_set_reflections": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, c_int_p]),
+    "dh_hc_sf_and_derivs_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DhAtomsDev), ctypes.c_void_p,
+                                               ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p, ctypes.c_void_p]),
     "dh_hc_per_hkl": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DhAtoms), ctypes.c_int, c_int_p, PER_HKL_FN,
                                      ctypes.c_void_p]),
     "dh_hc_jacobian_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DhAtoms), ctypes.c_int, c_int_p]),

@@ -165,6 +165,8 @@ class HansenCoppensStructureFactorCalculator:
         self.n_atoms = len(parameters.atom_to_type_map)
         self.n_types = len(parameters.type_parameters)
         self._keep = None
+        self._nh = self._n = 0          # sizes of the last upload (fetch reads them)
+        self._dev_nh = 0                # reflections of the device-resident calls
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -293,6 +295,70 @@ class HansenCoppensStructureFactorCalculator:
             _lib.ptr(f.view(np.float64), ctypes.c_double), _lib.ptr(dx, ctypes.c_double),
             _lib.ptr(du, ctypes.c_double), _lib.ptr(do, ctypes.c_double)))
         return f[:nh], TargetFunctionAtomicParamDerivatives(dx[:n], du[:n], do[:n])
+
+    # -- device-resident calls (torch tensors on the calculator's GPU) ---------------------------
+    def setReflections(self, hkl):
+        """Reflection list (host array) of the device-resident calls; kept until set again (a host
+        call or a change of calculation mode requires setting it again)."""
+        hkl_i = np.ascontiguousarray(np.asarray(hkl).reshape(-1, 3), dtype=np.int32)
+        _lib.check(self._lib.dh_hc_set_reflections(self._h, hkl_i.shape[0], _lib.ptr(hkl_i, ctypes.c_int)))
+        self._dev_nh = hkl_i.shape[0]
+
+    def _dev_args(self, xyz, adp_size, adp, occupancy, localCoordinateSystems, anomalous, countAtomContribution):
+        import torch
+        n = self.n_atoms
+        keep = []
+
+        def dev(t, dtype, shape, name):
+            if t is None:
+                return None
+            if not isinstance(t, torch.Tensor) or not t.is_cuda:
+                raise _lib.DhError(1, "%s: a torch tensor on the GPU is required" % name)
+            if t.dtype != dtype or tuple(t.shape) != shape or not t.is_contiguous():
+                raise _lib.DhError(1, "%s: contiguous %s of shape %s required" % (name, dtype, shape))
+            keep.append(t)
+            return t.data_ptr()
+
+        a = _lib.DhAtomsDev()
+        a.n_atoms = n
+        a.xyz = dev(xyz, torch.float64, (n, 3), "xyz")
+        a.adp_size = dev(adp_size, torch.int32, (n,), "adp_size")
+        a.adp = dev(adp, torch.float64, (n, 6), "adp")
+        a.occupancy = dev(occupancy, torch.float64, (n,), "occupancy")
+        a.lcs = dev(localCoordinateSystems, torch.float64, (n, 3, 3), "localCoordinateSystems")
+        if anomalous is not None:
+            a.anomalous = dev(torch.view_as_real(anomalous) if anomalous.is_complex() else anomalous, torch.float64,
+                              (n, 2), "anomalous")
+        if countAtomContribution is not None:
+            a.include = dev(countAtomContribution, torch.uint8, (n,), "countAtomContribution")
+        return a, keep, dev
+
+    def calculateOnDevice(self, xyz, adp_size, adp, occupancy, dTarget_dF=None, localCoordinateSystems=None,
+                          anomalous=None, countAtomContribution=None, derivativesSelector: DerivativesSelector = None,
+                          stream=None):
+        """calculateStructureFactorsAndDerivatives on torch tensors resident on the calculator's GPU,
+        ordered on `stream` (default: torch's current stream) without host synchronisation; the
+        reflections are those of setReflections.  xyz (n,3) / adp (n,6) float64 in the model's
+        conventions, adp_size (n,) int32, occupancy (n,) float64, dTarget_dF (n_hkl,) complex128
+        (None for F only), optional local frames (n,3,3), anomalous (n,) complex128 and
+        countAtomContribution (n,) uint8.  Returns (F, dxyz, dadp, docc) as new GPU tensors (the
+        derivatives None for F only)."""
+        import torch
+        sel = (derivativesSelector or DerivativesSelector()).bits() if dTarget_dF is not None else 0
+        a, keep, dev = self._dev_args(xyz, adp_size, adp, occupancy, localCoordinateSystems, anomalous,
+                                      countAtomContribution)
+        nh, n = getattr(self, "_dev_nh", 0), self.n_atoms
+        device = xyz.device
+        f = torch.empty(nh, dtype=torch.complex128, device=device)
+        dd = None
+        if dTarget_dF is not None:
+            dd = dev(torch.view_as_real(dTarget_dF), torch.float64, (nh, 2), "dTarget_dF")
+        outs = [torch.empty(s, dtype=torch.float64, device=device) for s in ((n, 3), (n, 6), (n,))] if sel else [None] * 3
+        st = stream if stream is not None else torch.cuda.current_stream(device)
+        _lib.check(self._lib.dh_hc_sf_and_derivs_dev(
+            self._h, ctypes.byref(a), dd, sel, torch.view_as_real(f).data_ptr() if nh else None,
+            *(o.data_ptr() if o is not None else None for o in outs), ctypes.c_void_p(st.cuda_stream)))
+        return (f,) + tuple(outs)
 
     # -- per-reflection derivatives (the per-hkl path) -------------------------------------------
     def _per_hkl(self, atoms, lcs, hkl, processor, include, anomalous):

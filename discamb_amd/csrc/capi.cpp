@@ -529,6 +529,88 @@ static void fetch(Calculator* c, double* f, double* d_xyz, double* d_adp, double
     }
 }
 
+// Device-resident calls: the layout upload (every atom in its slot, reflections from the host)
+// and the conversion matrices between the calculator's conventions and the engine's.
+static void set_reflections(Calculator* c, int n_hkl, const int* hkl)
+{
+    c->eng().invalidate_inputs();
+    c->rows_uploaded = false;
+    if (n_hkl < 0) throw Error(DH_E_ARGUMENT, "negative number of reflections");
+    if (n_hkl > 0 && !hkl) throw Error(DH_E_ARGUMENT, "null Miller indices");
+    const int n = (int)c->atom_type.size();
+    EngineAtoms ea;
+    ea.n = n;
+    ea.xyz_frac.assign(3 * (size_t)n, 0.0);
+    ea.u6.assign(6 * (size_t)n, 0.0);
+    ea.iso.assign(n, 0);
+    ea.weight.assign(n, 0.0);
+    ea.anom.assign(2 * (size_t)n, 0.0);
+    ea.lcs.assign(9 * (size_t)n, 0.0);
+    for (int i = 0; i < n; i++) ea.lcs[9 * (size_t)i] = ea.lcs[9 * (size_t)i + 4] = ea.lcs[9 * (size_t)i + 8] = 1.0;
+    ea.type = c->iam ? c->iam_atom_type : c->eng_atom_type;
+    ea.include.assign(n, 1);
+    std::vector<double> mult(n_hkl), anom(2 * (size_t)n);
+    for (int k = 0; k < n_hkl; k++) mult[k] = centering_multiplier(c->sg.centering, c->sg.obverse, hkl + 3 * k);
+    for (int i = 0; i < n; i++) {
+        anom[2 * (size_t)i] = c->model.anom[2 * c->atom_wfn[i]];
+        anom[2 * (size_t)i + 1] = c->model.anom[2 * c->atom_wfn[i] + 1];
+    }
+    const double two_pi2 = 2.0 * M_PI * M_PI;
+    DevConv conv;
+    conv.xyz_identity = c->xyz_conv != DH_XYZ_CARTESIAN;
+    std::memcpy(conv.c2f, c->cell.c2f, sizeof(conv.c2f));
+    conv.dxyz_identity = c->d_xyz_conv != DH_XYZ_CARTESIAN;
+    for (int j = 0; j < 3; j++)
+        for (int m = 0; m < 3; m++) conv.dxyz_m[3 * j + m] = c->cell.c2f[3 * m + j];   // d/dx_cart = c2f^T d/dx_frac
+    for (int k = 0; k < 6; k++) {
+        double e[6] = {0, 0, 0, 0, 0, 0}, uc[6], o[6];
+        e[k] = 1.0;
+        if (c->adp_conv == DH_ADP_U_CART) std::memcpy(uc, e, sizeof(uc));
+        else c->cell.u_cif_to_cart(e, uc);
+        if (c->d_adp_conv == DH_ADP_U_CART) std::memcpy(o, e, sizeof(o));
+        else c->cell.d_ucart_to_d_ucif(e, o);
+        for (int j = 0; j < 6; j++) {
+            conv.u2c[6 * j + k] = uc[j] * two_pi2;
+            conv.dadp_m[6 * j + k] = o[j];
+        }
+    }
+    conv.centric_origin = c->eng_centric_origin() ? 1 : 0;
+    c->eng().prepare_dev(ea, n_hkl, hkl, mult.data(), c->mult_w, anom, conv, c->prec);
+    c->n_atoms = n;
+    c->n_hkl = n_hkl;
+}
+
+static void sf_dev(Calculator* c, const dh_atoms_dev* at, const double* dtdf, unsigned selector, double* f,
+                   double* d_xyz, double* d_adp, double* d_occ, void* stream, bool no_alloc)
+{
+    if (!at) throw Error(DH_E_ARGUMENT, "null atoms");
+    if (at->n_atoms != (int)c->atom_type.size())
+        throw Error(DH_E_ARGUMENT, "number of atoms differs from the model's atom-to-type map");
+    if (selector & ~DH_D_ALL)
+        throw Error(DH_E_ARGUMENT, "device calls take DH_D_XYZ, DH_D_ADP and DH_D_OCC (multipole derivatives need "
+                                   "the host call)");
+    if (!c->lcs_defs.empty())
+        throw Error(DH_E_ARGUMENT, "device calls take the local frames as given (frames recomputed from positions "
+                                   "need the host call)");
+    DevCall call;
+    call.xyz = at->xyz;
+    call.adp_size = at->adp_size;
+    call.adp = at->adp;
+    call.occ = at->occupancy;
+    call.lcs = at->lcs;
+    call.anom = at->anomalous;
+    call.include = at->include;
+    call.dtdf = dtdf;
+    call.f = f;
+    call.d_xyz = d_xyz;
+    call.d_adp = d_adp;
+    call.d_occ = d_occ;
+    call.selector = selector;
+    Engine& e = c->eng();
+    e.run_dev(call, c->prec, static_cast<hipStream_t>(stream), no_alloc ? e.scratch() : nullptr,
+              no_alloc ? Engine::scratch_bytes() : 0);
+}
+
 // the multipole-adjoint kernels run when asked for, and for the chain rule through the frames
 static bool run_mp(Calculator* c, unsigned selector)
 {
@@ -801,6 +883,23 @@ int dh_hc_sf_and_derivs(dh_calculator* calc, const dh_atoms* atoms, int n_hkl, c
         check_selector(calc->c, selector);
         calc->c->eng().run(calc->c->prec, calc->c->last_selector != 0, run_mp(calc->c, calc->c->last_selector));
         fetch(calc->c, f, d_xyz, d_adp, d_occ);
+    });
+}
+
+int dh_hc_set_reflections(dh_calculator* calc, int n_hkl, const int* hkl)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        set_reflections(calc->c, n_hkl, hkl);
+    });
+}
+
+int dh_hc_sf_and_derivs_dev(dh_calculator* calc, const dh_atoms_dev* atoms, const double* dtarget_df,
+                            unsigned selector, double* f, double* d_xyz, double* d_adp, double* d_occ, void* stream)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        sf_dev(calc->c, atoms, dtarget_df, selector, f, d_xyz, d_adp, d_occ, stream, false);
     });
 }
 

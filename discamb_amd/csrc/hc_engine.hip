@@ -17,7 +17,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
+#include <string>
 
 #include "hc_device_math.hpp"
 #include "hc_fp64_trig.hpp"
@@ -78,9 +80,22 @@ struct AtomRec {
     int wfn, pad;
 };
 
+// Set while an allocation-free call runs on this thread: a buffer that would have to grow is an
+// error instead of an allocation.
+static thread_local bool g_no_alloc = false;
+
+static void refuse_allocation(size_t n)
+{
+    throw Error(4, "allocation-free call: " + std::to_string(n) +
+                       " bytes of device memory would have to be allocated (a buffer of the reflection records "
+                       "for another calculation mode, or partial sums beyond the static scratch); prepare with "
+                       "dh_hc_set_reflections in the current calculation mode");
+}
+
 void DeviceBuffer::ensure(size_t n)
 {
     if (n <= bytes && p) return;
+    if (g_no_alloc) refuse_allocation(n);
     release();
     size_t want = std::max<size_t>(n, 256);
     if (hipMalloc(&p, want) != hipSuccess) {
@@ -101,6 +116,7 @@ void DeviceBuffer::release()
 void HostBuffer::ensure(size_t n)
 {
     if (n <= bytes && p) return;
+    if (g_no_alloc) refuse_allocation(n);
     release();
     size_t want = std::max<size_t>(n, 4096);
     if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
@@ -2127,6 +2143,37 @@ static void host_ylm(const double* u, double* d)
     for (int i = 0; i < 25; i++) d[i] = v[i] * kHcDensityNorm[i];
 }
 
+// ------------------------------------------------------------------ static scratch
+// Memory of the allocation-free device calls (dh_hc_sf_and_derivs_dev_noalloc): every per-call
+// buffer of such a call (atom records, sorted dT/dF, partial sums, sorted F, raw derivatives) is
+// carved out of this array, so the call makes no device allocation.  Each GPU the library runs on
+// holds its own copy, at its own address, until the process ends; all calculators on that GPU
+// share it, so allocation-free calls that overlap on one GPU race.  Host code never passes the
+// array's name to a kernel (in host code it is a stand-in, not a device address): it takes the
+// device address once per device with hipGetSymbolAddress and passes that.
+static constexpr size_t kScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[64] = {};
+
+// Called with `device` current (engine set-up); once per device.
+static void* scratch_address(int device)
+{
+    if (device < 0 || device >= 64) throw Error(5, "HIP device index out of range for the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw Error(3, std::string("HIP error ") + hipGetErrorString(e) + " taking the scratch array's address");
+        g_scratch_addr[device] = ptr;
+    }
+    return g_scratch_addr[device];
+}
+
+size_t Engine::scratch_bytes() { return kScratchBytes; }
+
 static void upload_constants(int device)
 {
     if (device >= 0 && device < 64 && g_constants_ready[device]) return;
@@ -2264,6 +2311,7 @@ Engine::Engine(int device, const EngineModel& model, const double* cell6, const 
     if (device < 0 || device >= n_dev) throw Error(5, "HIP device index out of range");
     HIP_CHECK(hipSetDevice(device));
     upload_constants(device);
+    scratch_ = scratch_address(device);
     HIP_CHECK(hipStreamCreate(&stream_));
     HIP_CHECK(hipEventCreate(&ev0_));
     HIP_CHECK(hipEventCreate(&ev1_));
@@ -2373,6 +2421,7 @@ void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const doub
 
 void Engine::upload_checked(const EngineAtoms& at, int n_hkl, const int* hkl, const double* dtdf, bool rows_layout)
 {
+    dev_ready_ = false;
     const bool sort_hkl = !rows_layout;
     n_atoms_ = at.n;
     n_hkl_ = n_hkl;
@@ -2707,6 +2756,7 @@ void Engine::launch_main(Precision prec, bool derivs, bool mp, int chunk)
 
 int Engine::chunk_for(Precision prec, bool derivs, bool mp)
 {
+    if (chunk_override_ > 0) return chunk_override_;
     if (tune.hkl_chunk > 0) return tune.hkl_chunk;
     auto it = tuned_.find(shape_key(prec, derivs, mp));
     if (it != tuned_.end()) return it->second;
@@ -2864,6 +2914,321 @@ void Engine::download(double* f, double* d10)
             f[2 * (size_t)hkl_perm_[i]] = fs[2 * (size_t)i];
             f[2 * (size_t)hkl_perm_[i] + 1] = fs[2 * (size_t)i + 1];
         }
+}
+
+// ------------------------------------------------------------------ device-resident calls
+
+struct DevAtomArgs {
+    const double* xyz;
+    const int* adp_size;
+    const double* adp;
+    const double* occ;
+    const double* lcs;
+    const double* anom;
+    const unsigned char* include;
+    const double* mult_w;        // m / |G| per atom
+    const double* anom_default;  // f', f'' per atom from its wavefunction type
+    int n_slots, n_waves;
+};
+
+// AtomIn of every slot from the caller's arrays: the slot layout (atom index, type, padding)
+// comes from the skeleton prepare_dev uploaded, the values are converted to the engine's
+// conventions (fractional xyz, 2 pi^2 U_cart, weight occ * m/|G| or 0 for excluded atoms).  The
+// multipole class of every wave is copied with it.  skel and out may be the same array.
+__global__ void k_dev_atoms(const AtomIn* __restrict__ skel, const int* __restrict__ skel_cls, AtomIn* out,
+                            int* out_cls, DevAtomArgs a, DevConv c)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < a.n_waves) out_cls[s] = skel_cls[s];
+    if (s >= a.n_slots) return;
+    AtomIn r = skel[s];
+    const int i = r.orig;
+    if (i >= 0) {
+        const double* x = a.xyz + 3 * (size_t)i;
+        if (c.xyz_identity) {
+            for (int j = 0; j < 3; j++) r.xf[j] = x[j];
+        } else {
+            for (int j = 0; j < 3; j++) r.xf[j] = c.c2f[3 * j] * x[0] + c.c2f[3 * j + 1] * x[1] + c.c2f[3 * j + 2] * x[2];
+        }
+        const int na = a.adp_size[i];
+        const double* u = a.adp + 6 * (size_t)i;
+        for (int j = 0; j < 6; j++) r.u[j] = 0.0;
+        if (na == 6) {
+            for (int j = 0; j < 6; j++) {
+                double v = 0.0;
+                for (int k = 0; k < 6; k++) v += c.u2c[6 * j + k] * u[k];
+                r.u[j] = v;
+            }
+        } else if (na == 1) {
+            const double two_pi2 = 2.0 * M_PI * M_PI;
+            for (int j = 0; j < 3; j++) r.u[j] = u[0] * two_pi2;
+        }
+        const bool inc = a.include ? a.include[i] != 0 : true;
+        r.w = inc ? a.occ[i] * a.mult_w[i] : 0.0;
+        const double* an = a.anom ? a.anom + 2 * (size_t)i : a.anom_default + 2 * (size_t)i;
+        r.anom[0] = an[0];
+        r.anom[1] = an[1];
+        for (int j = 0; j < 9; j++) r.lcs[j] = a.lcs ? a.lcs[9 * (size_t)i + j] : (j % 4 == 0 ? 1.0 : 0.0);
+    }
+    out[s] = r;
+}
+
+// dT/dF in the engine's reflection order: conj(dT/dF) * centring multiplier (2 Re for a centre at
+// the origin), followed by the multipliers themselves (what upload() stages on the host).
+__global__ void k_dev_dtdf(const double* __restrict__ dtdf, const int* __restrict__ perm,
+                           const double* __restrict__ mult, int n_hkl, int centric_origin, double* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_hkl) return;
+    const int k = perm[i];
+    const double m = mult[k];
+    // F only (dtdf null): dT/dF is not used, the multipliers still scale F
+    double re = dtdf ? dtdf[2 * (size_t)k] * m : 0.0, im = dtdf ? -dtdf[2 * (size_t)k + 1] * m : 0.0;
+    if (centric_origin) {
+        re *= 2.0;
+        im = 0.0;
+    }
+    out[2 * (size_t)i] = re;
+    out[2 * (size_t)i + 1] = im;
+    out[2 * (size_t)n_hkl + i] = m;
+}
+
+// F back in caller order and the per-atom derivatives in the caller's conventions (what the host
+// fetch does: x 2 pi w, x 2 pi^2 w, x m/|G|, the U_iso sum and the convention matrices).
+__global__ void k_dev_out(const double* __restrict__ fs, const int* __restrict__ perm, const double* __restrict__ draw,
+                          int nd, int n_hkl, int n_atoms, DevAtomArgs a, DevConv c, unsigned sel,
+                          double* __restrict__ f, double* __restrict__ dxyz, double* __restrict__ dadp,
+                          double* __restrict__ docc)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx < n_hkl && f) {
+        const int k = perm[idx];
+        f[2 * (size_t)k] = fs[2 * (size_t)idx];
+        f[2 * (size_t)k + 1] = fs[2 * (size_t)idx + 1];
+    }
+    if (idx >= n_atoms) return;
+    const int i = idx;
+    const double* r = draw + (size_t)nd * i;
+    const bool inc = a.include ? a.include[i] != 0 : true;
+    const double w = inc ? a.occ[i] * a.mult_w[i] : 0.0;
+    if (dxyz) {
+        double* o = dxyz + 3 * (size_t)i;
+        if ((sel & 1u) && inc) {
+            const double two_pi = 2.0 * M_PI;
+            const double g[3] = {two_pi * w * r[0], two_pi * w * r[1], two_pi * w * r[2]};
+            for (int j = 0; j < 3; j++)
+                o[j] = c.dxyz_identity ? g[j] : c.dxyz_m[3 * j] * g[0] + c.dxyz_m[3 * j + 1] * g[1] + c.dxyz_m[3 * j + 2] * g[2];
+        } else {
+            o[0] = o[1] = o[2] = 0.0;
+        }
+    }
+    if (dadp) {
+        double* o = dadp + 6 * (size_t)i;
+        double out6[6] = {0, 0, 0, 0, 0, 0};
+        const int na = a.adp_size[i];
+        if ((sel & 2u) && inc && (na == 1 || na == 6)) {
+            const double two_pi2 = 2.0 * M_PI * M_PI;
+            double du[6];
+            for (int j = 0; j < 6; j++) du[j] = two_pi2 * w * r[3 + j];
+            if (na == 1) {
+                out6[0] = du[0] + du[1] + du[2];
+            } else {
+                for (int j = 0; j < 6; j++) {
+                    double v = 0.0;
+                    for (int k = 0; k < 6; k++) v += c.dadp_m[6 * j + k] * du[k];
+                    out6[j] = v;
+                }
+            }
+        }
+        for (int j = 0; j < 6; j++) o[j] = out6[j];
+    }
+    if (docc) docc[i] = ((sel & 4u) && inc) ? a.mult_w[i] * r[9] : 0.0;
+}
+
+void Engine::prepare_dev(const EngineAtoms& layout, int n_hkl, const int* hkl, const double* mult,
+                         const std::vector<double>& mult_w, const std::vector<double>& anom_default,
+                         const DevConv& conv, Precision prec)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    if ((int)mult_w.size() != layout.n || (int)anom_default.size() != 2 * layout.n)
+        throw Error(1, "internal: per-atom constants of the device layout");
+    for (int i = 0; i < layout.n; i++)
+        if (!layout.include[i]) throw Error(1, "internal: the device layout holds every atom");
+    // dT/dF zero, multipliers in the third part: the layout upload itself stages the reflections
+    std::vector<double> dd(3 * (size_t)std::max(n_hkl, 0), 0.0);
+    for (int k = 0; k < n_hkl; k++) dd[2 * (size_t)n_hkl + k] = mult[k];
+    upload(layout, n_hkl, hkl, dd.data(), false);
+    d_dev_perm_.ensure((size_t)std::max(n_hkl, 1) * sizeof(int));
+    d_dev_mult_.ensure((size_t)std::max(n_hkl, 1) * sizeof(double));
+    d_dev_atomc_.ensure((size_t)std::max(layout.n, 1) * 3 * sizeof(double));
+    std::vector<double> atomc(3 * (size_t)layout.n);
+    for (int i = 0; i < layout.n; i++) {
+        atomc[i] = mult_w[i];
+        atomc[layout.n + 2 * (size_t)i] = anom_default[2 * (size_t)i];
+        atomc[layout.n + 2 * (size_t)i + 1] = anom_default[2 * (size_t)i + 1];
+    }
+    if (n_hkl > 0) {
+        HIP_CHECK(hipMemcpyAsync(d_dev_perm_.p, hkl_perm_.data(), (size_t)n_hkl * sizeof(int), hipMemcpyHostToDevice,
+                                 stream_));
+        HIP_CHECK(hipMemcpyAsync(d_dev_mult_.p, mult, (size_t)n_hkl * sizeof(double), hipMemcpyHostToDevice, stream_));
+    }
+    if (layout.n > 0)
+        HIP_CHECK(hipMemcpyAsync(d_dev_atomc_.p, atomc.data(), atomc.size() * sizeof(double), hipMemcpyHostToDevice,
+                                 stream_));
+    dev_conv_ = conv;
+    // reflection records for this mode now, and the launch-shape queries, so that later calls
+    // (including allocation-free ones and captured ones) find them in place
+    if (n_hkl > 0 && n_sorted_ > 0) {
+        run_prep(prec, false);
+        (void)chunk_for(prec, true, false);
+        (void)chunk_for(prec, false, false);
+        const int n_waves = n_sorted_ / kWave;
+        d_f_.ensure((size_t)n_hkl * 2 * sizeof(double));
+        d_fpart_.ensure((size_t)n_waves * n_hkl * 2 * sizeof(double));
+    }
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    dev_ready_ = true;
+    ran_ = false;
+}
+
+namespace {
+// For one device-resident call: the engine's stream replaced by the caller's and, with a scratch
+// area, the per-call buffers replaced by views into it while allocation is refused.  Everything
+// is put back on exit, also when the call throws.
+struct DevScope {
+    struct Saved {
+        DeviceBuffer* b;
+        void* p;
+        size_t bytes;
+    };
+    Saved saved[8];
+    int n_saved = 0;
+    hipStream_t* stream_slot;
+    hipStream_t old_stream;
+    bool* timing_slot;
+    bool old_timing;
+    int* chunk_slot;
+    DevScope(hipStream_t* slot, hipStream_t st, bool* timing, int* chunk)
+        : stream_slot(slot), old_stream(*slot), timing_slot(timing), old_timing(*timing), chunk_slot(chunk)
+    {
+        *slot = st;
+        *timing = false;     // no host synchronisation inside a device-resident call
+    }
+    void view(DeviceBuffer& b, void* p, size_t bytes)
+    {
+        saved[n_saved++] = {&b, b.p, b.bytes};
+        b.p = p;
+        b.bytes = bytes;
+    }
+    ~DevScope()
+    {
+        for (int i = n_saved - 1; i >= 0; i--) {
+            saved[i].b->p = saved[i].p;
+            saved[i].b->bytes = saved[i].bytes;
+        }
+        *stream_slot = old_stream;
+        *timing_slot = old_timing;
+        *chunk_slot = 0;
+        g_no_alloc = false;
+    }
+};
+
+size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+}  // namespace
+
+void Engine::run_dev(const DevCall& call, Precision prec, hipStream_t stream, void* scratch_mem, size_t scratch_size)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    if (!dev_ready_ || !uploaded_)
+        throw Error(1, "device-resident call: set the reflections first (dh_hc_set_reflections)");
+    const int n = n_atoms_, n_hkl = n_hkl_;
+    if (n > 0 && (!call.xyz || !call.adp_size || !call.adp || !call.occ)) throw Error(1, "null atom array");
+    if (n_hkl > 0 && !call.f) throw Error(1, "null structure-factor output");
+    if (n_hkl > 0 && call.selector && !call.dtdf) throw Error(1, "null dT/dF");
+    const bool derivs = call.selector != 0;
+    const int n_ain = std::max(n_sorted_, 1);
+    const int n_waves = n_sorted_ / kWave;
+    AtomIn* skel = d_atoms_in_.as<AtomIn>();
+    const int* skel_cls = reinterpret_cast<const int*>(static_cast<const char*>(d_atoms_in_.p) + n_ain * sizeof(AtomIn));
+    DevScope scope(&stream_, stream, &timing, &chunk_override_);
+    if (scratch_mem) {
+        // carve the per-call buffers; the reflection chunk is lengthened if the derivative partials
+        // of the usual launch shape do not fit
+        const size_t b_ain = align256(n_ain * sizeof(AtomIn) + n_waves * sizeof(int) + 64);
+        const size_t b_atoms = align256(n_ain * sizeof(AtomRec));
+        const size_t b_dtdf = align256((size_t)std::max(n_hkl, 1) * 3 * sizeof(double));
+        const size_t b_f = align256((size_t)std::max(n_hkl, 1) * 2 * sizeof(double));
+        const size_t b_d = align256((size_t)std::max(n, 1) * kNdBasic * sizeof(double));
+        const size_t b_fpart = align256((size_t)n_waves * n_hkl * 2 * sizeof(double));
+        const size_t fixed = b_ain + b_atoms + b_dtdf + b_f + b_d + b_fpart;
+        const size_t per_chunk = (size_t)n_sorted_ * kNdBasic * sizeof(double);
+        if (fixed + (derivs ? per_chunk : 0) > scratch_size)
+            throw Error(4, "allocation-free call: " + std::to_string(n) + " atoms x " + std::to_string(n_hkl) +
+                               " reflections need more than the " + std::to_string(scratch_size >> 20) +
+                               " MiB static scratch");
+        size_t b_dpart = 256;
+        if (derivs && n_hkl > 0 && n_waves > 0) {
+            int chunk = chunk_for(prec, true, false);
+            int n_chunks = (n_hkl + chunk - 1) / chunk;
+            const size_t max_chunks = (scratch_size - fixed) / per_chunk;
+            if ((size_t)n_chunks > max_chunks) {
+                n_chunks = (int)max_chunks;
+                chunk = (n_hkl + n_chunks - 1) / n_chunks;
+                n_chunks = (n_hkl + chunk - 1) / chunk;
+            }
+            chunk_override_ = chunk;
+            b_dpart = align256((size_t)n_chunks * per_chunk);
+        }
+        char* base = static_cast<char*>(scratch_mem);
+        size_t off = 0;
+        auto carve = [&](DeviceBuffer& b, size_t bytes) {
+            scope.view(b, base + off, bytes);
+            off += bytes;
+        };
+        carve(d_atoms_in_, b_ain);
+        carve(d_atoms_, b_atoms);
+        carve(d_dtdf_, b_dtdf);
+        carve(d_f_, b_f);
+        carve(d_d_, b_d);
+        carve(d_fpart_, b_fpart);
+        carve(d_dpart_, b_dpart);
+        g_no_alloc = true;
+    }
+    DevAtomArgs aa;
+    aa.xyz = call.xyz;
+    aa.adp_size = call.adp_size;
+    aa.adp = call.adp;
+    aa.occ = call.occ;
+    aa.lcs = call.lcs;
+    aa.anom = call.anom;
+    aa.include = call.include;
+    aa.mult_w = d_dev_atomc_.as<double>();
+    aa.anom_default = d_dev_atomc_.as<double>() + n;
+    aa.n_slots = n_sorted_;
+    aa.n_waves = n_waves;
+    const int T = 256;
+    if (n_sorted_ > 0) {
+        int* out_cls = reinterpret_cast<int*>(static_cast<char*>(d_atoms_in_.p) + n_ain * sizeof(AtomIn));
+        hipLaunchKernelGGL(k_dev_atoms, dim3((n_sorted_ + T - 1) / T), dim3(T), 0, stream_, skel, skel_cls,
+                           d_atoms_in_.as<AtomIn>(), out_cls, aa, dev_conv_);
+        HIP_CHECK(hipGetLastError());
+    }
+    if (n_hkl > 0) {
+        hipLaunchKernelGGL(k_dev_dtdf, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, derivs ? call.dtdf : nullptr,
+                           d_dev_perm_.as<int>(), d_dev_mult_.as<double>(), n_hkl, dev_conv_.centric_origin,
+                           d_dtdf_.as<double>());
+        HIP_CHECK(hipGetLastError());
+    }
+    run(prec, derivs, false);
+    const int n_out = std::max(n_hkl, n);
+    if (n_out > 0) {
+        hipLaunchKernelGGL(k_dev_out, dim3((n_out + T - 1) / T), dim3(T), 0, stream_, d_f_.as<double>(),
+                           d_dev_perm_.as<int>(), d_d_.as<double>(), last_nd_, n_hkl, n, aa, dev_conv_, call.selector,
+                           call.f, call.d_xyz, call.d_adp, call.d_occ);
+        HIP_CHECK(hipGetLastError());
+    }
+    // results went to the caller's memory; nothing is left to fetch
+    ran_ = false;
 }
 
 // ------------------------------------------------------------------ per-reflection rows

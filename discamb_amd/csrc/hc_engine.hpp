@@ -51,6 +51,35 @@ struct HostBuffer {
     }
 };
 
+// Device-resident calls: the caller's atom arrays, dT/dF and outputs in device memory, in the
+// caller's order and conventions (the conversions run on the device).
+struct DevCall {
+    const double* xyz = nullptr;             // 3n
+    const int* adp_size = nullptr;           // n: 1 (U_iso) or 6; any other value: no ADP
+    const double* adp = nullptr;             // 6n
+    const double* occ = nullptr;             // n
+    const double* lcs = nullptr;             // 9n, or null: identity frames
+    const double* anom = nullptr;            // 2n, or null: the wavefunction types' f', f''
+    const unsigned char* include = nullptr;  // n, or null: every atom
+    const double* dtdf = nullptr;            // 2 n_hkl: dT/dA + i dT/dB (not conjugated)
+    double* f = nullptr;                     // 2 n_hkl
+    double* d_xyz = nullptr;                 // 3n, 6n, n (each may be null)
+    double* d_adp = nullptr;
+    double* d_occ = nullptr;
+    unsigned selector = 0;                   // xyz 1, ADP 2, occupancy 4
+};
+
+// Conversions between the caller's conventions and the engine's (fixed per calculator).
+struct DevConv {
+    int xyz_identity = 1;    // fractional input coordinates (copied)
+    double c2f[9];           // otherwise Cartesian -> fractional
+    double u2c[36];          // ADP input -> 2 pi^2 U_cart (row-major)
+    int dxyz_identity = 1;   // fractional output derivatives
+    double dxyz_m[9];        // otherwise caller xyz derivative = dxyz_m . d/dx_frac
+    double dadp_m[36];       // caller ADP derivative = dadp_m . d/dU_cart
+    int centric_origin = 0;  // dT/dF -> (2 Re, 0) (F = 2 Re of the half sum)
+};
+
 struct TuneParams {
     int hkl_chunk = 0;       // reflections per workgroup column; 0 = automatic
     bool general_path = false;  // force the general-rotation kernels for diagonal groups (tests)
@@ -98,6 +127,7 @@ public:
         uploaded_ = false;
         ran_ = false;
         jac_ok_ = false;
+        dev_ready_ = false;
     }
     // Per-reflection derivative rows after an upload with rows_layout: set the conventions,
     // then compute F and the rows of reflections [k0, k1) into device memory, asynchronously on
@@ -152,6 +182,23 @@ public:
     void batch_item(Precision prec, bool derivs, int chunk, BatchItem* out);
     void mark_batched_run(bool derivs);
     int tab_nodes(int wfn) const { return wfn < (int)tab_nodes_.size() ? tab_nodes_[wfn] : 3; }
+    // Device-resident calls.  prepare_dev uploads the reflection list (host memory) with every
+    // atom of the model in its slot (excluded atoms get weight 0 per call), keeps the reflection
+    // order, centring multipliers (mult, caller order) and per-atom constants (m/|G|, default
+    // f' + i f'') on the device, and prepares the reflection records for prec.  run_dev then runs
+    // one call on device pointers, ordered on `stream`, without host synchronisation.  With
+    // scratch != null every per-call buffer (atom records, sorted dT/dF, partial sums, sorted F,
+    // raw derivatives) is carved out of that memory and nothing is allocated: a call that would
+    // need more memory than scratch_bytes, or reflection records of another calculation mode,
+    // is refused.  Calls sharing one scratch must not overlap.
+    void prepare_dev(const EngineAtoms& layout, int n_hkl, const int* hkl, const double* mult,
+                     const std::vector<double>& mult_w, const std::vector<double>& anom_default,
+                     const DevConv& conv, Precision prec);
+    void run_dev(const DevCall& call, Precision prec, hipStream_t stream, void* scratch, size_t scratch_bytes);
+    bool dev_ready() const { return dev_ready_; }
+    // the static scratch array of the allocation-free calls on this engine's device, its size
+    void* scratch() const { return scratch_; }
+    static size_t scratch_bytes();
 
 private:
     void upload_checked(const EngineAtoms& atoms, int n_hkl, const int* hkl, const double* dtdf, bool rows_layout);
@@ -199,6 +246,11 @@ private:
     int n_atoms_ = 0, n_sorted_ = 0, n_hkl_ = 0;
     bool ran_ = false;
     bool uploaded_ = false;         // the last upload completed (run() needs one)
+    bool dev_ready_ = false;        // prepare_dev set the layout of device-resident calls
+    int chunk_override_ = 0;        // reflection chunk forced by an allocation-free call
+    DevConv dev_conv_;
+    DeviceBuffer d_dev_perm_, d_dev_mult_, d_dev_atomc_;
+    void* scratch_ = nullptr;       // device address of the static scratch array
     std::vector<int> order_;        // sorted slot -> caller atom index (-1 padding)
     std::vector<int> hkl_perm_;     // device reflection position -> caller reflection index
     std::vector<int> last_hkl_;     // reflections of the last upload (unchanged lists skip the sort)

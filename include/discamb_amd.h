@@ -183,6 +183,37 @@ int dh_hc_last_timing(dh_calculator* calc, double* main_kernel_ms, int* grid_x, 
 int dh_hc_engine_info(dh_calculator* calc, int* n_explicit_ops, int* group_order, int* centrosymmetric,
                       char* centering);
 
+/* ---- Device-resident calls: calculateStructureFactorsAndDerivatives (.h:148-165, .cpp:380-508) on
+ * device memory, for callers that keep their parameters on the GPU (torch tensors, refinement
+ * loops, hipGraph capture).  The reference has no device-memory form; its GPU engine takes host
+ * vectors (hc_sf_gpu::calculate_multipolar_structure_factors_gpu, src/GPU/cuda/HC_SF_GPU.h:10-69).
+ * The reflection list is set once from host memory; it fixes the reflection order, the centring
+ * multipliers and the reflection records, which every later device call on this calculator
+ * reuses (a host call, dh_hc_upload or a change of calculation mode requires setting it again). */
+int dh_hc_set_reflections(dh_calculator* calc, int n_hkl, const int* hkl);
+/* Atom parameters in device memory, layouts as dh_atoms (n_atoms = the model's atom count).
+ * adp_size: 1 (U_iso) or 6; any other value means no ADP.  lcs, anomalous, include may be NULL
+ * (identity frames, the wavefunction types' f' + i f'', every atom). */
+typedef struct {
+    int n_atoms;
+    const double* xyz;
+    const int* adp_size;
+    const double* adp;
+    const double* occupancy;
+    const double* lcs;
+    const double* anomalous;
+    const unsigned char* include;
+} dh_atoms_dev;
+/* One call on device pointers, ordered on `stream` (a hipStream_t; NULL = the null stream) and
+ * returning without waiting on the host.  dtarget_df: 2 n_hkl device doubles (dT/dA + i dT/dB, not
+ * conjugated; may be NULL when selector is 0); f: 2 n_hkl; d_xyz 3n, d_adp 6n, d_occ n device
+ * doubles (any may be NULL), in the calculator's derivative conventions.  selector: DH_D_XYZ |
+ * DH_D_ADP | DH_D_OCC bits (DH_D_MULTIPOLE and local frames recomputed from positions need the host
+ * call).  Results equal the host call's to rounding (the convention changes run on the device).
+ * Argument errors are reported synchronously; device-side values are not validated. */
+int dh_hc_sf_and_derivs_dev(dh_calculator* calc, const dh_atoms_dev* atoms, const double* dtarget_df,
+                            unsigned selector, double* f, double* d_xyz, double* d_adp, double* d_occ, void* stream);
+
 /* ---- Per-reflection derivatives: the reference's per-hkl overload
  *   template<T> calculateStructureFactorsAndDerivatives(atoms, lcs, hkl, T& singleHklResultsProcessor
  *       [, countAtomContribution])     HansenCoppensStructureFactorCalculator.h:206-242, 384-545

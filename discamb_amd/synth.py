@@ -104,7 +104,11 @@ def _fast_place(rng, n, cell, symops, min_sep=0.9):
     grid = {}
     out = []
     shifts = [(i, j, k) for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)]
+    misses = 0
     while len(out) < n:
+        if misses > 20000:
+            raise ValueError("cannot place %d atoms %.2f A apart in cell %s (%d placed)" % (n, min_sep, cell, len(out)))
+        misses += 1
         x = rng.uniform(0, 1, 3)
         imgs = np.array([(R @ x + t) % 1.0 for R, t in ops])
         ok = True
@@ -133,6 +137,7 @@ def _fast_place(rng, n, cell, symops, min_sep=0.9):
         if not ok:
             continue
         out.append(x)
+        misses = 0
         for y, c in zip(imgs, cells):
             grid.setdefault(c, []).append(y)
     return np.array(out)

@@ -118,6 +118,9 @@ SIGNATURES = {
     "dh_hc_sf_and_derivs_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DhAtomsDev), ctypes.c_void_p,
                                                ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_void_p, ctypes.c_void_p]),
+    "dh_hc_sf_and_derivs_dev_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DhAtomsDev), ctypes.c_void_p,
+                                                       ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p,
+                                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "dh_hc_per_hkl": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DhAtoms), ctypes.c_int, c_int_p, PER_HKL_FN,
                                      ctypes.c_void_p]),
     "dh_hc_jacobian_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DhAtoms), ctypes.c_int, c_int_p]),

@@ -299,7 +299,7 @@ class HansenCoppensStructureFactorCalculator:
     # -- device-resident calls (torch tensors on the calculator's GPU) ---------------------------
     def setReflections(self, hkl):
         """Reflection list (host array) of the device-resident calls; kept until set again (a host
-        call or a change of calculation mode requires setting it again)."""
+        call on this calculator replaces it)."""
         hkl_i = np.ascontiguousarray(np.asarray(hkl).reshape(-1, 3), dtype=np.int32)
         _lib.check(self._lib.dh_hc_set_reflections(self._h, hkl_i.shape[0], _lib.ptr(hkl_i, ctypes.c_int)))
         self._dev_nh = hkl_i.shape[0]
@@ -335,29 +335,38 @@ class HansenCoppensStructureFactorCalculator:
 
     def calculateOnDevice(self, xyz, adp_size, adp, occupancy, dTarget_dF=None, localCoordinateSystems=None,
                           anomalous=None, countAtomContribution=None, derivativesSelector: DerivativesSelector = None,
-                          stream=None):
+                          stream=None, no_alloc=False, out=None):
         """calculateStructureFactorsAndDerivatives on torch tensors resident on the calculator's GPU,
         ordered on `stream` (default: torch's current stream) without host synchronisation; the
         reflections are those of setReflections.  xyz (n,3) / adp (n,6) float64 in the model's
         conventions, adp_size (n,) int32, occupancy (n,) float64, dTarget_dF (n_hkl,) complex128
         (None for F only), optional local frames (n,3,3), anomalous (n,) complex128 and
         countAtomContribution (n,) uint8.  Returns (F, dxyz, dadp, docc) as new GPU tensors (the
-        derivatives None for F only)."""
+        derivatives None for F only), or writes them into `out` = (F, dxyz, dadp, docc) tensors.
+        no_alloc: the allocation-free entry (dh_hc_sf_and_derivs_dev_noalloc; with `out` given
+        nothing is allocated on the Python side either)."""
         import torch
         sel = (derivativesSelector or DerivativesSelector()).bits() if dTarget_dF is not None else 0
         a, keep, dev = self._dev_args(xyz, adp_size, adp, occupancy, localCoordinateSystems, anomalous,
                                       countAtomContribution)
         nh, n = getattr(self, "_dev_nh", 0), self.n_atoms
         device = xyz.device
-        f = torch.empty(nh, dtype=torch.complex128, device=device)
+        if out is not None:
+            f, outs = out[0], list(out[1:])
+            dev(torch.view_as_real(f), torch.float64, (nh, 2), "F output")
+            for o, shape in zip(outs, ((n, 3), (n, 6), (n,))):
+                dev(o, torch.float64, shape, "derivative output")
+        else:
+            f = torch.empty(nh, dtype=torch.complex128, device=device)
+            outs = [torch.empty(s, dtype=torch.float64, device=device) for s in ((n, 3), (n, 6), (n,))] if sel \
+                else [None] * 3
         dd = None
         if dTarget_dF is not None:
             dd = dev(torch.view_as_real(dTarget_dF), torch.float64, (nh, 2), "dTarget_dF")
-        outs = [torch.empty(s, dtype=torch.float64, device=device) for s in ((n, 3), (n, 6), (n,))] if sel else [None] * 3
         st = stream if stream is not None else torch.cuda.current_stream(device)
-        _lib.check(self._lib.dh_hc_sf_and_derivs_dev(
-            self._h, ctypes.byref(a), dd, sel, torch.view_as_real(f).data_ptr() if nh else None,
-            *(o.data_ptr() if o is not None else None for o in outs), ctypes.c_void_p(st.cuda_stream)))
+        fn = self._lib.dh_hc_sf_and_derivs_dev_noalloc if no_alloc else self._lib.dh_hc_sf_and_derivs_dev
+        _lib.check(fn(self._h, ctypes.byref(a), dd, sel, torch.view_as_real(f).data_ptr() if nh else None,
+                      *(o.data_ptr() if o is not None else None for o in outs), ctypes.c_void_p(st.cuda_stream)))
         return (f,) + tuple(outs)
 
     # -- per-reflection derivatives (the per-hkl path) -------------------------------------------

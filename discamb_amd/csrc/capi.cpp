@@ -903,6 +903,16 @@ int dh_hc_sf_and_derivs_dev(dh_calculator* calc, const dh_atoms_dev* atoms, cons
     });
 }
 
+int dh_hc_sf_and_derivs_dev_noalloc(dh_calculator* calc, const dh_atoms_dev* atoms, const double* dtarget_df,
+                                    unsigned selector, double* f, double* d_xyz, double* d_adp, double* d_occ,
+                                    void* stream)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        sf_dev(calc->c, atoms, dtarget_df, selector, f, d_xyz, d_adp, d_occ, stream, true);
+    });
+}
+
 int dh_hc_autotune(dh_calculator* calc, unsigned selector, int* chunk)
 {
     return guarded([&] {

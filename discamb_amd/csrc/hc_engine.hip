@@ -21,6 +21,7 @@
 #include <numeric>
 #include <string>
 
+#include "../../include/discamb_amd.h"
 #include "hc_device_math.hpp"
 #include "hc_fp64_trig.hpp"
 #include "hc_engine.hpp"
@@ -2151,7 +2152,7 @@ static void host_ylm(const double* u, double* d)
 // share it, so allocation-free calls that overlap on one GPU race.  Host code never passes the
 // array's name to a kernel (in host code it is a stand-in, not a device address): it takes the
 // device address once per device with hipGetSymbolAddress and passes that.
-static constexpr size_t kScratchBytes = (size_t)128 << 20;
+static constexpr size_t kScratchBytes = DH_NOALLOC_SCRATCH_BYTES;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 static std::mutex g_scratch_mu;
@@ -3108,8 +3109,10 @@ struct DevScope {
     bool* timing_slot;
     bool old_timing;
     int* chunk_slot;
-    DevScope(hipStream_t* slot, hipStream_t st, bool* timing, int* chunk)
-        : stream_slot(slot), old_stream(*slot), timing_slot(timing), old_timing(*timing), chunk_slot(chunk)
+    bool* ran_slot;
+    DevScope(hipStream_t* slot, hipStream_t st, bool* timing, int* chunk, bool* ran)
+        : stream_slot(slot), old_stream(*slot), timing_slot(timing), old_timing(*timing), chunk_slot(chunk),
+          ran_slot(ran)
     {
         *slot = st;
         *timing = false;     // no host synchronisation inside a device-resident call
@@ -3129,6 +3132,7 @@ struct DevScope {
         *stream_slot = old_stream;
         *timing_slot = old_timing;
         *chunk_slot = 0;
+        *ran_slot = false;     // results went to the caller's memory (or the call failed)
         g_no_alloc = false;
     }
 };
@@ -3150,7 +3154,7 @@ void Engine::run_dev(const DevCall& call, Precision prec, hipStream_t stream, vo
     const int n_waves = n_sorted_ / kWave;
     AtomIn* skel = d_atoms_in_.as<AtomIn>();
     const int* skel_cls = reinterpret_cast<const int*>(static_cast<const char*>(d_atoms_in_.p) + n_ain * sizeof(AtomIn));
-    DevScope scope(&stream_, stream, &timing, &chunk_override_);
+    DevScope scope(&stream_, stream, &timing, &chunk_override_, &ran_);
     if (scratch_mem) {
         // carve the per-call buffers; the reflection chunk is lengthened if the derivative partials
         // of the usual launch shape do not fit
@@ -3227,8 +3231,6 @@ void Engine::run_dev(const DevCall& call, Precision prec, hipStream_t stream, vo
                            call.f, call.d_xyz, call.d_adp, call.d_occ);
         HIP_CHECK(hipGetLastError());
     }
-    // results went to the caller's memory; nothing is left to fetch
-    ran_ = false;
 }
 
 // ------------------------------------------------------------------ per-reflection rows

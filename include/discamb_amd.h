@@ -189,7 +189,8 @@ int dh_hc_engine_info(dh_calculator* calc, int* n_explicit_ops, int* group_order
  * vectors (hc_sf_gpu::calculate_multipolar_structure_factors_gpu, src/GPU/cuda/HC_SF_GPU.h:10-69).
  * The reflection list is set once from host memory; it fixes the reflection order, the centring
  * multipliers and the reflection records, which every later device call on this calculator
- * reuses (a host call, dh_hc_upload or a change of calculation mode requires setting it again). */
+ * reuses.  A host call or dh_hc_upload on the calculator replaces it (set it again); after a change
+ * of calculation mode the device call prepares what the new mode needs on its first call. */
 int dh_hc_set_reflections(dh_calculator* calc, int n_hkl, const int* hkl);
 /* Atom parameters in device memory, layouts as dh_atoms (n_atoms = the model's atom count).
  * adp_size: 1 (U_iso) or 6; any other value means no ADP.  lcs, anomalous, include may be NULL
@@ -213,6 +214,23 @@ typedef struct {
  * Argument errors are reported synchronously; device-side values are not validated. */
 int dh_hc_sf_and_derivs_dev(dh_calculator* calc, const dh_atoms_dev* atoms, const double* dtarget_df,
                             unsigned selector, double* f, double* d_xyz, double* d_adp, double* d_occ, void* stream);
+/* dh_hc_sf_and_derivs_dev without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid allocation).  Same arguments, results and stream ordering; the call makes
+ * no hipMalloc / hipFree / pinned or library allocation.  Its per-call device buffers (atom
+ * records, sorted dT/dF, per-wave F and per-chunk derivative partial sums, sorted F, raw
+ * derivatives) come from one static scratch array of DH_NOALLOC_SCRATCH_BYTES per GPU, which every
+ * calculator on that GPU shares and which keeps the last call's contents: allocation-free calls
+ * that overlap on one GPU (two host threads, or two streams without ordering between them) race.
+ * Everything a call reads from the scratch it writes first.  The reflection records come from
+ * dh_hc_set_reflections, which must have run in the current calculation mode (and electron
+ * setting).  A call that needs more than the scratch (about 5000 atoms x 66 000 reflections in
+ * FP64; the reflection chunk is lengthened to fit the derivative partials) or records not
+ * prepared returns DH_E_MEMORY without computing anything. */
+#define DH_NOALLOC_SCRATCH_BYTES (128u << 20)
+int dh_hc_sf_and_derivs_dev_noalloc(dh_calculator* calc, const dh_atoms_dev* atoms, const double* dtarget_df,
+                                    unsigned selector, double* f, double* d_xyz, double* d_adp, double* d_occ,
+                                    void* stream);
 
 /* ---- Per-reflection derivatives: the reference's per-hkl overload
  *   template<T> calculateStructureFactorsAndDerivatives(atoms, lcs, hkl, T& singleHklResultsProcessor

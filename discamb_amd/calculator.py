@@ -224,12 +224,15 @@ class HansenCoppensStructureFactorCalculator:
     def setTuning(self, hkl_chunk: int):
         _lib.check(self._lib.dh_hc_set_tuning(self._h, int(hkl_chunk)))
 
-    def setKernelOptions(self, general_rotations: bool = False, radial: str = "auto"):
+    def setKernelOptions(self, general_rotations: bool = False, radial: str = "auto", direct_phases: bool = False):
         """general_rotations: run the general-rotation kernels even for groups whose operations
         are all diagonal sign matrices (those otherwise use the parity-folded kernels).
         radial: "records" (per-(reflection, type) tables), "tables" (per-(atom, reflection)
-        lookups in per-wavefunction tables) or "auto"."""
-        flags = (1 if general_rotations else 0) | {"auto": 0, "records": 2, "tables": 4}[radial]
+        lookups in per-wavefunction tables) or "auto".
+        direct_phases: FP64 terms evaluate every phase and Debye-Waller factor directly instead of
+        carrying them along hkl lines."""
+        flags = (1 if general_rotations else 0) | {"auto": 0, "records": 2, "tables": 4}[radial] | \
+            (8 if direct_phases else 0)
         _lib.check(self._lib.dh_hc_set_options(self._h, flags))
 
     def engineInfo(self):

@@ -333,6 +333,7 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
         dd[2 * k + 1] = im;
         dd[2 * (size_t)n_hkl + k] = m;
     }
+    c->eng().line_order = c->prec == kFp64;     // the FP64 kernels carry phases along hkl lines
     c->eng().upload(ea, n_hkl, hkl, dd.data(), rows);
     c->n_atoms = n;
     c->n_hkl = n_hkl;
@@ -575,6 +576,7 @@ static void set_reflections(Calculator* c, int n_hkl, const int* hkl)
         }
     }
     conv.centric_origin = c->eng_centric_origin() ? 1 : 0;
+    c->eng().line_order = c->prec == kFp64;
     c->eng().prepare_dev(ea, n_hkl, hkl, mult.data(), c->mult_w, anom, conv, c->prec);
     c->n_atoms = n;
     c->n_hkl = n_hkl;
@@ -835,13 +837,15 @@ int dh_hc_set_tuning(dh_calculator* calc, int hkl_chunk)
 int dh_hc_set_options(dh_calculator* calc, unsigned flags)
 {
     return guarded([&] {
-        const unsigned all = DH_OPT_GENERAL_ROTATIONS | DH_OPT_RADIAL_RECORDS | DH_OPT_RADIAL_TABLES;
+        const unsigned all = DH_OPT_GENERAL_ROTATIONS | DH_OPT_RADIAL_RECORDS | DH_OPT_RADIAL_TABLES |
+                             DH_OPT_DIRECT_PHASES;
         if (!calc || (flags & ~all) || ((flags & DH_OPT_RADIAL_RECORDS) && (flags & DH_OPT_RADIAL_TABLES)))
             throw Error(DH_E_ARGUMENT, "invalid options");
         for (Engine* e : {calc->c->engine.get(), calc->c->iam_engine.get()}) {
             if (!e) continue;
             e->tune.general_path = (flags & DH_OPT_GENERAL_ROTATIONS) != 0;
             e->tune.radial_mode = (flags & DH_OPT_RADIAL_RECORDS) ? 1 : (flags & DH_OPT_RADIAL_TABLES) ? 2 : 0;
+            e->tune.direct_phases = (flags & DH_OPT_DIRECT_PHASES) != 0;
         }
     });
 }

@@ -541,6 +541,42 @@ __global__ void k_hkl_prep_diag(const int* __restrict__ hkl, const OpDev* __rest
     if (k < n_hkl) hkl_prep_diag_item(k, hkl, ops, n_ops, c2f, rec32, hd, ht, hlen);
 }
 
+// FP64 diagonal-group line kernels: one record per reflection, [d_j(h) x 25, m(h) x 6, h x 3,
+// h.t_s for s < n_ops (<= 4), pad] (kDiag64Rec doubles), and |h|.
+__global__ void k_hkl_prep_diag64(const int* __restrict__ hkl, const OpDev* __restrict__ ops, int n_ops, int n_hkl,
+                                  const double* __restrict__ c2f, double* __restrict__ rec, double* __restrict__ hlen)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_hkl) return;
+    const int h0 = hkl[3 * k], h1 = hkl[3 * k + 1], h2 = hkl[3 * k + 2];
+    double hc[3];
+    for (int j = 0; j < 3; j++) hc[j] = h0 * c2f[j] + h1 * c2f[3 + j] + h2 * c2f[6 + j];
+    const double hl = sqrt(hc[0] * hc[0] + hc[1] * hc[1] + hc[2] * hc[2]);
+    hlen[k] = hl;
+    double y[25];
+    if (hl < 1e-10) {
+        for (int i = 0; i < 25; i++) y[i] = 0.0;
+    } else {
+        const double inv = 1.0 / hl;
+        ylm_density<double>(hc[0] * inv, hc[1] * inv, hc[2] * inv, y);
+    }
+    double* r = rec + (size_t)k * 40;
+    for (int i = 0; i < 25; i++) r[i] = y[i];
+    r[25] = hc[0] * hc[0];
+    r[26] = hc[1] * hc[1];
+    r[27] = hc[2] * hc[2];
+    r[28] = 2 * hc[0] * hc[1];
+    r[29] = 2 * hc[0] * hc[2];
+    r[30] = 2 * hc[1] * hc[2];
+    r[31] = h0;
+    r[32] = h1;
+    r[33] = h2;
+    for (int s = 0; s < 6; s++) {
+        const OpDev* o = s < n_ops ? ops + s : nullptr;
+        r[34 + s] = o ? h0 * o->tf[0] + h1 * o->tf[1] + h2 * o->tf[2] : 0.0;
+    }
+}
+
 __global__ __launch_bounds__(64) void k_tab_build(const TabRec* __restrict__ tabs, int n_wfn,
                                                   const WfnRec* __restrict__ wfns,
                             const TermRec* __restrict__ core, const TermRec* __restrict__ val, int n_nodes,
@@ -613,6 +649,8 @@ struct MainArgs {
     const void* tab;          // on-the-fly radial functions: per-wavefunction table (FP32 mixed / FP64)
     const TabRec* tabs;       // node range of each wavefunction in tab
     const double* hlen;       // |h| per reflection
+    const int* line_cont;     // FP64 line kernels: 1 where reflection k continues the line of k - 1
+    const double* line_ops;   // FP64 line kernels: per operation [Delta R_f (3), Delta.t, m(Delta_s) (6)]
 };
 
 // Pointers read out of argument structs in memory (the batched kernels) are generic to the
@@ -896,6 +934,152 @@ __device__ __forceinline__ void body_fp64(const MainArgs& a, int wave, int lane,
     }
 }
 
+// FP64 with the phase and Debye-Waller factors carried along reflection lines (the reference's
+// line recurrences, scattering_utilities.cpp:520-756, used at HansenCoppens_SF_Engine3.cpp:
+// 1258-1288).  Reflections are in line order: consecutive reflections k-1, k of one line differ
+// by Delta = one step along a fixed hkl axis (line_cont[k] = 1).  For operation s with h_s = h R_s,
+// x the atom's fractional position and U its (2 pi^2) Cartesian ADP quadratic form:
+//   phase:  exp(i phi(h + Delta)) = exp(i phi(h)) M_s,   M_s = exp(2 pi i (Delta R_f . x + Delta . t_s))
+//   DW:     T(h + Delta) = T(h) q,   q <- q c_s,   c_s = exp(-2 m(Delta_s) . u)
+// so a term continuing a line costs one complex and two real products for E = T exp(i phi) instead
+// of sin, cos and exp.  Lines and work chunks start with direct evaluation (q from the next
+// reflection's Debye-Waller multipliers); the recurrence error grows by a few ulp per step.  The
+// lane keeps E, q, M, c of every operation in registers, hence the compile-time operation count.
+template <int L, bool DERIV, bool OTF, int NOPS>
+__device__ __forceinline__ void body_fp64_line(const MainArgs& a, int wave, int lane, int chunk_id, int k0, int k1,
+                                               const unsigned* __restrict__ hop, const double* __restrict__ rad,
+                                               const unsigned* __restrict__ dtdf32, double2* __restrict__ stage,
+                                               double* __restrict__ fpart, double* __restrict__ dpart)
+{
+    constexpr int NLM = (L + 1) * (L + 1);
+    const int slot = wave * kWave + lane;
+    const AtomRec& at = cmem(a.atoms)[slot];
+    const double x0 = at.xf[0], x1 = at.xf[1], x2 = at.xf[2];
+    double u[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) u[j] = at.u[j];
+    double P[NLM];
+#pragma unroll
+    for (int j = 0; j < NLM; j++) P[j] = at.plm[j];
+    const double w = at.w, anr = at.anom[0], ani = at.anom[1];
+    const int t = at.type;
+    TabRec tr{0, 2};
+    double pv = 0, ik = 0, ikp = 0;
+    if (OTF) {
+        tr = cmem(a.tabs)[at.wfn];
+        pv = at.pval;
+        ik = at.inv_kappa;
+        ikp = at.inv_kappa_p;
+    }
+    const double* tab = cmem(static_cast<const double*>(a.tab));
+    const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
+    const int* cont = cmem(a.line_cont);
+    const unsigned* lop = cmem(reinterpret_cast<const unsigned*>(a.line_ops));
+    // per-(atom, operation) line multipliers
+    double Mr[NOPS], Mi[NOPS], Cq[NOPS], Er[NOPS], Ei[NOPS], Q[NOPS];
+#pragma unroll
+    for (int s = 0; s < NOPS; s++) {
+        const unsigned* o = lop + s * 20;
+        const double ph = __builtin_amdgcn_fract(fma(ld_f64(o, 2), x2, fma(ld_f64(o, 1), x1, fma(ld_f64(o, 0), x0, ld_f64(o, 3)))));
+        sincos_rev(ph, &Mi[s], &Mr[s]);
+        double e2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; j++) e2 = fma(ld_f64(o, 4 + j), u[j], e2);
+        Cq[s] = exp_neg(2.0 * e2);
+        Er[s] = Ei[s] = Q[s] = 0.0;
+    }
+    double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
+    for (int kb = k0; kb < k1; kb += kStage) {
+    const int nk = min(kStage, k1 - kb);
+    for (int kk = 0; kk < nk; ++kk) {
+        const int k = kb + kk;
+        // wave-uniform: every lane is on reflection k
+        const bool along = k > k0 && cont[k] != 0;
+        if (along) {
+#pragma unroll
+            for (int s = 0; s < NOPS; s++) {
+                const double qr = Q[s] * Mr[s], qi = Q[s] * Mi[s];
+                const double er = Er[s] * qr - Ei[s] * qi;
+                const double ei = Er[s] * qi + Ei[s] * qr;
+                Er[s] = er;
+                Ei[s] = ei;
+                Q[s] *= Cq[s];
+            }
+        } else {
+            const bool next = k + 1 < k1 && cont[k + 1] != 0;
+#pragma unroll
+            for (int s = 0; s < NOPS; s++) {
+                const unsigned* rw = hop + ((size_t)k * NOPS + s) * (2 * kHklopRec);
+                // h.(R x + t) reduced to [0, 1) revolutions
+                const double ph = __builtin_amdgcn_fract(
+                    fma(ld_f64(rw, 33), x2, fma(ld_f64(rw, 32), x1, fma(ld_f64(rw, 31), x0, ld_f64(rw, 34)))));
+                double sn, cs;
+                sincos_rev(ph, &sn, &cs);
+                double e = 0.0;
+#pragma unroll
+                for (int j = 0; j < 6; j++) e = fma(ld_f64(rw, 25 + j), u[j], e);
+                const double T = exp_neg(e);
+                Er[s] = T * cs;
+                Ei[s] = T * sn;
+                if (next) {
+                    const unsigned* rn = hop + ((size_t)(k + 1) * NOPS + s) * (2 * kHklopRec);
+                    double e1 = 0.0;
+#pragma unroll
+                    for (int j = 0; j < 6; j++) e1 = fma(ld_f64(rn, 25 + j), u[j], e1);
+                    Q[s] = exp_neg(e1 - e);
+                }
+            }
+        }
+        double S, G0, G1, G2, G3, G4;
+        if (OTF) {
+            double G[5];
+            radial_otf<L, double>(tab, tr, ld_f64(hl32, k), pv, ik, ikp, anr, S, G);
+            G0 = G[0]; G1 = G[1]; G2 = G[2]; G3 = G[3]; G4 = G[4];
+        } else {
+            const double* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
+            S = rr[0] + anr; G0 = rr[1]; G1 = rr[2]; G2 = rr[3]; G3 = rr[4]; G4 = rr[5];
+        }
+        const double cr = ld_f64(dtdf32, 2 * k), ci = ld_f64(dtdf32, 2 * k + 1);
+        double sr = 0, si = 0;
+#pragma unroll
+        for (int s = 0; s < NOPS; s++) {
+            const unsigned* rw = hop + ((size_t)k * NOPS + s) * (2 * kHklopRec);
+            double r[kHklopRec];
+#pragma unroll
+            for (int j = 0; j < kHklopRec; j++) r[j] = ld_f64(rw, j);
+            double A[5] = {0, 0, 0, 0, 0};
+            contract<L, double>(P, r, A);
+            double fr = S + G0 * A[0] + G2 * A[2];
+            double fi = ani * r[35] + G1 * A[1];
+            if (L == 4) {
+                fr += G4 * A[4];
+                fi += G3 * A[3];
+            }
+            const double Xr = fr * Er[s] - fi * Ei[s], Xi = fr * Ei[s] + fi * Er[s];
+            sr += Xr;
+            si += Xi;
+            if (DERIV) {
+                const double zr = cr * Xr - ci * Xi, zi = cr * Xi + ci * Xr;
+#pragma unroll
+                for (int j = 0; j < 6; j++) du[j] -= r[25 + j] * zr;
+#pragma unroll
+                for (int j = 0; j < 3; j++) dx[j] -= r[31 + j] * zi;
+            }
+        }
+        if (DERIV) docc += cr * sr - ci * si;
+        stage_f64(stage, kk, lane, w * sr, w * si);
+    }
+    flush_f64(stage, nk, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
+    }
+    if (DERIV) {
+        double* dp = dpart + ((size_t)chunk_id * a.n_slots + slot) * kNdBasic;
+        dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
+#pragma unroll
+        for (int j = 0; j < 6; j++) dp[3 + j] = du[j];
+        dp[9] = docc;
+    }
+}
+
 // Mixed precision: FP64 phase range reduction and FP64 accumulation across reflections, FP32
 // harmonics / radial / Debye-Waller / sincos / exp per term.
 template <int L, bool DERIV, int MP, bool OTF>
@@ -1130,6 +1314,228 @@ __device__ __forceinline__ void signed_contraction(const R (&C)[5][8], R* A)
 }
 
 // per-reflection values shared by all operations of one atom (working precision R)
+// FP64 line kernel for diagonal groups (every rotation a sign matrix, at most four explicit
+// operations): one record per reflection, [d_j(h) x 25, m(h) x 6, h x 3, h.t_s per operation,
+// pad], instead of one per (reflection, operation).  Operation s has sign pattern
+// (PATS >> 3 s) & 7 (bit i: axis i flips), so d_j(h_s) = +-d_j(h), the cross Debye-Waller
+// multipliers and h_s flip sign by parity and the flips are compile-time negations of the
+// record's scalar operands.  Phases and Debye-Waller factors are carried along the lines exactly
+// as in body_fp64_line.
+static constexpr int kDiag64Rec = 40;
+// Sign-pattern sets with an FP64 diagonal line kernel (bit 12 marks the code; op s in bits 3s..3s+2):
+// monoclinic 2-fold along y (P2_1/c, C2/c), along z and along x, an inversion partner listed
+// explicitly (a centre off the origin), and the orthorhombic 222 set in the order identity, 2z, 2y, 2x.
+static constexpr int kDiag64Pats0 = 4096 | (5 << 3);
+static constexpr int kDiag64Pats1 = 4096 | (3 << 3);
+static constexpr int kDiag64Pats2 = 4096 | (6 << 3);
+static constexpr int kDiag64Pats3 = 4096 | (7 << 3);
+static constexpr int kDiag64Pats4 = 4096 | (3 << 3) | (5 << 6) | (6 << 9);
+
+template <int PAT>
+__device__ __forceinline__ constexpr double psign(int key)
+{
+    return odd_under(key, PAT) ? -1.0 : 1.0;
+}
+
+template <int L, int PAT>
+__device__ __forceinline__ void contract_signed(const double* P, const double* Y, double* A)
+{
+    constexpr int NLM = (L + 1) * (L + 1);
+    A[0] = P[0] * Y[0];
+    A[1] = fma(P[3], psign<PAT>(harmonic_parity(3)) * Y[3],
+               fma(P[2], psign<PAT>(harmonic_parity(2)) * Y[2], P[1] * (psign<PAT>(harmonic_parity(1)) * Y[1])));
+    double a2 = 0;
+#pragma unroll
+    for (int j = 4; j < 9; j++) a2 = fma(P[j], psign<PAT>(harmonic_parity(j)) * Y[j], a2);
+    A[2] = a2;
+    A[3] = A[4] = 0.0;
+    if (L == 4) {
+        double a3 = 0, a4 = 0;
+#pragma unroll
+        for (int j = 9; j < 16; j++) a3 = fma(P[j < NLM ? j : 0], psign<PAT>(harmonic_parity(j)) * Y[j], a3);
+#pragma unroll
+        for (int j = 16; j < 25; j++) a4 = fma(P[j < NLM ? j : 0], psign<PAT>(harmonic_parity(j)) * Y[j], a4);
+        A[3] = a3;
+        A[4] = a4;
+    }
+}
+
+// Debye-Waller exponent m(h_s) . u with the cross terms signed by the pattern
+template <int PAT>
+__device__ __forceinline__ double dw_signed(const double* m, const double* u)
+{
+    double e = m[0] * u[0];
+    e = fma(m[1], u[1], e);
+    e = fma(m[2], u[2], e);
+    e = fma(psign<PAT>(3) * m[3], u[3], e);
+    e = fma(psign<PAT>(5) * m[4], u[4], e);
+    e = fma(psign<PAT>(6) * m[5], u[5], e);
+    return e;
+}
+
+template <int L, bool DERIV, bool OTF, int NOPS, int PATS>
+__device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave, int lane, int chunk_id, int k0,
+                                                    int k1, const unsigned* __restrict__ hop,
+                                                    const double* __restrict__ rad, const unsigned* __restrict__ dtdf32,
+                                                    double2* __restrict__ stage, double* __restrict__ fpart,
+                                                    double* __restrict__ dpart)
+{
+    constexpr int NLM = (L + 1) * (L + 1);
+    const int slot = wave * kWave + lane;
+    const AtomRec& at = cmem(a.atoms)[slot];
+    const double x[3] = {at.xf[0], at.xf[1], at.xf[2]};
+    double u[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) u[j] = at.u[j];
+    double P[NLM];
+#pragma unroll
+    for (int j = 0; j < NLM; j++) P[j] = at.plm[j];
+    const double w = at.w, anr = at.anom[0], ani = at.anom[1];
+    const int t = at.type;
+    TabRec tr{0, 2};
+    double pv = 0, ik = 0, ikp = 0;
+    if (OTF) {
+        tr = cmem(a.tabs)[at.wfn];
+        pv = at.pval;
+        ik = at.inv_kappa;
+        ikp = at.inv_kappa_p;
+    }
+    const double* tab = cmem(static_cast<const double*>(a.tab));
+    const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
+    const int* cont = cmem(a.line_cont);
+    const unsigned* lop = cmem(reinterpret_cast<const unsigned*>(a.line_ops));
+    double Mr[NOPS], Mi[NOPS], Cq[NOPS], Er[NOPS], Ei[NOPS], Q[NOPS], an[NOPS];
+#pragma unroll
+    for (int s = 0; s < NOPS; s++) {
+        const unsigned* o = lop + s * 20;
+        const double ph = __builtin_amdgcn_fract(fma(ld_f64(o, 2), x[2], fma(ld_f64(o, 1), x[1], fma(ld_f64(o, 0), x[0], ld_f64(o, 3)))));
+        sincos_rev(ph, &Mi[s], &Mr[s]);
+        double e2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; j++) e2 = fma(ld_f64(o, 4 + j), u[j], e2);
+        Cq[s] = exp_neg(2.0 * e2);
+        Er[s] = Ei[s] = Q[s] = 0.0;
+        an[s] = (double)a.diag_anom[(PATS >> (3 * s)) & 7] * ani;
+    }
+    double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
+    for (int kb = k0; kb < k1; kb += kStage) {
+    const int nk = min(kStage, k1 - kb);
+    for (int kk = 0; kk < nk; ++kk) {
+        const int k = kb + kk;
+        const unsigned* rw = hop + (size_t)k * (2 * kDiag64Rec);
+        double m[6], hf[3];
+#pragma unroll
+        for (int j = 0; j < 6; j++) m[j] = ld_f64(rw, 25 + j);
+#pragma unroll
+        for (int j = 0; j < 3; j++) hf[j] = ld_f64(rw, 31 + j);
+        const bool along = k > k0 && cont[k] != 0;
+        if (along) {
+#pragma unroll
+            for (int s = 0; s < NOPS; s++) {
+                const double qr = Q[s] * Mr[s], qi = Q[s] * Mi[s];
+                const double er = Er[s] * qr - Ei[s] * qi;
+                const double ei = Er[s] * qi + Ei[s] * qr;
+                Er[s] = er;
+                Ei[s] = ei;
+                Q[s] *= Cq[s];
+            }
+        } else {
+            const bool next = k + 1 < k1 && cont[k + 1] != 0;
+            double mn[6];
+            if (next) {
+                const unsigned* rn = hop + (size_t)(k + 1) * (2 * kDiag64Rec);
+#pragma unroll
+                for (int j = 0; j < 6; j++) mn[j] = ld_f64(rn, 25 + j);
+            }
+#pragma unroll
+            for (int s = 0; s < NOPS; s++) {
+                double sn, cs, e, e1 = 0.0;
+                const double hts = ld_f64(rw, 34 + s);
+                switch ((PATS >> (3 * s)) & 7) {
+#define DIAG64_DIRECT(PT)                                                                                       \
+    case PT: {                                                                                                \
+        const double ph = __builtin_amdgcn_fract(                                                             \
+            fma(psign<PT>(4) * hf[2], x[2], fma(psign<PT>(2) * hf[1], x[1], fma(psign<PT>(1) * hf[0], x[0], hts)))); \
+        sincos_rev(ph, &sn, &cs);                                                                             \
+        e = dw_signed<PT>(m, u);                                                                              \
+        if (next) e1 = dw_signed<PT>(mn, u);                                                                  \
+        break;                                                                                                \
+    }
+                    DIAG64_DIRECT(0) DIAG64_DIRECT(1) DIAG64_DIRECT(2) DIAG64_DIRECT(3)
+                    DIAG64_DIRECT(4) DIAG64_DIRECT(5) DIAG64_DIRECT(6) DIAG64_DIRECT(7)
+#undef DIAG64_DIRECT
+                }
+                const double T = exp_neg(e);
+                Er[s] = T * cs;
+                Ei[s] = T * sn;
+                if (next) Q[s] = exp_neg(e1 - e);
+            }
+        }
+        double S, G0, G1, G2, G3, G4;
+        if (OTF) {
+            double G[5];
+            radial_otf<L, double>(tab, tr, ld_f64(hl32, k), pv, ik, ikp, anr, S, G);
+            G0 = G[0]; G1 = G[1]; G2 = G[2]; G3 = G[3]; G4 = G[4];
+        } else {
+            const double* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
+            S = rr[0] + anr; G0 = rr[1]; G1 = rr[2]; G2 = rr[3]; G3 = rr[4]; G4 = rr[5];
+        }
+        const double cr = ld_f64(dtdf32, 2 * k), ci = ld_f64(dtdf32, 2 * k + 1);
+        double Y[25];
+#pragma unroll
+        for (int j = 0; j < 25; j++) Y[j] = j < NLM ? ld_f64(rw, j) : 0.0;
+        double sr = 0, si = 0;
+#pragma unroll
+        for (int s = 0; s < NOPS; s++) {
+            double A[5] = {0, 0, 0, 0, 0};
+            double zr = 0, zi = 0;
+            switch ((PATS >> (3 * s)) & 7) {
+#define DIAG64_TERM(PT)                                                                                       \
+    case PT: {                                                                                                \
+        contract_signed<L, PT>(P, Y, A);                                                                      \
+        double fr = S + G0 * A[0] + G2 * A[2];                                                                \
+        double fi = an[s] + G1 * A[1];                                                                        \
+        if (L == 4) {                                                                                         \
+            fr += G4 * A[4];                                                                                  \
+            fi += G3 * A[3];                                                                                  \
+        }                                                                                                     \
+        const double Xr = fr * Er[s] - fi * Ei[s], Xi = fr * Ei[s] + fi * Er[s];                              \
+        sr += Xr;                                                                                             \
+        si += Xi;                                                                                             \
+        if (DERIV) {                                                                                          \
+            zr = cr * Xr - ci * Xi;                                                                           \
+            zi = cr * Xi + ci * Xr;                                                                           \
+            du[0] -= m[0] * zr;                                                                               \
+            du[1] -= m[1] * zr;                                                                               \
+            du[2] -= m[2] * zr;                                                                               \
+            du[3] -= psign<PT>(3) * m[3] * zr;                                                                \
+            du[4] -= psign<PT>(5) * m[4] * zr;                                                                \
+            du[5] -= psign<PT>(6) * m[5] * zr;                                                                \
+            dx[0] -= psign<PT>(1) * hf[0] * zi;                                                               \
+            dx[1] -= psign<PT>(2) * hf[1] * zi;                                                               \
+            dx[2] -= psign<PT>(4) * hf[2] * zi;                                                               \
+        }                                                                                                     \
+        break;                                                                                                \
+    }
+                DIAG64_TERM(0) DIAG64_TERM(1) DIAG64_TERM(2) DIAG64_TERM(3)
+                DIAG64_TERM(4) DIAG64_TERM(5) DIAG64_TERM(6) DIAG64_TERM(7)
+#undef DIAG64_TERM
+            }
+        }
+        if (DERIV) docc += cr * sr - ci * si;
+        stage_f64(stage, kk, lane, w * sr, w * si);
+    }
+    flush_f64(stage, nk, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
+    }
+    if (DERIV) {
+        double* dp = dpart + ((size_t)chunk_id * a.n_slots + slot) * kNdBasic;
+        dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
+#pragma unroll
+        for (int j = 0; j < 6; j++) dp[3 + j] = du[j];
+        dp[9] = docc;
+    }
+}
+
 template <typename R>
 struct DiagRefl {
     R C[5][8];             // parity-class sums of P_j d_j(h)
@@ -1357,11 +1763,12 @@ __device__ __forceinline__ void body_mixed_diag(const MainArgs& a, int wave, int
 
 // One (atom block, reflection chunk) work unit: wave wib of the block runs atom wave
 // ablock * kWavesPerBlock + wib over the reflections of chunk chunk_id.
-template <bool MIXED, bool DERIV, int MP, bool DIAG, bool OTF>
+template <bool MIXED, bool DERIV, int MP, bool DIAG, bool OTF, int NOPS = 0, int DPATS = 0>
 __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int ablock, int wib, int lane, void* stage)
 {
     static_assert(MIXED || !DIAG, "the parity-folded kernels are mixed precision only");
     static_assert(!(MP && OTF), "the multipole adjoint uses radial records");
+    static_assert(NOPS == 0 || (!MIXED && MP == 0), "the line recurrences are FP64 without the multipole adjoint");
     const int wave = ablock * kWavesPerBlock + wib;
     if (wave >= a.n_waves) return;
     const int k0 = chunk_id * a.chunk;
@@ -1394,7 +1801,22 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
         const double* rad = cmem(static_cast<const double*>(a.radial));
         const double* radx = cmem(static_cast<const double*>(a.radx));
         double2* st = static_cast<double2*>(stage);
-        if (cls <= 2) {
+        if (NOPS > 0 && DPATS > 0) {
+            constexpr int N = NOPS > 0 ? NOPS : 1;
+            constexpr int PS = DPATS & 4095;
+            if (cls <= 2)
+                body_fp64_line_diag<2, DERIV, OTF, N, PS>(a, wave, lane, chunk_id, k0, k1, hop, rad, dtdf32, st, fpart,
+                                                          dpart);
+            else
+                body_fp64_line_diag<4, DERIV, OTF, N, PS>(a, wave, lane, chunk_id, k0, k1, hop, rad, dtdf32, st, fpart,
+                                                          dpart);
+        } else if (NOPS > 0) {
+            constexpr int N = NOPS > 0 ? NOPS : 1;
+            if (cls <= 2)
+                body_fp64_line<2, DERIV, OTF, N>(a, wave, lane, chunk_id, k0, k1, hop, rad, dtdf32, st, fpart, dpart);
+            else
+                body_fp64_line<4, DERIV, OTF, N>(a, wave, lane, chunk_id, k0, k1, hop, rad, dtdf32, st, fpart, dpart);
+        } else if (cls <= 2) {
             body_fp64<2, DERIV, MP, OTF>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, dtdf32, st, fpart, dpart);
         } else {
             body_fp64<4, DERIV, MP, OTF>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, dtdf32, st, fpart, dpart);
@@ -1411,7 +1833,7 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
 // split so that neither pass holds both the atom's P_lm and the B_lm accumulators; 3 replaces 1
 // when only the B_lm are wanted (the chain rule through the local frames): the plain pass in the
 // multipole row layout.
-template <bool MIXED, bool DERIV, int MP, bool DIAG, bool OTF>
+template <bool MIXED, bool DERIV, int MP, bool DIAG, bool OTF, int NOPS = 0, int DPATS = 0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && (MP == 0 || MP == 3) ? (DIAG || OTF ? 4 : 5) : (!MIXED && MP == 2 ? 3 : 1)))) void k_main(MainArgs a)
 {
     constexpr int kStageBytes = MIXED ? kStageBytesF32 : kStageBytesF64;
@@ -1421,8 +1843,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && (M
     if (logical >= a.n_blocks) return;
     const int chunk_id = logical / a.grid_x;
     const int wib = (int)(threadIdx.x / kWave);
-    main_unit<MIXED, DERIV, MP, DIAG, OTF>(a, chunk_id, logical - chunk_id * a.grid_x, wib, threadIdx.x & (kWave - 1),
-                                           stage_lds + wib * kStageBytes);
+    main_unit<MIXED, DERIV, MP, DIAG, OTF, NOPS, DPATS>(a, chunk_id, logical - chunk_id * a.grid_x, wib,
+                                                 threadIdx.x & (kWave - 1), stage_lds + wib * kStageBytes);
 }
 
 // ------------------------------------------------------------------ per-reflection derivatives
@@ -2420,10 +2842,82 @@ void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const doub
     uploaded_ = true;
 }
 
+int Engine::diag64_code() const
+{
+    if (!diag_ || (ops_.size() != 2 && ops_.size() != 4)) return 0;
+    int code = 4096;
+    for (size_t q = 0; q < ops_.size(); q++) code |= diag_pat_[q] << (3 * q);
+    for (int c : {kDiag64Pats0, kDiag64Pats1, kDiag64Pats2, kDiag64Pats3})
+        if (ops_.size() == 2 && code == c) return c;
+    if (ops_.size() == 4 && code == kDiag64Pats4) return code;
+    return 0;
+}
+
+// Reflections along hkl lines for the FP64 line kernels: the axis with the fewest lines (as the
+// reference chooses, scattering_utilities.cpp:371-435), lines in lexicographic order of the other
+// two indices, each by increasing index along the axis.  Fills line_cont (reflection continues the
+// previous one's line) and the per-operation step constants, and reorders key (caller index in
+// .second) into that order.
+void Engine::line_sort(int n_hkl, const int* hkl, std::vector<std::pair<double, int>>& key)
+{
+    int best = 2;
+    size_t best_lines = (size_t)-1;
+    for (int ax = 0; ax < 3; ax++) {
+        const int o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
+        std::vector<long long> ids(n_hkl);
+        for (int k = 0; k < n_hkl; k++)
+            ids[k] = ((long long)hkl[3 * k + o1] << 32) ^ (long long)(unsigned)hkl[3 * k + o2];
+        std::sort(ids.begin(), ids.end());
+        const size_t lines = std::unique(ids.begin(), ids.end()) - ids.begin();
+        if (lines < best_lines) {
+            best_lines = lines;
+            best = ax;
+        }
+    }
+    line_axis_ = best;
+    const int ax = best, o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
+    std::stable_sort(key.begin(), key.end(), [&](const std::pair<double, int>& x, const std::pair<double, int>& y) {
+        const int* a = hkl + 3 * x.second;
+        const int* b = hkl + 3 * y.second;
+        if (a[o1] != b[o1]) return a[o1] < b[o1];
+        if (a[o2] != b[o2]) return a[o2] < b[o2];
+        return a[ax] < b[ax];
+    });
+    line_cont_host_.assign(std::max(n_hkl, 1), 0);
+    for (int i = 1; i < n_hkl; i++) {
+        const int* a = hkl + 3 * key[i - 1].second;
+        const int* b = hkl + 3 * key[i].second;
+        line_cont_host_[i] = a[o1] == b[o1] && a[o2] == b[o2] && b[ax] == a[ax] + 1;
+    }
+    d_line_cont_.ensure(line_cont_host_.size() * sizeof(int));
+    HIP_CHECK(hipMemcpyAsync(d_line_cont_.p, line_cont_host_.data(), line_cont_host_.size() * sizeof(int),
+                             hipMemcpyHostToDevice, stream_));
+    // per operation: Delta R_f (the phase step in revolutions is Delta R_f . x + Delta . t), Delta . t,
+    // and m(Delta_s) with Delta_s = Delta in Cartesian reciprocal coordinates times R_s
+    line_ops_host_.assign(ops_.size() * 10, 0.0);
+    double dc[3];
+    for (int j = 0; j < 3; j++) dc[j] = c2f_[3 * ax + j];
+    for (size_t s = 0; s < ops_.size(); s++) {
+        const EngineOp& o = ops_[s];
+        double* q = &line_ops_host_[10 * s];
+        for (int j = 0; j < 3; j++) q[j] = o.rf[3 * ax + j];
+        q[3] = o.tf[ax];
+        double ds[3];
+        for (int j = 0; j < 3; j++) ds[j] = dc[0] * o.rc[j] + dc[1] * o.rc[3 + j] + dc[2] * o.rc[6 + j];
+        const double m[6] = {ds[0] * ds[0], ds[1] * ds[1], ds[2] * ds[2], 2 * ds[0] * ds[1], 2 * ds[0] * ds[2],
+                             2 * ds[1] * ds[2]};
+        for (int j = 0; j < 6; j++) q[4 + j] = m[j];
+    }
+    d_line_ops_.ensure(std::max<size_t>(line_ops_host_.size(), 1) * sizeof(double));
+    HIP_CHECK(hipMemcpyAsync(d_line_ops_.p, line_ops_host_.data(), line_ops_host_.size() * sizeof(double),
+                             hipMemcpyHostToDevice, stream_));
+}
+
 void Engine::upload_checked(const EngineAtoms& at, int n_hkl, const int* hkl, const double* dtdf, bool rows_layout)
 {
     dev_ready_ = false;
-    const bool sort_hkl = !rows_layout;
+    // reflection order: caller's (rows), along hkl lines (FP64 line kernels) or by |h|
+    const int order_mode = rows_layout ? 0 : (line_order && line_ops_ok() ? 2 : 1);
     n_atoms_ = at.n;
     n_hkl_ = n_hkl;
     rows_layout_ = rows_layout;
@@ -2502,7 +2996,7 @@ void Engine::upload_checked(const EngineAtoms& at, int n_hkl, const int* hkl, co
     // position i): consecutive reflections of a wave then read neighbouring nodes of the radial
     // tables.  |h| range for the form-factor tables.  A refinement passes the same list every
     // call: then the order, the tables' range and the device copy of the indices are kept.
-    const bool same_hkl = sort_hkl == last_sorted_ && (size_t)3 * n_hkl == last_hkl_.size() &&
+    const bool same_hkl = order_mode == last_order_ && (size_t)3 * n_hkl == last_hkl_.size() &&
                           (n_hkl == 0 || std::memcmp(last_hkl_.data(), hkl, (size_t)3 * n_hkl * sizeof(int)) == 0);
     d_hkl_.ensure(std::max(n_hkl, 1) * 3 * sizeof(int));
     d_dtdf_.ensure(std::max(n_hkl, 1) * 3 * sizeof(double));   // dtdf (2) + mult (1)
@@ -2520,14 +3014,15 @@ void Engine::upload_checked(const EngineAtoms& at, int n_hkl, const int* hkl, co
         }
         h_max_ = 0.0;
         for (int k = 0; k < n_hkl; k++) h_max_ = std::max(h_max_, std::sqrt(key[k].first));
-        if (sort_hkl) std::sort(key.begin(), key.end());     // (|h|^2, index): the stable order
+        if (order_mode == 1) std::sort(key.begin(), key.end());     // (|h|^2, index): the stable order
+        if (order_mode == 2) line_sort(n_hkl, hkl, key);
         hkl_perm_.resize(n_hkl);
         for (int k = 0; k < n_hkl; k++) hkl_perm_[k] = key[k].second;
         plan_tables();
         for (int i = 0; i < n_hkl; i++)
             for (int j = 0; j < 3; j++) hkl_s[3 * i + j] = hkl[3 * hkl_perm_[i] + j];
         last_hkl_.assign(hkl, hkl + 3 * (size_t)n_hkl);
-        last_sorted_ = sort_hkl;
+        last_order_ = order_mode;
         if (n_hkl > 0)
             HIP_CHECK(hipMemcpyAsync(d_hkl_.p, hkl_s, (size_t)n_hkl * 3 * sizeof(int), hipMemcpyHostToDevice,
                                      stream_));
@@ -2570,7 +3065,8 @@ void Engine::ensure_prep_buffers(Precision prec, bool mp)
     const bool mixed = prec == kMixed;
     const size_t n_rec = (size_t)n_hkl * ops_.size();
     if (!use_otf(mp)) d_radial_.ensure((size_t)n_hkl * n_types * kRadRec * (mixed ? sizeof(float) : sizeof(double)));
-    d_hklop_.ensure(n_rec * kHklopRec * (mixed ? sizeof(float) : sizeof(double)));
+    d_hklop_.ensure(std::max(n_rec * kHklopRec, (size_t)std::max(n_hkl, 1) * kDiag64Rec) *
+                    (mixed ? sizeof(float) : sizeof(double)));
     if (mixed) d_phase_.ensure(n_rec * 4 * sizeof(double));
     d_radial32_.ensure((size_t)std::max(n_hkl, 1) * sizeof(double));
     if (use_diag(mixed, mp)) d_ht_.ensure(std::max<size_t>(n_rec, 1) * sizeof(double));
@@ -2584,8 +3080,9 @@ void Engine::run_prep(Precision prec, bool mp, bool electron, bool general)
     // the reflection records, tables and radial records depend on the reflections, the types and
     // the path only: kept from the previous call with the same key until one of them changes
     const bool mixed_k = prec == kMixed;
+    const bool diag64 = use_diag64(mixed_k, mp) && !general;
     const int key = (mixed_k ? 1 : 0) | (mp ? 2 : 0) | (electron && electron_ ? 4 : 0) |
-                    (use_diag(mixed_k, mp) && !general ? 8 : 0) | (use_otf(mp) ? 16 : 0);
+                    (use_diag(mixed_k, mp) && !general ? 8 : 0) | (use_otf(mp) ? 16 : 0) | (diag64 ? 32 : 0);
     const bool reuse = prep_valid_ && prep_key_ == key;
     prep_valid_ = true;
     prep_key_ = key;
@@ -2607,6 +3104,9 @@ void Engine::run_prep(Precision prec, bool mp, bool electron, bool general)
             hipLaunchKernelGGL(k_hkl_prep_diag, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, d_hkl_.as<int>(), ops,
                                n_ops, n_hkl, c2f, d_hklop_.as<float>(), d_phase_.as<double>(), d_ht_.as<double>(),
                                hlen.as<double>());
+        } else if (diag64) {
+            hipLaunchKernelGGL(k_hkl_prep_diag64, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, d_hkl_.as<int>(),
+                               ops, n_ops, n_hkl, c2f, d_hklop_.as<double>(), hlen.as<double>());
         } else {
             hipLaunchKernelGGL(k_hkl_prep, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, d_hkl_.as<int>(), ops, n_ops,
                                n_hkl, c2f, mixed ? nullptr : d_hklop_.as<double>(), mixed ? d_hklop_.as<float>() : nullptr,
@@ -2642,10 +3142,38 @@ void Engine::run_prep(Precision prec, bool mp, bool electron, bool general)
     HIP_CHECK(hipGetLastError());
 }
 
-// mp_pass: 1 or 2 selects the pass of the multipole adjoint (0 without it)
-static const void* main_kernel(bool mixed, bool derivs, int mp_pass, bool diag, bool otf)
+// mp_pass: 1 or 2 selects the pass of the multipole adjoint (0 without it); line_ops > 0: the FP64
+// line kernel for that many operations
+static const void* main_kernel(bool mixed, bool derivs, int mp_pass, bool diag, bool otf, int line_ops = 0,
+                               int diag64 = 0)
 {
 #define KM(M, D, P, G, O) (const void*)k_main<M, D, P, G, O>
+#define KL(D, O, N) (const void*)k_main<false, D, 0, false, O, N>
+#define KD(N, C) (derivs ? (otf ? (const void*)k_main<false, true, 0, false, true, N, C>                              \
+                             : (const void*)k_main<false, true, 0, false, false, N, C>)                             \
+                      : (otf ? (const void*)k_main<false, false, 0, false, true, N, C>                              \
+                             : (const void*)k_main<false, false, 0, false, false, N, C>))
+    if (diag64 > 0 && !mixed && mp_pass == 0) {
+        switch (diag64) {
+        case kDiag64Pats0: return KD(2, kDiag64Pats0);
+        case kDiag64Pats1: return KD(2, kDiag64Pats1);
+        case kDiag64Pats2: return KD(2, kDiag64Pats2);
+        case kDiag64Pats3: return KD(2, kDiag64Pats3);
+        case kDiag64Pats4: return KD(4, kDiag64Pats4);
+        default: break;
+        }
+    }
+#undef KD
+    if (line_ops > 0 && !mixed && mp_pass == 0) {
+        switch (line_ops) {
+        case 1: return derivs ? (otf ? KL(true, true, 1) : KL(true, false, 1)) : (otf ? KL(false, true, 1) : KL(false, false, 1));
+        case 2: return derivs ? (otf ? KL(true, true, 2) : KL(true, false, 2)) : (otf ? KL(false, true, 2) : KL(false, false, 2));
+        case 3: return derivs ? (otf ? KL(true, true, 3) : KL(true, false, 3)) : (otf ? KL(false, true, 3) : KL(false, false, 3));
+        case 4: return derivs ? (otf ? KL(true, true, 4) : KL(true, false, 4)) : (otf ? KL(false, true, 4) : KL(false, false, 4));
+        default: break;
+        }
+    }
+#undef KL
     if (mp_pass == 1) return mixed ? (diag ? KM(true, true, 1, true, false) : KM(true, true, 1, false, false))
                                    : KM(false, true, 1, false, false);
     if (mp_pass == 2) return mixed ? (diag ? KM(true, true, 2, true, false) : KM(true, true, 2, false, false))
@@ -2667,13 +3195,15 @@ static const void* main_kernel(bool mixed, bool derivs, int mp_pass, bool diag, 
 
 int Engine::resident_waves(bool mixed, bool derivs, bool mp)
 {
-    const int key = (use_otf(mp) ? 16 : 0) + (use_diag(mixed, mp) ? 8 : 0) + (mixed ? 4 : 0) + (derivs ? 2 : 0) +
-                    (mp ? 1 : 0);
+    const int line = use_line(mixed, mp) ? (use_diag64(mixed, mp) ? 5 : (int)ops_.size()) : 0;
+    const int key = line * 32 + (use_otf(mp) ? 16 : 0) + (use_diag(mixed, mp) ? 8 : 0) + (mixed ? 4 : 0) +
+                    (derivs ? 2 : 0) + (mp ? 1 : 0);
     if (resident_waves_[key] > 0) return resident_waves_[key];
     int blocks = 0;
-    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, main_kernel(mixed, derivs, mp ? 1 : 0, use_diag(mixed, mp),
-                                                                                use_otf(mp)),
-                                                               256, 0));
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &blocks, main_kernel(mixed, derivs, mp ? 1 : 0, use_diag(mixed, mp), use_otf(mp), (int)ops_.size() * (line > 0),
+                             use_diag64(mixed, mp)),
+        256, 0));
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, device_));
     resident_waves_[key] = std::max(1, blocks) * kWavesPerBlock * prop.multiProcessorCount;
@@ -2721,11 +3251,13 @@ void Engine::main_args(Precision prec, bool derivs, bool mp, int chunk, MainArgs
     a.grid_x = (n_waves + kWavesPerBlock - 1) / kWavesPerBlock;
     a.n_blocks = a.grid_x * n_chunks;
     a.per_xcd = (a.n_blocks + kXcds - 1) / kXcds;
-    const bool diag = use_diag(mixed, mp);
-    a.ht = diag ? d_ht_.as<double>() : nullptr;
+    const bool diag = use_diag(mixed, mp) || use_diag64(mixed, mp);
+    a.ht = use_diag(mixed, mp) ? d_ht_.as<double>() : nullptr;
     a.tab = mixed ? d_tab32_.p : d_tab_.p;
     a.tabs = d_tabrec_.as<TabRec>();
     a.hlen = d_radial32_.as<double>();
+    a.line_cont = d_line_cont_.as<int>();
+    a.line_ops = d_line_ops_.as<double>();
     a.diag_mask = 0;
     for (int p = 0; p < 8; p++) {
         a.diag_op[p] = 0;
@@ -2749,8 +3281,11 @@ void Engine::launch_main(Precision prec, bool derivs, bool mp, int chunk)
     last_chunk = chunk;
     void* args[] = {&a};
     const int passes[2] = {mp ? (mp_scalars ? 1 : 3) : 0, 2};
+    const int line = use_line(prec == kMixed, mp) ? (int)ops_.size() : 0;
+    const int d64 = use_diag64(prec == kMixed, mp);
     for (int p = 0; p < (mp ? 2 : 1); p++) {
-        const void* kern = main_kernel(prec == kMixed, derivs, passes[p], use_diag(prec == kMixed, mp), use_otf(mp));
+        const void* kern =
+            main_kernel(prec == kMixed, derivs, passes[p], use_diag(prec == kMixed, mp), use_otf(mp), line, d64);
         HIP_CHECK(hipLaunchKernel(kern, grid, dim3(256), args, 0, stream_));
     }
 }
@@ -2766,7 +3301,8 @@ int Engine::chunk_for(Precision prec, bool derivs, bool mp)
 
 long long Engine::shape_key(Precision prec, bool derivs, bool mp) const
 {
-    return (((long long)n_sorted_ * 1000003LL + n_hkl_) * 64 + (long long)ops_.size()) * 32 +
+    return ((((long long)n_sorted_ * 1000003LL + n_hkl_) * 64 + (long long)ops_.size()) * 3 +
+            (use_line(prec == kMixed, mp) ? 1 : 0) + (use_diag64(prec == kMixed, mp) ? 1 : 0)) * 32 +
            (use_otf(mp) ? 16 : 0) + (use_diag(prec == kMixed, mp) ? 8 : 0) + (prec == kMixed ? 4 : 0) +
            (derivs ? 2 : 0) + (mp ? 1 : 0);
 }

@@ -84,6 +84,7 @@ struct TuneParams {
     int hkl_chunk = 0;       // reflections per workgroup column; 0 = automatic
     bool general_path = false;  // force the general-rotation kernels for diagonal groups (tests)
     int radial_mode = 0;        // 0 automatic, 1 per-(reflection, type) records, 2 on-the-fly tables
+    bool direct_phases = false; // FP64: evaluate every phase and Debye-Waller factor directly (tests)
 };
 
 // Atom data for one call, in the order the caller supplies them (already converted to the
@@ -143,6 +144,9 @@ public:
     // With the multipole adjoint: also dT/dL (9 per atom, local axes as columns, atom weight not
     // applied) for the chain rule through recomputed local frames; read with download_lcs_grad.
     bool lcs_grad = false;
+    // Order reflections along hkl lines at the next upload (set for the FP64 drop-in, whose line
+    // kernels carry phase and Debye-Waller factors along them); otherwise by increasing |h|.
+    bool line_order = false;
     // With the multipole adjoint: whether the Pval / kappa / kappa' sums (and the P_lm derivatives)
     // are wanted as well; without them only the B_lm pass is added to the plain one.
     bool mp_scalars = true;
@@ -230,7 +234,7 @@ private:
         if (tune.radial_mode != 0) return tune.radial_mode == 2;
         return (long long)model_.n_types * 8 > (long long)n_sorted_;
     }
-    int resident_waves_[32] = {};
+    int resident_waves_[32 * 6] = {};   // key: line kind (0-5) * 32 + path bits
     bool diag_ = false;               // every operation is a diagonal sign matrix
     int diag_pat_[8] = {};
     float diag_anom_[8] = {};
@@ -254,7 +258,26 @@ private:
     std::vector<int> order_;        // sorted slot -> caller atom index (-1 padding)
     std::vector<int> hkl_perm_;     // device reflection position -> caller reflection index
     std::vector<int> last_hkl_;     // reflections of the last upload (unchanged lists skip the sort)
-    bool last_sorted_ = false;
+    int last_order_ = -1;           // reflection order of the last upload: 0 caller, 1 by |h|, 2 lines
+    // FP64 line recurrences: the axis of the lines, per reflection whether it continues the line
+    // of the one before, per operation [Delta R_f (3), Delta . t, m(Delta_s) (6)]
+    int line_axis_ = 2;
+    DeviceBuffer d_line_cont_, d_line_ops_;
+    std::vector<int> line_cont_host_;
+    std::vector<double> line_ops_host_;
+    bool line_ops_ok() const { return !ops_.empty() && ops_.size() <= 4; }
+    void line_sort(int n_hkl, const int* hkl, std::vector<std::pair<double, int>>& key);
+    bool use_line(bool mixed, bool mp) const
+    {
+        return !mixed && !mp && !tune.direct_phases && last_order_ == 2 && line_ops_ok();
+    }
+    // the FP64 line kernel with per-reflection records for diagonal groups: its sign-pattern code, 0
+    // when the operations' patterns are not one of the compiled sets
+    int diag64_code() const;
+    int use_diag64(bool mixed, bool mp) const
+    {
+        return use_line(mixed, mp) && diag_ && !tune.general_path ? diag64_code() : 0;
+    }
     bool prep_valid_ = false;       // the prep buffers hold the records of prep_key_ (run_prep)
     int prep_key_ = -1;
     std::vector<int> tab_nodes_;    // radial-table node count per wavefunction

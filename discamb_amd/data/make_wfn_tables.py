@@ -1,6 +1,7 @@
 #!/usr/bin/env python3
 """Generate ``wfn_tables.json``: Clementi-Roetti atomic Slater-orbital wavefunctions and
-default deformation-valence parameters for the neutral atoms H..Kr.
+default deformation-valence parameters for the neutral atoms H..Kr and the ions of the table
+(Li+ .. Zn2+, O-, F-, S-, Cl-, Se-, Br-).
 
 The numbers are published tables (Clementi & Roetti 1974, At. Data Nucl. Data Tables 14, 177;
 Clementi & Raimondi 1963, J. Chem. Phys. 38, 2686).  DiSCaMB ships them as C arrays in
@@ -8,8 +9,8 @@ Clementi & Raimondi 1963, J. Chem. Phys. 38, 2686).  DiSCaMB ships them as C arr
 script reads those arrays as text (run it only where the reference checkout exists) and
 applies the same post-processing the reference applies when it builds its wavefunction bank:
 
-* orbital configuration parsing and core/valence split
-  (``SlaterOrbitalWfnData.cpp:13-160``),
+* orbital configuration parsing and core/valence split, including its rule that cations left
+  with 28 or 46 electrons have no valence orbitals (``SlaterOrbitalWfnData.cpp:13-160``),
 * exponents Bohr^-1 -> Angstrom^-1 with 1/0.52917721092, Slater normalisation
   ``N = (2a)^(k+1) sqrt(2a/(2k+2)!)`` and orbital renormalisation
   (``SlaterOrbitalWfnData.cpp:162-256``, ``HC_WfnData.h:33-37``),
@@ -115,6 +116,8 @@ def valence_orbitals(z, charge, occ):
     n_el = z - charge
     if n_el in (0, 2, 10, 18, 36, 54, 86):
         return []
+    if charge > 0 and n_el in (28, 46):
+        return []
     blk, per = block_of(z), period_of(z)
     val = []
     if blk in "SP":
@@ -154,7 +157,7 @@ def main():
     raimondi_scale = 1.8897261245
     table = {}
     for label, z, charge, orbs in wfns:
-        if charge != 0 or z > 36:
+        if z > 36:
             continue
         occ = confs[label]
         val = valence_orbitals(z, charge, occ)
@@ -193,9 +196,10 @@ def main():
             zeta = 2 * sum(occ[v] * shells[v] for v in val) / tot
         powers = def_val_powers(z)
         ubdb = list(powers)
-        if label == "P":
+        symbol = "".join(ch for ch in label if ch.isalpha())     # UBDB rule keyed on the element
+        if symbol == "P":
             ubdb = [4, 6, 6, 6, 6]
-        if label == "S":
+        if symbol == "S":
             ubdb = [1, 2, 4, 6, 8]
         entry["def_val_exponent"] = zeta
         entry["def_val_powers_standard"] = powers

@@ -335,3 +335,21 @@ def test_fp64_line_recurrences_vs_direct_and_oracle(sg, radial):
         rf, rx, ru, ro = hc_oracle.sf_and_derivs(w.crystal, w.params, w.crystal.atoms, w.lcs, hkl, g, n_threads=16)
         assert rel_err(f, rf) <= 1e-10 and rel_err(d.atomic_position_derivatives, rx) <= 1e-10
         assert rel_err(d.adp_derivatives, ru) <= 1e-10 and rel_err(d.occupancy_derivatives, ro) <= 1e-10
+
+
+@pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
+def test_ionic_wavefunctions(mode):
+    """Ionic Clementi-Roetti entries of the built-in bank (Fe3+, Cu+ without valence orbitals, O-,
+    Se-) in place of the neutral atoms' wavefunctions, against the oracle; also with electron
+    scattering (atomic numbers of the ions' elements)."""
+    from discamb_amd import model
+    w = synth.small_structure("P21/c", n_atoms=12, d_min=0.9, seed=41)
+    for i, label in enumerate(["Fe3+", "Cu+", "O-", "Se-"]):
+        w.params.wfn_parameters[i] = model.wfn_type_from_bank(label)
+    _compare(w, mode)
+    calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+    calc.setCalculationMode(mode)
+    calc.setElectronScattering(True)
+    f, d = calc.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, w.hkl, w.dtdf)
+    rf = hc_oracle.electron_sf_and_derivs(w.crystal, w.params, w.crystal.atoms, w.lcs, w.hkl, w.dtdf, n_threads=16)[0]
+    assert rel_err(f, rf) <= TOL[mode][0]

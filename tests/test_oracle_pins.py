@@ -147,7 +147,7 @@ def test_def_val_parameters_vs_reference(ref_lib, sym):
 
 def test_electron_count_of_densities():
     """f(0) of a Slater density = its electron count: core -> core electrons, valence -> 1."""
-    for sym in ("H", "C", "N", "O", "S", "Fe", "Br"):
+    for sym in ("H", "C", "N", "O", "S", "Fe", "Br", "Fe3+", "Cu+", "Zn2+", "O-", "Se-", "Al3+", "Ti3+"):
         w = model.wfn_type_from_bank(sym)
         for valence in (False, True):
             c, e, p = hc_oracle.sphere_density(w, valence)

@@ -354,6 +354,274 @@ int hc_oracle_engine(
     return 0;
 }
 
+/* Timing restatement of Engine3's acentric path with its hkl-line recurrences, the CPU baseline of
+ * the benchmark (its results are checked against hc_oracle_engine by the tests).  It follows the
+ * reference's structure rather than the oracle's direct evaluation:
+ *   - P_lm rotated once per atom to the global frame (calculateGlobalCoordinatesPlm,
+ *     HansenCoppens_SF_Engine3.cpp:594-654; here by an exact projection quadrature: 5 Gauss-Legendre
+ *     nodes in cos(theta) x 10 in phi integrate degree-8 products of harmonics exactly);
+ *   - per reflection and operation the harmonics, Debye-Waller multipliers and per type the radial
+ *     terms once (pre_atom_loop_sf_calc, .cpp:319-418);
+ *   - reflections along lines with the phase factor and Debye-Waller factor carried by products
+ *     (init_line_multipliers / calculate_line_phase_factors / calculate_line_temperature_factors,
+ *     scattering_utilities.cpp:520-756, used at Engine3.cpp:1258-1288), direct evaluation at each
+ *     line start; OpenMP over lines.
+ * Reflections arrive in line order: line i is [line_off[i], line_off[i+1]) and consecutive
+ * reflections of a line differ by the Cartesian step dh.  Acentric groups only (no inversion
+ * folding), no electron or core override. */
+/* g_fun with the power of the denominator by repeated multiplication (n <= 12) */
+static double g_fun_ipow(int l, int n, double h, double z)
+{
+    const double k = 2.0 * M_PI * h;
+    const int off = kHcGOffset[l][n];
+    const double d = k * k + z * z;
+    double num = 0.0, kp = 1.0, zp[16], dn = 1.0;
+    zp[0] = 1.0;
+    for (int i = 1; i < n; i++) zp[i] = zp[i - 1] * z;
+    for (int i = 0; i < n; i++) {
+        num += kHcGCoef[off + i] * kp * zp[n - 1 - i];
+        kp *= k;
+        dn *= d;
+    }
+    return num / dn;
+}
+
+static double sph_density_sf_ipow(const double *c, const double *e, const int *p, int n, double h)
+{
+    double s = 0.0;
+    for (int k = 0; k < n; k++) s += c[k] * g_fun_ipow(0, p[k] + 2, h, e[k]);
+    return s * 4.0 * M_PI;
+}
+
+static void legendre5(double *x, double *w)
+{
+    const double a = 0.5384693101056831, b = 0.9061798459386640;
+    const double wa = 0.4786286704993665, wb = 0.2369268850561891, w0 = 0.5688888888888889;
+    x[0] = -b; x[1] = -a; x[2] = 0.0; x[3] = a; x[4] = b;
+    w[0] = wb; w[1] = wa; w[2] = w0; w[3] = wa; w[4] = wb;
+}
+
+/* P_glob = D P_loc, D_ij = sum_q w_q d_i(u_q) d_j(L^T u_q) / sum_q w_q d_i(u_q)^2 (same l only) */
+static void global_plm(const double *plm, int max_l, const double *L, double *pg)
+{
+    double x[5], wx[5];
+    legendre5(x, wx);
+    double num[25] = {0}, den[25] = {0};
+    for (int it = 0; it < 5; it++)
+        for (int ip = 0; ip < 10; ip++) {
+            const double ct = x[it], st = sqrt(1.0 - ct * ct), ph = 2.0 * M_PI * ip / 10.0;
+            const double u[3] = {st * cos(ph), st * sin(ph), ct};
+            double ul[3], dg[25], dl[25];
+            for (int j = 0; j < 3; j++) ul[j] = L[j] * u[0] + L[3 + j] * u[1] + L[6 + j] * u[2];
+            ylm_density(u, dg);
+            ylm_density(ul, dl);
+            double rho[5] = {0, 0, 0, 0, 0};
+            for (int l = 0; l <= max_l; l++)
+                for (int i = l * l; i < (l + 1) * (l + 1); i++) rho[l] += plm[i] * dl[i];
+            for (int l = 0; l <= 4; l++)
+                for (int i = l * l; i < (l + 1) * (l + 1); i++) {
+                    num[i] += wx[it] * dg[i] * rho[l];
+                    den[i] += wx[it] * dg[i] * dg[i];
+                }
+        }
+    for (int i = 0; i < 25; i++) pg[i] = num[i] / den[i];
+}
+
+int hc_oracle_engine_lines(
+    const oracle_model *mdl,
+    int n_atoms, const int *atom_wfn, const int *atom_type, const double *xyz, const int *adp_n,
+    const double *adp, const double *occ, const double *mult_w, const double *lcs,
+    const double *anom_atom, const unsigned char *include,
+    int n_ops, const double *rot, const double *trans, int centro, const double *inv_t,
+    int n_hkl, const double *hv, int n_lines, const int *line_off, const double *dh, const double *dtdf,
+    double *f_out, double *d_xyz, double *d_adp, double *d_occ, int n_threads)
+{
+    (void)inv_t;
+    if (centro || g_core_f || g_el_z) return 1;
+    const double two_pi = 2.0 * M_PI, two_pi2 = 2.0 * M_PI * M_PI, four_pi = 4.0 * M_PI;
+    int *type_wfn = (int *)calloc(mdl->n_types > 0 ? mdl->n_types : 1, sizeof(int));
+    int *type_max_l = (int *)calloc(mdl->n_types > 0 ? mdl->n_types : 1, sizeof(int));
+    for (int a = 0; a < n_atoms; a++) type_wfn[atom_type[a]] = atom_wfn[a];
+    for (int t = 0; t < mdl->n_types; t++) {
+        int lp = -1;
+        for (int l = 0; l < mdl->plm_nl[t]; l++)
+            for (int i = l * l; i < (l + 1) * (l + 1); i++)
+                if (mdl->plm[25 * t + i] != 0.0) lp = l;
+        int lw = mdl->dv_nl[type_wfn[t]] - 1;
+        int ml = lw < 4 ? lw : 4;
+        type_max_l[t] = ml < lp ? ml : lp;
+    }
+    /* per atom: global P_lm, 6-component ADP (U_iso as U I), line constants per operation */
+    double *pg = (double *)calloc((size_t)n_atoms * 25 + 1, sizeof(double));
+    double *u6 = (double *)calloc((size_t)n_atoms * 6 + 1, sizeof(double));
+    int off = 0;
+    for (int a = 0; a < n_atoms; a++) {
+        const int t = atom_type[a];
+        if (type_max_l[t] >= 0) global_plm(mdl->plm + 25 * t, type_max_l[t], lcs + 9 * a, pg + 25 * a);
+        if (adp_n[a] == 6)
+            for (int j = 0; j < 6; j++) u6[6 * a + j] = adp[off + j];
+        else if (adp_n[a] == 1)
+            for (int j = 0; j < 3; j++) u6[6 * a + j] = adp[off];
+        off += adp_n[a];
+    }
+    /* Delta in each operation's frame and m(Delta_s) */
+    double *dhs = (double *)malloc(sizeof(double) * 3 * (n_ops + 1)), *dm = (double *)malloc(sizeof(double) * 6 * (n_ops + 1));
+    for (int s = 0; s < n_ops; s++) {
+        const double *R = rot + 9 * s;
+        double *d = dhs + 3 * s;
+        for (int j = 0; j < 3; j++) d[j] = dh[0] * R[j] + dh[1] * R[3 + j] + dh[2] * R[6 + j];
+        double *m = dm + 6 * s;
+        m[0] = d[0] * d[0]; m[1] = d[1] * d[1]; m[2] = d[2] * d[2];
+        m[3] = 2 * d[0] * d[1]; m[4] = 2 * d[0] * d[2]; m[5] = 2 * d[1] * d[2];
+    }
+    int max_len = 1;
+    for (int i = 0; i < n_lines; i++)
+        if (line_off[i + 1] - line_off[i] > max_len) max_len = line_off[i + 1] - line_off[i];
+    if (n_threads < 1) n_threads = 1;
+#ifndef _OPENMP
+    n_threads = 1;
+#endif
+    double *acc = (double *)calloc((size_t)n_threads * n_atoms * 10 + 1, sizeof(double));
+#pragma omp parallel num_threads(n_threads)
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double *my = acc + (size_t)tid * n_atoms * 10;
+        /* per-line tables: [k][s] harmonics (25), m (6), h_s (3); [k][w] core; [k][t] valence, g_l */
+        double *yt = (double *)malloc(sizeof(double) * (size_t)max_len * n_ops * 34);
+        double *fc = (double *)malloc(sizeof(double) * (size_t)max_len * (mdl->n_wfn + 1));
+        double *fv = (double *)malloc(sizeof(double) * (size_t)max_len * (mdl->n_types + 1));
+        double *gt = (double *)malloc(sizeof(double) * (size_t)max_len * 5 * (mdl->n_types + 1));
+        double *fl = (double *)malloc(sizeof(double) * (size_t)max_len * 2);
+#pragma omp for schedule(dynamic, 4)
+        for (int li = 0; li < n_lines; li++) {
+            const int k0 = line_off[li], len = line_off[li + 1] - k0;
+            for (int q = 0; q < len; q++) {
+                const double *h = hv + 3 * (k0 + q);
+                const double hl = sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+                const int h000 = hl < 1e-10;
+                for (int s = 0; s < n_ops; s++) {
+                    const double *R = rot + 9 * s;
+                    double *r = yt + ((size_t)q * n_ops + s) * 34;
+                    double hs[3];
+                    for (int j = 0; j < 3; j++) hs[j] = h[0] * R[j] + h[1] * R[3 + j] + h[2] * R[6 + j];
+                    if (h000) {
+                        for (int i = 0; i < 25; i++) r[i] = 0.0;
+                    } else {
+                        const double u[3] = {hs[0] / hl, hs[1] / hl, hs[2] / hl};
+                        ylm_density(u, r);
+                    }
+                    r[25] = hs[0] * hs[0]; r[26] = hs[1] * hs[1]; r[27] = hs[2] * hs[2];
+                    r[28] = 2 * hs[0] * hs[1]; r[29] = 2 * hs[0] * hs[2]; r[30] = 2 * hs[1] * hs[2];
+                    r[31] = hs[0]; r[32] = hs[1]; r[33] = hs[2];
+                }
+                for (int w = 0; w < mdl->n_wfn; w++)
+                    fc[(size_t)q * mdl->n_wfn + w] =
+                        sph_density_sf_ipow(mdl->core_c + mdl->core_off[w], mdl->core_e + mdl->core_off[w],
+                                       mdl->core_p + mdl->core_off[w], mdl->core_off[w + 1] - mdl->core_off[w], hl);
+                for (int t = 0; t < mdl->n_types; t++) {
+                    const int w = type_wfn[t];
+                    fv[(size_t)q * mdl->n_types + t] =
+                        sph_density_sf_ipow(mdl->val_c + mdl->val_off[w], mdl->val_e + mdl->val_off[w],
+                                       mdl->val_p + mdl->val_off[w], mdl->val_off[w + 1] - mdl->val_off[w],
+                                       hl / mdl->kappa[t]);
+                    for (int l = 0; l < 5; l++) {
+                        double g = 0.0;
+                        if (l < mdl->dv_nl[w] && !h000) {
+                            const int nn = mdl->dv_n[5 * w + l];
+                            const double zeta = mdl->dv_zeta[w];
+                            double norm = pow(zeta, nn + 3);
+                            for (int qq = 2; qq <= nn + 2; qq++) norm /= qq;
+                            g = norm * g_fun_ipow(l, nn + 2, hl / mdl->kappa_p[t], zeta);
+                        }
+                        gt[((size_t)q * mdl->n_types + t) * 5 + l] = four_pi * g;
+                    }
+                }
+                fl[2 * q] = fl[2 * q + 1] = 0.0;
+            }
+            for (int a = 0; a < n_atoms; a++) {
+                if (!include[a]) continue;
+                const int w = atom_wfn[a], t = atom_type[a], ml = type_max_l[t];
+                const double weight = occ[a] * mult_w[a];
+                const double *x = xyz + 3 * a, *U = u6 + 6 * a, *P = pg + 25 * a;
+                const double anr = anom_atom ? anom_atom[2 * a] : mdl->anom[2 * w];
+                const double ani = anom_atom ? anom_atom[2 * a + 1] : mdl->anom[2 * w + 1];
+                double *da = my + 10 * (size_t)a;
+                for (int s = 0; s < n_ops; s++) {
+                    const double *tt = trans + 3 * s, *d = dhs + 3 * s, *mD = dm + 6 * s;
+                    const double *h0 = hv + 3 * k0;
+                    const double *r0 = yt + (size_t)s * 34;
+                    /* line start: direct phase factor, Debye-Waller factor and its first ratio */
+                    const double ph0 = two_pi * (r0[31] * x[0] + r0[32] * x[1] + r0[33] * x[2] + h0[0] * tt[0] +
+                                                 h0[1] * tt[1] + h0[2] * tt[2]);
+                    const double e0 = r0[25] * U[0] + r0[26] * U[1] + r0[27] * U[2] + r0[28] * U[3] + r0[29] * U[4] +
+                                      r0[30] * U[5];
+                    const double T0 = exp(-e0);
+                    double er = T0 * cos(ph0), ei = T0 * sin(ph0), q = 1.0;
+                    if (len > 1) {
+                        const double *r1 = yt + ((size_t)n_ops + s) * 34;
+                        const double e1 = r1[25] * U[0] + r1[26] * U[1] + r1[27] * U[2] + r1[28] * U[3] +
+                                          r1[29] * U[4] + r1[30] * U[5];
+                        q = exp(-(e1 - e0));
+                    }
+                    const double phm = two_pi * (d[0] * x[0] + d[1] * x[1] + d[2] * x[2] + dh[0] * tt[0] + dh[1] * tt[1] +
+                                                 dh[2] * tt[2]);
+                    const double mr = cos(phm), mi = sin(phm);
+                    const double c = exp(-2.0 * (mD[0] * U[0] + mD[1] * U[1] + mD[2] * U[2] + mD[3] * U[3] +
+                                                 mD[4] * U[4] + mD[5] * U[5]));
+                    for (int k = 0; k < len; k++) {
+                        const double *r = yt + ((size_t)k * n_ops + s) * 34;
+                        const double *g = gt + ((size_t)k * mdl->n_types + t) * 5;
+                        double A[5] = {0, 0, 0, 0, 0};
+                        for (int l = 0; l <= ml; l++)
+                            for (int i = l * l; i < (l + 1) * (l + 1); i++) A[l] += P[i] * r[i];
+                        const double fr = fc[(size_t)k * mdl->n_wfn + w] + mdl->pval[t] * fv[(size_t)k * mdl->n_types + t] +
+                                          anr + g[0] * A[0] - g[2] * A[2] + g[4] * A[4];
+                        const double fi = ani + g[1] * A[1] - g[3] * A[3];
+                        const double Xr = fr * er - fi * ei, Xi = fr * ei + fi * er;
+                        fl[2 * k] += weight * Xr;
+                        fl[2 * k + 1] += weight * Xi;
+                        const double cr = dtdf[2 * (k0 + k)], ci = dtdf[2 * (k0 + k) + 1];
+                        const double zr = cr * Xr - ci * Xi, zi = cr * Xi + ci * Xr;
+                        for (int j = 0; j < 3; j++) da[j] -= r[31 + j] * zi;
+                        for (int j = 0; j < 6; j++) da[3 + j] -= r[25 + j] * zr;
+                        da[9] += zr;
+                        /* next reflection of the line */
+                        const double qr = q * mr, qi = q * mi;
+                        const double nr = er * qr - ei * qi, ni = er * qi + ei * qr;
+                        er = nr;
+                        ei = ni;
+                        q *= c;
+                    }
+                }
+            }
+            for (int k = 0; k < len; k++) {
+                f_out[2 * (k0 + k)] = fl[2 * k];
+                f_out[2 * (k0 + k) + 1] = fl[2 * k + 1];
+            }
+        }
+        free(yt); free(fc); free(fv); free(gt); free(fl);
+    }
+    off = 0;
+    for (int a = 0; a < n_atoms; a++) {
+        double sm[10] = {0};
+        for (int t = 0; t < n_threads; t++)
+            for (int j = 0; j < 10; j++) sm[j] += acc[((size_t)t * n_atoms + a) * 10 + j];
+        const double weight = occ[a] * mult_w[a];
+        for (int j = 0; j < 3; j++) d_xyz[3 * a + j] = two_pi * weight * sm[j];
+        if (adp_n[a] == 6)
+            for (int j = 0; j < 6; j++) d_adp[off + j] = two_pi2 * weight * sm[3 + j];
+        else if (adp_n[a] == 1)
+            d_adp[off] = two_pi2 * weight * (sm[3] + sm[4] + sm[5]);
+        off += adp_n[a];
+        d_occ[a] = mult_w[a] * sm[9];
+    }
+    free(acc); free(pg); free(u6); free(dhs); free(dm); free(type_wfn); free(type_max_l);
+    return 0;
+}
+
 /* Per-reflection structure factors and derivatives: the per-hkl engine that the reference's
  * template calculateStructureFactorsAndDerivatives<T>(atoms, lcs, hkl, processor[, include])
  * dispatches to (HansenCoppens_SF_Engine.h:512-780, Engine v1, not Engine3).  For every

@@ -290,6 +290,71 @@ def sf_and_derivs(crystal, params, atoms, lcs, hkl, dtdf, include=None, n_thread
     return F, d_xyz, d_adp, do
 
 
+def line_order(hkl):
+    """Reflections along lines: the hkl axis with the fewest lines (findPreferredHklOrderingDirection,
+    scattering_utilities.cpp:371-435), lines in lexicographic order of the other two indices, each by
+    increasing index along the axis and split where a step is not +1 (splitHklLines, :463-517).
+    Returns (permutation, line offsets, axis)."""
+    hkl = np.asarray(hkl, dtype=np.int64).reshape(-1, 3)
+    best, best_n = 2, None
+    for ax in range(3):
+        o = [(ax + 1) % 3, (ax + 2) % 3]
+        n = len({(a, b) for a, b in hkl[:, o]})
+        if best_n is None or n < best_n:
+            best, best_n = ax, n
+    o1, o2 = (best + 1) % 3, (best + 2) % 3
+    perm = np.lexsort((hkl[:, best], hkl[:, o2], hkl[:, o1]))
+    hs = hkl[perm]
+    brk = np.ones(len(hs), dtype=bool)
+    if len(hs) > 1:
+        same = (hs[1:, o1] == hs[:-1, o1]) & (hs[1:, o2] == hs[:-1, o2]) & (hs[1:, best] == hs[:-1, best] + 1)
+        brk[1:] = ~same
+    off = np.append(np.nonzero(brk)[0], len(hs)).astype(np.int32)
+    return perm, off, best
+
+
+def sf_and_derivs_lines(crystal, params, atoms, lcs, hkl, dtdf, include=None, n_threads=1, anomalous=None):
+    """sf_and_derivs restated the way the reference's Engine3 computes it (global-frame P_lm, per
+    reflection tables, phase and Debye-Waller factors carried along hkl lines: hc_oracle_engine_lines)
+    - the CPU baseline of the benchmark.  Acentric groups only.  Same results as sf_and_derivs to
+    rounding (the recurrence error grows by a few ulp per step)."""
+    x = _engine_inputs(crystal, params, atoms, lcs, hkl, include, "reference", anomalous)
+    if x["centro"]:
+        raise ValueError("the line-recurrence restatement covers acentric groups")
+    m, n, f2c, J, adp_n, mult = x["m"], x["n"], x["f2c"], x["J"], x["adp_n"], x["mult"]
+    perm, off, ax = line_order(hkl)
+    c2f = np.linalg.inv(f2c)
+    d = np.conj(np.asarray(dtdf, dtype=np.complex128)) * mult
+    dd = np.ascontiguousarray(np.stack([d.real, d.imag], 1)[perm])
+    h_lines = np.ascontiguousarray(x["h_cart"][perm])
+    step = np.ascontiguousarray(c2f[ax].astype(np.float64))
+    fo = np.zeros((m, 2))
+    dx = np.zeros((n, 3))
+    du = np.zeros(max(1, int(adp_n.sum())))
+    do = np.zeros(n)
+    P = ctypes.c_void_p
+    rc = lib().hc_oracle_engine_lines(*_atom_args(x), ctypes.c_int(m), P(h_lines.ctypes.data), ctypes.c_int(len(off) - 1),
+                                      P(off.ctypes.data), P(step.ctypes.data), P(dd.ctypes.data), P(fo.ctypes.data),
+                                      P(dx.ctypes.data), P(du.ctypes.data), P(do.ctypes.data), ctypes.c_int(n_threads))
+    if rc != 0:
+        raise RuntimeError("oracle line engine failed")
+    F = np.zeros(m, dtype=np.complex128)
+    F[perm] = (fo[:, 0] + 1j * fo[:, 1])
+    F *= mult
+    d_xyz = dx @ f2c if crystal.xyz_convention == "fractional" else dx
+    d_adp = np.zeros((n, 6))
+    o = 0
+    for i in range(n):
+        if adp_n[i] == 6:
+            g = du[o:o + 6]
+            d_adp[i] = g @ J if crystal.adp_convention == "U_cif" else g
+            o += 6
+        elif adp_n[i] == 1:
+            d_adp[i, 0] = du[o]
+            o += 1
+    return F, d_xyz, d_adp, do
+
+
 def per_hkl(crystal, params, atoms, lcs, hkl, include=None, n_threads=1, anomalous=None):
     """Per-reflection F and derivatives of the reference's per-hkl path (HansenCoppens_SF_Engine.h:
     512-780 behind HansenCoppensStructureFactorCalculator.h:384-545).  Returns (F (m,) complex,

@@ -706,7 +706,9 @@ __device__ __forceinline__ void stage_f32(float2* buf, int kk, int lane, float r
     buf[stage_slot(kk, lane, kGrpF32)] = make_float2(re, im);
 }
 
-__device__ __forceinline__ void flush_f32(const float2* buf, int nk, int lane, double* __restrict__ fp)
+// The mixed-precision partials are FP32 sums of FP32 terms: stored as FP32 (the stage-2 reduction
+// converts them exactly), half the bytes of FP64 partials.
+__device__ __forceinline__ void flush_f32(const float2* buf, int nk, int lane, float* __restrict__ fp)
 {
     lds_wave_sync();
     const int kk = lane >> 3, part = lane & 7;
@@ -1219,7 +1221,7 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
             dka += fka;
             dkp += fkp;
         }
-        if (!BO) flush_f32(stage, kend - kb, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
+        if (!BO) flush_f32(stage, kend - kb, lane, reinterpret_cast<float*>(fpart) + ((size_t)wave * a.n_hkl + kb) * 2);
     }
     if (DERIV) {
         // U multipliers carried log2(e); undo for the derivative
@@ -1736,7 +1738,7 @@ __device__ __forceinline__ void body_mixed_diag(const MainArgs& a, int wave, int
             dka += fka;
             dkp += fkp;
         }
-        if (!BO) flush_f32(stage, kend - kb, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
+        if (!BO) flush_f32(stage, kend - kb, lane, reinterpret_cast<float*>(fpart) + ((size_t)wave * a.n_hkl + kb) * 2);
     }
     if (DERIV) {
         const double inv_log2e = 0.69314718055994530942;
@@ -2194,7 +2196,7 @@ __device__ __forceinline__ void body_jac_mixed(const MainArgs& a, const JacArgs&
                 put_row(o + 9 * plane, co * s0r, co * s0i);
             }
         }
-        flush_f32(stage, nk, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
+        flush_f32(stage, nk, lane, reinterpret_cast<float*>(fpart) + ((size_t)wave * a.n_hkl + kb) * 2);
     }
 }
 
@@ -2240,7 +2242,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 4 :
 }
 
 // F of a block of reflections [k0, k0 + n): mult(h) * sum_waves partial, into f[k - k0].
-__global__ void k_reduce_f_rows(const double2* __restrict__ fpart, const double* __restrict__ mult, int n_waves,
+template <typename P2>
+__global__ void k_reduce_f_rows(const P2* __restrict__ fpart, const double* __restrict__ mult, int n_waves,
                                 int n_hkl, int k0, int n, double* __restrict__ f)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2248,7 +2251,7 @@ __global__ void k_reduce_f_rows(const double2* __restrict__ fpart, const double*
     const int k = k0 + i;
     double re = 0, im = 0;
     for (int w = 0; w < n_waves; w++) {
-        const double2 v = fpart[(size_t)w * n_hkl + k];
+        const P2 v = fpart[(size_t)w * n_hkl + k];
         re += v.x;
         im += v.y;
     }
@@ -2375,14 +2378,15 @@ __global__ __launch_bounds__(256) void k_ff(const AtomRec* __restrict__ atoms, i
 
 // F(h) = mult(h) * sum_waves partial  (centric at origin: 2 Re, 0).  Loads are issued eight at a
 // time and added in wave order.
-__device__ __forceinline__ void reduce_f_item(int k, const double2* __restrict__ fpart,
+template <typename P2>
+__device__ __forceinline__ void reduce_f_item(int k, const P2* __restrict__ fpart,
                                               const double* __restrict__ mult, int n_waves, int n_hkl,
                                               int centric_origin, double* __restrict__ f)
 {
     double re = 0, im = 0;
     int w = 0;
     for (; w + 8 <= n_waves; w += 8) {
-        double2 v[8];
+        P2 v[8];
 #pragma unroll
         for (int q = 0; q < 8; q++) v[q] = fpart[(size_t)(w + q) * n_hkl + k];
 #pragma unroll
@@ -2392,7 +2396,7 @@ __device__ __forceinline__ void reduce_f_item(int k, const double2* __restrict__
         }
     }
     for (; w < n_waves; w++) {
-        const double2 v = fpart[(size_t)w * n_hkl + k];
+        const P2 v = fpart[(size_t)w * n_hkl + k];
         re += v.x;
         im += v.y;
     }
@@ -2404,7 +2408,8 @@ __device__ __forceinline__ void reduce_f_item(int k, const double2* __restrict__
     f[2 * k + 1] = im * mult[k];
 }
 
-__global__ void k_reduce_f(const double2* __restrict__ fpart, const double* __restrict__ mult, int n_waves,
+template <typename P2>
+__global__ void k_reduce_f(const P2* __restrict__ fpart, const double* __restrict__ mult, int n_waves,
                            int n_hkl, int centric_origin, double* __restrict__ f)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -3227,7 +3232,8 @@ void Engine::main_args(Precision prec, bool derivs, bool mp, int chunk, MainArgs
     const int n_waves = n_slots / kWave;
     const int n_hkl = n_hkl_;
     const int n_chunks = (n_hkl + chunk - 1) / chunk;
-    d_fpart_.ensure((size_t)n_waves * n_hkl * 2 * sizeof(double));
+    d_fpart_.ensure(std::max<size_t>((size_t)n_waves * n_hkl * 2 * (mixed ? sizeof(float) : sizeof(double)), 1));
+    part_f32_ = mixed;
     const int nd = mp ? kNdMultipole : kNdBasic;
     if (derivs || mp) d_dpart_.ensure((size_t)n_chunks * n_slots * nd * sizeof(double));
     MainArgs& a = *out;
@@ -3348,13 +3354,18 @@ void Engine::reduce(bool derivs, bool mp, int chunk)
     // outputs sized here: the autotuner reaches this before any run() has allocated them
     d_f_.ensure((size_t)std::max(n_hkl, 1) * 2 * sizeof(double));
     if (derivs) d_d_.ensure((size_t)std::max(n_atoms_, 1) * nd * sizeof(double));
-    if (d_fpart_.bytes < (size_t)n_waves * n_hkl * 2 * sizeof(double) ||
+    if (d_fpart_.bytes < (size_t)n_waves * n_hkl * 2 * (part_f32_ ? sizeof(float) : sizeof(double)) ||
         (derivs && d_dpart_.bytes < (size_t)n_chunks * n_slots * nd * sizeof(double)))
         throw Error(1, "internal: stage-2 partial buffers smaller than the launch geometry");
     const int T = 64;
-    hipLaunchKernelGGL(k_reduce_f, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, d_fpart_.as<double2>(),
-                       d_dtdf_.as<double>() + 2 * (size_t)n_hkl, n_waves, n_hkl, centric_origin_ ? 1 : 0,
-                       d_f_.as<double>());
+    if (part_f32_)
+        hipLaunchKernelGGL(k_reduce_f<float2>, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, d_fpart_.as<float2>(),
+                           d_dtdf_.as<double>() + 2 * (size_t)n_hkl, n_waves, n_hkl, centric_origin_ ? 1 : 0,
+                           d_f_.as<double>());
+    else
+        hipLaunchKernelGGL(k_reduce_f<double2>, dim3((n_hkl + T - 1) / T), dim3(T), 0, stream_, d_fpart_.as<double2>(),
+                           d_dtdf_.as<double>() + 2 * (size_t)n_hkl, n_waves, n_hkl, centric_origin_ ? 1 : 0,
+                           d_f_.as<double>());
     HIP_CHECK(hipGetLastError());
     if (!derivs) return;
     hipLaunchKernelGGL(k_reduce_d, dim3((n_slots * nd + T - 1) / T), dim3(T), 0, stream_, d_dpart_.as<double>(),
@@ -3699,7 +3710,7 @@ void Engine::run_dev(const DevCall& call, Precision prec, hipStream_t stream, vo
         const size_t b_dtdf = align256((size_t)std::max(n_hkl, 1) * 3 * sizeof(double));
         const size_t b_f = align256((size_t)std::max(n_hkl, 1) * 2 * sizeof(double));
         const size_t b_d = align256((size_t)std::max(n, 1) * kNdBasic * sizeof(double));
-        const size_t b_fpart = align256((size_t)n_waves * n_hkl * 2 * sizeof(double));
+        const size_t b_fpart = align256((size_t)n_waves * n_hkl * 2 * (prec == kMixed ? sizeof(float) : sizeof(double)));
         const size_t fixed = b_ain + b_atoms + b_dtdf + b_f + b_d + b_fpart;
         const size_t per_chunk = (size_t)n_sorted_ * kNdBasic * sizeof(double);
         if (fixed + (derivs ? per_chunk : 0) > scratch_size)
@@ -3888,8 +3899,13 @@ void Engine::jacobian(int k0, int k1, double* f_dev, double* rows_dev, Precision
                               0, stream_));
     if (timing) HIP_CHECK(hipEventRecord(ev1_, stream_));
     const int T = 64;
-    hipLaunchKernelGGL(k_reduce_f_rows, dim3((n + T - 1) / T), dim3(T), 0, stream_, d_fpart_.as<double2>(),
-                       d_dtdf_.as<double>() + 2 * (size_t)n_hkl_, n_waves, n_hkl_, k0, n, f_dev);
+    if (mixed)
+        hipLaunchKernelGGL(k_reduce_f_rows<float2>, dim3((n + T - 1) / T), dim3(T), 0, stream_, d_fpart_.as<float2>(),
+                           d_dtdf_.as<double>() + 2 * (size_t)n_hkl_, n_waves, n_hkl_, k0, n, f_dev);
+    else
+        hipLaunchKernelGGL(k_reduce_f_rows<double2>, dim3((n + T - 1) / T), dim3(T), 0, stream_,
+                           d_fpart_.as<double2>(), d_dtdf_.as<double>() + 2 * (size_t)n_hkl_, n_waves, n_hkl_, k0, n,
+                           f_dev);
     HIP_CHECK(hipGetLastError());
     if (timing) {
         HIP_CHECK(hipEventSynchronize(ev1_));
@@ -4050,6 +4066,7 @@ struct BatchItem {
     double* f;
     double* d;
     int n_ops, n_hkl, n_wfn, n_types, n_slots, n_atoms, nd, n_chunks, centric_origin, diag, n_tab;
+    int part_f32;     // F partials stored as FP32 (mixed precision)
     MainArgs main;
 };
 
@@ -4136,8 +4153,12 @@ __global__ void k_batch_reduce_f(const BatchItem* __restrict__ it, const int* __
     if (i >= total) return;
     const int b = find_item(off, nb, i);
     const BatchItem& q = it[b];
-    reduce_f_item(i - off[b], reinterpret_cast<const double2*>(q.main.fpart), q.mult, q.main.n_waves, q.n_hkl,
-                  q.centric_origin, q.f);
+    if (q.part_f32)
+        reduce_f_item(i - off[b], reinterpret_cast<const float2*>(q.main.fpart), q.mult, q.main.n_waves, q.n_hkl,
+                      q.centric_origin, q.f);
+    else
+        reduce_f_item(i - off[b], reinterpret_cast<const double2*>(q.main.fpart), q.mult, q.main.n_waves, q.n_hkl,
+                      q.centric_origin, q.f);
 }
 
 __global__ void k_batch_reduce_d(const BatchItem* __restrict__ it, const int* __restrict__ off, int nb, int total)
@@ -4165,6 +4186,7 @@ void Engine::batch_item(Precision prec, bool derivs, int chunk, BatchItem* out)
     q.ops = d_ops_.as<OpDev>();
     q.c2f = reinterpret_cast<const double*>(static_cast<const char*>(d_ops_.p) + n_ops * sizeof(OpDev));
     q.diag = use_diag(mixed, false) ? 1 : 0;
+    q.part_f32 = mixed ? 1 : 0;
     q.rec64 = mixed ? nullptr : d_hklop_.as<double>();
     q.rec32 = mixed ? d_hklop_.as<float>() : nullptr;
     q.phase = mixed ? d_phase_.as<double>() : nullptr;

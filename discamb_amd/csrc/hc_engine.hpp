@@ -250,6 +250,7 @@ private:
     int n_atoms_ = 0, n_sorted_ = 0, n_hkl_ = 0;
     bool ran_ = false;
     bool uploaded_ = false;         // the last upload completed (run() needs one)
+    bool part_f32_ = false;         // the F partials of the last main launch are FP32 (mixed precision)
     bool dev_ready_ = false;        // prepare_dev set the layout of device-resident calls
     int chunk_override_ = 0;        // reflection chunk forced by an allocation-free call
     DevConv dev_conv_;

@@ -26,6 +26,7 @@ FP64_PEAK = 78.6e12
 # passes scaled by a plain-copy calibration (profiles/*_hbm_traffic.md, "bench-traffic" lines)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_main_p212121_5000at_1.2A_hbm_traffic.md")
 TRAFFIC_FILE_ROWS = os.path.join(ROOT, "profiles", "k_jac_rows_hbm_traffic.md")
+BATCH_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "batch256_hbm_traffic.md")
 
 
 def measured_traffic(kernel, path=TRAFFIC_FILE):
@@ -35,6 +36,19 @@ def measured_traffic(kernel, path=TRAFFIC_FILE):
             for line in fh:
                 if line.startswith("bench-traffic ") and line.split()[1] == kernel:
                     return float(line.split()[2])
+    except OSError:
+        pass
+    return None
+
+
+def batch_traffic(chunk, path=BATCH_TRAFFIC_FILE):
+    """HBM bytes per batched step recorded for the 256-structure workload at this mixed chunk, or None."""
+    try:
+        with open(path) as fh:
+            for line in fh:
+                f = line.split()
+                if line.startswith("batch-traffic ") and int(f[1]) == chunk:
+                    return float(f[2])
     except OSError:
         pass
     return None
@@ -169,8 +183,9 @@ def bench_batch(args):
     for mode in ("HIP_MIXED", "HIP"):
         for c in calcs:
             c.setCalculationMode(mode)
-        batch.setTuning(args.chunk)
-        chunk = batch.autotune() if args.chunk == 0 else args.chunk
+        fixed = (args.chunk_fp64 or args.chunk) if mode == "HIP" else args.chunk
+        batch.setTuning(fixed)
+        chunk = batch.autotune() if fixed == 0 else fixed
         for _ in range(args.warmup):
             batch.run(graph=True)
         dev = []
@@ -182,7 +197,7 @@ def bench_batch(args):
         out = [c.fetch() for c in calcs]
         # each structure alone: a one-structure batch (same kernels, graph replay), device time
         single = []
-        for c in calcs:
+        for c in calcs if args.singles else []:
             b1 = BatchCalculator([c])
             b1.setTuning(chunk)
             b1.run(graph=True)
@@ -195,13 +210,16 @@ def bench_batch(args):
         res[mode] = {"wall_ms": (t2 - t1) / args.steps * 1e3, "device_ms": float(np.median(dev)), "chunk": chunk,
                      "out": out, "single_ms": np.array(single), "units": batch.lastTiming()["units"],
                      "grid": batch.lastTiming()["grid"]}
-        print("%s: batched step %.3f ms wall, %.3f ms device; one structure at a time: sum %.3f ms, p50 %.3f ms, "
-              "p99 %.3f ms" % (mode, res[mode]["wall_ms"], res[mode]["device_ms"], np.sum(single),
-                               np.percentile(single, 50), np.percentile(single, 99)), file=sys.stderr, flush=True)
+        print("%s: batched step %.3f ms wall, %.3f ms device, chunk %d%s" % (
+            mode, res[mode]["wall_ms"], res[mode]["device_ms"], chunk, "; one structure at a time: sum %.3f ms, p50 %.3f "
+            "ms, p99 %.3f ms" % (np.sum(single), np.percentile(single, 50), np.percentile(single, 99)) if single else ""),
+            file=sys.stderr, flush=True)
     mx, fp = res["HIP_MIXED"], res["HIP"]
     errs = [float(np.abs(a[0] - b[0]).max() / np.abs(b[0]).max()) for a, b in zip(mx["out"], fp["out"])]
 
     def pct(r):
+        if len(r["single_ms"]) == 0:
+            return None
         return {"p50_ms": float(np.percentile(r["single_ms"], 50)), "p99_ms": float(np.percentile(r["single_ms"], 99)),
                 "sum_ms": float(np.sum(r["single_ms"]))}
 
@@ -224,7 +242,9 @@ def bench_batch(args):
                    "structures": len(ws), "terms_per_step": terms, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "roof": "FP32 vector ALU; whole batched step (prep + main + reductions)",
                      "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK, "traffic": None},
+                     "frac": achieved / FP32_PEAK, "traffic": batch_traffic(mx["chunk"]),
+                     "traffic_note": "HBM bytes per batched step, summed over the step's kernels, from the calibrated "
+                                     "FETCH_SIZE + WRITE_SIZE counters recorded for this workload and chunk"},
         "step_time": {"device_ms_mixed": mx["device_ms"], "wall_ms_mixed": mx["wall_ms"],
                       "device_ms_fp64": fp["device_ms"], "wall_ms_fp64": fp["wall_ms"],
                       "note": "device = events around the graph launch; wall adds descriptor upload and sync"},
@@ -232,7 +252,7 @@ def bench_batch(args):
         "per_structure_fp64": pct(fp),
         "fp64": {"value": terms / (fp["device_ms"] * 1e-3), "unit": "terms/s", "ms_per_step": fp["device_ms"]},
         "speedup_mixed_vs_fp64": fp["device_ms"] / mx["device_ms"],
-        "batching_speedup_mixed": pct(mx)["sum_ms"] / mx["device_ms"],
+        "batching_speedup_mixed": pct(mx)["sum_ms"] / mx["device_ms"] if pct(mx) else None,
         "accuracy_mixed_vs_fp64": {"max_over_structures_max_dF_over_maxF": max(errs)},
         "launch": {"chunk_mixed": mx["chunk"], "chunk_fp64": fp["chunk"], "units": mx["units"],
                    "persistent_grid": mx["grid"], "graph": True},
@@ -244,12 +264,18 @@ def bench_batch(args):
         sample = ws[:8]
         t0 = time.perf_counter()
         for w in sample:
-            hc_oracle.sf_and_derivs(w.crystal, w.params, w.crystal.atoms, w.lcs, w.hkl, w.dtdf, n_threads=threads)
+            centro = hc_oracle.decompose_group(w.crystal.symops)[2]
+            fn = hc_oracle.sf_and_derivs if centro else hc_oracle.sf_and_derivs_lines
+            fn(w.crystal, w.params, w.crystal.atoms, w.lcs, w.hkl, w.dtdf, n_threads=threads)
         dt = time.perf_counter() - t0
         st = sum(w.terms for w in sample)
         line["cpu_baseline"] = {"value": st / dt, "unit": "terms/s", "cores": threads, "kind": "port",
-                                "sample": "FP64 OpenMP oracle on the first 8 structures of the batch (%.3e terms), "
-                                          "%.1f s wall" % (st, dt)}
+                                "cpu_model": cpu_model(),
+                                "sample": "FP64 OpenMP restatement (hkl-line recurrences for the acentric groups, the "
+                                          "direct oracle for the centrosymmetric ones) on the first 8 structures of "
+                                          "the batch (%s; %.3e terms), %.1f s wall" % (
+                                              ", ".join(w.name for w in sample), st, dt),
+                                "calibration": CPU_CALIBRATION}
     print(json.dumps(line), flush=True)
 
 
@@ -625,6 +651,8 @@ def main():
                          "its atomic form factors; taam: whole calls with local frames recomputed from the "
                          "positions and the chain rule through them; autograd: torch least-squares gradient "
                          "steps (forward F, backward with the multipole adjoint)")
+    ap.add_argument("--no-singles", dest="singles", action="store_false",
+                    help="batch256: skip the one-structure-at-a-time timings")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain read+write elementwise kernel over MIB MiB (PMC byte calibration)")
     args = ap.parse_args()
@@ -685,8 +713,10 @@ def main():
     host_mx = time_host_calls(calc, w)
     host_mx_rt = time_host_calls(calc, w, retype=True)
     calc.setCalculationMode("HIP")
+    calc.setTuning(chunk64)
     host_64 = time_host_calls(calc, w)
     host_64_rt = time_host_calls(calc, w, retype=True)
+    calc.setTuning(args.chunk)
     calc.setCalculationMode("HIP_MIXED")
     print("whole calls from host buffers: mixed %.3f ms (new type parameters %.3f ms), FP64 %.3f ms (%.3f ms)" % (
         host_mx * 1e3, host_mx_rt * 1e3, host_64 * 1e3, host_64_rt * 1e3), file=sys.stderr, flush=True)
@@ -725,7 +755,10 @@ def main():
                      "flops_per_term": "66 + 2(L+1)^2 + 2(L+1): 126 (L=4), 90 (L=2)"},
         "fp64": {"value": terms / s64, "unit": "terms/s", "ms_per_step": s64 * 1e3, "kernel_ms": k64,
                  "achieved_tflops": flops / (k64 * 1e-3) / 1e12, "frac_of_fp64_peak": flops / (k64 * 1e-3) / FP64_PEAK,
-                 "frac_of_fp32_peak": flops / (k64 * 1e-3) / FP32_PEAK},
+                 "frac_of_fp32_peak": flops / (k64 * 1e-3) / FP32_PEAK,
+                 "kernel": "k_main<fp64,deriv,line,diag222>",
+                 "traffic": measured_traffic("k_main<fp64,deriv,line,diag222>")
+                 if args.workload == "p212121_5000" else None},
         "host_call": {"ms": host_mx * 1e3, "terms_per_s": terms / host_mx,
                       "fp64_ms": host_64 * 1e3, "fp64_terms_per_s": terms / host_64,
                       "after_new_type_parameters_ms": host_mx_rt * 1e3,

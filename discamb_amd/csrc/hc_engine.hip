@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -2838,6 +2839,7 @@ void Engine::upload(const EngineAtoms& at, int n_hkl, const int* hkl, const doub
         upload_checked(at, n_hkl, hkl, dtdf, rows_layout);
     } catch (...) {
         prep_valid_ = false;
+        hkl_gen_++;
         last_hkl_.clear();
         hkl_perm_.clear();
         n_atoms_ = n_hkl_ = n_sorted_ = n_included_ = 0;
@@ -3008,6 +3010,7 @@ void Engine::upload_checked(const EngineAtoms& at, int n_hkl, const int* hkl, co
     int* hkl_s = h_up_.at<int>(off_hkl);
     if (!same_hkl) {
         prep_valid_ = false;
+        hkl_gen_++;
         std::vector<std::pair<double, int>> key(n_hkl);
         for (int k = 0; k < n_hkl; k++) {
             double v = 0.0;
@@ -4118,32 +4121,32 @@ __global__ __launch_bounds__(64) void k_batch_atom_prep(const BatchItem* __restr
     atom_prep_item(blockIdx.x - off[b], threadIdx.x, fsh, q.ain, q.types, q.atoms);
 }
 
-// Persistent: every workgroup takes (structure, atom block, reflection chunk) units from the
-// device counter until none is left, so uneven structures share the machine without a tail per
-// structure.  Units are ordered costliest structure first.  Radial factors are evaluated on the
-// fly from each structure's per-wavefunction tables (no per-(reflection, type) records).
+// Persistent: every wave takes (structure, atom wave, reflection chunk) units from the device
+// counter until none is left, so uneven structures share the machine without a tail per structure
+// and a structure of fewer than four atom waves leaves no wave of a workgroup idle.  Units are
+// ordered costliest structure first, and within a structure chunk by chunk, so the waves working
+// at one time share a chunk's reflection records.  Radial factors are evaluated on the fly from
+// each structure's per-wavefunction tables (no per-(reflection, type) records).
 template <bool MIXED, bool DERIV, bool DIAG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 2))) void k_batch_main(
     const BatchItem* __restrict__ it, const int* __restrict__ off, int nb, int total, int* __restrict__ counter)
 {
     constexpr int kStageBytes = MIXED ? kStageBytesF32 : kStageBytesF64;
     __shared__ __align__(16) unsigned char stage_lds[kWavesPerBlock * kStageBytes];
-    __shared__ int s_unit;
     const int wib = (int)(threadIdx.x / kWave);
     const int lane = threadIdx.x & (kWave - 1);
     void* stage = stage_lds + wib * kStageBytes;
     for (;;) {
-        if (threadIdx.x == 0) s_unit = atomicAdd(counter, 1);
-        __syncthreads();
-        const int u = __builtin_amdgcn_readfirstlane(s_unit);
-        __syncthreads();
+        int got = 0;
+        if (lane == 0) got = atomicAdd(counter, 1);
+        const int u = __builtin_amdgcn_readfirstlane(__shfl(got, 0));
         if (u >= total) break;
         const int b = __builtin_amdgcn_readfirstlane(find_item(off, nb, u));
         const MainArgs& a = it[b].main;
         const int local = u - off[b];
-        const int chunk_id = local / a.grid_x;
-        const int ablock = local - chunk_id * a.grid_x;
-        main_unit<MIXED, DERIV, 0, DIAG, true>(a, chunk_id, ablock, wib, lane, stage);
+        const int chunk_id = local / a.n_waves;
+        const int wave = local - chunk_id * a.n_waves;
+        main_unit<MIXED, DERIV, 0, DIAG, true>(a, chunk_id, wave / kWavesPerBlock, wave % kWavesPerBlock, lane, stage);
     }
 }
 
@@ -4175,9 +4178,8 @@ void Engine::batch_item(Precision prec, bool derivs, int chunk, BatchItem* out)
     HIP_CHECK(hipSetDevice(device_));
     if (!uploaded_) throw Error(1, "batched step: a structure has no uploaded inputs");
     const bool mixed = prec == kMixed;
-    jac_prepped_ = false;     // the batched prep rewrites this engine's tables
+    jac_prepped_ = false;     // the batched step rewrites this engine's atom records
     prep_valid_ = false;
-    ensure_prep_buffers(prec, false);
     BatchItem& q = *out;
     std::memset(&q, 0, sizeof(q));
     main_args(prec, derivs, false, chunk, &q.main);
@@ -4187,11 +4189,7 @@ void Engine::batch_item(Precision prec, bool derivs, int chunk, BatchItem* out)
     q.c2f = reinterpret_cast<const double*>(static_cast<const char*>(d_ops_.p) + n_ops * sizeof(OpDev));
     q.diag = use_diag(mixed, false) ? 1 : 0;
     q.part_f32 = mixed ? 1 : 0;
-    q.rec64 = mixed ? nullptr : d_hklop_.as<double>();
-    q.rec32 = mixed ? d_hklop_.as<float>() : nullptr;
-    q.phase = mixed ? d_phase_.as<double>() : nullptr;
-    q.ht = q.diag ? d_ht_.as<double>() : nullptr;
-    q.hlen = d_radial32_.as<double>();
+    // reflection records (rec64/rec32, phase, ht, hlen) live in the batch's own buffer
     q.types = d_type_.as<TypeRec>();
     q.wfns = reinterpret_cast<const WfnRec*>(static_cast<const char*>(d_type_.p) + model_.n_types * sizeof(TypeRec));
     q.core = d_core_.as<TermRec>();
@@ -4200,8 +4198,6 @@ void Engine::batch_item(Precision prec, bool derivs, int chunk, BatchItem* out)
     q.tab = d_tab_.as<double>();
     q.tab32 = d_tab32_.as<float>();
     q.n_tab = model_.n_types > 0 ? n_tab_nodes_ : 0;
-    q.rad64 = mixed ? nullptr : d_radial_.as<double>();
-    q.rad32 = mixed ? d_radial_.as<float>() : nullptr;
     q.ain = d_atoms_in_.as<AtomIn>();
     q.atoms = d_atoms_.as<AtomRec>();
     q.nd = kNdBasic;
@@ -4235,8 +4231,11 @@ BatchEngine::BatchEngine(const std::vector<Engine*>& engines) : eng_(engines)
         if (!e || e->device() != device_) throw Error(1, "every structure of a batch must use the same device");
     HIP_CHECK(hipSetDevice(device_));
     HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    HIP_CHECK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
     HIP_CHECK(hipEventCreate(&ev0_));
     HIP_CHECK(hipEventCreate(&ev1_));
+    HIP_CHECK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
     d_counter_.ensure(256);
 }
 
@@ -4245,6 +4244,9 @@ BatchEngine::~BatchEngine()
     drop_graph();
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
+    if (fork_) (void)hipEventDestroy(fork_);
+    if (join_) (void)hipEventDestroy(join_);
+    if (side_) (void)hipStreamDestroy(side_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -4289,6 +4291,46 @@ void BatchEngine::plan(Precision prec, bool derivs, int chunk)
     std::vector<BatchItem> sorted(nb);
     for (int b = 0; b < nb; b++) sorted[b] = items[order[b]];
     share_tables(prec, order, sorted);
+    // reflection records: one 256-byte aligned slice per structure of the batch's own buffer
+    const bool mixed = prec == kMixed;
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    std::vector<size_t> sz(4 * (size_t)nb);
+    size_t rec_total = 0;
+    for (int b = 0; b < nb; b++) {
+        const BatchItem& q = sorted[b];
+        const size_t n_hkl = (size_t)q.n_hkl, n_rec = q.diag ? n_hkl : n_hkl * q.n_ops;
+        size_t* z = &sz[4 * (size_t)b];
+        z[0] = align(n_rec * kHklopRec * (mixed ? sizeof(float) : sizeof(double)));   // records
+        z[1] = mixed ? align(n_rec * 4 * sizeof(double)) : 0;                           // phases / h
+        z[2] = q.diag ? align(n_hkl * q.n_ops * sizeof(double)) : 0;                   // h.t per op
+        z[3] = align(n_hkl * sizeof(double));                                           // |h|
+        rec_total += z[0] + z[1] + z[2] + z[3];
+    }
+    d_rec_.ensure(std::max<size_t>(rec_total, 256));
+    char* base = static_cast<char*>(d_rec_.p);
+    rec_key_.assign({(unsigned long long)(mixed ? 1 : 0), (unsigned long long)(uintptr_t)d_rec_.p,
+                     (unsigned long long)nb});
+    for (int b = 0; b < nb; b++) {
+        BatchItem& q = sorted[b];
+        const size_t* z = &sz[4 * (size_t)b];
+        q.rec64 = mixed ? nullptr : reinterpret_cast<double*>(base);
+        q.rec32 = mixed ? reinterpret_cast<float*>(base) : nullptr;
+        q.main.hklop = base;
+        base += z[0];
+        q.phase = mixed ? reinterpret_cast<double*>(base) : nullptr;
+        q.main.phase = q.phase;
+        base += z[1];
+        q.ht = q.diag ? reinterpret_cast<double*>(base) : nullptr;
+        q.main.ht = q.ht;
+        base += z[2];
+        q.hlen = reinterpret_cast<double*>(base);
+        q.main.hlen = q.hlen;
+        base += z[3];
+        const Engine* e = eng_[order[b]];
+        rec_key_.push_back((unsigned long long)(uintptr_t)e);
+        rec_key_.push_back(e->hkl_generation());
+        rec_key_.push_back((unsigned long long)q.n_hkl);
+    }
     off_.assign((size_t)kNumSpaces * (nb + 1), 0);
     total_.assign(kNumSpaces, 0);
     for (int sp = 0; sp < kNumSpaces; sp++) {
@@ -4303,8 +4345,8 @@ void BatchEngine::plan(Precision prec, bool derivs, int chunk)
                 case kSpTab: n = q.n_tab; break;
                 case kSpSlot: n = q.n_slots; break;
                 case kSpRedD: n = derivs ? (long long)q.n_slots * q.nd : 0; break;
-                case kSpUnitD: n = (q.main.n_waves > 0 && q.main.diag_mask) ? q.main.n_blocks : 0; break;
-                case kSpUnitG: n = (q.main.n_waves > 0 && !q.main.diag_mask) ? q.main.n_blocks : 0; break;
+                case kSpUnitD: n = (q.main.n_waves > 0 && q.main.diag_mask) ? (long long)q.main.n_waves * q.n_chunks : 0; break;
+                case kSpUnitG: n = (q.main.n_waves > 0 && !q.main.diag_mask) ? (long long)q.main.n_waves * q.n_chunks : 0; break;
             }
             acc += n;
         }
@@ -4431,9 +4473,6 @@ void BatchEngine::launch(Precision prec, bool derivs)
     const int T = 256;
     hipLaunchKernelGGL(k_batch_fill, blocks(total_[kSpFill], T), dim3(T), 0, stream_, it, o(kSpFill), nb,
                        total_[kSpFill], d_counter_.as<int>());
-    if (total_[kSpHkl] > 0)
-        hipLaunchKernelGGL(k_batch_hkl_prep, blocks(total_[kSpHkl], T), dim3(T), 0, stream_, it, o(kSpHkl), nb,
-                           total_[kSpHkl]);
     if (n_shared_nodes_ > 0) {
         hipLaunchKernelGGL(k_tab_build, dim3(n_shared_nodes_), dim3(64), 0, stream_, d_gtabs_.as<TabRec>(),
                            n_shared_wfn_, d_gwfn_.as<WfnRec>(), d_gcore_.as<TermRec>(), d_gval_.as<TermRec>(),
@@ -4447,6 +4486,13 @@ void BatchEngine::launch(Precision prec, bool derivs)
         hipLaunchKernelGGL(k_batch_atom_prep, dim3(total_[kSpSlot]), dim3(64), 0, stream_, it, o(kSpSlot), nb);
     HIP_CHECK(hipGetLastError());
     last_grid = 0;
+    // the two kernel flavours run side by side (the general-rotation one on a second stream
+    // forked from and joined back to the main one), so neither waits for the other's tail
+    const bool both = total_[kSpUnitD] > 0 && total_[kSpUnitG] > 0;
+    if (both) {
+        HIP_CHECK(hipEventRecord(fork_, stream_));
+        HIP_CHECK(hipStreamWaitEvent(side_, fork_, 0));
+    }
     for (int flavour = 0; flavour < 2; flavour++) {
         const int sp = flavour == 0 ? kSpUnitD : kSpUnitG;
         if (total_[sp] == 0) continue;
@@ -4457,7 +4503,12 @@ void BatchEngine::launch(Precision prec, bool derivs)
         const int* ou = o(sp);
         int* cnt = d_counter_.as<int>() + flavour;
         void* args[] = {(void*)&it, (void*)&ou, (void*)&nbv, (void*)&tot, (void*)&cnt};
-        HIP_CHECK(hipLaunchKernel(batch_main_kernel(mixed, derivs, diag), dim3(grid), dim3(256), args, 0, stream_));
+        HIP_CHECK(hipLaunchKernel(batch_main_kernel(mixed, derivs, diag), dim3(grid), dim3(256), args, 0,
+                                  both && !diag ? side_ : stream_));
+    }
+    if (both) {
+        HIP_CHECK(hipEventRecord(join_, side_));
+        HIP_CHECK(hipStreamWaitEvent(stream_, join_, 0));
     }
     const int R = 64;
     if (total_[kSpHkl] > 0)
@@ -4466,6 +4517,18 @@ void BatchEngine::launch(Precision prec, bool derivs)
     if (derivs && total_[kSpRedD] > 0)
         hipLaunchKernelGGL(k_batch_reduce_d, blocks(total_[kSpRedD], R), dim3(R), 0, stream_, it, o(kSpRedD), nb,
                            total_[kSpRedD]);
+    HIP_CHECK(hipGetLastError());
+}
+
+// The per-reflection records depend on the reflection lists, the operations and the cell only: a
+// refinement's steps reuse them, so they are built outside the step's graph when rec_key_ changes.
+void BatchEngine::launch_hkl_prep()
+{
+    const int nb = (int)eng_.size();
+    if (total_[kSpHkl] > 0)
+        hipLaunchKernelGGL(k_batch_hkl_prep, dim3((unsigned)((total_[kSpHkl] + 255) / 256)), dim3(256), 0, stream_,
+                           d_items_.as<BatchItem>(), d_off_.as<int>() + (size_t)kSpHkl * (nb + 1), nb,
+                           total_[kSpHkl]);
     HIP_CHECK(hipGetLastError());
 }
 
@@ -4503,13 +4566,16 @@ void BatchEngine::run(Precision prec, bool derivs, bool graph)
             graph_key_ = key;
         }
         HIP_CHECK(hipEventRecord(ev0_, stream_));
+        if (rec_key_ != last_rec_key_) launch_hkl_prep();
         HIP_CHECK(hipGraphLaunch(exec_, stream_));
         HIP_CHECK(hipEventRecord(ev1_, stream_));
     } else {
         HIP_CHECK(hipEventRecord(ev0_, stream_));
+        if (rec_key_ != last_rec_key_) launch_hkl_prep();
         launch(prec, derivs);
         HIP_CHECK(hipEventRecord(ev1_, stream_));
     }
+    last_rec_key_ = rec_key_;
     HIP_CHECK(hipEventSynchronize(ev1_));
     HIP_CHECK(hipEventElapsedTime(&last_ms, ev0_, ev1_));
     for (Engine* e : eng_) e->mark_batched_run(derivs);

@@ -185,6 +185,8 @@ public:
     // part of the step; after the batched launch, mark the results as a completed run.
     void batch_item(Precision prec, bool derivs, int chunk, BatchItem* out);
     void mark_batched_run(bool derivs);
+    // changes whenever the device reflection list (its content or order) changes
+    unsigned long long hkl_generation() const { return hkl_gen_; }
     int tab_nodes(int wfn) const { return wfn < (int)tab_nodes_.size() ? tab_nodes_[wfn] : 3; }
     // Device-resident calls.  prepare_dev uploads the reflection list (host memory) with every
     // atom of the model in its slot (excluded atoms get weight 0 per call), keeps the reflection
@@ -280,6 +282,7 @@ private:
         return use_line(mixed, mp) && diag_ && !tune.general_path ? diag64_code() : 0;
     }
     bool prep_valid_ = false;       // the prep buffers hold the records of prep_key_ (run_prep)
+    unsigned long long hkl_gen_ = 0;
     int prep_key_ = -1;
     std::vector<int> tab_nodes_;    // radial-table node count per wavefunction
     // model tables
@@ -329,12 +332,17 @@ private:
     void plan(Precision prec, bool derivs, int chunk);
     void share_tables(Precision prec, const std::vector<int>& order, std::vector<BatchItem>& items);
     void launch(Precision prec, bool derivs);
+    void launch_hkl_prep();
     int chunk_for(Precision prec, bool derivs);
     void drop_graph();
+    // per-reflection records of every structure, owned by the batch: rebuilt only when a
+    // structure's reflection list, the precision or the buffer changes (rec_key_)
+    DeviceBuffer d_rec_;
+    std::vector<unsigned long long> rec_key_, last_rec_key_;
     std::vector<Engine*> eng_;
     int device_ = 0;
-    hipStream_t stream_ = nullptr;
-    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    hipStream_t stream_ = nullptr, side_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr, fork_ = nullptr, join_ = nullptr;
     DeviceBuffer d_items_, d_off_, d_counter_;
     // radial tables shared by the structures of the batch (one per distinct wavefunction)
     DeviceBuffer d_gwfn_, d_gcore_, d_gval_, d_gtabs_, d_ltabs_, d_gtab_, d_gtab32_;

@@ -107,6 +107,31 @@ def test_batch_graph_replays_follow_new_inputs(mode):
             assert max(e) == 0.0, (rep, ws[i].name, e)
 
 
+def test_batch_records_follow_new_reflections_and_precision():
+    """The batch keeps its per-reflection records between steps; they are rebuilt when a
+    structure gets a different reflection list of the same length (Friedel mates: same |h|, other
+    records) and when the precision changes.  Every step is compared with single calls."""
+    ws = _structures()[:5]
+    calcs = _calcs(ws, "HIP_MIXED")
+    batch = BatchCalculator(calcs)
+    for step, (mode, flip) in enumerate([("HIP_MIXED", False), ("HIP_MIXED", False), ("HIP_MIXED", True),
+                                         ("HIP", True), ("HIP_MIXED", True), ("HIP_MIXED", False)]):
+        hkls = [-w.hkl if (flip and i % 2 == 0) else w.hkl for i, w in enumerate(ws)]
+        single = []
+        for w, c, h in zip(ws, calcs, hkls):
+            c.setCalculationMode(mode)
+            single.append(c.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, h, w.dtdf))
+        for w, c, h in zip(ws, calcs, hkls):
+            c.upload(w.crystal.atoms, w.lcs, h, w.dtdf)
+        print("case record reuse step %d mode=%s flip=%s" % (step, mode, flip), flush=True)
+        batch.run(graph=True)
+        tol = 1e-12 if mode == "HIP" else 1e-6
+        for i, (w, c) in enumerate(zip(ws, calcs)):
+            f, d = c.fetch()
+            e = _errs(f, d, *single[i])
+            assert max(e) <= tol, (step, w.name, e)
+
+
 def test_batch_autotune_and_selector():
     ws = _structures()[:5]
     calcs = _calcs(ws, "HIP_MIXED")

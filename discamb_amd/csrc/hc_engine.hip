@@ -2951,19 +2951,41 @@ void Engine::upload_checked(const EngineAtoms& at, int n_hkl, const int* hkl, co
             wave_cls.push_back(c);
         }
     } else {
+        // within a class by wavefunction, then type: the lanes of a wave then share table rows
         std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
             int cx = cls_of(x), cy = cls_of(y);
             if (cx != cy) return cx < cy;
+            const int wx = model_.type_wfn[at.type[x]], wy = model_.type_wfn[at.type[y]];
+            if (wx != wy) return wx < wy;
             return at.type[x] < at.type[y];
         });
-        for (int pass = 2; pass <= 4; pass += 2) {
-            size_t start = order_.size();
-            for (int i : idx)
-                if (cls_of(i) == pass) order_.push_back(i);
-            size_t cnt = order_.size() - start;
-            size_t padded = (cnt + kWave - 1) / kWave * kWave;
-            while (order_.size() < start + padded) order_.push_back(-1);
-            for (size_t wv = 0; wv < padded / kWave; wv++) wave_cls.push_back(pass);
+        size_t n2 = 0;
+        for (int i : idx) n2 += cls_of(i) == 2 ? 1 : 0;
+        const size_t n4 = idx.size() - n2;
+        auto waves = [](size_t n) { return (n + kWave - 1) / kWave; };
+        if (waves(n2 + n4) < waves(n2) + waves(n4)) {
+            // one padded run when that saves a wave (small structures): a wave takes the larger
+            // class of its atoms; the l > 2 multipole and radial terms of a class-2 type are zero
+            for (int i : idx) order_.push_back(i);
+            while (order_.size() % kWave) order_.push_back(-1);
+            for (size_t wv = 0; wv < order_.size() / kWave; wv++) {
+                int c = 2;
+                for (int l = 0; l < kWave; l++) {
+                    const int i = order_[wv * kWave + l];
+                    if (i >= 0) c = std::max(c, cls_of(i));
+                }
+                wave_cls.push_back(c);
+            }
+        } else {
+            for (int pass = 2; pass <= 4; pass += 2) {
+                size_t start = order_.size();
+                for (int i : idx)
+                    if (cls_of(i) == pass) order_.push_back(i);
+                size_t cnt = order_.size() - start;
+                size_t padded = (cnt + kWave - 1) / kWave * kWave;
+                while (order_.size() < start + padded) order_.push_back(-1);
+                for (size_t wv = 0; wv < padded / kWave; wv++) wave_cls.push_back(pass);
+            }
         }
     }
     n_sorted_ = (int)order_.size();
@@ -4073,9 +4095,10 @@ struct BatchItem {
     MainArgs main;
 };
 
-// index spaces of the batched launches; offsets are prefix sums over structures
-// units are split by kernel flavour: parity-folded (kSpUnitD) and general rotations (kSpUnitG)
-enum BatchSpace { kSpFill = 0, kSpHkl, kSpTab, kSpSlot, kSpRedD, kSpUnitD, kSpUnitG, kNumSpaces };
+// index spaces of the batched launches; offsets are prefix sums over structures.  The main
+// kernel's (structure, atom wave, reflection chunk) units follow in kXcds queues, queue x holding
+// the chunks c with c % kXcds == x of every structure (kSpUnit + x).
+enum BatchSpace { kSpFill = 0, kSpHkl, kSpTab, kSpSlot, kSpRedD, kSpUnit, kNumSpaces = kSpUnit + kXcds };
 
 // structure b with off[b] <= i < off[b + 1] (empty structures are skipped)
 __device__ __forceinline__ int find_item(const int* __restrict__ off, int nb, int i)
@@ -4093,7 +4116,7 @@ __global__ void k_batch_fill(const BatchItem* __restrict__ it, const int* __rest
                              int* __restrict__ counter)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) counter[0] = counter[1] = 0;    // work counters of the persistent main kernels
+    if (i < kXcds) counter[i] = 0;    // work counters of the persistent main kernel's queues
     if (i >= total) return;
     const int b = find_item(off, nb, i);
     it[b].d[i - off[b]] = 0.0;
@@ -4122,31 +4145,47 @@ __global__ __launch_bounds__(64) void k_batch_atom_prep(const BatchItem* __restr
 }
 
 // Persistent: every wave takes (structure, atom wave, reflection chunk) units from the device
-// counter until none is left, so uneven structures share the machine without a tail per structure
-// and a structure of fewer than four atom waves leaves no wave of a workgroup idle.  Units are
-// ordered costliest structure first, and within a structure chunk by chunk, so the waves working
-// at one time share a chunk's reflection records.  Radial factors are evaluated on the fly from
-// each structure's per-wavefunction tables (no per-(reflection, type) records).
-template <bool MIXED, bool DERIV, bool DIAG>
+// counters until none is left, so uneven structures share the machine without a tail per
+// structure and a structure of fewer than four atom waves leaves no wave of a workgroup idle.
+// Workgroup b runs on XCD b % kXcds and drains that XCD's queue first: all atom waves of a chunk
+// then run on one XCD and its L2 holds the chunk's reflection records once.  A wave whose queue
+// is empty takes units from the other queues in turn.  In each queue units are ordered costliest
+// structure first, and within a structure chunk by chunk.  Parity-folded and general-rotation
+// structures share the kernel (the branch is uniform per unit).  Radial factors are evaluated on
+// the fly from the per-wavefunction tables (no per-(reflection, type) records).
+template <bool MIXED, bool DERIV>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 2))) void k_batch_main(
-    const BatchItem* __restrict__ it, const int* __restrict__ off, int nb, int total, int* __restrict__ counter)
+    const BatchItem* __restrict__ it, const int* __restrict__ uoff, int nb, int* __restrict__ counter)
 {
     constexpr int kStageBytes = MIXED ? kStageBytesF32 : kStageBytesF64;
     __shared__ __align__(16) unsigned char stage_lds[kWavesPerBlock * kStageBytes];
     const int wib = (int)(threadIdx.x / kWave);
     const int lane = threadIdx.x & (kWave - 1);
     void* stage = stage_lds + wib * kStageBytes;
-    for (;;) {
-        int got = 0;
-        if (lane == 0) got = atomicAdd(counter, 1);
-        const int u = __builtin_amdgcn_readfirstlane(__shfl(got, 0));
-        if (u >= total) break;
-        const int b = __builtin_amdgcn_readfirstlane(find_item(off, nb, u));
-        const MainArgs& a = it[b].main;
-        const int local = u - off[b];
-        const int chunk_id = local / a.n_waves;
-        const int wave = local - chunk_id * a.n_waves;
-        main_unit<MIXED, DERIV, 0, DIAG, true>(a, chunk_id, wave / kWavesPerBlock, wave % kWavesPerBlock, lane, stage);
+    const int home = (int)(blockIdx.x % kXcds);
+    for (int qi = 0; qi < kXcds; qi++) {
+        const int x = (home + qi) % kXcds;
+        const int* off = uoff + (size_t)x * (nb + 1);
+        const int total = off[nb];
+        for (;;) {
+            int got = 0;
+            if (lane == 0) got = atomicAdd(counter + x, 1);
+            const int u = __builtin_amdgcn_readfirstlane(__shfl(got, 0));
+            if (u >= total) break;
+            const int b = __builtin_amdgcn_readfirstlane(find_item(off, nb, u));
+            const BatchItem& q = it[b];
+            const MainArgs& a = q.main;
+            const int local = u - off[b];
+            const int c = local / a.n_waves;
+            const int wave = local - c * a.n_waves;
+            const int chunk_id = c * kXcds + x;
+            if (MIXED && q.diag)
+                main_unit<MIXED, DERIV, 0, MIXED, true>(a, chunk_id, wave / kWavesPerBlock, wave % kWavesPerBlock,
+                                                        lane, stage);
+            else
+                main_unit<MIXED, DERIV, 0, false, true>(a, chunk_id, wave / kWavesPerBlock, wave % kWavesPerBlock,
+                                                        lane, stage);
+        }
     }
 }
 
@@ -4231,11 +4270,8 @@ BatchEngine::BatchEngine(const std::vector<Engine*>& engines) : eng_(engines)
         if (!e || e->device() != device_) throw Error(1, "every structure of a batch must use the same device");
     HIP_CHECK(hipSetDevice(device_));
     HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    HIP_CHECK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
     HIP_CHECK(hipEventCreate(&ev0_));
     HIP_CHECK(hipEventCreate(&ev1_));
-    HIP_CHECK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
-    HIP_CHECK(hipEventCreateWithFlags(&join_, hipEventDisableTiming));
     d_counter_.ensure(256);
 }
 
@@ -4244,9 +4280,6 @@ BatchEngine::~BatchEngine()
     drop_graph();
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
-    if (fork_) (void)hipEventDestroy(fork_);
-    if (join_) (void)hipEventDestroy(join_);
-    if (side_) (void)hipStreamDestroy(side_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -4257,13 +4290,10 @@ void BatchEngine::drop_graph()
     graph_key_.clear();
 }
 
-static const void* batch_main_kernel(bool mixed, bool derivs, bool diag)
+static const void* batch_main_kernel(bool mixed, bool derivs)
 {
-    if (mixed) {
-        if (diag) return derivs ? (const void*)k_batch_main<true, true, true> : (const void*)k_batch_main<true, false, true>;
-        return derivs ? (const void*)k_batch_main<true, true, false> : (const void*)k_batch_main<true, false, false>;
-    }
-    return derivs ? (const void*)k_batch_main<false, true, false> : (const void*)k_batch_main<false, false, false>;
+    if (mixed) return derivs ? (const void*)k_batch_main<true, true> : (const void*)k_batch_main<true, false>;
+    return derivs ? (const void*)k_batch_main<false, true> : (const void*)k_batch_main<false, false>;
 }
 
 int BatchEngine::chunk_for(Precision prec, bool derivs)
@@ -4283,9 +4313,11 @@ void BatchEngine::plan(Precision prec, bool derivs, int chunk)
     for (int b = 0; b < nb; b++) eng_[b]->batch_item(prec, derivs, chunk, &items[b]);
     std::vector<int> order(nb);
     for (int b = 0; b < nb; b++) order[b] = b;
+    // work of a structure: its atom-wave lanes times reflections times (operations + 1), the 1 for
+    // the radial factors evaluated once per (atom, reflection)
     auto work = [&](int b) {
         const BatchItem& q = items[b];
-        return (double)q.n_slots * q.n_ops * q.n_hkl;
+        return (double)q.n_slots * (q.n_ops + 1) * q.n_hkl;
     };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
     std::vector<BatchItem> sorted(nb);
@@ -4345,8 +4377,11 @@ void BatchEngine::plan(Precision prec, bool derivs, int chunk)
                 case kSpTab: n = q.n_tab; break;
                 case kSpSlot: n = q.n_slots; break;
                 case kSpRedD: n = derivs ? (long long)q.n_slots * q.nd : 0; break;
-                case kSpUnitD: n = (q.main.n_waves > 0 && q.main.diag_mask) ? (long long)q.main.n_waves * q.n_chunks : 0; break;
-                case kSpUnitG: n = (q.main.n_waves > 0 && !q.main.diag_mask) ? (long long)q.main.n_waves * q.n_chunks : 0; break;
+                default: {    // queue x = sp - kSpUnit: chunks x, x + kXcds, ...
+                    const int x = sp - kSpUnit;
+                    const long long nc = q.n_chunks > x ? (q.n_chunks - 1 - x) / kXcds + 1 : 0;
+                    n = q.main.n_waves > 0 ? (long long)q.main.n_waves * nc : 0;
+                }
             }
             acc += n;
         }
@@ -4362,7 +4397,8 @@ void BatchEngine::plan(Precision prec, bool derivs, int chunk)
     HIP_CHECK(hipMemcpyAsync(d_off_.p, off_.data(), off_.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
     last_chunk = chunk;
-    last_units = total_[kSpUnitD] + total_[kSpUnitG];
+    last_units = 0;
+    for (int x = 0; x < kXcds; x++) last_units += total_[kSpUnit + x];
 }
 
 // Radial tables depend only on the wavefunction data (Slater terms, deformation-valence exponent
@@ -4485,30 +4521,16 @@ void BatchEngine::launch(Precision prec, bool derivs)
     if (total_[kSpSlot] > 0)
         hipLaunchKernelGGL(k_batch_atom_prep, dim3(total_[kSpSlot]), dim3(64), 0, stream_, it, o(kSpSlot), nb);
     HIP_CHECK(hipGetLastError());
-    last_grid = 0;
-    // the two kernel flavours run side by side (the general-rotation one on a second stream
-    // forked from and joined back to the main one), so neither waits for the other's tail
-    const bool both = total_[kSpUnitD] > 0 && total_[kSpUnitG] > 0;
-    if (both) {
-        HIP_CHECK(hipEventRecord(fork_, stream_));
-        HIP_CHECK(hipStreamWaitEvent(side_, fork_, 0));
-    }
-    for (int flavour = 0; flavour < 2; flavour++) {
-        const int sp = flavour == 0 ? kSpUnitD : kSpUnitG;
-        if (total_[sp] == 0) continue;
-        const bool diag = flavour == 0;
-        const int grid = std::min(grid_[(diag ? 4 : 0) + (mixed ? 2 : 0) + (derivs ? 1 : 0)], total_[sp]);
-        last_grid = std::max(last_grid, grid);
-        const int nbv = nb, tot = total_[sp];
-        const int* ou = o(sp);
-        int* cnt = d_counter_.as<int>() + flavour;
-        void* args[] = {(void*)&it, (void*)&ou, (void*)&nbv, (void*)&tot, (void*)&cnt};
-        HIP_CHECK(hipLaunchKernel(batch_main_kernel(mixed, derivs, diag), dim3(grid), dim3(256), args, 0,
-                                  both && !diag ? side_ : stream_));
-    }
-    if (both) {
-        HIP_CHECK(hipEventRecord(join_, side_));
-        HIP_CHECK(hipStreamWaitEvent(stream_, join_, 0));
+    if (last_units > 0) {
+        const int grid = std::min(grid_[(mixed ? 2 : 0) + (derivs ? 1 : 0)], std::max(last_units, kXcds));
+        last_grid = grid;
+        const int* uo = o(kSpUnit);
+        const int nbv = nb;
+        int* cnt = d_counter_.as<int>();
+        void* args[] = {(void*)&it, (void*)&uo, (void*)&nbv, (void*)&cnt};
+        HIP_CHECK(hipLaunchKernel(batch_main_kernel(mixed, derivs), dim3(grid), dim3(256), args, 0, stream_));
+    } else {
+        last_grid = 0;
     }
     const int R = 64;
     if (total_[kSpHkl] > 0)
@@ -4535,17 +4557,15 @@ void BatchEngine::launch_hkl_prep()
 void BatchEngine::run(Precision prec, bool derivs, bool graph)
 {
     HIP_CHECK(hipSetDevice(device_));
-    for (int diag = 0; diag < 2; diag++) {
-        int& g = grid_[(diag ? 4 : 0) + (prec == kMixed ? 2 : 0) + (derivs ? 1 : 0)];
-        if (g == 0) {
-            // persistent grid: every workgroup the device holds at once
-            int per_cu = 0;
-            HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, batch_main_kernel(prec == kMixed, derivs, diag && prec == kMixed), 256, 0));
-            hipDeviceProp_t prop;
-            HIP_CHECK(hipGetDeviceProperties(&prop, device_));
-            g = std::max(1, per_cu) * prop.multiProcessorCount;
-        }
+    int& g = grid_[(prec == kMixed ? 2 : 0) + (derivs ? 1 : 0)];
+    if (g == 0) {
+        // persistent grid: every workgroup the device holds at once
+        int per_cu = 0;
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_main_kernel(prec == kMixed, derivs),
+                                                               256, 0));
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, device_));
+        g = std::max(1, per_cu) * prop.multiProcessorCount;
     }
     const int chunk = chunk_for(prec, derivs);
     plan(prec, derivs, chunk);
@@ -4587,7 +4607,7 @@ int BatchEngine::autotune(Precision prec, bool derivs)
     const int saved = hkl_chunk;
     int best = 64;
     float best_ms = 1e30f;
-    for (int c : {16, 24, 32, 48, 64, 96, 128, 192, 256}) {
+    for (int c : {16, 32, 64, 96, 128, 192, 256, 384, 512}) {
         hkl_chunk = c;
         float ms_min = 1e30f;
         for (int rep = 0; rep < 2; rep++) {

@@ -341,8 +341,8 @@ private:
     std::vector<unsigned long long> rec_key_, last_rec_key_;
     std::vector<Engine*> eng_;
     int device_ = 0;
-    hipStream_t stream_ = nullptr, side_ = nullptr;
-    hipEvent_t ev0_ = nullptr, ev1_ = nullptr, fork_ = nullptr, join_ = nullptr;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     DeviceBuffer d_items_, d_off_, d_counter_;
     // radial tables shared by the structures of the batch (one per distinct wavefunction)
     DeviceBuffer d_gwfn_, d_gcore_, d_gval_, d_gtabs_, d_ltabs_, d_gtab_, d_gtab32_;
@@ -353,7 +353,7 @@ private:
     std::vector<unsigned char> graph_key_; // items_ + off_ + precision at capture time
     hipGraphExec_t exec_ = nullptr;
     std::map<long long, int> tuned_;
-    int grid_[8] = {};                     // persistent main-kernel grid per (flavour, precision, derivatives)
+    int grid_[4] = {};                     // persistent main-kernel grid per (precision, derivatives)
 };
 
 }  // namespace discamb_amd

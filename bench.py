@@ -731,7 +731,7 @@ def main():
            "d_adp": rel(dmx.adp_derivatives, d64.adp_derivatives),
            "d_occ": rel(dmx.occupancy_derivatives, d64.occupancy_derivatives)}
     achieved = flops / (kmx * 1e-3)
-    kname = "k_main<mixed,deriv,diag>"    # P2_1 2_1 2_1: every operation is a diagonal sign matrix
+    kname = "k_main<mixed,deriv,line,diag222>"    # P2_1 2_1 2_1: diagonal sign matrices, hkl-line recurrences
     line = {
         "metric": "atom*symop*hkl terms/s for F+derivs (mixed precision)",
         "value": terms / smx,

@@ -333,7 +333,7 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
         dd[2 * k + 1] = im;
         dd[2 * (size_t)n_hkl + k] = m;
     }
-    c->eng().line_order = c->prec == kFp64;     // the FP64 kernels carry phases along hkl lines
+    c->eng().line_order = true;     // both precisions carry phases along hkl lines (groups of <= 4 operations)
     c->eng().upload(ea, n_hkl, hkl, dd.data(), rows);
     c->n_atoms = n;
     c->n_hkl = n_hkl;
@@ -576,7 +576,7 @@ static void set_reflections(Calculator* c, int n_hkl, const int* hkl)
         }
     }
     conv.centric_origin = c->eng_centric_origin() ? 1 : 0;
-    c->eng().line_order = c->prec == kFp64;
+    c->eng().line_order = true;
     c->eng().prepare_dev(ea, n_hkl, hkl, mult.data(), c->mult_w, anom, conv, c->prec);
     c->n_atoms = n;
     c->n_hkl = n_hkl;

@@ -1768,6 +1768,353 @@ __device__ __forceinline__ void body_mixed_diag(const MainArgs& a, int wave, int
     }
 }
 
+// ------------------------------------------------------------------ mixed precision along hkl lines
+// Reflections ordered along hkl lines (line_cont[k] = 1 when k continues the line of k - 1 by one
+// step Delta along the line axis).  The FP32 factor E_s = T_s exp(i phi_s) of each operation is
+// evaluated directly (FP64 range reduction of the phase, FP32 sin, cos and exp2) at every
+// kLineRestart-th reflection of a staging block and wherever a line starts, and carried to the
+// next reflection of the same line by E <- E (Q M), Q <- Q C: M = exp(2 pi i (Delta R_s x + Delta t_s))
+// and C = exp(-2 Delta_s^T U Delta_s) are the atom's per-operation step factors (computed in FP64,
+// rounded once), Q = T(h + Delta) / T(h) comes from the two direct exponents.  The roundings of M,
+// Q, C and of each product compound along a chain (the error grows about quadratically with its
+// length); chains of at most kLineRestart - 1 = 3 steps keep max|dF|/max|F| within about twice
+// the direct FP32 evaluation's (8 steps: about four times).  Replaces the phase reduction, two
+// sines/cosines and the exponential of three terms in four by one complex and three real products.
+static constexpr int kLineRestart = 4;
+static_assert(kStage % kLineRestart == 0 && (kLineRestart & (kLineRestart - 1)) == 0,
+              "restarts fall on staging-block boundaries");
+template <int L, bool DERIV, bool OTF, int NOPS>
+__device__ __forceinline__ void line_steps_f32(const MainArgs& a, const double* xf, const double* u64, float (&Mr)[NOPS],
+                                               float (&Mi)[NOPS], float (&Cq)[NOPS])
+{
+    const unsigned* lop = cmem(reinterpret_cast<const unsigned*>(a.line_ops));
+#pragma unroll
+    for (int s = 0; s < NOPS; s++) {
+        const unsigned* o = lop + s * 20;
+        const double ph = __builtin_amdgcn_fract(
+            fma(ld_f64(o, 2), xf[2], fma(ld_f64(o, 1), xf[1], fma(ld_f64(o, 0), xf[0], ld_f64(o, 3)))));
+        double sn, cs;
+        sincos_rev(ph, &sn, &cs);
+        Mr[s] = (float)cs;
+        Mi[s] = (float)sn;
+        double e2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; j++) e2 = fma(ld_f64(o, 4 + j), u64[j], e2);
+        Cq[s] = (float)exp_neg(2.0 * e2);
+    }
+}
+
+// Diagonal groups (compile-time sign patterns PATS as in the FP64 diagonal line kernel), with the
+// parity-class contraction of body_mixed_diag.
+template <int L, bool DERIV, bool OTF, int NOPS, int PATS>
+__device__ __forceinline__ void body_mixed_line_diag(const MainArgs& a, int wave, int lane, int chunk_id, int k0,
+                                                     int k1, const float* __restrict__ hrec,
+                                                     const float* __restrict__ rad, const unsigned* __restrict__ hd32,
+                                                     const unsigned* __restrict__ ht32,
+                                                     const unsigned* __restrict__ dtdf32, float2* __restrict__ stage,
+                                                     double* __restrict__ fpart, double* __restrict__ dpart)
+{
+    constexpr int NLM = (L + 1) * (L + 1);
+    const int slot = wave * kWave + lane;
+    const AtomRec& at = cmem(a.atoms)[slot];
+    const double x0 = at.xf[0], x1 = at.xf[1], x2 = at.xf[2];
+    float u[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) u[j] = (float)at.u[j];
+    float P[NLM];
+#pragma unroll
+    for (int j = 0; j < NLM; j++) P[j] = (float)at.plm[j];
+    const float w = (float)at.w, anr = (float)at.anom[0], ani = (float)at.anom[1];
+    const int t = at.type;
+    TabRec tr{0, 2};
+    float pv = 0, ik = 0, ikp = 0;
+    if (OTF) {
+        tr = cmem(a.tabs)[at.wfn];
+        pv = (float)at.pval;
+        ik = (float)at.inv_kappa;
+        ikp = (float)at.inv_kappa_p;
+    }
+    const float* tab = cmem(static_cast<const float*>(a.tab));
+    const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
+    const int* cont = cmem(a.line_cont);
+    float Mr[NOPS], Mi[NOPS], Cq[NOPS], Er[NOPS], Ei[NOPS], Q[NOPS], an[NOPS];
+    {
+        const double xf[3] = {x0, x1, x2};
+        double u64[6];
+#pragma unroll
+        for (int j = 0; j < 6; j++) u64[j] = at.u[j];
+        line_steps_f32<L, DERIV, OTF, NOPS>(a, xf, u64, Mr, Mi, Cq);
+    }
+#pragma unroll
+    for (int s = 0; s < NOPS; s++) {
+        Er[s] = Ei[s] = Q[s] = 0.f;
+        an[s] = a.diag_anom[(PATS >> (3 * s)) & 7] * ani;
+    }
+    double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
+    for (int kb = k0; kb < k1; kb += kStage) {
+        const int kend = min(kb + kStage, k1);
+        float fx[3] = {0, 0, 0}, fu[6] = {0, 0, 0, 0, 0, 0}, focc = 0;
+        for (int k = kb; k < kend; ++k) {
+            const float* r = hrec + (size_t)k * kHklopRec;
+            // wave-uniform: every lane is on reflection k
+            const bool along = ((k - kb) & (kLineRestart - 1)) != 0 && cont[k] != 0;
+            if (along) {
+#pragma unroll
+                for (int s = 0; s < NOPS; s++) {
+                    const float qr = Q[s] * Mr[s], qi = Q[s] * Mi[s];
+                    const float er = Er[s] * qr - Ei[s] * qi;
+                    const float ei = Er[s] * qi + Ei[s] * qr;
+                    Er[s] = er;
+                    Ei[s] = ei;
+                    Q[s] *= Cq[s];
+                }
+            } else {
+                const bool next = k + 1 < kend && ((k + 1 - kb) & (kLineRestart - 1)) != 0 && cont[k + 1] != 0;
+                const unsigned* hd = hd32 + (size_t)k * 8;
+                const double h0 = ld_f64(hd, 0), h1 = ld_f64(hd, 1), h2 = ld_f64(hd, 2);
+                const unsigned* htk = ht32 + (size_t)k * a.n_ops * 2;
+                const float D = r[25] * u[0] + r[26] * u[1] + r[27] * u[2];
+                const float O3 = r[28] * u[3], O4 = r[29] * u[4], O5 = r[30] * u[5];
+                float Dn = 0.f, O3n = 0.f, O4n = 0.f, O5n = 0.f;
+                if (next) {
+                    const float* rn = r + kHklopRec;
+                    Dn = rn[25] * u[0] + rn[26] * u[1] + rn[27] * u[2];
+                    O3n = rn[28] * u[3];
+                    O4n = rn[29] * u[4];
+                    O5n = rn[30] * u[5];
+                }
+#pragma unroll
+                for (int s = 0; s < NOPS; s++) {
+                    const double hts = ld_f64(htk, s);
+                    switch ((PATS >> (3 * s)) & 7) {
+#define MLINE_DIRECT(PT)                                                                                      \
+    case PT: {                                                                                                \
+        constexpr float fxy = float(psign<PT>(3)), fxz = float(psign<PT>(5)), fyz = float(psign<PT>(6));      \
+        const double ph = __builtin_amdgcn_fract(                                                             \
+            fma(psign<PT>(4) * h2, x2, fma(psign<PT>(2) * h1, x1, fma(psign<PT>(1) * h0, x0, hts))));         \
+        const float phf = (float)ph;                                                                          \
+        const float sn = __builtin_amdgcn_sinf(phf), cs = __builtin_amdgcn_cosf(phf);                         \
+        const float e = D + fxy * O3 + fxz * O4 + fyz * O5;                                                   \
+        const float T = __builtin_amdgcn_exp2f(-e);                                                           \
+        Er[s] = T * cs;                                                                                       \
+        Ei[s] = T * sn;                                                                                       \
+        if (next) Q[s] = __builtin_amdgcn_exp2f(e - (Dn + fxy * O3n + fxz * O4n + fyz * O5n));                \
+        break;                                                                                                \
+    }
+                        MLINE_DIRECT(0) MLINE_DIRECT(1) MLINE_DIRECT(2) MLINE_DIRECT(3)
+                        MLINE_DIRECT(4) MLINE_DIRECT(5) MLINE_DIRECT(6) MLINE_DIRECT(7)
+#undef MLINE_DIRECT
+                    }
+                }
+            }
+            float S, G0, G1, G2, G3, G4;
+            if (OTF) {
+                float G[5];
+                radial_otf<L, float>(tab, tr, (float)ld_f64(hl32, k), pv, ik, ikp, anr, S, G);
+                G0 = G[0]; G1 = G[1]; G2 = G[2]; G3 = G[3]; G4 = G[4];
+            } else {
+                const float* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
+                S = rr[0] + anr; G0 = rr[1]; G1 = rr[2]; G2 = rr[3]; G3 = rr[4]; G4 = rr[5];
+            }
+            const float cr = (float)ld_f64(dtdf32, 2 * k), ci = (float)ld_f64(dtdf32, 2 * k + 1);
+            float C[5][8];
+            class_sums<L, float>(P, r, C);
+            float sr = 0, si = 0;
+#pragma unroll
+            for (int s = 0; s < NOPS; s++) {
+                switch ((PATS >> (3 * s)) & 7) {
+#define MLINE_TERM(PT)                                                                                        \
+    case PT: {                                                                                                \
+        float A[5];                                                                                           \
+        signed_contraction<L, PT>(C, A);                                                                      \
+        float fr = S + G0 * A[0] + G2 * A[2];                                                                 \
+        float fi = an[s] + G1 * A[1];                                                                         \
+        if (L == 4) {                                                                                         \
+            fr += G4 * A[4];                                                                                  \
+            fi += G3 * A[3];                                                                                  \
+        }                                                                                                     \
+        const float Xr = fr * Er[s] - fi * Ei[s], Xi = fr * Ei[s] + fi * Er[s];                               \
+        sr += Xr;                                                                                             \
+        si += Xi;                                                                                             \
+        if (DERIV) {                                                                                          \
+            const float zr = cr * Xr - ci * Xi, zi = cr * Xi + ci * Xr;                                       \
+            fu[0] -= r[25] * zr;                                                                              \
+            fu[1] -= r[26] * zr;                                                                              \
+            fu[2] -= r[27] * zr;                                                                              \
+            fu[3] -= (float(psign<PT>(3)) * r[28]) * zr;                                                      \
+            fu[4] -= (float(psign<PT>(5)) * r[29]) * zr;                                                      \
+            fu[5] -= (float(psign<PT>(6)) * r[30]) * zr;                                                      \
+            fx[0] -= (float(psign<PT>(1)) * r[31]) * zi;                                                      \
+            fx[1] -= (float(psign<PT>(2)) * r[32]) * zi;                                                      \
+            fx[2] -= (float(psign<PT>(4)) * r[33]) * zi;                                                      \
+        }                                                                                                     \
+        break;                                                                                                \
+    }
+                    MLINE_TERM(0) MLINE_TERM(1) MLINE_TERM(2) MLINE_TERM(3)
+                    MLINE_TERM(4) MLINE_TERM(5) MLINE_TERM(6) MLINE_TERM(7)
+#undef MLINE_TERM
+                }
+            }
+            if (DERIV) focc += cr * sr - ci * si;
+            stage_f32(stage, k - kb, lane, w * sr, w * si);
+        }
+        if (DERIV) {
+#pragma unroll
+            for (int j = 0; j < 3; j++) dx[j] += fx[j];
+#pragma unroll
+            for (int j = 0; j < 6; j++) du[j] += fu[j];
+            docc += focc;
+        }
+        flush_f32(stage, kend - kb, lane, reinterpret_cast<float*>(fpart) + ((size_t)wave * a.n_hkl + kb) * 2);
+    }
+    if (DERIV) {
+        const double inv_log2e = 0.69314718055994530942;
+        double* dp = dpart + ((size_t)chunk_id * a.n_slots + slot) * kNdBasic;
+        dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
+#pragma unroll
+        for (int j = 0; j < 6; j++) dp[3 + j] = du[j] * inv_log2e;
+        dp[9] = docc;
+    }
+}
+
+// General rotations (per-(reflection, operation) records and FP64 phase records of body_mixed),
+// NOPS explicit operations.
+template <int L, bool DERIV, bool OTF, int NOPS>
+__device__ __forceinline__ void body_mixed_line(const MainArgs& a, int wave, int lane, int chunk_id, int k0, int k1,
+                                                const float* __restrict__ hop, const float* __restrict__ rad,
+                                                const unsigned* __restrict__ phs32,
+                                                const unsigned* __restrict__ dtdf32, float2* __restrict__ stage,
+                                                double* __restrict__ fpart, double* __restrict__ dpart)
+{
+    constexpr int NLM = (L + 1) * (L + 1);
+    const int slot = wave * kWave + lane;
+    const AtomRec& at = cmem(a.atoms)[slot];
+    const double x0 = at.xf[0], x1 = at.xf[1], x2 = at.xf[2];
+    float u[6];
+#pragma unroll
+    for (int j = 0; j < 6; j++) u[j] = (float)at.u[j];
+    float P[NLM];
+#pragma unroll
+    for (int j = 0; j < NLM; j++) P[j] = (float)at.plm[j];
+    const float w = (float)at.w, anr = (float)at.anom[0], ani = (float)at.anom[1];
+    const int t = at.type;
+    TabRec tr{0, 2};
+    float pv = 0, ik = 0, ikp = 0;
+    if (OTF) {
+        tr = cmem(a.tabs)[at.wfn];
+        pv = (float)at.pval;
+        ik = (float)at.inv_kappa;
+        ikp = (float)at.inv_kappa_p;
+    }
+    const float* tab = cmem(static_cast<const float*>(a.tab));
+    const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
+    const int* cont = cmem(a.line_cont);
+    float Mr[NOPS], Mi[NOPS], Cq[NOPS], Er[NOPS], Ei[NOPS], Q[NOPS];
+    {
+        const double xf[3] = {x0, x1, x2};
+        double u64[6];
+#pragma unroll
+        for (int j = 0; j < 6; j++) u64[j] = at.u[j];
+        line_steps_f32<L, DERIV, OTF, NOPS>(a, xf, u64, Mr, Mi, Cq);
+    }
+#pragma unroll
+    for (int s = 0; s < NOPS; s++) Er[s] = Ei[s] = Q[s] = 0.f;
+    double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
+    for (int kb = k0; kb < k1; kb += kStage) {
+        const int kend = min(kb + kStage, k1);
+        float fx[3] = {0, 0, 0}, fu[6] = {0, 0, 0, 0, 0, 0}, focc = 0;
+        for (int k = kb; k < kend; ++k) {
+            const bool along = ((k - kb) & (kLineRestart - 1)) != 0 && cont[k] != 0;
+            if (along) {
+#pragma unroll
+                for (int s = 0; s < NOPS; s++) {
+                    const float qr = Q[s] * Mr[s], qi = Q[s] * Mi[s];
+                    const float er = Er[s] * qr - Ei[s] * qi;
+                    const float ei = Er[s] * qi + Ei[s] * qr;
+                    Er[s] = er;
+                    Ei[s] = ei;
+                    Q[s] *= Cq[s];
+                }
+            } else {
+                const bool next = k + 1 < kend && ((k + 1 - kb) & (kLineRestart - 1)) != 0 && cont[k + 1] != 0;
+#pragma unroll
+                for (int s = 0; s < NOPS; s++) {
+                    const float* r = hop + ((size_t)k * NOPS + s) * kHklopRec;
+                    const unsigned* pr = phs32 + ((size_t)k * NOPS + s) * 8;
+                    const double ph =
+                        fma(ld_f64(pr, 2), x2, fma(ld_f64(pr, 1), x1, fma(ld_f64(pr, 0), x0, ld_f64(pr, 3))));
+                    const float phf = (float)__builtin_amdgcn_fract(ph);
+                    const float sn = __builtin_amdgcn_sinf(phf), cs = __builtin_amdgcn_cosf(phf);
+                    float e = 0.f;
+#pragma unroll
+                    for (int j = 0; j < 6; j++) e = fmaf(r[25 + j], u[j], e);
+                    const float T = __builtin_amdgcn_exp2f(-e);
+                    Er[s] = T * cs;
+                    Ei[s] = T * sn;
+                    if (next) {
+                        const float* rn = hop + ((size_t)(k + 1) * NOPS + s) * kHklopRec;
+                        float en = 0.f;
+#pragma unroll
+                        for (int j = 0; j < 6; j++) en = fmaf(rn[25 + j], u[j], en);
+                        Q[s] = __builtin_amdgcn_exp2f(e - en);
+                    }
+                }
+            }
+            float S, G0, G1, G2, G3, G4;
+            if (OTF) {
+                float G[5];
+                radial_otf<L, float>(tab, tr, (float)ld_f64(hl32, k), pv, ik, ikp, anr, S, G);
+                G0 = G[0]; G1 = G[1]; G2 = G[2]; G3 = G[3]; G4 = G[4];
+            } else {
+                const float* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
+                S = rr[0] + anr; G0 = rr[1]; G1 = rr[2]; G2 = rr[3]; G3 = rr[4]; G4 = rr[5];
+            }
+            const float cr = (float)ld_f64(dtdf32, 2 * k), ci = (float)ld_f64(dtdf32, 2 * k + 1);
+            float sr = 0, si = 0;
+#pragma unroll
+            for (int s = 0; s < NOPS; s++) {
+                const float* r = hop + ((size_t)k * NOPS + s) * kHklopRec;
+                float A[5] = {0, 0, 0, 0, 0};
+                contract<L, float>(P, r, A);
+                float fr = S + G0 * A[0] + G2 * A[2];
+                float fi = ani * r[35] + G1 * A[1];
+                if (L == 4) {
+                    fr += G4 * A[4];
+                    fi += G3 * A[3];
+                }
+                const float Xr = fr * Er[s] - fi * Ei[s], Xi = fr * Ei[s] + fi * Er[s];
+                sr += Xr;
+                si += Xi;
+                if (DERIV) {
+                    const float zr = cr * Xr - ci * Xi, zi = cr * Xi + ci * Xr;
+#pragma unroll
+                    for (int j = 0; j < 6; j++) fu[j] -= r[25 + j] * zr;
+#pragma unroll
+                    for (int j = 0; j < 3; j++) fx[j] -= r[31 + j] * zi;
+                }
+            }
+            if (DERIV) focc += cr * sr - ci * si;
+            stage_f32(stage, k - kb, lane, w * sr, w * si);
+        }
+        if (DERIV) {
+#pragma unroll
+            for (int j = 0; j < 3; j++) dx[j] += fx[j];
+#pragma unroll
+            for (int j = 0; j < 6; j++) du[j] += fu[j];
+            docc += focc;
+        }
+        flush_f32(stage, kend - kb, lane, reinterpret_cast<float*>(fpart) + ((size_t)wave * a.n_hkl + kb) * 2);
+    }
+    if (DERIV) {
+        const double inv_log2e = 0.69314718055994530942;
+        double* dp = dpart + ((size_t)chunk_id * a.n_slots + slot) * kNdBasic;
+        dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
+#pragma unroll
+        for (int j = 0; j < 6; j++) dp[3 + j] = du[j] * inv_log2e;
+        dp[9] = docc;
+    }
+}
+
 // One (atom block, reflection chunk) work unit: wave wib of the block runs atom wave
 // ablock * kWavesPerBlock + wib over the reflections of chunk chunk_id.
 template <bool MIXED, bool DERIV, int MP, bool DIAG, bool OTF, int NOPS = 0, int DPATS = 0>
@@ -1775,7 +2122,8 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
 {
     static_assert(MIXED || !DIAG, "the parity-folded kernels are mixed precision only");
     static_assert(!(MP && OTF), "the multipole adjoint uses radial records");
-    static_assert(NOPS == 0 || (!MIXED && MP == 0), "the line recurrences are FP64 without the multipole adjoint");
+    static_assert(NOPS == 0 || MP == 0, "the line recurrences run without the multipole adjoint");
+    static_assert(!(MIXED && DIAG && NOPS > 0 && DPATS == 0), "a mixed diagonal line kernel needs its sign patterns");
     const int wave = ablock * kWavesPerBlock + wib;
     if (wave >= a.n_waves) return;
     const int k0 = chunk_id * a.chunk;
@@ -1791,7 +2139,24 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
         const float* radx = cmem(static_cast<const float*>(a.radx));
         const unsigned* phs = cmem(reinterpret_cast<const unsigned*>(a.phase));
         float2* st = static_cast<float2*>(stage);
-        if (DIAG) {
+        if (NOPS > 0 && DIAG) {
+            constexpr int N = NOPS > 0 ? NOPS : 1;
+            constexpr int PS = DPATS & 4095;
+            if (cls <= 2)
+                body_mixed_line_diag<2, DERIV, OTF, N, PS>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, ht32, dtdf32,
+                                                           st, fpart, dpart);
+            else
+                body_mixed_line_diag<4, DERIV, OTF, N, PS>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, ht32, dtdf32,
+                                                           st, fpart, dpart);
+        } else if (NOPS > 0) {
+            constexpr int N = NOPS > 0 ? NOPS : 1;
+            if (cls <= 2)
+                body_mixed_line<2, DERIV, OTF, N>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, dtdf32, st, fpart,
+                                                  dpart);
+            else
+                body_mixed_line<4, DERIV, OTF, N>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, dtdf32, st, fpart,
+                                                  dpart);
+        } else if (DIAG) {
             if (cls <= 2)
                 body_mixed_diag<2, DERIV, MP, OTF>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, phs, ht32, dtdf32,
                                                    st, fpart, dpart);
@@ -1841,7 +2206,7 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
 // when only the B_lm are wanted (the chain rule through the local frames): the plain pass in the
 // multipole row layout.
 template <bool MIXED, bool DERIV, int MP, bool DIAG, bool OTF, int NOPS = 0, int DPATS = 0>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && (MP == 0 || MP == 3) ? (DIAG || OTF ? 4 : 5) : (!MIXED && MP == 2 ? 3 : 1)))) void k_main(MainArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && (MP == 0 || MP == 3) ? (DIAG || OTF || NOPS > 0 ? 4 : 5) : (!MIXED && MP == 2 ? 3 : 1)))) void k_main(MainArgs a)
 {
     constexpr int kStageBytes = MIXED ? kStageBytesF32 : kStageBytesF64;
     __shared__ __align__(16) unsigned char stage_lds[kWavesPerBlock * kStageBytes];
@@ -3182,28 +3547,33 @@ static const void* main_kernel(bool mixed, bool derivs, int mp_pass, bool diag, 
                                int diag64 = 0)
 {
 #define KM(M, D, P, G, O) (const void*)k_main<M, D, P, G, O>
-#define KL(D, O, N) (const void*)k_main<false, D, 0, false, O, N>
-#define KD(N, C) (derivs ? (otf ? (const void*)k_main<false, true, 0, false, true, N, C>                              \
-                             : (const void*)k_main<false, true, 0, false, false, N, C>)                             \
-                      : (otf ? (const void*)k_main<false, false, 0, false, true, N, C>                              \
-                             : (const void*)k_main<false, false, 0, false, false, N, C>))
-    if (diag64 > 0 && !mixed && mp_pass == 0) {
+#define KL(M, N) (derivs ? (otf ? (const void*)k_main<M, true, 0, false, true, N>                                   \
+                             : (const void*)k_main<M, true, 0, false, false, N>)                                  \
+                      : (otf ? (const void*)k_main<M, false, 0, false, true, N>                                   \
+                             : (const void*)k_main<M, false, 0, false, false, N>))
+#define KD(M, N, C) (derivs ? (otf ? (const void*)k_main<M, true, 0, M, true, N, C>                                 \
+                                : (const void*)k_main<M, true, 0, M, false, N, C>)                                \
+                         : (otf ? (const void*)k_main<M, false, 0, M, true, N, C>                                 \
+                                : (const void*)k_main<M, false, 0, M, false, N, C>))
+    // line kernels: FP64 diagonal groups take per-reflection FP64 records, mixed diagonal groups the
+    // parity-folded FP32 records; other groups take the general records of their precision
+    if (diag64 > 0 && mp_pass == 0 && (!mixed || diag)) {
         switch (diag64) {
-        case kDiag64Pats0: return KD(2, kDiag64Pats0);
-        case kDiag64Pats1: return KD(2, kDiag64Pats1);
-        case kDiag64Pats2: return KD(2, kDiag64Pats2);
-        case kDiag64Pats3: return KD(2, kDiag64Pats3);
-        case kDiag64Pats4: return KD(4, kDiag64Pats4);
+        case kDiag64Pats0: return mixed ? KD(true, 2, kDiag64Pats0) : KD(false, 2, kDiag64Pats0);
+        case kDiag64Pats1: return mixed ? KD(true, 2, kDiag64Pats1) : KD(false, 2, kDiag64Pats1);
+        case kDiag64Pats2: return mixed ? KD(true, 2, kDiag64Pats2) : KD(false, 2, kDiag64Pats2);
+        case kDiag64Pats3: return mixed ? KD(true, 2, kDiag64Pats3) : KD(false, 2, kDiag64Pats3);
+        case kDiag64Pats4: return mixed ? KD(true, 4, kDiag64Pats4) : KD(false, 4, kDiag64Pats4);
         default: break;
         }
     }
 #undef KD
-    if (line_ops > 0 && !mixed && mp_pass == 0) {
+    if (line_ops > 0 && mp_pass == 0 && !(mixed && diag)) {
         switch (line_ops) {
-        case 1: return derivs ? (otf ? KL(true, true, 1) : KL(true, false, 1)) : (otf ? KL(false, true, 1) : KL(false, false, 1));
-        case 2: return derivs ? (otf ? KL(true, true, 2) : KL(true, false, 2)) : (otf ? KL(false, true, 2) : KL(false, false, 2));
-        case 3: return derivs ? (otf ? KL(true, true, 3) : KL(true, false, 3)) : (otf ? KL(false, true, 3) : KL(false, false, 3));
-        case 4: return derivs ? (otf ? KL(true, true, 4) : KL(true, false, 4)) : (otf ? KL(false, true, 4) : KL(false, false, 4));
+        case 1: return mixed ? KL(true, 1) : KL(false, 1);
+        case 2: return mixed ? KL(true, 2) : KL(false, 2);
+        case 3: return mixed ? KL(true, 3) : KL(false, 3);
+        case 4: return mixed ? KL(true, 4) : KL(false, 4);
         default: break;
         }
     }

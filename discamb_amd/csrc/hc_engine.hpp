@@ -272,7 +272,9 @@ private:
     void line_sort(int n_hkl, const int* hkl, std::vector<std::pair<double, int>>& key);
     bool use_line(bool mixed, bool mp) const
     {
-        return !mixed && !mp && !tune.direct_phases && last_order_ == 2 && line_ops_ok();
+        if (mp || tune.direct_phases || last_order_ != 2 || !line_ops_ok()) return false;
+        // the mixed parity-folded line kernels exist for the compiled sign-pattern sets only
+        return !use_diag(mixed, mp) || diag64_code() != 0;
     }
     // the FP64 line kernel with per-reflection records for diagonal groups: its sign-pattern code, 0
     // when the operations' patterns are not one of the compiled sets

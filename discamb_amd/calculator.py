@@ -229,8 +229,8 @@ class HansenCoppensStructureFactorCalculator:
         are all diagonal sign matrices (those otherwise use the parity-folded kernels).
         radial: "records" (per-(reflection, type) tables), "tables" (per-(atom, reflection)
         lookups in per-wavefunction tables) or "auto".
-        direct_phases: FP64 terms evaluate every phase and Debye-Waller factor directly instead of
-        carrying them along hkl lines."""
+        direct_phases: every phase and Debye-Waller factor is evaluated directly instead of carried
+        along hkl lines (reflections are then ordered by |h|); takes effect at the next upload."""
         flags = (1 if general_rotations else 0) | {"auto": 0, "records": 2, "tables": 4}[radial] | \
             (8 if direct_phases else 0)
         _lib.check(self._lib.dh_hc_set_options(self._h, flags))

@@ -3292,8 +3292,6 @@ void Engine::line_sort(int n_hkl, const int* hkl, std::vector<std::pair<double, 
 void Engine::upload_checked(const EngineAtoms& at, int n_hkl, const int* hkl, const double* dtdf, bool rows_layout)
 {
     dev_ready_ = false;
-    // reflection order: caller's (rows), along hkl lines (FP64 line kernels) or by |h|
-    const int order_mode = rows_layout ? 0 : (line_order && line_ops_ok() ? 2 : 1);
     n_atoms_ = at.n;
     n_hkl_ = n_hkl;
     rows_layout_ = rows_layout;
@@ -3358,6 +3356,12 @@ void Engine::upload_checked(const EngineAtoms& at, int n_hkl, const int* hkl, co
         }
     }
     n_sorted_ = (int)order_.size();
+    // reflection order: caller's (rows); along hkl lines when the line kernels apply and the
+    // radial factors come from per-(reflection, type) records; by |h| otherwise, so that per-atom
+    // table lookups of neighbouring reflections fall on the same table nodes (along a line |h|
+    // jumps by a whole reflection step and the lookups of a chunk spread over the whole table)
+    const int order_mode =
+        rows_layout ? 0 : (line_order && line_ops_ok() && !tune.direct_phases && !use_otf(false) ? 2 : 1);
     n_included_ = (int)idx.size();
     // page-locked staging: [atoms | wave classes | dT/dF and multipliers | sorted reflections]
     const size_t n_ain = std::max(n_sorted_, 1);
@@ -3547,17 +3551,12 @@ static const void* main_kernel(bool mixed, bool derivs, int mp_pass, bool diag, 
                                int diag64 = 0)
 {
 #define KM(M, D, P, G, O) (const void*)k_main<M, D, P, G, O>
-#define KL(M, N) (derivs ? (otf ? (const void*)k_main<M, true, 0, false, true, N>                                   \
-                             : (const void*)k_main<M, true, 0, false, false, N>)                                  \
-                      : (otf ? (const void*)k_main<M, false, 0, false, true, N>                                   \
-                             : (const void*)k_main<M, false, 0, false, false, N>))
-#define KD(M, N, C) (derivs ? (otf ? (const void*)k_main<M, true, 0, M, true, N, C>                                 \
-                                : (const void*)k_main<M, true, 0, M, false, N, C>)                                \
-                         : (otf ? (const void*)k_main<M, false, 0, M, true, N, C>                                 \
-                                : (const void*)k_main<M, false, 0, M, false, N, C>))
-    // line kernels: FP64 diagonal groups take per-reflection FP64 records, mixed diagonal groups the
+#define KL(M, N) (derivs ? (const void*)k_main<M, true, 0, false, false, N> : (const void*)k_main<M, false, 0, false, false, N>)
+#define KD(M, N, C) (derivs ? (const void*)k_main<M, true, 0, M, false, N, C> : (const void*)k_main<M, false, 0, M, false, N, C>)
+    // line kernels (radial records only: reflections are ordered along lines only on that path):
+    // FP64 diagonal groups take per-reflection FP64 records, mixed diagonal groups the
     // parity-folded FP32 records; other groups take the general records of their precision
-    if (diag64 > 0 && mp_pass == 0 && (!mixed || diag)) {
+    if (diag64 > 0 && mp_pass == 0 && !otf && (!mixed || diag)) {
         switch (diag64) {
         case kDiag64Pats0: return mixed ? KD(true, 2, kDiag64Pats0) : KD(false, 2, kDiag64Pats0);
         case kDiag64Pats1: return mixed ? KD(true, 2, kDiag64Pats1) : KD(false, 2, kDiag64Pats1);
@@ -3568,7 +3567,7 @@ static const void* main_kernel(bool mixed, bool derivs, int mp_pass, bool diag, 
         }
     }
 #undef KD
-    if (line_ops > 0 && mp_pass == 0 && !(mixed && diag)) {
+    if (line_ops > 0 && mp_pass == 0 && !otf && !(mixed && diag)) {
         switch (line_ops) {
         case 1: return mixed ? KL(true, 1) : KL(false, 1);
         case 2: return mixed ? KL(true, 2) : KL(false, 2);

@@ -272,7 +272,7 @@ private:
     void line_sort(int n_hkl, const int* hkl, std::vector<std::pair<double, int>>& key);
     bool use_line(bool mixed, bool mp) const
     {
-        if (mp || tune.direct_phases || last_order_ != 2 || !line_ops_ok()) return false;
+        if (mp || tune.direct_phases || last_order_ != 2 || !line_ops_ok() || use_otf(mp)) return false;
         // the mixed parity-folded line kernels exist for the compiled sign-pattern sets only
         return !use_diag(mixed, mp) || diag64_code() != 0;
     }

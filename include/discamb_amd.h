@@ -149,9 +149,10 @@ int dh_hc_set_tuning(dh_calculator* calc, int hkl_chunk);
  * engine takes tables when atom types are not shared by at least eight atoms on average. */
 #define DH_OPT_RADIAL_RECORDS 2u
 #define DH_OPT_RADIAL_TABLES 4u
-/* DH_OPT_DIRECT_PHASES: FP64 terms evaluate every phase factor and Debye-Waller factor directly
- * (by default groups with at most four explicit operations carry them along hkl lines, as the
- * reference's Engine3 does, scattering_utilities.cpp:520-756). */
+/* DH_OPT_DIRECT_PHASES: every term evaluates its phase factor and Debye-Waller factor directly,
+ * with reflections ordered by |h| (by default groups with at most four explicit operations order
+ * reflections along hkl lines and carry both factors along them, as the reference's Engine3 does,
+ * scattering_utilities.cpp:520-756); takes effect at the next upload. */
 #define DH_OPT_DIRECT_PHASES 8u
 int dh_hc_set_options(dh_calculator* calc, unsigned flags);
 

@@ -302,13 +302,16 @@ def test_failed_upload_leaves_no_stale_inputs():
     assert np.array_equal(d0.adp_derivatives, d1.adp_derivatives)
 
 
-@pytest.mark.parametrize("radial", ["auto", "records"])
+@pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
 @pytest.mark.parametrize("sg", ["P1", "P-1", "P21/c", "C2/c", "P212121", "P31", "P-1_shifted", "I222"])
-def test_fp64_line_recurrences_vs_direct_and_oracle(sg, radial):
-    """FP64 terms carried along hkl lines (groups with <= 4 explicit operations) against direct
-    evaluation of every phase and Debye-Waller factor (1e-12) and against the oracle (1e-10), with
+def test_line_recurrences_vs_direct_and_oracle(sg, mode):
+    """Terms carried along hkl lines (groups with <= 4 explicit operations, radial records) against
+    direct evaluation of every phase and Debye-Waller factor and against the oracle, with
     reflection lists that break lines (a random subset) and a short reflection chunk (many chunk
-    starts)."""
+    starts).  FP64: line vs direct 1e-12, oracle 1e-10.  Mixed precision (FP32 products along
+    the lines, direct evaluation every fourth reflection): both against the oracle at 1e-6 (F) and
+    1e-5 (derivatives), and the line path no further from the oracle than twice the direct path's
+    error plus 2e-7."""
     base = sg.split("_")[0]
     w = synth.small_structure(base if base != "I222" else "P212121", n_atoms=20, d_min=0.8, seed=23)
     if sg == "P-1_shifted":
@@ -320,21 +323,35 @@ def test_fp64_line_recurrences_vs_direct_and_oracle(sg, radial):
     rng = np.random.default_rng(31)
     sub = np.sort(rng.choice(len(w.hkl), int(0.8 * len(w.hkl)), replace=False))
     for hkl, g, chunk in ((w.hkl, w.dtdf, 0), (w.hkl[sub], w.dtdf[sub], 0), (w.hkl, w.dtdf, 13)):
-        print("case line recurrences %s radial=%s hkl=%d chunk=%d" % (sg, radial, len(hkl), chunk), flush=True)
+        print("case line recurrences %s mode=%s hkl=%d chunk=%d" % (sg, mode, len(hkl), chunk), flush=True)
         out = []
         for direct in (False, True):
             calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
-            calc.setKernelOptions(radial=radial, direct_phases=direct)
+            calc.setCalculationMode(mode)
+            calc.setKernelOptions(radial="records", direct_phases=direct)
             calc.setTuning(chunk)
             out.append(calc.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, hkl, g))
-        (f, d), (fd, dd) = out
-        e = (rel_err(f, fd), rel_err(d.atomic_position_derivatives, dd.atomic_position_derivatives),
-             rel_err(d.adp_derivatives, dd.adp_derivatives), rel_err(d.occupancy_derivatives, dd.occupancy_derivatives))
-        print("  line vs direct", e, flush=True)
-        assert max(e) <= 1e-12, e
         rf, rx, ru, ro = hc_oracle.sf_and_derivs(w.crystal, w.params, w.crystal.atoms, w.lcs, hkl, g, n_threads=16)
-        assert rel_err(f, rf) <= 1e-10 and rel_err(d.atomic_position_derivatives, rx) <= 1e-10
-        assert rel_err(d.adp_derivatives, ru) <= 1e-10 and rel_err(d.occupancy_derivatives, ro) <= 1e-10
+
+        def errs(fd):
+            f, d = fd
+            return (rel_err(f, rf), rel_err(d.atomic_position_derivatives, rx), rel_err(d.adp_derivatives, ru),
+                    rel_err(d.occupancy_derivatives, ro))
+        (f, d), (fd, dd) = out
+        el, ed = errs(out[0]), errs(out[1])
+        print("  vs oracle: line", el, "direct", ed, flush=True)
+        if mode == "HIP":
+            e = (rel_err(f, fd), rel_err(d.atomic_position_derivatives, dd.atomic_position_derivatives),
+                 rel_err(d.adp_derivatives, dd.adp_derivatives),
+                 rel_err(d.occupancy_derivatives, dd.occupancy_derivatives))
+            print("  line vs direct", e, flush=True)
+            assert max(e) <= 1e-12, e
+            assert max(el) <= 1e-10, el
+        else:
+            for e in (el, ed):
+                assert e[0] <= 1e-6 and max(e[1:]) <= 1e-5, e
+            for a, b in zip(el, ed):
+                assert a <= 2 * b + 2e-7, (el, ed)
 
 
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])

@@ -94,6 +94,9 @@ struct Calculator {
     std::vector<LcsDef> lcs_defs;
     bool lcs_chain = false;
     std::vector<double> last_lcs, last_xyz_frac;
+    // host staging of upload(), reused between calls
+    EngineAtoms up_atoms;
+    std::vector<double> up_dtdf;
     Engine& eng() { return iam ? *iam_engine : *engine; }
     bool eng_centric_origin() const { return !iam && centric_origin; }
     // state of the last upload
@@ -264,11 +267,13 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
                     "structure factors.");
     c->rows_uploaded = false;
     const double two_pi2 = 2.0 * M_PI * M_PI;
-    EngineAtoms ea;
+    // kept in the calculator between calls: every entry is written below, so a call with the
+    // sizes of the last one allocates and zeroes nothing
+    EngineAtoms& ea = c->up_atoms;
     ea.n = n;
     ea.xyz_frac.resize(3 * n);
-    ea.u6.assign(6 * n, 0.0);
-    ea.iso.assign(n, 0);
+    ea.u6.resize(6 * n);
+    ea.iso.resize(n);
     ea.weight.resize(n);
     ea.anom.resize(2 * n);
     ea.lcs.resize(9 * n);
@@ -288,6 +293,8 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
         if (na != 0 && na != 1 && na != 6) throw Error(DH_E_ARGUMENT, "ADP must have 0, 1 or 6 components");
         c->adp_size[i] = na;
         const double* u = at->adp + 6 * i;
+        ea.iso[i] = 0;
+        for (int j = 0; j < 6; j++) ea.u6[6 * i + j] = 0.0;
         if (na == 6) {
             double uc[6];
             if (c->adp_conv == DH_ADP_U_CART) std::memcpy(uc, u, sizeof(uc));
@@ -320,7 +327,8 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
     c->last_lcs = ea.lcs;
     c->last_xyz_frac = ea.xyz_frac;
     // centering multipliers, dT/dF <- conj(dT/dF) * mult  (.cpp:393-404)
-    std::vector<double> dd(3 * (size_t)n_hkl);
+    std::vector<double>& dd = c->up_dtdf;
+    dd.resize(3 * (size_t)n_hkl);
     for (int k = 0; k < n_hkl; k++) {
         double m = centering_multiplier(c->sg.centering, c->sg.obverse, hkl + 3 * k);
         double re = rows ? 0.0 : dtdf[2 * k] * m, im = rows ? 0.0 : -dtdf[2 * k + 1] * m;

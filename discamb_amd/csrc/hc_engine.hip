@@ -3113,6 +3113,7 @@ Engine::Engine(int device, const EngineModel& model, const double* cell6, const 
     HIP_CHECK(hipStreamCreate(&stream_));
     HIP_CHECK(hipEventCreate(&ev0_));
     HIP_CHECK(hipEventCreate(&ev1_));
+    HIP_CHECK(hipEventCreateWithFlags(&ev_up_, hipEventDisableTiming));
     Cell c;
     c.set(cell6);
     std::memcpy(f2c_, c.f2c, sizeof(f2c_));
@@ -3187,6 +3188,7 @@ Engine::~Engine()
 {
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
+    if (ev_up_) (void)hipEventDestroy(ev_up_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -3291,17 +3293,32 @@ void Engine::line_sort(int n_hkl, const int* hkl, std::vector<std::pair<double, 
 
 void Engine::upload_checked(const EngineAtoms& at, int n_hkl, const int* hkl, const double* dtdf, bool rows_layout)
 {
+    // the copies of the previous upload read the page-locked staging buffer: they must be done
+    // before it is rewritten (or reallocated)
+    if (up_pending_) {
+        HIP_CHECK(hipEventSynchronize(ev_up_));
+        up_pending_ = false;
+    }
     dev_ready_ = false;
     n_atoms_ = at.n;
     n_hkl_ = n_hkl;
     rows_layout_ = rows_layout;
+    // The atom order depends only on the atoms' types, the include mask and the layout: a
+    // refinement's calls keep the order of the first.
+    const bool same_atoms = !order_.empty() && rows_layout == last_rows_layout_ &&
+                            at.type == last_atom_type_ && at.include == last_include_;
+    if (!same_atoms) {
+    last_rows_layout_ = rows_layout;
+    last_atom_type_ = at.type;
+    last_include_ = at.include;
     // sort included atoms by multipole class, then type; pad each class to whole waves
     std::vector<int> idx;
     for (int i = 0; i < at.n; i++)
         if (at.include[i]) idx.push_back(i);
     auto cls_of = [&](int i) { return type_class(at.type[i]); };
     order_.clear();
-    std::vector<int> wave_cls;
+    std::vector<int>& wave_cls = wave_cls_;
+    wave_cls.clear();
     if (rows_layout) {
         // caller order (included atoms), so that a wave's 64 atoms are adjacent in every
         // per-reflection row and its stores are contiguous; a wave takes the larger multipole
@@ -3355,6 +3372,9 @@ void Engine::upload_checked(const EngineAtoms& at, int n_hkl, const int* hkl, co
             }
         }
     }
+    n_included_ = (int)idx.size();
+    }
+    const std::vector<int>& wave_cls = wave_cls_;
     n_sorted_ = (int)order_.size();
     // reflection order: caller's (rows); along hkl lines when the line kernels apply and the
     // radial factors come from per-(reflection, type) records; by |h| otherwise, so that per-atom
@@ -3362,7 +3382,6 @@ void Engine::upload_checked(const EngineAtoms& at, int n_hkl, const int* hkl, co
     // jumps by a whole reflection step and the lookups of a chunk spread over the whole table)
     const int order_mode =
         rows_layout ? 0 : (line_order && line_ops_ok() && !tune.direct_phases && !use_otf(false) ? 2 : 1);
-    n_included_ = (int)idx.size();
     // page-locked staging: [atoms | wave classes | dT/dF and multipliers | sorted reflections]
     const size_t n_ain = std::max(n_sorted_, 1);
     const size_t off_cls = n_ain * sizeof(AtomIn);
@@ -3440,8 +3459,9 @@ void Engine::upload_checked(const EngineAtoms& at, int n_hkl, const int* hkl, co
     if (n_hkl > 0)
         HIP_CHECK(hipMemcpyAsync(d_dtdf_.p, dtdf_s, (size_t)n_hkl * 3 * sizeof(double), hipMemcpyHostToDevice,
                                  stream_));
-    // keep host staging alive until the copies complete
-    HIP_CHECK(hipStreamSynchronize(stream_));
+    // the copies complete on the stream, ahead of the kernels of run(); the next upload waits
+    HIP_CHECK(hipEventRecord(ev_up_, stream_));
+    up_pending_ = true;
 }
 
 __global__ void k_fill(double* __restrict__ p, size_t n, double v)

@@ -245,6 +245,12 @@ private:
     int device_;
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    hipEvent_t ev_up_ = nullptr;    // recorded after the copies of the last upload
+    bool up_pending_ = false;       // ... which may still read the staging buffer
+    // inputs of the atom order of the last upload (unchanged: the order is kept)
+    bool last_rows_layout_ = false;
+    std::vector<int> last_atom_type_, wave_cls_;
+    std::vector<unsigned char> last_include_;
     EngineModel model_;
     std::vector<EngineOp> ops_;
     bool centric_origin_;

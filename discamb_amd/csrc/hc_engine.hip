@@ -21,6 +21,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <type_traits>
 
 #include "../../include/discamb_amd.h"
 #include "hc_device_math.hpp"
@@ -1380,6 +1381,92 @@ __device__ __forceinline__ double dw_signed(const double* m, const double* u)
     return e;
 }
 
+// Form factors of every operation of a diagonal sign-pattern set at once.  Under operation s
+// the harmonics of parity class c change sign when odd_under(c, pattern s); the classes with the
+// same signs under every operation (the same sign key) enter every f_s together, so
+//   f_s = S + sum_l G_l A_l(s) = sum_k sign_s(k) B[k],  B[k] = sum_l G_l sum_{j in l, key(j) = k} P_j d_j
+// (real parts from even l and S, imaginary parts from odd l): one pass over the harmonics and
+// the radial factors per reflection, then NOPS signed sums of the few keys, instead of a signed
+// contraction and an assembly per operation.
+template <int NOPS, int PATS>
+__host__ __device__ constexpr int sign_key(int c)
+{
+    int k = 0;
+    for (int s = 0; s < NOPS; s++)
+        if (odd_under(c, (PATS >> (3 * s)) & 7)) k |= 1 << s;
+    return k;
+}
+
+template <int L, int NOPS, int PATS>
+__host__ __device__ constexpr bool key_bin_used(int l, int k)
+{
+    if (l > L) return false;
+    for (int j = l * l; j < (l + 1) * (l + 1); j++)
+        if (sign_key<NOPS, PATS>(harmonic_parity(j)) == k) return true;
+    return false;
+}
+
+template <int L, int NOPS, int PATS>
+__host__ __device__ constexpr bool key_used(int k, bool odd)
+{
+    for (int l = odd ? 1 : 0; l <= L; l += 2)
+        if (key_bin_used<L, NOPS, PATS>(l, k)) return true;
+    return !odd && k == 0;    // S enters the all-plus key
+}
+
+// Compile-time loop: f(std::integral_constant<int, i>) for i = B .. E - 1, so that tables indexed
+// by i (harmonic parity, sign keys) are evaluated by the compiler, not at run time.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+template <int L, int NOPS, int PATS, typename R>
+__device__ __forceinline__ void keyed_form_factors(const R* P, const R* Y, R S, R G0, R G1, R G2, R G3, R G4,
+                                                   R (&fr)[NOPS], R (&fi)[NOPS])
+{
+    constexpr int NK = 1 << NOPS;
+    constexpr int NLM = (L + 1) * (L + 1);
+    R C[5][NK];
+    static_for<0, 5>([&](auto Li) {
+        static_for<0, NK>([&](auto Ki) { C[decltype(Li)::value][decltype(Ki)::value] = R(0); });
+    });
+    static_for<0, NLM>([&](auto Ji) {
+        constexpr int j = decltype(Ji)::value;
+        constexpr int l = harmonic_l(j), k = sign_key<NOPS, PATS>(harmonic_parity(j));
+        C[l][k] = fma(P[j], Y[j], C[l][k]);
+    });
+    const R G[5] = {G0, G1, G2, G3, G4};
+    R Br[NK], Bi[NK];
+    static_for<0, NK>([&](auto Ki) {
+        constexpr int k = decltype(Ki)::value;
+        Br[k] = k == 0 ? S : R(0);
+        Bi[k] = R(0);
+        static_for<0, L + 1>([&](auto Li) {
+            constexpr int l = decltype(Li)::value;
+            if constexpr (key_bin_used<L, NOPS, PATS>(l, k)) {
+                if constexpr (l % 2 == 0) Br[k] = fma(G[l], C[l][k], Br[k]);
+                else Bi[k] = fma(G[l], C[l][k], Bi[k]);
+            }
+        });
+    });
+    static_for<0, NOPS>([&](auto Si) {
+        constexpr int s = decltype(Si)::value;
+        R a = R(0), b = R(0);
+        static_for<0, NK>([&](auto Ki) {
+            constexpr int k = decltype(Ki)::value;
+            if constexpr (key_used<L, NOPS, PATS>(k, false)) a = ((k >> s) & 1) ? a - Br[k] : a + Br[k];
+            if constexpr (key_used<L, NOPS, PATS>(k, true)) b = ((k >> s) & 1) ? b - Bi[k] : b + Bi[k];
+        });
+        fr[s] = a;
+        fi[s] = b;
+    });
+}
+
 template <int L, bool DERIV, bool OTF, int NOPS, int PATS>
 __device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave, int lane, int chunk_id, int k0,
                                                     int k1, const unsigned* __restrict__ hop,
@@ -1917,22 +2004,15 @@ __device__ __forceinline__ void body_mixed_line_diag(const MainArgs& a, int wave
                 S = rr[0] + anr; G0 = rr[1]; G1 = rr[2]; G2 = rr[3]; G3 = rr[4]; G4 = rr[5];
             }
             const float cr = (float)ld_f64(dtdf32, 2 * k), ci = (float)ld_f64(dtdf32, 2 * k + 1);
-            float C[5][8];
-            class_sums<L, float>(P, r, C);
+            float frs[NOPS], fis[NOPS];
+            keyed_form_factors<L, NOPS, PATS, float>(P, r, S, G0, G1, G2, G3, G4, frs, fis);
             float sr = 0, si = 0;
 #pragma unroll
             for (int s = 0; s < NOPS; s++) {
                 switch ((PATS >> (3 * s)) & 7) {
 #define MLINE_TERM(PT)                                                                                        \
     case PT: {                                                                                                \
-        float A[5];                                                                                           \
-        signed_contraction<L, PT>(C, A);                                                                      \
-        float fr = S + G0 * A[0] + G2 * A[2];                                                                 \
-        float fi = an[s] + G1 * A[1];                                                                         \
-        if (L == 4) {                                                                                         \
-            fr += G4 * A[4];                                                                                  \
-            fi += G3 * A[3];                                                                                  \
-        }                                                                                                     \
+        const float fr = frs[s], fi = an[s] + fis[s];                                                         \
         const float Xr = fr * Er[s] - fi * Ei[s], Xi = fr * Ei[s] + fi * Er[s];                               \
         sr += Xr;                                                                                             \
         si += Xi;                                                                                             \

@@ -1467,6 +1467,19 @@ __device__ __forceinline__ void keyed_form_factors(const R* P, const R* Y, R S, 
     });
 }
 
+// sum_s sign_s x_s with the signs of parity class C under the operations of PATS
+template <int NOPS, int PATS, int C, typename R>
+__device__ __forceinline__ R class_signed_sum(const R (&x)[NOPS])
+{
+    R a = R(0);
+    static_for<0, NOPS>([&](auto Si) {
+        constexpr int s = decltype(Si)::value;
+        if constexpr (odd_under(C, (PATS >> (3 * s)) & 7)) a -= x[s];
+        else a += x[s];
+    });
+    return a;
+}
+
 template <int L, bool DERIV, bool OTF, int NOPS, int PATS>
 __device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave, int lane, int chunk_id, int k0,
                                                     int k1, const unsigned* __restrict__ hop,
@@ -2006,36 +2019,34 @@ __device__ __forceinline__ void body_mixed_line_diag(const MainArgs& a, int wave
             const float cr = (float)ld_f64(dtdf32, 2 * k), ci = (float)ld_f64(dtdf32, 2 * k + 1);
             float frs[NOPS], fis[NOPS];
             keyed_form_factors<L, NOPS, PATS, float>(P, r, S, G0, G1, G2, G3, G4, frs, fis);
-            float sr = 0, si = 0;
+            // X_s = f_s E_s; the sums over operations that F and the derivatives need are the
+            // terms signed by a parity class (the ADP cross terms xy, xz, yz and the x, y, z
+            // components change sign with the operation like the harmonics of that class), so
+            // they are formed once per reflection and multiplied by the adjoint c once
+            float Xr[NOPS], Xi[NOPS];
 #pragma unroll
             for (int s = 0; s < NOPS; s++) {
-                switch ((PATS >> (3 * s)) & 7) {
-#define MLINE_TERM(PT)                                                                                        \
-    case PT: {                                                                                                \
-        const float fr = frs[s], fi = an[s] + fis[s];                                                         \
-        const float Xr = fr * Er[s] - fi * Ei[s], Xi = fr * Ei[s] + fi * Er[s];                               \
-        sr += Xr;                                                                                             \
-        si += Xi;                                                                                             \
-        if (DERIV) {                                                                                          \
-            const float zr = cr * Xr - ci * Xi, zi = cr * Xi + ci * Xr;                                       \
-            fu[0] -= r[25] * zr;                                                                              \
-            fu[1] -= r[26] * zr;                                                                              \
-            fu[2] -= r[27] * zr;                                                                              \
-            fu[3] -= (float(psign<PT>(3)) * r[28]) * zr;                                                      \
-            fu[4] -= (float(psign<PT>(5)) * r[29]) * zr;                                                      \
-            fu[5] -= (float(psign<PT>(6)) * r[30]) * zr;                                                      \
-            fx[0] -= (float(psign<PT>(1)) * r[31]) * zi;                                                      \
-            fx[1] -= (float(psign<PT>(2)) * r[32]) * zi;                                                      \
-            fx[2] -= (float(psign<PT>(4)) * r[33]) * zi;                                                      \
-        }                                                                                                     \
-        break;                                                                                                \
-    }
-                    MLINE_TERM(0) MLINE_TERM(1) MLINE_TERM(2) MLINE_TERM(3)
-                    MLINE_TERM(4) MLINE_TERM(5) MLINE_TERM(6) MLINE_TERM(7)
-#undef MLINE_TERM
-                }
+                const float fr = frs[s], fi = an[s] + fis[s];
+                Xr[s] = fr * Er[s] - fi * Ei[s];
+                Xi[s] = fr * Ei[s] + fi * Er[s];
             }
-            if (DERIV) focc += cr * sr - ci * si;
+            const float sr = class_signed_sum<NOPS, PATS, 0>(Xr), si = class_signed_sum<NOPS, PATS, 0>(Xi);
+            if (DERIV) {
+                const float z0 = cr * sr - ci * si;
+                focc += z0;
+                fu[0] -= r[25] * z0;
+                fu[1] -= r[26] * z0;
+                fu[2] -= r[27] * z0;
+                static_for<0, 3>([&](auto Ji) {
+                    constexpr int j = decltype(Ji)::value;
+                    constexpr int cu = j == 0 ? 3 : j == 1 ? 5 : 6;    // xy, xz, yz
+                    const float wr = class_signed_sum<NOPS, PATS, cu>(Xr), wi = class_signed_sum<NOPS, PATS, cu>(Xi);
+                    fu[3 + j] -= r[28 + j] * (cr * wr - ci * wi);
+                    constexpr int cx = 1 << j;                          // x, y, z
+                    const float vr = class_signed_sum<NOPS, PATS, cx>(Xr), vi = class_signed_sum<NOPS, PATS, cx>(Xi);
+                    fx[j] -= r[31 + j] * (cr * vi + ci * vr);
+                });
+            }
             stage_f32(stage, k - kb, lane, w * sr, w * si);
         }
         if (DERIV) {

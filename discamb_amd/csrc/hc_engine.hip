@@ -1591,45 +1591,47 @@ __device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave,
         double Y[25];
 #pragma unroll
         for (int j = 0; j < 25; j++) Y[j] = j < NLM ? ld_f64(rw, j) : 0.0;
-        double sr = 0, si = 0;
+        double Xr[NOPS], Xi[NOPS];
 #pragma unroll
         for (int s = 0; s < NOPS; s++) {
             double A[5] = {0, 0, 0, 0, 0};
-            double zr = 0, zi = 0;
             switch ((PATS >> (3 * s)) & 7) {
 #define DIAG64_TERM(PT)                                                                                       \
     case PT: {                                                                                                \
         contract_signed<L, PT>(P, Y, A);                                                                      \
-        double fr = S + G0 * A[0] + G2 * A[2];                                                                \
-        double fi = an[s] + G1 * A[1];                                                                        \
-        if (L == 4) {                                                                                         \
-            fr += G4 * A[4];                                                                                  \
-            fi += G3 * A[3];                                                                                  \
-        }                                                                                                     \
-        const double Xr = fr * Er[s] - fi * Ei[s], Xi = fr * Ei[s] + fi * Er[s];                              \
-        sr += Xr;                                                                                             \
-        si += Xi;                                                                                             \
-        if (DERIV) {                                                                                          \
-            zr = cr * Xr - ci * Xi;                                                                           \
-            zi = cr * Xi + ci * Xr;                                                                           \
-            du[0] -= m[0] * zr;                                                                               \
-            du[1] -= m[1] * zr;                                                                               \
-            du[2] -= m[2] * zr;                                                                               \
-            du[3] -= psign<PT>(3) * m[3] * zr;                                                                \
-            du[4] -= psign<PT>(5) * m[4] * zr;                                                                \
-            du[5] -= psign<PT>(6) * m[5] * zr;                                                                \
-            dx[0] -= psign<PT>(1) * hf[0] * zi;                                                               \
-            dx[1] -= psign<PT>(2) * hf[1] * zi;                                                               \
-            dx[2] -= psign<PT>(4) * hf[2] * zi;                                                               \
-        }                                                                                                     \
         break;                                                                                                \
     }
                 DIAG64_TERM(0) DIAG64_TERM(1) DIAG64_TERM(2) DIAG64_TERM(3)
                 DIAG64_TERM(4) DIAG64_TERM(5) DIAG64_TERM(6) DIAG64_TERM(7)
 #undef DIAG64_TERM
             }
+            double fr = S + G0 * A[0] + G2 * A[2];
+            double fi = an[s] + G1 * A[1];
+            if (L == 4) {
+                fr += G4 * A[4];
+                fi += G3 * A[3];
+            }
+            Xr[s] = fr * Er[s] - fi * Ei[s];
+            Xi[s] = fr * Ei[s] + fi * Er[s];
         }
-        if (DERIV) docc += cr * sr - ci * si;
+        // sums over the operations signed by parity class (as in the mixed line kernel)
+        const double sr = class_signed_sum<NOPS, PATS, 0>(Xr), si = class_signed_sum<NOPS, PATS, 0>(Xi);
+        if (DERIV) {
+            const double z0 = cr * sr - ci * si;
+            docc += z0;
+            du[0] -= m[0] * z0;
+            du[1] -= m[1] * z0;
+            du[2] -= m[2] * z0;
+            static_for<0, 3>([&](auto Ji) {
+                constexpr int j = decltype(Ji)::value;
+                constexpr int cu = j == 0 ? 3 : j == 1 ? 5 : 6;    // xy, xz, yz
+                const double wr = class_signed_sum<NOPS, PATS, cu>(Xr), wi = class_signed_sum<NOPS, PATS, cu>(Xi);
+                du[3 + j] -= m[3 + j] * (cr * wr - ci * wi);
+                constexpr int cx = 1 << j;                          // x, y, z
+                const double vr = class_signed_sum<NOPS, PATS, cx>(Xr), vi = class_signed_sum<NOPS, PATS, cx>(Xi);
+                dx[j] -= hf[j] * (cr * vi + ci * vr);
+            });
+        }
         stage_f64(stage, kk, lane, w * sr, w * si);
     }
     flush_f64(stage, nk, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);

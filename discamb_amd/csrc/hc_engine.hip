@@ -1338,6 +1338,8 @@ static constexpr int kDiag64Pats1 = 4096 | (3 << 3);
 static constexpr int kDiag64Pats2 = 4096 | (6 << 3);
 static constexpr int kDiag64Pats3 = 4096 | (7 << 3);
 static constexpr int kDiag64Pats4 = 4096 | (3 << 3) | (5 << 6) | (6 << 9);
+// added to a sign-pattern code: the mixed parity-folded kernel of that set without line recurrences
+static constexpr int kDiagDirect = 8192;
 
 template <int PAT>
 __device__ __forceinline__ constexpr double psign(int key)
@@ -1906,9 +1908,11 @@ __device__ __forceinline__ void line_steps_f32(const MainArgs& a, const double* 
     }
 }
 
-// Diagonal groups (compile-time sign patterns PATS as in the FP64 diagonal line kernel), with the
-// parity-class contraction of body_mixed_diag.
-template <int L, bool DERIV, bool OTF, int NOPS, int PATS>
+// Diagonal groups (compile-time sign patterns PATS as in the FP64 diagonal line kernel): form
+// factors from sign-keyed harmonic sums, derivative sums per parity class.  LINES false
+// evaluates every term directly (reflections in any order: the batched step, whose per-atom
+// table lookups want |h| order).
+template <int L, bool DERIV, bool OTF, int NOPS, int PATS, bool LINES = true>
 __device__ __forceinline__ void body_mixed_line_diag(const MainArgs& a, int wave, int lane, int chunk_id, int k0,
                                                      int k1, const float* __restrict__ hrec,
                                                      const float* __restrict__ rad, const unsigned* __restrict__ hd32,
@@ -1938,9 +1942,9 @@ __device__ __forceinline__ void body_mixed_line_diag(const MainArgs& a, int wave
     }
     const float* tab = cmem(static_cast<const float*>(a.tab));
     const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
-    const int* cont = cmem(a.line_cont);
+    const int* cont = LINES ? cmem(a.line_cont) : nullptr;
     float Mr[NOPS], Mi[NOPS], Cq[NOPS], Er[NOPS], Ei[NOPS], Q[NOPS], an[NOPS];
-    {
+    if (LINES) {
         const double xf[3] = {x0, x1, x2};
         double u64[6];
 #pragma unroll
@@ -1949,6 +1953,7 @@ __device__ __forceinline__ void body_mixed_line_diag(const MainArgs& a, int wave
     }
 #pragma unroll
     for (int s = 0; s < NOPS; s++) {
+        if (!LINES) Mr[s] = Mi[s] = Cq[s] = 0.f;
         Er[s] = Ei[s] = Q[s] = 0.f;
         an[s] = a.diag_anom[(PATS >> (3 * s)) & 7] * ani;
     }
@@ -1959,7 +1964,7 @@ __device__ __forceinline__ void body_mixed_line_diag(const MainArgs& a, int wave
         for (int k = kb; k < kend; ++k) {
             const float* r = hrec + (size_t)k * kHklopRec;
             // wave-uniform: every lane is on reflection k
-            const bool along = ((k - kb) & (kLineRestart - 1)) != 0 && cont[k] != 0;
+            const bool along = LINES && ((k - kb) & (kLineRestart - 1)) != 0 && cont[k] != 0;
             if (along) {
 #pragma unroll
                 for (int s = 0; s < NOPS; s++) {
@@ -1971,7 +1976,8 @@ __device__ __forceinline__ void body_mixed_line_diag(const MainArgs& a, int wave
                     Q[s] *= Cq[s];
                 }
             } else {
-                const bool next = k + 1 < kend && ((k + 1 - kb) & (kLineRestart - 1)) != 0 && cont[k + 1] != 0;
+                const bool next =
+                    LINES && k + 1 < kend && ((k + 1 - kb) & (kLineRestart - 1)) != 0 && cont[k + 1] != 0;
                 const unsigned* hd = hd32 + (size_t)k * 8;
                 const double h0 = ld_f64(hd, 0), h1 = ld_f64(hd, 1), h2 = ld_f64(hd, 2);
                 const unsigned* htk = ht32 + (size_t)k * a.n_ops * 2;
@@ -2235,12 +2241,13 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
         if (NOPS > 0 && DIAG) {
             constexpr int N = NOPS > 0 ? NOPS : 1;
             constexpr int PS = DPATS & 4095;
+            constexpr bool LN = (DPATS & kDiagDirect) == 0;
             if (cls <= 2)
-                body_mixed_line_diag<2, DERIV, OTF, N, PS>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, ht32, dtdf32,
-                                                           st, fpart, dpart);
+                body_mixed_line_diag<2, DERIV, OTF, N, PS, LN>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, ht32,
+                                                               dtdf32, st, fpart, dpart);
             else
-                body_mixed_line_diag<4, DERIV, OTF, N, PS>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, ht32, dtdf32,
-                                                           st, fpart, dpart);
+                body_mixed_line_diag<4, DERIV, OTF, N, PS, LN>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, ht32,
+                                                               dtdf32, st, fpart, dpart);
         } else if (NOPS > 0) {
             constexpr int N = NOPS > 0 ? NOPS : 1;
             if (cls <= 2)
@@ -4578,8 +4585,22 @@ struct BatchItem {
     double* d;
     int n_ops, n_hkl, n_wfn, n_types, n_slots, n_atoms, nd, n_chunks, centric_origin, diag, n_tab;
     int part_f32;     // F partials stored as FP32 (mixed precision)
+    int body;         // kernel body of the batched main kernel (BatchBody)
     MainArgs main;
 };
+
+// Kernel bodies of the batched step.  Each runs as its own persistent launch over the structures
+// that use it (contiguous in the batch order): one kernel holding every body would be allocated
+// the registers of the largest and spill in all of them.
+enum BatchBody {
+    kBodyGeneral = 0,    // general rotations (per-(reflection, operation) records)
+    kBodyDiag,           // parity-folded records, sign patterns read at run time
+    kBodyPats0,          // parity-folded records, 2/m (b axis) set compiled in
+    kBodyPats4,          // ... 222 set
+    kNumBodies
+};
+static_assert(kNumBodies <= 4 && kNumBodies * kXcds * sizeof(int) <= 256,
+              "the batch's per-body grids and queue counters are sized for at most four bodies");
 
 // index spaces of the batched launches; offsets are prefix sums over structures.  The main
 // kernel's (structure, atom wave, reflection chunk) units follow in kXcds queues, queue x holding
@@ -4602,7 +4623,7 @@ __global__ void k_batch_fill(const BatchItem* __restrict__ it, const int* __rest
                              int* __restrict__ counter)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < kXcds) counter[i] = 0;    // work counters of the persistent main kernel's queues
+    if (i < kXcds * kNumBodies) counter[i] = 0;    // work counters of the persistent main kernels' queues
     if (i >= total) return;
     const int b = find_item(off, nb, i);
     it[b].d[i - off[b]] = 0.0;
@@ -4635,13 +4656,15 @@ __global__ __launch_bounds__(64) void k_batch_atom_prep(const BatchItem* __restr
 // structure and a structure of fewer than four atom waves leaves no wave of a workgroup idle.
 // Workgroup b runs on XCD b % kXcds and drains that XCD's queue first: all atom waves of a chunk
 // then run on one XCD and its L2 holds the chunk's reflection records once.  A wave whose queue
-// is empty takes units from the other queues in turn.  In each queue units are ordered costliest
-// structure first, and within a structure chunk by chunk.  Parity-folded and general-rotation
-// structures share the kernel (the branch is uniform per unit).  Radial factors are evaluated on
-// the fly from the per-wavefunction tables (no per-(reflection, type) records).
-template <bool MIXED, bool DERIV>
+// is empty takes units from the other queues in turn.  One launch per kernel body (BODY) covers
+// the nb structures that use it: `it` and `uoff` start at the first of them, queue x's prefix
+// offsets are uoff + x * stride.  In each queue units are ordered costliest structure first, and
+// within a structure chunk by chunk.  Radial factors are evaluated on the fly from the
+// per-wavefunction tables (no per-(reflection, type) records); reflections are in |h| order, so
+// every term is evaluated directly.
+template <bool MIXED, bool DERIV, int BODY>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 2))) void k_batch_main(
-    const BatchItem* __restrict__ it, const int* __restrict__ uoff, int nb, int* __restrict__ counter)
+    const BatchItem* __restrict__ it, const int* __restrict__ uoff, int stride, int nb, int* __restrict__ counter)
 {
     constexpr int kStageBytes = MIXED ? kStageBytesF32 : kStageBytesF64;
     __shared__ __align__(16) unsigned char stage_lds[kWavesPerBlock * kStageBytes];
@@ -4651,12 +4674,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 :
     const int home = (int)(blockIdx.x % kXcds);
     for (int qi = 0; qi < kXcds; qi++) {
         const int x = (home + qi) % kXcds;
-        const int* off = uoff + (size_t)x * (nb + 1);
-        const int total = off[nb];
+        const int* off = uoff + (size_t)x * stride;
+        const int base = off[0], total = off[nb];
         for (;;) {
             int got = 0;
             if (lane == 0) got = atomicAdd(counter + x, 1);
-            const int u = __builtin_amdgcn_readfirstlane(__shfl(got, 0));
+            const int u = base + __builtin_amdgcn_readfirstlane(__shfl(got, 0));
             if (u >= total) break;
             const int b = __builtin_amdgcn_readfirstlane(find_item(off, nb, u));
             const BatchItem& q = it[b];
@@ -4665,12 +4688,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 :
             const int c = local / a.n_waves;
             const int wave = local - c * a.n_waves;
             const int chunk_id = c * kXcds + x;
-            if (MIXED && q.diag)
-                main_unit<MIXED, DERIV, 0, MIXED, true>(a, chunk_id, wave / kWavesPerBlock, wave % kWavesPerBlock,
-                                                        lane, stage);
+            const int ab = wave / kWavesPerBlock, wb = wave % kWavesPerBlock;
+            if constexpr (MIXED && BODY == kBodyDiag)
+                main_unit<true, DERIV, 0, true, true>(a, chunk_id, ab, wb, lane, stage);
+            else if constexpr (MIXED && BODY == kBodyPats0)
+                main_unit<true, DERIV, 0, true, true, 2, kDiag64Pats0 | kDiagDirect>(a, chunk_id, ab, wb, lane, stage);
+            else if constexpr (MIXED && BODY == kBodyPats4)
+                main_unit<true, DERIV, 0, true, true, 4, kDiag64Pats4 | kDiagDirect>(a, chunk_id, ab, wb, lane, stage);
             else
-                main_unit<MIXED, DERIV, 0, false, true>(a, chunk_id, wave / kWavesPerBlock, wave % kWavesPerBlock,
-                                                        lane, stage);
+                main_unit<MIXED, DERIV, 0, false, true>(a, chunk_id, ab, wb, lane, stage);
         }
     }
 }
@@ -4739,6 +4765,11 @@ void Engine::batch_item(Precision prec, bool derivs, int chunk, BatchItem* out)
     q.n_atoms = n_atoms_;
     q.n_chunks = n_hkl_ > 0 ? (n_hkl_ + chunk - 1) / chunk : 0;
     q.centric_origin = centric_origin_ ? 1 : 0;
+    q.body = kBodyGeneral;
+    if (q.diag) {
+        const int c = diag64_code();
+        q.body = c == kDiag64Pats4 ? kBodyPats4 : c == kDiag64Pats0 ? kBodyPats0 : kBodyDiag;
+    }
 }
 
 void Engine::mark_batched_run(bool derivs)
@@ -4776,10 +4807,17 @@ void BatchEngine::drop_graph()
     graph_key_.clear();
 }
 
-static const void* batch_main_kernel(bool mixed, bool derivs)
+static const void* batch_main_kernel(bool mixed, bool derivs, int body)
 {
-    if (mixed) return derivs ? (const void*)k_batch_main<true, true> : (const void*)k_batch_main<true, false>;
-    return derivs ? (const void*)k_batch_main<false, true> : (const void*)k_batch_main<false, false>;
+#define KB(M, B) (derivs ? (const void*)k_batch_main<M, true, B> : (const void*)k_batch_main<M, false, B>)
+    if (!mixed) return KB(false, kBodyGeneral);
+    switch (body) {
+        case kBodyDiag: return KB(true, kBodyDiag);
+        case kBodyPats0: return KB(true, kBodyPats0);
+        case kBodyPats4: return KB(true, kBodyPats4);
+        default: return KB(true, kBodyGeneral);
+    }
+#undef KB
 }
 
 int BatchEngine::chunk_for(Precision prec, bool derivs)
@@ -4805,7 +4843,15 @@ void BatchEngine::plan(Precision prec, bool derivs, int chunk)
         const BatchItem& q = items[b];
         return (double)q.n_slots * (q.n_ops + 1) * q.n_hkl;
     };
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+    // structures of one kernel body together (the body with the most work first), costliest
+    // structure first within a body
+    std::vector<double> body_work(kNumBodies, 0.0);
+    for (int b = 0; b < nb; b++) body_work[items[b].body] += work(b);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const int bx = items[x].body, by = items[y].body;
+        if (bx != by) return body_work[bx] != body_work[by] ? body_work[bx] > body_work[by] : bx < by;
+        return work(x) > work(y);
+    });
     std::vector<BatchItem> sorted(nb);
     for (int b = 0; b < nb; b++) sorted[b] = items[order[b]];
     share_tables(prec, order, sorted);
@@ -4885,6 +4931,16 @@ void BatchEngine::plan(Precision prec, bool derivs, int chunk)
     last_chunk = chunk;
     last_units = 0;
     for (int x = 0; x < kXcds; x++) last_units += total_[kSpUnit + x];
+    // one persistent launch per kernel body over its contiguous range of structures
+    groups_.clear();
+    for (int b = 0; b < nb; b++) {
+        const int bd = sorted[b].body;
+        if (groups_.empty() || groups_.back().body != bd) groups_.push_back({bd, b, b, 0});
+        BodyGroup& g = groups_.back();
+        g.b1 = b + 1;
+        for (int x = 0; x < kXcds; x++)
+            g.units += off_[(size_t)(kSpUnit + x) * (nb + 1) + b + 1] - off_[(size_t)(kSpUnit + x) * (nb + 1) + b];
+    }
 }
 
 // Radial tables depend only on the wavefunction data (Slater terms, deformation-valence exponent
@@ -5007,16 +5063,18 @@ void BatchEngine::launch(Precision prec, bool derivs)
     if (total_[kSpSlot] > 0)
         hipLaunchKernelGGL(k_batch_atom_prep, dim3(total_[kSpSlot]), dim3(64), 0, stream_, it, o(kSpSlot), nb);
     HIP_CHECK(hipGetLastError());
-    if (last_units > 0) {
-        const int grid = std::min(grid_[(mixed ? 2 : 0) + (derivs ? 1 : 0)], std::max(last_units, kXcds));
-        last_grid = grid;
-        const int* uo = o(kSpUnit);
-        const int nbv = nb;
-        int* cnt = d_counter_.as<int>();
-        void* args[] = {(void*)&it, (void*)&uo, (void*)&nbv, (void*)&cnt};
-        HIP_CHECK(hipLaunchKernel(batch_main_kernel(mixed, derivs), dim3(grid), dim3(256), args, 0, stream_));
-    } else {
-        last_grid = 0;
+    last_grid = 0;
+    for (size_t gi = 0; gi < groups_.size(); gi++) {
+        const BodyGroup& g = groups_[gi];
+        if (g.units == 0) continue;
+        const int grid = std::min(grid_[(mixed ? 2 : 0) + (derivs ? 1 : 0)][g.body], std::max(g.units, kXcds));
+        last_grid = std::max(last_grid, grid);
+        const BatchItem* itg = it + g.b0;
+        const int* uo = o(kSpUnit) + g.b0;
+        const int stride = nb + 1, n = g.b1 - g.b0;
+        int* cnt = d_counter_.as<int>() + (size_t)g.body * kXcds;
+        void* args[] = {(void*)&itg, (void*)&uo, (void*)&stride, (void*)&n, (void*)&cnt};
+        HIP_CHECK(hipLaunchKernel(batch_main_kernel(mixed, derivs, g.body), dim3(grid), dim3(256), args, 0, stream_));
     }
     const int R = 64;
     if (total_[kSpHkl] > 0)
@@ -5043,18 +5101,20 @@ void BatchEngine::launch_hkl_prep()
 void BatchEngine::run(Precision prec, bool derivs, bool graph)
 {
     HIP_CHECK(hipSetDevice(device_));
-    int& g = grid_[(prec == kMixed ? 2 : 0) + (derivs ? 1 : 0)];
-    if (g == 0) {
-        // persistent grid: every workgroup the device holds at once
-        int per_cu = 0;
-        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_main_kernel(prec == kMixed, derivs),
-                                                               256, 0));
-        hipDeviceProp_t prop;
-        HIP_CHECK(hipGetDeviceProperties(&prop, device_));
-        g = std::max(1, per_cu) * prop.multiProcessorCount;
-    }
     const int chunk = chunk_for(prec, derivs);
     plan(prec, derivs, chunk);
+    for (const BodyGroup& bg : groups_) {
+        int& g = grid_[(prec == kMixed ? 2 : 0) + (derivs ? 1 : 0)][bg.body];
+        if (g == 0) {
+            // persistent grid: every workgroup the device holds at once
+            int per_cu = 0;
+            HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, batch_main_kernel(prec == kMixed, derivs, bg.body), 256, 0));
+            hipDeviceProp_t prop;
+            HIP_CHECK(hipGetDeviceProperties(&prop, device_));
+            g = std::max(1, per_cu) * prop.multiProcessorCount;
+        }
+    }
     std::vector<unsigned char> key(items_);
     key.insert(key.end(), reinterpret_cast<const unsigned char*>(off_.data()),
                reinterpret_cast<const unsigned char*>(off_.data() + off_.size()));

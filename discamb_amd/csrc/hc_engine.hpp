@@ -361,7 +361,12 @@ private:
     std::vector<unsigned char> graph_key_; // items_ + off_ + precision at capture time
     hipGraphExec_t exec_ = nullptr;
     std::map<long long, int> tuned_;
-    int grid_[4] = {};                     // persistent main-kernel grid per (precision, derivatives)
+    int grid_[4][4] = {};                  // persistent main-kernel grid per (precision, derivatives, body)
+    // structures [b0, b1) of the batch order run kernel body `body` as one persistent launch
+    struct BodyGroup {
+        int body, b0, b1, units;
+    };
+    std::vector<BodyGroup> groups_;
 };
 
 }  // namespace discamb_amd

@@ -3687,6 +3687,18 @@ static const void* main_kernel(bool mixed, bool derivs, int mp_pass, bool diag, 
         }
     }
 #undef KD
+    // mixed parity-folded kernel with a compiled sign-pattern set, every term direct (table lookups)
+    if ((diag64 & kDiagDirect) && mixed && diag && otf && mp_pass == 0) {
+        switch (diag64 & ~kDiagDirect) {
+        case kDiag64Pats0:
+            return derivs ? (const void*)k_main<true, true, 0, true, true, 2, kDiag64Pats0 | kDiagDirect>
+                          : (const void*)k_main<true, false, 0, true, true, 2, kDiag64Pats0 | kDiagDirect>;
+        case kDiag64Pats4:
+            return derivs ? (const void*)k_main<true, true, 0, true, true, 4, kDiag64Pats4 | kDiagDirect>
+                          : (const void*)k_main<true, false, 0, true, true, 4, kDiag64Pats4 | kDiagDirect>;
+        default: break;
+        }
+    }
     if (line_ops > 0 && mp_pass == 0 && !otf && !(mixed && diag)) {
         switch (line_ops) {
         case 1: return mixed ? KL(true, 1) : KL(false, 1);
@@ -3716,16 +3728,28 @@ static const void* main_kernel(bool mixed, bool derivs, int mp_pass, bool diag, 
 #undef KM
 }
 
+int Engine::main_code(bool mixed, bool mp) const
+{
+    const int d64 = use_diag64(mixed, mp);
+    if (d64) return d64;
+    if (mixed && !mp && use_diag(mixed, mp) && use_otf(mp)) {
+        const int c = diag64_code();
+        if (c == kDiag64Pats0 || c == kDiag64Pats4) return c | kDiagDirect;
+    }
+    return 0;
+}
+
 int Engine::resident_waves(bool mixed, bool derivs, bool mp)
 {
-    const int line = use_line(mixed, mp) ? (use_diag64(mixed, mp) ? 5 : (int)ops_.size()) : 0;
+    const int line = use_line(mixed, mp) ? (use_diag64(mixed, mp) ? 5 : (int)ops_.size())
+                                         : ((main_code(mixed, mp) & kDiagDirect) ? 6 : 0);
     const int key = line * 32 + (use_otf(mp) ? 16 : 0) + (use_diag(mixed, mp) ? 8 : 0) + (mixed ? 4 : 0) +
                     (derivs ? 2 : 0) + (mp ? 1 : 0);
     if (resident_waves_[key] > 0) return resident_waves_[key];
     int blocks = 0;
     HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &blocks, main_kernel(mixed, derivs, mp ? 1 : 0, use_diag(mixed, mp), use_otf(mp), (int)ops_.size() * (line > 0),
-                             use_diag64(mixed, mp)),
+        &blocks, main_kernel(mixed, derivs, mp ? 1 : 0, use_diag(mixed, mp), use_otf(mp),
+                             use_line(mixed, mp) ? (int)ops_.size() : 0, main_code(mixed, mp)),
         256, 0));
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, device_));
@@ -3806,7 +3830,7 @@ void Engine::launch_main(Precision prec, bool derivs, bool mp, int chunk)
     void* args[] = {&a};
     const int passes[2] = {mp ? (mp_scalars ? 1 : 3) : 0, 2};
     const int line = use_line(prec == kMixed, mp) ? (int)ops_.size() : 0;
-    const int d64 = use_diag64(prec == kMixed, mp);
+    const int d64 = main_code(prec == kMixed, mp);
     for (int p = 0; p < (mp ? 2 : 1); p++) {
         const void* kern =
             main_kernel(prec == kMixed, derivs, passes[p], use_diag(prec == kMixed, mp), use_otf(mp), line, d64);

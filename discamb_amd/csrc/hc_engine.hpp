@@ -236,7 +236,7 @@ private:
         if (tune.radial_mode != 0) return tune.radial_mode == 2;
         return (long long)model_.n_types * 8 > (long long)n_sorted_;
     }
-    int resident_waves_[32 * 6] = {};   // key: line kind (0-5) * 32 + path bits
+    int resident_waves_[32 * 7] = {};   // key: kernel kind (0-6) * 32 + path bits
     bool diag_ = false;               // every operation is a diagonal sign matrix
     int diag_pat_[8] = {};
     float diag_anom_[8] = {};
@@ -285,6 +285,9 @@ private:
     // the FP64 line kernel with per-reflection records for diagonal groups: its sign-pattern code, 0
     // when the operations' patterns are not one of the compiled sets
     int diag64_code() const;
+    // sign-pattern code passed to main_kernel: use_diag64, or for the mixed parity-folded kernel
+    // with per-atom table lookups (no line recurrences) a compiled set marked kDiagDirect
+    int main_code(bool mixed, bool mp) const;
     int use_diag64(bool mixed, bool mp) const
     {
         return use_line(mixed, mp) && diag_ && !tune.general_path ? diag64_code() : 0;

@@ -42,16 +42,21 @@ def measured_traffic(kernel, path=TRAFFIC_FILE):
 
 
 def batch_traffic(chunk, path=BATCH_TRAFFIC_FILE):
-    """HBM bytes per batched step recorded for the 256-structure workload at this mixed chunk, or None."""
+    """(chunk, HBM bytes per batched step) recorded for the 256-structure workload: at this mixed
+    chunk if recorded, else at the recorded chunk closest to it; (None, None) without a record."""
+    rec = {}
     try:
         with open(path) as fh:
             for line in fh:
                 f = line.split()
-                if line.startswith("batch-traffic ") and int(f[1]) == chunk:
-                    return float(f[2])
+                if line.startswith("batch-traffic "):
+                    rec[int(f[1])] = float(f[2])
     except OSError:
         pass
-    return None
+    if not rec:
+        return None, None
+    c = min(rec, key=lambda x: abs(x - chunk))
+    return c, rec[c]
 
 
 def flops_per_term(L):
@@ -242,9 +247,11 @@ def bench_batch(args):
                    "structures": len(ws), "terms_per_step": terms, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "roof": "FP32 vector ALU; whole batched step (prep + main + reductions)",
                      "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK, "traffic": batch_traffic(mx["chunk"]),
+                     "frac": achieved / FP32_PEAK, "traffic": batch_traffic(mx["chunk"])[1],
+                     "traffic_chunk": batch_traffic(mx["chunk"])[0],
                      "traffic_note": "HBM bytes per batched step, summed over the step's kernels, from the calibrated "
-                                     "FETCH_SIZE + WRITE_SIZE counters recorded for this workload and chunk"},
+                                     "FETCH_SIZE + WRITE_SIZE counters recorded for this workload at traffic_chunk "
+                                     "(profiles/batch256_hbm_traffic.md)"},
         "step_time": {"device_ms_mixed": mx["device_ms"], "wall_ms_mixed": mx["wall_ms"],
                       "device_ms_fp64": fp["device_ms"], "wall_ms_fp64": fp["wall_ms"],
                       "note": "device = events around the graph launch; wall adds descriptor upload and sync"},

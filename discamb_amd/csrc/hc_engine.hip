@@ -1427,10 +1427,95 @@ __device__ __forceinline__ void static_for(F&& f)
     }
 }
 
+// The sign keys are linear in the class bits over GF(2) (odd_under(c1 ^ c2, p) = odd_under(c1, p) ^
+// odd_under(c2, p)), so the keys of all classes span the subspace generated by the keys of x, y
+// and z.  key_dim: its dimension D; key_of(alpha): the key with coordinates alpha in the basis;
+// op_vector(s): operation s's signs on the basis keys (bit i: basis key i is odd under s).  For the
+// compiled sets the NOPS operations take the 2^D sign vectors once each, so sums over operations
+// signed by key are Walsh-Hadamard transforms (D * 2^D additions instead of NOPS * (NOPS - 1)).
+template <int NOPS, int PATS>
+__host__ __device__ constexpr int key_basis(int i)
+{
+    int v[3] = {0, 0, 0}, d = 0;
+    const int c3[3] = {1, 2, 4};
+    for (int q = 0; q < 3; q++) {
+        const int k = sign_key<NOPS, PATS>(c3[q]);
+        bool in = false;
+        for (int a = 0; a < (1 << d); a++) {
+            int x = 0;
+            for (int j = 0; j < d; j++)
+                if ((a >> j) & 1) x ^= v[j];
+            if (x == k) in = true;
+        }
+        if (!in) v[d++] = k;
+    }
+    return i < 0 ? d : v[i];
+}
+
+template <int NOPS, int PATS>
+__host__ __device__ constexpr int key_dim()
+{
+    return key_basis<NOPS, PATS>(-1);
+}
+
+template <int NOPS, int PATS>
+__host__ __device__ constexpr int key_of(int alpha)
+{
+    int x = 0;
+    for (int j = 0; j < key_dim<NOPS, PATS>(); j++)
+        if ((alpha >> j) & 1) x ^= key_basis<NOPS, PATS>(j);
+    return x;
+}
+
+template <int NOPS, int PATS>
+__host__ __device__ constexpr int key_coords(int k)
+{
+    for (int a = 0; a < (1 << key_dim<NOPS, PATS>()); a++)
+        if (key_of<NOPS, PATS>(a) == k) return a;
+    return -1;
+}
+
+template <int NOPS, int PATS>
+__host__ __device__ constexpr int op_vector(int s)
+{
+    int b = 0;
+    for (int j = 0; j < key_dim<NOPS, PATS>(); j++)
+        if ((key_basis<NOPS, PATS>(j) >> s) & 1) b |= 1 << j;
+    return b;
+}
+
+template <int NOPS, int PATS>
+__host__ __device__ constexpr bool ops_span_keys()
+{
+    if (NOPS != (1 << key_dim<NOPS, PATS>())) return false;
+    for (int s = 0; s < NOPS; s++)
+        for (int t = 0; t < s; t++)
+            if (op_vector<NOPS, PATS>(s) == op_vector<NOPS, PATS>(t)) return false;
+    return true;
+}
+
+// in-place Walsh-Hadamard transform: x[b] <- sum_a (-1)^popcount(a & b) x[a]
+template <int D, typename R>
+__device__ __forceinline__ void wht(R (&x)[1 << D])
+{
+    static_for<0, D>([&](auto Ii) {
+        constexpr int i = decltype(Ii)::value;
+        static_for<0, (1 << D)>([&](auto Ai) {
+            constexpr int a = decltype(Ai)::value;
+            if constexpr (((a >> i) & 1) == 0) {
+                const R p = x[a], q = x[a | (1 << i)];
+                x[a] = p + q;
+                x[a | (1 << i)] = p - q;
+            }
+        });
+    });
+}
+
 template <int L, int NOPS, int PATS, typename R>
 __device__ __forceinline__ void keyed_form_factors(const R* P, const R* Y, R S, R G0, R G1, R G2, R G3, R G4,
                                                    R (&fr)[NOPS], R (&fi)[NOPS])
 {
+    static_assert(ops_span_keys<NOPS, PATS>(), "the operations must take every sign vector of the key basis once");
     constexpr int NK = 1 << NOPS;
     constexpr int NLM = (L + 1) * (L + 1);
     R C[5][NK];
@@ -1456,30 +1541,40 @@ __device__ __forceinline__ void keyed_form_factors(const R* P, const R* Y, R S, 
             }
         });
     });
+    // f_s = sum_k sign_s(k) B[k]: a Walsh-Hadamard transform over the key basis
+    constexpr int D = key_dim<NOPS, PATS>();
+    R ar[1 << D], ai[1 << D];
+    static_for<0, (1 << D)>([&](auto Ai) {
+        constexpr int al = decltype(Ai)::value;
+        ar[al] = Br[key_of<NOPS, PATS>(al)];
+        ai[al] = Bi[key_of<NOPS, PATS>(al)];
+    });
+    wht<D>(ar);
+    wht<D>(ai);
     static_for<0, NOPS>([&](auto Si) {
         constexpr int s = decltype(Si)::value;
-        R a = R(0), b = R(0);
-        static_for<0, NK>([&](auto Ki) {
-            constexpr int k = decltype(Ki)::value;
-            if constexpr (key_used<L, NOPS, PATS>(k, false)) a = ((k >> s) & 1) ? a - Br[k] : a + Br[k];
-            if constexpr (key_used<L, NOPS, PATS>(k, true)) b = ((k >> s) & 1) ? b - Bi[k] : b + Bi[k];
-        });
-        fr[s] = a;
-        fi[s] = b;
+        fr[s] = ar[op_vector<NOPS, PATS>(s)];
+        fi[s] = ai[op_vector<NOPS, PATS>(s)];
     });
 }
 
-// sum_s sign_s x_s with the signs of parity class C under the operations of PATS
-template <int NOPS, int PATS, int C, typename R>
-__device__ __forceinline__ R class_signed_sum(const R (&x)[NOPS])
+// Sums over the operations signed like each parity class: W[alpha] = sum_s sign x_s, class c's
+// sum at W[key_coords(c)]; one Walsh-Hadamard transform of the operations' values.
+template <int NOPS, int PATS, typename R>
+__device__ __forceinline__ void class_sums_all(const R (&x)[NOPS], R (&w)[1 << key_dim<NOPS, PATS>()])
 {
-    R a = R(0);
+    constexpr int D = key_dim<NOPS, PATS>();
     static_for<0, NOPS>([&](auto Si) {
         constexpr int s = decltype(Si)::value;
-        if constexpr (odd_under(C, (PATS >> (3 * s)) & 7)) a -= x[s];
-        else a += x[s];
+        w[op_vector<NOPS, PATS>(s)] = x[s];
     });
-    return a;
+    wht<D>(w);
+}
+
+template <int NOPS, int PATS>
+__host__ __device__ constexpr int class_coords(int c)
+{
+    return key_coords<NOPS, PATS>(sign_key<NOPS, PATS>(c));
 }
 
 template <int L, bool DERIV, bool OTF, int NOPS, int PATS>
@@ -1617,7 +1712,10 @@ __device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave,
             Xi[s] = fr * Ei[s] + fi * Er[s];
         }
         // sums over the operations signed by parity class (as in the mixed line kernel)
-        const double sr = class_signed_sum<NOPS, PATS, 0>(Xr), si = class_signed_sum<NOPS, PATS, 0>(Xi);
+        double Wr[1 << key_dim<NOPS, PATS>()], Wi[1 << key_dim<NOPS, PATS>()];
+        class_sums_all<NOPS, PATS>(Xr, Wr);
+        class_sums_all<NOPS, PATS>(Xi, Wi);
+        const double sr = Wr[class_coords<NOPS, PATS>(0)], si = Wi[class_coords<NOPS, PATS>(0)];
         if (DERIV) {
             const double z0 = cr * sr - ci * si;
             docc += z0;
@@ -1626,12 +1724,10 @@ __device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave,
             du[2] -= m[2] * z0;
             static_for<0, 3>([&](auto Ji) {
                 constexpr int j = decltype(Ji)::value;
-                constexpr int cu = j == 0 ? 3 : j == 1 ? 5 : 6;    // xy, xz, yz
-                const double wr = class_signed_sum<NOPS, PATS, cu>(Xr), wi = class_signed_sum<NOPS, PATS, cu>(Xi);
-                du[3 + j] -= m[3 + j] * (cr * wr - ci * wi);
-                constexpr int cx = 1 << j;                          // x, y, z
-                const double vr = class_signed_sum<NOPS, PATS, cx>(Xr), vi = class_signed_sum<NOPS, PATS, cx>(Xi);
-                dx[j] -= hf[j] * (cr * vi + ci * vr);
+                constexpr int cu = class_coords<NOPS, PATS>(j == 0 ? 3 : j == 1 ? 5 : 6);    // xy, xz, yz
+                du[3 + j] -= m[3 + j] * (cr * Wr[cu] - ci * Wi[cu]);
+                constexpr int cx = class_coords<NOPS, PATS>(1 << j);                         // x, y, z
+                dx[j] -= hf[j] * (cr * Wi[cx] + ci * Wr[cx]);
             });
         }
         stage_f64(stage, kk, lane, w * sr, w * si);
@@ -2038,7 +2134,10 @@ __device__ __forceinline__ void body_mixed_line_diag(const MainArgs& a, int wave
                 Xr[s] = fr * Er[s] - fi * Ei[s];
                 Xi[s] = fr * Ei[s] + fi * Er[s];
             }
-            const float sr = class_signed_sum<NOPS, PATS, 0>(Xr), si = class_signed_sum<NOPS, PATS, 0>(Xi);
+            float Wr[1 << key_dim<NOPS, PATS>()], Wi[1 << key_dim<NOPS, PATS>()];
+            class_sums_all<NOPS, PATS>(Xr, Wr);
+            class_sums_all<NOPS, PATS>(Xi, Wi);
+            const float sr = Wr[class_coords<NOPS, PATS>(0)], si = Wi[class_coords<NOPS, PATS>(0)];
             if (DERIV) {
                 const float z0 = cr * sr - ci * si;
                 focc += z0;
@@ -2047,12 +2146,10 @@ __device__ __forceinline__ void body_mixed_line_diag(const MainArgs& a, int wave
                 fu[2] -= r[27] * z0;
                 static_for<0, 3>([&](auto Ji) {
                     constexpr int j = decltype(Ji)::value;
-                    constexpr int cu = j == 0 ? 3 : j == 1 ? 5 : 6;    // xy, xz, yz
-                    const float wr = class_signed_sum<NOPS, PATS, cu>(Xr), wi = class_signed_sum<NOPS, PATS, cu>(Xi);
-                    fu[3 + j] -= r[28 + j] * (cr * wr - ci * wi);
-                    constexpr int cx = 1 << j;                          // x, y, z
-                    const float vr = class_signed_sum<NOPS, PATS, cx>(Xr), vi = class_signed_sum<NOPS, PATS, cx>(Xi);
-                    fx[j] -= r[31 + j] * (cr * vi + ci * vr);
+                    constexpr int cu = class_coords<NOPS, PATS>(j == 0 ? 3 : j == 1 ? 5 : 6);    // xy, xz, yz
+                    fu[3 + j] -= r[28 + j] * (cr * Wr[cu] - ci * Wi[cu]);
+                    constexpr int cx = class_coords<NOPS, PATS>(1 << j);                         // x, y, z
+                    fx[j] -= r[31 + j] * (cr * Wi[cx] + ci * Wr[cx]);
                 });
             }
             stage_f32(stage, k - kb, lane, w * sr, w * si);
@@ -2232,13 +2329,13 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
     const unsigned* ht32 = cmem(reinterpret_cast<const unsigned*>(a.ht));
     double* fpart = gmem(a.fpart);
     double* dpart = gmem(a.dpart);
-    if (MIXED) {
+    if constexpr (MIXED) {
         const float* hop = cmem(static_cast<const float*>(a.hklop));
         const float* rad = cmem(static_cast<const float*>(a.radial));
         const float* radx = cmem(static_cast<const float*>(a.radx));
         const unsigned* phs = cmem(reinterpret_cast<const unsigned*>(a.phase));
         float2* st = static_cast<float2*>(stage);
-        if (NOPS > 0 && DIAG) {
+        if constexpr (NOPS > 0 && DIAG) {
             constexpr int N = NOPS > 0 ? NOPS : 1;
             constexpr int PS = DPATS & 4095;
             constexpr bool LN = (DPATS & kDiagDirect) == 0;
@@ -2248,7 +2345,7 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
             else
                 body_mixed_line_diag<4, DERIV, OTF, N, PS, LN>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, ht32,
                                                                dtdf32, st, fpart, dpart);
-        } else if (NOPS > 0) {
+        } else if constexpr (NOPS > 0) {
             constexpr int N = NOPS > 0 ? NOPS : 1;
             if (cls <= 2)
                 body_mixed_line<2, DERIV, OTF, N>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, dtdf32, st, fpart,
@@ -2256,7 +2353,7 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
             else
                 body_mixed_line<4, DERIV, OTF, N>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, dtdf32, st, fpart,
                                                   dpart);
-        } else if (DIAG) {
+        } else if constexpr (DIAG) {
             if (cls <= 2)
                 body_mixed_diag<2, DERIV, MP, OTF>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, phs, ht32, dtdf32,
                                                    st, fpart, dpart);
@@ -2273,7 +2370,7 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
         const double* rad = cmem(static_cast<const double*>(a.radial));
         const double* radx = cmem(static_cast<const double*>(a.radx));
         double2* st = static_cast<double2*>(stage);
-        if (NOPS > 0 && DPATS > 0) {
+        if constexpr (NOPS > 0 && DPATS > 0) {
             constexpr int N = NOPS > 0 ? NOPS : 1;
             constexpr int PS = DPATS & 4095;
             if (cls <= 2)
@@ -2282,7 +2379,7 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
             else
                 body_fp64_line_diag<4, DERIV, OTF, N, PS>(a, wave, lane, chunk_id, k0, k1, hop, rad, dtdf32, st, fpart,
                                                           dpart);
-        } else if (NOPS > 0) {
+        } else if constexpr (NOPS > 0) {
             constexpr int N = NOPS > 0 ? NOPS : 1;
             if (cls <= 2)
                 body_fp64_line<2, DERIV, OTF, N>(a, wave, lane, chunk_id, k0, k1, hop, rad, dtdf32, st, fpart, dpart);

@@ -94,9 +94,11 @@ struct Calculator {
     std::vector<LcsDef> lcs_defs;
     bool lcs_chain = false;
     std::vector<double> last_lcs, last_xyz_frac;
-    // host staging of upload(), reused between calls
+    // host staging of upload(), reused between calls; centring multipliers of the last reflection list
     EngineAtoms up_atoms;
     std::vector<double> up_dtdf;
+    std::vector<int> mult_hkl;
+    std::vector<double> mult_k;
     Engine& eng() { return iam ? *iam_engine : *engine; }
     bool eng_centric_origin() const { return !iam && centric_origin; }
     // state of the last upload
@@ -326,21 +328,33 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
     }
     c->last_lcs = ea.lcs;
     c->last_xyz_frac = ea.xyz_frac;
-    // centering multipliers, dT/dF <- conj(dT/dF) * mult  (.cpp:393-404)
+    // centering multipliers, dT/dF <- conj(dT/dF) * mult  (.cpp:393-404); the multipliers depend
+    // on the reflections only and are kept for a list equal to the last one
+    if (c->mult_hkl.size() != 3 * (size_t)n_hkl ||
+        (n_hkl > 0 && std::memcmp(c->mult_hkl.data(), hkl, 3 * (size_t)n_hkl * sizeof(int)) != 0)) {
+        c->mult_hkl.assign(hkl, hkl + 3 * (size_t)n_hkl);
+        c->mult_k.resize(n_hkl);
+        for (int k = 0; k < n_hkl; k++) c->mult_k[k] = centering_multiplier(c->sg.centering, c->sg.obverse, hkl + 3 * k);
+    }
     std::vector<double>& dd = c->up_dtdf;
     dd.resize(3 * (size_t)n_hkl);
-    for (int k = 0; k < n_hkl; k++) {
-        double m = centering_multiplier(c->sg.centering, c->sg.obverse, hkl + 3 * k);
-        double re = rows ? 0.0 : dtdf[2 * k] * m, im = rows ? 0.0 : -dtdf[2 * k + 1] * m;
-        if (c->eng_centric_origin()) {
-            // F = 2 Re(sum) and derivatives 2 Re(c) Re(.)  (Engine3.cpp:1924-1958)
-            re *= 2.0;
-            im = 0.0;
+    // F = 2 Re(sum) and derivatives 2 Re(c) Re(.) for a centre at the origin (Engine3.cpp:1924-1958)
+    const bool centric = c->eng_centric_origin();
+    const double* mk = c->mult_k.data();
+    if (rows) {
+        for (int k = 0; k < 2 * n_hkl; k++) dd[k] = 0.0;
+    } else if (centric) {
+        for (int k = 0; k < n_hkl; k++) {
+            dd[2 * k] = dtdf[2 * k] * mk[k] * 2.0;
+            dd[2 * k + 1] = 0.0;
         }
-        dd[2 * k] = re;
-        dd[2 * k + 1] = im;
-        dd[2 * (size_t)n_hkl + k] = m;
+    } else {
+        for (int k = 0; k < n_hkl; k++) {
+            dd[2 * k] = dtdf[2 * k] * mk[k];
+            dd[2 * k + 1] = -dtdf[2 * k + 1] * mk[k];
+        }
     }
+    std::memcpy(dd.data() + 2 * (size_t)n_hkl, mk, (size_t)n_hkl * sizeof(double));
     c->eng().line_order = true;     // both precisions carry phases along hkl lines (groups of <= 4 operations)
     c->eng().upload(ea, n_hkl, hkl, dd.data(), rows);
     c->n_atoms = n;

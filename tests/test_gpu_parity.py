@@ -303,7 +303,8 @@ def test_failed_upload_leaves_no_stale_inputs():
 
 
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
-@pytest.mark.parametrize("sg", ["P1", "P-1", "P21/c", "C2/c", "P212121", "P31", "P-1_shifted", "I222"])
+@pytest.mark.parametrize("sg", ["P1", "P-1", "P21/c", "C2/c", "P212121", "P31", "P-1_shifted", "I222", "P2_1-c",
+                                "P2_1-a", "Pmm2"])
 def test_line_recurrences_vs_direct_and_oracle(sg, mode):
     """Terms carried along hkl lines (groups with <= 4 explicit operations, radial records) against
     direct evaluation of every phase and Debye-Waller factor and against the oracle, with
@@ -312,8 +313,15 @@ def test_line_recurrences_vs_direct_and_oracle(sg, mode):
     the lines, direct evaluation every fourth reflection): both against the oracle at 1e-6 (F) and
     1e-5 (derivatives), and the line path no further from the oracle than twice the direct path's
     error plus 2e-7."""
-    base = sg.split("_")[0]
-    w = synth.small_structure(base if base != "I222" else "P212121", n_atoms=20, d_min=0.8, seed=23)
+    base = {"I222": "P212121", "P2_1-c": "P212121", "P2_1-a": "P212121", "Pmm2": "P212121"}.get(sg, sg.split("_")[0])
+    w = synth.small_structure(base, n_atoms=20, d_min=0.8, seed=23)
+    # 2-fold screw axis along c and along a (the other compiled two-operation sign sets), and a
+    # diagonal four-operation set without a compiled kernel (mirror planes)
+    extra = {"P2_1-c": ["x,y,z", "-x,-y,z+1/2"], "P2_1-a": ["x,y,z", "x+1/2,-y,-z"],
+             "Pmm2": ["x,y,z", "-x,-y,z", "x,-y,z", "-x,y,z"]}
+    if sg in extra:
+        w.crystal.symops = extra[sg]
+        w.crystal.atoms.multiplicity[:] = len(extra[sg])
     if sg == "P-1_shifted":
         w.crystal.symops = ["x,y,z", "-x+1/2,-y,-z+1/2"]
     if sg == "I222":

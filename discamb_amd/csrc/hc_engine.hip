@@ -1608,16 +1608,21 @@ __device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave,
     const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
     const int* cont = cmem(a.line_cont);
     const unsigned* lop = cmem(reinterpret_cast<const unsigned*>(a.line_ops));
-    double Mr[NOPS], Mi[NOPS], Cq[NOPS], Er[NOPS], Ei[NOPS], Q[NOPS], an[NOPS];
+    // With diagonal rotations the step of every operation is +-Delta, so the Debye-Waller step
+    // ratio C = exp(-2 Delta_s^T U Delta_s) is the same for all of them.
+    double Mr[NOPS], Mi[NOPS], Er[NOPS], Ei[NOPS], Q[NOPS], an[NOPS];
+    double Cq;
+    {
+        double e2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; j++) e2 = fma(ld_f64(lop, 4 + j), u[j], e2);
+        Cq = exp_neg(2.0 * e2);
+    }
 #pragma unroll
     for (int s = 0; s < NOPS; s++) {
         const unsigned* o = lop + s * 20;
         const double ph = __builtin_amdgcn_fract(fma(ld_f64(o, 2), x[2], fma(ld_f64(o, 1), x[1], fma(ld_f64(o, 0), x[0], ld_f64(o, 3)))));
         sincos_rev(ph, &Mi[s], &Mr[s]);
-        double e2 = 0.0;
-#pragma unroll
-        for (int j = 0; j < 6; j++) e2 = fma(ld_f64(o, 4 + j), u[j], e2);
-        Cq[s] = exp_neg(2.0 * e2);
         Er[s] = Ei[s] = Q[s] = 0.0;
         an[s] = (double)a.diag_anom[(PATS >> (3 * s)) & 7] * ani;
     }
@@ -1641,7 +1646,7 @@ __device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave,
                 const double ei = Er[s] * qi + Ei[s] * qr;
                 Er[s] = er;
                 Ei[s] = ei;
-                Q[s] *= Cq[s];
+                Q[s] *= Cq;
             }
         } else {
             const bool next = k + 1 < k1 && cont[k + 1] != 0;
@@ -2050,6 +2055,8 @@ __device__ __forceinline__ void body_mixed_line_diag(const MainArgs& a, int wave
 #pragma unroll
     for (int s = 0; s < NOPS; s++) {
         if (!LINES) Mr[s] = Mi[s] = Cq[s] = 0.f;
+        // diagonal rotations step every operation by +-Delta: one Debye-Waller step ratio for all
+        if (s > 0) Cq[s] = Cq[0];
         Er[s] = Ei[s] = Q[s] = 0.f;
         an[s] = a.diag_anom[(PATS >> (3 * s)) & 7] * ani;
     }

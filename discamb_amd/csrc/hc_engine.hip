@@ -1511,14 +1511,13 @@ __device__ __forceinline__ void wht(R (&x)[1 << D])
     });
 }
 
+// Harmonic sums binned by (l, sign key): C[l][k] = sum_{j in l, key(j) = k} P_j d_j.
 template <int L, int NOPS, int PATS, typename R>
-__device__ __forceinline__ void keyed_form_factors(const R* P, const R* Y, R S, R G0, R G1, R G2, R G3, R G4,
-                                                   R (&fr)[NOPS], R (&fi)[NOPS])
+__device__ __forceinline__ void keyed_bins(const R* P, const R* Y, R (&C)[5][1 << NOPS])
 {
     static_assert(ops_span_keys<NOPS, PATS>(), "the operations must take every sign vector of the key basis once");
     constexpr int NK = 1 << NOPS;
     constexpr int NLM = (L + 1) * (L + 1);
-    R C[5][NK];
     static_for<0, 5>([&](auto Li) {
         static_for<0, NK>([&](auto Ki) { C[decltype(Li)::value][decltype(Ki)::value] = R(0); });
     });
@@ -1527,6 +1526,15 @@ __device__ __forceinline__ void keyed_form_factors(const R* P, const R* Y, R S, 
         constexpr int l = harmonic_l(j), k = sign_key<NOPS, PATS>(harmonic_parity(j));
         C[l][k] = fma(P[j], Y[j], C[l][k]);
     });
+}
+
+// f_s = S + sum_l G_l A_l(s) for every operation from the binned sums (real parts from even l
+// and S, imaginary parts from odd l)
+template <int L, int NOPS, int PATS, typename R>
+__device__ __forceinline__ void keyed_weights(const R (&C)[5][1 << NOPS], R S, R G0, R G1, R G2, R G3, R G4,
+                                              R (&fr)[NOPS], R (&fi)[NOPS])
+{
+    constexpr int NK = 1 << NOPS;
     const R G[5] = {G0, G1, G2, G3, G4};
     R Br[NK], Bi[NK];
     static_for<0, NK>([&](auto Ki) {
@@ -2012,16 +2020,22 @@ __device__ __forceinline__ void line_steps_f32(const MainArgs& a, const double* 
 // Diagonal groups (compile-time sign patterns PATS as in the FP64 diagonal line kernel): form
 // factors from sign-keyed harmonic sums, derivative sums per parity class.  LINES false
 // evaluates every term directly (reflections in any order: the batched step, whose per-atom
-// table lookups want |h| order).
-template <int L, bool DERIV, bool OTF, int NOPS, int PATS, bool LINES = true>
+// table lookups want |h| order).  MP: the passes of the multipole adjoint as in body_mixed_diag
+// (1: F, xyz/U/occupancy and the Pval, kappa, kappa' sums; 2: the B_lm sums only; 3: the plain
+// pass in the multipole row layout).
+template <int L, bool DERIV, bool OTF, int NOPS, int PATS, bool LINES = true, int MP = 0>
 __device__ __forceinline__ void body_mixed_line_diag(const MainArgs& a, int wave, int lane, int chunk_id, int k0,
                                                      int k1, const float* __restrict__ hrec,
-                                                     const float* __restrict__ rad, const unsigned* __restrict__ hd32,
+                                                     const float* __restrict__ rad, const float* __restrict__ radx,
+                                                     const unsigned* __restrict__ hd32,
                                                      const unsigned* __restrict__ ht32,
                                                      const unsigned* __restrict__ dtdf32, float2* __restrict__ stage,
                                                      double* __restrict__ fpart, double* __restrict__ dpart)
 {
     constexpr int NLM = (L + 1) * (L + 1);
+    constexpr bool SC = MP == 1;    // multipole pass 1: F, xyz/U/occ and the Pval, kappa, kappa' sums
+    constexpr bool BO = MP == 2;    // multipole pass 2: the global-frame B_lm sums only
+    static_assert(!(MP && OTF), "the multipole adjoint uses radial records");
     const int slot = wave * kWave + lane;
     const AtomRec& at = cmem(a.atoms)[slot];
     const double x0 = at.xf[0], x1 = at.xf[1], x2 = at.xf[2];
@@ -2061,9 +2075,17 @@ __device__ __forceinline__ void body_mixed_line_diag(const MainArgs& a, int wave
         an[s] = a.diag_anom[(PATS >> (3 * s)) & 7] * ani;
     }
     double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
+    double B[BO ? NLM : 1];
+#pragma unroll
+    for (int j = 0; j < (BO ? NLM : 1); j++) B[j] = 0;
+    double dpv = 0, dka = 0, dkp = 0;
     for (int kb = k0; kb < k1; kb += kStage) {
         const int kend = min(kb + kStage, k1);
         float fx[3] = {0, 0, 0}, fu[6] = {0, 0, 0, 0, 0, 0}, focc = 0;
+        float fB[BO ? NLM : 1];
+#pragma unroll
+        for (int j = 0; j < (BO ? NLM : 1); j++) fB[j] = 0;
+        float fpv = 0, fka = 0, fkp = 0;
         for (int k = kb; k < kend; ++k) {
             const float* r = hrec + (size_t)k * kHklopRec;
             // wave-uniform: every lane is on reflection k
@@ -2128,8 +2150,47 @@ __device__ __forceinline__ void body_mixed_line_diag(const MainArgs& a, int wave
                 S = rr[0] + anr; G0 = rr[1]; G1 = rr[2]; G2 = rr[3]; G3 = rr[4]; G4 = rr[5];
             }
             const float cr = (float)ld_f64(dtdf32, 2 * k), ci = (float)ld_f64(dtdf32, 2 * k + 1);
+            if constexpr (BO) {
+                // B_j += G'_l d_j(h) V[class(j)], V the class-signed sums of Re (even classes) or
+                // Im (odd classes) of c E_s
+                float cer[NOPS], cei[NOPS];
+#pragma unroll
+                for (int s = 0; s < NOPS; s++) {
+                    cer[s] = cr * Er[s] - ci * Ei[s];
+                    cei[s] = cr * Ei[s] + ci * Er[s];
+                }
+                float Vr[1 << key_dim<NOPS, PATS>()], Vi[1 << key_dim<NOPS, PATS>()];
+                class_sums_all<NOPS, PATS>(cer, Vr);
+                class_sums_all<NOPS, PATS>(cei, Vi);
+                const float Gs[5] = {G0, -G1, G2, -G3, G4};
+                static_for<0, NLM>([&](auto Ji) {
+                    constexpr int j = decltype(Ji)::value;
+                    constexpr int c = harmonic_parity(j), q = class_coords<NOPS, PATS>(c);
+                    const float v = (__builtin_popcount((unsigned)c) & 1) ? Vi[q] : Vr[q];
+                    fB[j] += (Gs[harmonic_l(j)] * r[j]) * v;
+                });
+                continue;
+            }
+            float C[5][1 << NOPS];
+            keyed_bins<L, NOPS, PATS, float>(P, r, C);
             float frs[NOPS], fis[NOPS];
-            keyed_form_factors<L, NOPS, PATS, float>(P, r, S, G0, G1, G2, G3, G4, frs, fis);
+            keyed_weights<L, NOPS, PATS, float>(C, S, G0, G1, G2, G3, G4, frs, fis);
+            if constexpr (SC) {
+                // kappa' sums: the form factors with dG_l / dkappa' (no spherical or anomalous
+                // term) against c E_s; Pval and kappa sums: f_val and dS / dkappa against sum Re(c E_s)
+                const float* rx = radx + ((size_t)k * a.n_types + t) * kRadRec;
+                float dfr[NOPS], dfi[NOPS];
+                keyed_weights<L, NOPS, PATS, float>(C, 0.f, rx[2], rx[3], rx[4], rx[5], rx[6], dfr, dfi);
+                float Rh = 0.f;
+#pragma unroll
+                for (int s = 0; s < NOPS; s++) {
+                    const float cer = cr * Er[s] - ci * Ei[s], cei = cr * Ei[s] + ci * Er[s];
+                    Rh += cer;
+                    fkp += dfr[s] * cer - dfi[s] * cei;
+                }
+                fpv += rx[0] * Rh;
+                fka += rx[1] * Rh;
+            }
             // X_s = f_s E_s; the sums over operations that F and the derivatives need are the
             // terms signed by a parity class (the ADP cross terms xy, xz, yz and the x, y, z
             // components change sign with the operation like the harmonics of that class), so
@@ -2161,22 +2222,44 @@ __device__ __forceinline__ void body_mixed_line_diag(const MainArgs& a, int wave
             }
             stage_f32(stage, k - kb, lane, w * sr, w * si);
         }
-        if (DERIV) {
+        if (DERIV && !BO) {
 #pragma unroll
             for (int j = 0; j < 3; j++) dx[j] += fx[j];
 #pragma unroll
             for (int j = 0; j < 6; j++) du[j] += fu[j];
             docc += focc;
         }
-        flush_f32(stage, kend - kb, lane, reinterpret_cast<float*>(fpart) + ((size_t)wave * a.n_hkl + kb) * 2);
+        if (BO) {
+#pragma unroll
+            for (int j = 0; j < (BO ? NLM : 1); j++) B[j] += fB[j];
+        }
+        if (SC) {
+            dpv += fpv;
+            dka += fka;
+            dkp += fkp;
+        }
+        if (!BO) flush_f32(stage, kend - kb, lane, reinterpret_cast<float*>(fpart) + ((size_t)wave * a.n_hkl + kb) * 2);
     }
     if (DERIV) {
         const double inv_log2e = 0.69314718055994530942;
-        double* dp = dpart + ((size_t)chunk_id * a.n_slots + slot) * kNdBasic;
-        dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
+        constexpr int ND = MP ? kNdMultipole : kNdBasic;
+        double* dp = dpart + ((size_t)chunk_id * a.n_slots + slot) * ND;
+        if (!BO) {
+            dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
 #pragma unroll
-        for (int j = 0; j < 6; j++) dp[3 + j] = du[j] * inv_log2e;
-        dp[9] = docc;
+            for (int j = 0; j < 6; j++) dp[3 + j] = du[j] * inv_log2e;
+            dp[9] = docc;
+        }
+        if (BO) {
+#pragma unroll
+            for (int j = 0; j < 25; j++) dp[10 + j] = j < NLM ? B[j < NLM ? j : 0] : 0.0;
+        }
+        if (SC) {
+            dp[35] = dpv;
+            dp[36] = dka;
+            dp[37] = dkp;
+        }
+        if (MP == 3) dp[35] = dp[36] = dp[37] = 0.0;
     }
 }
 
@@ -2325,7 +2408,7 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
 {
     static_assert(MIXED || !DIAG, "the parity-folded kernels are mixed precision only");
     static_assert(!(MP && OTF), "the multipole adjoint uses radial records");
-    static_assert(NOPS == 0 || MP == 0, "the line recurrences run without the multipole adjoint");
+    static_assert(NOPS == 0 || MP == 0 || (MIXED && DIAG), "multipole passes along lines: mixed parity-folded only");
     static_assert(!(MIXED && DIAG && NOPS > 0 && DPATS == 0), "a mixed diagonal line kernel needs its sign patterns");
     const int wave = ablock * kWavesPerBlock + wib;
     if (wave >= a.n_waves) return;
@@ -2347,11 +2430,11 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
             constexpr int PS = DPATS & 4095;
             constexpr bool LN = (DPATS & kDiagDirect) == 0;
             if (cls <= 2)
-                body_mixed_line_diag<2, DERIV, OTF, N, PS, LN>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, ht32,
-                                                               dtdf32, st, fpart, dpart);
+                body_mixed_line_diag<2, DERIV, OTF, N, PS, LN, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx,
+                                                                   phs, ht32, dtdf32, st, fpart, dpart);
             else
-                body_mixed_line_diag<4, DERIV, OTF, N, PS, LN>(a, wave, lane, chunk_id, k0, k1, hop, rad, phs, ht32,
-                                                               dtdf32, st, fpart, dpart);
+                body_mixed_line_diag<4, DERIV, OTF, N, PS, LN, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx,
+                                                                   phs, ht32, dtdf32, st, fpart, dpart);
         } else if constexpr (NOPS > 0) {
             constexpr int N = NOPS > 0 ? NOPS : 1;
             if (cls <= 2)
@@ -3779,7 +3862,22 @@ static const void* main_kernel(bool mixed, bool derivs, int mp_pass, bool diag, 
 #define KD(M, N, C) (derivs ? (const void*)k_main<M, true, 0, M, false, N, C> : (const void*)k_main<M, false, 0, M, false, N, C>)
     // line kernels (radial records only: reflections are ordered along lines only on that path):
     // FP64 diagonal groups take per-reflection FP64 records, mixed diagonal groups the
-    // parity-folded FP32 records; other groups take the general records of their precision
+    // parity-folded FP32 records; other groups take the general records of their precision.
+    // The multipole passes run along lines in the mixed parity-folded kernels only.
+    if (diag64 > 0 && mp_pass > 0 && !otf && mixed && diag && (diag64 & kDiagDirect) == 0) {
+#define KP(N, C) (mp_pass == 1 ? (const void*)k_main<true, true, 1, true, false, N, C>                             \
+                               : mp_pass == 2 ? (const void*)k_main<true, true, 2, true, false, N, C>              \
+                                              : (const void*)k_main<true, true, 3, true, false, N, C>)
+        switch (diag64) {
+        case kDiag64Pats0: return KP(2, kDiag64Pats0);
+        case kDiag64Pats1: return KP(2, kDiag64Pats1);
+        case kDiag64Pats2: return KP(2, kDiag64Pats2);
+        case kDiag64Pats3: return KP(2, kDiag64Pats3);
+        case kDiag64Pats4: return KP(4, kDiag64Pats4);
+        default: break;
+        }
+#undef KP
+    }
     if (diag64 > 0 && mp_pass == 0 && !otf && (!mixed || diag)) {
         switch (diag64) {
         case kDiag64Pats0: return mixed ? KD(true, 2, kDiag64Pats0) : KD(false, 2, kDiag64Pats0);

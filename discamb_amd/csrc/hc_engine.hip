@@ -1585,14 +1585,21 @@ __host__ __device__ constexpr int class_coords(int c)
     return key_coords<NOPS, PATS>(sign_key<NOPS, PATS>(c));
 }
 
-template <int L, bool DERIV, bool OTF, int NOPS, int PATS>
+// MP: the passes of the multipole adjoint as in body_fp64 (1: F, xyz/U/occupancy and the Pval,
+// kappa, kappa' sums; 2: the B_lm sums only, the class-signed sums of c E_s; 3: the plain pass in
+// the multipole row layout).
+template <int L, bool DERIV, bool OTF, int NOPS, int PATS, int MP = 0>
 __device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave, int lane, int chunk_id, int k0,
                                                     int k1, const unsigned* __restrict__ hop,
-                                                    const double* __restrict__ rad, const unsigned* __restrict__ dtdf32,
+                                                    const double* __restrict__ rad, const double* __restrict__ radx,
+                                                    const unsigned* __restrict__ dtdf32,
                                                     double2* __restrict__ stage, double* __restrict__ fpart,
                                                     double* __restrict__ dpart)
 {
     constexpr int NLM = (L + 1) * (L + 1);
+    constexpr bool SC = MP == 1;
+    constexpr bool BO = MP == 2;
+    static_assert(!(MP && OTF), "the multipole adjoint uses radial records");
     const int slot = wave * kWave + lane;
     const AtomRec& at = cmem(a.atoms)[slot];
     const double x[3] = {at.xf[0], at.xf[1], at.xf[2]};
@@ -1635,6 +1642,10 @@ __device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave,
         an[s] = (double)a.diag_anom[(PATS >> (3 * s)) & 7] * ani;
     }
     double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
+    double B[BO ? NLM : 1];
+#pragma unroll
+    for (int j = 0; j < (BO ? NLM : 1); j++) B[j] = 0;
+    double dpv = 0, dka = 0, dkp = 0;
     for (int kb = k0; kb < k1; kb += kStage) {
     const int nk = min(kStage, k1 - kb);
     for (int kk = 0; kk < nk; ++kk) {
@@ -1701,6 +1712,35 @@ __device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave,
         double Y[25];
 #pragma unroll
         for (int j = 0; j < 25; j++) Y[j] = j < NLM ? ld_f64(rw, j) : 0.0;
+        if constexpr (BO) {
+            // B_j += G'_l d_j(h) V[class(j)], V the class-signed sums of Re (even classes) or Im
+            // (odd classes) of c E_s
+            double cer[NOPS], cei[NOPS];
+#pragma unroll
+            for (int s = 0; s < NOPS; s++) {
+                cer[s] = cr * Er[s] - ci * Ei[s];
+                cei[s] = cr * Ei[s] + ci * Er[s];
+            }
+            double Vr[1 << key_dim<NOPS, PATS>()], Vi[1 << key_dim<NOPS, PATS>()];
+            class_sums_all<NOPS, PATS>(cer, Vr);
+            class_sums_all<NOPS, PATS>(cei, Vi);
+            const double Gs[5] = {G0, -G1, G2, -G3, G4};
+            static_for<0, NLM>([&](auto Ji) {
+                constexpr int j = decltype(Ji)::value;
+                constexpr int c = harmonic_parity(j), q = class_coords<NOPS, PATS>(c);
+                const double v = (__builtin_popcount((unsigned)c) & 1) ? Vi[q] : Vr[q];
+                B[j] = fma(Gs[harmonic_l(j)] * Y[j], v, B[j]);
+            });
+            continue;
+        }
+        double dG[5] = {0, 0, 0, 0, 0}, fval = 0, dSk = 0, Rh = 0;
+        if constexpr (SC) {
+            const double* rx = radx + ((size_t)k * a.n_types + t) * kRadRec;
+            fval = rx[0];
+            dSk = rx[1];
+#pragma unroll
+            for (int l = 0; l < 5; l++) dG[l] = rx[2 + l];
+        }
         double Xr[NOPS], Xi[NOPS];
 #pragma unroll
         for (int s = 0; s < NOPS; s++) {
@@ -1723,6 +1763,15 @@ __device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave,
             }
             Xr[s] = fr * Er[s] - fi * Ei[s];
             Xi[s] = fr * Ei[s] + fi * Er[s];
+            if constexpr (SC) {
+                const double cer = cr * Er[s] - ci * Ei[s], cei = cr * Ei[s] + ci * Er[s];
+                Rh += cer;
+                dkp += (dG[0] * A[0] + dG[2] * A[2] + dG[4] * A[4]) * cer - (dG[1] * A[1] + dG[3] * A[3]) * cei;
+            }
+        }
+        if constexpr (SC) {
+            dpv += fval * Rh;
+            dka += dSk * Rh;
         }
         // sums over the operations signed by parity class (as in the mixed line kernel)
         double Wr[1 << key_dim<NOPS, PATS>()], Wi[1 << key_dim<NOPS, PATS>()];
@@ -1745,14 +1794,27 @@ __device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave,
         }
         stage_f64(stage, kk, lane, w * sr, w * si);
     }
-    flush_f64(stage, nk, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
+    if (!BO) flush_f64(stage, nk, lane, fpart + ((size_t)wave * a.n_hkl + kb) * 2);
     }
     if (DERIV) {
-        double* dp = dpart + ((size_t)chunk_id * a.n_slots + slot) * kNdBasic;
-        dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
+        constexpr int ND = MP ? kNdMultipole : kNdBasic;
+        double* dp = dpart + ((size_t)chunk_id * a.n_slots + slot) * ND;
+        if (!BO) {
+            dp[0] = dx[0]; dp[1] = dx[1]; dp[2] = dx[2];
 #pragma unroll
-        for (int j = 0; j < 6; j++) dp[3 + j] = du[j];
-        dp[9] = docc;
+            for (int j = 0; j < 6; j++) dp[3 + j] = du[j];
+            dp[9] = docc;
+        }
+        if (BO) {
+#pragma unroll
+            for (int j = 0; j < 25; j++) dp[10 + j] = j < NLM ? B[j < NLM ? j : 0] : 0.0;
+        }
+        if (SC) {
+            dp[35] = dpv;
+            dp[36] = dka;
+            dp[37] = dkp;
+        }
+        if (MP == 3) dp[35] = dp[36] = dp[37] = 0.0;
     }
 }
 
@@ -2408,7 +2470,8 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
 {
     static_assert(MIXED || !DIAG, "the parity-folded kernels are mixed precision only");
     static_assert(!(MP && OTF), "the multipole adjoint uses radial records");
-    static_assert(NOPS == 0 || MP == 0 || (MIXED && DIAG), "multipole passes along lines: mixed parity-folded only");
+    static_assert(NOPS == 0 || MP == 0 || (MIXED && DIAG) || (!MIXED && DPATS > 0),
+                  "multipole passes along lines: parity-folded (mixed) and diagonal-record (FP64) kernels only");
     static_assert(!(MIXED && DIAG && NOPS > 0 && DPATS == 0), "a mixed diagonal line kernel needs its sign patterns");
     const int wave = ablock * kWavesPerBlock + wib;
     if (wave >= a.n_waves) return;
@@ -2464,11 +2527,11 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
             constexpr int N = NOPS > 0 ? NOPS : 1;
             constexpr int PS = DPATS & 4095;
             if (cls <= 2)
-                body_fp64_line_diag<2, DERIV, OTF, N, PS>(a, wave, lane, chunk_id, k0, k1, hop, rad, dtdf32, st, fpart,
-                                                          dpart);
+                body_fp64_line_diag<2, DERIV, OTF, N, PS, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, dtdf32,
+                                                              st, fpart, dpart);
             else
-                body_fp64_line_diag<4, DERIV, OTF, N, PS>(a, wave, lane, chunk_id, k0, k1, hop, rad, dtdf32, st, fpart,
-                                                          dpart);
+                body_fp64_line_diag<4, DERIV, OTF, N, PS, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, dtdf32,
+                                                              st, fpart, dpart);
         } else if constexpr (NOPS > 0) {
             constexpr int N = NOPS > 0 ? NOPS : 1;
             if (cls <= 2)
@@ -3864,10 +3927,11 @@ static const void* main_kernel(bool mixed, bool derivs, int mp_pass, bool diag, 
     // FP64 diagonal groups take per-reflection FP64 records, mixed diagonal groups the
     // parity-folded FP32 records; other groups take the general records of their precision.
     // The multipole passes run along lines in the mixed parity-folded kernels only.
-    if (diag64 > 0 && mp_pass > 0 && !otf && mixed && diag && (diag64 & kDiagDirect) == 0) {
-#define KP(N, C) (mp_pass == 1 ? (const void*)k_main<true, true, 1, true, false, N, C>                             \
-                               : mp_pass == 2 ? (const void*)k_main<true, true, 2, true, false, N, C>              \
-                                              : (const void*)k_main<true, true, 3, true, false, N, C>)
+    if (diag64 > 0 && mp_pass > 0 && !otf && (!mixed || diag) && (diag64 & kDiagDirect) == 0) {
+#define KPM(M, N, C) (mp_pass == 1 ? (const void*)k_main<M, true, 1, M, false, N, C>                              \
+                                   : mp_pass == 2 ? (const void*)k_main<M, true, 2, M, false, N, C>               \
+                                                  : (const void*)k_main<M, true, 3, M, false, N, C>)
+#define KP(N, C) (mixed ? KPM(true, N, C) : KPM(false, N, C))
         switch (diag64) {
         case kDiag64Pats0: return KP(2, kDiag64Pats0);
         case kDiag64Pats1: return KP(2, kDiag64Pats1);
@@ -3877,6 +3941,7 @@ static const void* main_kernel(bool mixed, bool derivs, int mp_pass, bool diag, 
         default: break;
         }
 #undef KP
+#undef KPM
     }
     if (diag64 > 0 && mp_pass == 0 && !otf && (!mixed || diag)) {
         switch (diag64) {

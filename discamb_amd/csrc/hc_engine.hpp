@@ -281,7 +281,7 @@ private:
         if (tune.direct_phases || last_order_ != 2 || !line_ops_ok() || use_otf(mp)) return false;
         // the mixed parity-folded line kernels exist for the compiled sign-pattern sets only, and
         // carry the multipole passes too; the other line kernels run without them
-        if (mp) return use_diag(mixed, mp) && diag64_code() != 0;
+        if (mp) return diag_ && !tune.general_path && diag64_code() != 0;
         return !use_diag(mixed, mp) || diag64_code() != 0;
     }
     // the FP64 line kernel with per-reflection records for diagonal groups: its sign-pattern code, 0

@@ -24,14 +24,20 @@ def _oracle_T(w, params, dtdf):
     return float(np.sum((np.conj(dtdf) * F).real))
 
 
+@pytest.mark.parametrize("radial", ["auto", "records"])
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
 @pytest.mark.parametrize("sg", ["P21/c", "P212121"])
-def test_multipole_derivatives_vs_oracle_finite_differences(sg, mode):
+def test_multipole_derivatives_vs_oracle_finite_differences(sg, mode, radial):
+    """Multipole-parameter derivatives against central differences of the oracle.  radial
+    "records" orders the reflections along hkl lines, so both multipole passes run the line
+    kernels (2/m and 222 sign sets); "auto" takes per-atom table lookups in |h| order for the
+    first pass' F and the direct kernels."""
     import copy
     w = synth.small_structure(sg, n_atoms=8, d_min=1.2)
-    print("case multipole fd", sg, mode, flush=True)
+    print("case multipole fd", sg, mode, radial, flush=True)
     calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
     calc.setCalculationMode(mode)
+    calc.setKernelOptions(radial=radial)
     sel = DerivativesSelector(True, True, True, False, True)
     f, d = calc.calculateStructureFactorsAndDerivatives(w.crystal.atoms, w.lcs, w.hkl, w.dtdf, None, sel)
     td = calc.typeDerivatives()

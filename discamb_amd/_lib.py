@@ -151,6 +151,14 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("discamb_amd: %s is not built; run __graft_entry__.build() (make -C discamb_amd)"
                            % LIB_PATH)
+    # PyTorch-ROCm wheels carry their own libamdhip64 / libhsa-runtime64, which torch's libraries
+    # name without the version suffix.  Loaded after this library, torch would map a second HIP and
+    # HSA runtime into the process, and that second runtime finds no GPU.  Importing torch first
+    # lets this library's libamdhip64.so.7 bind to torch's copy, so the process has one runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

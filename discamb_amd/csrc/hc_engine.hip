@@ -2556,7 +2556,7 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
 // when only the B_lm are wanted (the chain rule through the local frames): the plain pass in the
 // multipole row layout.
 template <bool MIXED, bool DERIV, int MP, bool DIAG, bool OTF, int NOPS = 0, int DPATS = 0>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && (MP == 0 || MP == 3) ? (DIAG || OTF || NOPS > 0 ? 4 : 5) : (!MIXED && MP == 2 ? 3 : 1)))) void k_main(MainArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && (MP == 0 || MP == 3) ? (DIAG || OTF || NOPS > 0 ? 4 : 5) : (!MIXED && MP == 2 ? 3 : (!MIXED && MP == 1 && NOPS > 0 ? 2 : 1))))) void k_main(MainArgs a)
 {
     constexpr int kStageBytes = MIXED ? kStageBytesF32 : kStageBytesF64;
     __shared__ __align__(16) unsigned char stage_lds[kWavesPerBlock * kStageBytes];

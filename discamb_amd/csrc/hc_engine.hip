@@ -395,6 +395,53 @@ __device__ __forceinline__ void radial_otf(const R* __restrict__ tab, TabRec tr,
     }
 }
 
+// radial_otf on the FP32 linear tables with each lane's last core, valence and deformation
+// intervals kept in registers.  Reflections in |h| order fall into one interval (kLinStep wide)
+// for several consecutive reflections, so a lane reloads an interval only when its index
+// changes: a divergent load, issued for the lanes whose interval moved.  Without it every lane
+// gathers two or three table lines per reflection, a working set (64 lanes x 16 waves per CU)
+// that overflows L1.  The values are radial_otf's, from the same table entries.
+template <int L>
+struct LinTabCache {
+    int ic = -1, iv = -1, id = -1;
+    float c0 = 0.f, c1 = 0.f, v0 = 0.f, v1 = 0.f;
+    float d[2 * (L + 1)];
+
+    __device__ __forceinline__ void eval(const float* __restrict__ tab, TabRec tr, float h, float pval,
+                                         float inv_kappa, float inv_kappa_p, float anr, float& S, float (&G)[5])
+    {
+        constexpr float inv_lstep = float(1.0 / kLinStep);
+        const int nl = kLinRatio * tr.n;
+        const float* base = tab + (size_t)kLinRatio * tr.off * kTabNode;
+        const float xc = h * inv_lstep;
+        const int jc = min((int)xc, nl - 2);
+        if (jc != ic) {
+            ic = jc;
+            c0 = base[(size_t)jc * kTabNode];
+            c1 = base[(size_t)jc * kTabNode + 1];
+        }
+        const float xv = h * (inv_kappa * inv_lstep);
+        const int jv = min((int)xv, nl - 2);
+        if (jv != iv) {
+            iv = jv;
+            v0 = base[(size_t)jv * kTabNode + 2];
+            v1 = base[(size_t)jv * kTabNode + 3];
+        }
+        S = (c0 + (xc - (float)jc) * c1) + pval * (v0 + (xv - (float)jv) * v1) + anr;
+        const float xd = h * (inv_kappa_p * inv_lstep);
+        const int jd = min((int)xd, nl - 2);
+        if (jd != id) {
+            id = jd;
+            const float* a = base + (size_t)jd * kTabNode + 4;
+#pragma unroll
+            for (int j = 0; j < 2 * (L + 1); j++) d[j] = a[j];
+        }
+        const float t = xd - (float)jd;
+#pragma unroll
+        for (int l = 0; l < 5; l++) G[l] = l <= L ? d[2 * l] + t * d[2 * l + 1] : 0.f;
+    }
+};
+
 // Per (reflection, type): S = f_core + Pval f_val(h/kappa) and
 // G_l = 4 pi (i^l sign) N_l g_l(n_l + 2, h/kappa', zeta), l <= lp.  With radx set (multipole
 // adjoint) f_val and its derivative are summed in closed form and the record
@@ -1121,6 +1168,7 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
         ikp = (float)at.inv_kappa_p;
     }
     const float* tab = cmem(static_cast<const float*>(a.tab));
+    LinTabCache<L> rtc;    // the lane's last table intervals (reflections in |h| order)
     const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
     double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
     double B[BO ? NLM : 1];
@@ -1139,7 +1187,7 @@ __device__ __forceinline__ void body_mixed(const MainArgs& a, int wave, int lane
             float S, G0, G1, G2, G3, G4;
             if (OTF) {
                 float G[5];
-                radial_otf<L, float>(tab, tr, (float)ld_f64(hl32, k), pv, ik, ikp, anr, S, G);
+                rtc.eval(tab, tr, (float)ld_f64(hl32, k), pv, ik, ikp, anr, S, G);
                 G0 = G[0]; G1 = G[1]; G2 = G[2]; G3 = G[3]; G4 = G[4];
             } else {
                 const float* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
@@ -1927,6 +1975,7 @@ __device__ __forceinline__ void body_mixed_diag(const MainArgs& a, int wave, int
         ikp = (float)at.inv_kappa_p;
     }
     const float* tab = cmem(static_cast<const float*>(a.tab));
+    LinTabCache<L> rtc;    // the lane's last table intervals (reflections in |h| order)
     const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
     double dx[3] = {0, 0, 0}, du[6] = {0, 0, 0, 0, 0, 0}, docc = 0;
     double B[BO ? NLM : 1];
@@ -1945,7 +1994,7 @@ __device__ __forceinline__ void body_mixed_diag(const MainArgs& a, int wave, int
             DiagRefl<float> q;
             if (OTF) {
                 float G[5];
-                radial_otf<L, float>(tab, tr, (float)ld_f64(hl32, k), pv, ik, ikp, anr, q.S, G);
+                rtc.eval(tab, tr, (float)ld_f64(hl32, k), pv, ik, ikp, anr, q.S, G);
                 q.G0 = G[0]; q.G1 = G[1]; q.G2 = G[2]; q.G3 = G[3]; q.G4 = G[4];
             } else {
                 const float* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
@@ -2118,6 +2167,7 @@ __device__ __forceinline__ void body_mixed_line_diag(const MainArgs& a, int wave
         ikp = (float)at.inv_kappa_p;
     }
     const float* tab = cmem(static_cast<const float*>(a.tab));
+    LinTabCache<L> rtc;    // the lane's last table intervals (reflections in |h| order)
     const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
     const int* cont = LINES ? cmem(a.line_cont) : nullptr;
     float Mr[NOPS], Mi[NOPS], Cq[NOPS], Er[NOPS], Ei[NOPS], Q[NOPS], an[NOPS];
@@ -2205,7 +2255,7 @@ __device__ __forceinline__ void body_mixed_line_diag(const MainArgs& a, int wave
             float S, G0, G1, G2, G3, G4;
             if (OTF) {
                 float G[5];
-                radial_otf<L, float>(tab, tr, (float)ld_f64(hl32, k), pv, ik, ikp, anr, S, G);
+                rtc.eval(tab, tr, (float)ld_f64(hl32, k), pv, ik, ikp, anr, S, G);
                 G0 = G[0]; G1 = G[1]; G2 = G[2]; G3 = G[3]; G4 = G[4];
             } else {
                 const float* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
@@ -2355,6 +2405,7 @@ __device__ __forceinline__ void body_mixed_line(const MainArgs& a, int wave, int
         ikp = (float)at.inv_kappa_p;
     }
     const float* tab = cmem(static_cast<const float*>(a.tab));
+    LinTabCache<L> rtc;    // the lane's last table intervals (reflections in |h| order)
     const unsigned* hl32 = cmem(reinterpret_cast<const unsigned*>(a.hlen));
     const int* cont = cmem(a.line_cont);
     float Mr[NOPS], Mi[NOPS], Cq[NOPS], Er[NOPS], Ei[NOPS], Q[NOPS];
@@ -2411,7 +2462,7 @@ __device__ __forceinline__ void body_mixed_line(const MainArgs& a, int wave, int
             float S, G0, G1, G2, G3, G4;
             if (OTF) {
                 float G[5];
-                radial_otf<L, float>(tab, tr, (float)ld_f64(hl32, k), pv, ik, ikp, anr, S, G);
+                rtc.eval(tab, tr, (float)ld_f64(hl32, k), pv, ik, ikp, anr, S, G);
                 G0 = G[0]; G1 = G[1]; G2 = G[2]; G3 = G[3]; G4 = G[4];
             } else {
                 const float* rr = rad + ((size_t)k * a.n_types + t) * kRadRec;
@@ -4952,9 +5003,12 @@ __global__ __launch_bounds__(64) void k_batch_atom_prep(const BatchItem* __restr
 // offsets are uoff + x * stride.  In each queue units are ordered costliest structure first, and
 // within a structure chunk by chunk.  Radial factors are evaluated on the fly from the
 // per-wavefunction tables (no per-(reflection, type) records); reflections are in |h| order, so
-// every term is evaluated directly.
+// every term is evaluated directly, and the mixed bodies keep each lane's table intervals in
+// registers (LinTabCache).  Mixed bodies run at four waves per SIMD (128 VGPRs), which hides the
+// table gathers' latency better than three waves without the spills; the body with run-time
+// sign patterns would spill at four and runs at three.
 template <bool MIXED, bool DERIV, int BODY>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? 3 : 2))) void k_batch_main(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? (BODY == kBodyDiag ? 3 : 4) : 2))) void k_batch_main(
     const BatchItem* __restrict__ it, const int* __restrict__ uoff, int stride, int nb, int* __restrict__ counter)
 {
     constexpr int kStageBytes = MIXED ? kStageBytesF32 : kStageBytesF64;

@@ -2605,9 +2605,12 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
 // xyz/U/occupancy derivatives and the Pval, kappa, kappa' sums; 2: the 25 global-frame B_lm sums),
 // split so that neither pass holds both the atom's P_lm and the B_lm accumulators; 3 replaces 1
 // when only the B_lm are wanted (the chain rule through the local frames): the plain pass in the
-// multipole row layout.
+// multipole row layout.  Waves per SIMD the register allocation must allow: mixed plain passes
+// four (five for the general-rotation kernel); mixed multipole passes along lines four, except
+// pass 1 of a four-operation set (it would spill 39 VGPRs at four); FP64 pass 1 along lines two,
+// FP64 pass 2 three; the others uncapped.
 template <bool MIXED, bool DERIV, int MP, bool DIAG, bool OTF, int NOPS = 0, int DPATS = 0>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && (MP == 0 || MP == 3) ? (DIAG || OTF || NOPS > 0 ? 4 : 5) : (!MIXED && MP == 2 ? 3 : (!MIXED && MP == 1 && NOPS > 0 ? 2 : 1))))) void k_main(MainArgs a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED && (MP == 0 || MP == 3) ? (DIAG || OTF || NOPS > 0 ? 4 : 5) : MIXED && NOPS > 0 && (MP == 2 || NOPS == 2) ? 4 : (!MIXED && MP == 2 ? 3 : (!MIXED && MP == 1 && NOPS > 0 ? 2 : 1))))) void k_main(MainArgs a)
 {
     constexpr int kStageBytes = MIXED ? kStageBytesF32 : kStageBytesF64;
     __shared__ __align__(16) unsigned char stage_lds[kWavesPerBlock * kStageBytes];

@@ -553,9 +553,9 @@ def bench_autograd(args):
     5000-atom structure: Fobs from a perturbed copy (xyz + N(0, 0.02 A), U x U(0.9, 1.1)); a step
     is the forward F (F-only kernel), the loss and the backward pass (fused adjoint with the
     multipole-parameter derivatives), gradients w.r.t. xyz, U, occupancy, P_lm, Pval, kappa and
-    kappa'.  The full-parameter step keeps its tensors on the host (the multipole derivatives take the
-    host call); a second step with the structural parameters as GPU tensors runs the device-resident
-    calls."""
+    kappa'.  Three steps per precision: every parameter as a GPU tensor (device-resident forward and
+    backward, the multipole adjoint and its per-type sums on the device: the line's value), the same
+    with host tensors (host calls), and the structural parameters alone as GPU tensors."""
     import torch
     from discamb_amd import synth
     from discamb_amd.torch_ops import StructureFactors
@@ -572,69 +572,86 @@ def bench_autograd(args):
     with torch.no_grad():
         fobs = ref(*pert).abs()
     del ref
-    res, dev = {}, {}
+    res, gpu_all, dev = {}, {}, {}
     fobs_dev = fobs.to("cuda")
-    for mode in ("HIP_MIXED", "HIP"):
-        sf = StructureFactors(w.crystal, w.params, w.lcs, w.hkl, mode=mode)
-        params = sf.initial(w.params)
 
-        def step():
-            for p in params:
-                p.grad = None
-            T = ((sf(*params).abs() - fobs) ** 2).sum()
-            T.backward()
-            return T
-
+    def timed(run, sync):
         for _ in range(args.warmup):
-            step()
-        sf.calc.setTiming(True)
-        ms, kms = [], []
-        for _ in range(args.steps):
-            t0 = time.perf_counter()
-            step()
-            ms.append((time.perf_counter() - t0) * 1e3)
-            kms.append(sf.calc.lastTiming()["main_ms"])
-        res[mode] = (float(np.median(ms)), float(np.mean(kms)))
-        print("autograd %s: %.3f ms/step (median), backward main kernel %.3f ms" % (mode, *res[mode]),
-              file=sys.stderr, flush=True)
-        sf.calc.setTiming(False)
-        # structural parameters as GPU tensors: device-resident forward and backward
-        dparams = sf.initial(w.params, device="cuda")
-        for p in dparams[3:]:
-            p.requires_grad_(False)
-
-        def dstep():
-            for p in dparams[:3]:
-                p.grad = None
-            T = ((sf(*dparams).abs() - fobs_dev) ** 2).sum()
-            T.backward()
-            return T
-
-        for _ in range(args.warmup):
-            dstep()
-        torch.cuda.synchronize()
+            run()
+        if sync:
+            torch.cuda.synchronize()
         ms = []
         for _ in range(args.steps):
             t0 = time.perf_counter()
-            dstep()
-            torch.cuda.synchronize()
+            run()
+            if sync:
+                torch.cuda.synchronize()
             ms.append((time.perf_counter() - t0) * 1e3)
-        dev[mode] = float(np.median(ms))
+        return float(np.median(ms))
+
+    for mode in ("HIP_MIXED", "HIP"):
+        sf = StructureFactors(w.crystal, w.params, w.lcs, w.hkl, mode=mode)
+
+        def backward_kernel_ms(step):
+            # main kernels of the backward call (the step's last engine call), from one extra step
+            # with HIP-event timing (host calls only: device-resident calls are not event-timed)
+            sf.calc.setTiming(True)
+            step()
+            torch.cuda.synchronize()
+            kms = sf.calc.lastTiming()["main_ms"]
+            sf.calc.setTiming(False)
+            return kms
+
+        def stepper(params, fo):
+            def step():
+                for p in params:
+                    p.grad = None
+                T = ((sf(*params).abs() - fo) ** 2).sum()
+                T.backward()
+                return T
+            return step
+
+        # every parameter as a GPU tensor: device-resident forward and backward, multipole
+        # gradients (per-type sums) formed on the device
+        gparams = sf.initial(w.params, device="cuda")
+        gpu_all[mode] = timed(stepper(gparams, fobs_dev), True)
+        print("autograd %s, GPU tensors, all parameters: %.3f ms/step (median)" % (mode, gpu_all[mode]),
+              file=sys.stderr, flush=True)
+        # the same step with the parameters on the host: host calls for forward and backward
+        params = sf.initial(w.params)
+        res[mode] = (timed(stepper(params, fobs), False), backward_kernel_ms(stepper(params, fobs)))
+        print("autograd %s, host tensors, all parameters: %.3f ms/step (median), backward main kernel %.3f ms" % (
+            mode, *res[mode]), file=sys.stderr, flush=True)
+        # structural parameters as GPU tensors: the plain adjoint, no multipole passes
+        dparams = sf.initial(w.params, device="cuda")
+        for p in dparams[3:]:
+            p.requires_grad_(False)
+        dev[mode] = timed(stepper(dparams, fobs_dev), True)
         print("autograd %s, GPU tensors, xyz/U/occ: %.3f ms/step (median)" % (mode, dev[mode]), file=sys.stderr,
               flush=True)
-    mx, fp = res["HIP_MIXED"], res["HIP"]
+    mx, fp = gpu_all["HIP_MIXED"], gpu_all["HIP"]
+    kmx, kfp = res["HIP_MIXED"][1], res["HIP"][1]
     wl = ("P212121 5000 atoms (C1575 N425 O475 S50 H2475), 40 multipole types, cell 50x60x70 A, 66592 unique "
           "hkl to 1.2 A, 4 explicit ops; least-squares target against a perturbed copy")
     line = {"metric": "atom*symop*hkl terms/s per torch gradient step (forward F + backward incl. multipole "
                       "parameters; mixed)",
-            "value": terms / (mx[0] * 1e-3), "unit": "terms/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": mx[0], "higher_is_better": True, "scaling": "weak",
+            "value": terms / (mx * 1e-3), "unit": "terms/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": mx, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32 terms, f64 phase reduction and accumulation",
             "data": "synthetic, seeded (numpy PCG64 seed 1000; perturbation seed 33)",
             "config": {"workload": wl, "terms_per_step": terms, "parallelism": "single GPU",
-                       "timed": "host step: forward call, loss, backward call, gradients on the host"},
-            "backward_main_kernel_ms": mx[1],
-            "fp64": {"value": terms / (fp[0] * 1e-3), "ms_per_step": fp[0], "backward_main_kernel_ms": fp[1]},
+                       "timed": "parameters as GPU tensors (xyz, U, occupancy, P_lm, Pval, kappa, kappa'): "
+                                "device-resident forward (F only) and backward (both multipole-adjoint passes, "
+                                "per-type sums on the device) on torch's stream; synchronised at the end of "
+                                "each step"},
+            "backward_main_kernel_ms": kmx,
+            "backward_main_kernel_note": "both multipole-adjoint passes of the backward call, HIP events, measured "
+                                         "on the host-tensor step (the same kernels)",
+            "fp64": {"value": terms / (fp * 1e-3), "ms_per_step": fp, "backward_main_kernel_ms": kfp},
+            "host_tensors": {
+                "timed": "the same step with the parameters as host tensors: host calls for forward and backward, "
+                         "gradients copied back",
+                "mixed_ms_per_step": res["HIP_MIXED"][0], "fp64_ms_per_step": res["HIP"][0]},
             "gpu_tensors_structural": {
                 "timed": "parameters as GPU tensors, gradients of xyz / U / occupancy only: device-resident "
                          "forward (F only) and backward (F + derivatives) on torch's stream, no host copies; "

@@ -118,6 +118,8 @@ SIGNATURES = {
     "dh_hc_sf_and_derivs_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DhAtomsDev), ctypes.c_void_p,
                                                ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_void_p, ctypes.c_void_p]),
+    "dh_hc_sf_and_derivs_mp_dev": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DhAtomsDev), ctypes.c_void_p,
+                                                  ctypes.c_uint] + [ctypes.c_void_p] * 9),
     "dh_hc_sf_and_derivs_dev_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DhAtomsDev), ctypes.c_void_p,
                                                        ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p,
                                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),

@@ -347,9 +347,15 @@ class HansenCoppensStructureFactorCalculator:
         countAtomContribution (n,) uint8.  Returns (F, dxyz, dadp, docc) as new GPU tensors (the
         derivatives None for F only), or writes them into `out` = (F, dxyz, dadp, docc) tensors.
         no_alloc: the allocation-free entry (dh_hc_sf_and_derivs_dev_noalloc; with `out` given
-        nothing is allocated on the Python side either)."""
+        nothing is allocated on the Python side either).  A selector with d_multipole runs
+        dh_hc_sf_and_derivs_mp_dev and appends the per-type multipole derivatives
+        (TypeParameterDerivatives of GPU tensors: p_lm (n_types, 25), p_val, kappa, kappa_prime);
+        it takes neither no_alloc nor `out`."""
         import torch
         sel = (derivativesSelector or DerivativesSelector()).bits() if dTarget_dF is not None else 0
+        mp = bool(sel & 8)
+        if mp and (no_alloc or out is not None):
+            raise ValueError("multipole derivatives on the device: no allocation-free form and no `out` tensors")
         a, keep, dev = self._dev_args(xyz, adp_size, adp, occupancy, localCoordinateSystems, anomalous,
                                       countAtomContribution)
         nh, n = getattr(self, "_dev_nh", 0), self.n_atoms
@@ -367,9 +373,18 @@ class HansenCoppensStructureFactorCalculator:
         if dTarget_dF is not None:
             dd = dev(torch.view_as_real(dTarget_dF), torch.float64, (nh, 2), "dTarget_dF")
         st = stream if stream is not None else torch.cuda.current_stream(device)
+        fptr = torch.view_as_real(f).data_ptr() if nh else None
+        optrs = [o.data_ptr() if o is not None else None for o in outs]
+        if mp:
+            nt = max(self.n_types, 1)
+            td = TypeParameterDerivatives(*(torch.zeros(s, dtype=torch.float64, device=device)
+                                            for s in ((nt, 25), (nt,), (nt,), (nt,))))
+            _lib.check(self._lib.dh_hc_sf_and_derivs_mp_dev(
+                self._h, ctypes.byref(a), dd, sel, fptr, *optrs, td.p_lm.data_ptr(), td.p_val.data_ptr(),
+                td.kappa.data_ptr(), td.kappa_prime.data_ptr(), ctypes.c_void_p(st.cuda_stream)))
+            return (f,) + tuple(outs) + (td,)
         fn = self._lib.dh_hc_sf_and_derivs_dev_noalloc if no_alloc else self._lib.dh_hc_sf_and_derivs_dev
-        _lib.check(fn(self._h, ctypes.byref(a), dd, sel, torch.view_as_real(f).data_ptr() if nh else None,
-                      *(o.data_ptr() if o is not None else None for o in outs), ctypes.c_void_p(st.cuda_stream)))
+        _lib.check(fn(self._h, ctypes.byref(a), dd, sel, fptr, *optrs, ctypes.c_void_p(st.cuda_stream)))
         return (f,) + tuple(outs)
 
     # -- per-reflection derivatives (the per-hkl path) -------------------------------------------

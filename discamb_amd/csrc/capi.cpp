@@ -600,19 +600,37 @@ static void set_reflections(Calculator* c, int n_hkl, const int* hkl)
     conv.centric_origin = c->eng_centric_origin() ? 1 : 0;
     c->eng().line_order = true;
     c->eng().prepare_dev(ea, n_hkl, hkl, mult.data(), c->mult_w, anom, conv, c->prec);
+    if (!c->iam) {
+        // the per-type sums of the multipole derivatives (dh_hc_sf_and_derivs_mp_dev)
+        std::vector<int> n_l(c->types.size());
+        for (size_t t = 0; t < c->types.size(); t++) n_l[t] = c->types[t].n_l;
+        c->eng().set_type_sum_map(c->atom_type, n_l);
+    }
     c->n_atoms = n;
     c->n_hkl = n_hkl;
 }
 
+static void check_selector(Calculator* c, unsigned selector);
+
+// mp: the multipole-parameter derivatives as well (dh_hc_sf_and_derivs_mp_dev), summed per type
+// into the four device arrays of `tsum`
 static void sf_dev(Calculator* c, const dh_atoms_dev* at, const double* dtdf, unsigned selector, double* f,
-                   double* d_xyz, double* d_adp, double* d_occ, void* stream, bool no_alloc)
+                   double* d_xyz, double* d_adp, double* d_occ, void* stream, bool no_alloc, bool mp = false,
+                   double* const* tsum = nullptr)
 {
     if (!at) throw Error(DH_E_ARGUMENT, "null atoms");
     if (at->n_atoms != (int)c->atom_type.size())
         throw Error(DH_E_ARGUMENT, "number of atoms differs from the model's atom-to-type map");
-    if (selector & ~DH_D_ALL)
-        throw Error(DH_E_ARGUMENT, "device calls take DH_D_XYZ, DH_D_ADP and DH_D_OCC (multipole derivatives need "
-                                   "the host call)");
+    if (mp) {
+        if (selector & ~(DH_D_ALL | DH_D_MULTIPOLE))
+            throw Error(DH_E_ARGUMENT, "unknown derivative selector bits");
+        check_selector(c, selector);
+        if (!(selector & DH_D_MULTIPOLE)) mp = false;
+        selector &= DH_D_ALL;
+    } else if (selector & ~DH_D_ALL) {
+        throw Error(DH_E_ARGUMENT, "dh_hc_sf_and_derivs_dev takes DH_D_XYZ, DH_D_ADP and DH_D_OCC (multipole "
+                                   "derivatives: dh_hc_sf_and_derivs_mp_dev or the host call)");
+    }
     if (!c->lcs_defs.empty())
         throw Error(DH_E_ARGUMENT, "device calls take the local frames as given (frames recomputed from positions "
                                    "need the host call)");
@@ -630,6 +648,13 @@ static void sf_dev(Calculator* c, const dh_atoms_dev* at, const double* dtdf, un
     call.d_adp = d_adp;
     call.d_occ = d_occ;
     call.selector = selector;
+    if (mp) {
+        call.multipole = true;
+        call.d_plm = tsum[0];
+        call.d_pval = tsum[1];
+        call.d_kappa = tsum[2];
+        call.d_kappa_prime = tsum[3];
+    }
     Engine& e = c->eng();
     e.run_dev(call, c->prec, static_cast<hipStream_t>(stream), no_alloc ? e.scratch() : nullptr,
               no_alloc ? Engine::scratch_bytes() : 0);
@@ -926,6 +951,17 @@ int dh_hc_sf_and_derivs_dev(dh_calculator* calc, const dh_atoms_dev* atoms, cons
     return guarded([&] {
         if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
         sf_dev(calc->c, atoms, dtarget_df, selector, f, d_xyz, d_adp, d_occ, stream, false);
+    });
+}
+
+int dh_hc_sf_and_derivs_mp_dev(dh_calculator* calc, const dh_atoms_dev* atoms, const double* dtarget_df,
+                               unsigned selector, double* f, double* d_xyz, double* d_adp, double* d_occ,
+                               double* d_plm, double* d_p_val, double* d_kappa, double* d_kappa_prime, void* stream)
+{
+    return guarded([&] {
+        if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
+        double* const tsum[4] = {d_plm, d_p_val, d_kappa, d_kappa_prime};
+        sf_dev(calc->c, atoms, dtarget_df, selector, f, d_xyz, d_adp, d_occ, stream, false, true, tsum);
     });
 }
 

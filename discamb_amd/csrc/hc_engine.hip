@@ -4456,6 +4456,57 @@ __global__ void k_dev_out(const double* __restrict__ fs, const int* __restrict__
     if (docc) docc[i] = ((sel & 4u) && inc) ? a.mult_w[i] * r[9] : 0.0;
 }
 
+// Per-type sums of a device call's multipole derivatives, as the host call forms them:
+// dP[t][j] = sum over the atoms a of type t of w_a dT/dP_j (local frame, rows 10..34) and the
+// Pval, kappa, kappa' sums from rows 35..37, w_a = occupancy x m/|G| (0 for excluded atoms);
+// P_lm entries with l >= n_l[t] are not parameters and read 0.  One workgroup per type: thread
+// (column c, lane q) sums the type's atoms among a = q, q + 8, ..., then lane 0 adds the eight
+// lane sums in a fixed order (no atomics: the sums are reproducible bit for bit).
+__global__ __launch_bounds__(256) void k_dev_type_sums(const double* __restrict__ draw, int nd, int n_atoms,
+                                                       const int* __restrict__ map, const int* __restrict__ n_l,
+                                                       DevAtomArgs a, double* __restrict__ dplm,
+                                                       double* __restrict__ dpval, double* __restrict__ dkappa,
+                                                       double* __restrict__ dkp)
+{
+    __shared__ double part[8][32];
+    const int t = blockIdx.x;
+    const int c = threadIdx.x & 31, q = threadIdx.x >> 5;
+    double s = 0.0;
+    if (c < 28)
+        for (int i = q; i < n_atoms; i += 8) {
+            const bool inc = map[i] == t && (a.include ? a.include[i] != 0 : true);
+            if (inc) s += a.occ[i] * a.mult_w[i] * draw[(size_t)nd * i + 10 + c];
+        }
+    part[q][c] = s;
+    __syncthreads();
+    if (q != 0 || c >= 28) return;
+    double v = 0.0;
+    for (int r = 0; r < 8; r++) v += part[r][c];
+    if (c < 25) {
+        const int l = c < 1 ? 0 : c < 4 ? 1 : c < 9 ? 2 : c < 16 ? 3 : 4;
+        if (dplm) dplm[25 * (size_t)t + c] = l < n_l[t] ? v : 0.0;
+    } else if (c == 25) {
+        if (dpval) dpval[t] = v;
+    } else if (c == 26) {
+        if (dkappa) dkappa[t] = v;
+    } else if (dkp) {
+        dkp[t] = v;
+    }
+}
+
+void Engine::set_type_sum_map(const std::vector<int>& atom_type, const std::vector<int>& n_l)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    if ((int)atom_type.size() != n_atoms_) throw Error(1, "internal: type map of another atom count");
+    std::vector<int> m(atom_type);
+    m.insert(m.end(), n_l.begin(), n_l.end());
+    // earlier calls on any stream may still read the map: wait for them before rewriting it
+    HIP_CHECK(hipDeviceSynchronize());
+    d_tsum_map_.ensure(std::max<size_t>(m.size(), 1) * sizeof(int));
+    HIP_CHECK(hipMemcpy(d_tsum_map_.p, m.data(), m.size() * sizeof(int), hipMemcpyHostToDevice));
+    tsum_types_ = (int)n_l.size();
+}
+
 void Engine::prepare_dev(const EngineAtoms& layout, int n_hkl, const int* hkl, const double* mult,
                          const std::vector<double>& mult_w, const std::vector<double>& anom_default,
                          const DevConv& conv, Precision prec)
@@ -4559,7 +4610,11 @@ void Engine::run_dev(const DevCall& call, Precision prec, hipStream_t stream, vo
     if (n > 0 && (!call.xyz || !call.adp_size || !call.adp || !call.occ)) throw Error(1, "null atom array");
     if (n_hkl > 0 && !call.f) throw Error(1, "null structure-factor output");
     if (n_hkl > 0 && call.selector && !call.dtdf) throw Error(1, "null dT/dF");
-    const bool derivs = call.selector != 0;
+    const bool mp = call.multipole;
+    const bool derivs = call.selector != 0 || mp;
+    if (mp && scratch_mem) throw Error(1, "allocation-free calls take no multipole derivatives");
+    if (mp && tsum_types_ == 0) throw Error(1, "internal: no atom-type map for the multipole sums");
+    if (mp && n_hkl > 0 && !call.dtdf) throw Error(1, "null dT/dF");
     const int n_ain = std::max(n_sorted_, 1);
     const int n_waves = n_sorted_ / kWave;
     AtomIn* skel = d_atoms_in_.as<AtomIn>();
@@ -4633,12 +4688,22 @@ void Engine::run_dev(const DevCall& call, Precision prec, hipStream_t stream, vo
                            d_dtdf_.as<double>());
         HIP_CHECK(hipGetLastError());
     }
-    run(prec, derivs, false);
+    if (mp) {
+        mp_scalars = true;       // pass 1 with the Pval, kappa, kappa' sums
+        lcs_grad = false;
+    }
+    run(prec, derivs, mp);
     const int n_out = std::max(n_hkl, n);
     if (n_out > 0) {
         hipLaunchKernelGGL(k_dev_out, dim3((n_out + T - 1) / T), dim3(T), 0, stream_, d_f_.as<double>(),
                            d_dev_perm_.as<int>(), d_d_.as<double>(), last_nd_, n_hkl, n, aa, dev_conv_, call.selector,
                            call.f, call.d_xyz, call.d_adp, call.d_occ);
+        HIP_CHECK(hipGetLastError());
+    }
+    if (mp) {
+        const int* map = d_tsum_map_.as<int>();
+        hipLaunchKernelGGL(k_dev_type_sums, dim3(tsum_types_), dim3(256), 0, stream_, d_d_.as<double>(), last_nd_, n,
+                           map, map + n, aa, call.d_plm, call.d_pval, call.d_kappa, call.d_kappa_prime);
         HIP_CHECK(hipGetLastError());
     }
 }

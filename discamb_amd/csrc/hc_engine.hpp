@@ -67,6 +67,13 @@ struct DevCall {
     double* d_adp = nullptr;
     double* d_occ = nullptr;
     unsigned selector = 0;                   // xyz 1, ADP 2, occupancy 4
+    // multipole-parameter derivatives (the two adjoint passes), summed per calculator type:
+    // 25 n_types, n_types, n_types, n_types (each may be null)
+    bool multipole = false;
+    double* d_plm = nullptr;
+    double* d_pval = nullptr;
+    double* d_kappa = nullptr;
+    double* d_kappa_prime = nullptr;
 };
 
 // Conversions between the caller's conventions and the engine's (fixed per calculator).
@@ -201,6 +208,9 @@ public:
                      const std::vector<double>& mult_w, const std::vector<double>& anom_default,
                      const DevConv& conv, Precision prec);
     void run_dev(const DevCall& call, Precision prec, hipStream_t stream, void* scratch, size_t scratch_bytes);
+    // the calculator type of every atom and each type's multipole order + 1 (P_lm entries with
+    // l >= n_l are not parameters): the per-type sums of the device calls' multipole derivatives
+    void set_type_sum_map(const std::vector<int>& atom_type, const std::vector<int>& n_l);
     bool dev_ready() const { return dev_ready_; }
     // the static scratch array of the allocation-free calls on this engine's device, its size
     void* scratch() const { return scratch_; }
@@ -263,6 +273,8 @@ private:
     int chunk_override_ = 0;        // reflection chunk forced by an allocation-free call
     DevConv dev_conv_;
     DeviceBuffer d_dev_perm_, d_dev_mult_, d_dev_atomc_;
+    DeviceBuffer d_tsum_map_;       // set_type_sum_map: atom -> type (n_atoms), then n_l per type
+    int tsum_types_ = 0;
     void* scratch_ = nullptr;       // device address of the static scratch array
     std::vector<int> order_;        // sorted slot -> caller atom index (-1 padding)
     std::vector<int> hkl_perm_;     // device reflection position -> caller reflection index

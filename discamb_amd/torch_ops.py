@@ -10,12 +10,13 @@ Parameters (all float64; results follow the device of ``xyz``):
   xyz (n, 3) fractional, adp (n, 6) U_cif (U_iso in column 0 for isotropic atoms), occ (n,),
   plm (n_types, 25) local-frame P_lm in the l*l + l + m layout, pval, kappa, kappa_prime (n_types,).
 
-With the parameters on the calculator's GPU the forward and the structural (xyz / ADP / occupancy)
-backward run as device-resident calls (dh_hc_sf_and_derivs_dev) on torch's current stream: no
-host copies and no host synchronisation.  Gradients of the multipole parameters (P_lm, Pval,
-kappa, kappa') take the host call, which also returns the per-type sums.  Type parameters are
-sent to the engine only when their tensors change (tracked by tensor version counters, so an
-unchanged model costs no device-to-host copy).
+With xyz on the calculator's GPU the forward and the backward run as device-resident calls on
+torch's current stream, without host copies or host synchronisation: dh_hc_sf_and_derivs_dev for
+the structural (xyz / ADP / occupancy) gradients, dh_hc_sf_and_derivs_mp_dev when the multipole
+parameters (P_lm, Pval, kappa, kappa') want gradients too (per-type sums formed on the device).
+With xyz on the host both take the host call.  Type parameters are sent to the engine only when
+their tensors change (tracked by tensor version counters, so an unchanged model costs no
+device-to-host copy).
 """
 from __future__ import annotations
 
@@ -40,9 +41,7 @@ class _SF(torch.autograd.Function):
         owner._set_types(plm, pval, kappa, kappa_prime)
         ctx.owner = owner
         ctx.save_for_backward(xyz, adp, occ, plm, pval, kappa, kappa_prime)
-        # multipole-parameter gradients take the host call in backward: then forward does too, so
-        # that the two do not replace each other's reflection layout every step
-        if owner._on_device(xyz) and not any(ctx.needs_input_grad[4:8]):
+        if owner._on_device(xyz):
             owner._ensure_device_layout()
             return owner.calc.calculateOnDevice(xyz.detach().contiguous(), owner._adp_size_dev, adp.detach().contiguous(),
                                                 occ.detach().contiguous(), None, owner._lcs_dev,
@@ -59,14 +58,21 @@ class _SF(torch.autograd.Function):
         owner._set_types(plm, pval, kappa, kappa_prime)
         want_mp = any(ctx.needs_input_grad[4:8])
         dev = xyz.device
-        if owner._on_device(xyz) and not want_mp:
+        if owner._on_device(xyz):
             owner._ensure_device_layout()
             g = grad_f.detach().to(torch.complex128).contiguous()
-            _, dx, du, do = owner.calc.calculateOnDevice(
+            res = owner.calc.calculateOnDevice(
                 xyz.detach().contiguous(), owner._adp_size_dev, adp.detach().contiguous(), occ.detach().contiguous(), g,
                 owner._lcs_dev, countAtomContribution=owner._include_dev,
-                derivativesSelector=DerivativesSelector(True, True, True))
-            return None, dx, du, do, None, None, None, None
+                derivativesSelector=DerivativesSelector(True, True, True, False, want_mp))
+            out = [None, res[1], res[2], res[3]]
+            if want_mp:
+                td = res[4]
+                out += [td.p_lm.to(plm.device), td.p_val.to(pval.device), td.kappa.to(kappa.device),
+                        td.kappa_prime.to(kappa_prime.device)]
+            else:
+                out += [None, None, None, None]
+            return tuple(out)
         owner._set_atoms(xyz, adp, occ)
         g = grad_f.detach().to("cpu", torch.complex128).numpy()
         sel = DerivativesSelector(True, True, True, False, want_mp)

@@ -214,11 +214,25 @@ typedef struct {
  * returning without waiting on the host.  dtarget_df: 2 n_hkl device doubles (dT/dA + i dT/dB, not
  * conjugated; may be NULL when selector is 0); f: 2 n_hkl; d_xyz 3n, d_adp 6n, d_occ n device
  * doubles (any may be NULL), in the calculator's derivative conventions.  selector: DH_D_XYZ |
- * DH_D_ADP | DH_D_OCC bits (DH_D_MULTIPOLE and local frames recomputed from positions need the host
- * call).  Results equal the host call's to rounding (the convention changes run on the device).
+ * DH_D_ADP | DH_D_OCC bits (DH_D_MULTIPOLE: dh_hc_sf_and_derivs_mp_dev; local frames recomputed
+ * from positions need the host call).  Results equal the host call's to rounding (the convention
+ * changes run on the device).
  * Argument errors are reported synchronously; device-side values are not validated. */
 int dh_hc_sf_and_derivs_dev(dh_calculator* calc, const dh_atoms_dev* atoms, const double* dtarget_df,
                             unsigned selector, double* f, double* d_xyz, double* d_adp, double* d_occ, void* stream);
+/* dh_hc_sf_and_derivs_dev with the multipole-parameter derivatives: selector may add
+ * DH_D_MULTIPOLE, and d_plm (25 n_types), d_p_val, d_kappa, d_kappa_prime (n_types each) device
+ * doubles (any may be NULL) receive dT/dP_lm (local frame, [l*l + l + m]), dT/dPval, dT/dkappa and
+ * dT/dkappa' summed over the atoms of each type, the values dh_hc_fetch_type_derivatives returns
+ * after the host call (to rounding: the sums run in another order), with P_lm entries beyond a
+ * type's multipole order 0.  Without DH_D_MULTIPOLE it is dh_hc_sf_and_derivs_dev and leaves the
+ * four arrays unwritten.  Replaces the host call with DH_D_MULTIPOLE (dh_hc_sf_and_derivs +
+ * dh_hc_fetch_type_derivatives) for callers whose parameters live on the GPU (torch autograd);
+ * the reference has no device-memory form and no multipole-parameter derivatives in
+ * calculateStructureFactorsAndDerivatives (.h:148-165). */
+int dh_hc_sf_and_derivs_mp_dev(dh_calculator* calc, const dh_atoms_dev* atoms, const double* dtarget_df,
+                               unsigned selector, double* f, double* d_xyz, double* d_adp, double* d_occ,
+                               double* d_plm, double* d_p_val, double* d_kappa, double* d_kappa_prime, void* stream);
 /* dh_hc_sf_and_derivs_dev without any allocation.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under hipGraph capture, tight serving loops,
  * harnesses that forbid allocation).  Same arguments, results and stream ordering; the call makes

@@ -11,6 +11,8 @@ max|d - d_ref| / max|d_ref| per output class):
   mixed device call vs oracle           1e-6 (F), 1e-5 (derivatives)
   graph replay vs eager device call     bitwise
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -172,8 +174,22 @@ def test_device_call_errors():
         calc.calculateOnDevice(**args)
     calc.setReflections(w.hkl)
     g = torch.tensor(w.dtdf, dtype=torch.complex128, device="cuda")
-    with pytest.raises(RuntimeError):          # multipole derivatives need the host call
-        calc.calculateOnDevice(dTarget_dF=g, derivativesSelector=DerivativesSelector(d_multipole=True), **args)
+    mp = DerivativesSelector(d_multipole=True)
+    with pytest.raises(ValueError):            # no allocation-free form of the multipole call
+        calc.calculateOnDevice(dTarget_dF=g, derivativesSelector=mp, no_alloc=True, **args)
+    # and the C entry points without the multipole outputs refuse the DH_D_MULTIPOLE bit
+    a, keep, _ = calc._dev_args(args["xyz"], args["adp_size"], args["adp"], args["occupancy"],
+                                args.get("localCoordinateSystems"), None, None)
+    f = torch.empty(len(w.hkl), dtype=torch.complex128, device="cuda")
+    gr = torch.view_as_real(g)
+    for fn in (calc._lib.dh_hc_sf_and_derivs_dev, calc._lib.dh_hc_sf_and_derivs_dev_noalloc):
+        rc = fn(calc._h, ctypes.byref(a), gr.data_ptr(), mp.bits(), torch.view_as_real(f).data_ptr(), None, None,
+                None, None)
+        assert rc != 0
+    torch.cuda.synchronize()
+    # with the outputs: accepted
+    res = calc.calculateOnDevice(dTarget_dF=g, derivativesSelector=mp, **args)
+    assert len(res) == 5 and res[4].p_lm.shape == (calc.n_types, 25)
     bad = dict(args)
     bad["xyz"] = args["xyz"].float()
     with pytest.raises(RuntimeError):          # wrong dtype

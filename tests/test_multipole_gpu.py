@@ -161,8 +161,8 @@ def test_autograd_device_path_config1_vs_oracle(mode):
     """Parameters as GPU tensors (the device-resident calls, no host round trip) on the 5000-atom
     structure: T = sum (|F| - |Fobs|)^2; the xyz / U / occupancy gradient equals the oracle's
     derivatives for dT/dF = 2 (|F| - |Fobs|) F / |F| (FP64 1e-10, mixed 1e-5 normwise) and the
-    host path's gradient; with multipole-parameter gradients requested the host call is taken
-    and gives the same structural gradient."""
+    host path's gradient; with multipole-parameter gradients requested as well the device call
+    takes the multipole adjoint and gives the same structural gradient."""
     import torch
     from discamb_amd.torch_ops import StructureFactors
     w = synth.config1()
@@ -203,3 +203,40 @@ def test_autograd_device_path_config1_vs_oracle(mode):
     ((F.abs() - fo) ** 2).sum().backward()
     assert params[3].grad is not None and params[3].grad.device.type == "cuda"
     assert rel_err(params[0].grad.cpu().numpy(), grads["cuda"][0]) <= (1e-12 if mode == "HIP" else 1e-6)
+
+
+@pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
+@pytest.mark.parametrize("sg", ["P1", "P21/c", "P212121"])
+def test_autograd_device_multipole_gradients_vs_host_path(sg, mode):
+    """Every parameter as a GPU tensor: the backward runs dh_hc_sf_and_derivs_mp_dev (multipole
+    adjoint and per-type sums on the device) and gives the gradients of the host path (parameters
+    on the host: dh_hc_sf_and_derivs + dh_hc_fetch_type_derivatives) for xyz, U, occupancy, P_lm,
+    Pval, kappa and kappa' (FP64 1e-12, mixed 1e-6 normwise); an excluded atom and the P_lm
+    entries beyond a type's multipole order get no gradient."""
+    import torch
+    from discamb_amd.torch_ops import StructureFactors
+    w = synth.small_structure(sg, n_atoms=10, d_min=1.0, seed=21)
+    rng = np.random.default_rng(5)
+    include = np.ones(w.crystal.atoms.n, dtype=bool)
+    include[3] = False
+    sf = StructureFactors(w.crystal, w.params, w.lcs, w.hkl, mode=mode, include=include)
+    fobs = np.abs(rng.normal(size=len(w.hkl)) + 1j * rng.normal(size=len(w.hkl))) * 10.0
+    grads = {}
+    for where in ("cuda", "cpu"):
+        params = sf.initial(w.params, device=where)
+        print("case device multipole gradients sg=%s mode=%s params on %s" % (sg, mode, where), flush=True)
+        F = sf(*params)
+        T = ((F.abs() - torch.tensor(fobs, device=where)) ** 2).sum()
+        T.backward()
+        for p in params:
+            assert p.grad is not None and p.grad.device.type == where
+        grads[where] = [p.grad.detach().cpu().numpy() for p in params]
+    tol = 1e-12 if mode == "HIP" else 1e-6
+    names = ["xyz", "adp", "occ", "plm", "pval", "kappa", "kappa_prime"]
+    e = {n: rel_err(g, r) for n, g, r in zip(names, grads["cuda"], grads["cpu"])}
+    print("  device vs host path", e, flush=True)
+    assert max(e.values()) <= tol, e
+    assert np.all(grads["cuda"][0][3] == 0) and grads["cuda"][2][3] == 0
+    for t, tp in enumerate(w.params.type_parameters):
+        n_l = len(tp.p_lm)
+        assert np.all(grads["cuda"][3][t, n_l * n_l:] == 0)

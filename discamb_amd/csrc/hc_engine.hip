@@ -1378,14 +1378,29 @@ __device__ __forceinline__ void signed_contraction(const R (&C)[5][8], R* A)
 // record's scalar operands.  Phases and Debye-Waller factors are carried along the lines exactly
 // as in body_fp64_line.
 static constexpr int kDiag64Rec = 40;
-// Sign-pattern sets with an FP64 diagonal line kernel (bit 12 marks the code; op s in bits 3s..3s+2):
-// monoclinic 2-fold along y (P2_1/c, C2/c), along z and along x, an inversion partner listed
-// explicitly (a centre off the origin), and the orthorhombic 222 set in the order identity, 2z, 2y, 2x.
+// Sign-pattern sets with compiled kernels (bit 12 marks the code; operation s has pattern bits
+// 3s..3s+2, bit i of a pattern: axis i flips).  The engine lists a diagonal group's operations in
+// increasing pattern order, so every set is stored in that order: the identity first, then a
+// 2-fold along y (P2_1/c, C2/c, P2_1), along z or along x, an inversion partner listed explicitly
+// (a centre off the origin), a mirror normal to x, y (Pc, Cc) or z; the orthorhombic 222 set
+// (identity, 2z, 2y, 2x) and mm2 with the 2-fold along z (Pna2_1, Pca2_1: identity, m_x, m_y, 2z),
+// along y or along x.
 static constexpr int kDiag64Pats0 = 4096 | (5 << 3);
 static constexpr int kDiag64Pats1 = 4096 | (3 << 3);
 static constexpr int kDiag64Pats2 = 4096 | (6 << 3);
 static constexpr int kDiag64Pats3 = 4096 | (7 << 3);
 static constexpr int kDiag64Pats4 = 4096 | (3 << 3) | (5 << 6) | (6 << 9);
+static constexpr int kDiag64Pats5 = 4096 | (1 << 3);
+static constexpr int kDiag64Pats6 = 4096 | (2 << 3);
+static constexpr int kDiag64Pats7 = 4096 | (4 << 3);
+static constexpr int kDiag64Pats8 = 4096 | (1 << 3) | (2 << 6) | (3 << 9);
+static constexpr int kDiag64Pats9 = 4096 | (1 << 3) | (4 << 6) | (5 << 9);
+static constexpr int kDiag64Pats10 = 4096 | (2 << 3) | (4 << 6) | (6 << 9);
+// X(code, number of operations) for every compiled set
+#define DH_FOR_EACH_PATTERN_SET(X)                                                                    \
+    X(kDiag64Pats0, 2) X(kDiag64Pats1, 2) X(kDiag64Pats2, 2) X(kDiag64Pats3, 2) X(kDiag64Pats5, 2)     \
+    X(kDiag64Pats6, 2) X(kDiag64Pats7, 2) X(kDiag64Pats4, 4) X(kDiag64Pats8, 4) X(kDiag64Pats9, 4)     \
+    X(kDiag64Pats10, 4)
 // added to a sign-pattern code: the mixed parity-folded kernel of that set without line recurrences
 static constexpr int kDiagDirect = 8192;
 
@@ -3504,10 +3519,48 @@ void Engine::plan_tables()
     HIP_CHECK(hipMemcpy(d_tabrec_.p, tr.data(), tr.size() * sizeof(TabRec), hipMemcpyHostToDevice));
 }
 
-Engine::Engine(int device, const EngineModel& model, const double* cell6, const std::vector<EngineOp>& ops,
-               bool centric_origin)
-    : device_(device), model_(model), ops_(ops), centric_origin_(centric_origin)
+// Sign pattern of an operation whose fractional and Cartesian rotations are the same diagonal sign
+// matrix (bit i: axis i flips), -1 otherwise.
+static int diagonal_pattern(const EngineOp& op)
 {
+    int pat = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            const int f = op.rf[3 * i + j];
+            const double c = op.rc[3 * i + j];
+            if (i != j) {
+                if (f != 0 || std::fabs(c) > 1e-12) return -1;
+            } else {
+                if ((f != 1 && f != -1) || std::fabs(c - f) > 1e-12) return -1;
+                if (f == -1) pat |= 1 << i;
+            }
+        }
+    return pat;
+}
+
+// A diagonal group's operations in increasing sign-pattern order, so that every listing of a set
+// with compiled kernels reaches them (F and the derivatives are sums over the operations; only the
+// order of those sums' roundings changes).  Other groups keep the caller's order.
+static std::vector<EngineOp> canonical_ops(const std::vector<EngineOp>& ops)
+{
+    std::vector<int> pat(ops.size());
+    for (size_t s = 0; s < ops.size(); s++) {
+        pat[s] = diagonal_pattern(ops[s]);
+        if (pat[s] < 0) return ops;
+    }
+    std::vector<size_t> idx(ops.size());
+    std::iota(idx.begin(), idx.end(), size_t(0));
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return pat[a] < pat[b]; });
+    std::vector<EngineOp> out;
+    for (size_t i : idx) out.push_back(ops[i]);
+    return out;
+}
+
+Engine::Engine(int device, const EngineModel& model, const double* cell6, const std::vector<EngineOp>& ops_in,
+               bool centric_origin)
+    : device_(device), model_(model), ops_(canonical_ops(ops_in)), centric_origin_(centric_origin)
+{
+    const std::vector<EngineOp>& ops = ops_;
     int n_dev = 0;
     if (hipGetDeviceCount(&n_dev) != hipSuccess || n_dev == 0) throw Error(5, "no HIP device available");
     if (device < 0 || device >= n_dev) throw Error(5, "HIP device index out of range");
@@ -3565,18 +3618,11 @@ Engine::Engine(int device, const EngineModel& model, const double* cell6, const 
     // (with a single operation the direct contraction is cheaper than the class sums)
     diag_ = ops.size() >= 2 && ops.size() <= 8;
     for (size_t s = 0; s < ops.size() && diag_; s++) {
-        int pat = 0;
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-                const int f = ops[s].rf[3 * i + j];
-                const double c = ops[s].rc[3 * i + j];
-                if (i != j) {
-                    if (f != 0 || std::fabs(c) > 1e-12) diag_ = false;
-                } else {
-                    if ((f != 1 && f != -1) || std::fabs(c - f) > 1e-12) diag_ = false;
-                    if (f == -1) pat |= 1 << i;
-                }
-            }
+        const int pat = diagonal_pattern(ops[s]);
+        if (pat < 0) {
+            diag_ = false;
+            break;
+        }
         for (size_t q = 0; q < s && diag_; q++)
             if (diag_pat_[q] == pat) diag_ = false;    // each sign pattern at most once
         diag_pat_[s] = pat;
@@ -3629,9 +3675,10 @@ int Engine::diag64_code() const
     if (!diag_ || (ops_.size() != 2 && ops_.size() != 4)) return 0;
     int code = 4096;
     for (size_t q = 0; q < ops_.size(); q++) code |= diag_pat_[q] << (3 * q);
-    for (int c : {kDiag64Pats0, kDiag64Pats1, kDiag64Pats2, kDiag64Pats3})
-        if (ops_.size() == 2 && code == c) return c;
-    if (ops_.size() == 4 && code == kDiag64Pats4) return code;
+#define DH_SET_MATCH(C, N)                                                                            \
+    if ((int)ops_.size() == N && code == C) return C;
+    DH_FOR_EACH_PATTERN_SET(DH_SET_MATCH)
+#undef DH_SET_MATCH
     return 0;
 }
 
@@ -3986,39 +4033,37 @@ static const void* main_kernel(bool mixed, bool derivs, int mp_pass, bool diag, 
                                    : mp_pass == 2 ? (const void*)k_main<M, true, 2, M, false, N, C>               \
                                                   : (const void*)k_main<M, true, 3, M, false, N, C>)
 #define KP(N, C) (mixed ? KPM(true, N, C) : KPM(false, N, C))
+#define DH_CASE(C, N)                                                                                 \
+    case C: return KP(N, C);
         switch (diag64) {
-        case kDiag64Pats0: return KP(2, kDiag64Pats0);
-        case kDiag64Pats1: return KP(2, kDiag64Pats1);
-        case kDiag64Pats2: return KP(2, kDiag64Pats2);
-        case kDiag64Pats3: return KP(2, kDiag64Pats3);
-        case kDiag64Pats4: return KP(4, kDiag64Pats4);
+            DH_FOR_EACH_PATTERN_SET(DH_CASE)
         default: break;
         }
+#undef DH_CASE
 #undef KP
 #undef KPM
     }
     if (diag64 > 0 && mp_pass == 0 && !otf && (!mixed || diag)) {
+#define DH_CASE(C, N)                                                                                 \
+    case C: return mixed ? KD(true, N, C) : KD(false, N, C);
         switch (diag64) {
-        case kDiag64Pats0: return mixed ? KD(true, 2, kDiag64Pats0) : KD(false, 2, kDiag64Pats0);
-        case kDiag64Pats1: return mixed ? KD(true, 2, kDiag64Pats1) : KD(false, 2, kDiag64Pats1);
-        case kDiag64Pats2: return mixed ? KD(true, 2, kDiag64Pats2) : KD(false, 2, kDiag64Pats2);
-        case kDiag64Pats3: return mixed ? KD(true, 2, kDiag64Pats3) : KD(false, 2, kDiag64Pats3);
-        case kDiag64Pats4: return mixed ? KD(true, 4, kDiag64Pats4) : KD(false, 4, kDiag64Pats4);
+            DH_FOR_EACH_PATTERN_SET(DH_CASE)
         default: break;
         }
+#undef DH_CASE
     }
 #undef KD
     // mixed parity-folded kernel with a compiled sign-pattern set, every term direct (table lookups)
     if ((diag64 & kDiagDirect) && mixed && diag && otf && mp_pass == 0) {
+#define DH_CASE(C, N)                                                                                 \
+    case C:                                                                                           \
+        return derivs ? (const void*)k_main<true, true, 0, true, true, N, C | kDiagDirect>             \
+                      : (const void*)k_main<true, false, 0, true, true, N, C | kDiagDirect>;
         switch (diag64 & ~kDiagDirect) {
-        case kDiag64Pats0:
-            return derivs ? (const void*)k_main<true, true, 0, true, true, 2, kDiag64Pats0 | kDiagDirect>
-                          : (const void*)k_main<true, false, 0, true, true, 2, kDiag64Pats0 | kDiagDirect>;
-        case kDiag64Pats4:
-            return derivs ? (const void*)k_main<true, true, 0, true, true, 4, kDiag64Pats4 | kDiagDirect>
-                          : (const void*)k_main<true, false, 0, true, true, 4, kDiag64Pats4 | kDiagDirect>;
+            DH_FOR_EACH_PATTERN_SET(DH_CASE)
         default: break;
         }
+#undef DH_CASE
     }
     if (line_ops > 0 && mp_pass == 0 && !otf && !(mixed && diag)) {
         switch (line_ops) {
@@ -4055,7 +4100,7 @@ int Engine::main_code(bool mixed, bool mp) const
     if (d64) return d64;
     if (mixed && !mp && use_diag(mixed, mp) && use_otf(mp)) {
         const int c = diag64_code();
-        if (c == kDiag64Pats0 || c == kDiag64Pats4) return c | kDiagDirect;
+        if (c != 0) return c | kDiagDirect;
     }
     return 0;
 }

@@ -73,6 +73,23 @@ def test_config0_full(mode):
     _compare(synth.config0(), mode)
 
 
+@pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
+@pytest.mark.parametrize("sg,ops", [("Pna21", ["x,y,z", "-x,-y,z+1/2", "x+1/2,-y+1/2,z", "-x+1/2,y+1/2,z+1/2"]),
+                                    ("Pm2m", ["x,y,z", "-x,y,-z", "x,y,-z", "-x,y,z"]),
+                                    ("Pc", ["x,y,z", "x,-y,z+1/2"]),
+                                    ("P212121-reordered", ["x,y,z", "-x,y+1/2,-z+1/2", "x+1/2,-y+1/2,-z",
+                                                           "-x+1/2,-y,z+1/2"])])
+def test_compiled_sign_sets_table_lookups(sg, ops, mode):
+    """Mirror and mm2 sign sets and a reordered 222 listing on the per-atom table path (the mixed
+    parity-folded kernels of the compiled sets with every term direct; FP64: the general table
+    kernel) against the oracle."""
+    w = synth.small_structure("P212121", n_atoms=16, d_min=0.9, seed=29)
+    w.crystal.symops = ops
+    w.crystal.atoms.multiplicity[:] = len(ops)
+    w.name = "small_" + sg
+    _compare(w, mode, radial="tables")
+
+
 @pytest.mark.parametrize("radial", ["auto", "tables"])
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
 def test_config1_reflection_subset(mode, radial):
@@ -304,7 +321,8 @@ def test_failed_upload_leaves_no_stale_inputs():
 
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
 @pytest.mark.parametrize("sg", ["P1", "P-1", "P21/c", "C2/c", "P212121", "P31", "P-1_shifted", "I222", "P2_1-c",
-                                "P2_1-a", "Pmm2"])
+                                "P2_1-a", "Pmm2", "Pna21", "P2mm", "Pm2m", "Pc", "Pm-x", "Pm-z", "P212121-reordered",
+                                "P2/c-shifted"])
 def test_line_recurrences_vs_direct_and_oracle(sg, mode):
     """Terms carried along hkl lines (groups with <= 4 explicit operations, radial records) against
     direct evaluation of every phase and Debye-Waller factor and against the oracle, with
@@ -313,12 +331,19 @@ def test_line_recurrences_vs_direct_and_oracle(sg, mode):
     the lines, direct evaluation every fourth reflection): both against the oracle at 1e-6 (F) and
     1e-5 (derivatives), and the line path no further from the oracle than twice the direct path's
     error plus 2e-7."""
-    base = {"I222": "P212121", "P2_1-c": "P212121", "P2_1-a": "P212121", "Pmm2": "P212121"}.get(sg, sg.split("_")[0])
-    w = synth.small_structure(base, n_atoms=20, d_min=0.8, seed=23)
-    # 2-fold screw axis along c and along a (the other compiled two-operation sign sets), and a
-    # diagonal four-operation set without a compiled kernel (mirror planes)
+    # every compiled sign-pattern set: 2-fold screw axes along c and along a, mirrors normal to y
+    # (Pc), x and z, mm2 with the 2-fold along z (Pmm2, and Pna2_1 with translations), x and y, the
+    # 222 set listed in another order (the engine sorts the operations); and a diagonal
+    # four-operation set without compiled kernels (2/m with the inversion centre off the origin)
     extra = {"P2_1-c": ["x,y,z", "-x,-y,z+1/2"], "P2_1-a": ["x,y,z", "x+1/2,-y,-z"],
-             "Pmm2": ["x,y,z", "-x,-y,z", "x,-y,z", "-x,y,z"]}
+             "Pmm2": ["x,y,z", "-x,-y,z", "x,-y,z", "-x,y,z"],
+             "Pna21": ["x,y,z", "-x,-y,z+1/2", "x+1/2,-y+1/2,z", "-x+1/2,y+1/2,z+1/2"],
+             "P2mm": ["x,y,z", "x,-y,-z", "x,y,-z", "x,-y,z"], "Pm2m": ["x,y,z", "-x,y,-z", "x,y,-z", "-x,y,z"],
+             "Pc": ["x,y,z", "x,-y,z+1/2"], "Pm-x": ["x,y,z", "-x,y,z"], "Pm-z": ["x,y,z", "x,y,-z"],
+             "P212121-reordered": ["x,y,z", "x+1/2,-y+1/2,-z", "-x,y+1/2,-z+1/2", "-x+1/2,-y,z+1/2"],
+             "P2/c-shifted": ["x,y,z", "-x,y,-z", "-x+1/2,-y,-z+1/2", "x+1/2,-y,z+1/2"]}
+    base = "P212121" if sg in extra or sg == "I222" else sg.split("_")[0]
+    w = synth.small_structure(base, n_atoms=20, d_min=0.8, seed=23)
     if sg in extra:
         w.crystal.symops = extra[sg]
         w.crystal.atoms.multiplicity[:] = len(extra[sg])

@@ -464,36 +464,15 @@ __device__ __forceinline__ void radial_prep_item(int idx, const double* __restri
     const TypeRec& ty = types[t];
     const TabRec tr = tabs[ty.wfn];
     const bool want_d = radx64 || radx32;
-    double fv = 0.0, dfv = 0.0;
     const double hk = h * ty.inv_kappa;
-    if (want_d) {
-        const WfnRec wf = wfns[ty.wfn];
-        for (int i = 0; i < wf.val_n; i++) {
-            const TermRec tv = val[wf.val_off + i];
-            double dg;
-            fv += tv.c * g_function_dh(0, tv.p + 2, hk, tv.e, &dg);
-            dfv += tv.c * dg;
-        }
-        fv *= four_pi;
-        dfv *= four_pi;
-    } else {
-        fv = tab_at<double>(tab, tr, hk * (1.0 / kTabStep), 2);
-    }
+    // S and G_l come from the tables with or without the derivative records, so the records of a
+    // multipole-adjoint prep equal a plain one's bit for bit (and serve plain calls)
+    const double fv = tab_at<double>(tab, tr, hk * (1.0 / kTabStep), 2);
     double r[kRadRec] = {0, 0, 0, 0, 0, 0, 0, 0};
     double x[kRadRec] = {0, 0, 0, 0, 0, 0, 0, 0};
     r[0] = tab_at<double>(tab, tr, h * (1.0 / kTabStep), 0) + ty.pval * fv;
-    x[0] = fv;
-    // d f_val(h/kappa) / d kappa = f_val'(h/kappa) * (-h / kappa^2)
-    x[1] = ty.pval * dfv * (-hk * ty.inv_kappa);
     const double hp = h * ty.inv_kappa_p;
-    if (want_d) {
-        for (int l = 0; l <= ty.lp; l++) {
-            double dg;
-            const double g = g_function_dh(l, ty.n[l] + 2, hp, ty.zeta, &dg);
-            r[1 + l] = ty.gscale[l] * g;
-            x[2 + l] = ty.gscale[l] * dg * (-hp * ty.inv_kappa_p);
-        }
-    } else {
+    {
         // G_l from the wavefunction's table (same Hermite basis for every l)
         double wt[4];
         const int i = hermite<double>(hp * (1.0 / kTabStep), tr.n, wt);
@@ -501,6 +480,26 @@ __device__ __forceinline__ void radial_prep_item(int idx, const double* __restri
         for (int l = 0; l <= ty.lp; l++)
             r[1 + l] = wt[0] * a[2 * l] + wt[1] * a[kTabNode + 2 * l] + wt[2] * a[2 * l + 1] +
                        wt[3] * a[kTabNode + 2 * l + 1];
+    }
+    if (want_d) {
+        // the derivative records from the closed forms: f_val and its derivative summed over the
+        // valence terms, dG_l / dkappa' from the g-function derivatives
+        const WfnRec wf = wfns[ty.wfn];
+        double fvc = 0.0, dfv = 0.0;
+        for (int i = 0; i < wf.val_n; i++) {
+            const TermRec tv = val[wf.val_off + i];
+            double dg;
+            fvc += tv.c * g_function_dh(0, tv.p + 2, hk, tv.e, &dg);
+            dfv += tv.c * dg;
+        }
+        x[0] = fvc * four_pi;
+        // d f_val(h/kappa) / d kappa = f_val'(h/kappa) * (-h / kappa^2)
+        x[1] = ty.pval * (dfv * four_pi) * (-hk * ty.inv_kappa);
+        for (int l = 0; l <= ty.lp; l++) {
+            double dg;
+            (void)g_function_dh(l, ty.n[l] + 2, hp, ty.zeta, &dg);
+            x[2 + l] = ty.gscale[l] * dg * (-hp * ty.inv_kappa_p);
+        }
     }
     if (ez) {
         if (h < 1e-10) {
@@ -3957,9 +3956,11 @@ void Engine::run_prep(Precision prec, bool mp, bool electron, bool general)
     const bool diag64 = use_diag64(mixed_k, mp) && !general;
     const int key = (mixed_k ? 1 : 0) | (mp ? 2 : 0) | (electron && electron_ ? 4 : 0) |
                     (use_diag(mixed_k, mp) && !general ? 8 : 0) | (use_otf(mp) ? 16 : 0) | (diag64 ? 32 : 0);
-    const bool reuse = prep_valid_ && prep_key_ == key;
+    // the records of a multipole-adjoint prep (key | 2) are those of the plain prep plus the
+    // derivative records: they serve plain calls too (the torch step alternates the two)
+    const bool reuse = prep_valid_ && (prep_key_ == key || prep_key_ == (key | 2));
     prep_valid_ = true;
-    prep_key_ = key;
+    if (!reuse) prep_key_ = key;
     const int n_ops = (int)ops_.size();
     const int n_types = std::max(model_.n_types, 1);
     const int n_hkl = n_hkl_;
@@ -4504,23 +4505,24 @@ __global__ void k_dev_out(const double* __restrict__ fs, const int* __restrict__
 // Per-type sums of a device call's multipole derivatives, as the host call forms them:
 // dP[t][j] = sum over the atoms a of type t of w_a dT/dP_j (local frame, rows 10..34) and the
 // Pval, kappa, kappa' sums from rows 35..37, w_a = occupancy x m/|G| (0 for excluded atoms);
-// P_lm entries with l >= n_l[t] are not parameters and read 0.  One workgroup per type: thread
-// (column c, lane q) sums the type's atoms among a = q, q + 8, ..., then lane 0 adds the eight
-// lane sums in a fixed order (no atomics: the sums are reproducible bit for bit).
-__global__ __launch_bounds__(256) void k_dev_type_sums(const double* __restrict__ draw, int nd, int n_atoms,
-                                                       const int* __restrict__ map, const int* __restrict__ n_l,
-                                                       DevAtomArgs a, double* __restrict__ dplm,
-                                                       double* __restrict__ dpval, double* __restrict__ dkappa,
-                                                       double* __restrict__ dkp)
+// P_lm entries with l >= n_l[t] are not parameters and read 0.  The atoms of type t are
+// list[off[t] .. off[t + 1]) (ascending).  One workgroup per type: thread (column c, lane q) sums
+// the list entries q, q + 8, ..., then lane 0 adds the eight lane sums in a fixed order (no
+// atomics: the sums are reproducible bit for bit).
+__global__ __launch_bounds__(256) void k_dev_type_sums(const double* __restrict__ draw, int nd,
+                                                       const int* __restrict__ list, const int* __restrict__ off,
+                                                       const int* __restrict__ n_l, DevAtomArgs a,
+                                                       double* __restrict__ dplm, double* __restrict__ dpval,
+                                                       double* __restrict__ dkappa, double* __restrict__ dkp)
 {
     __shared__ double part[8][32];
     const int t = blockIdx.x;
     const int c = threadIdx.x & 31, q = threadIdx.x >> 5;
     double s = 0.0;
     if (c < 28)
-        for (int i = q; i < n_atoms; i += 8) {
-            const bool inc = map[i] == t && (a.include ? a.include[i] != 0 : true);
-            if (inc) s += a.occ[i] * a.mult_w[i] * draw[(size_t)nd * i + 10 + c];
+        for (int e = off[t] + q; e < off[t + 1]; e += 8) {
+            const int i = list[e];
+            if (a.include ? a.include[i] != 0 : true) s += a.occ[i] * a.mult_w[i] * draw[(size_t)nd * i + 10 + c];
         }
     part[q][c] = s;
     __syncthreads();
@@ -4543,7 +4545,17 @@ void Engine::set_type_sum_map(const std::vector<int>& atom_type, const std::vect
 {
     HIP_CHECK(hipSetDevice(device_));
     if ((int)atom_type.size() != n_atoms_) throw Error(1, "internal: type map of another atom count");
-    std::vector<int> m(atom_type);
+    // [atoms in type order (ascending within a type), type offsets (n_types + 1), n_l (n_types)]
+    const int nt = (int)n_l.size();
+    std::vector<int> off(nt + 1, 0);
+    for (int t : atom_type) {
+        if (t < 0 || t >= nt) throw Error(1, "internal: atom type out of range");
+        off[t + 1]++;
+    }
+    for (int t = 0; t < nt; t++) off[t + 1] += off[t];
+    std::vector<int> m(atom_type.size()), fill_at(off.begin(), off.end() - 1);
+    for (size_t i = 0; i < atom_type.size(); i++) m[fill_at[atom_type[i]]++] = (int)i;
+    m.insert(m.end(), off.begin(), off.end());
     m.insert(m.end(), n_l.begin(), n_l.end());
     // earlier calls on any stream may still read the map: wait for them before rewriting it
     HIP_CHECK(hipDeviceSynchronize());
@@ -4746,9 +4758,11 @@ void Engine::run_dev(const DevCall& call, Precision prec, hipStream_t stream, vo
         HIP_CHECK(hipGetLastError());
     }
     if (mp) {
-        const int* map = d_tsum_map_.as<int>();
-        hipLaunchKernelGGL(k_dev_type_sums, dim3(tsum_types_), dim3(256), 0, stream_, d_d_.as<double>(), last_nd_, n,
-                           map, map + n, aa, call.d_plm, call.d_pval, call.d_kappa, call.d_kappa_prime);
+        const int* list = d_tsum_map_.as<int>();
+        const int* off = list + n;
+        hipLaunchKernelGGL(k_dev_type_sums, dim3(tsum_types_), dim3(256), 0, stream_, d_d_.as<double>(), last_nd_,
+                           list, off, off + tsum_types_ + 1, aa, call.d_plm, call.d_pval, call.d_kappa,
+                           call.d_kappa_prime);
         HIP_CHECK(hipGetLastError());
     }
 }

@@ -273,7 +273,7 @@ private:
     int chunk_override_ = 0;        // reflection chunk forced by an allocation-free call
     DevConv dev_conv_;
     DeviceBuffer d_dev_perm_, d_dev_mult_, d_dev_atomc_;
-    DeviceBuffer d_tsum_map_;       // set_type_sum_map: atom -> type (n_atoms), then n_l per type
+    DeviceBuffer d_tsum_map_;       // set_type_sum_map: atoms by type, type offsets, n_l per type
     int tsum_types_ = 0;
     void* scratch_ = nullptr;       // device address of the static scratch array
     std::vector<int> order_;        // sorted slot -> caller atom index (-1 padding)

@@ -201,11 +201,15 @@ def bench_batch(args):
             dev.append(batch.lastTiming()["step_ms"])
         t2 = time.perf_counter()
         out = [c.fetch() for c in calcs]
-        # each structure alone: a one-structure batch (same kernels, graph replay), device time
+        # each structure alone: a one-structure batch (same kernels, graph replay, its own tuned
+        # reflection chunk), device time
         single = []
         for c in calcs if args.singles else []:
             b1 = BatchCalculator([c])
-            b1.setTuning(chunk)
+            if fixed:
+                b1.setTuning(fixed)
+            else:
+                b1.autotune()
             b1.run(graph=True)
             t = []
             for _ in range(3):

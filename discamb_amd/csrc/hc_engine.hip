@@ -595,11 +595,11 @@ __global__ void k_hkl_prep_diag(const int* __restrict__ hkl, const OpDev* __rest
 
 // FP64 diagonal-group line kernels: one record per reflection, [d_j(h) x 25, m(h) x 6, h x 3,
 // h.t_s for s < n_ops (<= 4), pad] (kDiag64Rec doubles), and |h|.
-__global__ void k_hkl_prep_diag64(const int* __restrict__ hkl, const OpDev* __restrict__ ops, int n_ops, int n_hkl,
-                                  const double* __restrict__ c2f, double* __restrict__ rec, double* __restrict__ hlen)
+__device__ __forceinline__ void hkl_prep_diag64_item(int k, const int* __restrict__ hkl,
+                                                     const OpDev* __restrict__ ops, int n_ops,
+                                                     const double* __restrict__ c2f, double* __restrict__ rec,
+                                                     double* __restrict__ hlen)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_hkl) return;
     const int h0 = hkl[3 * k], h1 = hkl[3 * k + 1], h2 = hkl[3 * k + 2];
     double hc[3];
     for (int j = 0; j < 3; j++) hc[j] = h0 * c2f[j] + h1 * c2f[3 + j] + h2 * c2f[6 + j];
@@ -627,6 +627,13 @@ __global__ void k_hkl_prep_diag64(const int* __restrict__ hkl, const OpDev* __re
         const OpDev* o = s < n_ops ? ops + s : nullptr;
         r[34 + s] = o ? h0 * o->tf[0] + h1 * o->tf[1] + h2 * o->tf[2] : 0.0;
     }
+}
+
+__global__ void k_hkl_prep_diag64(const int* __restrict__ hkl, const OpDev* __restrict__ ops, int n_ops, int n_hkl,
+                                  const double* __restrict__ c2f, double* __restrict__ rec, double* __restrict__ hlen)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_hkl) hkl_prep_diag64_item(k, hkl, ops, n_ops, c2f, rec, hlen);
 }
 
 __global__ __launch_bounds__(64) void k_tab_build(const TabRec* __restrict__ tabs, int n_wfn,
@@ -1650,7 +1657,7 @@ __host__ __device__ constexpr int class_coords(int c)
 // MP: the passes of the multipole adjoint as in body_fp64 (1: F, xyz/U/occupancy and the Pval,
 // kappa, kappa' sums; 2: the B_lm sums only, the class-signed sums of c E_s; 3: the plain pass in
 // the multipole row layout).
-template <int L, bool DERIV, bool OTF, int NOPS, int PATS, int MP = 0>
+template <int L, bool DERIV, bool OTF, int NOPS, int PATS, int MP = 0, bool LINES = true>
 __device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave, int lane, int chunk_id, int k0,
                                                     int k1, const unsigned* __restrict__ hop,
                                                     const double* __restrict__ rad, const double* __restrict__ radx,
@@ -1686,10 +1693,11 @@ __device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave,
     const int* cont = cmem(a.line_cont);
     const unsigned* lop = cmem(reinterpret_cast<const unsigned*>(a.line_ops));
     // With diagonal rotations the step of every operation is +-Delta, so the Debye-Waller step
-    // ratio C = exp(-2 Delta_s^T U Delta_s) is the same for all of them.
+    // ratio C = exp(-2 Delta_s^T U Delta_s) is the same for all of them.  LINES false (reflections
+    // in |h| order for the table lookups): every term direct, no step factors.
     double Mr[NOPS], Mi[NOPS], Er[NOPS], Ei[NOPS], Q[NOPS], an[NOPS];
-    double Cq;
-    {
+    double Cq = 0.0;
+    if constexpr (LINES) {
         double e2 = 0.0;
 #pragma unroll
         for (int j = 0; j < 6; j++) e2 = fma(ld_f64(lop, 4 + j), u[j], e2);
@@ -1697,9 +1705,13 @@ __device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave,
     }
 #pragma unroll
     for (int s = 0; s < NOPS; s++) {
-        const unsigned* o = lop + s * 20;
-        const double ph = __builtin_amdgcn_fract(fma(ld_f64(o, 2), x[2], fma(ld_f64(o, 1), x[1], fma(ld_f64(o, 0), x[0], ld_f64(o, 3)))));
-        sincos_rev(ph, &Mi[s], &Mr[s]);
+        Mr[s] = Mi[s] = 0.0;
+        if constexpr (LINES) {
+            const unsigned* o = lop + s * 20;
+            const double ph = __builtin_amdgcn_fract(
+                fma(ld_f64(o, 2), x[2], fma(ld_f64(o, 1), x[1], fma(ld_f64(o, 0), x[0], ld_f64(o, 3)))));
+            sincos_rev(ph, &Mi[s], &Mr[s]);
+        }
         Er[s] = Ei[s] = Q[s] = 0.0;
         an[s] = (double)a.diag_anom[(PATS >> (3 * s)) & 7] * ani;
     }
@@ -1718,7 +1730,7 @@ __device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave,
         for (int j = 0; j < 6; j++) m[j] = ld_f64(rw, 25 + j);
 #pragma unroll
         for (int j = 0; j < 3; j++) hf[j] = ld_f64(rw, 31 + j);
-        const bool along = k > k0 && cont[k] != 0;
+        const bool along = LINES && k > k0 && cont[k] != 0;
         if (along) {
 #pragma unroll
             for (int s = 0; s < NOPS; s++) {
@@ -1730,7 +1742,7 @@ __device__ __forceinline__ void body_fp64_line_diag(const MainArgs& a, int wave,
                 Q[s] *= Cq;
             }
         } else {
-            const bool next = k + 1 < k1 && cont[k + 1] != 0;
+            const bool next = LINES && k + 1 < k1 && cont[k + 1] != 0;
             double mn[6];
             if (next) {
                 const unsigned* rn = hop + (size_t)(k + 1) * (2 * kDiag64Rec);
@@ -2591,12 +2603,13 @@ __device__ __forceinline__ void main_unit(const MainArgs& a, int chunk_id, int a
         if constexpr (NOPS > 0 && DPATS > 0) {
             constexpr int N = NOPS > 0 ? NOPS : 1;
             constexpr int PS = DPATS & 4095;
+            constexpr bool LN = (DPATS & kDiagDirect) == 0;
             if (cls <= 2)
-                body_fp64_line_diag<2, DERIV, OTF, N, PS, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, dtdf32,
-                                                              st, fpart, dpart);
+                body_fp64_line_diag<2, DERIV, OTF, N, PS, MP, LN>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx,
+                                                                  dtdf32, st, fpart, dpart);
             else
-                body_fp64_line_diag<4, DERIV, OTF, N, PS, MP>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx, dtdf32,
-                                                              st, fpart, dpart);
+                body_fp64_line_diag<4, DERIV, OTF, N, PS, MP, LN>(a, wave, lane, chunk_id, k0, k1, hop, rad, radx,
+                                                                  dtdf32, st, fpart, dpart);
         } else if constexpr (NOPS > 0) {
             constexpr int N = NOPS > 0 ? NOPS : 1;
             if (cls <= 2)
@@ -3953,7 +3966,10 @@ void Engine::run_prep(Precision prec, bool mp, bool electron, bool general)
     // the reflection records, tables and radial records depend on the reflections, the types and
     // the path only: kept from the previous call with the same key until one of them changes
     const bool mixed_k = prec == kMixed;
-    const bool diag64 = use_diag64(mixed_k, mp) && !general;
+    // per-reflection FP64 diagonal records: the FP64 line kernels and the FP64 direct kernels of
+    // the compiled sign-pattern sets (table lookups)
+    const bool diag64 = (use_diag64(mixed_k, mp) || (!mixed_k && (main_code(mixed_k, mp) & kDiagDirect) != 0)) &&
+                        !general;
     const int key = (mixed_k ? 1 : 0) | (mp ? 2 : 0) | (electron && electron_ ? 4 : 0) |
                     (use_diag(mixed_k, mp) && !general ? 8 : 0) | (use_otf(mp) ? 16 : 0) | (diag64 ? 32 : 0);
     // the records of a multipole-adjoint prep (key | 2) are those of the plain prep plus the
@@ -4054,6 +4070,18 @@ static const void* main_kernel(bool mixed, bool derivs, int mp_pass, bool diag, 
 #undef DH_CASE
     }
 #undef KD
+    // FP64 diagonal-record kernel with a compiled sign-pattern set, every term direct (table lookups)
+    if ((diag64 & kDiagDirect) && !mixed && otf && mp_pass == 0) {
+#define DH_CASE(C, N)                                                                                 \
+    case C:                                                                                           \
+        return derivs ? (const void*)k_main<false, true, 0, false, true, N, C | kDiagDirect>           \
+                      : (const void*)k_main<false, false, 0, false, true, N, C | kDiagDirect>;
+        switch (diag64 & ~kDiagDirect) {
+            DH_FOR_EACH_PATTERN_SET(DH_CASE)
+        default: break;
+        }
+#undef DH_CASE
+    }
     // mixed parity-folded kernel with a compiled sign-pattern set, every term direct (table lookups)
     if ((diag64 & kDiagDirect) && mixed && diag && otf && mp_pass == 0) {
 #define DH_CASE(C, N)                                                                                 \
@@ -4099,7 +4127,7 @@ int Engine::main_code(bool mixed, bool mp) const
 {
     const int d64 = use_diag64(mixed, mp);
     if (d64) return d64;
-    if (mixed && !mp && use_diag(mixed, mp) && use_otf(mp)) {
+    if (!mp && use_otf(mp) && (mixed ? use_diag(mixed, mp) : diag_ && !tune.general_path)) {
         const int c = diag64_code();
         if (c != 0) return c | kDiagDirect;
     }
@@ -4166,7 +4194,7 @@ void Engine::main_args(Precision prec, bool derivs, bool mp, int chunk, MainArgs
     a.grid_x = (n_waves + kWavesPerBlock - 1) / kWavesPerBlock;
     a.n_blocks = a.grid_x * n_chunks;
     a.per_xcd = (a.n_blocks + kXcds - 1) / kXcds;
-    const bool diag = use_diag(mixed, mp) || use_diag64(mixed, mp);
+    const bool diag = use_diag(mixed, mp) || use_diag64(mixed, mp) || (main_code(mixed, mp) & kDiagDirect) != 0;
     a.ht = use_diag(mixed, mp) ? d_ht_.as<double>() : nullptr;
     a.tab = mixed ? d_tab32_.p : d_tab_.p;
     a.tabs = d_tabrec_.as<TabRec>();
@@ -5053,6 +5081,7 @@ struct BatchItem {
     double* f;
     double* d;
     int n_ops, n_hkl, n_wfn, n_types, n_slots, n_atoms, nd, n_chunks, centric_origin, diag, n_tab;
+    int diag64;       // FP64: per-reflection diagonal records (kDiag64Rec) for a compiled sign-pattern body
     int part_f32;     // F partials stored as FP32 (mixed precision)
     int body;         // kernel body of the batched main kernel (BatchBody)
     MainArgs main;
@@ -5060,7 +5089,9 @@ struct BatchItem {
 
 // Kernel bodies of the batched step.  Each runs as its own persistent launch over the structures
 // that use it (contiguous in the batch order): one kernel holding every body would be allocated
-// the registers of the largest and spill in all of them.
+// the registers of the largest and spill in all of them.  The launches run concurrently (forked
+// streams), so their tails overlap.  Mixed precision uses every body; FP64 the general body and,
+// for 222 structures, the FP64 parity-folded direct body.
 enum BatchBody {
     kBodyGeneral = 0,    // general rotations (per-(reflection, operation) records)
     kBodyDiag,           // parity-folded records, sign patterns read at run time
@@ -5106,6 +5137,7 @@ __global__ void k_batch_hkl_prep(const BatchItem* __restrict__ it, const int* __
     const BatchItem& q = it[b];
     const int k = i - off[b];
     if (q.diag) hkl_prep_diag_item(k, q.hkl, q.ops, q.n_ops, q.c2f, q.rec32, q.phase, q.ht, q.hlen);
+    else if (q.diag64) hkl_prep_diag64_item(k, q.hkl, q.ops, q.n_ops, q.c2f, q.rec64, q.hlen);
     else hkl_prep_item(k, q.hkl, q.ops, q.n_ops, q.c2f, q.rec64, q.rec32, q.phase, q.hlen);
 }
 
@@ -5167,6 +5199,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MIXED ? (BO
                 main_unit<true, DERIV, 0, true, true, 2, kDiag64Pats0 | kDiagDirect>(a, chunk_id, ab, wb, lane, stage);
             else if constexpr (MIXED && BODY == kBodyPats4)
                 main_unit<true, DERIV, 0, true, true, 4, kDiag64Pats4 | kDiagDirect>(a, chunk_id, ab, wb, lane, stage);
+            else if constexpr (!MIXED && BODY == kBodyPats4)
+                main_unit<false, DERIV, 0, false, true, 4, kDiag64Pats4 | kDiagDirect>(a, chunk_id, ab, wb, lane, stage);
             else
                 main_unit<MIXED, DERIV, 0, false, true>(a, chunk_id, ab, wb, lane, stage);
         }
@@ -5242,6 +5276,23 @@ void Engine::batch_item(Precision prec, bool derivs, int chunk, BatchItem* out)
         const int c = diag64_code();
         q.body = c == kDiag64Pats4 ? kBodyPats4 : c == kDiag64Pats0 ? kBodyPats0 : kBodyDiag;
     }
+    q.diag64 = 0;
+    if (!mixed && diag_ && !tune.general_path) {
+        // FP64 222 structures: the parity-folded direct body on per-reflection records (2/m
+        // structures measured faster on the general body in the batched step)
+        const int c = diag64_code();
+        if (c == kDiag64Pats4) {
+            q.diag64 = 1;
+            q.body = kBodyPats4;
+            MainArgs& a = q.main;
+            a.diag_mask = 0;
+            for (int s = 0; s < n_ops; s++) {
+                a.diag_mask |= 1 << diag_pat_[s];
+                a.diag_op[diag_pat_[s]] = s;
+                a.diag_anom[diag_pat_[s]] = diag_anom_[s];
+            }
+        }
+    }
 }
 
 void Engine::mark_batched_run(bool derivs)
@@ -5261,6 +5312,11 @@ BatchEngine::BatchEngine(const std::vector<Engine*>& engines) : eng_(engines)
     HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     HIP_CHECK(hipEventCreate(&ev0_));
     HIP_CHECK(hipEventCreate(&ev1_));
+    HIP_CHECK(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+    for (int i = 0; i < 3; i++) {
+        HIP_CHECK(hipStreamCreateWithFlags(&side_[i], hipStreamNonBlocking));
+        HIP_CHECK(hipEventCreateWithFlags(&ev_join_[i], hipEventDisableTiming));
+    }
     d_counter_.ensure(256);
 }
 
@@ -5269,6 +5325,11 @@ BatchEngine::~BatchEngine()
     drop_graph();
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
+    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    for (int i = 0; i < 3; i++) {
+        if (ev_join_[i]) (void)hipEventDestroy(ev_join_[i]);
+        if (side_[i]) (void)hipStreamDestroy(side_[i]);
+    }
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -5282,7 +5343,7 @@ void BatchEngine::drop_graph()
 static const void* batch_main_kernel(bool mixed, bool derivs, int body)
 {
 #define KB(M, B) (derivs ? (const void*)k_batch_main<M, true, B> : (const void*)k_batch_main<M, false, B>)
-    if (!mixed) return KB(false, kBodyGeneral);
+    if (!mixed) return body == kBodyPats4 ? KB(false, kBodyPats4) : KB(false, kBodyGeneral);
     switch (body) {
         case kBodyDiag: return KB(true, kBodyDiag);
         case kBodyPats0: return KB(true, kBodyPats0);
@@ -5336,7 +5397,8 @@ void BatchEngine::plan(Precision prec, bool derivs, int chunk)
         const BatchItem& q = sorted[b];
         const size_t n_hkl = (size_t)q.n_hkl, n_rec = q.diag ? n_hkl : n_hkl * q.n_ops;
         size_t* z = &sz[4 * (size_t)b];
-        z[0] = align(n_rec * kHklopRec * (mixed ? sizeof(float) : sizeof(double)));   // records
+        z[0] = q.diag64 ? align(n_hkl * kDiag64Rec * sizeof(double))                   // records
+                        : align(n_rec * kHklopRec * (mixed ? sizeof(float) : sizeof(double)));
         z[1] = mixed ? align(n_rec * 4 * sizeof(double)) : 0;                           // phases / h
         z[2] = q.diag ? align(n_hkl * q.n_ops * sizeof(double)) : 0;                   // h.t per op
         z[3] = align(n_hkl * sizeof(double));                                           // |h|
@@ -5536,9 +5598,20 @@ void BatchEngine::launch(Precision prec, bool derivs)
         hipLaunchKernelGGL(k_batch_atom_prep, dim3(total_[kSpSlot]), dim3(64), 0, stream_, it, o(kSpSlot), nb);
     HIP_CHECK(hipGetLastError());
     last_grid = 0;
+    // one persistent launch per kernel body: the first on stream_, the others on side streams
+    // forked after the prep, so that a body's workgroups start as those of the previous body
+    // retire (the launches' tails overlap); joined back before the reductions
+    if (groups_.size() > 4) throw Error(1, "internal: more kernel bodies than side streams");
+    HIP_CHECK(hipEventRecord(ev_fork_, stream_));
+    bool joined[3] = {false, false, false};
     for (size_t gi = 0; gi < groups_.size(); gi++) {
         const BodyGroup& g = groups_[gi];
         if (g.units == 0) continue;
+        hipStream_t s = stream_;
+        if (gi > 0) {
+            s = side_[gi - 1];
+            HIP_CHECK(hipStreamWaitEvent(s, ev_fork_, 0));
+        }
         const int grid = std::min(grid_[(mixed ? 2 : 0) + (derivs ? 1 : 0)][g.body], std::max(g.units, kXcds));
         last_grid = std::max(last_grid, grid);
         const BatchItem* itg = it + g.b0;
@@ -5546,8 +5619,14 @@ void BatchEngine::launch(Precision prec, bool derivs)
         const int stride = nb + 1, n = g.b1 - g.b0;
         int* cnt = d_counter_.as<int>() + (size_t)g.body * kXcds;
         void* args[] = {(void*)&itg, (void*)&uo, (void*)&stride, (void*)&n, (void*)&cnt};
-        HIP_CHECK(hipLaunchKernel(batch_main_kernel(mixed, derivs, g.body), dim3(grid), dim3(256), args, 0, stream_));
+        HIP_CHECK(hipLaunchKernel(batch_main_kernel(mixed, derivs, g.body), dim3(grid), dim3(256), args, 0, s));
+        if (gi > 0) {
+            HIP_CHECK(hipEventRecord(ev_join_[gi - 1], s));
+            joined[gi - 1] = true;
+        }
     }
+    for (int i = 0; i < 3; i++)
+        if (joined[i]) HIP_CHECK(hipStreamWaitEvent(stream_, ev_join_[i], 0));
     const int R = 64;
     if (total_[kSpHkl] > 0)
         hipLaunchKernelGGL(k_batch_reduce_f, blocks(total_[kSpHkl], R), dim3(R), 0, stream_, it, o(kSpHkl), nb,

@@ -368,6 +368,10 @@ private:
     int device_ = 0;
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    // the kernel bodies' persistent launches run concurrently: body group g > 0 on side_[g - 1],
+    // forked from stream_ after the prep (ev_fork_) and joined back before the reductions (ev_join_)
+    hipStream_t side_[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_fork_ = nullptr, ev_join_[3] = {nullptr, nullptr, nullptr};
     DeviceBuffer d_items_, d_off_, d_counter_;
     // radial tables shared by the structures of the batch (one per distinct wavefunction)
     DeviceBuffer d_gwfn_, d_gcore_, d_gval_, d_gtabs_, d_ltabs_, d_gtab_, d_gtab32_;

@@ -101,6 +101,18 @@ struct Calculator {
     std::vector<double> mult_k;
     Engine& eng() { return iam ? *iam_engine : *engine; }
     bool eng_centric_origin() const { return !iam && centric_origin; }
+    // Groups whose operations go beyond sign matrices but contain a diagonal subgroup H of two or
+    // four operations (tetragonal, hexagonal, cubic): host calls run on cos_engine, built with the
+    // operations of H, over the images g(x) of every atom under the coset representatives
+    // cos_reps (coset_plan), and fold the images' derivatives back.  Device calls, per-reflection
+    // rows, form factors and batched steps keep the main engine and the full operation list.
+    std::unique_ptr<Engine> cos_engine;
+    std::vector<EngineOp> cos_reps;
+    EngineAtoms cos_atoms;
+    std::vector<double> cos_rows;
+    Engine* last_run = nullptr;       // the engine of the last run (dh_hc_last_timing)
+    Engine& host_eng() { return iam ? *iam_engine : (cos_engine && !engine->tune.general_path) ? *cos_engine : *engine; }
+    bool cosets() { return !iam && &host_eng() == cos_engine.get(); }
     // state of the last upload
     int n_atoms = 0, n_hkl = 0;
     std::vector<int> adp_size;
@@ -125,6 +137,217 @@ struct Calculator {
 
 // explicit operations; with expand_centre the inversion partners (-R, -t + t_inv) are listed
 // too, their anomalous term conjugated (partner_anom = -1, Engine3) or not (+1, Engine v1)
+// A rotation that only flips coordinate axes, fractional and Cartesian alike: its sign pattern
+// (bit i: axis i flips), or -1.
+static int sign_pattern(const EngineOp& o)
+{
+    int pat = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            const int f = o.rf[3 * i + j];
+            const double c = o.rc[3 * i + j];
+            if (i != j) {
+                if (f != 0 || std::fabs(c) > 1e-12) return -1;
+            } else {
+                if ((f != 1 && f != -1) || std::fabs(c - f) > 1e-12) return -1;
+                if (f == -1) pat |= 1 << i;
+            }
+        }
+    return pat;
+}
+
+// h o g: x -> R_h (R_g x + t_g) + t_h
+static EngineOp compose_ops(const EngineOp& h, const EngineOp& g)
+{
+    EngineOp o;
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) {
+            int r = 0;
+            double c = 0.0;
+            for (int k = 0; k < 3; k++) {
+                r += h.rf[3 * i + k] * g.rf[3 * k + j];
+                c += h.rc[3 * i + k] * g.rc[3 * k + j];
+            }
+            o.rf[3 * i + j] = r;
+            o.rc[3 * i + j] = c;
+        }
+        double t = h.tf[i];
+        for (int k = 0; k < 3; k++) t += h.rf[3 * i + k] * g.tf[k];
+        o.tf[i] = t;
+    }
+    o.anom_sign = h.anom_sign * g.anom_sign;
+    return o;
+}
+
+static bool same_op_mod1(const EngineOp& a, const EngineOp& b)
+{
+    for (int i = 0; i < 9; i++)
+        if (a.rf[i] != b.rf[i]) return false;
+    for (int i = 0; i < 3; i++) {
+        const double d = a.tf[i] - b.tf[i];
+        if (std::fabs(d - std::nearbyint(d)) > 1e-9) return false;
+    }
+    return true;
+}
+
+// Coset decomposition of the explicit operations over their diagonal subgroup H: every operation
+// is s = h o g for exactly one h in H and one representative g (the identity first), so
+// sum_s X(s(x)) = sum_g sum_h X(h(g(x))) and the calculation can run over the images g(x) of the
+// atoms with the operations of H, on the parity-folded kernels of H's sign set.  Built only when H
+// has two or four operations without the inversion, some operation lies outside H and no
+// operation conjugates the anomalous term.
+static bool coset_plan(const std::vector<EngineOp>& ops, std::vector<EngineOp>* h_ops, std::vector<EngineOp>* reps)
+{
+    h_ops->clear();
+    reps->clear();
+    for (const EngineOp& o : ops) {
+        if (o.anom_sign != 1.0) return false;
+        const int p = sign_pattern(o);
+        if (p == 7) return false;
+        if (p >= 0) h_ops->push_back(o);
+    }
+    const size_t nh = h_ops->size();
+    if ((nh != 2 && nh != 4) || ops.size() <= nh || ops.size() % nh != 0) return false;
+    for (size_t a = 0; a < nh; a++)
+        for (size_t b = a + 1; b < nh; b++)
+            if (sign_pattern((*h_ops)[a]) == sign_pattern((*h_ops)[b])) return false;
+    EngineOp id;
+    for (int i = 0; i < 9; i++) id.rf[i] = i % 4 == 0 ? 1 : 0, id.rc[i] = i % 4 == 0 ? 1.0 : 0.0;
+    for (int i = 0; i < 3; i++) id.tf[i] = 0.0;
+    id.anom_sign = 1.0;
+    std::vector<bool> covered(ops.size(), false);
+    auto cover = [&](const EngineOp& g) {
+        for (const EngineOp& h : *h_ops) {
+            const EngineOp c = compose_ops(h, g);
+            size_t j = 0;
+            while (j < ops.size() && !same_op_mod1(c, ops[j])) j++;
+            if (j == ops.size() || covered[j]) return false;
+            covered[j] = true;
+        }
+        reps->push_back(g);
+        return true;
+    };
+    if (!cover(id)) return false;
+    for (size_t s = 0; s < ops.size(); s++)
+        if (!covered[s] && !cover(ops[s])) return false;
+    return reps->size() * nh == ops.size();
+}
+
+// The images g(x) of every atom under the coset representatives, image k of atom i at i K + k:
+// position R_g x + t_g, U -> R U R^T (Cartesian), local frame R L, the same type, weight and f', f''.
+static void expand_atoms(const std::vector<EngineOp>& reps, const EngineAtoms& a, EngineAtoms* x)
+{
+    const int K = (int)reps.size(), n = a.n;
+    x->n = n * K;
+    x->xyz_frac.resize(3 * (size_t)x->n);
+    x->u6.resize(6 * (size_t)x->n);
+    x->iso.resize(x->n);
+    x->weight.resize(x->n);
+    x->anom.resize(2 * (size_t)x->n);
+    x->lcs.resize(9 * (size_t)x->n);
+    x->type.resize(x->n);
+    x->include.resize(x->n);
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < K; k++) {
+            const EngineOp& g = reps[k];
+            const size_t o = (size_t)i * K + k;
+            for (int r = 0; r < 3; r++) {
+                double v = g.tf[r];
+                for (int c = 0; c < 3; c++) v += g.rf[3 * r + c] * a.xyz_frac[3 * (size_t)i + c];
+                x->xyz_frac[3 * o + r] = v;
+            }
+            const double* u = &a.u6[6 * (size_t)i];
+            if (a.iso[i]) {
+                for (int j = 0; j < 6; j++) x->u6[6 * o + j] = u[j];
+            } else {
+                const double U[9] = {u[0], u[3], u[4], u[3], u[1], u[5], u[4], u[5], u[2]};
+                double RU[9], V[9];
+                for (int r = 0; r < 3; r++)
+                    for (int c = 0; c < 3; c++) {
+                        double s = 0.0;
+                        for (int m = 0; m < 3; m++) s += g.rc[3 * r + m] * U[3 * m + c];
+                        RU[3 * r + c] = s;
+                    }
+                for (int r = 0; r < 3; r++)
+                    for (int c = 0; c < 3; c++) {
+                        double s = 0.0;
+                        for (int m = 0; m < 3; m++) s += RU[3 * r + m] * g.rc[3 * c + m];
+                        V[3 * r + c] = s;
+                    }
+                const double v6[6] = {V[0], V[4], V[8], 0.5 * (V[1] + V[3]), 0.5 * (V[2] + V[6]), 0.5 * (V[5] + V[7])};
+                for (int j = 0; j < 6; j++) x->u6[6 * o + j] = v6[j];
+            }
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) {
+                    double s = 0.0;
+                    for (int m = 0; m < 3; m++) s += g.rc[3 * r + m] * a.lcs[9 * (size_t)i + 3 * m + c];
+                    x->lcs[9 * o + 3 * r + c] = s;
+                }
+            x->iso[o] = a.iso[i];
+            x->weight[o] = a.weight[i];
+            x->anom[2 * o] = a.anom[2 * (size_t)i];
+            x->anom[2 * o + 1] = a.anom[2 * (size_t)i + 1];
+            x->type[o] = a.type[i];
+            x->include[o] = a.include[i];
+        }
+}
+
+// Derivative rows of the images summed back onto their atoms: dT/dx = sum_k R_k^T dT/dx_k
+// (fractional), dT/dU = sum_k R_k^T G_k R_k with G_k the symmetric gradient of the image's U
+// (off-diagonal entries of the 6-vector count twice), occupancy and the multipole rows (local
+// frame, Pval, kappa, kappa') added.
+static void fold_rows(const std::vector<EngineOp>& reps, int n, int nd, const double* img, double* out)
+{
+    const int K = (int)reps.size();
+    for (int i = 0; i < n; i++) {
+        double* r = out + (size_t)nd * i;
+        for (int j = 0; j < nd; j++) r[j] = 0.0;
+        for (int k = 0; k < K; k++) {
+            const EngineOp& g = reps[k];
+            const double* q = img + (size_t)nd * ((size_t)i * K + k);
+            for (int c = 0; c < 3; c++)
+                for (int m = 0; m < 3; m++) r[c] += g.rf[3 * m + c] * q[m];
+            const double G[9] = {q[3], 0.5 * q[6], 0.5 * q[7], 0.5 * q[6], q[4], 0.5 * q[8], 0.5 * q[7], 0.5 * q[8], q[5]};
+            double GR[9], M[9];
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) {
+                    double s = 0.0;
+                    for (int m = 0; m < 3; m++) s += G[3 * a + m] * g.rc[3 * m + b];
+                    GR[3 * a + b] = s;
+                }
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) {
+                    double s = 0.0;
+                    for (int m = 0; m < 3; m++) s += g.rc[3 * m + a] * GR[3 * m + b];
+                    M[3 * a + b] = s;
+                }
+            r[3] += M[0];
+            r[4] += M[4];
+            r[5] += M[8];
+            r[6] += M[1] + M[3];
+            r[7] += M[2] + M[6];
+            r[8] += M[5] + M[7];
+            for (int j = 9; j < nd; j++) r[j] += q[j];
+        }
+    }
+}
+
+// Gradient of T with respect to each atom's frame matrix L from its images' (frame R_k L):
+// dT/dL = sum_k R_k^T dT/dL_k.
+static void fold_frames(const std::vector<EngineOp>& reps, int n, const double* img, double* out)
+{
+    const int K = (int)reps.size();
+    for (int i = 0; i < n; i++)
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                double s = 0.0;
+                for (int k = 0; k < K; k++)
+                    for (int m = 0; m < 3; m++)
+                        s += reps[k].rc[3 * m + a] * img[9 * ((size_t)i * K + k) + 3 * m + b];
+                out[9 * (size_t)i + 3 * a + b] = s;
+            }
+}
+
 static std::vector<EngineOp> make_ops(const Cell& c, const SpaceGroupInfo& g, bool expand_centre,
                                       double partner_anom = -1.0)
 {
@@ -231,6 +454,11 @@ static Calculator* create(const dh_model_desc* d, int device)
     bool centric_shift = c->sg.centro && !zero_t;
     std::vector<EngineOp> eops = make_ops(c->cell, c->sg, centric_shift);
     c->engine.reset(new Engine(device, c->model, d->cell, eops, c->centric_origin));
+    std::vector<EngineOp> h_ops;
+    if (coset_plan(eops, &h_ops, &c->cos_reps))
+        c->cos_engine.reset(new Engine(device, c->model, d->cell, h_ops, c->centric_origin));
+    else
+        c->cos_reps.clear();
     c->device = device;
     for (int w = 0; w < d->n_wfn; w++) {
         c->wfn_label.push_back(d->wfn_label && d->wfn_label[w] ? std::string(d->wfn_label[w]) : std::string());
@@ -255,8 +483,10 @@ static void ensure_iam_engine(Calculator* c)
 static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl, const double* dtdf,
                    bool rows = false)
 {
-    // whatever fails below, the engine keeps no inputs of an earlier call to run on
+    // whatever fails below, the engines keep no inputs of an earlier call to run on
     c->eng().invalidate_inputs();
+    Engine& E = rows ? c->eng() : c->host_eng();
+    if (&E != &c->eng()) E.invalidate_inputs();
     if (!at) throw Error(DH_E_ARGUMENT, "null atoms");
     const int n = at->n_atoms;
     if (n != (int)c->atom_type.size())
@@ -355,8 +585,13 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
         }
     }
     std::memcpy(dd.data() + 2 * (size_t)n_hkl, mk, (size_t)n_hkl * sizeof(double));
-    c->eng().line_order = true;     // both precisions carry phases along hkl lines (groups of <= 4 operations)
-    c->eng().upload(ea, n_hkl, hkl, dd.data(), rows);
+    E.line_order = true;     // both precisions carry phases along hkl lines (groups of <= 4 operations)
+    if (!rows && c->cosets()) {
+        expand_atoms(c->cos_reps, ea, &c->cos_atoms);
+        E.upload(c->cos_atoms, n_hkl, hkl, dd.data(), rows);
+    } else {
+        E.upload(ea, n_hkl, hkl, dd.data(), rows);
+    }
     c->n_atoms = n;
     c->n_hkl = n_hkl;
     if (!rows) return;
@@ -406,6 +641,7 @@ static void rows_to_host(Calculator* c, int k0, int k1, double* f, double* rows)
         c->rows_dev[0].ensure((size_t)n * row * sizeof(double));
         c->f_dev[0].ensure((size_t)n * 2 * sizeof(double));
         c->eng().jacobian(b0, b1, c->f_dev[0].as<double>(), c->rows_dev[0].as<double>(), c->prec);
+        c->last_run = &c->eng();
         if (f)
             HIP_CHECK(hipMemcpyAsync(f + 2 * (size_t)(b0 - k0), c->f_dev[0].p, (size_t)n * 2 * sizeof(double),
                                      hipMemcpyDeviceToHost, st));
@@ -438,6 +674,7 @@ static void per_hkl(Calculator* c, dh_per_hkl_fn fn, void* user)
         if (b < nb) {
             const int q = b & 1, b0 = b * B, b1 = std::min(n_hkl, b0 + B), n = b1 - b0;
             c->eng().jacobian(b0, b1, c->f_dev[q].as<double>(), c->rows_dev[q].as<double>(), c->prec);
+            c->last_run = &c->eng();
             HIP_CHECK(hipMemcpyAsync(c->f_host[q].p, c->f_dev[q].p, (size_t)n * 2 * sizeof(double),
                                      hipMemcpyDeviceToHost, st));
             HIP_CHECK(hipMemcpyAsync(c->rows_host[q].p, c->rows_dev[q].p, (size_t)n * row * sizeof(double),
@@ -462,9 +699,17 @@ static void fetch(Calculator* c, double* f, double* d_xyz, double* d_adp, double
 {
     const int n = c->n_atoms;
     const bool want_d = c->last_selector != 0;
-    const int nd = c->eng().nd();
+    Engine& E = c->host_eng();
+    const int nd = E.nd();
     c->d10.assign((size_t)std::max(n, 1) * nd, 0.0);
-    c->eng().download(f, want_d ? c->d10.data() : nullptr);
+    if (c->cosets() && want_d) {
+        // the images' rows, folded back onto their atoms
+        c->cos_rows.resize((size_t)std::max(n, 1) * c->cos_reps.size() * nd);
+        E.download(f, c->cos_rows.data());
+        fold_rows(c->cos_reps, n, nd, c->cos_rows.data(), c->d10.data());
+    } else {
+        E.download(f, want_d ? c->d10.data() : nullptr);
+    }
     c->have_mp = false;
     if (!want_d) {
         if (d_xyz) std::memset(d_xyz, 0, sizeof(double) * 3 * n);
@@ -477,7 +722,13 @@ static void fetch(Calculator* c, double* f, double* d_xyz, double* d_adp, double
     std::vector<double> chain;
     if (c->lcs_chain && !c->lcs_defs.empty() && (c->last_selector & DH_D_XYZ)) {
         std::vector<double> g(9 * (size_t)n);
-        c->eng().download_lcs_grad(g.data());
+        if (c->cosets()) {
+            std::vector<double> gi(9 * (size_t)n * c->cos_reps.size());
+            E.download_lcs_grad(gi.data());
+            fold_frames(c->cos_reps, n, gi.data(), g.data());
+        } else {
+            E.download_lcs_grad(g.data());
+        }
         chain.assign(3 * (size_t)n, 0.0);
         // per-frame contributions on host threads, then one serial scatter
         std::vector<int> who((size_t)n * kLcsMaxAtoms), cnt(n);
@@ -664,8 +915,8 @@ static void sf_dev(Calculator* c, const dh_atoms_dev* at, const double* dtdf, un
 static bool run_mp(Calculator* c, unsigned selector)
 {
     const bool chain = c->lcs_chain && !c->lcs_defs.empty() && (selector & DH_D_XYZ) && !c->iam;
-    c->eng().lcs_grad = chain;
-    c->eng().mp_scalars = (selector & DH_D_MULTIPOLE) != 0;
+    c->host_eng().lcs_grad = chain;
+    c->host_eng().mp_scalars = (selector & DH_D_MULTIPOLE) != 0;
     return (selector & DH_D_MULTIPOLE) != 0 || chain;
 }
 
@@ -727,6 +978,7 @@ static void set_electron(Calculator* c, bool on)
         }
     }
     c->engine->set_electron(on, z_f0);
+    if (c->cos_engine) c->cos_engine->set_electron(on, z_f0);
     c->electron = on;
 }
 
@@ -740,6 +992,7 @@ struct dh_calculator {
 
 struct dh_batch {
     std::vector<Calculator*> calcs;
+    std::vector<Engine*> engines;     // the calculators' host-call engines at creation
     std::unique_ptr<BatchEngine> engine;
 };
 
@@ -823,6 +1076,7 @@ int dh_hc_set_iam_table(dh_calculator* calc, const char* table)
         ngauss_lookup("H", t, nullptr);     // rejects an unknown table name
         if (t == c->iam_table) return;
         c->iam_table = t;
+        if (c->last_run == c->iam_engine.get()) c->last_run = nullptr;
         c->iam_engine.reset();
         if (c->iam) {
             ensure_iam_engine(c);
@@ -878,6 +1132,7 @@ int dh_hc_set_tuning(dh_calculator* calc, int hkl_chunk)
         if (!calc || hkl_chunk < 0) throw Error(DH_E_ARGUMENT, "invalid tuning");
         calc->c->engine->tune.hkl_chunk = hkl_chunk;
         if (calc->c->iam_engine) calc->c->iam_engine->tune.hkl_chunk = hkl_chunk;
+        if (calc->c->cos_engine) calc->c->cos_engine->tune.hkl_chunk = hkl_chunk;
     });
 }
 
@@ -888,7 +1143,7 @@ int dh_hc_set_options(dh_calculator* calc, unsigned flags)
                              DH_OPT_DIRECT_PHASES;
         if (!calc || (flags & ~all) || ((flags & DH_OPT_RADIAL_RECORDS) && (flags & DH_OPT_RADIAL_TABLES)))
             throw Error(DH_E_ARGUMENT, "invalid options");
-        for (Engine* e : {calc->c->engine.get(), calc->c->iam_engine.get()}) {
+        for (Engine* e : {calc->c->engine.get(), calc->c->iam_engine.get(), calc->c->cos_engine.get()}) {
             if (!e) continue;
             e->tune.general_path = (flags & DH_OPT_GENERAL_ROTATIONS) != 0;
             e->tune.radial_mode = (flags & DH_OPT_RADIAL_RECORDS) ? 1 : (flags & DH_OPT_RADIAL_TABLES) ? 2 : 0;
@@ -911,7 +1166,9 @@ int dh_hc_run(dh_calculator* calc, unsigned selector)
         if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
         calc->c->last_selector = selector & (DH_D_ALL | DH_D_MULTIPOLE);
         check_selector(calc->c, selector);
-        calc->c->eng().run(calc->c->prec, calc->c->last_selector != 0, run_mp(calc->c, calc->c->last_selector));
+        Engine& e = calc->c->host_eng();
+        e.run(calc->c->prec, calc->c->last_selector != 0, run_mp(calc->c, calc->c->last_selector));
+        calc->c->last_run = &e;
     });
 }
 
@@ -932,7 +1189,9 @@ int dh_hc_sf_and_derivs(dh_calculator* calc, const dh_atoms* atoms, int n_hkl, c
         upload(calc->c, atoms, n_hkl, hkl, dtarget_df);
         calc->c->last_selector = selector & (DH_D_ALL | DH_D_MULTIPOLE);
         check_selector(calc->c, selector);
-        calc->c->eng().run(calc->c->prec, calc->c->last_selector != 0, run_mp(calc->c, calc->c->last_selector));
+        Engine& e = calc->c->host_eng();
+        e.run(calc->c->prec, calc->c->last_selector != 0, run_mp(calc->c, calc->c->last_selector));
+        calc->c->last_run = &e;
         fetch(calc->c, f, d_xyz, d_adp, d_occ);
     });
 }
@@ -980,7 +1239,7 @@ int dh_hc_autotune(dh_calculator* calc, unsigned selector, int* chunk)
     return guarded([&] {
         if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
         check_selector(calc->c, selector);
-        int c = calc->c->eng().autotune(calc->c->prec, (selector & (DH_D_ALL | DH_D_MULTIPOLE)) != 0,
+        int c = calc->c->host_eng().autotune(calc->c->prec, (selector & (DH_D_ALL | DH_D_MULTIPOLE)) != 0,
                                           run_mp(calc->c, selector));
         if (chunk) *chunk = c;
     });
@@ -1003,6 +1262,7 @@ int dh_hc_set_type_parameters(dh_calculator* calc, int n_types, const double* pl
         std::vector<AtomType> et(c->model.n_types);
         for (int e = 0; e < c->model.n_types; e++) et[e] = t[c->model.type_origin[e]];
         c->engine->set_types(et);
+        if (c->cos_engine) c->cos_engine->set_types(et);
         c->types = t;
         c->model = c->engine->model();
     });
@@ -1029,6 +1289,7 @@ int dh_hc_set_timing(dh_calculator* calc, int on)
         if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
         calc->c->engine->timing = on != 0;
         if (calc->c->iam_engine) calc->c->iam_engine->timing = on != 0;
+        if (calc->c->cos_engine) calc->c->cos_engine->timing = on != 0;
     });
 }
 
@@ -1036,7 +1297,7 @@ int dh_hc_last_timing(dh_calculator* calc, double* ms, int* gx, int* gy, int* ch
 {
     return guarded([&] {
         if (!calc) throw Error(DH_E_ARGUMENT, "null calculator");
-        Engine& e = calc->c->eng();
+        Engine& e = calc->c->last_run ? *calc->c->last_run : calc->c->eng();
         if (ms) *ms = e.last_main_ms;
         if (gx) *gx = e.last_grid_x;
         if (gy) *gy = e.last_grid_y;
@@ -1098,6 +1359,7 @@ int dh_hc_jacobian_rows_dev(dh_calculator* calc, int k0, int k1, void* f_dev, vo
         HIP_CHECK(hipEventRecord(c->ev_in, caller));
         HIP_CHECK(hipStreamWaitEvent(st, c->ev_in, 0));
         c->eng().jacobian(k0, k1, static_cast<double*>(f_dev), static_cast<double*>(rows_dev), c->prec);
+        c->last_run = &c->eng();
         HIP_CHECK(hipEventRecord(c->ev_out, st));
         HIP_CHECK(hipStreamWaitEvent(caller, c->ev_out, 0));
     });
@@ -1117,6 +1379,7 @@ static void form_factors(Calculator* c, const double* lcs, const unsigned char* 
         throw Error(DH_E_ARGUMENT, "using functionality available only for CPU calcluations of multipole model form "
                                    "factors");
     c->engine->form_factors(n, c->atom_type.data(), frames.data(), n_h, h_cart, f);
+    c->last_run = c->engine.get();
     if (include)   // countAtom: excluded atoms get 0 (HansenCoppens_SF_Engine.cpp:467-471)
         for (int k = 0; k < n_h; k++)
             for (int i = 0; i < n; i++)
@@ -1162,8 +1425,10 @@ int dh_batch_create(dh_calculator* const* calcs, int n, dh_batch** out)
                 throw Error(DH_E_ARGUMENT, "batches take multipole-model X-ray calculators (not CPU_IAM, electron "
                                            "or the local-frame chain rule)");
             b->calcs.push_back(calcs[i]->c);
-            eng.push_back(calcs[i]->c->engine.get());
+            // the engine that holds the calculator's host upload (its coset engine where it has one)
+            eng.push_back(&calcs[i]->c->host_eng());
         }
+        b->engines = eng;
         b->engine.reset(new BatchEngine(eng));
         *out = b.release();
     });
@@ -1174,11 +1439,15 @@ void dh_batch_destroy(dh_batch* batch) { delete batch; }
 static Precision batch_precision(dh_batch* b)
 {
     const Precision p = b->calcs[0]->prec;
-    for (Calculator* c : b->calcs) {
+    for (size_t i = 0; i < b->calcs.size(); i++) {
+        Calculator* c = b->calcs[i];
         if (c->prec != p) throw Error(DH_E_ARGUMENT, "calculators of a batch must share the calculation mode");
         if (c->iam || c->electron || c->lcs_chain)
             throw Error(DH_E_ARGUMENT, "batches take multipole-model X-ray calculators (not CPU_IAM, electron or the "
                                        "local-frame chain rule)");
+        if (&c->host_eng() != b->engines[i])
+            throw Error(DH_E_ARGUMENT, "a calculator of the batch changed its kernel options since the batch was "
+                                       "created");
     }
     return p;
 }

@@ -56,7 +56,7 @@ def test_frames_follow_positions(sg):
 
 
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
-@pytest.mark.parametrize("sg", ["P21/c", "P212121"])
+@pytest.mark.parametrize("sg", ["P21/c", "P212121", "P43212"])
 def test_chain_rule_matches_finite_differences(sg, mode):
     w = synth.small_structure(sg, n_atoms=6, d_min=1.3, seed=8)
     defs = definitions(6, 2)

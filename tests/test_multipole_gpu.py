@@ -26,12 +26,13 @@ def _oracle_T(w, params, dtdf):
 
 @pytest.mark.parametrize("radial", ["auto", "records"])
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
-@pytest.mark.parametrize("sg", ["P21/c", "P212121"])
+@pytest.mark.parametrize("sg", ["P21/c", "P212121", "P43212"])
 def test_multipole_derivatives_vs_oracle_finite_differences(sg, mode, radial):
     """Multipole-parameter derivatives against central differences of the oracle.  radial
     "records" orders the reflections along hkl lines, so both multipole passes run the line
     kernels (2/m and 222 sign sets); "auto" takes per-atom table lookups in |h| order for the
-    first pass' F and the direct kernels."""
+    first pass' F and the direct kernels.  P4_3 2_1 2 runs the 222 kernels over the atoms' images
+    under the coset representatives of its 222 subgroup and folds their derivatives back."""
     import copy
     w = synth.small_structure(sg, n_atoms=8, d_min=1.2)
     print("case multipole fd", sg, mode, radial, flush=True)

@@ -102,10 +102,11 @@ struct Calculator {
     Engine& eng() { return iam ? *iam_engine : *engine; }
     bool eng_centric_origin() const { return !iam && centric_origin; }
     // Groups whose operations go beyond sign matrices but contain a diagonal subgroup H of two or
-    // four operations (tetragonal, hexagonal, cubic): host calls run on cos_engine, built with the
-    // operations of H, over the images g(x) of every atom under the coset representatives
-    // cos_reps (coset_plan), and fold the images' derivatives back.  Device calls, per-reflection
-    // rows, form factors and batched steps keep the main engine and the full operation list.
+    // four operations (tetragonal, hexagonal, cubic): the summed calls (host and device-resident)
+    // run on cos_engine, built with the operations of H, over the images g(x) of every atom under
+    // the coset representatives cos_reps (coset_plan), and fold the images' derivatives back;
+    // batched steps take it too.  Per-reflection rows and form factors keep the main engine and
+    // the full operation list.
     std::unique_ptr<Engine> cos_engine;
     std::vector<EngineOp> cos_reps;
     EngineAtoms cos_atoms;
@@ -808,6 +809,8 @@ static void fetch(Calculator* c, double* f, double* d_xyz, double* d_adp, double
 static void set_reflections(Calculator* c, int n_hkl, const int* hkl)
 {
     c->eng().invalidate_inputs();
+    Engine& E = c->host_eng();         // the engine of the summed calls (its coset engine where it has one)
+    if (&E != &c->eng()) E.invalidate_inputs();
     c->rows_uploaded = false;
     if (n_hkl < 0) throw Error(DH_E_ARGUMENT, "negative number of reflections");
     if (n_hkl > 0 && !hkl) throw Error(DH_E_ARGUMENT, "null Miller indices");
@@ -849,13 +852,35 @@ static void set_reflections(Calculator* c, int n_hkl, const int* hkl)
         }
     }
     conv.centric_origin = c->eng_centric_origin() ? 1 : 0;
-    c->eng().line_order = true;
-    c->eng().prepare_dev(ea, n_hkl, hkl, mult.data(), c->mult_w, anom, conv, c->prec);
+    E.line_order = true;
+    std::vector<int> sum_type = c->atom_type;
+    if (c->cosets()) {
+        // K images of every atom (image k of atom i at i K + k), built per call from the caller's
+        // arrays by k_dev_atoms
+        const int K = (int)c->cos_reps.size();
+        EngineAtoms im;
+        expand_atoms(c->cos_reps, ea, &im);
+        std::vector<double> mw((size_t)n * K), an(2 * (size_t)n * K);
+        sum_type.resize((size_t)n * K);
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < K; k++) {
+                const size_t o = (size_t)i * K + k;
+                mw[o] = c->mult_w[i];
+                an[2 * o] = anom[2 * (size_t)i];
+                an[2 * o + 1] = anom[2 * (size_t)i + 1];
+                sum_type[o] = c->atom_type[i];
+            }
+        E.prepare_dev(im, n_hkl, hkl, mult.data(), mw, an, conv, c->prec);
+        E.set_dev_cosets(c->cos_reps);
+    } else {
+        E.prepare_dev(ea, n_hkl, hkl, mult.data(), c->mult_w, anom, conv, c->prec);
+        E.set_dev_cosets({});
+    }
     if (!c->iam) {
         // the per-type sums of the multipole derivatives (dh_hc_sf_and_derivs_mp_dev)
         std::vector<int> n_l(c->types.size());
         for (size_t t = 0; t < c->types.size(); t++) n_l[t] = c->types[t].n_l;
-        c->eng().set_type_sum_map(c->atom_type, n_l);
+        E.set_type_sum_map(sum_type, n_l);
     }
     c->n_atoms = n;
     c->n_hkl = n_hkl;
@@ -906,7 +931,7 @@ static void sf_dev(Calculator* c, const dh_atoms_dev* at, const double* dtdf, un
         call.d_kappa = tsum[2];
         call.d_kappa_prime = tsum[3];
     }
-    Engine& e = c->eng();
+    Engine& e = c->host_eng();
     e.run_dev(call, c->prec, static_cast<hipStream_t>(stream), no_alloc ? e.scratch() : nullptr,
               no_alloc ? Engine::scratch_bytes() : 0);
 }

@@ -4411,10 +4411,18 @@ struct DevAtomArgs {
     const double* lcs;
     const double* anom;
     const unsigned char* include;
-    const double* mult_w;        // m / |G| per atom
-    const double* anom_default;  // f', f'' per atom from its wavefunction type
+    const double* mult_w;        // m / |G| per atom (per image with coset representatives)
+    const double* anom_default;  // f', f'' per atom from its wavefunction type (per image)
     int n_slots, n_waves;
+    // coset representatives (set_dev_cosets): engine atom i is image i % K of caller atom i / K,
+    // representative k = reps[21 k ..]: fractional rotation (9), translation (3), Cartesian
+    // rotation (9).  K = 1: no images.
+    int K;
+    const double* reps;
 };
+
+// caller atom of engine atom (image) i
+__device__ __forceinline__ int dev_atom(const DevAtomArgs& a, int i) { return a.K > 1 ? i / a.K : i; }
 
 // AtomIn of every slot from the caller's arrays: the slot layout (atom index, type, padding)
 // comes from the skeleton prepare_dev uploaded, the values are converted to the engine's
@@ -4427,8 +4435,9 @@ __global__ void k_dev_atoms(const AtomIn* __restrict__ skel, const int* __restri
     if (s < a.n_waves) out_cls[s] = skel_cls[s];
     if (s >= a.n_slots) return;
     AtomIn r = skel[s];
-    const int i = r.orig;
-    if (i >= 0) {
+    const int img = r.orig;
+    if (img >= 0) {
+        const int i = dev_atom(a, img);    // the caller's atom
         const double* x = a.xyz + 3 * (size_t)i;
         if (c.xyz_identity) {
             for (int j = 0; j < 3; j++) r.xf[j] = x[j];
@@ -4449,11 +4458,39 @@ __global__ void k_dev_atoms(const AtomIn* __restrict__ skel, const int* __restri
             for (int j = 0; j < 3; j++) r.u[j] = u[0] * two_pi2;
         }
         const bool inc = a.include ? a.include[i] != 0 : true;
-        r.w = inc ? a.occ[i] * a.mult_w[i] : 0.0;
-        const double* an = a.anom ? a.anom + 2 * (size_t)i : a.anom_default + 2 * (size_t)i;
+        r.w = inc ? a.occ[i] * a.mult_w[img] : 0.0;
+        const double* an = a.anom ? a.anom + 2 * (size_t)i : a.anom_default + 2 * (size_t)img;
         r.anom[0] = an[0];
         r.anom[1] = an[1];
         for (int j = 0; j < 9; j++) r.lcs[j] = a.lcs ? a.lcs[9 * (size_t)i + j] : (j % 4 == 0 ? 1.0 : 0.0);
+        if (a.K > 1) {
+            // image under representative k: position R x + t, U -> R U R^T, frame R L
+            const double* g = a.reps + 21 * (size_t)(img % a.K);
+            double xf[3], L[9];
+            for (int p = 0; p < 3; p++) xf[p] = g[9 + p] + g[3 * p] * r.xf[0] + g[3 * p + 1] * r.xf[1] + g[3 * p + 2] * r.xf[2];
+            for (int p = 0; p < 3; p++) r.xf[p] = xf[p];
+            const double* R = g + 12;
+            if (na == 6) {
+                const double U[9] = {r.u[0], r.u[3], r.u[4], r.u[3], r.u[1], r.u[5], r.u[4], r.u[5], r.u[2]};
+                double RU[9], V[9];
+                for (int p = 0; p < 3; p++)
+                    for (int q = 0; q < 3; q++)
+                        RU[3 * p + q] = R[3 * p] * U[q] + R[3 * p + 1] * U[3 + q] + R[3 * p + 2] * U[6 + q];
+                for (int p = 0; p < 3; p++)
+                    for (int q = 0; q < 3; q++)
+                        V[3 * p + q] = RU[3 * p] * R[3 * q] + RU[3 * p + 1] * R[3 * q + 1] + RU[3 * p + 2] * R[3 * q + 2];
+                r.u[0] = V[0];
+                r.u[1] = V[4];
+                r.u[2] = V[8];
+                r.u[3] = 0.5 * (V[1] + V[3]);
+                r.u[4] = 0.5 * (V[2] + V[6]);
+                r.u[5] = 0.5 * (V[5] + V[7]);
+            }
+            for (int p = 0; p < 3; p++)
+                for (int q = 0; q < 3; q++)
+                    L[3 * p + q] = R[3 * p] * r.lcs[q] + R[3 * p + 1] * r.lcs[3 + q] + R[3 * p + 2] * r.lcs[6 + q];
+            for (int j = 0; j < 9; j++) r.lcs[j] = L[j];
+        }
     }
     out[s] = r;
 }
@@ -4493,9 +4530,39 @@ __global__ void k_dev_out(const double* __restrict__ fs, const int* __restrict__
     }
     if (idx >= n_atoms) return;
     const int i = idx;
-    const double* r = draw + (size_t)nd * i;
+    double r[10];
+    if (a.K > 1) {
+        // the images' rows folded back: xyz through R^T (fractional), the symmetric U gradient
+        // through R^T G R (Cartesian), occupancy added
+        for (int j = 0; j < 10; j++) r[j] = 0.0;
+        for (int k = 0; k < a.K; k++) {
+            const double* q = draw + (size_t)nd * ((size_t)i * a.K + k);
+            const double* g = a.reps + 21 * (size_t)k;
+            const double* R = g + 12;
+            for (int p = 0; p < 3; p++) r[p] += g[p] * q[0] + g[3 + p] * q[1] + g[6 + p] * q[2];
+            const double G[9] = {q[3], 0.5 * q[6], 0.5 * q[7], 0.5 * q[6], q[4], 0.5 * q[8], 0.5 * q[7], 0.5 * q[8], q[5]};
+            double M[9];
+            for (int p = 0; p < 3; p++)
+                for (int t = 0; t < 3; t++) {
+                    double v = 0.0;
+                    for (int m = 0; m < 3; m++)
+                        v += R[3 * m + p] * (G[3 * m] * R[t] + G[3 * m + 1] * R[3 + t] + G[3 * m + 2] * R[6 + t]);
+                    M[3 * p + t] = v;
+                }
+            r[3] += M[0];
+            r[4] += M[4];
+            r[5] += M[8];
+            r[6] += M[1] + M[3];
+            r[7] += M[2] + M[6];
+            r[8] += M[5] + M[7];
+            r[9] += q[9];
+        }
+    } else {
+        for (int j = 0; j < 10; j++) r[j] = draw[(size_t)nd * i + j];
+    }
+    const double mw = a.mult_w[(size_t)i * a.K];
     const bool inc = a.include ? a.include[i] != 0 : true;
-    const double w = inc ? a.occ[i] * a.mult_w[i] : 0.0;
+    const double w = inc ? a.occ[i] * mw : 0.0;
     if (dxyz) {
         double* o = dxyz + 3 * (size_t)i;
         if ((sel & 1u) && inc) {
@@ -4527,7 +4594,7 @@ __global__ void k_dev_out(const double* __restrict__ fs, const int* __restrict__
         }
         for (int j = 0; j < 6; j++) o[j] = out6[j];
     }
-    if (docc) docc[i] = ((sel & 4u) && inc) ? a.mult_w[i] * r[9] : 0.0;
+    if (docc) docc[i] = ((sel & 4u) && inc) ? mw * r[9] : 0.0;
 }
 
 // Per-type sums of a device call's multipole derivatives, as the host call forms them:
@@ -4549,8 +4616,8 @@ __global__ __launch_bounds__(256) void k_dev_type_sums(const double* __restrict_
     double s = 0.0;
     if (c < 28)
         for (int e = off[t] + q; e < off[t + 1]; e += 8) {
-            const int i = list[e];
-            if (a.include ? a.include[i] != 0 : true) s += a.occ[i] * a.mult_w[i] * draw[(size_t)nd * i + 10 + c];
+            const int img = list[e], i = dev_atom(a, img);
+            if (a.include ? a.include[i] != 0 : true) s += a.occ[i] * a.mult_w[img] * draw[(size_t)nd * img + 10 + c];
         }
     part[q][c] = s;
     __syncthreads();
@@ -4567,6 +4634,24 @@ __global__ __launch_bounds__(256) void k_dev_type_sums(const double* __restrict_
     } else if (dkp) {
         dkp[t] = v;
     }
+}
+
+void Engine::set_dev_cosets(const std::vector<EngineOp>& reps)
+{
+    HIP_CHECK(hipSetDevice(device_));
+    const int K = std::max<int>(1, (int)reps.size());
+    if (n_atoms_ % K != 0) throw Error(1, "internal: images of another atom count");
+    std::vector<double> r(21 * (size_t)K, 0.0);
+    for (size_t k = 0; k < reps.size(); k++) {
+        for (int j = 0; j < 9; j++) r[21 * k + j] = reps[k].rf[j];
+        for (int j = 0; j < 3; j++) r[21 * k + 9 + j] = reps[k].tf[j];
+        for (int j = 0; j < 9; j++) r[21 * k + 12 + j] = reps[k].rc[j];
+    }
+    // earlier calls on any stream may still read the representatives: wait before rewriting them
+    HIP_CHECK(hipDeviceSynchronize());
+    d_dev_reps_.ensure(r.size() * sizeof(double));
+    HIP_CHECK(hipMemcpy(d_dev_reps_.p, r.data(), r.size() * sizeof(double), hipMemcpyHostToDevice));
+    dev_K_ = K;
 }
 
 void Engine::set_type_sum_map(const std::vector<int>& atom_type, const std::vector<int>& n_l)
@@ -4691,7 +4776,9 @@ void Engine::run_dev(const DevCall& call, Precision prec, hipStream_t stream, vo
     HIP_CHECK(hipSetDevice(device_));
     if (!dev_ready_ || !uploaded_)
         throw Error(1, "device-resident call: set the reflections first (dh_hc_set_reflections)");
-    const int n = n_atoms_, n_hkl = n_hkl_;
+    // with coset representatives (set_dev_cosets) the engine's atoms are the K images of each of
+    // the caller's n atoms
+    const int n_img = n_atoms_, n = n_atoms_ / dev_K_, n_hkl = n_hkl_;
     if (n > 0 && (!call.xyz || !call.adp_size || !call.adp || !call.occ)) throw Error(1, "null atom array");
     if (n_hkl > 0 && !call.f) throw Error(1, "null structure-factor output");
     if (n_hkl > 0 && call.selector && !call.dtdf) throw Error(1, "null dT/dF");
@@ -4712,7 +4799,7 @@ void Engine::run_dev(const DevCall& call, Precision prec, hipStream_t stream, vo
         const size_t b_atoms = align256(n_ain * sizeof(AtomRec));
         const size_t b_dtdf = align256((size_t)std::max(n_hkl, 1) * 3 * sizeof(double));
         const size_t b_f = align256((size_t)std::max(n_hkl, 1) * 2 * sizeof(double));
-        const size_t b_d = align256((size_t)std::max(n, 1) * kNdBasic * sizeof(double));
+        const size_t b_d = align256((size_t)std::max(n_img, 1) * kNdBasic * sizeof(double));
         const size_t b_fpart = align256((size_t)n_waves * n_hkl * 2 * (prec == kMixed ? sizeof(float) : sizeof(double)));
         const size_t fixed = b_ain + b_atoms + b_dtdf + b_f + b_d + b_fpart;
         const size_t per_chunk = (size_t)n_sorted_ * kNdBasic * sizeof(double);
@@ -4757,9 +4844,11 @@ void Engine::run_dev(const DevCall& call, Precision prec, hipStream_t stream, vo
     aa.anom = call.anom;
     aa.include = call.include;
     aa.mult_w = d_dev_atomc_.as<double>();
-    aa.anom_default = d_dev_atomc_.as<double>() + n;
+    aa.anom_default = d_dev_atomc_.as<double>() + n_img;   // [m/|G| | f', f''] per engine atom
     aa.n_slots = n_sorted_;
     aa.n_waves = n_waves;
+    aa.K = dev_K_;
+    aa.reps = d_dev_reps_.as<double>();
     const int T = 256;
     if (n_sorted_ > 0) {
         int* out_cls = reinterpret_cast<int*>(static_cast<char*>(d_atoms_in_.p) + n_ain * sizeof(AtomIn));
@@ -4787,7 +4876,7 @@ void Engine::run_dev(const DevCall& call, Precision prec, hipStream_t stream, vo
     }
     if (mp) {
         const int* list = d_tsum_map_.as<int>();
-        const int* off = list + n;
+        const int* off = list + n_img;
         hipLaunchKernelGGL(k_dev_type_sums, dim3(tsum_types_), dim3(256), 0, stream_, d_d_.as<double>(), last_nd_,
                            list, off, off + tsum_types_ + 1, aa, call.d_plm, call.d_pval, call.d_kappa,
                            call.d_kappa_prime);

@@ -211,6 +211,10 @@ public:
     // the calculator type of every atom and each type's multipole order + 1 (P_lm entries with
     // l >= n_l are not parameters): the per-type sums of the device calls' multipole derivatives
     void set_type_sum_map(const std::vector<int>& atom_type, const std::vector<int>& n_l);
+    // device-resident calls over the images of the caller's atoms under coset representatives:
+    // prepare_dev laid out K images per atom (image k of atom i at i K + k); run_dev builds them
+    // from the caller's arrays and folds their derivatives back.  Empty: no images.
+    void set_dev_cosets(const std::vector<EngineOp>& reps);
     bool dev_ready() const { return dev_ready_; }
     // the static scratch array of the allocation-free calls on this engine's device, its size
     void* scratch() const { return scratch_; }
@@ -274,6 +278,8 @@ private:
     DevConv dev_conv_;
     DeviceBuffer d_dev_perm_, d_dev_mult_, d_dev_atomc_;
     DeviceBuffer d_tsum_map_;       // set_type_sum_map: atoms by type, type offsets, n_l per type
+    DeviceBuffer d_dev_reps_;       // set_dev_cosets: 21 doubles per representative
+    int dev_K_ = 1;
     int tsum_types_ = 0;
     void* scratch_ = nullptr;       // device address of the static scratch array
     std::vector<int> order_;        // sorted slot -> caller atom index (-1 padding)

@@ -207,7 +207,7 @@ def test_autograd_device_path_config1_vs_oracle(mode):
 
 
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
-@pytest.mark.parametrize("sg", ["P1", "P21/c", "P212121"])
+@pytest.mark.parametrize("sg", ["P1", "P21/c", "P212121", "P43212"])
 def test_autograd_device_multipole_gradients_vs_host_path(sg, mode):
     """Every parameter as a GPU tensor: the backward runs dh_hc_sf_and_derivs_mp_dev (multipole
     adjoint and per-type sums on the device) and gives the gradients of the host path (parameters

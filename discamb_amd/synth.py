@@ -28,6 +28,10 @@ SPACE_GROUPS = {
     "P43212": ["x,y,z", "-x,-y,z+1/2", "-y+1/2,x+1/2,z+3/4", "y+1/2,-x+1/2,z+1/4", "-x+1/2,y+1/2,-z+3/4",
                "x+1/2,-y+1/2,-z+1/4", "y,x,-z", "-y,-x,-z+1/2"],
     "P31": ["x,y,z", "-y,x-y,z+1/3", "-x+y,-x,z+2/3"],
+    # a two-operation diagonal subgroup {1, 2_1 along c} under three (P6_3, hexagonal axes) or two
+    # (P4/m, centrosymmetric) coset representatives
+    "P63": ["x,y,z", "-y,x-y,z", "-x+y,-x,z", "-x,-y,z+1/2", "y,-x+y,z+1/2", "x-y,x,z+1/2"],
+    "P4/m": ["x,y,z", "-x,-y,z", "-y,x,z", "y,-x,z", "-x,-y,-z", "x,y,-z", "y,-x,-z", "-y,x,-z"],
 }
 
 
@@ -245,9 +249,9 @@ def small_structure(sg, n_atoms=12, d_min=1.0, seed=7, cell=None, lmax_heavy=4):
             cell = (7.5, 8.0, 9.5, 92.0, 104.0, 88.0)
         elif sg in ("P21/c", "C2/c"):
             cell = (8.0, 9.0, 10.0, 90.0, 105.0, 90.0)
-        elif sg == "P43212":
+        elif sg in ("P43212", "P4/m"):
             cell = (8.5, 8.5, 10.0, 90.0, 90.0, 90.0)
-        elif sg == "P31":
+        elif sg in ("P31", "P63"):
             cell = (8.5, 8.5, 10.0, 90.0, 90.0, 120.0)
         else:
             cell = (8.0, 9.0, 10.0, 90.0, 90.0, 90.0)

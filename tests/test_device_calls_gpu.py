@@ -89,7 +89,7 @@ def _check_case(w, mode, include=None, anomalous=None, selector=None):
     return calc, f, dx, du, do
 
 
-CASES = ["P1", "P-1", "P21/c", "C2/c", "P212121", "P43212", "P31", "P-1_shifted", "I222"]
+CASES = ["P1", "P-1", "P21/c", "C2/c", "P212121", "P43212", "P31", "P63", "P4/m", "P-1_shifted", "I222"]
 
 
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])

@@ -26,13 +26,14 @@ def _oracle_T(w, params, dtdf):
 
 @pytest.mark.parametrize("radial", ["auto", "records"])
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
-@pytest.mark.parametrize("sg", ["P21/c", "P212121", "P43212"])
+@pytest.mark.parametrize("sg", ["P21/c", "P212121", "P43212", "P63"])
 def test_multipole_derivatives_vs_oracle_finite_differences(sg, mode, radial):
     """Multipole-parameter derivatives against central differences of the oracle.  radial
     "records" orders the reflections along hkl lines, so both multipole passes run the line
     kernels (2/m and 222 sign sets); "auto" takes per-atom table lookups in |h| order for the
     first pass' F and the direct kernels.  P4_3 2_1 2 runs the 222 kernels over the atoms' images
-    under the coset representatives of its 222 subgroup and folds their derivatives back."""
+    under the coset representatives of its 222 subgroup and folds their derivatives back; P6_3
+    runs the 2/m-set kernels of its {1, 2_1} subgroup over three images in hexagonal axes."""
     import copy
     w = synth.small_structure(sg, n_atoms=8, d_min=1.2)
     print("case multipole fd", sg, mode, radial, flush=True)
@@ -207,7 +208,7 @@ def test_autograd_device_path_config1_vs_oracle(mode):
 
 
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
-@pytest.mark.parametrize("sg", ["P1", "P21/c", "P212121", "P43212"])
+@pytest.mark.parametrize("sg", ["P1", "P21/c", "P212121", "P43212", "P63"])
 def test_autograd_device_multipole_gradients_vs_host_path(sg, mode):
     """Every parameter as a GPU tensor: the backward runs dh_hc_sf_and_derivs_mp_dev (multipole
     adjoint and per-type sums on the device) and gives the gradients of the host path (parameters

@@ -114,6 +114,14 @@ struct Calculator {
     Engine* last_run = nullptr;       // the engine of the last run (dh_hc_last_timing)
     Engine& host_eng() { return iam ? *iam_engine : (cos_engine && !engine->tune.general_path) ? *cos_engine : *engine; }
     bool cosets() { return !iam && &host_eng() == cos_engine.get(); }
+    // Every engine of the calculator drops its inputs: after a new reflection list only the engine
+    // prepared for it may serve device-resident calls (which engine serves them changes with the
+    // kernel options and the IAM table, and another engine's older list could differ in length).
+    void invalidate_all()
+    {
+        for (Engine* e : {engine.get(), cos_engine.get(), iam_engine.get()})
+            if (e) e->invalidate_inputs();
+    }
     // state of the last upload
     int n_atoms = 0, n_hkl = 0;
     std::vector<int> adp_size;
@@ -808,9 +816,8 @@ static void fetch(Calculator* c, double* f, double* d_xyz, double* d_adp, double
 // and the conversion matrices between the calculator's conventions and the engine's.
 static void set_reflections(Calculator* c, int n_hkl, const int* hkl)
 {
-    c->eng().invalidate_inputs();
+    c->invalidate_all();
     Engine& E = c->host_eng();         // the engine of the summed calls (its coset engine where it has one)
-    if (&E != &c->eng()) E.invalidate_inputs();
     c->rows_uploaded = false;
     if (n_hkl < 0) throw Error(DH_E_ARGUMENT, "negative number of reflections");
     if (n_hkl > 0 && !hkl) throw Error(DH_E_ARGUMENT, "null Miller indices");

@@ -203,6 +203,31 @@ def test_device_call_errors():
         calc.fetch()
 
 
+def test_device_call_reflections_after_kernel_option_changes():
+    """Only the engine prepared by the latest setReflections serves device-resident calls.  In
+    P4_3 2_1 2 the general-rotation option switches between the coset engine and the general one;
+    a list set on the coset engine and then replaced while the option pointed at the other engine
+    is never used: switching back needs the reflections set again."""
+    w = _structure("P43212", n_atoms=10, d_min=1.0)
+    print("case device call after option changes P43212 atoms=%d hkl=%d" % (w.crystal.atoms.n, len(w.hkl)), flush=True)
+    short = np.ascontiguousarray(w.hkl[: len(w.hkl) // 2])
+    calc = HansenCoppensStructureFactorCalculator(w.crystal, w.params)
+    args = _dev(w.crystal.atoms, w.lcs)
+    calc.setReflections(short)                       # coset engine, shorter list
+    calc.setKernelOptions(general_rotations=True)
+    calc.setReflections(w.hkl)                       # general engine, full list
+    f_general = _np(calc.calculateOnDevice(**args)[0])
+    calc.setKernelOptions(general_rotations=False)
+    with pytest.raises(RuntimeError):                # the coset engine's shorter list was dropped
+        calc.calculateOnDevice(**args)
+    calc.setReflections(w.hkl)
+    f_coset = _np(calc.calculateOnDevice(**args)[0])
+    torch.cuda.synchronize()
+    ref = hc_oracle.sf_and_derivs(w.crystal, w.params, w.crystal.atoms, w.lcs, w.hkl, w.dtdf, n_threads=16)[0]
+    for f in (f_general, f_coset):
+        assert f.shape == ref.shape and rel_err(f, ref) <= 1e-10
+
+
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])
 def test_device_call_in_a_captured_graph(mode):
     """The device call captured into a torch CUDA graph and replayed after new coordinates are

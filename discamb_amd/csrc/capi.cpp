@@ -114,9 +114,9 @@ struct Calculator {
     Engine* last_run = nullptr;       // the engine of the last run (dh_hc_last_timing)
     Engine& host_eng() { return iam ? *iam_engine : (cos_engine && !engine->tune.general_path) ? *cos_engine : *engine; }
     bool cosets() { return !iam && &host_eng() == cos_engine.get(); }
-    // Every engine of the calculator drops its inputs: after a new reflection list only the engine
-    // prepared for it may serve device-resident calls (which engine serves them changes with the
-    // kernel options and the IAM table, and another engine's older list could differ in length).
+    // Every engine of the calculator drops its inputs: after an upload or a new reflection list
+    // only the engine that took it may run (which engine serves a call changes with the kernel
+    // options and the IAM table, and another engine's older inputs could differ in size).
     void invalidate_all()
     {
         for (Engine* e : {engine.get(), cos_engine.get(), iam_engine.get()})
@@ -493,9 +493,8 @@ static void upload(Calculator* c, const dh_atoms* at, int n_hkl, const int* hkl,
                    bool rows = false)
 {
     // whatever fails below, the engines keep no inputs of an earlier call to run on
-    c->eng().invalidate_inputs();
+    c->invalidate_all();
     Engine& E = rows ? c->eng() : c->host_eng();
-    if (&E != &c->eng()) E.invalidate_inputs();
     if (!at) throw Error(DH_E_ARGUMENT, "null atoms");
     const int n = at->n_atoms;
     if (n != (int)c->atom_type.size())

@@ -226,6 +226,21 @@ def test_device_call_reflections_after_kernel_option_changes():
     ref = hc_oracle.sf_and_derivs(w.crystal, w.params, w.crystal.atoms, w.lcs, w.hkl, w.dtdf, n_threads=16)[0]
     for f in (f_general, f_coset):
         assert f.shape == ref.shape and rel_err(f, ref) <= 1e-10
+    # the split host calls likewise: an upload replaces the inputs of every engine
+    print("case host upload/run after option changes P43212", flush=True)
+    a = w.crystal.atoms
+    calc.upload(a, w.lcs, short, w.dtdf[: len(short)])      # coset engine, shorter list
+    calc.run()
+    calc.fetch()
+    calc.setKernelOptions(general_rotations=True)
+    calc.upload(a, w.lcs, w.hkl, w.dtdf)                     # general engine, full list
+    calc.setKernelOptions(general_rotations=False)
+    with pytest.raises(RuntimeError):                        # the coset engine's upload was dropped
+        calc.run()
+    calc.upload(a, w.lcs, w.hkl, w.dtdf)
+    calc.run()
+    f_host = calc.fetch()[0]
+    assert rel_err(f_host, ref) <= 1e-10
 
 
 @pytest.mark.parametrize("mode", ["HIP", "HIP_MIXED"])

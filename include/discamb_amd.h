@@ -154,6 +154,11 @@ int dh_hc_set_tuning(dh_calculator* calc, int hkl_chunk);
  * reflections along hkl lines and carry both factors along them, as the reference's Engine3 does,
  * scattering_utilities.cpp:520-756); takes effect at the next upload. */
 #define DH_OPT_DIRECT_PHASES 8u
+/* Groups with a diagonal subgroup beyond it (tetragonal, hexagonal) run their summed calls on a
+ * second engine over coset images unless DH_OPT_GENERAL_ROTATIONS is set; an upload or
+ * dh_hc_set_reflections goes to the engine the options select at that moment, so after a change of
+ * DH_OPT_GENERAL_ROTATIONS (or of the IAM table) dh_hc_run and the device-resident calls return an
+ * error until the inputs or the reflections are set again. */
 int dh_hc_set_options(dh_calculator* calc, unsigned flags);
 
 /* calculateStructureFactorsAndDerivatives: synchronous, host pointers. hkl is 3*n_hkl int,

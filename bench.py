@@ -677,10 +677,11 @@ def main():
     ap.add_argument("--chunk", type=int, default=0, help="fixed reflection chunk (disables the autotuner)")
     ap.add_argument("--chunk-fp64", type=int, default=0, help="fixed reflection chunk for the FP64 path only")
     ap.add_argument("--workload", default="p212121_5000",
-                    choices=["p212121_5000", "p212121_20000", "batch256", "per_hkl", "iam", "electron",
-                             "form_factors", "taam", "autograd"],
+                    choices=["p212121_5000", "p212121_20000", "p43212_5000", "batch256", "per_hkl", "iam",
+                             "electron", "form_factors", "taam", "autograd"],
                     help="p212121_5000: one 5000-atom structure at 1.2 A (headline); p212121_20000: 20 000 atoms "
-                         "at 1.5 A; batch256: 256 uneven structures per batched step; per_hkl: per-reflection "
+                         "at 1.5 A; p43212_5000: 5000 atoms in P4_3 2_1 2 at 1.4 A (coset images over the 222 "
+                         "subgroup); batch256: 256 uneven structures per batched step; per_hkl: per-reflection "
                          "derivative rows; iam / electron: the headline structure in those modes; form_factors: "
                          "its atomic form factors; taam: whole calls with local frames recomputed from the "
                          "positions and the chain rule through them; autograd: torch least-squares gradient "
@@ -722,6 +723,15 @@ def main():
                    "to 1.5 A, 4 explicit ops" % len(w.hkl))
         seed_text = "synthetic, seeded (numpy PCG64 seed 2000)"
         cpu_label, cpu_refl = "the same 20000-atom P212121 structure", 16648
+    elif args.workload == "p43212_5000":
+        rng = np.random.Generator(np.random.PCG64(1000))
+        comp = synth.protein_composition(5000, rng)
+        w = synth.build_workload("p43212_5000at_1.4A", rng, (60.0, 60.0, 70.0, 90.0, 90.0, 90.0), "P43212", comp,
+                                 1.4, synth.PROTEIN_TYPES)
+        wl_text = ("P43212 5000 atoms (protein composition), cell 60x60x70 A, %d unique hkl to 1.4 A, 8 explicit ops "
+                   "(222 subgroup x 2 coset representatives)" % len(w.hkl))
+        seed_text = "synthetic, seeded (numpy PCG64 seed 1000)"
+        cpu_label, cpu_refl = "the same 5000-atom P43212 structure", 4000
     else:
         w = synth.config1()
         wl_text = ("P212121 5000 atoms (C1575 N425 O475 S50 H2475), 40 multipole types, cell 50x60x70 A, 66592 "
